@@ -1,0 +1,186 @@
+"""Benchmark of the sparse-voxel-DAG hot path on one MI355X.
+
+Workload (the GPU configuration of the headline metric): 2048^3 fBm terrain
+(11 levels, ~1.4e7 surface voxels, seed 1), voxel list resident in HBM as
+sorted Morton codes.  One step =
+    build: voxel list -> SVO -> DAG -> SDAG -> .svdag + .esvdag + .ussvdag + .ssvdag
+           images in HBM (the svbuilder lossless path),
+    trace: 1920x1080 primary rays + 1 shadow ray per hit over the .ssvdag
+           (seeded orbit camera, FOV 60, maxIters 300, pixel tolerance 1).
+value = build Gvox/s (voxels / build time); traversal Mrays/s is reported
+beside it.  The roofline object is for the traversal kernel (the dominant
+kernel): achieved = algorithmic bytes fetched by the shader's fetch functions
+(counted per ray by the kernel itself) / its event-timed duration.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+"""
+import argparse
+import importlib.util
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG_DIR = os.path.join(ROOT, "svdag-compression_amd")
+HBM_PEAK_GBS = 8000.0
+
+
+def load_pkg():
+    spec = importlib.util.spec_from_file_location("svdag_amd", os.path.join(PKG_DIR, "__init__.py"),
+                                                  submodule_search_locations=[PKG_DIR])
+    m = importlib.util.module_from_spec(spec)
+    sys.modules["svdag_amd"] = m
+    spec.loader.exec_module(m)
+    return m
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def build_step(S, ctx, dcodes_ptr, n, L):
+    g = S.Octree.from_morton(ctx, (dcodes_ptr, n), L)
+    g.to_dag()
+    g.encode(S.FMT_SVDAG)
+    g.encode(S.FMT_ESVDAG)
+    g.to_sdag()
+    g.encode(S.FMT_USSVDAG)
+    g.encode(S.FMT_SSVDAG)
+    return g
+
+
+def cpu_baseline(W, levels_full, sample_levels):
+    """The oracle (CPU restatement of the reference's single-thread svbuilder
+    reduction + encoders) timed on a bounded sample of the same terrain."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    # sample: the terrain generator at a smaller grid (same seed / octaves)
+    codes = W.terrain_codes(sample_levels, seed=1)
+    t0 = time.perf_counter()
+    o = O.OracleOctree.from_morton(codes, sample_levels)
+    t1 = time.perf_counter()
+    o.to_dag()
+    o.init_child_levels()
+    o.encode(0)
+    o.encode(2)
+    o.to_sdag()
+    o.encode(1)
+    o.encode(2)
+    t2 = time.perf_counter()
+    return dict(value=codes.size / (t2 - t1) / 1e9, unit="Gvox/s", cores=1, kind="port",
+                sample=f"terrain {1 << sample_levels}^3 ({sample_levels} levels, {codes.size} voxels, seed 1): "
+                       f"toDAG + toSDAG + 4 encoders, std::map / std::sort, single thread, "
+                       f"{t2 - t1:.2f} s (SVO build {t1 - t0:.2f} s excluded)")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--levels", type=int, default=11)
+    ap.add_argument("--width", type=int, default=1920)
+    ap.add_argument("--height", type=int, default=1080)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-sample-levels", type=int, default=10)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(each GPU box has one MI355X); run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    S = load_pkg()
+    import svdag_amd.workloads as W
+
+    L = args.levels
+    log(f"[bench] generating terrain {1 << L}^3 ...")
+    codes = W.terrain_codes(L, seed=1)
+    n = int(codes.size)
+    log(f"[bench] {n} voxels")
+    dcodes = torch.from_numpy(codes.view(np.int64)).to("cuda:0")
+    ctx = S.Context(0)
+    params = W.trace_params(args.width, args.height, seed=1, levels=L, shadows=True, light=(1.0, 1.0, 1.0))
+    nray = args.width * args.height
+    prim = torch.zeros(nray * 6, dtype=torch.int32, device="cuda:0")
+    shad = torch.zeros(nray * 6, dtype=torch.int32, device="cuda:0")
+    torch.cuda.synchronize()
+
+    def step():
+        t0 = time.perf_counter()
+        g = build_step(S, ctx, dcodes.data_ptr(), n, L)
+        ctx.sync()
+        t1 = time.perf_counter()
+        sc = S.Scene.from_octree(g, S.FMT_SSVDAG)
+        ctx.sync()
+        t2 = time.perf_counter()
+        sc.trace_device(params, prim.data_ptr(), shad.data_ptr())
+        ctx.sync()
+        t3 = time.perf_counter()
+        trace_ms = ctx.kernel_ms("trace")
+        sc.free()
+        g.free()
+        return t1 - t0, t3 - t2, trace_ms
+
+    for i in range(args.warmup):
+        step()
+        log(f"[bench] warmup {i + 1}/{args.warmup} done")
+    torch.cuda.synchronize()
+    tb, tt, tk = [], [], []
+    t_start = time.perf_counter()
+    for i in range(args.steps):
+        b, t, k = step()
+        tb.append(b)
+        tt.append(t)
+        tk.append(k)
+        log(f"[bench] step {i + 1}/{args.steps}: build {b * 1e3:.2f} ms, trace {t * 1e3:.2f} ms (kernel {k:.3f} ms)")
+    torch.cuda.synchronize()
+    t_total = time.perf_counter() - t_start
+
+    # per-ray stats of the last frame
+    pr = prim.cpu().numpy().view(S.RAY_DTYPE)
+    sh = shad.cpu().numpy().view(S.RAY_DTYPE)
+    n_shadow = int((pr["t"] >= 0).sum())
+    rays = nray + n_shadow
+    bytes_fetched = int(pr["bytes"].astype(np.int64).sum() + sh["bytes"].astype(np.int64).sum())
+    build_s = float(np.median(tb))
+    trace_kernel_ms = float(np.median(tk))
+    gvox = n / build_s / 1e9
+    mrays = rays / (trace_kernel_ms * 1e-3) / 1e6
+    achieved = bytes_fetched / (trace_kernel_ms * 1e-3) / 1e9
+
+    out = {
+        "metric": "SSVDAG build Gvox/s, traversal Mrays/s; learned vs exhaustive lossy @ same err",
+        "value": round(gvox, 4),
+        "unit": "Gvox/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(t_total / args.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32",
+        "data": "synthetic",
+        "config": {"workload": f"fBm terrain {1 << L}^3 ({L} levels, seed 1, {n} voxels): lossless "
+                               f"SVDAG+ESVDAG+USSVDAG+SSVDAG build, then {args.width}x{args.height} primary "
+                               f"+ shadow rays on the SSVDAG",
+                   "levels": L, "voxels": n, "rays_primary": nray, "rays_shadow": n_shadow},
+        "build_ms": round(build_s * 1e3, 3),
+        "traversal_mrays_s": round(mrays, 2),
+        "trace_kernel_ms": round(trace_kernel_ms, 4),
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                     "kernel": "k_trace<SSVDAG>", "algorithmic_bytes": bytes_fetched},
+    }
+    if not args.no_cpu_baseline:
+        log("[bench] cpu baseline (oracle) ...")
+        out["cpu_baseline"] = cpu_baseline(W, L, args.cpu_sample_levels)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,194 @@
+"""ctypes wrapper of liboracle_svdag.so -- TEST INFRASTRUCTURE ONLY.
+
+The oracle is the CPU restatement of the reference path (svdag_oracle.cpp).
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may use
+it, and only as the checker / CPU baseline, never as the product.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liboracle_svdag.so")
+NULL_NODE = 0xFFFFFFFE
+_lib = None
+
+
+def build() -> None:
+    out = subprocess.run(["make", "-C", _HERE], capture_output=True, text=True)
+    if out.returncode != 0:
+        raise RuntimeError("oracle build failed:\n" + out.stdout + out.stderr)
+
+
+class TraceParams(C.Structure):
+    _fields_ = [("view_inv", C.c_float * 16), ("proj_inv", C.c_float * 16),
+                ("width", C.c_uint32), ("height", C.c_uint32), ("max_iters", C.c_uint32),
+                ("draw_level", C.c_uint32), ("projection_factor", C.c_float),
+                ("light_pos", C.c_float * 3), ("shadows", C.c_uint32), ("pad", C.c_uint32)]
+
+
+RAY_DTYPE = np.dtype([("t", "<f4"), ("level", "<f4"), ("iters", "<f4"), ("node", "<f4"),
+                      ("path", "<u4"), ("bytes", "<u4")])
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        vp, u32, u64, i32 = C.c_void_p, C.c_uint32, C.c_uint64, C.c_int
+        sig = {
+            "orc_new": (vp, []), "orc_free": (None, [vp]),
+            "orc_build_morton": (i32, [vp, vp, u64, u32]),
+            "orc_build_mesh": (i32, [vp, vp, u64, u32, vp]),
+            "orc_to_dag": (i32, [vp]), "orc_to_sdag": (i32, [vp]),
+            "orc_init_child_levels": (None, [vp]), "orc_merge_cross_levels": (i32, [vp]),
+            "orc_levels": (u32, [vp]), "orc_state": (i32, [vp]), "orc_level_size": (u64, [vp, u32]),
+            "orc_nnodes": (u64, [vp]), "orc_nvoxels": (u64, [vp]), "orc_nnodes_svo": (u64, [vp]),
+            "orc_cross_merged": (u64, [vp]), "orc_root_side": (C.c_double, [vp]),
+            "orc_get_level": (i32, [vp, u32, vp, vp, vp]),
+            "orc_set_level": (i32, [vp, u32, vp, u64]),
+            "orc_init_levels": (i32, [vp, u32, i32]),
+            "orc_encode": (C.c_int64, [vp, i32, vp, u64]),
+            "orc_std_sort_refcount": (None, [vp, vp, u64]),
+            "orc_trace_frame": (i32, [vp, u64, i32, C.POINTER(TraceParams), vp, vp, u32, u32]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+class OracleOctree:
+    """GeomOctree restated on the CPU (std::map dedup, libstdc++ std::sort)."""
+
+    def __init__(self):
+        self.L = lib()
+        self.h = self.L.orc_new()
+
+    def __del__(self):
+        try:
+            if self.h:
+                self.L.orc_free(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+    @classmethod
+    def from_morton(cls, codes: np.ndarray, levels: int):
+        o = cls()
+        codes = np.ascontiguousarray(codes, dtype=np.uint64)
+        rc = o.L.orc_build_morton(o.h, _p(codes), codes.size, levels)
+        assert rc == 0, rc
+        return o
+
+    @classmethod
+    def from_mesh(cls, tris: np.ndarray, levels: int, bbox_min, bbox_max):
+        """tris: float32 [T, 3, 3]; bbox in double (svbuilder/main.cpp:150-155)."""
+        o = cls()
+        t = np.ascontiguousarray(tris, dtype=np.float32)
+        bb = np.array(list(bbox_min) + list(bbox_max), dtype=np.float64)
+        rc = o.L.orc_build_mesh(o.h, _p(t), t.shape[0], levels, _p(bb))
+        assert rc == 0, rc
+        return o
+
+    @classmethod
+    def from_levels(cls, level_words, state: int):
+        o = cls()
+        o.L.orc_init_levels(o.h, len(level_words), state)
+        for l, w in enumerate(level_words):
+            w = np.ascontiguousarray(w, dtype=np.uint32).reshape(-1, 9)
+            o.L.orc_set_level(o.h, l, _p(w), w.shape[0])
+        return o
+
+    def to_dag(self):
+        rc = self.L.orc_to_dag(self.h)
+        assert rc == 0, rc
+
+    def to_sdag(self):
+        rc = self.L.orc_to_sdag(self.h)
+        assert rc == 0, rc
+
+    def init_child_levels(self):
+        self.L.orc_init_child_levels(self.h)
+
+    def merge_cross_levels(self) -> int:
+        return self.L.orc_merge_cross_levels(self.h)
+
+    @property
+    def levels(self):
+        return self.L.orc_levels(self.h)
+
+    @property
+    def n_nodes(self):
+        return self.L.orc_nnodes(self.h)
+
+    @property
+    def n_voxels(self):
+        return self.L.orc_nvoxels(self.h)
+
+    @property
+    def cross_merged(self):
+        return self.L.orc_cross_merged(self.h)
+
+    def level_size(self, lev):
+        return self.L.orc_level_size(self.h, lev)
+
+    def level(self, lev, with_inv=False, with_child_levels=False):
+        n = self.level_size(lev)
+        w = np.zeros((n, 9), np.uint32)
+        inv = np.zeros(n, np.uint8)
+        cl = np.zeros((n, 8), np.uint32)
+        if n:
+            self.L.orc_get_level(self.h, lev, _p(w), _p(inv), _p(cl))
+        out = [w]
+        if with_inv:
+            out.append(inv)
+        if with_child_levels:
+            out.append(cl)
+        return out[0] if len(out) == 1 else tuple(out)
+
+    def levels_words(self):
+        return [self.level(l) for l in range(self.levels)]
+
+    def encode(self, fmt: int) -> bytes:
+        n = self.L.orc_encode(self.h, fmt, None, 0)
+        assert n >= 0, n
+        buf = np.zeros(n, np.uint8)
+        n2 = self.L.orc_encode(self.h, fmt, _p(buf), n)
+        assert n2 == n
+        return buf.tobytes()
+
+
+def std_sort_refcount(idx: np.ndarray, val: np.ndarray):
+    """libstdc++ std::sort of (idx, val) pairs by val descending (encoded_ssvdag.cpp:273-275)."""
+    i = np.ascontiguousarray(idx, dtype=np.uint32).copy()
+    v = np.ascontiguousarray(val, dtype=np.uint32).copy()
+    lib().orc_std_sort_refcount(_p(i), _p(v), i.size)
+    return i, v
+
+
+def trace_frame(file_bytes: bytes, fmt: int, params, shadows: bool = True, row0: int = 0, nrows: int | None = None):
+    """CPU port of the octree_dda shader over an encoded file (fmt 0/1/2)."""
+    p = TraceParams()
+    C.memmove(C.addressof(p), C.addressof(params), C.sizeof(TraceParams))
+    n = p.width * p.height
+    prim = np.full(n, 0, RAY_DTYPE)
+    shad = np.full(n, 0, RAY_DTYPE) if shadows else None
+    buf = np.frombuffer(file_bytes, np.uint8)
+    f = 2 if fmt == 3 else fmt
+    rc = lib().orc_trace_frame(_p(buf), buf.size, f, C.byref(p), _p(prim), _p(shad) if shad is not None else None,
+                               row0, p.height if nrows is None else nrows)
+    assert rc == 0, rc
+    return prim, shad

@@ -1,0 +1,310 @@
+"""MI355X sparse-voxel-DAG engine: ctypes binding of libsvdag_amd.so.
+
+The product path is the HIP library built from ``csrc/`` (see ``Makefile``);
+this module only marshals arguments across its C ABI (``include/svdag_amd.h``).
+It never falls back to CPU code: if the library is missing or no GPU is
+present, the calls raise.
+
+Classes mirror the reference's C++ API:
+    Octree   ~ GeomOctree      (src/symvox/geom_octree.hpp:32-190)
+    encode() ~ Encoded*::encode + save (encoded_{svdag,ussvdag,ssvdag}.cpp)
+    Scene    ~ EncodedOctree::load + OctreeDDARenderer (svviewer)
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libsvdag_amd.so")
+
+# status codes / constants (include/svdag_amd.h)
+OK = 0
+S_EMPTY, S_SVO, S_DAG, S_SDAG = 0, 1, 2, 3
+FMT_SVDAG, FMT_USSVDAG, FMT_SSVDAG, FMT_ESVDAG = 0, 1, 2, 3
+VIEW_SVO, VIEW_DAG, VIEW_SDAG = 0, 1, 2
+NULL_NODE = 0xFFFFFFFE
+FMT_EXT = {FMT_SVDAG: ".svdag", FMT_USSVDAG: ".ussvdag", FMT_SSVDAG: ".ssvdag", FMT_ESVDAG: ".esvdag"}
+
+
+class SvdError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"svd error {code}: {msg}")
+        self.code = code
+
+
+class TraceParams(C.Structure):
+    _fields_ = [("view_inv", C.c_float * 16), ("proj_inv", C.c_float * 16),
+                ("width", C.c_uint32), ("height", C.c_uint32), ("max_iters", C.c_uint32),
+                ("draw_level", C.c_uint32), ("projection_factor", C.c_float),
+                ("light_pos", C.c_float * 3), ("shadows", C.c_uint32), ("pad", C.c_uint32)]
+
+
+RAY_DTYPE = np.dtype([("t", "<f4"), ("level", "<f4"), ("iters", "<f4"), ("node", "<f4"),
+                      ("path", "<u4"), ("bytes", "<u4")])
+
+_lib = None
+
+
+def build(verbose: bool = False) -> None:
+    """Compile libsvdag_amd.so for gfx950 in-tree (always recompiles changed sources)."""
+    out = subprocess.run(["make", "-C", _HERE, "-j8"], capture_output=True, text=True)
+    if out.returncode != 0:
+        raise RuntimeError("libsvdag_amd build failed:\n" + out.stdout[-4000:] + out.stderr[-4000:])
+    if verbose:
+        print(out.stdout[-2000:])
+
+
+def lib():
+    """Load the HIP library (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: run build() first")
+    # One HIP runtime per process: torch ships its own libamdhip64 (loaded by
+    # file name), so let torch load it first; our library's NEEDED
+    # libamdhip64.so.7 then binds to that copy instead of a second one.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
+    L = C.CDLL(LIB_PATH)
+    vp, u32, u64, i32 = C.c_void_p, C.c_uint32, C.c_uint64, C.c_int
+    P = C.POINTER
+    sig = {
+        "svd_ctx_create": (i32, [i32, P(vp)]),
+        "svd_ctx_destroy": (i32, [vp]),
+        "svd_last_error": (C.c_char_p, [vp]),
+        "svd_ctx_synchronize": (i32, [vp]),
+        "svd_ctx_stream": (vp, [vp]),
+        "svd_octree_from_morton": (i32, [vp, vp, u64, u32, i32, P(vp)]),
+        "svd_octree_from_svo": (i32, [vp, u32, P(u64), P(vp), P(C.c_float), C.c_double, P(vp)]),
+        "svd_octree_free": (i32, [vp]),
+        "svd_reduce_dag": (i32, [vp]),
+        "svd_reduce_sym": (i32, [vp]),
+        "svd_cross_level_merge": (i32, [vp, P(u64)]),
+        "svd_octree_state": (i32, [vp, P(i32)]),
+        "svd_octree_levels": (i32, [vp, P(u32)]),
+        "svd_octree_counts": (i32, [vp, P(u64), P(u64)]),
+        "svd_level_size": (i32, [vp, i32, u32, P(u64)]),
+        "svd_get_level": (i32, [vp, i32, u32, vp, vp, vp]),
+        "svd_encode": (i32, [vp, i32, P(u64)]),
+        "svd_encoded_copy": (i32, [vp, i32, vp, u64]),
+        "svd_encoded_device": (i32, [vp, i32, P(vp), P(u64)]),
+        "svd_scene_create": (i32, [vp, i32, vp, u64, i32, P(vp)]),
+        "svd_scene_from_octree": (i32, [vp, i32, P(vp)]),
+        "svd_scene_free": (i32, [vp]),
+        "svd_trace": (i32, [vp, P(TraceParams), vp, vp]),
+        "svd_last_kernel_ms": (i32, [vp, C.c_char_p, P(C.c_double)]),
+        "svd_refcount_order": (i32, [vp, vp, vp, u64]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+EXPORTED = ["svd_ctx_create", "svd_ctx_destroy", "svd_last_error", "svd_ctx_synchronize", "svd_ctx_stream",
+            "svd_octree_from_morton", "svd_octree_from_svo", "svd_octree_free", "svd_reduce_dag",
+            "svd_reduce_sym", "svd_cross_level_merge", "svd_octree_state", "svd_octree_levels",
+            "svd_octree_counts", "svd_level_size", "svd_get_level", "svd_encode", "svd_encoded_copy",
+            "svd_encoded_device", "svd_scene_create", "svd_scene_from_octree", "svd_scene_free", "svd_trace",
+            "svd_last_kernel_ms", "svd_refcount_order"]
+
+
+class Context:
+    """One device + one HIP stream (svd_ctx)."""
+
+    def __init__(self, device: int = 0):
+        self.L = lib()
+        h = C.c_void_p()
+        rc = self.L.svd_ctx_create(device, C.byref(h))
+        if rc != OK:
+            raise SvdError(rc, f"cannot create a context on device {device}")
+        self.h = h
+
+    def check(self, rc):
+        if rc != OK:
+            raise SvdError(rc, self.L.svd_last_error(self.h).decode())
+
+    def sync(self):
+        self.check(self.L.svd_ctx_synchronize(self.h))
+
+    @property
+    def stream(self) -> int:
+        return self.L.svd_ctx_stream(self.h) or 0
+
+    def kernel_ms(self, name: str) -> float:
+        v = C.c_double()
+        self.check(self.L.svd_last_kernel_ms(self.h, name.encode(), C.byref(v)))
+        return v.value
+
+    def refcount_order(self, idx, val):
+        """Device emulation of std::sort by reference count (returns sorted copies)."""
+        i = np.ascontiguousarray(idx, dtype=np.uint32).copy()
+        v = np.ascontiguousarray(val, dtype=np.uint32).copy()
+        self.check(self.L.svd_refcount_order(self.h, i.ctypes.data_as(C.c_void_p), v.ctypes.data_as(C.c_void_p),
+                                             i.size))
+        return i, v
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.svd_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Octree:
+    """GeomOctree on the device (svd_octree)."""
+
+    def __init__(self, ctx: Context, handle):
+        self.ctx, self.h = ctx, handle
+
+    @classmethod
+    def from_morton(cls, ctx: Context, codes, levels: int):
+        """codes: sorted unique uint64 numpy array, or a device pointer (int) with count in codes_n."""
+        h = C.c_void_p()
+        if isinstance(codes, tuple):          # (device_ptr, n)
+            ptr, n = codes
+            ctx.check(ctx.L.svd_octree_from_morton(ctx.h, C.c_void_p(ptr), n, levels, 1, C.byref(h)))
+        else:
+            codes = np.ascontiguousarray(codes, dtype=np.uint64)
+            ctx.check(ctx.L.svd_octree_from_morton(ctx.h, codes.ctypes.data_as(C.c_void_p), codes.size, levels, 0,
+                                                   C.byref(h)))
+        return cls(ctx, h)
+
+    @classmethod
+    def from_svo(cls, ctx: Context, level_words, bbox, root_side: float):
+        """level_words: list of (n_l, 9) uint32 arrays (word0 = mask, words 1..8 = children)."""
+        L = len(level_words)
+        arrs = [np.ascontiguousarray(w, dtype=np.uint32).reshape(-1, 9) for w in level_words]
+        sizes = (C.c_uint64 * L)(*[a.shape[0] for a in arrs])
+        ptrs = (C.c_void_p * L)(*[a.ctypes.data_as(C.c_void_p).value for a in arrs])
+        bb = (C.c_float * 6)(*[float(x) for x in bbox])
+        h = C.c_void_p()
+        ctx.check(ctx.L.svd_octree_from_svo(ctx.h, L, sizes, ptrs, bb, float(root_side), C.byref(h)))
+        return cls(ctx, h)
+
+    def free(self):
+        if self.h:
+            self.ctx.L.svd_octree_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+    def to_dag(self):
+        self.ctx.check(self.ctx.L.svd_reduce_dag(self.h))
+
+    def to_sdag(self):
+        self.ctx.check(self.ctx.L.svd_reduce_sym(self.h))
+
+    def merge_cross_levels(self) -> int:
+        n = C.c_uint64()
+        self.ctx.check(self.ctx.L.svd_cross_level_merge(self.h, C.byref(n)))
+        return n.value
+
+    @property
+    def state(self) -> int:
+        s = C.c_int()
+        self.ctx.check(self.ctx.L.svd_octree_state(self.h, C.byref(s)))
+        return s.value
+
+    @property
+    def levels(self) -> int:
+        v = C.c_uint32()
+        self.ctx.check(self.ctx.L.svd_octree_levels(self.h, C.byref(v)))
+        return v.value
+
+    def counts(self):
+        a, b = C.c_uint64(), C.c_uint64()
+        self.ctx.check(self.ctx.L.svd_octree_counts(self.h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def level_size(self, view: int, lev: int) -> int:
+        n = C.c_uint64()
+        self.ctx.check(self.ctx.L.svd_level_size(self.h, view, lev, C.byref(n)))
+        return n.value
+
+    def level(self, view: int, lev: int, with_inv=False, with_child_levels=False):
+        n = self.level_size(view, lev)
+        w = np.zeros((n, 9), np.uint32)
+        inv = np.zeros(n, np.uint8) if with_inv else None
+        cl = np.zeros((n, 8), np.uint32) if with_child_levels else None
+        if n:
+            self.ctx.check(self.ctx.L.svd_get_level(self.h, view, lev, w.ctypes.data_as(C.c_void_p),
+                                                    inv.ctypes.data_as(C.c_void_p) if inv is not None else None,
+                                                    cl.ctypes.data_as(C.c_void_p) if cl is not None else None))
+        out = [w]
+        if with_inv:
+            out.append(inv)
+        if with_child_levels:
+            out.append(cl)
+        return out[0] if len(out) == 1 else tuple(out)
+
+    def encode(self, fmt: int) -> int:
+        n = C.c_uint64()
+        self.ctx.check(self.ctx.L.svd_encode(self.h, fmt, C.byref(n)))
+        return n.value
+
+    def encoded_bytes(self, fmt: int) -> bytes:
+        ptr, n = C.c_void_p(), C.c_uint64()
+        self.ctx.check(self.ctx.L.svd_encoded_device(self.h, fmt, C.byref(ptr), C.byref(n)))
+        buf = np.zeros(n.value, np.uint8)
+        self.ctx.check(self.ctx.L.svd_encoded_copy(self.h, fmt, buf.ctypes.data_as(C.c_void_p), n.value))
+        return buf.tobytes()
+
+    def encoded_device(self, fmt: int):
+        ptr, n = C.c_void_p(), C.c_uint64()
+        self.ctx.check(self.ctx.L.svd_encoded_device(self.h, fmt, C.byref(ptr), C.byref(n)))
+        return ptr.value, n.value
+
+
+class Scene:
+    """An encoded file resident on the device, ready for traversal (svd_scene)."""
+
+    def __init__(self, ctx: Context, handle, fmt: int, keep=None):
+        self.ctx, self.h, self.fmt, self._keep = ctx, handle, fmt, keep
+
+    @classmethod
+    def from_bytes(cls, ctx: Context, fmt: int, data: bytes):
+        buf = np.frombuffer(data, np.uint8)
+        h = C.c_void_p()
+        ctx.check(ctx.L.svd_scene_create(ctx.h, fmt, buf.ctypes.data_as(C.c_void_p), buf.size, 0, C.byref(h)))
+        return cls(ctx, h, fmt)
+
+    @classmethod
+    def from_octree(cls, octree: Octree, fmt: int):
+        h = C.c_void_p()
+        octree.ctx.check(octree.ctx.L.svd_scene_from_octree(octree.h, fmt, C.byref(h)))
+        return cls(octree.ctx, h, fmt, keep=octree)
+
+    def free(self):
+        if self.h:
+            self.ctx.L.svd_scene_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+    def trace_device(self, params: TraceParams, prim_ptr: int, shad_ptr: int | None):
+        """Trace one frame into device buffers (e.g. torch tensors' data_ptr())."""
+        self.ctx.check(self.ctx.L.svd_trace(self.h, C.byref(params), C.c_void_p(prim_ptr),
+                                            C.c_void_p(shad_ptr) if shad_ptr else None))

@@ -1,0 +1,518 @@
+// C ABI of libsvdag_amd: handle management, error mapping, host <-> device
+// transfers.  Every entry point converts internal svd::Error exceptions into
+// an int status plus a message stored in the context.
+#include "svd_internal.hpp"
+#include <algorithm>
+#include <new>
+
+namespace svd {
+
+static size_t size_class(size_t b) {
+  if (b <= 4096) return (b + 255) & ~size_t(255);
+  // 8 classes per power of two above 4 KiB
+  size_t p = 1;
+  while ((p << 1) <= b) p <<= 1;
+  const size_t step = p / 8;
+  return (b + step - 1) / step * step;
+}
+
+Arena::~Arena() {
+  trim();
+  for (auto& kv : live_) (void)hipFree(kv.first);
+  live_.clear();
+}
+
+void* Arena::get(size_t bytes) {
+  const size_t cls = size_class(bytes);
+  auto it = free_.find(cls);
+  if (it != free_.end()) {
+    void* p = it->second;
+    free_.erase(it);
+    live_[p] = cls;
+    return p;
+  }
+  void* p = nullptr;
+  if (hipMalloc(&p, cls) != hipSuccess) {
+    (void)hipGetLastError();
+    trim();
+    if (hipMalloc(&p, cls) != hipSuccess) {
+      (void)hipGetLastError();
+      throw Error{SVD_ERR_NOMEM, "device allocation of " + std::to_string(cls) + " bytes failed"};
+    }
+  }
+  live_[p] = cls;
+  return p;
+}
+
+void Arena::put(void* p) {
+  auto it = live_.find(p);
+  if (it == live_.end()) return;
+  free_.insert(std::make_pair(it->second, p));
+  live_.erase(it);
+}
+
+void Arena::trim() {
+  for (auto& kv : free_) (void)hipFree(kv.second);
+  free_.clear();
+}
+
+uint64_t read_u32(svd_ctx* c, const uint32_t* d) {
+  uint32_t v = 0;
+  read_u32s(c, d, &v, 1);
+  return v;
+}
+
+void read_u32s(svd_ctx* c, const uint32_t* d, uint32_t* h, size_t n) {
+  uint32_t* stage = (uint32_t*)c->pinned;
+  size_t done = 0;
+  while (done < n) {
+    const size_t k = std::min<size_t>(n - done, 1024);
+    SVD_HIP(hipMemcpyAsync(stage, d + done, 4 * k, hipMemcpyDeviceToHost, c->stream));
+    SVD_HIP(hipStreamSynchronize(c->stream));
+    memcpy(h + done, stage, 4 * k);
+    done += k;
+  }
+}
+
+}  // namespace svd
+
+using namespace svd;
+
+namespace {
+
+template <typename F>
+int guarded(svd_ctx* c, F&& f) {
+  if (!c) return SVD_ERR_ARG;
+  try {
+    SVD_HIP(hipSetDevice(c->device));
+    f();
+    c->err.clear();
+    return SVD_OK;
+  } catch (const Error& e) {
+    c->err = e.msg;
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    c->err = "host allocation failed";
+    return SVD_ERR_NOMEM;
+  } catch (const std::exception& e) {
+    c->err = e.what();
+    return SVD_ERR_DEVICE;
+  }
+}
+
+void free_levels(svd_ctx* c, std::vector<Level>& v) {
+  for (auto& l : v) { dfree(c, l.rec); dfree(c, l.inv); dfree(c, l.childLevels); }
+  v.clear();
+}
+
+const std::vector<Level>* view_of(svd_octree* o, int view) {
+  switch (view) {
+    case SVD_VIEW_SVO: return &o->svo;
+    case SVD_VIEW_DAG: return o->hasX ? &o->xdag : &o->dag;
+    case SVD_VIEW_SDAG: return &o->sdag;
+    default: return nullptr;
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int svd_ctx_create(int device, svd_ctx** out) {
+  if (!out) return SVD_ERR_ARG;
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev) return SVD_ERR_DEVICE;
+  if (hipSetDevice(device) != hipSuccess) return SVD_ERR_DEVICE;
+  svd_ctx* c = new (std::nothrow) svd_ctx();
+  if (!c) return SVD_ERR_NOMEM;
+  c->device = device;
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipHostMalloc((void**)&c->pinned, 8192, hipHostMallocDefault) != hipSuccess ||
+      hipEventCreate(&c->ev[0]) != hipSuccess || hipEventCreate(&c->ev[1]) != hipSuccess) {
+    delete c;
+    return SVD_ERR_DEVICE;
+  }
+  c->arena = new Arena(device);
+  *out = c;
+  return SVD_OK;
+}
+
+int svd_ctx_destroy(svd_ctx* c) {
+  if (!c) return SVD_ERR_ARG;
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->stream);
+  delete c->arena;
+  if (c->pinned) (void)hipHostFree(c->pinned);
+  if (c->ev[0]) (void)hipEventDestroy(c->ev[0]);
+  if (c->ev[1]) (void)hipEventDestroy(c->ev[1]);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return SVD_OK;
+}
+
+const char* svd_last_error(const svd_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int svd_ctx_synchronize(svd_ctx* c) {
+  return guarded(c, [&] { SVD_HIP(hipStreamSynchronize(c->stream)); });
+}
+
+void* svd_ctx_stream(svd_ctx* c) { return c ? (void*)c->stream : nullptr; }
+
+int svd_octree_from_morton(svd_ctx* c, const uint64_t* codes, uint64_t n, uint32_t levels, int on_device,
+                           svd_octree** out) {
+  if (!out || (n && !codes)) return SVD_ERR_ARG;
+  if (levels < 2 || levels > kMaxLevels) return SVD_ERR_LEVELS;
+  *out = nullptr;
+  svd_octree* o = new svd_octree();
+  o->ctx = c;
+  o->levels = levels;
+  int rc = guarded(c, [&] {
+    Timer t(c, "build_svo");
+    const uint64_t* d = codes;
+    uint64_t* tmp = nullptr;
+    if (!on_device && n) {
+      tmp = dalloc<uint64_t>(c, n);
+      SVD_HIP(hipMemcpyAsync(tmp, codes, 8 * n, hipMemcpyHostToDevice, c->stream));
+      d = tmp;
+    }
+    build_svo_from_morton(o, d, n);
+    SVD_HIP(hipStreamSynchronize(c->stream));
+    dfree(c, tmp);
+  });
+  if (rc) { free_levels(c, o->svo); delete o; return rc; }
+  *out = o;
+  return SVD_OK;
+}
+
+int svd_octree_from_svo(svd_ctx* c, uint32_t levels, const uint64_t* sizes, const uint32_t* const* words,
+                        const float bbox[6], double rootSide, svd_octree** out) {
+  if (!out || !sizes || !words || !bbox) return SVD_ERR_ARG;
+  if (levels < 2 || levels > kMaxLevels) return SVD_ERR_LEVELS;
+  *out = nullptr;
+  svd_octree* o = new svd_octree();
+  o->ctx = c;
+  o->levels = levels;
+  for (int i = 0; i < 6; ++i) o->bbox[i] = bbox[i];
+  o->rootSide = rootSide;
+  int rc = guarded(c, [&] {
+    o->svo.assign(levels, Level());
+    uint64_t nNodes = 0, nVox = 0;
+    if (sizes[0] != 1) throw Error{SVD_ERR_ARG, "level 0 must hold exactly the root"};
+    // validate child pointers on the host: every set mask bit of an inner node
+    // points into the next level, cleared bits hold the null pointer
+    for (uint32_t l = 0; l + 1 < levels; ++l) {
+      if (!sizes[l]) continue;
+      if (!words[l]) throw Error{SVD_ERR_ARG, "missing level words"};
+      for (uint64_t i = 0; i < sizes[l]; ++i) {
+        const uint32_t* w = words[l] + 9 * i;
+        for (int c = 0; c < 8; ++c) {
+          const bool bit = (w[0] >> c) & 1u;
+          if (bit ? (w[1 + c] >= sizes[l + 1]) : (w[1 + c] != kNull))
+            throw Error{SVD_ERR_POINTER, "level " + std::to_string(l) + " node " + std::to_string(i) +
+                                             ": child pointer inconsistent with its mask"};
+        }
+      }
+    }
+    for (uint32_t l = 0; l < levels; ++l) {
+      Level& lv = o->svo[l];
+      lv.n = sizes[l];
+      lv.leaf = (l == levels - 1);
+      if (l > 0) nNodes += lv.n;
+      if (!lv.n) continue;
+      if (!words[l]) throw Error{SVD_ERR_ARG, "missing level words"};
+      if (lv.leaf) {
+        std::vector<uint32_t> m(lv.n);
+        for (uint64_t i = 0; i < lv.n; ++i) { m[i] = words[l][9 * i] & 0xFFu; nVox += __builtin_popcount(m[i]); }
+        lv.rec = dalloc<uint32_t>(c, lv.n);
+        SVD_HIP(hipMemcpyAsync(lv.rec, m.data(), 4 * lv.n, hipMemcpyHostToDevice, c->stream));
+        SVD_HIP(hipStreamSynchronize(c->stream));
+      } else {
+        lv.rec = dalloc<uint32_t>(c, lv.n * kRecWords);
+        SVD_HIP(hipMemcpyAsync(lv.rec, words[l], 4 * kRecWords * lv.n, hipMemcpyHostToDevice, c->stream));
+      }
+    }
+    SVD_HIP(hipStreamSynchronize(c->stream));
+    o->nVoxels = nVox;
+    o->nNodesSVO = nNodes;
+    o->state = SVD_S_SVO;
+  });
+  if (rc) { free_levels(c, o->svo); delete o; return rc; }
+  *out = o;
+  return SVD_OK;
+}
+
+int svd_octree_free(svd_octree* o) {
+  if (!o) return SVD_ERR_ARG;
+  svd_ctx* c = o->ctx;
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->stream);
+  free_levels(c, o->svo);
+  free_levels(c, o->dag);
+  free_levels(c, o->sdag);
+  free_levels(c, o->xdag);
+  for (auto* m : o->sdagMap) dfree(c, m);
+  for (auto& e : o->enc) dfree(c, e.d);
+  delete o;
+  return SVD_OK;
+}
+
+int svd_reduce_dag(svd_octree* o) {
+  if (!o) return SVD_ERR_ARG;
+  return guarded(o->ctx, [&] {
+    Timer t(o->ctx, "to_dag");
+    reduce_dag(o);
+    free_levels(o->ctx, o->xdag);
+    o->hasX = false;
+    for (auto& e : o->enc) e.valid = false;
+  });
+}
+
+int svd_reduce_sym(svd_octree* o) {
+  if (!o) return SVD_ERR_ARG;
+  return guarded(o->ctx, [&] {
+    Timer t(o->ctx, "to_sdag");
+    reduce_sym(o);
+  });
+}
+
+int svd_cross_level_merge(svd_octree* o, uint64_t* eliminated) {
+  if (!o) return SVD_ERR_ARG;
+  return guarded(o->ctx, [&] {
+    Timer t(o->ctx, "cross_merge");
+    cross_level_merge(o);
+    if (eliminated) *eliminated = o->crossMerged;
+  });
+}
+
+int svd_octree_state(svd_octree* o, int* s) {
+  if (!o || !s) return SVD_ERR_ARG;
+  *s = o->state;
+  return SVD_OK;
+}
+
+int svd_octree_levels(svd_octree* o, uint32_t* l) {
+  if (!o || !l) return SVD_ERR_ARG;
+  *l = o->levels;
+  return SVD_OK;
+}
+
+int svd_octree_counts(svd_octree* o, uint64_t* nn, uint64_t* nv) {
+  if (!o) return SVD_ERR_ARG;
+  if (nn) {
+    switch (o->state) {
+      case SVD_S_SVO: *nn = o->nNodesSVO; break;
+      case SVD_S_DAG: *nn = o->hasX ? o->nNodesX : o->nNodesDAG; break;
+      case SVD_S_SDAG: *nn = o->nNodesSDAG; break;
+      default: *nn = 0;
+    }
+  }
+  if (nv) *nv = o->nVoxels;
+  return SVD_OK;
+}
+
+int svd_level_size(svd_octree* o, int view, uint32_t lev, uint64_t* n) {
+  if (!o || !n) return SVD_ERR_ARG;
+  const std::vector<Level>* v = view_of(o, view);
+  if (!v || lev >= v->size()) return SVD_ERR_ARG;
+  *n = (*v)[lev].n;
+  return SVD_OK;
+}
+
+int svd_get_level(svd_octree* o, int view, uint32_t lev, uint32_t* words, uint8_t* inv, uint32_t* childLevels) {
+  if (!o || !words) return SVD_ERR_ARG;
+  const std::vector<Level>* v = view_of(o, view);
+  if (!v || lev >= v->size()) return SVD_ERR_ARG;
+  svd_ctx* c = o->ctx;
+  return guarded(c, [&] {
+    const Level& lv = (*v)[lev];
+    if (!lv.n) return;
+    if (lv.leaf) {
+      std::vector<uint32_t> m(lv.n);
+      SVD_HIP(hipMemcpyAsync(m.data(), lv.rec, 4 * lv.n, hipMemcpyDeviceToHost, c->stream));
+      SVD_HIP(hipStreamSynchronize(c->stream));
+      for (uint64_t i = 0; i < lv.n; ++i) {
+        words[9 * i] = m[i];
+        for (int k = 1; k < 9; ++k) words[9 * i + k] = kNull;
+      }
+    } else {
+      SVD_HIP(hipMemcpyAsync(words, lv.rec, 4 * kRecWords * lv.n, hipMemcpyDeviceToHost, c->stream));
+    }
+    if (inv) {
+      if (lv.inv) SVD_HIP(hipMemcpyAsync(inv, lv.inv, lv.n, hipMemcpyDeviceToHost, c->stream));
+      else memset(inv, 0, lv.n);
+    }
+    if (childLevels) {
+      if (lv.childLevels) SVD_HIP(hipMemcpyAsync(childLevels, lv.childLevels, 32 * lv.n, hipMemcpyDeviceToHost, c->stream));
+      else for (uint64_t i = 0; i < 8 * lv.n; ++i) childLevels[i] = lev + 1;
+    }
+    SVD_HIP(hipStreamSynchronize(c->stream));
+  });
+}
+
+int svd_encode(svd_octree* o, int fmt, uint64_t* bytes) {
+  if (!o || fmt < 0 || fmt > 3) return SVD_ERR_ARG;
+  return guarded(o->ctx, [&] {
+    Timer t(o->ctx, "encode");
+    encode(o, fmt);
+    if (bytes) *bytes = o->enc[fmt].bytes;
+  });
+}
+
+int svd_encoded_copy(svd_octree* o, int fmt, void* dst, uint64_t cap) {
+  if (!o || fmt < 0 || fmt > 3 || !dst) return SVD_ERR_ARG;
+  if (!o->enc[fmt].valid) return SVD_ERR_STATE;
+  if (cap < o->enc[fmt].bytes) return SVD_ERR_ARG;
+  svd_ctx* c = o->ctx;
+  return guarded(c, [&] {
+    SVD_HIP(hipMemcpyAsync(dst, o->enc[fmt].d, o->enc[fmt].bytes, hipMemcpyDeviceToHost, c->stream));
+    SVD_HIP(hipStreamSynchronize(c->stream));
+  });
+}
+
+int svd_encoded_device(svd_octree* o, int fmt, const void** dptr, uint64_t* bytes) {
+  if (!o || fmt < 0 || fmt > 3 || !dptr) return SVD_ERR_ARG;
+  if (!o->enc[fmt].valid) return SVD_ERR_STATE;
+  *dptr = o->enc[fmt].d;
+  if (bytes) *bytes = o->enc[fmt].bytes;
+  return SVD_OK;
+}
+
+// ---------------------------------------------------------------- scenes
+static int scene_setup(svd_ctx* c, int fmt, const uint8_t* dimg, uint64_t bytes, svd_scene* s) {
+  return guarded(c, [&] {
+    auto rd = [&](uint64_t off, void* h, size_t n) {
+      if (off + n > bytes) throw Error{SVD_ERR_FORMAT, "encoded image truncated"};
+      SVD_HIP(hipMemcpyAsync(h, dimg + off, n, hipMemcpyDeviceToHost, c->stream));
+      SVD_HIP(hipStreamSynchronize(c->stream));
+    };
+    uint8_t hdr[36];
+    rd(0, hdr, 36);
+    float rs;
+    memcpy(s->bmin, hdr, 12);
+    memcpy(s->bmax, hdr + 12, 12);
+    memcpy(&rs, hdr + 24, 4);
+    memcpy(&s->levels, hdr + 28, 4);
+    if (s->levels < 2 || s->levels > kMaxLevels) throw Error{SVD_ERR_FORMAT, "bad level count"};
+    s->rootHalfSide = rs / 2.0f;   // EncodedOctree::getHalfSide() (encoded_octree.hpp:68-70)
+    for (int a = 0; a < 3; ++a) s->center[a] = (s->bmin[a] + s->bmax[a]) * 0.5f;
+    s->fmt = fmt;
+    if (fmt == SVD_FMT_SVDAG || fmt == SVD_FMT_USSVDAG) {
+      uint32_t cnt;
+      rd(40, &cnt, 4);
+      if (44 + 4ull * cnt > bytes) throw Error{SVD_ERR_FORMAT, "node array truncated"};
+      s->nodes = (const uint32_t*)(dimg + 44);
+      s->nWords = cnt;
+    } else {
+      if (s->levels < 3) throw Error{SVD_ERR_FORMAT, "SSVDAG needs 3 levels"};
+      uint32_t nInner, nLeafBytes, nOff;
+      rd(36, &nInner, 4);
+      const uint64_t leafCnt = 40 + 2ull * nInner;
+      rd(leafCnt, &nLeafBytes, 4);
+      const uint64_t offCnt = leafCnt + 4 + nLeafBytes;
+      rd(offCnt, &nOff, 4);
+      if (nOff != s->levels - 2 || offCnt + 4 + 4ull * nOff > bytes || (nLeafBytes & 7))
+        throw Error{SVD_ERR_FORMAT, "bad SSVDAG layout"};
+      // aligned private copies for 8-byte brick loads
+      const uint64_t innerBytes = 2ull * nInner;
+      const uint64_t own = ((innerBytes + 15) & ~15ull) + nLeafBytes + 4ull * nOff + 64;
+      s->owned = dalloc<uint8_t>(c, own);
+      uint8_t* p = s->owned;
+      SVD_HIP(hipMemcpyAsync(p, dimg + 40, innerBytes, hipMemcpyDeviceToDevice, c->stream));
+      s->inner = (const uint16_t*)p;
+      p += (innerBytes + 15) & ~15ull;
+      SVD_HIP(hipMemcpyAsync(p, dimg + leafCnt + 4, nLeafBytes, hipMemcpyDeviceToDevice, c->stream));
+      s->leaves = p;
+      p += (nLeafBytes + 15) & ~15ull;
+      SVD_HIP(hipMemcpyAsync(p, dimg + offCnt + 4, 4ull * nOff, hipMemcpyDeviceToDevice, c->stream));
+      s->levelOffsets = (uint32_t*)p;
+      s->nLevelOffsets = nOff;
+      s->nInner = nInner;
+      s->nLeaves = nLeafBytes / 8;
+      SVD_HIP(hipStreamSynchronize(c->stream));
+    }
+  });
+}
+
+int svd_scene_create(svd_ctx* c, int fmt, const void* file, uint64_t bytes, int on_device, svd_scene** out) {
+  if (!c || !file || !out || fmt < 0 || fmt > 3) return SVD_ERR_ARG;
+  *out = nullptr;
+  svd_scene* s = new svd_scene();
+  s->ctx = c;
+  uint8_t* img = nullptr;
+  int rc = guarded(c, [&] {
+    img = dalloc<uint8_t>(c, bytes);
+    SVD_HIP(hipMemcpyAsync(img, file, bytes, on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, c->stream));
+    SVD_HIP(hipStreamSynchronize(c->stream));
+  });
+  if (rc) { delete s; return rc; }
+  rc = scene_setup(c, fmt, img, bytes, s);
+  if (rc) { dfree(c, img); dfree(c, s->owned); delete s; return rc; }
+  if (fmt == SVD_FMT_SVDAG || fmt == SVD_FMT_USSVDAG) {
+    // keep the image alive: it backs s->nodes
+    s->owned = img;
+  } else {
+    dfree(c, img);
+  }
+  *out = s;
+  return SVD_OK;
+}
+
+int svd_scene_from_octree(svd_octree* o, int fmt, svd_scene** out) {
+  if (!o || !out || fmt < 0 || fmt > 3) return SVD_ERR_ARG;
+  if (!o->enc[fmt].valid) return SVD_ERR_STATE;
+  *out = nullptr;
+  svd_scene* s = new svd_scene();
+  s->ctx = o->ctx;
+  int rc = scene_setup(o->ctx, fmt, o->enc[fmt].d, o->enc[fmt].bytes, s);
+  if (rc) { dfree(o->ctx, s->owned); delete s; return rc; }
+  *out = s;
+  return SVD_OK;
+}
+
+int svd_scene_free(svd_scene* s) {
+  if (!s) return SVD_ERR_ARG;
+  (void)hipSetDevice(s->ctx->device);
+  (void)hipStreamSynchronize(s->ctx->stream);
+  dfree(s->ctx, s->owned);
+  delete s;
+  return SVD_OK;
+}
+
+int svd_trace(svd_scene* s, const svd_trace_params* p, svd_ray_result* prim, svd_ray_result* shad) {
+  if (!s || !p || !prim) return SVD_ERR_ARG;
+  return guarded(s->ctx, [&] {
+    Timer t(s->ctx, "trace");
+    trace_frame(s, p, prim, shad);
+  });
+}
+
+int svd_refcount_order(svd_ctx* c, uint32_t* idx, uint32_t* val, uint64_t n) {
+  if (!c || (n && (!idx || !val))) return SVD_ERR_ARG;
+  return guarded(c, [&] {
+    if (!n) return;
+    uint32_t* di = dalloc<uint32_t>(c, n);
+    uint32_t* dv = dalloc<uint32_t>(c, n);
+    SVD_HIP(hipMemcpyAsync(di, idx, 4 * n, hipMemcpyHostToDevice, c->stream));
+    SVD_HIP(hipMemcpyAsync(dv, val, 4 * n, hipMemcpyHostToDevice, c->stream));
+    {
+      Timer t(c, "refsort");
+      refsort(c, di, dv, n);
+    }
+    SVD_HIP(hipMemcpyAsync(idx, di, 4 * n, hipMemcpyDeviceToHost, c->stream));
+    SVD_HIP(hipMemcpyAsync(val, dv, 4 * n, hipMemcpyDeviceToHost, c->stream));
+    SVD_HIP(hipStreamSynchronize(c->stream));
+    dfree(c, di);
+    dfree(c, dv);
+  });
+}
+
+int svd_last_kernel_ms(svd_ctx* c, const char* name, double* ms) {
+  if (!c || !name || !ms) return SVD_ERR_ARG;
+  auto it = c->kernel_ms.find(name);
+  if (it == c->kernel_ms.end()) return SVD_ERR_ARG;
+  *ms = it->second;
+  return SVD_OK;
+}
+
+}  // extern "C"

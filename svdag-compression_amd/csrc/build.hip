@@ -1,0 +1,635 @@
+// Bottom-up level reduction on the device:
+//   * SVO levels from a sorted Morton voxel list (point-major DFS order),
+//   * toDAG  : exact-key hash dedup, first occurrence wins (geom_octree.cpp:462-548),
+//   * toSDAG : 8 mirror variants per node resolved in the reference's order
+//              (geom_octree.cpp:551-697).
+// Node records are 9 u32 words: w0 = mask | mX<<8 | mY<<16 | mZ<<24,
+// w1..w8 = children[0..7]; the finest level stores only the mask word.
+#include "svd_internal.hpp"
+#include <algorithm>
+
+namespace svd {
+
+namespace {
+
+constexpr int kB = 256;
+
+// ---------------------------------------------------------------- SVO build
+__device__ __forceinline__ bool head64(const uint64_t* code, uint64_t i) {
+  return i == 0 || (code[i] >> 3) != (code[i - 1] >> 3);
+}
+
+// Input contract: strictly ascending codes below 2^(3*levels).
+__global__ void k_check_codes(const uint64_t* __restrict__ code, uint64_t n, uint64_t limit, uint32_t* __restrict__ bad) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (code[i] >= limit || (i > 0 && code[i] <= code[i - 1])) atomicAdd(bad, 1u);
+}
+
+__global__ void k_heads(const uint64_t* __restrict__ code, uint64_t n, uint32_t* __restrict__ head) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) head[i] = head64(code, i) ? 1u : 0u;
+}
+
+// Finest level: one node per run of voxels sharing code >> 3.
+__global__ void k_leaf_level(const uint64_t* __restrict__ code, uint64_t n, const uint32_t* __restrict__ pidx,
+                             uint32_t* __restrict__ mask, uint64_t* __restrict__ pcode) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || !head64(code, i)) return;
+  const uint64_t p = code[i] >> 3;
+  uint32_t m = 0;
+  for (uint64_t j = i; j < n && j < i + 8 && (code[j] >> 3) == p; ++j) m |= 1u << (uint32_t)(code[j] & 7u);
+  mask[pidx[i]] = m;
+  pcode[pidx[i]] = p;
+}
+
+// Inner level: one node per run of child codes sharing code >> 3; the
+// child pointer is the child's position in its own level.
+__global__ void k_inner_level(const uint64_t* __restrict__ ccode, uint64_t n, const uint32_t* __restrict__ pidx,
+                              uint32_t* __restrict__ rec, uint64_t* __restrict__ pcode) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || !head64(ccode, i)) return;
+  const uint64_t p = ccode[i] >> 3;
+  uint32_t w[kRecWords];
+  w[0] = 0;
+#pragma unroll
+  for (int c = 0; c < 8; ++c) w[1 + c] = kNull;
+  for (uint64_t j = i; j < n && j < i + 8 && (ccode[j] >> 3) == p; ++j) {
+    const uint32_t s = (uint32_t)(ccode[j] & 7u);
+    w[0] |= 1u << s;
+    w[1 + s] = (uint32_t)j;
+  }
+  uint32_t* o = rec + (uint64_t)pidx[i] * kRecWords;
+#pragma unroll
+  for (int k = 0; k < kRecWords; ++k) o[k] = w[k];
+  if (pcode) pcode[pidx[i]] = p;
+}
+
+// ---------------------------------------------------------------- hash dedup
+__device__ __forceinline__ void load_rec(const uint32_t* __restrict__ rec, uint64_t i, uint32_t w[kRecWords]) {
+  const uint32_t* r = rec + i * kRecWords;
+#pragma unroll
+  for (int k = 0; k < kRecWords; ++k) w[k] = r[k];
+}
+
+__device__ __forceinline__ bool rec_eq(const uint32_t* __restrict__ rec, uint32_t j, const uint32_t w[kRecWords]) {
+  const uint32_t* r = rec + (uint64_t)j * kRecWords;
+#pragma unroll
+  for (int k = 0; k < kRecWords; ++k)
+    if (r[k] != w[k]) return false;
+  return true;
+}
+
+// Insert node i; the table slot ends holding the smallest index of all nodes
+// with an equal key (atomicMin).  Empty nodes (mask 0) are skipped, like the
+// reference's `if (!n.hasChildren()) continue;`.
+__global__ void k_insert(const uint32_t* __restrict__ rec, uint64_t n, uint32_t* __restrict__ table, uint32_t cmask,
+                         uint32_t* __restrict__ slot) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint32_t w[kRecWords];
+  load_rec(rec, i, w);
+  if ((w[0] & 0xFFu) == 0) { slot[i] = kNone; return; }
+  uint32_t h = (uint32_t)hash_rec(w) & cmask;
+  while (true) {
+    uint32_t v = __atomic_load_n(&table[h], __ATOMIC_RELAXED);
+    if (v == kNone) {
+      uint32_t old = atomicCAS(&table[h], kNone, (uint32_t)i);
+      if (old == kNone) { slot[i] = h; return; }
+      v = old;
+    }
+    if (rec_eq(rec, v, w)) {
+      atomicMin(&table[h], (uint32_t)i);
+      slot[i] = h;
+      return;
+    }
+    h = (h + 1) & cmask;
+  }
+}
+
+// Probe for a key without inserting; returns the stored (minimum) index.
+__device__ __forceinline__ uint32_t lookup(const uint32_t* __restrict__ rec, const uint32_t* __restrict__ table,
+                                           uint32_t cmask, const uint32_t w[kRecWords]) {
+  uint32_t h = (uint32_t)hash_rec(w) & cmask;
+  while (true) {
+    uint32_t v = table[h];
+    if (v == kNone) return kNone;
+    if (rec_eq(rec, v, w)) return v;
+    h = (h + 1) & cmask;
+  }
+}
+
+__global__ void k_rep_flags(const uint32_t* __restrict__ slot, const uint32_t* __restrict__ table, uint64_t n,
+                            uint32_t* __restrict__ flag) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t s = slot[i];
+  flag[i] = (s != kNone && table[s] == (uint32_t)i) ? 1u : 0u;
+}
+
+__global__ void k_compact(const uint32_t* __restrict__ rec, const uint32_t* __restrict__ slot,
+                          const uint32_t* __restrict__ table, const uint32_t* __restrict__ pos, uint64_t n,
+                          uint32_t* __restrict__ out, uint32_t* __restrict__ remap) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t s = slot[i];
+  if (s == kNone) { remap[i] = kNone; return; }
+  const uint32_t r = table[s];
+  remap[i] = pos[r];
+  if (r == (uint32_t)i) {
+    const uint32_t* src = rec + i * kRecWords;
+    uint32_t* dst = out + (uint64_t)pos[i] * kRecWords;
+#pragma unroll
+    for (int k = 0; k < kRecWords; ++k) dst[k] = src[k];
+  }
+}
+
+// children[c] <- remap[children[c]] where mask bit c is set (geom_octree.cpp:519-538)
+__global__ void k_remap_children(const uint32_t* __restrict__ in, uint64_t n, const uint32_t* __restrict__ remap,
+                                 uint32_t* __restrict__ out) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint32_t w[kRecWords];
+  load_rec(in, i, w);
+  const uint32_t m = w[0] & 0xFFu;
+#pragma unroll
+  for (int c = 0; c < 8; ++c)
+    if ((m >> c) & 1u) w[1 + c] = remap[w[1 + c]];
+  uint32_t* o = out + i * kRecWords;
+#pragma unroll
+  for (int k = 0; k < kRecWords; ++k) o[k] = w[k];
+}
+
+// ---------------------------------------------------------------- leaf level
+__global__ void k_leaf_firstocc(const uint32_t* __restrict__ mask, uint64_t n, uint32_t* __restrict__ firstocc) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t m = mask[i] & 0xFFu;
+  if (m) atomicMin(&firstocc[m], (uint32_t)i);
+}
+
+// Distinct masks ranked by first occurrence -> DAG ids.
+__global__ void k_leaf_rank(const uint32_t* __restrict__ firstocc, uint32_t* __restrict__ idOfMask,
+                            uint32_t* __restrict__ dagMask, uint32_t* __restrict__ count) {
+  __shared__ uint32_t fo[256];
+  const int m = threadIdx.x;
+  fo[m] = firstocc[m];
+  __syncthreads();
+  uint32_t id = kNone;
+  if (fo[m] != kNone) {
+    uint32_t r = 0;
+    for (int k = 0; k < 256; ++k) r += (fo[k] < fo[m]) ? 1u : 0u;
+    id = r;
+    dagMask[r] = (uint32_t)m;
+  }
+  idOfMask[m] = id;
+  if (m == 0) {
+    uint32_t cnt = 0;
+    for (int k = 0; k < 256; ++k) cnt += fo[k] != kNone;
+    *count = cnt;
+  }
+}
+
+__global__ void k_leaf_remap(const uint32_t* __restrict__ mask, uint64_t n, const uint32_t* __restrict__ idOfMask,
+                             uint32_t* __restrict__ remap) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t m = mask[i] & 0xFFu;
+  remap[i] = m ? idOfMask[m] : kNone;
+}
+
+// ---------------------------------------------------------------- symmetry
+// Transform order of the reference lookup: I, X, Y, Z, XY, XZ, YZ, XYZ.
+__device__ __constant__ uint8_t kFlip[8] = {0, 4, 2, 1, 6, 5, 3, 7};
+
+__device__ __forceinline__ uint32_t perm8(uint32_t m, uint32_t flip) {
+  uint32_t r = 0;
+#pragma unroll
+  for (int c = 0; c < 8; ++c) r |= ((m >> c) & 1u) << (c ^ flip);
+  return r;
+}
+
+// Node::mirror with applyToChildren (octree_node.cpp:88-195) by transform t,
+// then (if childInv) GeomOctree::invertInvs (geom_octree.cpp:822-831).
+__device__ __forceinline__ void make_variant(const uint32_t k0[kRecWords], int t, const uint8_t* __restrict__ childInv,
+                                             uint32_t v[kRecWords]) {
+  const uint32_t flip = kFlip[t];
+  const uint32_t tx = (flip >> 2) & 1u, ty = (flip >> 1) & 1u, tz = flip & 1u;
+  uint32_t mask = perm8(k0[0] & 0xFFu, flip);
+  uint32_t mx = perm8((k0[0] >> 8) & 0xFFu, flip);
+  uint32_t my = perm8((k0[0] >> 16) & 0xFFu, flip);
+  uint32_t mz = perm8((k0[0] >> 24) & 0xFFu, flip);
+#pragma unroll
+  for (int c = 0; c < 8; ++c) v[1 + (c ^ flip)] = k0[1 + c];
+  uint32_t ex = 0;
+#pragma unroll
+  for (int c = 0; c < 8; ++c) ex |= (v[1 + c] != kNull ? 1u : 0u) << c;
+  if (tx) mx ^= ex;
+  if (ty) my ^= ex;
+  if (tz) mz ^= ex;
+  if (childInv) {
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      if (!((ex >> c) & 1u)) continue;
+      const uint32_t ci = childInv[v[1 + c]];
+      if (tx && (ci & 1u)) mx &= ~(1u << c);
+      if (ty && (ci & 2u)) my &= ~(1u << c);
+      if (tz && (ci & 4u)) mz &= ~(1u << c);
+    }
+  }
+  v[0] = mask | (mx << 8) | (my << 16) | (mz << 24);
+}
+
+__device__ __forceinline__ bool same_rec(const uint32_t a[kRecWords], const uint32_t b[kRecWords]) {
+#pragma unroll
+  for (int k = 0; k < kRecWords; ++k)
+    if (a[k] != b[k]) return false;
+  return true;
+}
+
+// Identity key K0 of each DAG node of a level: children moved to the SDAG
+// ids of the level below, mirror bits set from their correspondence
+// (geom_octree.cpp:666-679); plus raw invariance bits (:603-606).
+__global__ void k_sdag_keys(const uint32_t* __restrict__ dag, uint64_t n, const uint32_t* __restrict__ childMap,
+                            uint32_t* __restrict__ key, uint8_t* __restrict__ inv) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint32_t w[kRecWords];
+  load_rec(dag, i, w);
+  uint32_t mx = (w[0] >> 8) & 0xFFu, my = (w[0] >> 16) & 0xFFu, mz = (w[0] >> 24) & 0xFFu;
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    if (w[1 + c] == kNull) continue;
+    const uint32_t m = childMap[w[1 + c]];
+    w[1 + c] = m & 0x1FFFFFFFu;
+    mx |= ((m >> 29) & 1u) << c;
+    my |= ((m >> 30) & 1u) << c;
+    mz |= ((m >> 31) & 1u) << c;
+  }
+  w[0] = (w[0] & 0xFFu) | (mx << 8) | (my << 16) | (mz << 24);
+  uint32_t* o = key + i * kRecWords;
+#pragma unroll
+  for (int k = 0; k < kRecWords; ++k) o[k] = w[k];
+  if (inv) {
+    uint32_t b = 0, v[kRecWords];
+    make_variant(w, 1, nullptr, v); b |= same_rec(v, w) ? 1u : 0u;
+    make_variant(w, 2, nullptr, v); b |= same_rec(v, w) ? 2u : 0u;
+    make_variant(w, 3, nullptr, v); b |= same_rec(v, w) ? 4u : 0u;
+    inv[i] = (uint8_t)b;
+  }
+}
+
+enum : uint32_t { kUndecided = 0, kRep = 1, kNonRep = 2 };
+
+// First-occurrence nodes look up their 7 mirrored variants; an earlier node
+// carrying that key is a candidate that, if it is a representative, takes
+// this node.  Later duplicates of a key are never representatives.
+__global__ void k_sdag_cands(const uint32_t* __restrict__ key, uint64_t n, const uint32_t* __restrict__ table,
+                             uint32_t cmask, const uint32_t* __restrict__ slot, const uint8_t* __restrict__ childInv,
+                             uint32_t* __restrict__ cand, uint32_t* __restrict__ state) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const bool first = table[slot[i]] == (uint32_t)i;
+  uint32_t* cd = cand + i * 7;
+  if (!first) {
+#pragma unroll
+    for (int t = 0; t < 7; ++t) cd[t] = kNone;
+    state[i] = kNonRep;
+    return;
+  }
+  uint32_t k0[kRecWords], v[kRecWords];
+  load_rec(key, i, k0);
+  bool any = false;
+  for (int t = 1; t < 8; ++t) {
+    make_variant(k0, t, childInv, v);
+    uint32_t r = lookup(key, table, cmask, v);
+    if (r != kNone && r < (uint32_t)i) { cd[t - 1] = r; any = true; }
+    else cd[t - 1] = kNone;
+  }
+  state[i] = any ? kUndecided : kRep;
+}
+
+// One resolution round.  state only moves Undecided -> final, so a stale
+// read only delays a decision.
+__global__ void k_sdag_resolve(const uint32_t* __restrict__ cand, uint64_t n, uint32_t* state,
+                               uint32_t* __restrict__ remaining) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (__atomic_load_n(&state[i], __ATOMIC_RELAXED) != kUndecided) return;
+  const uint32_t* cd = cand + i * 7;
+  bool allNon = true, anyRep = false;
+  for (int t = 0; t < 7; ++t) {
+    const uint32_t c = cd[t];
+    if (c == kNone) continue;
+    const uint32_t s = __atomic_load_n(&state[c], __ATOMIC_RELAXED);
+    if (s == kRep) { anyRep = true; break; }
+    if (s != kNonRep) allNon = false;
+  }
+  if (anyRep) __atomic_store_n(&state[i], kNonRep, __ATOMIC_RELAXED);
+  else if (allNon) __atomic_store_n(&state[i], kRep, __ATOMIC_RELAXED);
+  else atomicAdd(remaining, 1u);
+}
+
+__global__ void k_state_flags(const uint32_t* __restrict__ state, uint64_t n, uint32_t* __restrict__ flag) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) flag[i] = state[i] == kRep ? 1u : 0u;
+}
+
+// Correspondence of every DAG node: itself if a representative, else the
+// first transform (in reference order) whose key is an earlier
+// representative's identity key.  Representatives are compacted in order.
+__global__ void k_sdag_outcome(const uint32_t* __restrict__ key, uint64_t n, const uint32_t* __restrict__ table,
+                               uint32_t cmask, const uint32_t* __restrict__ slot, const uint32_t* __restrict__ state,
+                               const uint32_t* __restrict__ pos, const uint8_t* __restrict__ childInv,
+                               const uint8_t* __restrict__ invIn, uint32_t* __restrict__ map,
+                               uint32_t* __restrict__ out, uint8_t* __restrict__ invOut, uint32_t* __restrict__ err) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint32_t k0[kRecWords];
+  load_rec(key, i, k0);
+  if (state[i] == kRep) {
+    const uint32_t p = pos[i];
+    map[i] = p;
+    uint32_t* o = out + (uint64_t)p * kRecWords;
+#pragma unroll
+    for (int k = 0; k < kRecWords; ++k) o[k] = k0[k];
+    if (invOut) invOut[p] = invIn ? invIn[i] : 0;
+    return;
+  }
+  uint32_t v[kRecWords];
+  for (int t = 0; t < 8; ++t) {
+    uint32_t r;
+    if (t == 0) r = table[slot[i]];
+    else { make_variant(k0, t, childInv, v); r = lookup(key, table, cmask, v); }
+    if (r != kNone && r < (uint32_t)i && state[r] == kRep) {
+      const uint32_t f = kFlip[t];
+      map[i] = pos[r] | (((f >> 2) & 1u) << 29) | (((f >> 1) & 1u) << 30) | ((f & 1u) << 31);
+      return;
+    }
+  }
+  map[i] = kNone;
+  atomicAdd(err, 1u);
+}
+
+// Finest level (at most 255 distinct masks): the reference loop run
+// sequentially by one lane over an LDS table.
+__global__ void k_sdag_leaves(const uint32_t* __restrict__ dagMask, uint32_t n, uint32_t* __restrict__ map,
+                              uint32_t* __restrict__ sdagMask, uint8_t* __restrict__ inv, uint32_t* __restrict__ count) {
+  __shared__ uint32_t repOf[256];
+  repOf[threadIdx.x] = kNone;
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  uint32_t cnt = 0;
+  for (uint32_t i = 0; i < n; ++i) {
+    const uint32_t m = dagMask[i] & 0xFFu;
+    uint32_t b = 0;
+    if (perm8(m, 4) == m) b |= 1;
+    if (perm8(m, 2) == m) b |= 2;
+    if (perm8(m, 1) == m) b |= 4;
+    bool found = false;
+    for (int t = 0; t < 8 && !found; ++t) {
+      const uint32_t f = kFlip[t];
+      const uint32_t r = repOf[perm8(m, f)];
+      if (r != kNone) {
+        map[i] = r | (((f >> 2) & 1u) << 29) | (((f >> 1) & 1u) << 30) | ((f & 1u) << 31);
+        found = true;
+      }
+    }
+    if (!found) {
+      repOf[m] = cnt;
+      sdagMask[cnt] = m;
+      inv[cnt] = (uint8_t)b;
+      map[i] = cnt;
+      ++cnt;
+    }
+  }
+  *count = cnt;
+}
+
+uint32_t table_cap(uint64_t n) {
+  uint64_t c = 1024;
+  while (c < 2 * n) c <<= 1;
+  return (uint32_t)c;
+}
+
+}  // namespace
+
+// ============================================================================
+void build_svo_from_morton(svd_octree* o, const uint64_t* dcodes, uint64_t n) {
+  svd_ctx* c = o->ctx;
+  const uint32_t L = o->levels;
+  o->svo.assign(L, Level());
+  o->nVoxels = n;
+  if (n == 0) {
+    // Empty scene: a single empty root (buildSVO on no geometry).
+    o->svo[0].rec = dalloc<uint32_t>(c, kRecWords);
+    uint32_t h[kRecWords] = {0, kNull, kNull, kNull, kNull, kNull, kNull, kNull, kNull};
+    SVD_HIP(hipMemcpyAsync(o->svo[0].rec, h, sizeof(h), hipMemcpyHostToDevice, c->stream));
+    o->svo[0].n = 1;
+    for (uint32_t l = 1; l < L; ++l) o->svo[l].leaf = (l == L - 1);
+    if (L == 1) o->svo[0].leaf = true;
+    o->nNodesSVO = 0;
+    o->state = SVD_S_SVO;
+    SVD_HIP(hipStreamSynchronize(c->stream));
+    return;
+  }
+  uint32_t* scanbuf = dalloc<uint32_t>(c, n + 1);
+  uint32_t* total = dalloc<uint32_t>(c, 1);
+  fill_u32(c, total, 0u, 1);
+  k_check_codes<<<div_up(n, kB), kB, 0, c->stream>>>(dcodes, n, 1ull << (3 * L), total);
+  check_launch("k_check_codes");
+  if (read_u32(c, total) != 0) {
+    dfree(c, scanbuf); dfree(c, total);
+    throw Error{SVD_ERR_ARG, "Morton codes must be strictly ascending and below 2^(3*levels)"};
+  }
+  uint64_t* codeA = dalloc<uint64_t>(c, n);
+  uint64_t* codeB = dalloc<uint64_t>(c, n);
+  // finest level
+  k_heads<<<div_up(n, kB), kB, 0, c->stream>>>(dcodes, n, scanbuf);
+  scan_exclusive(c, scanbuf, scanbuf, n, total);
+  uint64_t nl = read_u32(c, total);
+  Level& leaf = o->svo[L - 1];
+  leaf.leaf = true;
+  leaf.n = nl;
+  leaf.rec = dalloc<uint32_t>(c, nl);
+  k_leaf_level<<<div_up(n, kB), kB, 0, c->stream>>>(dcodes, n, scanbuf, leaf.rec, codeA);
+  check_launch("k_leaf_level");
+  uint64_t nNodes = (L >= 2) ? nl : 0;
+  uint64_t nc = nl;
+  for (int lev = (int)L - 2; lev >= 0; --lev) {
+    k_heads<<<div_up(nc, kB), kB, 0, c->stream>>>(codeA, nc, scanbuf);
+    scan_exclusive(c, scanbuf, scanbuf, nc, total);
+    uint64_t np = read_u32(c, total);
+    Level& lv = o->svo[lev];
+    lv.n = np;
+    lv.rec = dalloc<uint32_t>(c, np * kRecWords);
+    k_inner_level<<<div_up(nc, kB), kB, 0, c->stream>>>(codeA, nc, scanbuf, lv.rec, codeB);
+    check_launch("k_inner_level");
+    std::swap(codeA, codeB);
+    nc = np;
+    if (lev > 0) nNodes += np;
+  }
+  o->nNodesSVO = nNodes;   // non-root nodes created (GeomOctree::_nNodes after buildSVO)
+  dfree(c, scanbuf); dfree(c, total); dfree(c, codeA); dfree(c, codeB);
+  o->state = SVD_S_SVO;
+}
+
+// Generic dedup of one inner level's records -> remap + compacted records.
+static uint64_t dedup_level(svd_ctx* c, const uint32_t* rec, uint64_t n, uint32_t* remap, uint32_t** outRec) {
+  const uint32_t cap = table_cap(n);
+  uint32_t* table = dalloc<uint32_t>(c, cap);
+  uint32_t* slot = dalloc<uint32_t>(c, n);
+  uint32_t* pos = dalloc<uint32_t>(c, n + 1);
+  uint32_t* total = dalloc<uint32_t>(c, 1);
+  fill_u32(c, table, kNone, cap);
+  k_insert<<<div_up(n, kB), kB, 0, c->stream>>>(rec, n, table, cap - 1, slot);
+  k_rep_flags<<<div_up(n, kB), kB, 0, c->stream>>>(slot, table, n, pos);
+  check_launch("k_insert");
+  scan_exclusive(c, pos, pos, n, total);
+  const uint64_t d = read_u32(c, total);
+  uint32_t* out = dalloc<uint32_t>(c, (d ? d : 1) * kRecWords);
+  k_compact<<<div_up(n, kB), kB, 0, c->stream>>>(rec, slot, table, pos, n, out, remap);
+  check_launch("k_compact");
+  dfree(c, table); dfree(c, slot); dfree(c, pos); dfree(c, total);
+  *outRec = out;
+  return d;
+}
+
+void reduce_dag(svd_octree* o) {
+  svd_ctx* c = o->ctx;
+  if (!(o->state == SVD_S_SVO)) throw Error{SVD_ERR_STATE, "toDAG: octree is not an SVO"};
+  const uint32_t L = o->levels;
+  for (auto& lv : o->dag) { dfree(c, lv.rec); }
+  o->dag.assign(L, Level());
+  if (L == 1) {
+    o->dag[0] = o->svo[0];
+    o->dag[0].rec = dalloc<uint32_t>(c, 1);
+    SVD_HIP(hipMemcpyAsync(o->dag[0].rec, o->svo[0].rec, 4, hipMemcpyDeviceToDevice, c->stream));
+    o->nNodesDAG = 1;
+    o->state = SVD_S_DAG;
+    return;
+  }
+  uint64_t nNodes = 1;
+  // finest level: dedup of 8-bit masks by first occurrence
+  const Level& sl = o->svo[L - 1];
+  uint32_t* remap = dalloc<uint32_t>(c, sl.n + 1);
+  {
+    uint32_t* firstocc = dalloc<uint32_t>(c, 256);
+    uint32_t* idOfMask = dalloc<uint32_t>(c, 256);
+    uint32_t* count = dalloc<uint32_t>(c, 1);
+    Level& dl = o->dag[L - 1];
+    dl.leaf = true;
+    dl.rec = dalloc<uint32_t>(c, 256);
+    fill_u32(c, firstocc, kNone, 256);
+    if (sl.n) k_leaf_firstocc<<<div_up(sl.n, kB), kB, 0, c->stream>>>(sl.rec, sl.n, firstocc);
+    k_leaf_rank<<<1, 256, 0, c->stream>>>(firstocc, idOfMask, dl.rec, count);
+    if (sl.n) k_leaf_remap<<<div_up(sl.n, kB), kB, 0, c->stream>>>(sl.rec, sl.n, idOfMask, remap);
+    check_launch("leaf dedup");
+    dl.n = read_u32(c, count);
+    nNodes += dl.n;
+    dfree(c, firstocc); dfree(c, idOfMask); dfree(c, count);
+  }
+  for (int lev = (int)L - 2; lev >= 0; --lev) {
+    const Level& s = o->svo[lev];
+    uint32_t* keyed = dalloc<uint32_t>(c, (s.n ? s.n : 1) * kRecWords);
+    if (s.n) k_remap_children<<<div_up(s.n, kB), kB, 0, c->stream>>>(s.rec, s.n, remap, keyed);
+    check_launch("k_remap_children");
+    Level& d = o->dag[lev];
+    if (lev == 0) {
+      // the root level is never reduced (geom_octree.cpp:483)
+      d.rec = keyed;
+      d.n = s.n;
+    } else {
+      uint32_t* nremap = dalloc<uint32_t>(c, s.n + 1);
+      d.n = dedup_level(c, keyed, s.n, nremap, &d.rec);
+      dfree(c, keyed);
+      dfree(c, remap);
+      remap = nremap;
+      nNodes += d.n;
+    }
+  }
+  dfree(c, remap);
+  o->nNodesDAG = nNodes;
+  o->state = SVD_S_DAG;
+}
+
+void reduce_sym(svd_octree* o) {
+  svd_ctx* c = o->ctx;
+  if (o->state != SVD_S_DAG && o->state != SVD_S_SDAG) throw Error{SVD_ERR_STATE, "toSDAG: octree is not a DAG"};
+  const uint32_t L = o->levels;
+  for (auto& lv : o->sdag) { dfree(c, lv.rec); dfree(c, lv.inv); }
+  for (auto* m : o->sdagMap) dfree(c, m);
+  o->sdag.assign(L, Level());
+  o->sdagMap.assign(L, nullptr);
+  if (L == 1) throw Error{SVD_ERR_LEVELS, "toSDAG needs at least 2 levels"};
+  uint64_t nNodes = 0;
+  uint32_t* count = dalloc<uint32_t>(c, 1);
+  uint32_t* err = dalloc<uint32_t>(c, 1);
+  fill_u32(c, err, 0u, 1);
+  {
+    const Level& dl = o->dag[L - 1];
+    Level& sl = o->sdag[L - 1];
+    sl.leaf = true;
+    sl.rec = dalloc<uint32_t>(c, 256);
+    sl.inv = dalloc<uint8_t>(c, 256);
+    o->sdagMap[L - 1] = dalloc<uint32_t>(c, dl.n + 1);
+    k_sdag_leaves<<<1, 256, 0, c->stream>>>(dl.rec, (uint32_t)dl.n, o->sdagMap[L - 1], sl.rec, sl.inv, count);
+    check_launch("k_sdag_leaves");
+    sl.n = read_u32(c, count);
+    nNodes += sl.n;
+  }
+  for (int lev = (int)L - 2; lev >= 0; --lev) {
+    const Level& d = o->dag[lev];
+    Level& s = o->sdag[lev];
+    const uint64_t n = d.n;
+    const uint8_t* childInv = (lev < (int)L - 1) ? o->sdag[lev + 1].inv : nullptr;
+    uint32_t* key = dalloc<uint32_t>(c, (n ? n : 1) * kRecWords);
+    uint8_t* inv = dalloc<uint8_t>(c, n + 1);
+    if (n) k_sdag_keys<<<div_up(n, kB), kB, 0, c->stream>>>(d.rec, n, o->sdagMap[lev + 1], key, inv);
+    check_launch("k_sdag_keys");
+    if (lev == 0) {
+      // root: children remapped, never reduced, invariant bits stay clear
+      s.rec = key;
+      s.n = n;
+      s.inv = inv;
+      fill_u8(c, inv, 0, n);
+      o->sdagMap[0] = nullptr;
+      break;
+    }
+    const uint32_t cap = table_cap(n);
+    uint32_t* table = dalloc<uint32_t>(c, cap);
+    uint32_t* slot = dalloc<uint32_t>(c, n);
+    uint32_t* cand = dalloc<uint32_t>(c, n * 7);
+    uint32_t* state = dalloc<uint32_t>(c, n);
+    uint32_t* pos = dalloc<uint32_t>(c, n + 1);
+    fill_u32(c, table, kNone, cap);
+    k_insert<<<div_up(n, kB), kB, 0, c->stream>>>(key, n, table, cap - 1, slot);
+    k_sdag_cands<<<div_up(n, kB), kB, 0, c->stream>>>(key, n, table, cap - 1, slot, childInv, cand, state);
+    check_launch("k_sdag_cands");
+    for (int round = 0;; ++round) {
+      fill_u32(c, count, 0u, 1);
+      k_sdag_resolve<<<div_up(n, kB), kB, 0, c->stream>>>(cand, n, state, count);
+      check_launch("k_sdag_resolve");
+      if (read_u32(c, count) == 0) break;
+      if (round > (int)n + 2) throw Error{SVD_ERR_DEVICE, "toSDAG: resolution did not converge"};
+    }
+    k_state_flags<<<div_up(n, kB), kB, 0, c->stream>>>(state, n, pos);
+    scan_exclusive(c, pos, pos, n, count);
+    s.n = read_u32(c, count);
+    s.rec = dalloc<uint32_t>(c, (s.n ? s.n : 1) * kRecWords);
+    s.inv = dalloc<uint8_t>(c, s.n + 1);
+    o->sdagMap[lev] = dalloc<uint32_t>(c, n + 1);
+    k_sdag_outcome<<<div_up(n, kB), kB, 0, c->stream>>>(key, n, table, cap - 1, slot, state, pos, childInv, inv,
+                                                         o->sdagMap[lev], s.rec, s.inv, err);
+    check_launch("k_sdag_outcome");
+    nNodes += s.n;
+    dfree(c, key); dfree(c, inv); dfree(c, table); dfree(c, slot); dfree(c, cand); dfree(c, state); dfree(c, pos);
+  }
+  const uint64_t nerr = read_u32(c, err);
+  dfree(c, count); dfree(c, err);
+  if (nerr) throw Error{SVD_ERR_DEVICE, "toSDAG: node without correspondence"};
+  o->nNodesSDAG = nNodes;
+  o->state = SVD_S_SDAG;
+}
+
+}  // namespace svd

@@ -1,0 +1,498 @@
+// Exact emulation of the reference-count ordering of the SSVDAG encoder:
+//   std::sort(hist.begin(), hist.end(),
+//             [](pair a, pair b) { return a.second > b.second; });
+// (encoded_ssvdag.cpp:273-275, libstdc++ introsort: median-of-3 pivot moved
+// to the front, Hoare "unguarded partition", 16-element threshold, heapsort
+// once the depth budget 2*floor(log2 n) is spent, final insertion sort).
+//
+// The comparator looks at the count only, so the permutation of equal counts
+// depends on every swap the library makes.  One partition step is restated
+// data-parallel: the k-th stop of the left scanner (value <= pivot) is swapped
+// with the k-th stop of the right scanner (value >= pivot) for every k with
+// L_k < R_k, and the cut is L_K if L_K < R_{K-1} else R_{K-1}.  Partitions of
+// disjoint segments are independent, so all segments of one depth run
+// together.  The final insertion sort never moves an element across a
+// partition boundary, so it reduces to a stable insertion sort of each
+// <=16-element leaf segment.
+#include "svd_internal.hpp"
+#include "block_scan.hpp"
+
+namespace svd {
+
+namespace {
+
+constexpr int kThreshold = 16;
+constexpr int kLdsN = 2048;           // segments up to this size finish in one workgroup
+constexpr int kBT = 256;              // threads per workgroup
+constexpr int kPer = kLdsN / kBT;     // elements per thread in the workgroup phase
+constexpr int kTile = 2048;           // elements per workgroup in the device-wide rounds
+
+struct Seg { uint32_t first, last; int32_t depth; uint32_t pad; };
+
+__device__ __forceinline__ bool comp(uint32_t a, uint32_t b) { return a > b; }
+
+// ---- libstdc++ heap routines (bits/stl_heap.h) on a generic array view -----
+template <typename Arr>
+__device__ void adjust_heap(Arr& a, int64_t hole, int64_t len, uint32_t vv, uint32_t vi) {
+  const int64_t top = hole;
+  int64_t sc = hole;
+  while (sc < (len - 1) / 2) {
+    sc = 2 * (sc + 1);
+    if (comp(a.v(sc), a.v(sc - 1))) sc--;
+    a.set(hole, a.v(sc), a.i(sc));
+    hole = sc;
+  }
+  if ((len & 1) == 0 && sc == (len - 2) / 2) {
+    sc = 2 * (sc + 1);
+    a.set(hole, a.v(sc - 1), a.i(sc - 1));
+    hole = sc - 1;
+  }
+  int64_t parent = (hole - 1) / 2;
+  while (hole > top && comp(a.v(parent), vv)) {
+    a.set(hole, a.v(parent), a.i(parent));
+    hole = parent;
+    parent = (hole - 1) / 2;
+  }
+  a.set(hole, vv, vi);
+}
+
+// std::__partial_sort(first, last, last) == make_heap + sort_heap
+template <typename Arr>
+__device__ void heap_sort(Arr& a, int64_t len) {
+  if (len >= 2) {
+    int64_t parent = (len - 2) / 2;
+    while (true) {
+      uint32_t vv = a.v(parent), vi = a.i(parent);
+      adjust_heap(a, parent, len, vv, vi);
+      if (parent == 0) break;
+      parent--;
+    }
+  }
+  int64_t last = len;
+  while (last > 1) {
+    --last;
+    uint32_t vv = a.v(last), vi = a.i(last);
+    a.set(last, a.v(0), a.i(0));
+    adjust_heap(a, 0, last, vv, vi);
+  }
+}
+
+// Stable insertion sort (the effect of __final_insertion_sort on one leaf).
+template <typename Arr>
+__device__ void insertion_sort(Arr& a, int64_t len) {
+  for (int64_t i = 1; i < len; ++i) {
+    uint32_t vv = a.v(i), vi = a.i(i);
+    int64_t j = i;
+    while (j > 0 && comp(vv, a.v(j - 1))) { a.set(j, a.v(j - 1), a.i(j - 1)); --j; }
+    a.set(j, vv, vi);
+  }
+}
+
+// __move_median_to_first(first, first+1, mid, last-1): returns the position
+// whose element is swapped to `first`.
+template <typename V>
+__device__ __forceinline__ int64_t median_pos(const V& val, int64_t first, int64_t last) {
+  const int64_t mid = first + (last - first) / 2;
+  const int64_t A = first + 1, B = mid, C = last - 1;
+  const uint32_t a = val(A), b = val(B), c = val(C);
+  if (comp(a, b)) {
+    if (comp(b, c)) return B;
+    if (comp(a, c)) return C;
+    return A;
+  }
+  if (comp(a, c)) return A;
+  if (comp(b, c)) return C;
+  return B;
+}
+
+struct GArr {   // global-memory view of one segment
+  uint32_t* vp; uint32_t* ip;
+  __device__ uint32_t v(int64_t k) const { return vp[k]; }
+  __device__ uint32_t i(int64_t k) const { return ip[k]; }
+  __device__ void set(int64_t k, uint32_t vv, uint32_t ii) { vp[k] = vv; ip[k] = ii; }
+};
+struct LArr {   // LDS view of one segment
+  uint32_t* vp; uint32_t* ip;
+  __device__ uint32_t v(int64_t k) const { return vp[k]; }
+  __device__ uint32_t i(int64_t k) const { return ip[k]; }
+  __device__ void set(int64_t k, uint32_t vv, uint32_t ii) { vp[k] = vv; ip[k] = ii; }
+};
+
+struct Lists {
+  Seg* big; uint32_t* nbig;
+  Seg* small; uint32_t* nsmall;
+  Seg* heap; uint32_t* nheap;
+  Seg* leaf; uint32_t* nleaf;
+};
+
+__device__ __forceinline__ void classify(const Lists& L, uint32_t first, uint32_t last, int depth) {
+  const uint32_t size = last - first;
+  Seg s{first, last, depth, 0};
+  if (size <= (uint32_t)kThreshold) { if (size > 1) L.leaf[atomicAdd(L.nleaf, 1u)] = s; }
+  else if (size <= (uint32_t)kLdsN) L.small[atomicAdd(L.nsmall, 1u)] = s;
+  else if (depth == 0) L.heap[atomicAdd(L.nheap, 1u)] = s;
+  else L.big[atomicAdd(L.nbig, 1u)] = s;
+}
+
+// ---------------------------------------------------------------- device-wide rounds
+struct SegState { uint32_t pivot, K, nL, nR, blk0, nblk, cut, pad; };
+
+__global__ void k_g_median(const Seg* __restrict__ segs, uint32_t ns, uint32_t* val, uint32_t* idx,
+                           SegState* __restrict__ st, uint32_t* __restrict__ nblk) {
+  uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s > ns) return;
+  if (s == ns) { nblk[s] = 0; return; }
+  const Seg g = segs[s];
+  auto vf = [&](int64_t k) { return val[k]; };
+  const int64_t m = median_pos(vf, g.first, g.last);
+  const uint32_t tv = val[g.first], ti = idx[g.first];
+  val[g.first] = val[m]; idx[g.first] = idx[m];
+  val[m] = tv; idx[m] = ti;
+  SegState z{};
+  z.pivot = val[g.first];
+  st[s] = z;
+  nblk[s] = (g.last - g.first + kTile - 1) / kTile;
+}
+
+__device__ __forceinline__ int find_seg(const uint32_t* __restrict__ blk0, uint32_t ns, uint32_t b) {
+  // largest s with blk0[s] <= b (blk0 has ns+1 entries, blk0[ns] = total)
+  if (b >= blk0[ns]) return -1;
+  uint32_t lo = 0, hi = ns;
+  while (hi - lo > 1) {
+    uint32_t mid = (lo + hi) / 2;
+    if (blk0[mid] <= b) lo = mid; else hi = mid;
+  }
+  return (int)lo;
+}
+
+// Per-tile counts of left/right stoppers.
+__global__ void __launch_bounds__(kBT) k_g_count(const Seg* __restrict__ segs, uint32_t ns,
+                                                 const uint32_t* __restrict__ blk0, const SegState* __restrict__ st,
+                                                 const uint32_t* __restrict__ val, uint32_t* __restrict__ cntL,
+                                                 uint32_t* __restrict__ cntR) {
+  __shared__ uint32_t lds[kBT / 64 + 1];
+  const int s = find_seg(blk0, ns, blockIdx.x);
+  if (s < 0) return;
+  const Seg g = segs[s];
+  const uint32_t p = st[s].pivot;
+  const uint64_t base = (uint64_t)g.first + (uint64_t)(blockIdx.x - blk0[s]) * kTile;
+  uint32_t l = 0, r = 0;
+  for (int k = 0; k < kTile / kBT; ++k) {
+    const uint64_t pos = base + (uint64_t)k * kBT + threadIdx.x;
+    if (pos < g.last) {
+      const uint32_t v = val[pos];
+      l += (pos > g.first && v <= p) ? 1u : 0u;
+      r += (v >= p) ? 1u : 0u;
+    }
+  }
+  l = block_reduce_sum(l, lds);
+  r = block_reduce_sum(r, lds);
+  if (threadIdx.x == 0) { cntL[blockIdx.x] = l; cntR[blockIdx.x] = r; }
+}
+
+// Rank every stopper inside its segment; Lpos[first+k] / Rpos[first+k] = k-th stop.
+__global__ void __launch_bounds__(kBT) k_g_scatter(const Seg* __restrict__ segs, uint32_t ns,
+                                                   const uint32_t* __restrict__ blk0, SegState* __restrict__ st,
+                                                   const uint32_t* __restrict__ val, const uint32_t* __restrict__ exL,
+                                                   const uint32_t* __restrict__ exR, uint32_t* __restrict__ Lpos,
+                                                   uint32_t* __restrict__ Rpos) {
+  __shared__ uint32_t lds[kBT / 64 + 1];
+  const int s = find_seg(blk0, ns, blockIdx.x);
+  if (s < 0) return;
+  const Seg g = segs[s];
+  const uint32_t p = st[s].pivot;
+  const uint32_t b0 = blk0[s], b1 = blk0[s + 1];
+  const uint32_t segL0 = exL[b0], segR0 = exR[b0];
+  const uint32_t nR = exR[b1] - segR0;
+  if (blockIdx.x == b0 && threadIdx.x == 0) { st[s].nL = exL[b1] - segL0; st[s].nR = nR; }
+  const uint64_t base = (uint64_t)g.first + (uint64_t)(blockIdx.x - b0) * kTile;
+  uint32_t offL = exL[blockIdx.x] - segL0, offR = exR[blockIdx.x] - segR0;
+  for (int k = 0; k < kTile / kBT; ++k) {
+    const uint64_t pos = base + (uint64_t)k * kBT + threadIdx.x;
+    uint32_t fl = 0, fr = 0;
+    if (pos < g.last) {
+      const uint32_t v = val[pos];
+      fl = (pos > g.first && v <= p) ? 1u : 0u;
+      fr = (v >= p) ? 1u : 0u;
+    }
+    uint32_t aggL, aggR;
+    const uint32_t rl = block_exclusive_sum(fl, lds, aggL);
+    const uint32_t rr = block_exclusive_sum(fr, lds, aggR);
+    if (fl) Lpos[g.first + offL + rl] = (uint32_t)pos;
+    if (fr) Rpos[g.first + (nR - 1 - (offR + rr))] = (uint32_t)pos;
+    offL += aggL;
+    offR += aggR;
+  }
+}
+
+// Swap the k-th left stop with the k-th right stop while L_k < R_k; record K.
+__global__ void __launch_bounds__(kBT) k_g_swap(const Seg* __restrict__ segs, uint32_t ns,
+                                                const uint32_t* __restrict__ blk0, SegState* __restrict__ st,
+                                                uint32_t* val, uint32_t* idx, const uint32_t* __restrict__ Lpos,
+                                                const uint32_t* __restrict__ Rpos) {
+  const int s = find_seg(blk0, ns, blockIdx.x);
+  if (s < 0) return;
+  const Seg g = segs[s];
+  const uint32_t m = min(st[s].nL, st[s].nR);
+  const uint64_t kb = (uint64_t)(blockIdx.x - blk0[s]) * kTile;
+  for (int j = 0; j < kTile / kBT; ++j) {
+    const uint64_t k = kb + (uint64_t)j * kBT + threadIdx.x;
+    if (k >= m) break;
+    const uint32_t l = Lpos[g.first + k], r = Rpos[g.first + k];
+    if (l < r) {
+      const uint32_t tv = val[l], ti = idx[l];
+      val[l] = val[r]; idx[l] = idx[r];
+      val[r] = tv; idx[r] = ti;
+      const bool nextOk = (k + 1 < m) && (Lpos[g.first + k + 1] < Rpos[g.first + k + 1]);
+      if (!nextOk) st[s].K = (uint32_t)(k + 1);
+    }
+  }
+}
+
+__global__ void k_g_split(const Seg* __restrict__ segs, uint32_t ns, const SegState* __restrict__ st,
+                          const uint32_t* __restrict__ Lpos, const uint32_t* __restrict__ Rpos, Lists L) {
+  uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= ns) return;
+  const Seg g = segs[s];
+  const SegState z = st[s];
+  const uint32_t rprev = z.K ? Rpos[g.first + z.K - 1] : g.last;
+  const uint32_t cut = (z.K < z.nL && Lpos[g.first + z.K] < rprev) ? Lpos[g.first + z.K] : rprev;
+  classify(L, cut, g.last, g.depth - 1);
+  classify(L, g.first, cut, g.depth - 1);
+}
+
+// ---------------------------------------------------------------- per-thread finishers
+__global__ void k_leaf_sort(const Seg* __restrict__ segs, const uint32_t* __restrict__ ns, uint32_t* val, uint32_t* idx) {
+  uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= *ns) return;
+  const Seg g = segs[s];
+  GArr a{val + g.first, idx + g.first};
+  insertion_sort(a, (int64_t)(g.last - g.first));
+}
+
+__global__ void k_heap_sort(const Seg* __restrict__ segs, const uint32_t* __restrict__ ns, uint32_t* val, uint32_t* idx) {
+  uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= *ns) return;
+  const Seg g = segs[s];
+  GArr a{val + g.first, idx + g.first};
+  heap_sort(a, (int64_t)(g.last - g.first));
+  insertion_sort(a, (int64_t)(g.last - g.first));   // no-op on sorted input (kept for exactness)
+}
+
+// ---------------------------------------------------------------- workgroup phase
+// One workgroup runs the complete introsort recursion of one segment of
+// 17..2048 elements in LDS, all subsegments of one depth together.
+__global__ void __launch_bounds__(kBT) k_block_sort(const Seg* __restrict__ segs, const uint32_t* __restrict__ nsp,
+                                                    uint32_t* val, uint32_t* idx) {
+  __shared__ uint32_t sv[kLdsN], si[kLdsN];
+  __shared__ uint16_t segOf[kLdsN];
+  __shared__ uint16_t exL[kLdsN + 1], exR[kLdsN + 1];
+  __shared__ uint16_t Lp[kLdsN], Rp[kLdsN];
+  __shared__ uint16_t aFirst[2][128], aLast[2][128];
+  __shared__ int8_t aDepth[2][128];
+  __shared__ uint32_t aPivot[128];
+  __shared__ uint16_t aK[128], aCut[128], aLeft[128], aRight[128];
+  __shared__ uint32_t lds[kBT / 64 + 1];
+  __shared__ uint32_t nActive[2];
+
+  if (blockIdx.x >= *nsp) return;
+  const Seg g = segs[blockIdx.x];
+  const uint32_t n = g.last - g.first;
+  const int tid = threadIdx.x;
+  for (uint32_t k = tid; k < n; k += kBT) { sv[k] = val[g.first + k]; si[k] = idx[g.first + k]; segOf[k] = 0; }
+  if (tid == 0) {
+    nActive[0] = 0;
+    if (g.depth == 0) {
+      LArr a{sv, si};
+      heap_sort(a, n);
+    } else {
+      aFirst[0][0] = 0; aLast[0][0] = (uint16_t)n; aDepth[0][0] = (int8_t)g.depth;
+      nActive[0] = 1;
+    }
+  }
+  __syncthreads();
+  int cur = 0;
+  while (nActive[cur] > 0) {
+    const uint32_t na = nActive[cur];
+    const int nxt = cur ^ 1;
+    // 1. median-of-3 to the front, pivot
+    if (tid < (int)na) {
+      const int64_t f = aFirst[cur][tid], l = aLast[cur][tid];
+      auto vf = [&](int64_t k) { return sv[k]; };
+      const int64_t m = median_pos(vf, f, l);
+      const uint32_t tv = sv[f], ti = si[f];
+      sv[f] = sv[m]; si[f] = si[m]; sv[m] = tv; si[m] = ti;
+      aPivot[tid] = sv[f];
+      aK[tid] = 0;
+    }
+    if (tid == 0) nActive[nxt] = 0;
+    __syncthreads();
+    // 2. stopper flags + workgroup prefix (exclusive) over all n positions
+    uint32_t fl[kPer], fr[kPer], sl = 0, sr = 0;
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+      const uint32_t e = tid * kPer + k;
+      fl[k] = fr[k] = 0;
+      if (e < n) {
+        const uint32_t s = segOf[e];
+        if (s != 0xFFFFu) {
+          const uint32_t p = aPivot[s];
+          fl[k] = (e > aFirst[cur][s] && sv[e] <= p) ? 1u : 0u;
+          fr[k] = (sv[e] >= p) ? 1u : 0u;
+        }
+      }
+      sl += fl[k]; sr += fr[k];
+    }
+    uint32_t totL, totR;
+    uint32_t pl = block_exclusive_sum(sl, lds, totL);
+    uint32_t pr = block_exclusive_sum(sr, lds, totR);
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+      const uint32_t e = tid * kPer + k;
+      if (e < kLdsN) { exL[e] = (uint16_t)pl; exR[e] = (uint16_t)pr; }
+      pl += fl[k]; pr += fr[k];
+    }
+    if (tid == 0) { exL[kLdsN] = (uint16_t)totL; exR[kLdsN] = (uint16_t)totR; }
+    __syncthreads();
+    // 3. ranks -> Lp / Rp
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+      const uint32_t e = tid * kPer + k;
+      if (e >= n) continue;
+      const uint32_t s = segOf[e];
+      if (s == 0xFFFFu) continue;
+      const uint32_t f = aFirst[cur][s], l = aLast[cur][s];
+      if (fl[k]) Lp[f + (exL[e] - exL[f])] = (uint16_t)e;
+      if (fr[k]) Rp[f + (exR[l] - 1 - exR[e])] = (uint16_t)e;
+    }
+    __syncthreads();
+    // 4. paired swaps, K
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+      const uint32_t e = tid * kPer + k;
+      if (e >= n) continue;
+      const uint32_t s = segOf[e];
+      if (s == 0xFFFFu) continue;
+      const uint32_t f = aFirst[cur][s], l = aLast[cur][s];
+      const uint32_t nL = exL[l] - exL[f], nR = exR[l] - exR[f];
+      const uint32_t m = min(nL, nR);
+      const uint32_t kk = e - f;
+      if (kk >= m) continue;
+      const uint32_t a = Lp[f + kk], b = Rp[f + kk];
+      if (a < b) {
+        const uint32_t tv = sv[a], ti = si[a];
+        sv[a] = sv[b]; si[a] = si[b]; sv[b] = tv; si[b] = ti;
+        const bool nextOk = (kk + 1 < m) && (Lp[f + kk + 1] < Rp[f + kk + 1]);
+        if (!nextOk) aK[s] = (uint16_t)(kk + 1);
+      }
+    }
+    __syncthreads();
+    // 5. cut, children: <=16 -> insertion sort now, depth 0 -> heap sort now, else next round
+    if (tid < (int)na) {
+      const uint32_t f = aFirst[cur][tid], l = aLast[cur][tid];
+      const uint32_t nL = exL[l] - exL[f];
+      const uint32_t K = aK[tid];
+      const uint32_t rprev = K ? Rp[f + K - 1] : l;
+      const uint32_t cut = (K < nL && Lp[f + K] < rprev) ? Lp[f + K] : rprev;
+      const int d = aDepth[cur][tid] - 1;
+      aCut[tid] = (uint16_t)cut;
+      uint16_t ids[2];
+      const uint32_t cf[2] = {f, cut}, cl[2] = {cut, l};
+      for (int h = 0; h < 2; ++h) {
+        const uint32_t sz = cl[h] - cf[h];
+        ids[h] = 0xFFFFu;
+        LArr a{sv + cf[h], si + cf[h]};
+        if (sz <= (uint32_t)kThreshold) insertion_sort(a, sz);
+        else if (d == 0) heap_sort(a, sz);
+        else {
+          const uint32_t slotn = atomicAdd(&nActive[nxt], 1u);
+          aFirst[nxt][slotn] = (uint16_t)cf[h]; aLast[nxt][slotn] = (uint16_t)cl[h]; aDepth[nxt][slotn] = (int8_t)d;
+          ids[h] = (uint16_t)slotn;
+        }
+      }
+      aLeft[tid] = ids[0]; aRight[tid] = ids[1];
+    }
+    __syncthreads();
+    // 6. new segment ids
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+      const uint32_t e = tid * kPer + k;
+      if (e >= n) continue;
+      const uint32_t s = segOf[e];
+      if (s == 0xFFFFu) continue;
+      segOf[e] = e < aCut[s] ? aLeft[s] : aRight[s];
+    }
+    cur = nxt;
+    __syncthreads();
+  }
+  for (uint32_t k = tid; k < n; k += kBT) { val[g.first + k] = sv[k]; idx[g.first + k] = si[k]; }
+}
+
+}  // namespace
+
+void refsort(svd_ctx* c, uint32_t* idx, uint32_t* val, uint64_t n) {
+  if (n <= 1) return;
+  if (n > 0xFFFFFFF0ull) throw Error{SVD_ERR_CAPACITY, "refsort: level too large"};
+  int lg = 63 - __builtin_clzll(n);
+  const uint64_t capBig = n / kLdsN + 2;
+  const uint64_t capSeg = n + 2;
+  Lists L;
+  Seg* bigA = dalloc<Seg>(c, capBig);
+  Seg* bigB = dalloc<Seg>(c, capBig);
+  L.small = dalloc<Seg>(c, capSeg);
+  L.heap = dalloc<Seg>(c, capSeg);
+  L.leaf = dalloc<Seg>(c, capSeg);
+  uint32_t* counters = dalloc<uint32_t>(c, 8);
+  L.nbig = counters + 0; L.nsmall = counters + 1; L.nheap = counters + 2; L.nleaf = counters + 3;
+  fill_u32(c, counters, 0u, 8);
+  // root segment classification (host knows n)
+  Seg root{0, (uint32_t)n, 2 * lg, 0};
+  uint32_t hcnt[4] = {0, 0, 0, 0};
+  Seg* dst = nullptr;
+  if (n <= (uint64_t)kThreshold) { dst = L.leaf; hcnt[3] = 1; }
+  else if (n <= (uint64_t)kLdsN) { dst = L.small; hcnt[1] = 1; }
+  else { dst = bigA; hcnt[0] = 1; }
+  SVD_HIP(hipMemcpyAsync(dst, &root, sizeof(Seg), hipMemcpyHostToDevice, c->stream));
+  SVD_HIP(hipMemcpyAsync(counters, hcnt, sizeof(hcnt), hipMemcpyHostToDevice, c->stream));
+  uint32_t nbig = hcnt[0];
+  if (nbig) {
+    uint32_t* Lpos = dalloc<uint32_t>(c, n);
+    uint32_t* Rpos = dalloc<uint32_t>(c, n);
+    SegState* st = dalloc<SegState>(c, capBig);
+    uint32_t* blk0 = dalloc<uint32_t>(c, capBig + 1);
+    const uint64_t maxBlocks = n / kTile + capBig + 2;
+    uint32_t* cntL = dalloc<uint32_t>(c, maxBlocks + 1);
+    uint32_t* cntR = dalloc<uint32_t>(c, maxBlocks + 1);
+    Seg* cur = bigA;
+    Seg* nxt = bigB;
+    while (nbig) {
+      L.big = nxt;
+      fill_u32(c, L.nbig, 0u, 1);
+      k_g_median<<<div_up(nbig + 1, 256), 256, 0, c->stream>>>(cur, nbig, val, idx, st, blk0);
+      scan_exclusive(c, blk0, blk0, nbig + 1, nullptr);
+      const uint32_t grid = (uint32_t)(n / kTile + nbig + 1);
+      fill_u32(c, cntL, 0u, grid + 1);
+      fill_u32(c, cntR, 0u, grid + 1);
+      k_g_count<<<grid, kBT, 0, c->stream>>>(cur, nbig, blk0, st, val, cntL, cntR);
+      scan_exclusive(c, cntL, cntL, grid + 1, nullptr);
+      scan_exclusive(c, cntR, cntR, grid + 1, nullptr);
+      k_g_scatter<<<grid, kBT, 0, c->stream>>>(cur, nbig, blk0, st, val, cntL, cntR, Lpos, Rpos);
+      k_g_swap<<<grid, kBT, 0, c->stream>>>(cur, nbig, blk0, st, val, idx, Lpos, Rpos);
+      k_g_split<<<div_up(nbig, 256), 256, 0, c->stream>>>(cur, nbig, st, Lpos, Rpos, L);
+      check_launch("refsort round");
+      nbig = (uint32_t)read_u32(c, L.nbig);
+      std::swap(cur, nxt);
+    }
+    dfree(c, Lpos); dfree(c, Rpos); dfree(c, st); dfree(c, blk0); dfree(c, cntL); dfree(c, cntR);
+  }
+  // finishers: the lists only grow now; read their sizes once
+  uint32_t h[4];
+  read_u32s(c, counters, h, 4);
+  if (h[1]) k_block_sort<<<h[1], kBT, 0, c->stream>>>(L.small, L.nsmall, val, idx);
+  if (h[2]) k_heap_sort<<<div_up(h[2], 64), 64, 0, c->stream>>>(L.heap, L.nheap, val, idx);
+  if (h[3]) k_leaf_sort<<<div_up(h[3], 64), 64, 0, c->stream>>>(L.leaf, L.nleaf, val, idx);
+  check_launch("refsort finish");
+  dfree(c, bigA); dfree(c, bigB); dfree(c, L.small); dfree(c, L.heap); dfree(c, L.leaf); dfree(c, counters);
+}
+
+}  // namespace svd

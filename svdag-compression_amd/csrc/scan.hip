@@ -1,0 +1,113 @@
+// Device-wide exclusive prefix sums (reduce / scan-partials / down-sweep),
+// wave64 shuffles + LDS, 2048 items per 256-thread workgroup.
+#include "svd_internal.hpp"
+#include "block_scan.hpp"
+
+namespace svd {
+
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kItems = 8;
+constexpr int kTile = kThreads * kItems;
+
+__global__ void __launch_bounds__(kThreads) k_scan_reduce(const uint32_t* __restrict__ in, uint64_t n,
+                                                          uint32_t* __restrict__ partial) {
+  __shared__ uint32_t lds[kThreads / 64];
+  const uint64_t base = (uint64_t)blockIdx.x * kTile;
+  uint32_t s = 0;
+#pragma unroll
+  for (int k = 0; k < kItems; ++k) {
+    uint64_t i = base + (uint64_t)k * kThreads + threadIdx.x;
+    if (i < n) s += in[i];
+  }
+  s = block_reduce_sum(s, lds);
+  if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+__global__ void __launch_bounds__(kThreads) k_scan_partials(uint32_t* __restrict__ partial, uint32_t np,
+                                                            uint32_t* __restrict__ total) {
+  __shared__ uint32_t lds[kThreads / 64 + 1];
+  uint32_t carry = 0;
+  for (uint32_t base = 0; base < np; base += kThreads) {
+    uint32_t i = base + threadIdx.x;
+    uint32_t v = i < np ? partial[i] : 0u;
+    uint32_t agg;
+    uint32_t ex = block_exclusive_sum(v, lds, agg);
+    if (i < np) partial[i] = carry + ex;
+    carry += agg;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0 && total) *total = carry;
+}
+
+__global__ void __launch_bounds__(kThreads) k_scan_down(const uint32_t* __restrict__ in, uint32_t* __restrict__ out,
+                                                        uint64_t n, const uint32_t* __restrict__ partial) {
+  __shared__ uint32_t lds[kThreads / 64 + 1];
+  const uint64_t base = (uint64_t)blockIdx.x * kTile + (uint64_t)threadIdx.x * kItems;
+  uint32_t v[kItems];
+  uint32_t s = 0;
+#pragma unroll
+  for (int k = 0; k < kItems; ++k) {
+    uint64_t i = base + k;
+    v[k] = i < n ? in[i] : 0u;
+    s += v[k];
+  }
+  uint32_t agg;
+  uint32_t ex = block_exclusive_sum(s, lds, agg) + partial[blockIdx.x];
+#pragma unroll
+  for (int k = 0; k < kItems; ++k) {
+    uint64_t i = base + k;
+    if (i < n) out[i] = ex;
+    ex += v[k];
+  }
+}
+
+__global__ void k_fill(uint32_t* d, uint32_t v, uint64_t n) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) d[i] = v;
+}
+__global__ void k_fill8(uint8_t* d, uint8_t v, uint64_t n) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) d[i] = v;
+}
+__global__ void k_iota(uint32_t* d, uint64_t n) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) d[i] = (uint32_t)i;
+}
+
+}  // namespace
+
+void scan_exclusive(svd_ctx* c, const uint32_t* in, uint32_t* out, uint64_t n, uint32_t* total) {
+  if (n == 0) {
+    if (total) fill_u32(c, total, 0u, 1);
+    return;
+  }
+  const uint32_t nb = div_up(n, kTile);
+  uint32_t* partial = dalloc<uint32_t>(c, nb);
+  k_scan_reduce<<<nb, kThreads, 0, c->stream>>>(in, n, partial);
+  k_scan_partials<<<1, kThreads, 0, c->stream>>>(partial, nb, total);
+  k_scan_down<<<nb, kThreads, 0, c->stream>>>(in, out, n, partial);
+  check_launch("scan_exclusive");
+  dfree(c, partial);
+}
+
+void fill_u32(svd_ctx* c, uint32_t* d, uint32_t v, uint64_t n) {
+  if (!n) return;
+  k_fill<<<div_up(n, 256), 256, 0, c->stream>>>(d, v, n);
+  check_launch("fill_u32");
+}
+
+void fill_u8(svd_ctx* c, uint8_t* d, uint8_t v, uint64_t n) {
+  if (!n) return;
+  k_fill8<<<div_up(n, 256), 256, 0, c->stream>>>(d, v, n);
+  check_launch("fill_u8");
+}
+
+void iota_u32(svd_ctx* c, uint32_t* d, uint64_t n) {
+  if (!n) return;
+  k_iota<<<div_up(n, 256), 256, 0, c->stream>>>(d, n);
+  check_launch("iota_u32");
+}
+
+}  // namespace svd

@@ -1,0 +1,182 @@
+// Internal declarations of libsvdag_amd (MI355X / gfx950 only).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+#include <map>
+#include "../../include/svdag_amd.h"
+
+namespace svd {
+
+constexpr uint32_t kNull = 0xFFFFFFFEu;    // octree.cpp:22 null child pointer
+constexpr uint32_t kNone = 0xFFFFFFFFu;    // "no entry" marker in device tables
+constexpr int kRecWords = 9;               // node record: mask|mX<<8|mY<<16|mZ<<24, children[8]
+constexpr uint32_t kMaxLevels = 21;        // 3*21 = 63 Morton bits
+
+struct Error {
+  int code;
+  std::string msg;
+};
+
+// ---- device memory: a small caching allocator per context ----------------
+class Arena {
+ public:
+  explicit Arena(int dev) : dev_(dev) { (void)dev_; }
+  ~Arena();
+  void* get(size_t bytes);                  // throws Error on failure
+  void put(void* p);
+  void trim();
+ private:
+  int dev_;
+  std::multimap<size_t, void*> free_;        // size class -> block
+  std::map<void*, size_t> live_;             // block -> size class
+};
+
+}  // namespace svd
+
+struct svd_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  svd::Arena* arena = nullptr;
+  uint64_t* pinned = nullptr;                // small pinned staging for size read-backs
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  std::map<std::string, double> kernel_ms;   // last measured per family
+};
+
+namespace svd {
+
+// Level arrays on the device.  Inner levels hold 9-word records; the finest
+// level (leaf 2^3 masks) holds one word per node (the mask) when leaf=true.
+struct Level {
+  uint32_t* rec = nullptr;        // n * (leaf ? 1 : 9) words
+  uint8_t* inv = nullptr;         // invariant bits (SDAG view only)
+  uint32_t* childLevels = nullptr;// 8 per node (cross-level view only)
+  uint64_t n = 0;
+  bool leaf = false;
+};
+
+struct Encoded {
+  uint8_t* d = nullptr;           // full file image on the device
+  uint64_t bytes = 0;
+  bool valid = false;
+};
+
+}  // namespace svd
+
+struct svd_octree {
+  svd_ctx* ctx = nullptr;
+  uint32_t levels = 0;
+  int state = SVD_S_EMPTY;
+  float bbox[6] = {0, 0, 0, 1, 1, 1};
+  double rootSide = 1.0;
+  uint64_t nVoxels = 0;
+  uint64_t nNodesSVO = 0, nNodesDAG = 0, nNodesSDAG = 0, nNodesX = 0;
+  uint64_t crossMerged = 0;
+  bool hasX = false;              // cross-level view present
+  std::vector<svd::Level> svo, dag, sdag, xdag;
+  // toSDAG map of every DAG level: sdag id (bits 0..28) | mirror xyz << 29
+  std::vector<uint32_t*> sdagMap;
+  svd::Encoded enc[4];
+};
+
+struct svd_scene {
+  svd_ctx* ctx = nullptr;
+  int fmt = 0;
+  uint32_t levels = 0;
+  float bmin[3], bmax[3], center[3], rootHalfSide;
+  const uint32_t* nodes = nullptr;     // SVDAG / USSVDAG words
+  const uint16_t* inner = nullptr;     // SSVDAG inner u16
+  const uint8_t* leaves = nullptr;     // SSVDAG leaf bytes (8 per brick)
+  uint32_t* levelOffsets = nullptr;    // device copy
+  uint32_t nWords = 0, nInner = 0, nLeaves = 0;   // element counts (bounds checks)
+  uint8_t* owned = nullptr;            // owned device image (when loaded from bytes)
+  uint32_t nLevelOffsets = 0;
+};
+
+namespace svd {
+
+// ---- error helpers ----------------------------------------------------------
+#define SVD_HIP(call)                                                                  \
+  do {                                                                                 \
+    hipError_t e_ = (call);                                                            \
+    if (e_ != hipSuccess)                                                              \
+      throw ::svd::Error{SVD_ERR_DEVICE, std::string(#call) + ": " + hipGetErrorString(e_)}; \
+  } while (0)
+
+inline void check_launch(const char* what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) throw Error{SVD_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e)};
+}
+
+inline uint32_t div_up(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
+
+template <typename T>
+inline T* dalloc(svd_ctx* c, uint64_t count) {
+  return (T*)c->arena->get(count * sizeof(T) + 16);
+}
+inline void dfree(svd_ctx* c, void* p) { if (p) c->arena->put(p); }
+
+// Read back a few u32/u64 device values into host memory (synchronises).
+uint64_t read_u32(svd_ctx* c, const uint32_t* d);
+void read_u32s(svd_ctx* c, const uint32_t* d, uint32_t* h, size_t n);
+
+// ---- scan (scan.hip) ---------------------------------------------------------
+// Exclusive prefix sum of n u32 values; out may alias in.  *total (device) gets
+// the sum (may be null).  Scratch comes from the context arena.
+void scan_exclusive(svd_ctx* c, const uint32_t* in, uint32_t* out, uint64_t n, uint32_t* total);
+void fill_u32(svd_ctx* c, uint32_t* d, uint32_t v, uint64_t n);
+void iota_u32(svd_ctx* c, uint32_t* d, uint64_t n);
+void fill_u8(svd_ctx* c, uint8_t* d, uint8_t v, uint64_t n);
+
+// ---- build / reduction (build.hip) -----------------------------------------
+void build_svo_from_morton(svd_octree* o, const uint64_t* dcodes, uint64_t n);
+void reduce_dag(svd_octree* o);
+void reduce_sym(svd_octree* o);
+void cross_level_merge(svd_octree* o);
+
+// ---- refcount order: exact emulation of libstdc++ std::sort (refsort.hip) --
+// Sorts (idx, val) pairs in place exactly like
+// std::sort(first, last, [](a, b) { return a.second > b.second; })
+// (encoded_ssvdag.cpp:273-275).
+void refsort(svd_ctx* c, uint32_t* idx, uint32_t* val, uint64_t n);
+
+// ---- encoders (encode.hip) -----------------------------------------------------
+void encode(svd_octree* o, int fmt);
+
+// ---- traversal (trace.hip) -----------------------------------------------------
+void trace_frame(svd_scene* s, const svd_trace_params* p, svd_ray_result* prim, svd_ray_result* shad);
+
+// ---- timing helpers ------------------------------------------------------------
+struct Timer {
+  svd_ctx* c;
+  const char* name;
+  Timer(svd_ctx* ctx, const char* n) : c(ctx), name(n) { (void)hipEventRecord(c->ev[0], c->stream); }
+  ~Timer() {
+    (void)hipEventRecord(c->ev[1], c->stream);
+    if (hipEventSynchronize(c->ev[1]) == hipSuccess) {
+      float ms = 0.f;
+      if (hipEventElapsedTime(&ms, c->ev[0], c->ev[1]) == hipSuccess) c->kernel_ms[name] = ms;
+    }
+  }
+};
+
+// ---- device helpers --------------------------------------------------------------
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {
+  z ^= z >> 30; z *= 0xbf58476d1ce4e5b9ull;
+  z ^= z >> 27; z *= 0x94d049bb133111ebull;
+  z ^= z >> 31;
+  return z;
+}
+
+__device__ __forceinline__ uint64_t hash_rec(const uint32_t w[kRecWords]) {
+  uint64_t h = 0x9e3779b97f4a7c15ull;
+#pragma unroll
+  for (int i = 0; i < kRecWords; ++i) h = mix64(h ^ ((uint64_t)w[i] + ((uint64_t)i << 40)));
+  return h;
+}
+
+}  // namespace svd

@@ -1,0 +1,413 @@
+// Per-ray octree DDA traversal (shaders/octree_dda.frag.glsl:99-581) as a
+// compute kernel: one ray per lane, 16x16 pixel tiles per workgroup, the
+// descent stack (INNER_LEVELS+1 entries) in LDS laid out [depth][lane].
+// FP32 throughout; this file is compiled with -ffp-contract=off so every
+// product / sum rounds exactly like the shader's individual GLSL operations.
+// min/max follow the GLSL definitions: min(x,y) = y<x ? y : x.
+#include "svd_internal.hpp"
+
+namespace svd {
+
+namespace {
+
+constexpr int kTW = 16, kTH = 16, kBT = kTW * kTH;
+
+struct Dev {
+  const uint32_t* nodes;
+  const uint16_t* inner;
+  const uint2* leaves;
+  const uint32_t* levelOffsets;
+  uint32_t nWords, nInner, nLeaves;   // element counts, for bounds checks
+  int innerLevels, leafSize;
+  float bmin[3], bmax[3], center[3], rootHalfSide;
+};
+
+__device__ __forceinline__ float gmin(float x, float y) { return y < x ? y : x; }
+__device__ __forceinline__ float gmax(float x, float y) { return x < y ? y : x; }
+__device__ __forceinline__ float gsign(float x) { return x > 0.f ? 1.f : (x < 0.f ? -1.f : 0.f); }
+__device__ __forceinline__ int isign(float x) { return x > 0.f ? 1 : (x < 0.f ? -1 : 0); }
+
+struct TS {
+  float t_current;
+  int node_index;
+  uint32_t hdr;
+  int mm0, mm1, mm2;
+  uint32_t leaf_lo, leaf_hi;
+  int idx[3];
+  int li[3];
+  uint32_t cli;
+  float tn[3];
+  float inv[3];
+  int di[3];
+  int cns;
+  float cell;
+  uint32_t level;
+};
+
+struct Stack {
+  uint32_t* node;   // [depth * kBT + lane]
+  uint32_t* hdr;
+  uint32_t* msk;
+  int lane;
+  uint32_t size;
+};
+
+template <int FMT>
+struct Tracer {
+  const Dev& s;
+  Stack st;
+  uint32_t path;
+  uint32_t bytes;
+  bool bad;   // a fetch fell outside the scene arrays (malformed file)
+
+  __device__ Tracer(const Dev& d, Stack k) : s(d), st(k), path(2166136261u), bytes(0), bad(false) {}
+
+  __device__ __forceinline__ uint32_t word(uint32_t i) {
+    if (i < s.nWords) return s.nodes[i];
+    bad = true;
+    return 0u;
+  }
+  __device__ __forceinline__ uint32_t half(uint32_t i) {
+    if (i < s.nInner) return s.inner[i];
+    bad = true;
+    return 0u;
+  }
+
+  __device__ __forceinline__ void push(int node, uint32_t hdr, int m0, int m1, int m2, uint32_t level) {
+    const int mask = m0 | (m1 << 1) | (m2 << 2) | (int)(level << 3);
+    const uint32_t o = st.size * kBT + st.lane;
+    st.node[o] = (uint32_t)node; st.hdr[o] = hdr; st.msk[o] = (uint32_t)mask;
+    ++st.size;
+  }
+  __device__ __forceinline__ void pop(TS& ts) {
+    --st.size;
+    const uint32_t o = st.size * kBT + st.lane;
+    ts.node_index = (int)st.node[o];
+    ts.hdr = st.hdr[o];
+    const int mask = (int)st.msk[o];
+    ts.mm0 = mask & 1; ts.mm1 = (mask >> 1) & 1; ts.mm2 = (mask >> 2) & 1;
+    ts.level = (uint32_t)(mask >> 3) & 255u;
+  }
+  __device__ __forceinline__ static uint32_t lin_idx(const TS& ts, int sz) {
+    const int i0 = (1 - 2 * ts.mm0) * ts.li[0] + ts.mm0 * (sz - 1);
+    const int i1 = (1 - 2 * ts.mm1) * ts.li[1] + ts.mm1 * (sz - 1);
+    const int i2 = (1 - 2 * ts.mm2) * ts.li[2] + ts.mm2 * (sz - 1);
+    return (uint32_t)(i2 + sz * (i1 + sz * i0));
+  }
+  __device__ __forceinline__ bool fetch_voxel_bit(const TS& ts) const {
+    if (FMT != SVD_FMT_SSVDAG) return (ts.hdr & (1u << ts.cli)) != 0;
+    if ((int)ts.level < s.innerLevels) return ((ts.hdr >> (ts.cli << 1)) & 3u) != 0;
+    const uint32_t word = ts.cli & 32u;
+    const uint32_t bm = 1u << (ts.cli & 31u);
+    return ((word == 0 ? ts.leaf_lo : ts.leaf_hi) & bm) != 0;
+  }
+  __device__ __forceinline__ void fetch_data(TS& ts) {
+    if (FMT != SVD_FMT_SSVDAG) { ts.hdr = word((uint32_t)ts.node_index); bytes += 4; return; }
+    if ((int)ts.level < s.innerLevels) {
+      ts.hdr = half((uint32_t)(ts.node_index + (int)s.levelOffsets[ts.level]));
+      bytes += 2;
+    } else {
+      uint2 v = make_uint2(0u, 0u);
+      if ((uint32_t)ts.node_index < s.nLeaves) v = s.leaves[(uint32_t)ts.node_index];
+      else bad = true;
+      bytes += 8;
+      ts.leaf_lo = v.x; ts.leaf_hi = v.y;
+    }
+  }
+  __device__ __forceinline__ void fetch_child_index_in(TS& ts) {
+    if (FMT != SVD_FMT_SSVDAG) {
+      const int pos = __popc((ts.hdr & 0xFFu) >> ts.cli);
+      ts.node_index = (int)word((uint32_t)(ts.node_index + pos));
+      bytes += 4;
+      if (FMT == SVD_FMT_USSVDAG) {
+        ts.mm0 ^= (ts.hdr & (1u << (ts.cli + 8))) != 0;
+        ts.mm1 ^= (ts.hdr & (1u << (ts.cli + 16))) != 0;
+        ts.mm2 ^= (ts.hdr & (1u << (ts.cli + 24))) != 0;
+      }
+      return;
+    }
+    int off = 1 + (int)s.levelOffsets[ts.level] + ts.node_index;
+    // childIndir table (octree_dda_renderer.cpp:533-561), computed in registers
+    for (int i = 7; i > (int)ts.cli; --i) { const int m = (int)((ts.hdr >> (2 * i)) & 3u); off += m < 2 ? m : 2; }
+    const int cm = (int)((ts.hdr >> (ts.cli << 1)) & 3u);
+    int cx = (int)half((uint32_t)off);
+    bytes += 2;
+    const int m0 = (cx >> 13) & 1, m1 = (cx >> 14) & 1, m2 = (cx >> 15) & 1;
+    cx &= -57345;
+    if (cm > 1) {
+      const int cy = (int)half((uint32_t)off + 1);
+      bytes += 2;
+      cx = (cx << 16) | cy;
+      cx |= (cm & 1) << 29;
+    }
+    ts.mm0 ^= m0; ts.mm1 ^= m1; ts.mm2 ^= m2;
+    ts.node_index = cx;
+  }
+  __device__ __forceinline__ static void next_crossing(TS& ts, const float o[3]) {
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      const int dic = ts.di[a] > 0 ? ts.di[a] : 0;
+      const float pn = (float)(ts.idx[a] + dic) * ts.cell;
+      ts.tn[a] = (pn - o[a]) * ts.inv[a];
+    }
+  }
+  __device__ __forceinline__ static void next_mask(const TS& ts, int m[3]) {
+    m[0] = (ts.tn[0] < ts.tn[1]) & (ts.tn[0] <= ts.tn[2]);
+    m[1] = (ts.tn[1] < ts.tn[2]) & (ts.tn[1] <= ts.tn[0]);
+    m[2] = (ts.tn[2] < ts.tn[0]) & (ts.tn[2] <= ts.tn[1]);
+  }
+  __device__ __forceinline__ void dda_next(TS& ts, int delta[3]) {
+    int m[3];
+    next_mask(ts, m);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) { delta[a] = m[a] * ts.di[a]; ts.idx[a] += delta[a]; ts.li[a] += delta[a]; }
+    float tc = (float)m[0] * ts.tn[0];
+    tc = tc + (float)m[1] * ts.tn[1];
+    tc = tc + (float)m[2] * ts.tn[2];
+    ts.t_current = tc;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) ts.tn[a] = ts.tn[a] + ((float)m[a] * ts.cell) * fabsf(ts.inv[a]);
+  }
+  __device__ __forceinline__ static bool in_bounds(const int li[3], int sz) {
+    return li[0] < sz && li[1] < sz && li[2] < sz && 0 <= li[0] && 0 <= li[1] && 0 <= li[2];
+  }
+  __device__ __forceinline__ void up_in(const float o[3], TS& ts) {
+    uint32_t dl = ts.level;
+    pop(ts);
+    dl -= ts.level;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) ts.idx[a] >>= dl;
+    ts.cell *= (float)(1 << dl);
+    ts.cns = (int)ts.level < s.innerLevels ? 2 : s.leafSize;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) ts.li[a] = ts.idx[a] & 1;
+    next_crossing(ts, o);
+  }
+  __device__ __forceinline__ void go_down_one_level(const float o[3], const float d[3], TS& ts) {
+    ++ts.level;
+    ts.cell *= 0.5f;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      const float pa = o[a] + ts.t_current * d[a];
+      const float pc = (float)(ts.idx[a] * 2 + 1) * ts.cell;
+      ts.idx[a] = ts.idx[a] * 2 + (pc < pa ? 1 : 0);
+    }
+    next_crossing(ts, o);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) ts.li[a] = ts.idx[a] & (ts.cns - 1);
+  }
+  __device__ __forceinline__ void down_in(const float o[3], const float d[3], TS& ts) {
+    int m[3];
+    next_mask(ts, m);
+    int nl[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) nl[a] = ts.li[a] + m[a] * ts.di[a];
+    if (in_bounds(nl, 2)) push(ts.node_index, ts.hdr, ts.mm0, ts.mm1, ts.mm2, ts.level);
+    fetch_child_index_in(ts);
+    go_down_one_level(o, d, ts);
+    if ((int)ts.level == s.innerLevels) {
+      ts.cns = s.leafSize;
+      int vc = s.leafSize / 2;
+      while (vc > 1) { go_down_one_level(o, d, ts); vc >>= 1; }
+    }
+    path = (path ^ (uint32_t)ts.node_index) * 16777619u;
+    path = (path ^ ts.level) * 16777619u;
+  }
+
+  __device__ void trace(const float oin[3], const float din[3], float tmin, float tmax, float projf,
+                        uint32_t maxIters, uint32_t drawLevel, float out[4], int step[3]) {
+    st.size = 0;
+    step[0] = step[1] = step[2] = 0;
+    const float eps = 1e-4f;
+    const float scale = 1.0f / (2.0f * s.rootHalfSide);
+    float o[3], d[3], omin[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      omin[a] = s.center[a] - s.rootHalfSide;
+      o[a] = (oin[a] - omin[a]) * scale;
+      const float sg = gsign(din[a]);
+      d[a] = din[a];
+      if (d[a] * sg < eps) d[a] = sg * eps;
+    }
+    tmin = tmin * scale;
+    tmax = tmax * scale;
+    float tMin[3], tMax[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      const float cmn = (s.bmin[a] - omin[a]) * scale;
+      const float cmx = (s.bmax[a] - omin[a]) * scale;
+      const float t1 = (cmn - o[a]) / d[a];
+      const float t2 = (cmx - o[a]) / d[a];
+      tMin[a] = gmin(t1, t2);
+      tMax[a] = gmax(t1, t2);
+    }
+    const float tix = gmax(gmax(tMin[0], 0.0f), gmax(tMin[1], tMin[2]));
+    const float tiy = gmin(tMax[0], gmin(tMax[1], tMax[2]));
+    tmin = gmax(tix, tmin + 1e-10f);
+    tmax = gmin(tiy, tmax);
+    if (!(tix < tiy)) { out[0] = -4.f; out[1] = 0.f; out[2] = 0.f; out[3] = -1.f; return; }
+    const float oscale = 2.0f * s.rootHalfSide;
+    TS ts = {};
+    ts.t_current = tmin;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) { ts.inv[a] = 1.0f / d[a]; ts.di[a] = isign(d[a]); }
+    ts.mm0 = ts.mm1 = ts.mm2 = 0;
+    ts.level = 0;
+    ts.cell = 0.5f;
+    ts.leaf_lo = ts.leaf_hi = 0;
+    {
+      const float voxel_eps = 1.0f / (256.f * 1024.f);
+      const float tt = ts.t_current + voxel_eps;
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        const float pa = o[a] + tt * d[a];
+        ts.idx[a] = (int)(pa / ts.cell);
+      }
+      next_crossing(ts, o);
+#pragma unroll
+      for (int a = 0; a < 3; ++a) ts.li[a] = ts.idx[a] % 2;
+    }
+    ts.cns = 2;
+    ts.node_index = 0;
+    fetch_data(ts);
+    ts.cli = lin_idx(ts, ts.cns);
+    int it = 0;
+    const uint32_t maxLevel = min((uint32_t)s.innerLevels, drawLevel - 1);
+    do {
+      const bool full = fetch_voxel_bit(ts);
+      if (!full) {
+        dda_next(ts, step);
+        if (!in_bounds(ts.li, ts.cns)) {
+          if (st.size == 0) { out[0] = -1.f; out[1] = 0.f; out[2] = (float)it; out[3] = -1.f; return; }
+          up_in(o, ts);
+        }
+      } else {
+        const bool hit = (ts.level >= maxLevel) || ((ts.cell * projf) < ts.t_current);
+        if (hit) {
+          out[0] = ts.t_current * oscale; out[1] = (float)ts.level; out[2] = (float)it; out[3] = (float)ts.node_index;
+          return;
+        }
+        down_in(o, d, ts);
+        fetch_data(ts);
+      }
+      ts.cli = lin_idx(ts, ts.cns);
+      ++it;
+    } while ((ts.t_current < tmax) && ((uint32_t)it < maxIters));
+    if ((uint32_t)it >= maxIters) { out[0] = -3.f; out[1] = 0.f; out[2] = (float)it; out[3] = -1.f; return; }
+    out[0] = -2.f; out[1] = 0.f; out[2] = (float)it; out[3] = -1.f;
+  }
+};
+
+__device__ __forceinline__ void mat4_mul(const float* m, const float v[4], float r[4]) {
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    float s = m[0 * 4 + i] * v[0];
+    s = s + m[1 * 4 + i] * v[1];
+    s = s + m[2 * 4 + i] * v[2];
+    s = s + m[3 * 4 + i] * v[3];
+    r[i] = s;
+  }
+}
+
+__device__ __forceinline__ float dot3(const float a[3], const float b[3]) {
+  float s = a[0] * b[0];
+  s = s + a[1] * b[1];
+  s = s + a[2] * b[2];
+  return s;
+}
+
+__device__ __forceinline__ void normalize3(float v[3]) {
+  const float l = sqrtf(dot3(v, v));
+  v[0] = v[0] / l; v[1] = v[1] / l; v[2] = v[2] / l;
+}
+
+template <int FMT>
+__global__ void __launch_bounds__(kBT) k_trace(Dev s, svd_trace_params p, svd_ray_result* __restrict__ prim,
+                                               svd_ray_result* __restrict__ shad) {
+  extern __shared__ uint32_t stackMem[];
+  const int depth = s.innerLevels + 1;
+  const int lane = threadIdx.y * kTW + threadIdx.x;
+  const uint32_t px = blockIdx.x * kTW + threadIdx.x;
+  const uint32_t py = blockIdx.y * kTH + threadIdx.y;
+  if (px >= p.width || py >= p.height) return;
+  Stack stk{stackMem, stackMem + depth * kBT, stackMem + 2 * depth * kBT, lane, 0};
+  Tracer<FMT> tr(s, stk);
+  // computeCameraRay (octree_dda.frag.glsl:587-603)
+  const float fx = (float)px + 0.5f, fy = (float)py + 0.5f;
+  const float sx = (fx / (float)p.width) * 2.0f - 1.0f;
+  const float sy = (fy / (float)p.height) * 2.0f - 1.0f;
+  const float s0[4] = {sx, sy, 0.f, 1.f}, s1[4] = {sx, sy, 1.f, 1.f};
+  float w0[4], w1[4];
+  mat4_mul(p.proj_inv, s0, w0);
+  mat4_mul(p.proj_inv, s1, w1);
+  float rd[3] = {w1[0] / w1[3] - w0[0] / w0[3], w1[1] / w1[3] - w0[1] / w0[3], w1[2] / w1[3] - w0[2] / w0[3]};
+  normalize3(rd);
+  const float z4[4] = {0.f, 0.f, 0.f, 1.f}, d4[4] = {rd[0], rd[1], rd[2], 1.f};
+  float op[4], ep[4];
+  mat4_mul(p.view_inv, z4, op);
+  mat4_mul(p.view_inv, d4, ep);
+  const float o[3] = {op[0], op[1], op[2]};
+  float d[3] = {ep[0] - op[0], ep[1] - op[1], ep[2] - op[2]};
+  normalize3(d);
+  float res[4];
+  int step[3];
+  tr.trace(o, d, 0.0f, 1e30f, p.projection_factor, p.max_iters, p.draw_level, res, step);
+  if (tr.bad) { res[0] = -6.f; res[1] = 0.f; res[3] = -1.f; }
+  const size_t k = (size_t)py * p.width + px;
+  prim[k] = svd_ray_result{res[0], res[1], res[2], res[3], tr.path, tr.bytes};
+  if (!shad) return;
+  svd_ray_result sr{-5.f, 0.f, 0.f, -1.f, 0u, 0u};
+  if (p.shadows && res[0] >= 0.f) {
+    // octree_dda.frag.glsl:686-688, 753-754, 803-815 (normSamples = 0)
+    const float hit[3] = {o[0] + res[0] * d[0], o[1] + res[0] * d[1], o[2] + res[0] * d[2]};
+    const float cellSize = (2.0f * s.rootHalfSide) / ldexpf(1.0f, (int)res[1]);
+    float hn[3] = {-(float)step[0], -(float)step[1], -(float)step[2]};
+    normalize3(hn);
+    const float pp[3] = {hit[0] + (hn[0] * cellSize) * 0.5f, hit[1] + (hn[1] * cellSize) * 0.5f,
+                         hit[2] + (hn[2] * cellSize) * 0.5f};
+    const float l2p[3] = {pp[0] - p.light_pos[0], pp[1] - p.light_pos[1], pp[2] - p.light_pos[2]};
+    const float len = sqrtf(dot3(l2p, l2p));
+    const float sProj = len / cellSize;
+    const float sd[3] = {-(l2p[0] / len), -(l2p[1] / len), -(l2p[2] / len)};
+    float sres[4];
+    int sstep[3];
+    tr.path = 2166136261u;
+    tr.bytes = 0;
+    tr.bad = false;
+    tr.trace(pp, sd, 1e-3f * 0.5f, len * 0.5f, sProj, p.max_iters, p.draw_level, sres, sstep);
+    if (tr.bad) { sres[0] = -6.f; sres[1] = 0.f; sres[3] = -1.f; }
+    sr = svd_ray_result{sres[0], sres[1], sres[2], sres[3], tr.path, tr.bytes};
+  }
+  shad[k] = sr;
+}
+
+}  // namespace
+
+void trace_frame(svd_scene* sc, const svd_trace_params* p, svd_ray_result* prim, svd_ray_result* shad) {
+  svd_ctx* c = sc->ctx;
+  Dev d{};
+  d.nodes = sc->nodes;
+  d.inner = sc->inner;
+  d.leaves = (const uint2*)sc->leaves;
+  d.levelOffsets = sc->levelOffsets;
+  d.nWords = sc->nWords;
+  d.nInner = sc->nInner;
+  d.nLeaves = sc->nLeaves;
+  if (sc->fmt == SVD_FMT_SSVDAG || sc->fmt == SVD_FMT_ESVDAG) { d.innerLevels = (int)sc->levels - 2; d.leafSize = 4; }
+  else { d.innerLevels = (int)sc->levels - 1; d.leafSize = 2; }
+  for (int a = 0; a < 3; ++a) { d.bmin[a] = sc->bmin[a]; d.bmax[a] = sc->bmax[a]; d.center[a] = sc->center[a]; }
+  d.rootHalfSide = sc->rootHalfSide;
+  if (p->width == 0 || p->height == 0) return;
+  if (p->draw_level == 0) throw Error{SVD_ERR_ARG, "draw_level must be >= 1"};
+  const dim3 grid(div_up(p->width, kTW), div_up(p->height, kTH));
+  const dim3 block(kTW, kTH);
+  const size_t shmem = (size_t)3 * (d.innerLevels + 1) * kBT * sizeof(uint32_t);
+  switch (sc->fmt) {
+    case SVD_FMT_SVDAG: k_trace<SVD_FMT_SVDAG><<<grid, block, shmem, c->stream>>>(d, *p, prim, shad); break;
+    case SVD_FMT_USSVDAG: k_trace<SVD_FMT_USSVDAG><<<grid, block, shmem, c->stream>>>(d, *p, prim, shad); break;
+    default: k_trace<SVD_FMT_SSVDAG><<<grid, block, shmem, c->stream>>>(d, *p, prim, shad); break;
+  }
+  check_launch("k_trace");
+}
+
+}  // namespace svd

@@ -1,0 +1,254 @@
+"""GPU parity: the HIP path (through the C ABI) against the oracle.
+
+Integer / byte work must be bit-exact: SVO / DAG / SDAG level arrays,
+invariant bits, refcount order, and all four encoded file images.  The
+traversal must return, for every ray, the same (t, level, iterations, node,
+descent-path hash, fetched bytes) as the CPU port of the shader; t is FP32
+and is required to be bit-identical (tolerance 0, stricter than the 1e-5
+relative bound), because both sides execute the same FP32 operations with no
+contraction.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from decode import decode_words, decode_ssvdag
+
+import svdag_amd as S
+import svdag_amd.workloads as W
+from test_oracle import clustered_codes, random_codes
+
+pytestmark = pytest.mark.gpu
+
+SCENES = {
+    "terrain7": lambda: (W.terrain_codes(7, seed=1), 7),
+    "random6": lambda: (random_codes(6, 20000, 3), 6),
+    "clustered7": lambda: (clustered_codes(7, 2), 7),
+    "caves6": lambda: (W.cave_codes(6, seed=3), 6),
+    "single": lambda: (W.sorted_codes(np.array([5]), np.array([9]), np.array([2])), 5),
+    "full8": lambda: (W.sorted_codes(*[a.ravel() for a in np.meshgrid(np.arange(8), np.arange(8), np.arange(8))]), 3),
+}
+
+
+def oracle_pipeline(codes, L):
+    o = O.OracleOctree.from_morton(codes, L)
+    svo = o.levels_words()
+    o.to_dag()
+    dag = [o.level(l) for l in range(L)]
+    o.init_child_levels()
+    files = {S.FMT_SVDAG: o.encode(0)}
+    if L >= 3:
+        files[S.FMT_ESVDAG] = o.encode(2)
+    o.to_sdag()
+    sdag = [o.level(l, with_inv=True) for l in range(L)]
+    files[S.FMT_USSVDAG] = o.encode(1)
+    if L >= 3:
+        files[S.FMT_SSVDAG] = o.encode(2)
+    return svo, dag, sdag, files
+
+
+def gpu_pipeline(ctx, codes, L):
+    g = S.Octree.from_morton(ctx, codes, L)
+    svo = [g.level(S.VIEW_SVO, l) for l in range(L)]
+    g.to_dag()
+    dag = [g.level(S.VIEW_DAG, l) for l in range(L)]
+    files = {}
+    g.encode(S.FMT_SVDAG)
+    files[S.FMT_SVDAG] = g.encoded_bytes(S.FMT_SVDAG)
+    if L >= 3:
+        g.encode(S.FMT_ESVDAG)
+        files[S.FMT_ESVDAG] = g.encoded_bytes(S.FMT_ESVDAG)
+    g.to_sdag()
+    sdag = [g.level(S.VIEW_SDAG, l, with_inv=True) for l in range(L)]
+    g.encode(S.FMT_USSVDAG)
+    files[S.FMT_USSVDAG] = g.encoded_bytes(S.FMT_USSVDAG)
+    if L >= 3:
+        g.encode(S.FMT_SSVDAG)
+        files[S.FMT_SSVDAG] = g.encoded_bytes(S.FMT_SSVDAG)
+    return g, svo, dag, sdag, files
+
+
+@pytest.mark.parametrize("name", list(SCENES))
+def test_build_levels_and_files_bit_exact(ctx, name):
+    codes, L = SCENES[name]()
+    print(f"[case] {name} L={L} voxels={codes.size}", flush=True)
+    svo_o, dag_o, sdag_o, files_o = oracle_pipeline(codes, L)
+    g, svo_g, dag_g, sdag_g, files_g = gpu_pipeline(ctx, codes, L)
+    for l in range(L):
+        np.testing.assert_array_equal(svo_g[l], svo_o[l], err_msg=f"SVO level {l}")
+        np.testing.assert_array_equal(dag_g[l], dag_o[l], err_msg=f"DAG level {l}")
+        np.testing.assert_array_equal(sdag_g[l][0], sdag_o[l][0], err_msg=f"SDAG level {l}")
+        if l > 0:
+            np.testing.assert_array_equal(sdag_g[l][1], sdag_o[l][1], err_msg=f"SDAG inv level {l}")
+    for fmt, want in files_o.items():
+        got = files_g[fmt]
+        assert len(got) == len(want), (fmt, len(got), len(want))
+        assert got == want, f"format {fmt} differs at byte {next(i for i in range(len(got)) if got[i] != want[i])}"
+
+
+def test_mesh_svo_reduction_bit_exact(ctx):
+    """Config-1 style input: icosphere mesh voxelized in the reference's
+    triangle-major order (oracle buildSVO), reduced and encoded on the GPU."""
+    V, F = W.spheres_mesh(n_spheres=16, subdiv=2, seed=0)
+    bmin, bmax = W.mesh_bbox(V)
+    L = 7
+    o = O.OracleOctree.from_mesh(V[F], L, bmin.astype(np.float64), bmax.astype(np.float64))
+    svo = o.levels_words()
+    rs = O.lib().orc_root_side(o.h)
+    g = S.Octree.from_svo(ctx, svo, list(bmin) + list(bmax), rs)
+    o.to_dag(); o.init_child_levels()
+    g.to_dag()
+    for l in range(L):
+        np.testing.assert_array_equal(g.level(S.VIEW_DAG, l), o.level(l), err_msg=f"DAG level {l}")
+    for fmt_o, fmt_g in ((0, S.FMT_SVDAG), (2, S.FMT_ESVDAG)):
+        g.encode(fmt_g)
+        assert g.encoded_bytes(fmt_g) == o.encode(fmt_o)
+    o.to_sdag(); g.to_sdag()
+    for fmt_o, fmt_g in ((1, S.FMT_USSVDAG), (2, S.FMT_SSVDAG)):
+        g.encode(fmt_g)
+        assert g.encoded_bytes(fmt_g) == o.encode(fmt_o)
+
+
+def _killer(n):
+    val = [-1] * n
+    state = {"solid": 0, "cand": 0}
+    import functools
+
+    def cmp(x, y):
+        if val[x] < 0 and val[y] < 0:
+            if x == state["cand"]:
+                val[x] = state["solid"]
+            else:
+                val[y] = state["solid"]
+            state["solid"] += 1
+        if val[x] < 0:
+            state["cand"] = x
+        elif val[y] < 0:
+            state["cand"] = y
+        return -1 if (val[x] >= 0 and (val[y] < 0 or val[x] > val[y])) else 1
+    sorted(range(n), key=functools.cmp_to_key(cmp))
+    return np.array([v if v >= 0 else state["solid"] for v in val], np.uint32)
+
+
+@pytest.mark.parametrize("case", ["ones", "small", "mixed", "sorted", "big", "huge", "tiny"])
+def test_refcount_order_equals_std_sort(ctx, case):
+    rng = np.random.default_rng(hash(case) % 1000)
+    n = {"ones": 300000, "small": 5000, "mixed": 777777, "sorted": 200000, "big": 2_500_000, "huge": 3000,
+         "tiny": 17}[case]
+    if case == "ones":
+        vals = np.ones(n, np.uint32)
+    elif case == "small":
+        vals = rng.integers(0, 4, n).astype(np.uint32)
+    elif case == "sorted":
+        vals = np.sort(rng.integers(0, 9, n)).astype(np.uint32)
+    elif case == "huge":
+        vals = rng.integers(0, 2 ** 31, n).astype(np.uint32)
+    else:
+        vals = np.minimum(rng.geometric(0.55, n), 60).astype(np.uint32)
+    idx = np.arange(n, dtype=np.uint32)
+    wi, wv = O.std_sort_refcount(idx, vals)
+    gi, gv = ctx.refcount_order(idx, vals)
+    np.testing.assert_array_equal(gv, wv)
+    np.testing.assert_array_equal(gi, wi)
+
+
+def test_refcount_order_heap_fallback(ctx):
+    """McIlroy adversary input drives libstdc++ into its heapsort fallback."""
+    for n in (200, 3000, 9000):
+        vals = _killer(n)
+        idx = np.arange(n, dtype=np.uint32)
+        wi, _ = O.std_sort_refcount(idx, vals)
+        gi, _ = ctx.refcount_order(idx, vals)
+        np.testing.assert_array_equal(gi, wi)
+
+
+@pytest.mark.parametrize("fmt", [S.FMT_SVDAG, S.FMT_USSVDAG, S.FMT_SSVDAG])
+def test_trace_matches_shader_port(ctx, fmt):
+    codes, L = W.terrain_codes(8, seed=1), 8
+    _, _, _, files = oracle_pipeline(codes, L)
+    buf = files[fmt]
+    p = W.trace_params(256, 192, seed=4, levels=L, shadows=True, light=(1.0, 0.6, 1.0))
+    want_p, want_s = O.trace_frame(buf, fmt, p, shadows=True)
+    sc = S.Scene.from_bytes(ctx, fmt, buf)
+    import torch
+    n = p.width * p.height
+    prim = torch.zeros(n * 6, dtype=torch.int32, device="cuda")
+    shad = torch.zeros(n * 6, dtype=torch.int32, device="cuda")
+    torch.cuda.synchronize()
+    sc.trace_device(p, prim.data_ptr(), shad.data_ptr())
+    ctx.sync()
+    got_p = prim.cpu().numpy().view(S.RAY_DTYPE)
+    got_s = shad.cpu().numpy().view(S.RAY_DTYPE)
+    assert (want_p["t"] >= 0).sum() > n // 8, "camera sees too little of the scene"
+    for k in ("t", "level", "iters", "node", "path", "bytes"):
+        np.testing.assert_array_equal(got_p[k], want_p[k], err_msg=f"primary {k}")
+        np.testing.assert_array_equal(got_s[k], want_s[k], err_msg=f"shadow {k}")
+
+
+def test_trace_no_lod_all_formats(ctx):
+    codes, L = clustered_codes(6, 8), 6
+    _, _, _, files = oracle_pipeline(codes, L)
+    p = W.trace_params(128, 96, seed=2, levels=L, shadows=True)
+    p.projection_factor = 1e30
+    import torch
+    n = p.width * p.height
+    for fmt in (S.FMT_SVDAG, S.FMT_USSVDAG, S.FMT_SSVDAG):
+        want_p, want_s = O.trace_frame(files[fmt], fmt, p, shadows=True)
+        sc = S.Scene.from_bytes(ctx, fmt, files[fmt])
+        prim = torch.zeros(n * 6, dtype=torch.int32, device="cuda")
+        shad = torch.zeros(n * 6, dtype=torch.int32, device="cuda")
+        torch.cuda.synchronize()
+        sc.trace_device(p, prim.data_ptr(), shad.data_ptr())
+        ctx.sync()
+        np.testing.assert_array_equal(prim.cpu().numpy().view(S.RAY_DTYPE), want_p)
+        np.testing.assert_array_equal(shad.cpu().numpy().view(S.RAY_DTYPE), want_s)
+
+
+def test_full_size_terrain_properties(ctx):
+    """2048^3 fBm terrain (11 levels): size-independent properties.
+    Every file decodes back to the input voxel set, and the per-level DAG node
+    counts equal the number of distinct (mask, children) rows found by an
+    independent sort-based dedup (np.unique) of the GPU's own SVO."""
+    L = 11
+    codes = W.terrain_codes(L, seed=1)
+    print(f"[case] terrain L={L} voxels={codes.size}", flush=True)
+    g = S.Octree.from_morton(ctx, codes, L)
+    remap = None
+    counts = []
+    for l in range(L - 1, 0, -1):
+        w = g.level(S.VIEW_SVO, l)
+        if remap is not None:
+            ch = w[:, 1:].copy()
+            m = ch != S.NULL_NODE
+            ch[m] = remap[ch[m]]
+            w = np.ascontiguousarray(np.concatenate([w[:, :1], ch], 1))
+        rows = w.view(np.dtype((np.void, 36))).ravel()
+        u, first, inv = np.unique(rows, return_index=True, return_inverse=True)
+        order = np.argsort(first)
+        rank = np.empty(len(u), np.uint32)
+        rank[order] = np.arange(len(u), dtype=np.uint32)
+        remap = rank[inv.ravel()]
+        counts.append((l, len(u)))
+        print(f"  level {l}: svo {len(w)} distinct {len(u)}", flush=True)
+    g.to_dag()
+    for l, c in counts:
+        assert g.level_size(S.VIEW_DAG, l) == c, (l, g.level_size(S.VIEW_DAG, l), c)
+    for fmt in (S.FMT_SVDAG, S.FMT_ESVDAG):
+        g.encode(fmt)
+    np.testing.assert_array_equal(decode_words(g.encoded_bytes(S.FMT_SVDAG), False), codes)
+    np.testing.assert_array_equal(decode_ssvdag(g.encoded_bytes(S.FMT_ESVDAG)), codes)
+    g.to_sdag()
+    g.encode(S.FMT_SSVDAG)
+    np.testing.assert_array_equal(decode_ssvdag(g.encoded_bytes(S.FMT_SSVDAG)), codes)
+
+
+def test_errors_are_reported(ctx):
+    with pytest.raises(S.SvdError):
+        S.Octree.from_morton(ctx, np.array([5, 3], np.uint64), 4)     # not ascending
+    with pytest.raises(S.SvdError):
+        S.Octree.from_morton(ctx, np.array([1 << 20], np.uint64), 4)  # out of range
+    g = S.Octree.from_morton(ctx, np.array([1, 2, 3], np.uint64), 4)
+    with pytest.raises(S.SvdError):
+        g.encode(S.FMT_SVDAG)                                          # not a DAG yet
+    with pytest.raises(S.SvdError):
+        g.to_sdag()
