@@ -146,6 +146,17 @@ int svd_scene_free(svd_scene* sc);
  * DEVICE arrays of width*height svd_ray_result; shadow may be NULL.          */
 int svd_trace(svd_scene* sc, const svd_trace_params* p, svd_ray_result* primary, svd_ray_result* shadow);
 
+/* ---- Gray-code attribute bit-tree --------------------------------------------
+ * The reference lists the feature (readme.md:8) but has no code for it; the
+ * encoding is the one documented with this library: per-voxel u8 attributes
+ * in voxel Morton (DFS) order, coded Gray (gray=1, b^(b>>1)) or binary, one
+ * pruned binary tree per bit plane emitted in pre-order.  Blob: u32 magic
+ * 'SVAB', u32 n, u32 gray, u32 8, u32 bits[8], then the planes' u32 words.
+ * attrs on the device (attrs_on_device=1) or the host; out is a HOST buffer.
+ * Returns the blob size in *bytes; copies it when cap is large enough.      */
+int svd_attr_bittree_encode(svd_ctx* ctx, const uint8_t* attrs, uint64_t n, int gray, int attrs_on_device,
+                            void* out, uint64_t cap, uint64_t* bytes);
+
 /* ---- reference-count order ------------------------------------------------
  * The permutation std::sort(hist, [](a,b){ return a.second > b.second; })
  * produces in EncodedSSVDAG::encode (encoded_ssvdag.cpp:273-275), computed on

@@ -46,6 +46,7 @@ def lib():
             "orc_new": (vp, []), "orc_free": (None, [vp]),
             "orc_build_morton": (i32, [vp, vp, u64, u32]),
             "orc_build_mesh": (i32, [vp, vp, u64, u32, vp]),
+            "orc_build_dag_mesh": (i32, [vp, vp, u64, u32, u32, vp]),
             "orc_to_dag": (i32, [vp]), "orc_to_sdag": (i32, [vp]),
             "orc_init_child_levels": (None, [vp]), "orc_merge_cross_levels": (i32, [vp]),
             "orc_levels": (u32, [vp]), "orc_state": (i32, [vp]), "orc_level_size": (u64, [vp, u32]),
@@ -57,6 +58,8 @@ def lib():
             "orc_encode": (C.c_int64, [vp, i32, vp, u64]),
             "orc_std_sort_refcount": (None, [vp, vp, u64]),
             "orc_trace_frame": (i32, [vp, u64, i32, C.POINTER(TraceParams), vp, vp, u32, u32]),
+            "orc_attr_bittree_encode": (C.c_int64, [vp, u64, i32, vp, u64]),
+            "orc_attr_bittree_decode": (i32, [vp, u64, vp, u64]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -100,6 +103,16 @@ class OracleOctree:
         t = np.ascontiguousarray(tris, dtype=np.float32)
         bb = np.array(list(bbox_min) + list(bbox_max), dtype=np.float64)
         rc = o.L.orc_build_mesh(o.h, _p(t), t.shape[0], levels, _p(bb))
+        assert rc == 0, rc
+        return o
+
+    @classmethod
+    def from_mesh_subtrees(cls, tris: np.ndarray, levels: int, step_level: int, bbox_min, bbox_max):
+        """GeomOctree::buildDAG (geom_octree.cpp:289-435): subtree builder, ends in the DAG state."""
+        o = cls()
+        t = np.ascontiguousarray(tris, dtype=np.float32)
+        bb = np.array(list(bbox_min) + list(bbox_max), dtype=np.float64)
+        rc = o.L.orc_build_dag_mesh(o.h, _p(t), t.shape[0], levels, step_level, _p(bb))
         assert rc == 0, rc
         return o
 
@@ -192,3 +205,20 @@ def trace_frame(file_bytes: bytes, fmt: int, params, shadows: bool = True, row0:
                                row0, p.height if nrows is None else nrows)
     assert rc == 0, rc
     return prim, shad
+
+
+def attr_bittree_encode(attrs: np.ndarray, gray: bool) -> bytes:
+    """Gray-code / binary attribute bit-tree of a per-voxel u8 sequence (voxel Morton order)."""
+    a = np.ascontiguousarray(attrs, dtype=np.uint8)
+    n = lib().orc_attr_bittree_encode(_p(a), a.size, int(gray), None, 0)
+    buf = np.zeros(n, np.uint8)
+    lib().orc_attr_bittree_encode(_p(a), a.size, int(gray), _p(buf), n)
+    return buf.tobytes()
+
+
+def attr_bittree_decode(blob: bytes, n: int) -> np.ndarray:
+    b = np.frombuffer(blob, np.uint8)
+    out = np.zeros(max(n, 1), np.uint8)
+    rc = lib().orc_attr_bittree_decode(_p(b), b.size, _p(out), out.size)
+    assert rc == 0, rc
+    return out[:n]

@@ -25,6 +25,7 @@
 #include <vector>
 #include <string>
 #include <algorithm>
+#include <array>
 #include <utility>
 
 namespace orc {
@@ -257,6 +258,105 @@ static void buildSVOMesh(Octree& o, const float* tris, size_t ntris, unsigned le
   }
   cleanEmptyNodes(o);
   finishSVO(o);
+}
+
+static int toDAG(Octree& o);
+
+// GeomOctree::buildDAG (geom_octree.cpp:289-435): a top SVO of stepLevel+1
+// levels; every set child of its finest nodes is re-voxelized as its own
+// octree (box = that cell) and reduced with toDAG; the subtrees are joined in
+// (i ascending, j = 7..0) order with pointer offsets; a final toDAG.
+static void buildDAGMesh(Octree& o, const float* tris, size_t ntris, unsigned levels, unsigned stepLevel,
+                         const double bmin[3], const double bmax[3]) {
+  const unsigned stepLevels = stepLevel + 1;
+  std::vector<std::array<double, 3>> leaves;
+  // top SVO with leaf centres (buildSVO(stepLevels, bbox, true, &leavesCenters))
+  {
+    for (int i = 0; i < 3; ++i) { o.bboxMin[i] = (float)bmin[i]; o.bboxMax[i] = (float)bmax[i]; }
+    o.levels = stepLevels;
+    float sides[3];
+    for (int i = 0; i < 3; ++i) sides[i] = ((o.bboxMax[i] - o.bboxMin[i]) * 0.5f) * 2.0f;
+    o.rootSide = std::max(std::max(sides[0], sides[1]), sides[2]);
+    o.data.assign(stepLevels, {});
+    o.data[0].push_back(Node());
+    o.nNodes = 0;
+    double center[3];
+    for (int i = 0; i < 3; ++i) center[i] = (bmin[i] + bmax[i]) * 0.5;
+    struct QI { id_t id; uint8_t level; double c[3]; };
+    std::stack<QI> st;
+    for (size_t t = 0; t < ntris; ++t) {
+      const float* tri = tris + 9 * t;
+      st.push(QI{0, 0, {center[0], center[1], center[2]}});
+      while (!st.empty()) {
+        QI qi = st.top(); st.pop();
+        double k = o.rootSide / double(1u << (qi.level + 2));
+        double cc[8][3];
+        for (int c = 0; c < 8; ++c) {
+          cc[c][0] = qi.c[0] + ((c & 4) ? +k : -k);
+          cc[c][1] = qi.c[1] + ((c & 2) ? +k : -k);
+          cc[c][2] = qi.c[2] + ((c & 1) ? +k : -k);
+        }
+        for (int i = 7; i >= 0; --i) {
+          if (!testTriBox(cc[i], k, tri)) continue;
+          Node& node = o.data[qi.level][qi.id];
+          node.mask |= (uint8_t)(1u << i);
+          if (!node.existsChildPointer(i) && qi.level < stepLevels - 1) {
+            node.children[i] = (id_t)o.data[qi.level + 1].size();
+            o.data[qi.level + 1].emplace_back();
+            o.nNodes++;
+            if (qi.level == stepLevels - 2) leaves.push_back({cc[i][0], cc[i][1], cc[i][2]});
+          }
+          if (qi.level + 1u < stepLevels) {
+            const Node& nn = o.data[qi.level][qi.id];
+            st.push(QI{nn.children[i], (uint8_t)(qi.level + 1), {cc[i][0], cc[i][1], cc[i][2]}});
+          }
+        }
+      }
+    }
+    cleanEmptyNodes(o);
+    finishSVO(o);
+  }
+  const double lhs = o.rootSide / double(1u << stepLevels);   // getHalfSideD(stepLevels - 1)
+  size_t nVox = 0;
+  std::map<unsigned, Octree*> subs;
+  for (size_t i = 0; i < o.data[stepLevels - 1].size(); ++i)
+    for (int j = 7; j >= 0; --j) {
+      if (!o.data[stepLevels - 1][i].existsChild(j)) continue;
+      const double* c = leaves[i].data();
+      double lo[3], hi[3];
+      for (int a = 0; a < 3; ++a) {
+        const int bit = a == 0 ? 4 : (a == 1 ? 2 : 1);
+        const double corner = c[a] + ((j & bit) ? lhs : -lhs);
+        lo[a] = std::min(c[a], corner);
+        hi[a] = std::max(c[a], corner);
+      }
+      Octree* s = new Octree();
+      buildSVOMesh(*s, tris, ntris, levels - stepLevels, lo, hi);
+      nVox += s->nVoxels;
+      toDAG(*s);
+      subs[(unsigned)(i * 8 + j)] = s;
+    }
+  o.data.resize(levels);
+  o.levels = levels;
+  for (size_t i = 0; i < o.data[stepLevels - 1].size(); ++i)
+    for (int j = 7; j >= 0; --j) {
+      if (!o.data[stepLevels - 1][i].existsChild(j)) continue;
+      Octree* s = subs[(unsigned)(i * 8 + j)];
+      o.data[stepLevels - 1][i].children[j] = (id_t)o.data[stepLevels].size();
+      for (unsigned k = stepLevels; k < levels; ++k) {
+        if (k < levels - 1) {
+          const id_t off = (id_t)o.data[k + 1].size();
+          for (Node& m : s->data[k - stepLevels])
+            for (int n = 0; n < 8; ++n)
+              if (m.existsChildPointer(n)) m.children[n] += off;
+        }
+        o.data[k].insert(o.data[k].end(), s->data[k - stepLevels].begin(), s->data[k - stepLevels].end());
+      }
+      delete s;
+    }
+  o.nVoxels = nVox;
+  o.state = S_SVO;
+  toDAG(o);
 }
 
 // SVO from an integer voxel set given as 3-bit interleaved Morton codes
@@ -1008,6 +1108,110 @@ static void camera_ray(const TraceParams& p, uint32_t px, uint32_t py, V3& o, V3
   d = normalize3(v3(ep[0] - op[0], ep[1] - op[1], ep[2] - op[2]));
 }
 
+// --------------------------------------------------------------------------
+// Gray-code attribute bit-tree.  The reference names the feature only
+// (readme.md:8); the definition restated here is the one documented for
+// this repository: attributes in voxel (Morton / DFS) order, code =
+// Gray (b ^ b>>1) or binary, per bit plane a pruned binary tree over the
+// sequence padded to a power of two with its last value, emitted in
+// pre-order: single element -> its bit; uniform node -> 0, bit; mixed -> 1,
+// left, right.  Bits are packed LSB-first in u32 words, each plane starting
+// on a word boundary.  PARITY UNPINNED against the reference (no code).
+// --------------------------------------------------------------------------
+static void bittree_plane(const std::vector<uint8_t>& bits, std::vector<uint8_t>& out) {
+  const size_t V = bits.size();
+  size_t P = 1;
+  while (P < V) P <<= 1;
+  auto bit = [&](size_t i) -> uint8_t { return bits[i < V ? i : V - 1]; };
+  // change-point prefix for O(1) uniformity queries
+  std::vector<uint32_t> C(P + 1, 0);
+  for (size_t i = 1; i < P; ++i) C[i] = C[i - 1] + (bit(i) != bit(i - 1) ? 1u : 0u);
+  struct F { size_t lo, hi; };
+  std::vector<F> st{{0, P}};
+  while (!st.empty()) {
+    F f = st.back(); st.pop_back();
+    if (f.hi - f.lo == 1) { out.push_back(bit(f.lo)); continue; }
+    const bool uni = C[f.hi - 1] == C[f.lo];
+    if (uni) { out.push_back(0); out.push_back(bit(f.lo)); continue; }
+    out.push_back(1);
+    const size_t mid = (f.lo + f.hi) / 2;
+    st.push_back({mid, f.hi});   // right after left (stack)
+    st.push_back({f.lo, mid});
+  }
+}
+
+static std::vector<uint8_t> bittree_encode(const uint8_t* a, size_t V, int gray) {
+  std::vector<uint32_t> planeWords[8];
+  uint32_t planeBits[8] = {0};
+  for (int k = 0; k < 8; ++k) {
+    std::vector<uint8_t> bits(V), out;
+    for (size_t i = 0; i < V; ++i) {
+      const uint8_t c = gray ? (uint8_t)(a[i] ^ (a[i] >> 1)) : a[i];
+      bits[i] = (c >> k) & 1u;
+    }
+    if (V) bittree_plane(bits, out);
+    planeBits[k] = (uint32_t)out.size();
+    planeWords[k].assign((out.size() + 31) / 32, 0u);
+    for (size_t i = 0; i < out.size(); ++i) planeWords[k][i / 32] |= (uint32_t)out[i] << (i % 32);
+  }
+  std::vector<uint8_t> file;
+  const uint32_t magic = 0x42415653u;   // "SVAB"
+  const uint32_t hdr[4] = {magic, (uint32_t)V, (uint32_t)gray, 8u};
+  put_bytes(file, hdr, 16);
+  put_bytes(file, planeBits, 32);
+  for (int k = 0; k < 8; ++k) put_bytes(file, planeWords[k].data(), 4 * planeWords[k].size());
+  return file;
+}
+
+static int bittree_decode(const uint8_t* f, size_t len, std::vector<uint8_t>& a) {
+  if (len < 48) return -4;
+  uint32_t hdr[4], planeBits[8];
+  memcpy(hdr, f, 16);
+  memcpy(planeBits, f + 16, 32);
+  if (hdr[0] != 0x42415653u || hdr[3] != 8) return -4;
+  const size_t V = hdr[1];
+  const int gray = (int)hdr[2];
+  size_t P = 1;
+  while (P < V) P <<= 1;
+  std::vector<uint8_t> code(V, 0);
+  size_t off = 48;
+  for (int k = 0; k < 8; ++k) {
+    const size_t nw = (planeBits[k] + 31) / 32;
+    if (off + 4 * nw > len) return -4;
+    const uint32_t* w = (const uint32_t*)(f + off);
+    size_t pos = 0;
+    auto next = [&]() -> uint32_t { uint32_t b = (w[pos / 32] >> (pos % 32)) & 1u; ++pos; return b; };
+    struct Fr { size_t lo, hi; };
+    std::vector<Fr> st;
+    if (V) st.push_back({0, P});
+    while (!st.empty()) {
+      Fr fr = st.back(); st.pop_back();
+      if (pos >= planeBits[k]) return -4;
+      if (fr.hi - fr.lo == 1) {
+        const uint32_t b = next();
+        if (fr.lo < V) code[fr.lo] |= (uint8_t)(b << k);
+        continue;
+      }
+      if (next() == 0) {
+        const uint32_t b = next();
+        for (size_t i = fr.lo; i < fr.hi && i < V; ++i) code[i] |= (uint8_t)(b << k);
+        continue;
+      }
+      const size_t mid = (fr.lo + fr.hi) / 2;
+      st.push_back({mid, fr.hi});
+      st.push_back({fr.lo, mid});
+    }
+    off += 4 * nw;
+  }
+  a.resize(V);
+  for (size_t i = 0; i < V; ++i) {
+    uint8_t c = code[i];
+    if (gray) { uint8_t b = c; for (int s = 1; s < 8; s <<= 1) b ^= (uint8_t)(b >> s); c = b; }
+    a[i] = c;
+  }
+  return 0;
+}
+
 }  // namespace orc
 
 // ============================================================================
@@ -1034,6 +1238,13 @@ int orc_build_morton(void* h, const uint64_t* codes, uint64_t n, uint32_t levels
 int orc_build_mesh(void* h, const float* tris, uint64_t ntris, uint32_t levels, const double bbox[6]) {
   if (levels < 1 || levels > 21) return -2;
   buildSVOMesh(*(Octree*)h, tris, (size_t)ntris, levels, bbox, bbox + 3);
+  return 0;
+}
+
+int orc_build_dag_mesh(void* h, const float* tris, uint64_t ntris, uint32_t levels, uint32_t stepLevel,
+                       const double bbox[6]) {
+  if (levels < 2 || levels > 21 || stepLevel + 1 >= levels) return -2;
+  buildDAGMesh(*(Octree*)h, tris, (size_t)ntris, levels, stepLevel, bbox, bbox + 3);
   return 0;
 }
 
@@ -1182,6 +1393,26 @@ int orc_trace_frame(const uint8_t* file, uint64_t len, int fmt, const TraceParam
         shadow[k] = OrcTraceOut{sres[0], sres[1], sres[2], sres[3], tr.path, tr.bytes};
       }
     }
+  return 0;
+}
+
+}  // extern "C"
+
+extern "C" {
+
+// Attribute bit-tree blob (two-call: returns the size; copies when cap suffices).
+int64_t orc_attr_bittree_encode(const uint8_t* a, uint64_t n, int gray, uint8_t* out, uint64_t cap) {
+  std::vector<uint8_t> f = bittree_encode(a, (size_t)n, gray);
+  if (out && cap >= f.size()) memcpy(out, f.data(), f.size());
+  return (int64_t)f.size();
+}
+
+int orc_attr_bittree_decode(const uint8_t* f, uint64_t len, uint8_t* a, uint64_t cap) {
+  std::vector<uint8_t> v;
+  int rc = bittree_decode(f, (size_t)len, v);
+  if (rc) return rc;
+  if (cap < v.size()) return -1;
+  memcpy(a, v.data(), v.size());
   return 0;
 }
 

@@ -101,6 +101,7 @@ def lib():
         "svd_trace": (i32, [vp, P(TraceParams), vp, vp]),
         "svd_last_kernel_ms": (i32, [vp, C.c_char_p, P(C.c_double)]),
         "svd_refcount_order": (i32, [vp, vp, vp, u64]),
+        "svd_attr_bittree_encode": (i32, [vp, vp, u64, i32, i32, vp, u64, P(u64)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -115,7 +116,7 @@ EXPORTED = ["svd_ctx_create", "svd_ctx_destroy", "svd_last_error", "svd_ctx_sync
             "svd_reduce_sym", "svd_cross_level_merge", "svd_octree_state", "svd_octree_levels",
             "svd_octree_counts", "svd_level_size", "svd_get_level", "svd_encode", "svd_encoded_copy",
             "svd_encoded_device", "svd_scene_create", "svd_scene_from_octree", "svd_scene_free", "svd_trace",
-            "svd_last_kernel_ms", "svd_refcount_order"]
+            "svd_last_kernel_ms", "svd_refcount_order", "svd_attr_bittree_encode"]
 
 
 class Context:
@@ -152,6 +153,21 @@ class Context:
         self.check(self.L.svd_refcount_order(self.h, i.ctypes.data_as(C.c_void_p), v.ctypes.data_as(C.c_void_p),
                                              i.size))
         return i, v
+
+    def attr_bittree(self, attrs, gray: bool) -> bytes:
+        """Gray-code (or binary) attribute bit-tree of per-voxel u8 attributes in voxel Morton order."""
+        if isinstance(attrs, tuple):                 # (device_ptr, n)
+            ptr, n = attrs
+            src, on_dev = C.c_void_p(ptr), 1
+        else:
+            a = np.ascontiguousarray(attrs, dtype=np.uint8)
+            src, n, on_dev = a.ctypes.data_as(C.c_void_p), a.size, 0
+        size = C.c_uint64()
+        self.check(self.L.svd_attr_bittree_encode(self.h, src, n, int(gray), on_dev, None, 0, C.byref(size)))
+        buf = np.zeros(size.value, np.uint8)
+        self.check(self.L.svd_attr_bittree_encode(self.h, src, n, int(gray), on_dev, buf.ctypes.data_as(C.c_void_p),
+                                                  size.value, C.byref(size)))
+        return buf.tobytes()
 
     def close(self):
         if getattr(self, "h", None):

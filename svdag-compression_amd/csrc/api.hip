@@ -507,6 +507,24 @@ int svd_refcount_order(svd_ctx* c, uint32_t* idx, uint32_t* val, uint64_t n) {
   });
 }
 
+int svd_attr_bittree_encode(svd_ctx* c, const uint8_t* attrs, uint64_t n, int gray, int on_device, void* out,
+                            uint64_t cap, uint64_t* bytes) {
+  if (!c || (n && !attrs) || !bytes) return SVD_ERR_ARG;
+  return guarded(c, [&] {
+    Timer t(c, "attr");
+    const uint8_t* d = attrs;
+    uint8_t* tmp = nullptr;
+    if (!on_device && n) {
+      tmp = dalloc<uint8_t>(c, n);
+      SVD_HIP(hipMemcpyAsync(tmp, attrs, n, hipMemcpyHostToDevice, c->stream));
+      d = tmp;
+    }
+    *bytes = attr_bittree(c, d, n, gray ? 1 : 0, (uint8_t*)out, out ? cap : 0);
+    SVD_HIP(hipStreamSynchronize(c->stream));
+    dfree(c, tmp);
+  });
+}
+
 int svd_last_kernel_ms(svd_ctx* c, const char* name, double* ms) {
   if (!c || !name || !ms) return SVD_ERR_ARG;
   auto it = c->kernel_ms.find(name);

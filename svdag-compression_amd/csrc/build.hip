@@ -133,7 +133,9 @@ __global__ void k_compact(const uint32_t* __restrict__ rec, const uint32_t* __re
   uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const uint32_t s = slot[i];
-  if (s == kNone) { remap[i] = kNone; return; }
+  // an empty (skipped) node keeps correspondence 0, like the reference's
+  // zero-initialised correspondences vector (geom_octree.cpp:489-495)
+  if (s == kNone) { remap[i] = 0; return; }
   const uint32_t r = table[s];
   remap[i] = pos[r];
   if (r == (uint32_t)i) {
@@ -195,7 +197,7 @@ __global__ void k_leaf_remap(const uint32_t* __restrict__ mask, uint64_t n, cons
   uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const uint32_t m = mask[i] & 0xFFu;
-  remap[i] = m ? idOfMask[m] : kNone;
+  remap[i] = m ? idOfMask[m] : 0u;
 }
 
 // ---------------------------------------------------------------- symmetry

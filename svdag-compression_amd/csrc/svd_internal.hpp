@@ -144,6 +144,9 @@ void cross_level_merge(svd_octree* o);
 // (encoded_ssvdag.cpp:273-275).
 void refsort(svd_ctx* c, uint32_t* idx, uint32_t* val, uint64_t n);
 
+// ---- attribute bit-tree (attr.hip) ---------------------------------------------
+uint64_t attr_bittree(svd_ctx* c, const uint8_t* dattrs, uint64_t V, int gray, uint8_t* dst, uint64_t cap);
+
 // ---- encoders (encode.hip) -----------------------------------------------------
 void encode(svd_octree* o, int fmt);
 

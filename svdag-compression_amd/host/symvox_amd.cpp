@@ -1,0 +1,695 @@
+// symvox_amd.cpp -- implementation of the reference-compatible C++ API on top
+// of the C ABI of libsvdag_amd.so (GPU reductions, encoders, traversal).
+#include "symvox_amd.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <fstream>
+#include <map>
+#include <set>
+#include <stack>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace svd {
+
+template <typename T>
+void aabox<T>::to_empty() {
+  for (int i = 0; i < 3; ++i) { b[0][i] = (T)1e30; b[1][i] = (T)-1e30; }
+}
+template <typename T>
+void aabox<T>::merge(const vec3<T>& p) {
+  for (int i = 0; i < 3; ++i) { b[0][i] = std::min(b[0][i], p[i]); b[1][i] = std::max(b[1][i], p[i]); }
+}
+template <typename T>
+vec3<T> aabox<T>::center() const {
+  return vec3<T>((b[0][0] + b[1][0]) * (T)0.5, (b[0][1] + b[1][1]) * (T)0.5, (b[0][2] + b[1][2]) * (T)0.5);
+}
+template <typename T>
+vec3<T> aabox<T>::half_side_lengths() const {
+  return vec3<T>((b[1][0] - b[0][0]) * (T)0.5, (b[1][1] - b[0][1]) * (T)0.5, (b[1][2] - b[0][2]) * (T)0.5);
+}
+template struct aabox<float>;
+template struct aabox<double>;
+
+svd_ctx* default_context() {
+  static svd_ctx* ctx = nullptr;
+  if (!ctx) {
+    int rc = svd_ctx_create(0, &ctx);
+    if (rc != SVD_OK) {
+      fprintf(stderr, "ERROR: cannot create the GPU context (status %d)\n", rc);
+      exit(1);
+    }
+  }
+  return ctx;
+}
+
+static double now_s() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+static void check(int rc, const char* what) {
+  if (rc != SVD_OK) {
+    fprintf(stderr, "ERROR: %s failed (%d): %s\n", what, rc, svd_last_error(default_context()));
+    exit(1);
+  }
+}
+
+}  // namespace svd
+
+using namespace svd;
+
+// ============================================================================ Scene
+Scene::Scene() { _bbox.to_empty(); }
+
+void Scene::loadObj(std::string fileName, bool, bool, bool, bool, bool doBuildTriVector) {
+  printf("* Loading '%s'...\n", fileName.c_str());
+  FILE* file = fopen(fileName.c_str(), "r");
+  if (!file) {
+    printf("ERROR: Scene:loadOBJ: Can't open file %s\n", fileName.c_str());
+    exit(1);
+  }
+  _bbox.to_empty();
+  _vertices.clear();
+  _indexed.clear();
+  const int len = 10000;
+  std::vector<char> buf(len);
+  char* str = buf.data();
+  size_t nTris = 0;
+  std::vector<int64_t> tri(3);
+  while (fgets(str, len, file) != NULL) {
+    if (str[0] == 'v') {
+      if (str[1] == 'n' || str[1] == 't') continue;
+      point3f p;
+      if (sscanf(str + 1, "%f %f %f", &p[0], &p[1], &p[2]) != 3) continue;
+      _bbox.merge(p);
+      _vertices.push_back(p);
+    } else if (str[0] == 'f') {
+      // scene.cpp:138-216: tokens split on ' ', the last one loses its final char
+      std::vector<std::string> triples;
+      char* save = nullptr;
+      for (char* pch = strtok_r(str + 1, " ", &save); pch; pch = strtok_r(nullptr, " ", &save)) triples.push_back(pch);
+      if (triples.empty()) continue;
+      if (!triples.back().empty()) triples.back().resize(triples.back().size() - 1);
+      auto parse = [&](const std::string& s, int64_t& out) {
+        int idxV = 0, idxN = 0;
+        int n = sscanf(s.c_str(), "%d//%d", &idxV, &idxN);
+        if (idxV < 0) idxV = int(_vertices.size()) + idxV + 1;
+        if (n >= 1) out = (int64_t)idxV - 1;
+      };
+      for (int i = 0; i < 3 && i < (int)triples.size(); ++i) parse(triples[i], tri[i]);
+      for (int i = 0; i < 3; ++i) _indexed.push_back((uint32_t)tri[i]);
+      nTris++;
+      if (triples.size() > 4) {
+        for (int i = 0; i < 3; ++i) parse(triples[(i + 2) % 4], tri[i]);
+        for (int i = 0; i < 3; ++i) _indexed.push_back((uint32_t)tri[i]);
+        nTris++;
+      }
+    }
+  }
+  fclose(file);
+  printf("\t- Loaded %zu triangles\n", nTris);
+  printf("\t- Bbox: [%.3f %.3f %.3f]  [%.3f %.3f %.3f]\n", _bbox[0][0], _bbox[0][1], _bbox[0][2], _bbox[1][0],
+         _bbox[1][1], _bbox[1][2]);
+  if (doBuildTriVector) buildTriVector(true);
+}
+
+bool Scene::loadPly(std::string fileName) {
+  printf("* Loading '%s'...\n", fileName.c_str());
+  std::ifstream in(fileName);
+  if (!in.is_open()) { printf("ERROR: can't open %s\n", fileName.c_str()); return false; }
+  std::string line;
+  size_t nv = 0, nf = 0;
+  while (std::getline(in, line)) {
+    if (line.rfind("element vertex", 0) == 0) nv = std::stoull(line.substr(15));
+    else if (line.rfind("element face", 0) == 0) nf = std::stoull(line.substr(13));
+    else if (line == "end_header") break;
+  }
+  _bbox.to_empty();
+  _vertices.resize(nv);
+  for (size_t i = 0; i < nv; ++i) {
+    in >> _vertices[i][0] >> _vertices[i][1] >> _vertices[i][2];
+    _bbox.merge(_vertices[i]);
+  }
+  _indexed.clear();
+  for (size_t f = 0; f < nf; ++f) {
+    size_t k;
+    in >> k;
+    std::vector<uint32_t> idx(k);
+    for (size_t j = 0; j < k; ++j) in >> idx[j];
+    for (size_t j = 1; j + 1 < k; ++j) { _indexed.push_back(idx[0]); _indexed.push_back(idx[j]); _indexed.push_back(idx[j + 1]); }
+  }
+  buildTriVector(true);
+  printf("\t- Loaded %zu triangles\n", getNRawTriangles());
+  return true;
+}
+
+// scene.cpp:394-416: only triangles whose three indices are valid
+void Scene::buildTriVector(bool clearOtherData) {
+  _triangles.clear();
+  for (size_t t = 0; t + 2 < _indexed.size(); t += 3) {
+    const uint32_t a = _indexed[t], b = _indexed[t + 1], c = _indexed[t + 2];
+    if (a < _vertices.size() && b < _vertices.size() && c < _vertices.size()) {
+      _triangles.push_back(_vertices[a]);
+      _triangles.push_back(_vertices[b]);
+      _triangles.push_back(_vertices[c]);
+    }
+  }
+  if (clearOtherData) { _vertices.clear(); _vertices.shrink_to_fit(); _indexed.clear(); _indexed.shrink_to_fit(); }
+}
+
+// ============================================================================ voxelizer
+namespace {
+
+// Akenine-Moller triangle / box overlap in double (test_triangle_box.cpp:38-184)
+bool planeBoxOverlap(const double n[3], const double v[3], double h) {
+  double vmin[3], vmax[3];
+  for (int q = 0; q < 3; ++q) {
+    if (n[q] > 0.0f) { vmin[q] = -h - v[q]; vmax[q] = h - v[q]; }
+    else { vmin[q] = h - v[q]; vmax[q] = -h - v[q]; }
+  }
+  if (n[0] * vmin[0] + n[1] * vmin[1] + n[2] * vmin[2] > 0.0) return false;
+  if (n[0] * vmax[0] + n[1] * vmax[1] + n[2] * vmax[2] >= 0.0) return true;
+  return false;
+}
+
+bool triBox(const double c[3], double h, const point3f* tri) {
+  double v0[3], v1[3], v2[3], e0[3], e1[3], e2[3];
+  for (int i = 0; i < 3; ++i) {
+    v0[i] = (double)tri[0][i] - c[i];
+    v1[i] = (double)tri[1][i] - c[i];
+    v2[i] = (double)tri[2][i] - c[i];
+  }
+  for (int i = 0; i < 3; ++i) { e0[i] = v1[i] - v0[i]; e1[i] = v2[i] - v1[i]; e2[i] = v0[i] - v2[i]; }
+  auto axis = [&](double a, double b, double fa, double fb, const double* p, const double* q, int i0, int i1,
+                  double s0, double s1) {
+    // p_k = s0*a*p[i0] + s1*b*p[i1] restated per SAT axis family
+    const double pa = s0 * a * p[i0] + s1 * b * p[i1];
+    const double pb = s0 * a * q[i0] + s1 * b * q[i1];
+    double mn, mx;
+    if (pa < pb) { mn = pa; mx = pb; } else { mn = pb; mx = pa; }
+    const double rad = (fa + fb) * h;
+    return !(mn > rad || mx < -rad);
+  };
+  // X-tests: p = a*v[Y] - b*v[Z]
+  // Y-tests: p = -a*v[X] + b*v[Z]
+  // Z-tests: p = a*v[X] - b*v[Y]
+  double fex = fabs(e0[0]), fey = fabs(e0[1]), fez = fabs(e0[2]);
+  if (!axis(e0[2], e0[1], fez, fey, v0, v2, 1, 2, 1, -1)) return false;   // X01
+  if (!axis(e0[2], e0[0], fez, fex, v0, v2, 0, 2, -1, 1)) return false;   // Y02
+  {                                                                        // Z12 (v1, v2; p2<p1 ordering)
+    const double p1 = e0[1] * v1[0] - e0[0] * v1[1], p2 = e0[1] * v2[0] - e0[0] * v2[1];
+    double mn, mx;
+    if (p2 < p1) { mn = p2; mx = p1; } else { mn = p1; mx = p2; }
+    const double rad = (fey + fex) * h;
+    if (mn > rad || mx < -rad) return false;
+  }
+  fex = fabs(e1[0]); fey = fabs(e1[1]); fez = fabs(e1[2]);
+  if (!axis(e1[2], e1[1], fez, fey, v0, v2, 1, 2, 1, -1)) return false;   // X01
+  if (!axis(e1[2], e1[0], fez, fex, v0, v2, 0, 2, -1, 1)) return false;   // Y02
+  if (!axis(e1[1], e1[0], fey, fex, v0, v1, 0, 1, 1, -1)) return false;   // Z0
+  fex = fabs(e2[0]); fey = fabs(e2[1]); fez = fabs(e2[2]);
+  if (!axis(e2[2], e2[1], fez, fey, v0, v1, 1, 2, 1, -1)) return false;   // X2
+  if (!axis(e2[2], e2[0], fez, fex, v0, v1, 0, 2, -1, 1)) return false;   // Y1
+  {                                                                        // Z12
+    const double p1 = e2[1] * v1[0] - e2[0] * v1[1], p2 = e2[1] * v2[0] - e2[0] * v2[1];
+    double mn, mx;
+    if (p2 < p1) { mn = p2; mx = p1; } else { mn = p1; mx = p2; }
+    const double rad = (fey + fex) * h;
+    if (mn > rad || mx < -rad) return false;
+  }
+  for (int a = 0; a < 3; ++a) {
+    double mn = v0[a], mx = v0[a];
+    if (v1[a] < mn) mn = v1[a];
+    if (v1[a] > mx) mx = v1[a];
+    if (v2[a] < mn) mn = v2[a];
+    if (v2[a] > mx) mx = v2[a];
+    if (mn > h || mx < -h) return false;
+  }
+  const double nrm[3] = {e0[1] * e1[2] - e0[2] * e1[1], e0[2] * e1[0] - e0[0] * e1[2], e0[0] * e1[1] - e0[1] * e1[0]};
+  return planeBoxOverlap(nrm, v0, h);
+}
+
+struct VoxOut {
+  GeomOctree::NodeData data;
+  aabox3f bbox;
+  double rootSide = 0;
+  size_t nNodes = 0, nVoxels = 0;
+};
+
+// GeomOctree::buildSVO (geom_octree.cpp:171-280) + cleanEmptyNodes (:437-456)
+void voxelize_scene(const Scene& scene, unsigned levels, const aabox3d& bb, VoxOut& out,
+                    std::vector<point3d>* leavesCenters) {
+  out.bbox = aabox3f(point3f((float)bb[0][0], (float)bb[0][1], (float)bb[0][2]),
+                     point3f((float)bb[1][0], (float)bb[1][1], (float)bb[1][2]));
+  const point3f sides = out.bbox.half_side_lengths();
+  out.rootSide = std::max(std::max(sides[0] * 2.0f, sides[1] * 2.0f), sides[2] * 2.0f);
+  auto& D = out.data;
+  D.assign(levels, {});
+  D[0].push_back(GeomOctree::Node());
+  out.nNodes = 0;
+  struct QI { uint32_t id; uint8_t level; double c[3]; };
+  std::stack<QI> st;
+  const point3d ctr = bb.center();
+  for (size_t t = 0; t < scene.getNRawTriangles(); ++t) {
+    const point3f* tri = scene.getTrianglePtr(t);
+    st.push(QI{0, 0, {ctr[0], ctr[1], ctr[2]}});
+    while (!st.empty()) {
+      QI qi = st.top();
+      st.pop();
+      const double k = out.rootSide / double(1u << (qi.level + 2));
+      double cc[8][3];
+      for (int c = 0; c < 8; ++c) {
+        cc[c][0] = qi.c[0] + ((c & 4) ? +k : -k);
+        cc[c][1] = qi.c[1] + ((c & 2) ? +k : -k);
+        cc[c][2] = qi.c[2] + ((c & 1) ? +k : -k);
+      }
+      for (int i = 7; i >= 0; --i) {
+        if (!triBox(cc[i], k, tri)) continue;
+        GeomOctree::Node& node = D[qi.level][qi.id];
+        node.childrenBitmask |= (uint8_t)(1u << i);
+        if (!node.existsChildPointer(i) && qi.level < levels - 1) {
+          node.children[i] = (uint32_t)D[qi.level + 1].size();
+          D[qi.level + 1].emplace_back();
+          out.nNodes++;
+          if (leavesCenters && qi.level == levels - 2) leavesCenters->push_back(point3d(cc[i][0], cc[i][1], cc[i][2]));
+        }
+        if (qi.level + 1u < levels) {
+          const uint32_t ch = D[qi.level][qi.id].children[i];
+          st.push(QI{ch, (uint8_t)(qi.level + 1), {cc[i][0], cc[i][1], cc[i][2]}});
+        }
+      }
+    }
+  }
+  for (int lev = (int)levels - 1; lev > 0; --lev) {
+    std::set<uint32_t> empty;
+    for (uint32_t i = 0; i < D[lev].size(); ++i) if (!D[lev][i].hasChildren()) empty.insert(i);
+    if (empty.empty()) continue;
+    for (auto& n : D[lev - 1])
+      for (int j = 0; j < 8; ++j)
+        if (empty.count(n.children[j])) { n.childrenBitmask &= (uint8_t)~(1u << j); n.children[j] = GeomOctree::nullNode; }
+  }
+  out.nVoxels = 0;
+  for (auto& n : D[levels - 1]) out.nVoxels += n.getNChildren();
+}
+
+std::vector<std::vector<uint32_t>> to_words(const GeomOctree::NodeData& D) {
+  std::vector<std::vector<uint32_t>> w(D.size());
+  for (size_t l = 0; l < D.size(); ++l) {
+    w[l].resize(9 * D[l].size());
+    for (size_t i = 0; i < D[l].size(); ++i) {
+      const auto& n = D[l][i];
+      w[l][9 * i] = n.childrenBitmask;
+      for (int c = 0; c < 8; ++c) w[l][9 * i + 1 + c] = n.children[c];
+    }
+  }
+  return w;
+}
+
+}  // namespace
+
+// ============================================================================ GeomOctree
+GeomOctree::GeomOctree(Scene* scene) : _scene(scene) { _bbox.to_empty(); }
+
+GeomOctree::~GeomOctree() {
+  if (_oct) svd_octree_free(_oct);
+}
+
+void GeomOctree::resizeSceneBbox(aabox3f bbox) {
+  _bbox = bbox;
+  const point3f s = _bbox.half_side_lengths();
+  _rootSide = std::max(std::max(s[0] * 2.0f, s[1] * 2.0f), s[2] * 2.0f);
+}
+
+void GeomOctree::uploadSVO(NodeData& data) {
+  auto w = to_words(data);
+  std::vector<uint64_t> sizes(data.size());
+  std::vector<const uint32_t*> ptrs(data.size());
+  for (size_t l = 0; l < data.size(); ++l) { sizes[l] = data[l].size(); ptrs[l] = w[l].data(); }
+  const float bb[6] = {_bbox[0][0], _bbox[0][1], _bbox[0][2], _bbox[1][0], _bbox[1][1], _bbox[1][2]};
+  if (_oct) { svd_octree_free(_oct); _oct = nullptr; }
+  check(svd_octree_from_svo(default_context(), (uint32_t)data.size(), sizes.data(), ptrs.data(), bb, _rootSide, &_oct),
+        "svd_octree_from_svo");
+  _cacheState = -1;
+}
+
+void GeomOctree::buildSVO(unsigned int levels, aabox3d bbox, bool internalCall) {
+  if (!internalCall) printf("* Building SVO... ");
+  fflush(stdout);
+  const double t0 = now_s();
+  VoxOut v;
+  voxelize_scene(*_scene, levels, bbox, v, nullptr);
+  _bbox = v.bbox;
+  _rootSide = v.rootSide;
+  _levels = levels;
+  _stats.buildSVOTime = now_s() - t0;
+  _stats.nNodesSVO = v.nNodes;
+  _stats.nNodesLastLevSVO = v.data[levels - 1].size();
+  _stats.simulatedEncodedSVOSize = (_stats.nNodesSVO - _stats.nNodesLastLevSVO) * 4;
+  _stats.nTotalVoxels = v.nVoxels;
+  uploadSVO(v.data);
+  _state = S_SVO;
+  if (!internalCall) printf("OK! [%.3f s]\n", _stats.buildSVOTime);
+}
+
+void GeomOctree::buildDAG(unsigned int levels, unsigned int stepLevel, aabox3d bbox, bool verbose) {
+  printf("* Building DAG [stepLevel: %i]\n", stepLevel);
+  fflush(stdout);
+  const double t0 = now_s();
+  const unsigned stepLevels = stepLevel + 1;
+  if (stepLevels >= levels) { buildSVO(levels, bbox, true); toDAG(true); return; }
+  // top SVO of stepLevels levels (geom_octree.cpp:302)
+  std::vector<point3d> leavesCenters;
+  VoxOut top;
+  voxelize_scene(*_scene, stepLevels, bbox, top, &leavesCenters);
+  _bbox = top.bbox;
+  _rootSide = top.rootSide;
+  const double lhs = top.rootSide / double(1u << stepLevels);   // getHalfSideD(stepLevels-1)
+  auto& T = top.data;
+  const size_t nTop = T[stepLevels - 1].size();
+  // subtrees, one per set bit of the top SVO's finest nodes (geom_octree.cpp:331-378)
+  std::vector<VoxOut> sub(nTop * 8);
+  std::vector<std::pair<size_t, int>> jobs;
+  for (size_t i = 0; i < nTop; ++i)
+    for (int j = 7; j >= 0; --j)
+      if (T[stepLevels - 1][i].existsChild(j)) jobs.push_back({i, j});
+  printf("\t- Building %zu subtrees (host voxelization) ", jobs.size());
+#ifdef _OPENMP
+  printf("[%d threads]... ", omp_get_max_threads());
+#endif
+  fflush(stdout);
+#pragma omp parallel for schedule(dynamic, 2)
+  for (long k = 0; k < (long)jobs.size(); ++k) {
+    const size_t i = jobs[k].first;
+    const int j = jobs[k].second;
+    const point3d c = leavesCenters[i];
+    const point3d corner(c[0] + ((j & 4) ? lhs : -lhs), c[1] + ((j & 2) ? lhs : -lhs), c[2] + ((j & 1) ? lhs : -lhs));
+    aabox3d lb;
+    lb.merge(c);
+    lb.merge(corner);
+    voxelize_scene(*_scene, levels - stepLevels, lb, sub[i * 8 + j], nullptr);
+  }
+  printf("OK! [%.3f s]\n", now_s() - t0);
+  // join in the reference's order (i ascending, j = 7..0; geom_octree.cpp:397-417)
+  NodeData J(levels);
+  for (unsigned l = 0; l < stepLevels; ++l) J[l] = T[l];
+  size_t nNodes = top.nNodes, nVox = 0, lastLev = 0;
+  for (size_t i = 0; i < nTop; ++i)
+    for (int j = 7; j >= 0; --j) {
+      if (!T[stepLevels - 1][i].existsChild(j)) continue;
+      VoxOut& s = sub[i * 8 + j];
+      J[stepLevels - 1][i].children[j] = (uint32_t)J[stepLevels].size();
+      for (unsigned k = stepLevels; k < levels; ++k) {
+        auto lvl = s.data[k - stepLevels];
+        if (k < levels - 1) {
+          const uint32_t off = (uint32_t)J[k + 1].size();
+          for (auto& n : lvl)
+            for (int c = 0; c < 8; ++c)
+              if (n.existsChildPointer(c)) n.children[c] += off;
+        }
+        J[k].insert(J[k].end(), lvl.begin(), lvl.end());
+      }
+      nNodes += s.nNodes;
+      nVox += s.nVoxels;
+      lastLev += s.data[levels - 1 - stepLevels].size();
+    }
+  _levels = levels;
+  _stats.nNodesSVO = nNodes;
+  _stats.nNodesLastLevSVO = lastLev;
+  _stats.simulatedEncodedSVOSize = (nNodes - lastLev) * 4;
+  _stats.nTotalVoxels = nVox;
+  uploadSVO(J);
+  _state = S_SVO;
+  // one GPU reduction over the joined subtrees gives the same DAG as the
+  // reference's per-subtree toDAG followed by the final toDAG: both keep the
+  // first occurrence of every key in join order.
+  toDAG(true);
+  _stats.buildDAGTime = now_s() - t0;
+  if (verbose) printf("\t- Finished! Total time [%.3f s]\n", _stats.buildDAGTime);
+}
+
+bool GeomOctree::buildFromMorton(const uint64_t* codes, size_t n, unsigned int levels, bool onDevice) {
+  if (_oct) { svd_octree_free(_oct); _oct = nullptr; }
+  int rc = svd_octree_from_morton(default_context(), codes, n, levels, onDevice ? 1 : 0, &_oct);
+  if (rc != SVD_OK) { printf("ERROR: %s\n", svd_last_error(default_context())); return false; }
+  _levels = levels;
+  _bbox = aabox3f(point3f(0, 0, 0), point3f(1, 1, 1));
+  _rootSide = 1.0;
+  uint64_t nn = 0, nv = 0;
+  svd_octree_counts(_oct, &nn, &nv);
+  _stats.nTotalVoxels = nv;
+  _stats.nNodesSVO = nn;
+  _state = S_SVO;
+  _cacheState = -1;
+  return true;
+}
+
+void GeomOctree::toDAG(bool internalCall) {
+  if (_state == S_SVO || _state == S_DAG) {
+    if (!internalCall) printf("* Transforming SVO -> DAG ... ");
+    fflush(stdout);
+  } else {
+    printf("ERROR! This is not a SVO!\n");
+    return;
+  }
+  if (_state == S_DAG) return;   // the device DAG is already reduced
+  const double t0 = now_s();
+  check(svd_reduce_dag(_oct), "toDAG");
+  _stats.toDAGTime = now_s() - t0;
+  uint64_t nn = 0;
+  svd_octree_counts(_oct, &nn, nullptr);
+  _state = S_DAG;
+  _stats.nNodesDAG = nn;
+  svd_level_size(_oct, SVD_VIEW_DAG, _levels - 1, &nn);
+  _stats.nNodesLastLevDAG = nn;
+  _cacheState = -1;
+  if (!internalCall) {
+    printf("\n");
+    for (unsigned lev = _levels - 1; lev > 0; --lev) {
+      uint64_t a = 0, b = 0;
+      svd_level_size(_oct, SVD_VIEW_SVO, lev, &a);
+      svd_level_size(_oct, SVD_VIEW_DAG, lev, &b);
+      printf("Reduced level %u from %lu to %lu nodes\n", lev, (unsigned long)a, (unsigned long)b);
+    }
+    printf("OK! [%.3f s]\n", _stats.toDAGTime);
+  }
+}
+
+void GeomOctree::toSDAG(bool internalCall, bool skipSymmetry) {
+  if (!internalCall) {
+    if (_state == S_SVO) printf("* Transforming SVO -> SDAG (not best way)... ");
+    else if (_state == S_DAG) printf("* Transforming DAG -> SDAG (normal)... ");
+    else { printf("ERROR! This is not a DAG or SDAG!\n"); return; }
+    fflush(stdout);
+  }
+  if (skipSymmetry) { printf("ERROR! pointer-only SDAG (skipSymmetry) is not supported\n"); return; }
+  if (_state == S_SVO) toDAG(true);
+  const double t0 = now_s();
+  check(svd_reduce_sym(_oct), "toSDAG");
+  _stats.toSDAGTime = now_s() - t0;
+  uint64_t nn = 0;
+  svd_octree_counts(_oct, &nn, nullptr);
+  _stats.nNodesSDAG = nn;
+  _state = S_SDAG;
+  _cacheState = -1;
+  if (!internalCall) printf("OK! [%.3f s]\n", _stats.toSDAGTime);
+}
+
+unsigned int GeomOctree::mergeAcrossAllLevels() {
+  if (_state != S_DAG) { printf("ERROR! This is not a DAG!\n"); return 0; }
+  const double t0 = now_s();
+  uint64_t n = 0;
+  int rc = svd_cross_level_merge(_oct, &n);
+  _stats.crossMergeTime = now_s() - t0;
+  if (rc != SVD_OK) printf("\t\t- %s\n", svd_last_error(default_context()));
+  uint64_t nn = 0;
+  svd_octree_counts(_oct, &nn, nullptr);
+  _stats.nNodesDAG = nn;
+  _stats.nCrossLevelMerged = n;
+  _cacheState = -1;
+  printf("Total number of nodes that was removed: %lu\n", (unsigned long)n);
+  return (unsigned)n;
+}
+
+size_t GeomOctree::getNNodes() const {
+  uint64_t nn = 0;
+  if (_oct) svd_octree_counts(_oct, &nn, nullptr);
+  return nn;
+}
+
+const GeomOctree::NodeData& GeomOctree::getNodeData() const {
+  if (_cacheState == (int)_state) return _cache;
+  _cache.assign(_levels, {});
+  const int view = _state == S_SDAG ? SVD_VIEW_SDAG : (_state == S_DAG ? SVD_VIEW_DAG : SVD_VIEW_SVO);
+  for (unsigned l = 0; l < _levels && _oct; ++l) {
+    uint64_t n = 0;
+    svd_level_size(_oct, view, l, &n);
+    std::vector<uint32_t> w(9 * n + 9);
+    std::vector<uint8_t> inv(n + 1);
+    std::vector<uint32_t> cl(8 * n + 8);
+    if (n) svd_get_level(_oct, view, l, w.data(), inv.data(), cl.data());
+    _cache[l].resize(n);
+    for (uint64_t i = 0; i < n; ++i) {
+      Node& nd = _cache[l][i];
+      nd.childrenBitmask = (uint8_t)(w[9 * i] & 0xFF);
+      nd.childrenMirroredBitmask[0] = (uint8_t)(w[9 * i] >> 8);
+      nd.childrenMirroredBitmask[1] = (uint8_t)(w[9 * i] >> 16);
+      nd.childrenMirroredBitmask[2] = (uint8_t)(w[9 * i] >> 24);
+      nd.invariantBitmask = inv[i];
+      for (int c = 0; c < 8; ++c) { nd.children[c] = w[9 * i + 1 + c]; nd.childLevels[c] = cl[8 * i + c]; }
+    }
+  }
+  _cacheState = (int)_state;
+  return _cache;
+}
+
+// ============================================================================ encoders
+bool EncodedOctree::encodeWith(const GeomOctree& octree, int fmt, int needState, const char* what) {
+  if (needState >= 0 && octree.getState() != needState) return false;
+  (void)what;
+  uint64_t bytes = 0;
+  check(svd_encode(octree.handle(), fmt, &bytes), "encode");
+  _file.resize(bytes);
+  check(svd_encoded_copy(octree.handle(), fmt, _file.data(), bytes), "encoded copy");
+  parse();
+  return true;
+}
+
+bool EncodedOctree::save(const std::string filename) const {
+  printf("* Saving '%s'... ", filename.c_str());
+  fflush(stdout);
+  std::ofstream ofs(filename, std::ios::out | std::ios::binary);
+  if (!ofs.is_open()) { printf("FAILED!!!\n"); return false; }
+  ofs.write((const char*)_file.data(), (std::streamsize)_file.size());
+  printf("OK!\n");
+  return true;
+}
+
+bool EncodedOctree::load(const std::string filename) {
+  printf("* Loading '%s'... ", filename.c_str());
+  fflush(stdout);
+  std::ifstream ifs(filename, std::ios::in | std::ios::binary);
+  if (!ifs.is_open()) { printf("FAILED!!!\n"); return false; }
+  _file.assign(std::istreambuf_iterator<char>(ifs), std::istreambuf_iterator<char>());
+  parse();
+  printf("OK!\n");
+  return true;
+}
+
+static void parse_header(const std::vector<uint8_t>& f, aabox3f& bb, float& rs, unsigned& lv, size_t& nn) {
+  if (f.size() < 36) return;
+  memcpy(bb[0].to_pointer(), f.data(), 12);
+  memcpy(bb[1].to_pointer(), f.data() + 12, 12);
+  memcpy(&rs, f.data() + 24, 4);
+  memcpy(&lv, f.data() + 28, 4);
+  uint32_t n32 = 0;
+  memcpy(&n32, f.data() + 32, 4);
+  nn = n32;
+}
+
+void EncodedSVDAG::parse() {
+  parse_header(_file, _sceneBBox, _rootSide, _levels, _nNodes);
+  if (_file.size() >= 44) {
+    memcpy(&_firstLeafPtr, _file.data() + 36, 4);
+    uint32_t cnt = 0;
+    memcpy(&cnt, _file.data() + 40, 4);
+    _nWords = cnt;
+  }
+}
+
+void EncodedSVDAG::encode(const GeomOctree& octree) {
+  printf("* Encoding to SVDAG... ");
+  if (!encodeWith(octree, SVD_FMT_SVDAG, GeomOctree::S_DAG, "SVDAG")) {
+    printf("FAILED! Octree is not in DAG state\n");
+    return;
+  }
+  printf("OK!\n");
+}
+
+void EncodedUSSVDAG::encode(const GeomOctree& octree) {
+  printf("* Encoding to USSVDAG... ");
+  if (!encodeWith(octree, SVD_FMT_USSVDAG, GeomOctree::S_SDAG, "USSVDAG")) {
+    printf("FAILED! Octree is not in SDAG state\n");
+    return;
+  }
+  printf("OK!\n");
+}
+
+void EncodedSSVDAG::parse() {
+  parse_header(_file, _sceneBBox, _rootSide, _levels, _nNodes);
+  if (_file.size() < 40) return;
+  uint32_t nInner = 0, nLeaf = 0, nOff = 0;
+  memcpy(&nInner, _file.data() + 36, 4);
+  const size_t lc = 40 + 2ull * nInner;
+  if (lc + 4 > _file.size()) return;
+  memcpy(&nLeaf, _file.data() + lc, 4);
+  const size_t oc = lc + 4 + nLeaf;
+  if (oc + 4 > _file.size()) return;
+  memcpy(&nOff, _file.data() + oc, 4);
+  _nInner = nInner;
+  _nLeafBytes = nLeaf;
+  _levelOffsets.resize(nOff);
+  if (oc + 4 + 4ull * nOff <= _file.size()) memcpy(_levelOffsets.data(), _file.data() + oc + 4, 4ull * nOff);
+}
+
+void EncodedSSVDAG::encode(const GeomOctree& octree) {
+  printf("* Encoding to SSVDAG... ");
+  if (octree.getState() != GeomOctree::S_SDAG && octree.getState() != GeomOctree::S_DAG) {
+    printf("FAILED! Octree is not in SDAG state\n");
+    return;
+  }
+  _esvdag = octree.getState() == GeomOctree::S_DAG;
+  encodeWith(octree, _esvdag ? SVD_FMT_ESVDAG : SVD_FMT_SSVDAG, -1, "SSVDAG");
+  printf("OK!\n");
+}
+
+// ============================================================================ traversal
+bool traceHeadless(const EncodedOctree& eo, const svd_trace_params& p, HeadlessTraceResult& out,
+                   std::vector<svd_ray_result>* primary) {
+  svd_ctx* c = default_context();
+  const std::vector<uint8_t>& f = eo.fileImage();
+  int fmt = eo.format();
+  if (fmt == SVD_FMT_ESVDAG) fmt = SVD_FMT_SSVDAG;
+  svd_scene* sc = nullptr;
+  if (svd_scene_create(c, fmt, f.data(), f.size(), 0, &sc) != SVD_OK) {
+    printf("ERROR: %s\n", svd_last_error(c));
+    return false;
+  }
+  const size_t n = (size_t)p.width * p.height;
+  svd_ray_result *dp = nullptr, *ds = nullptr;
+  if (hipMalloc((void**)&dp, n * sizeof(svd_ray_result)) != hipSuccess ||
+      hipMalloc((void**)&ds, n * sizeof(svd_ray_result)) != hipSuccess) {
+    printf("ERROR: device allocation failed\n");
+    svd_scene_free(sc);
+    return false;
+  }
+  int rc = svd_trace(sc, &p, dp, ds);
+  double ms = 0;
+  svd_last_kernel_ms(c, "trace", &ms);
+  std::vector<svd_ray_result> hp(n), hs(n);
+  if (rc == SVD_OK) {
+    (void)hipMemcpy(hp.data(), dp, n * sizeof(svd_ray_result), hipMemcpyDeviceToHost);
+    (void)hipMemcpy(hs.data(), ds, n * sizeof(svd_ray_result), hipMemcpyDeviceToHost);
+  }
+  (void)hipFree(dp);
+  (void)hipFree(ds);
+  svd_scene_free(sc);
+  if (rc != SVD_OK) { printf("ERROR: %s\n", svd_last_error(c)); return false; }
+  out.kernelMs = ms;
+  out.primaryRays = n;
+  out.hits = 0;
+  out.shadowRays = 0;
+  for (size_t i = 0; i < n; ++i) {
+    if (hp[i].t >= 0) out.hits++;
+    if (hs[i].t != -5.f) out.shadowRays++;
+  }
+  out.mraysPerSec = (double)(out.primaryRays + out.shadowRays) / (ms * 1e-3) / 1e6;
+  if (primary) *primary = hp;
+  return true;
+}

@@ -196,6 +196,18 @@ def cave_codes(levels: int, seed: int = 3, octaves: int = 5, threshold: float = 
     return np.unique(np.concatenate(out)) if out else np.zeros(0, np.uint64)
 
 
+# ---------------------------------------------------------------- materials (config 5)
+def material_attrs(codes: np.ndarray, levels: int, seed: int = 4, n_values: int = 16,
+                   base_cells: float = 8.0) -> np.ndarray:
+    """Per-voxel u8 material in the codes' order: 3D value noise at the voxel
+    centre (seed 4), quantized to n_values spatially coherent levels."""
+    x, y, z = morton_decode(codes)
+    s = base_cells / (1 << levels)
+    v = value_noise3((x.astype(np.float64) + 0.5) * s, (y.astype(np.float64) + 0.5) * s,
+                     (z.astype(np.float64) + 0.5) * s, seed)
+    return np.minimum(np.floor(v * n_values), n_values - 1).astype(np.uint8)
+
+
 # ---------------------------------------------------------------- icosphere meshes (config 1)
 def icosphere(subdiv: int):
     t = (1.0 + math.sqrt(5.0)) / 2.0

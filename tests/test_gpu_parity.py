@@ -252,3 +252,44 @@ def test_errors_are_reported(ctx):
         g.encode(S.FMT_SVDAG)                                          # not a DAG yet
     with pytest.raises(S.SvdError):
         g.to_sdag()
+
+
+@pytest.mark.parametrize("name", ["terrain7", "clustered7", "caves6", "random6"])
+def test_cross_level_merge_bit_exact(ctx, name):
+    codes, L = SCENES[name]()
+    print(f"[case] csvdag {name} L={L}", flush=True)
+    o = O.OracleOctree.from_morton(codes, L)
+    o.to_dag()
+    o.init_child_levels()
+    assert o.merge_cross_levels() == 0
+    g = S.Octree.from_morton(ctx, codes, L)
+    g.to_dag()
+    assert g.merge_cross_levels() == o.cross_merged
+    for l in range(L):
+        w_o, cl_o = o.level(l, with_child_levels=True)
+        w_g, cl_g = g.level(S.VIEW_DAG, l, with_child_levels=True)
+        np.testing.assert_array_equal(w_g, w_o, err_msg=f"level {l}")
+        if l < L - 1:
+            np.testing.assert_array_equal(cl_g, cl_o, err_msg=f"childLevels {l}")
+    assert g.counts()[0] == o.n_nodes
+    g.encode(S.FMT_SVDAG)
+    assert g.encoded_bytes(S.FMT_SVDAG) == o.encode(0)
+
+
+@pytest.mark.parametrize("n", [0, 1, 3, 1000, 65536, 300001])
+@pytest.mark.parametrize("gray", [False, True])
+def test_attr_bittree_bit_exact(ctx, n, gray):
+    rng = np.random.default_rng(n + gray)
+    a = np.repeat(rng.integers(0, 16, max(n // 7, 1)), 7)[:n].astype(np.uint8)
+    assert ctx.attr_bittree(a, gray) == O.attr_bittree_encode(a, gray)
+
+
+def test_attr_bittree_full_size(ctx):
+    """2048^3 terrain with quantized value-noise materials: GPU blob decodes
+    (with the oracle's decoder) back to the attributes, Gray and binary."""
+    L = 11
+    codes = W.terrain_codes(L, seed=1)
+    a = W.material_attrs(codes, L, seed=4)
+    for gray in (True, False):
+        blob = ctx.attr_bittree(a, gray)
+        np.testing.assert_array_equal(O.attr_bittree_decode(blob, a.size), a)

@@ -404,3 +404,59 @@ def test_library_loads_without_gpu():
     L = svdag_amd.lib()
     for name in svdag_amd.EXPORTED:
         assert hasattr(L, name)
+
+
+# ---------------------------------------------------------------- CSVDAG (oracle)
+@pytest.mark.parametrize("name", ["terrain6", "clustered6", "caves5", "random5"])
+def test_cross_level_merge_is_lossless(name):
+    """Cross-level pointers must not change the voxel set read by the shader
+    (a leaf mask equals the mask of the deeper-level node it points to)."""
+    codes, L = SCENES[name]()
+    o = O.OracleOctree.from_morton(codes, L)
+    o.to_dag()
+    o.init_child_levels()
+    before = [o.level_size(l) for l in range(L)]
+    assert o.merge_cross_levels() == 0
+    after = [o.level_size(l) for l in range(L)]
+    assert o.cross_merged == sum(before) - sum(after)
+    assert o.n_nodes == 1 + sum(after[1:])
+    np.testing.assert_array_equal(decode_words(o.encode(0), False), codes)
+
+
+# ---------------------------------------------------------------- Gray-code attribute bit-tree (oracle)
+def _plane_bits(blob):
+    import struct
+    return struct.unpack_from("<8I", blob, 16)
+
+
+@pytest.mark.parametrize("n", [0, 1, 2, 5, 64, 1000, 4097])
+@pytest.mark.parametrize("gray", [False, True])
+def test_bittree_roundtrip(n, gray):
+    a = np.random.default_rng(n).integers(0, 256, n).astype(np.uint8)
+    assert np.array_equal(O.attr_bittree_decode(O.attr_bittree_encode(a, gray), n), a)
+
+
+def test_bittree_closed_forms():
+    # constant sequence: every plane is one uniform root -> 2 bits; a single voxel -> 1 bit
+    assert _plane_bits(O.attr_bittree_encode(np.full(100, 9, np.uint8), False)) == (2,) * 8
+    assert _plane_bits(O.attr_bittree_encode(np.array([3], np.uint8), True)) == (1,) * 8
+    # alternating plane 0 over P = 128: every internal node mixed -> 2P - 1 bits
+    alt = (np.arange(128) % 2).astype(np.uint8)
+    assert _plane_bits(O.attr_bittree_encode(alt, False))[0] == 255
+    # a step of +-1 in a Gray-coded ramp changes exactly one plane: with one
+    # element per value, planes k = 0..3 flip 8, 4, 2, 1 times (15 in total);
+    # a uniform plane costs 2 bits, so planes 4..7 cost 8 bits together
+    ramp = np.arange(16, dtype=np.uint8)
+    g = _plane_bits(O.attr_bittree_encode(ramp, True))
+    assert g[4:] == (2, 2, 2, 2)
+    gcode = ramp ^ (ramp >> 1)
+    flips = [int(np.sum(((gcode[1:] >> k) & 1) != ((gcode[:-1] >> k) & 1))) for k in range(4)]
+    assert flips == [8, 4, 2, 1]
+
+
+def test_bittree_gray_beats_binary_on_coherent_materials():
+    codes = W.terrain_codes(6, seed=1)
+    a = W.material_attrs(codes, 6, seed=4)
+    g = sum(_plane_bits(O.attr_bittree_encode(a, True)))
+    b = sum(_plane_bits(O.attr_bittree_encode(a, False)))
+    assert g < b, (g, b)
