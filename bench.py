@@ -61,7 +61,6 @@ def cpu_baseline(W, levels_full, sample_levels):
     codes = W.terrain_codes(sample_levels, seed=1)
     t0 = time.perf_counter()
     o = O.OracleOctree.from_morton(codes, sample_levels)
-    t1 = time.perf_counter()
     o.to_dag()
     o.init_child_levels()
     o.encode(0)
@@ -70,10 +69,24 @@ def cpu_baseline(W, levels_full, sample_levels):
     o.encode(1)
     o.encode(2)
     t2 = time.perf_counter()
-    return dict(value=codes.size / (t2 - t1) / 1e9, unit="Gvox/s", cores=1, kind="port",
+    return dict(value=codes.size / (t2 - t0) / 1e9, unit="Gvox/s", cores=1, kind="port",
                 sample=f"terrain {1 << sample_levels}^3 ({sample_levels} levels, {codes.size} voxels, seed 1): "
-                       f"toDAG + toSDAG + 4 encoders, std::map / std::sort, single thread, "
-                       f"{t2 - t1:.2f} s (SVO build {t1 - t0:.2f} s excluded)")
+                       f"SVO from the Morton list + toDAG + toSDAG + 4 encoders (the GPU step's build region), "
+                       f"std::map / std::sort, single thread, {t2 - t0:.2f} s")
+
+
+def measured_traffic(L, args):
+    """HBM bytes per trace launch from the committed counter summary (two
+    rocprofv3 --pmc runs of this workload, FETCH_SIZE and WRITE_SIZE, each
+    scaled by the same counter over a plain copy of known size), or None when
+    no summary matches this workload."""
+    path = os.path.join(ROOT, "profiles", "k_trace_ssvdag_terrain2048_pmc.md")
+    if L != 11 or (args.width, args.height) != (1920, 1080) or not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return int(line.split(":")[1])
+    return None
 
 
 def main():
@@ -85,7 +98,9 @@ def main():
     ap.add_argument("--width", type=int, default=1920)
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-sample-levels", type=int, default=10)
+    ap.add_argument("--cpu-sample-levels", type=int, default=11)
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB three times (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -93,6 +108,16 @@ def main():
         sys.exit(2)
 
     import torch
+    if args.copy_probe:
+        nb = args.copy_probe << 20
+        a = torch.ones(nb // 4, dtype=torch.int32, device="cuda:0")
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": nb, "launches": 3}), flush=True)
+        return
     S = load_pkg()
     import svdag_amd.workloads as W
 
@@ -173,7 +198,7 @@ def main():
         "traversal_mrays_s": round(mrays, 2),
         "trace_kernel_ms": round(trace_kernel_ms, 4),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": measured_traffic(L, args),
                      "kernel": "k_trace<SSVDAG>", "algorithmic_bytes": bytes_fetched},
     }
     if not args.no_cpu_baseline:

@@ -163,11 +163,24 @@ __global__ void k_remap_children(const uint32_t* __restrict__ in, uint64_t n, co
 }
 
 // ---------------------------------------------------------------- leaf level
-__global__ void k_leaf_firstocc(const uint32_t* __restrict__ mask, uint64_t n, uint32_t* __restrict__ firstocc) {
-  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const uint32_t m = mask[i] & 0xFFu;
-  if (m) atomicMin(&firstocc[m], (uint32_t)i);
+// Each block folds a contiguous chunk of masks into an LDS table first, so the
+// global table sees at most 255 atomics per block instead of one per node
+// (all nodes hit the same 255 addresses).
+constexpr int kFoBlocks = 1024;
+__global__ void __launch_bounds__(256) k_leaf_firstocc(const uint32_t* __restrict__ mask, uint64_t n,
+                                                       uint32_t* __restrict__ firstocc) {
+  __shared__ uint32_t fo[256];
+  fo[threadIdx.x] = kNone;
+  __syncthreads();
+  const uint64_t chunk = (n + gridDim.x - 1) / gridDim.x;
+  const uint64_t lo = (uint64_t)blockIdx.x * chunk, hi = lo + chunk < n ? lo + chunk : n;
+  for (uint64_t i = lo + threadIdx.x; i < hi; i += blockDim.x) {
+    const uint32_t m = mask[i] & 0xFFu;
+    if (m && fo[m] > (uint32_t)i) atomicMin(&fo[m], (uint32_t)i);
+  }
+  __syncthreads();
+  const uint32_t v = fo[threadIdx.x];
+  if (v != kNone) atomicMin(&firstocc[threadIdx.x], v);
 }
 
 // Distinct masks ranked by first occurrence -> DAG ids.
@@ -523,7 +536,10 @@ void reduce_dag(svd_octree* o) {
     dl.leaf = true;
     dl.rec = dalloc<uint32_t>(c, 256);
     fill_u32(c, firstocc, kNone, 256);
-    if (sl.n) k_leaf_firstocc<<<div_up(sl.n, kB), kB, 0, c->stream>>>(sl.rec, sl.n, firstocc);
+    if (sl.n) {
+      const uint64_t nb = div_up(sl.n, 256) < (uint64_t)kFoBlocks ? div_up(sl.n, 256) : (uint64_t)kFoBlocks;
+      k_leaf_firstocc<<<nb, 256, 0, c->stream>>>(sl.rec, sl.n, firstocc);
+    }
     k_leaf_rank<<<1, 256, 0, c->stream>>>(firstocc, idOfMask, dl.rec, count);
     if (sl.n) k_leaf_remap<<<div_up(sl.n, kB), kB, 0, c->stream>>>(sl.rec, sl.n, idOfMask, remap);
     check_launch("leaf dedup");
