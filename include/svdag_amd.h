@@ -157,6 +157,97 @@ int svd_trace(svd_scene* sc, const svd_trace_params* p, svd_ray_result* primary,
 int svd_attr_bittree_encode(svd_ctx* ctx, const uint8_t* attrs, uint64_t n, int gray, int attrs_on_device,
                             void* out, uint64_t cap, uint64_t* bytes);
 
+/* ---- lossy DAG (LSVDAG) -----------------------------------------------------
+ * GeomOctree::toLossyDag (geom_octree_extension.cpp:797-1179).  Needs S_DAG;
+ * replaces the DAG view by the lossy DAG (then exactly re-deduplicated, :1176)
+ * and stays in S_DAG, so the encoders and svd_reduce_sym run on it as in
+ * svbuilder (main.cpp:234-257).  The reference clusters its similarity graph
+ * with the external MCL binary (cluster.hpp:134-192); this library's rule is
+ * deterministic instead: level by level from lev_hi down to lev_lo, a node j
+ * whose effective reference count (sortByEffectiveRefCount, :521-560) is at
+ * most max_refs lists the first list_cap earlier nodes i < j whose 512-bit
+ * occupancy three levels down differs from its own in at most `bound` cells
+ * (exhaustive proposer) or the proposed ones of those (learned proposer:
+ * k-nearest neighbours of MLP embeddings), keeps those whose subtree differs
+ * from its own in at most `bound` voxels (exact count), and merges into the
+ * first of them that stays a representative.  bound = max_diff, or when
+ * max_diff is 0, round((L-lev)^2 * diff_factor) (:901).                     */
+typedef struct svd_mlp svd_mlp;
+
+#define SVD_PROPOSER_EXHAUSTIVE 0
+#define SVD_PROPOSER_LEARNED    1
+
+typedef struct svd_lossy_params {
+  uint32_t lev_lo, lev_hi;     /* levels merged, processed lev_hi .. lev_lo (1 <= lo <= hi <= L-2) */
+  uint32_t max_diff;           /* voxel bound per merge; 0 = use diff_factor   */
+  uint32_t max_refs;           /* includedNodeRefCount (main.cpp:112)          */
+  uint32_t list_cap;           /* earlier neighbours kept per node, 1..64      */
+  float diff_factor;           /* allowedLossyDiffFactor (main.cpp:111)        */
+  uint32_t proposer;           /* SVD_PROPOSER_*                               */
+  uint32_t knn_k;              /* learned: neighbours per node (16)            */
+  uint32_t knn_tables, knn_bits, knn_window;  /* learned: LSH search shape (4, 16, 32) */
+  uint32_t pad;
+  uint64_t knn_seed;           /* learned: LSH hyperplane seed                 */
+  svd_mlp* model;              /* learned: trained encoder                     */
+  uint32_t* knn_record;        /* optional HOST buffer: the kNN lists used, level by level */
+  uint64_t knn_record_cap;     /* its capacity in uint32 entries               */
+} svd_lossy_params;
+
+typedef struct svd_lossy_level_stats {
+  uint64_t level, bound, nodes_before, nodes_after, candidates, listed, verified, merged;
+  uint64_t error_bound;        /* sum over merges of voxel difference x effective refs */
+  double ms_embed, ms_knn, ms_lists, ms_verify, ms_merge;
+  double ms_total;             /* device time of the level, signatures to remap */
+} svd_lossy_level_stats;
+
+/* stats: HOST array of stats_cap entries (levels hi..lo); may be NULL. */
+int svd_lossy_merge(svd_octree* oct, const svd_lossy_params* p, svd_lossy_level_stats* stats, uint32_t stats_cap);
+/* The 512-bit occupancy (16 uint32 per node, bit c1*64+c2*8+c3 = child slot
+ * path) of every node of DAG level lev, written to the DEVICE array sig.     */
+int svd_lossy_signatures(svd_octree* oct, uint32_t lev, uint32_t* sig_dev);
+
+/* ---- learned candidate encoder (no reference counterpart: the reference's
+ * lossy path has no learned part) ---------------------------------------------
+ * MLP 512 -> 256 -> 128 -> 32, bf16 weights, fp32 accumulate on MFMA.
+ * Parameters are fp32 masters in one flat array of SVD_MLP_PARAMS floats:
+ * W1[512][256], b1[256], W2[256][128], b2[128], W3[128][32], b3[32].        */
+#define SVD_MLP_PARAMS 168352u
+
+typedef struct svd_mlp_train_params {
+  uint32_t lev_lo, lev_hi;     /* DAG levels whose nodes are sampled           */
+  uint32_t pairs_per_step;     /* P (multiple of 16)                           */
+  uint32_t steps;
+  float lr;                    /* Adam learning rate                           */
+  uint32_t pad;
+  uint64_t seed;               /* pair sampling seed                           */
+} svd_mlp_train_params;
+
+typedef struct svd_mlp_train_stats {
+  float first_loss, last_loss; /* mean loss of the first / last 16 steps       */
+  double ms_train;             /* device time of all steps                     */
+  uint64_t nodes, pairs;
+} svd_mlp_train_stats;
+
+int svd_mlp_create(svd_ctx* ctx, uint64_t seed, svd_mlp** out);
+int svd_mlp_free(svd_mlp* m);
+int svd_mlp_get_params(svd_mlp* m, float* host, uint64_t count);
+int svd_mlp_set_params(svd_mlp* m, const float* host, uint64_t count);
+/* Embeddings (n x 32 fp32, DEVICE) of n signatures (n x 16 uint32, DEVICE). */
+int svd_mlp_embed(svd_mlp* m, const uint32_t* sig_dev, uint64_t n, float* emb_dev);
+/* One forward + backward without update: rows (2p, 2p+1) of sig_dev are pair
+ * p, labels_dev[p] its Hamming distance.  loss and grads (SVD_MLP_PARAMS
+ * floats) are HOST outputs.                                                 */
+int svd_mlp_loss_grad(svd_mlp* m, const uint32_t* sig_dev, const uint32_t* labels_dev, uint64_t npairs,
+                      float* loss, float* grads);
+/* Train on node pairs of oct's DAG levels with exact Hamming labels (Adam);
+ * the step is captured once in a HIP graph and replayed.                     */
+int svd_mlp_train(svd_mlp* m, svd_octree* oct, const svd_mlp_train_params* p, svd_mlp_train_stats* st);
+/* k nearest neighbours (squared L2) of n points of dim 32 (DEVICE), searched
+ * in `tables` LSH orderings of `bits` sign bits within +-window positions;
+ * out_dev: n x k indices, ascending by (distance, index), 0xFFFFFFFF-padded. */
+int svd_knn(svd_ctx* ctx, const float* emb_dev, uint64_t n, uint32_t k, uint32_t tables, uint32_t bits,
+            uint32_t window, uint64_t seed, uint32_t* out_dev);
+
 /* ---- reference-count order ------------------------------------------------
  * The permutation std::sort(hist, [](a,b){ return a.second > b.second; })
  * produces in EncodedSSVDAG::encode (encoded_ssvdag.cpp:273-275), computed on

@@ -31,6 +31,15 @@ class TraceParams(C.Structure):
                 ("light_pos", C.c_float * 3), ("shadows", C.c_uint32), ("pad", C.c_uint32)]
 
 
+class LossyParams(C.Structure):
+    """Leading fields of svd_lossy_params (include/svdag_amd.h)."""
+    _fields_ = [("lev_lo", C.c_uint32), ("lev_hi", C.c_uint32), ("max_diff", C.c_uint32),
+                ("max_refs", C.c_uint32), ("list_cap", C.c_uint32), ("diff_factor", C.c_float)]
+
+
+LOSSY_STATS_DTYPE = np.dtype([(f, "<u8") for f in ("level", "bound", "nodes_before", "nodes_after", "candidates",
+                                                   "listed", "verified", "merged", "error_bound")])
+
 RAY_DTYPE = np.dtype([("t", "<f4"), ("level", "<f4"), ("iters", "<f4"), ("node", "<f4"),
                       ("path", "<u4"), ("bytes", "<u4")])
 
@@ -60,6 +69,11 @@ def lib():
             "orc_trace_frame": (i32, [vp, u64, i32, C.POINTER(TraceParams), vp, vp, u32, u32]),
             "orc_attr_bittree_encode": (C.c_int64, [vp, u64, i32, vp, u64]),
             "orc_attr_bittree_decode": (i32, [vp, u64, vp, u64]),
+            "orc_lossy_merge": (i32, [vp, C.POINTER(LossyParams), vp, u32, vp, u32]),
+            "orc_lossy_signatures": (i32, [vp, u32, vp]),
+            "orc_effective_refcounts": (i32, [vp, u32, vp]),
+            "orc_subtree_diff": (u64, [vp, u32, u32, u32, u64]),
+            "orc_knn_lsh": (None, [vp, u64, u32, u32, u32, u32, u32, u64, vp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -172,6 +186,30 @@ class OracleOctree:
             out.append(cl)
         return out[0] if len(out) == 1 else tuple(out)
 
+    def lossy_merge(self, lev_lo, lev_hi, max_diff=0, diff_factor=1.0, max_refs=1, list_cap=32,
+                    proposals=None, k=16):
+        """Lossy DAG with the exhaustive proposer, or with given per-level kNN
+        lists (concatenated n_l x k arrays, levels hi..lo).  Returns per-level stats."""
+        p = LossyParams(lev_lo, lev_hi, max_diff, max_refs, list_cap, diff_factor)
+        st = np.zeros(64, LOSSY_STATS_DTYPE)
+        prop = None if proposals is None else np.ascontiguousarray(proposals, dtype=np.uint32)
+        rc = self.L.orc_lossy_merge(self.h, C.byref(p), _p(prop) if prop is not None else None, k, _p(st), 64)
+        assert rc == 0, rc
+        return st[: lev_hi - lev_lo + 1]
+
+    def lossy_signatures(self, lev):
+        sig = np.zeros((self.level_size(lev), 16), np.uint32)
+        assert self.L.orc_lossy_signatures(self.h, lev, _p(sig)) == 0
+        return sig
+
+    def effective_refcounts(self, lev):
+        r = np.zeros(self.level_size(lev), np.uint64)
+        assert self.L.orc_effective_refcounts(self.h, lev, _p(r)) == 0
+        return r
+
+    def subtree_diff(self, lev, a, b, cap):
+        return int(self.L.orc_subtree_diff(self.h, lev, a, b, cap))
+
     def levels_words(self):
         return [self.level(l) for l in range(self.levels)]
 
@@ -222,3 +260,12 @@ def attr_bittree_decode(blob: bytes, n: int) -> np.ndarray:
     rc = lib().orc_attr_bittree_decode(_p(b), b.size, _p(out), out.size)
     assert rc == 0, rc
     return out[:n]
+
+
+def knn_lsh(emb: np.ndarray, k: int = 16, tables: int = 4, bits: int = 16, window: int = 32, seed: int = 7):
+    """The learned proposer's LSH neighbour search, restated sequentially."""
+    e = np.ascontiguousarray(emb, dtype=np.float32)
+    n, dim = e.shape
+    out = np.zeros((n, k), np.uint32)
+    lib().orc_knn_lsh(_p(e), n, dim, k, tables, bits, window, seed, _p(out))
+    return out

@@ -1417,3 +1417,299 @@ int orc_attr_bittree_decode(const uint8_t* f, uint64_t len, uint8_t* a, uint64_t
 }
 
 }  // extern "C"
+
+// ============================================================================
+// Lossy DAG (LSVDAG) -- the build's deterministic definition of
+// GeomOctree::toLossyDag (geom_octree_extension.cpp:797-1179).  The reference
+// clusters its similarity graph with the external MCL binary
+// (cluster.hpp:134-192), which is absent, so the clustering rule below is this
+// project's own; the parts kept from the reference are the effective
+// reference count (sortByEffectiveRefCount, :521-560), the bottom-up level
+// order (:882), the per-level bound round((L-lev)^2 * factor) (:901), the
+// includedNodeRefCount filter (:945) and the final toDAG (:1176).
+//
+//   * distance d(a,b) = number of voxels in which the two subtrees differ
+//     (exact; the reference's diffSubtrees :294-368 approximates it);
+//   * signature = 512-bit occupancy of the node's cells three levels down
+//     (voxels for level L-3); Hamming(sig) <= d, so it is an exact pre-filter;
+//   * node j (effective refcount <= max_refs) lists the first list_cap
+//     earlier nodes i < j that pass the pre-filter (exhaustive), or the
+//     proposed ones (learned), keeps those with d <= bound, and merges into
+//     the first of them that stays a representative.
+// ============================================================================
+namespace orc {
+
+struct LossyParams {             // identical layout to svd_lossy_params' leading fields
+  uint32_t lev_lo, lev_hi, max_diff, max_refs, list_cap;
+  float diff_factor;
+};
+
+struct LossyLevelStats {         // per processed level, hi..lo
+  uint64_t level, bound, nodes_before, nodes_after, candidates, listed, verified, merged, error_bound;
+};
+
+static std::vector<std::vector<uint64_t>> effectiveRefCounts(const Octree& o) {
+  std::vector<std::vector<uint64_t>> r(o.levels);
+  r[0].assign(o.data[0].size(), 1);
+  for (unsigned lev = 1; lev < o.levels; ++lev) {
+    r[lev].assign(o.data[lev].size(), 0);
+    for (size_t p = 0; p < o.data[lev - 1].size(); ++p) {
+      const Node& n = o.data[lev - 1][p];
+      for (int c = 0; c < 8; ++c)
+        if (n.existsChild(c) && n.existsChildPointer(c)) r[lev][n.children[c]] += r[lev - 1][p];
+    }
+  }
+  return r;
+}
+
+static void voxelCounts(const Octree& o, unsigned lev, std::vector<std::vector<uint64_t>>& vox) {
+  const auto& v = o.data[lev];
+  vox[lev].assign(v.size(), 0);
+  for (size_t i = 0; i < v.size(); ++i) {
+    if (lev == o.levels - 1) { vox[lev][i] = (uint64_t)__builtin_popcount(v[i].mask); continue; }
+    uint64_t s = 0;
+    for (int c = 0; c < 8; ++c) if (v[i].existsChild(c)) s += vox[lev + 1][v[i].children[c]];
+    vox[lev][i] = s;
+  }
+}
+
+static void lossySignature(const Octree& o, unsigned lev, id_t n, uint32_t sig[16]) {
+  for (int w = 0; w < 16; ++w) sig[w] = 0;
+  const unsigned L = o.levels;
+  auto set = [&](unsigned b) { sig[b >> 5] |= 1u << (b & 31); };
+  const Node& a = o.data[lev][n];
+  for (int c1 = 0; c1 < 8; ++c1) {
+    if (!a.existsChild(c1)) continue;
+    if (lev + 1 >= L) { set(c1); continue; }
+    const Node& b = o.data[lev + 1][a.children[c1]];
+    for (int c2 = 0; c2 < 8; ++c2) {
+      if (!b.existsChild(c2)) continue;
+      if (lev + 2 >= L) { set(c1 * 8 + c2); continue; }
+      const Node& d = o.data[lev + 2][b.children[c2]];
+      for (int c3 = 0; c3 < 8; ++c3) if (d.existsChild(c3)) set(c1 * 64 + c2 * 8 + c3);
+    }
+  }
+}
+
+static unsigned hamming512(const uint32_t* a, const uint32_t* b) {
+  unsigned h = 0;
+  for (int w = 0; w < 16; ++w) h += (unsigned)__builtin_popcount(a[w] ^ b[w]);
+  return h;
+}
+
+// acc + d(a, b), returning as soon as the running total exceeds cap.
+static uint64_t subtreeDiff(const Octree& o, const std::vector<std::vector<uint64_t>>& vox, unsigned lev, id_t a,
+                            id_t b, uint64_t cap, uint64_t acc) {
+  if (a == b) return acc;
+  const Node& A = o.data[lev][a];
+  const Node& B = o.data[lev][b];
+  if (lev == o.levels - 1) return acc + (uint64_t)__builtin_popcount((unsigned)(A.mask ^ B.mask));
+  for (int c = 0; c < 8; ++c) {
+    const bool ea = A.existsChild(c), eb = B.existsChild(c);
+    if (ea && eb) acc = subtreeDiff(o, vox, lev + 1, A.children[c], B.children[c], cap, acc);
+    else if (ea) acc += vox[lev + 1][A.children[c]];
+    else if (eb) acc += vox[lev + 1][B.children[c]];
+    if (acc > cap) return acc;
+  }
+  return acc;
+}
+
+static uint64_t levelBound(const LossyParams& p, unsigned L, unsigned lev) {
+  if (p.max_diff) return p.max_diff;
+  const double d = (double)(L - lev);
+  return (uint64_t)std::llround(d * d * (double)p.diff_factor);
+}
+
+// proposals: per processed level (hi..lo order), n_l x k neighbour indices
+// (kNone-padded) from the learned proposer, or null for the exhaustive one.
+static int lossyMerge(Octree& o, const LossyParams& p, const uint32_t* proposals, uint32_t k,
+                      std::vector<LossyLevelStats>& stats) {
+  if (o.state != S_DAG) return -3;
+  const unsigned L = o.levels;
+  if (L < 3 || p.lev_lo < 1 || p.lev_lo > p.lev_hi || p.lev_hi > L - 2 || p.list_cap < 1 || p.list_cap > 64)
+    return -1;
+  const auto refs = effectiveRefCounts(o);
+  std::vector<std::vector<uint64_t>> vox(L);
+  for (int lev = (int)L - 1; lev > (int)p.lev_hi; --lev) voxelCounts(o, (unsigned)lev, vox);
+  size_t propOff = 0;
+  for (int lev = (int)p.lev_hi; lev >= (int)p.lev_lo; --lev) {
+    voxelCounts(o, (unsigned)lev, vox);
+    const size_t N = o.data[lev].size();
+    const uint64_t bound = levelBound(p, L, (unsigned)lev);
+    LossyLevelStats st = {};
+    st.level = (uint64_t)lev; st.bound = bound; st.nodes_before = N;
+    std::vector<uint32_t> sig(16 * N);
+    for (size_t i = 0; i < N; ++i) lossySignature(o, (unsigned)lev, (id_t)i, &sig[16 * i]);
+    std::vector<char> cand(N);
+    for (size_t j = 0; j < N; ++j) { cand[j] = refs[lev][j] <= p.max_refs; st.candidates += cand[j]; }
+    // earlier-neighbour lists
+    std::vector<std::vector<std::pair<id_t, uint64_t>>> lists(N);
+    std::vector<std::vector<id_t>> prop;
+    if (proposals) {
+      prop.assign(N, {});
+      for (size_t x = 0; x < N; ++x)
+        for (uint32_t t = 0; t < k; ++t) {
+          const uint32_t y = proposals[propOff + x * k + t];
+          if (y == 0xFFFFFFFFu || y == x) continue;
+          const size_t lo = std::min<size_t>(x, y), hi = std::max<size_t>(x, y);
+          prop[hi].push_back((id_t)lo);
+        }
+      propOff += N * k;
+    }
+    for (size_t j = 0; j < N; ++j) {
+      if (!cand[j]) continue;
+      std::vector<id_t> pre;
+      if (proposals) {
+        std::sort(prop[j].begin(), prop[j].end());
+        prop[j].erase(std::unique(prop[j].begin(), prop[j].end()), prop[j].end());
+        for (id_t i : prop[j]) {
+          if (hamming512(&sig[16 * i], &sig[16 * j]) <= bound) pre.push_back(i);
+          if (pre.size() == p.list_cap) break;
+        }
+      } else {
+        for (size_t i = 0; i < j && pre.size() < p.list_cap; ++i)
+          if (hamming512(&sig[16 * i], &sig[16 * j]) <= bound) pre.push_back((id_t)i);
+      }
+      st.listed += pre.size();
+      for (id_t i : pre) {
+        const uint64_t d = subtreeDiff(o, vox, (unsigned)lev, i, (id_t)j, bound, 0);
+        if (d <= bound) lists[j].push_back({i, d});
+      }
+      st.verified += lists[j].size();
+    }
+    // sequential resolution in index order
+    std::vector<char> rep(N, 1);
+    std::vector<id_t> target(N);
+    for (size_t j = 0; j < N; ++j) {
+      target[j] = (id_t)j;
+      for (const auto& e : lists[j])
+        if (rep[e.first]) {
+          rep[j] = 0; target[j] = e.first;
+          st.merged += 1; st.error_bound += e.second * refs[lev][j];
+          break;
+        }
+    }
+    std::vector<id_t> newId(N, 0);
+    std::vector<Node> kept;
+    std::vector<uint64_t> keptVox;
+    for (size_t j = 0; j < N; ++j)
+      if (rep[j]) { newId[j] = (id_t)kept.size(); kept.push_back(o.data[lev][j]); keptVox.push_back(vox[lev][j]); }
+    std::vector<id_t> corr(N);
+    for (size_t j = 0; j < N; ++j) corr[j] = newId[target[j]];
+    o.data[lev] = kept;
+    vox[lev] = keptVox;
+    for (Node& bn : o.data[lev - 1])
+      for (int c = 0; c < 8; ++c)
+        if (bn.existsChild(c)) bn.children[c] = corr[bn.children[c]];
+    st.nodes_after = kept.size();
+    stats.push_back(st);
+  }
+  return toDAG(o);
+}
+
+// ---- LSH kNN over embeddings (the learned proposer's neighbour search) ----
+// Hyperplanes: SplitMix64(seed) stream, component = ((z >> 40) * 2^-24) * 2 - 1.
+// Key of table t = bits b (b = 0..bits-1) set where dot(e, h[t][b]) > 0, with
+// dot = sequential float sum e0*h0 + e1*h1 + ... (no contraction).  Per table
+// the nodes are stably sorted by key; node x at sorted position q takes the
+// positions q-W..q+W (not q) as candidates.  Distance = sequential float sum
+// of squared differences.  Result: the k smallest (distance, index) pairs over
+// all tables, distinct indices, ascending; kNone-padded.
+static void lshHyperplanes(uint64_t seed, uint32_t tables, uint32_t bits, uint32_t dim, std::vector<float>& h) {
+  h.resize((size_t)tables * bits * dim);
+  uint64_t s = seed;
+  for (size_t i = 0; i < h.size(); ++i) {
+    s += 0x9E3779B97F4A7C15ull;
+    uint64_t z = s;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    h[i] = (float)(z >> 40) * (1.0f / 16777216.0f) * 2.0f - 1.0f;
+  }
+}
+
+static void knnLSH(const float* e, size_t n, uint32_t dim, uint32_t k, uint32_t tables, uint32_t bits, uint32_t W,
+                   uint64_t seed, uint32_t* out) {
+  std::vector<float> h;
+  lshHyperplanes(seed, tables, bits, dim, h);
+  std::vector<std::vector<std::pair<float, uint32_t>>> best(n);
+  std::vector<uint32_t> key(n), order(n), pos(n);
+  auto dist = [&](size_t a, size_t b) {
+    float s = 0.f;
+    for (uint32_t d = 0; d < dim; ++d) { const float t = e[a * dim + d] - e[b * dim + d]; const float q = t * t; s = s + q; }
+    return s;
+  };
+  for (uint32_t t = 0; t < tables; ++t) {
+    for (size_t x = 0; x < n; ++x) {
+      uint32_t kk = 0;
+      for (uint32_t b = 0; b < bits; ++b) {
+        const float* hp = &h[((size_t)t * bits + b) * dim];
+        float s = 0.f;
+        for (uint32_t d = 0; d < dim; ++d) { const float q = e[x * dim + d] * hp[d]; s = s + q; }
+        if (s > 0.f) kk |= 1u << b;
+      }
+      key[x] = kk;
+      order[x] = (uint32_t)x;
+    }
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return key[a] < key[b]; });
+    for (size_t q = 0; q < n; ++q) pos[order[q]] = (uint32_t)q;
+    for (size_t x = 0; x < n; ++x) {
+      const int64_t q = pos[x];
+      for (int64_t r = q - (int64_t)W; r <= q + (int64_t)W; ++r) {
+        if (r < 0 || r >= (int64_t)n || r == q) continue;
+        const uint32_t y = order[(size_t)r];
+        best[x].push_back({dist(x, y), y});
+      }
+    }
+  }
+  for (size_t x = 0; x < n; ++x) {
+    auto& b = best[x];
+    std::sort(b.begin(), b.end());
+    b.erase(std::unique(b.begin(), b.end(), [](const std::pair<float, uint32_t>& u, const std::pair<float, uint32_t>& v) {
+              return u.second == v.second; }), b.end());
+    for (uint32_t i = 0; i < k; ++i) out[x * k + i] = i < b.size() ? b[i].second : 0xFFFFFFFFu;
+  }
+}
+
+}  // namespace orc
+
+extern "C" {
+
+int orc_lossy_merge(void* h, const orc::LossyParams* p, const uint32_t* proposals, uint32_t k,
+                    orc::LossyLevelStats* stats, uint32_t stats_cap) {
+  std::vector<orc::LossyLevelStats> st;
+  const int rc = orc::lossyMerge(*(orc::Octree*)h, *p, proposals, k, st);
+  for (size_t i = 0; i < st.size() && i < stats_cap; ++i) stats[i] = st[i];
+  return rc;
+}
+
+int orc_lossy_signatures(void* h, uint32_t lev, uint32_t* sig) {
+  orc::Octree& o = *(orc::Octree*)h;
+  if (lev >= o.levels) return -2;
+  for (size_t i = 0; i < o.data[lev].size(); ++i) orc::lossySignature(o, lev, (orc::id_t)i, sig + 16 * i);
+  return 0;
+}
+
+int orc_effective_refcounts(void* h, uint32_t lev, uint64_t* out) {
+  orc::Octree& o = *(orc::Octree*)h;
+  if (lev >= o.levels) return -2;
+  const auto r = orc::effectiveRefCounts(o);
+  for (size_t i = 0; i < r[lev].size(); ++i) out[i] = r[lev][i];
+  return 0;
+}
+
+// d(a, b) at level lev, capped: min(d, cap + 1).
+uint64_t orc_subtree_diff(void* h, uint32_t lev, uint32_t a, uint32_t b, uint64_t cap) {
+  orc::Octree& o = *(orc::Octree*)h;
+  std::vector<std::vector<uint64_t>> vox(o.levels);
+  for (int l = (int)o.levels - 1; l >= (int)lev; --l) orc::voxelCounts(o, (unsigned)l, vox);
+  const uint64_t d = orc::subtreeDiff(o, vox, lev, a, b, cap, 0);
+  return d > cap ? cap + 1 : d;
+}
+
+void orc_knn_lsh(const float* e, uint64_t n, uint32_t dim, uint32_t k, uint32_t tables, uint32_t bits, uint32_t window,
+                 uint64_t seed, uint32_t* out) {
+  orc::knnLSH(e, (size_t)n, dim, k, tables, bits, window, seed, out);
+}
+
+}  // extern "C"

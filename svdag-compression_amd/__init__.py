@@ -43,6 +43,34 @@ class TraceParams(C.Structure):
                 ("light_pos", C.c_float * 3), ("shadows", C.c_uint32), ("pad", C.c_uint32)]
 
 
+PROPOSER_EXHAUSTIVE, PROPOSER_LEARNED = 0, 1
+MLP_PARAMS = 168352
+
+
+class LossyParams(C.Structure):
+    _fields_ = [("lev_lo", C.c_uint32), ("lev_hi", C.c_uint32), ("max_diff", C.c_uint32), ("max_refs", C.c_uint32),
+                ("list_cap", C.c_uint32), ("diff_factor", C.c_float), ("proposer", C.c_uint32),
+                ("knn_k", C.c_uint32), ("knn_tables", C.c_uint32), ("knn_bits", C.c_uint32),
+                ("knn_window", C.c_uint32), ("pad", C.c_uint32), ("knn_seed", C.c_uint64), ("model", C.c_void_p),
+                ("knn_record", C.c_void_p), ("knn_record_cap", C.c_uint64)]
+
+
+LOSSY_STATS_DTYPE = np.dtype([(f, "<u8") for f in ("level", "bound", "nodes_before", "nodes_after", "candidates",
+                                                   "listed", "verified", "merged", "error_bound")]
+                             + [(f, "<f8") for f in ("ms_embed", "ms_knn", "ms_lists", "ms_verify", "ms_merge",
+                                                     "ms_total")])
+
+
+class TrainParams(C.Structure):
+    _fields_ = [("lev_lo", C.c_uint32), ("lev_hi", C.c_uint32), ("pairs_per_step", C.c_uint32),
+                ("steps", C.c_uint32), ("lr", C.c_float), ("pad", C.c_uint32), ("seed", C.c_uint64)]
+
+
+class TrainStats(C.Structure):
+    _fields_ = [("first_loss", C.c_float), ("last_loss", C.c_float), ("ms_train", C.c_double),
+                ("nodes", C.c_uint64), ("pairs", C.c_uint64)]
+
+
 RAY_DTYPE = np.dtype([("t", "<f4"), ("level", "<f4"), ("iters", "<f4"), ("node", "<f4"),
                       ("path", "<u4"), ("bytes", "<u4")])
 
@@ -102,6 +130,16 @@ def lib():
         "svd_last_kernel_ms": (i32, [vp, C.c_char_p, P(C.c_double)]),
         "svd_refcount_order": (i32, [vp, vp, vp, u64]),
         "svd_attr_bittree_encode": (i32, [vp, vp, u64, i32, i32, vp, u64, P(u64)]),
+        "svd_lossy_merge": (i32, [vp, P(LossyParams), vp, u32]),
+        "svd_lossy_signatures": (i32, [vp, u32, vp]),
+        "svd_mlp_create": (i32, [vp, u64, P(vp)]),
+        "svd_mlp_free": (i32, [vp]),
+        "svd_mlp_get_params": (i32, [vp, vp, u64]),
+        "svd_mlp_set_params": (i32, [vp, vp, u64]),
+        "svd_mlp_embed": (i32, [vp, vp, u64, vp]),
+        "svd_mlp_loss_grad": (i32, [vp, vp, vp, u64, P(C.c_float), vp]),
+        "svd_mlp_train": (i32, [vp, vp, P(TrainParams), P(TrainStats)]),
+        "svd_knn": (i32, [vp, vp, u64, u32, u32, u32, u32, u64, vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -116,7 +154,9 @@ EXPORTED = ["svd_ctx_create", "svd_ctx_destroy", "svd_last_error", "svd_ctx_sync
             "svd_reduce_sym", "svd_cross_level_merge", "svd_octree_state", "svd_octree_levels",
             "svd_octree_counts", "svd_level_size", "svd_get_level", "svd_encode", "svd_encoded_copy",
             "svd_encoded_device", "svd_scene_create", "svd_scene_from_octree", "svd_scene_free", "svd_trace",
-            "svd_last_kernel_ms", "svd_refcount_order", "svd_attr_bittree_encode"]
+            "svd_last_kernel_ms", "svd_refcount_order", "svd_attr_bittree_encode", "svd_lossy_merge",
+            "svd_lossy_signatures", "svd_mlp_create", "svd_mlp_free", "svd_mlp_get_params", "svd_mlp_set_params",
+            "svd_mlp_embed", "svd_mlp_loss_grad", "svd_mlp_train", "svd_knn"]
 
 
 class Context:
@@ -168,6 +208,11 @@ class Context:
         self.check(self.L.svd_attr_bittree_encode(self.h, src, n, int(gray), on_dev, buf.ctypes.data_as(C.c_void_p),
                                                   size.value, C.byref(size)))
         return buf.tobytes()
+
+    def knn(self, emb_ptr: int, n: int, k: int = 16, tables: int = 4, bits: int = 16, window: int = 32,
+            seed: int = 7, out_ptr: int = 0):
+        """k nearest neighbours of n 32-d fp32 points (device pointers)."""
+        self.check(self.L.svd_knn(self.h, C.c_void_p(emb_ptr), n, k, tables, bits, window, seed, C.c_void_p(out_ptr)))
 
     def close(self):
         if getattr(self, "h", None):
@@ -288,6 +333,76 @@ class Octree:
         ptr, n = C.c_void_p(), C.c_uint64()
         self.ctx.check(self.ctx.L.svd_encoded_device(self.h, fmt, C.byref(ptr), C.byref(n)))
         return ptr.value, n.value
+
+    def lossy_merge(self, lev_lo: int, lev_hi: int, max_diff: int = 0, diff_factor: float = 1.0, max_refs: int = 1,
+                    list_cap: int = 32, model: "MLP | None" = None, k: int = 16, tables: int = 4, bits: int = 16,
+                    window: int = 32, knn_seed: int = 7, record_knn: bool = False):
+        """GeomOctree::toLossyDag on the device.  model=None: exhaustive
+        proposer; else the learned one.  Returns per-level stats (and the kNN
+        lists used, concatenated level by level, when record_knn)."""
+        p = LossyParams(lev_lo, lev_hi, max_diff, max_refs, list_cap, diff_factor,
+                        PROPOSER_LEARNED if model is not None else PROPOSER_EXHAUSTIVE, k, tables, bits, window, 0,
+                        knn_seed, model.h if model is not None else None, None, 0)
+        rec = None
+        if record_knn and model is not None:
+            cap = sum(self.level_size(VIEW_DAG, l) for l in range(lev_lo, lev_hi + 1)) * k
+            rec = np.zeros(max(cap, 1), np.uint32)
+            p.knn_record = rec.ctypes.data_as(C.c_void_p).value
+            p.knn_record_cap = cap
+        st = np.zeros(lev_hi - lev_lo + 1, LOSSY_STATS_DTYPE)
+        self.ctx.check(self.ctx.L.svd_lossy_merge(self.h, C.byref(p), st.ctypes.data_as(C.c_void_p), st.size))
+        return (st, rec) if record_knn else st
+
+    def lossy_signatures(self, lev: int, sig_ptr: int):
+        """512-bit occupancies (16 u32 per node) of DAG level lev into a device buffer."""
+        self.ctx.check(self.ctx.L.svd_lossy_signatures(self.h, lev, C.c_void_p(sig_ptr)))
+
+
+class MLP:
+    """The learned candidate encoder (svd_mlp): 512 -> 256 -> 128 -> 32, bf16 MFMA."""
+
+    def __init__(self, ctx: Context, seed: int = 1):
+        self.ctx = ctx
+        h = C.c_void_p()
+        ctx.check(ctx.L.svd_mlp_create(ctx.h, seed, C.byref(h)))
+        self.h = h
+
+    def free(self):
+        if getattr(self, "h", None):
+            self.ctx.L.svd_mlp_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+    def params(self) -> np.ndarray:
+        p = np.zeros(MLP_PARAMS, np.float32)
+        self.ctx.check(self.ctx.L.svd_mlp_get_params(self.h, p.ctypes.data_as(C.c_void_p), p.size))
+        return p
+
+    def set_params(self, p: np.ndarray):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        self.ctx.check(self.ctx.L.svd_mlp_set_params(self.h, p.ctypes.data_as(C.c_void_p), p.size))
+
+    def embed(self, sig_ptr: int, n: int, emb_ptr: int):
+        self.ctx.check(self.ctx.L.svd_mlp_embed(self.h, C.c_void_p(sig_ptr), n, C.c_void_p(emb_ptr)))
+
+    def loss_grad(self, sig_ptr: int, labels_ptr: int, npairs: int):
+        loss = C.c_float()
+        g = np.zeros(MLP_PARAMS, np.float32)
+        self.ctx.check(self.ctx.L.svd_mlp_loss_grad(self.h, C.c_void_p(sig_ptr), C.c_void_p(labels_ptr), npairs,
+                                                    C.byref(loss), g.ctypes.data_as(C.c_void_p)))
+        return loss.value, g
+
+    def train(self, octree: Octree, lev_lo: int, lev_hi: int, pairs_per_step: int = 2048, steps: int = 1024,
+              lr: float = 1e-3, seed: int = 11) -> TrainStats:
+        p = TrainParams(lev_lo, lev_hi, pairs_per_step, steps, lr, 0, seed)
+        st = TrainStats()
+        self.ctx.check(self.ctx.L.svd_mlp_train(self.h, octree.h, C.byref(p), C.byref(st)))
+        return st
 
 
 class Scene:

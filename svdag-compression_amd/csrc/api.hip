@@ -534,3 +534,90 @@ int svd_last_kernel_ms(svd_ctx* c, const char* name, double* ms) {
 }
 
 }  // extern "C"
+
+// ---- lossy DAG ---------------------------------------------------------------
+int svd_lossy_merge(svd_octree* o, const svd_lossy_params* p, svd_lossy_level_stats* stats, uint32_t cap) {
+  if (!o || !p) return SVD_ERR_ARG;
+  return guarded(o->ctx, [&] {
+    std::vector<svd_lossy_level_stats> st;
+    {
+      Timer t(o->ctx, "lossy");
+      lossy_merge(o, p, st);
+    }
+    free_levels(o->ctx, o->xdag);
+    o->hasX = false;
+    free_levels(o->ctx, o->sdag);
+    for (auto* m : o->sdagMap) dfree(o->ctx, m);
+    o->sdagMap.clear();
+    for (auto& e : o->enc) e.valid = false;
+    if (stats)
+      for (size_t i = 0; i < st.size() && i < cap; ++i) stats[i] = st[i];
+  });
+}
+
+int svd_lossy_signatures(svd_octree* o, uint32_t lev, uint32_t* sig) {
+  if (!o || !sig) return SVD_ERR_ARG;
+  return guarded(o->ctx, [&] {
+    lossy_signatures(o, lev, sig);
+    SVD_HIP(hipStreamSynchronize(o->ctx->stream));
+  });
+}
+
+// ---- learned encoder ---------------------------------------------------------
+int svd_mlp_create(svd_ctx* c, uint64_t seed, svd_mlp** out) {
+  if (!c || !out) return SVD_ERR_ARG;
+  svd_mlp* m = new svd_mlp();
+  m->ctx = c;
+  const int rc = guarded(c, [&] {
+    mlp_alloc(m);
+    mlp_init(m, seed);
+  });
+  if (rc != SVD_OK) { (void)guarded(c, [&] { mlp_release(m); }); delete m; return rc; }
+  *out = m;
+  return SVD_OK;
+}
+
+int svd_mlp_free(svd_mlp* m) {
+  if (!m) return SVD_ERR_ARG;
+  const int rc = guarded(m->ctx, [&] { mlp_release(m); });
+  delete m;
+  return rc;
+}
+
+int svd_mlp_get_params(svd_mlp* m, float* host, uint64_t count) {
+  if (!m || !host || count != SVD_MLP_PARAMS) return SVD_ERR_ARG;
+  return guarded(m->ctx, [&] { mlp_get_params(m, host); });
+}
+
+int svd_mlp_set_params(svd_mlp* m, const float* host, uint64_t count) {
+  if (!m || !host || count != SVD_MLP_PARAMS) return SVD_ERR_ARG;
+  return guarded(m->ctx, [&] { mlp_set_params(m, host); });
+}
+
+int svd_mlp_embed(svd_mlp* m, const uint32_t* sig, uint64_t n, float* emb) {
+  if (!m || (n && (!sig || !emb))) return SVD_ERR_ARG;
+  return guarded(m->ctx, [&] {
+    Timer t(m->ctx, "mlp_embed");
+    if (n) mlp_embed(m, sig, n, emb);
+  });
+}
+
+int svd_mlp_loss_grad(svd_mlp* m, const uint32_t* sig, const uint32_t* labels, uint64_t npairs, float* loss,
+                      float* grads) {
+  if (!m || !sig || !labels || !loss || !grads || npairs < 16 || npairs % 16) return SVD_ERR_ARG;
+  return guarded(m->ctx, [&] { mlp_loss_grad(m, sig, labels, npairs, loss, grads); });
+}
+
+int svd_mlp_train(svd_mlp* m, svd_octree* o, const svd_mlp_train_params* p, svd_mlp_train_stats* st) {
+  if (!m || !o || !p || o->ctx != m->ctx) return SVD_ERR_ARG;
+  return guarded(m->ctx, [&] { mlp_train(m, o, p, st); });
+}
+
+int svd_knn(svd_ctx* c, const float* emb, uint64_t n, uint32_t k, uint32_t tables, uint32_t bits, uint32_t window,
+            uint64_t seed, uint32_t* out) {
+  if (!c || (n && (!emb || !out))) return SVD_ERR_ARG;
+  return guarded(c, [&] {
+    Timer t(c, "knn");
+    knn_lsh(c, emb, n, k, tables, bits, window, seed, out);
+  });
+}

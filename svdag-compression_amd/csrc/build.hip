@@ -510,29 +510,25 @@ static uint64_t dedup_level(svd_ctx* c, const uint32_t* rec, uint64_t n, uint32_
   return d;
 }
 
-void reduce_dag(svd_octree* o) {
-  svd_ctx* c = o->ctx;
-  if (!(o->state == SVD_S_SVO)) throw Error{SVD_ERR_STATE, "toDAG: octree is not an SVO"};
-  const uint32_t L = o->levels;
-  for (auto& lv : o->dag) { dfree(c, lv.rec); }
-  o->dag.assign(L, Level());
+// toDAG over any set of levels (the SVO, or a lossy DAG before its final
+// dedup, geom_octree_extension.cpp:1176): dst gets the reduced levels.
+uint64_t dedup_all(svd_ctx* c, uint32_t L, const std::vector<Level>& src, std::vector<Level>& dst) {
+  dst.assign(L, Level());
   if (L == 1) {
-    o->dag[0] = o->svo[0];
-    o->dag[0].rec = dalloc<uint32_t>(c, 1);
-    SVD_HIP(hipMemcpyAsync(o->dag[0].rec, o->svo[0].rec, 4, hipMemcpyDeviceToDevice, c->stream));
-    o->nNodesDAG = 1;
-    o->state = SVD_S_DAG;
-    return;
+    dst[0] = src[0];
+    dst[0].rec = dalloc<uint32_t>(c, 1);
+    SVD_HIP(hipMemcpyAsync(dst[0].rec, src[0].rec, 4, hipMemcpyDeviceToDevice, c->stream));
+    return 1;
   }
   uint64_t nNodes = 1;
   // finest level: dedup of 8-bit masks by first occurrence
-  const Level& sl = o->svo[L - 1];
+  const Level& sl = src[L - 1];
   uint32_t* remap = dalloc<uint32_t>(c, sl.n + 1);
   {
     uint32_t* firstocc = dalloc<uint32_t>(c, 256);
     uint32_t* idOfMask = dalloc<uint32_t>(c, 256);
     uint32_t* count = dalloc<uint32_t>(c, 1);
-    Level& dl = o->dag[L - 1];
+    Level& dl = dst[L - 1];
     dl.leaf = true;
     dl.rec = dalloc<uint32_t>(c, 256);
     fill_u32(c, firstocc, kNone, 256);
@@ -548,11 +544,11 @@ void reduce_dag(svd_octree* o) {
     dfree(c, firstocc); dfree(c, idOfMask); dfree(c, count);
   }
   for (int lev = (int)L - 2; lev >= 0; --lev) {
-    const Level& s = o->svo[lev];
+    const Level& s = src[lev];
     uint32_t* keyed = dalloc<uint32_t>(c, (s.n ? s.n : 1) * kRecWords);
     if (s.n) k_remap_children<<<div_up(s.n, kB), kB, 0, c->stream>>>(s.rec, s.n, remap, keyed);
     check_launch("k_remap_children");
-    Level& d = o->dag[lev];
+    Level& d = dst[lev];
     if (lev == 0) {
       // the root level is never reduced (geom_octree.cpp:483)
       d.rec = keyed;
@@ -567,7 +563,15 @@ void reduce_dag(svd_octree* o) {
     }
   }
   dfree(c, remap);
-  o->nNodesDAG = nNodes;
+  return nNodes;
+}
+
+void reduce_dag(svd_octree* o) {
+  svd_ctx* c = o->ctx;
+  if (!(o->state == SVD_S_SVO)) throw Error{SVD_ERR_STATE, "toDAG: octree is not an SVO"};
+  for (auto& lv : o->dag) { dfree(c, lv.rec); }
+  o->dag.clear();
+  o->nNodesDAG = dedup_all(c, o->levels, o->svo, o->dag);
   o->state = SVD_S_DAG;
 }
 

@@ -1,0 +1,143 @@
+// k-nearest-neighbour search over 32-d embeddings for the learned lossy-DAG
+// proposer: `tables` random-hyperplane LSH orderings; per ordering each point
+// compares itself with the points within +-window sorted positions.  Exact,
+// deterministic arithmetic (sequential fp32 sums, no contraction) so the CPU
+// restatement reproduces the lists bit for bit:
+//   key bit b of table t = dot(e, h[t][b]) > 0, h from a SplitMix64 stream
+//   (component = ((z >> 40) * 2^-24) * 2 - 1);
+//   stable sort by key; distance = sum_d (e_x[d] - e_y[d])^2;
+//   result = the k smallest distinct (distance, index) pairs over all tables.
+#include "svd_internal.hpp"
+#include <rocprim/device/device_radix_sort.hpp>
+
+namespace svd {
+
+namespace {
+
+constexpr int kDim = 32;
+constexpr int kMaxK = 32;
+constexpr int kTile = 256;
+constexpr int kRow = kDim + 1;             // padded LDS row (no bank conflicts)
+
+__global__ void k_lsh_keys(const float* __restrict__ e, uint64_t n, const float* __restrict__ h, uint32_t bits,
+                           uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float x[kDim];
+#pragma unroll
+  for (int d = 0; d < kDim; ++d) x[d] = e[i * kDim + d];
+  uint32_t key = 0;
+  for (uint32_t b = 0; b < bits; ++b) {
+    const float* hp = h + (uint64_t)b * kDim;
+    float s = 0.f;
+#pragma unroll
+    for (int d = 0; d < kDim; ++d) { const float q = x[d] * hp[d]; s = s + q; }
+    if (s > 0.f) key |= 1u << b;
+  }
+  keys[i] = key;
+  vals[i] = (uint32_t)i;
+}
+
+// Block = 256 consecutive sorted positions; the window's rows are staged in
+// LDS.  Each lane keeps its point's running top-k (registers) across tables
+// through best_d / best_i in HBM.
+__global__ void __launch_bounds__(kTile) k_window_topk(const float* __restrict__ e, const uint32_t* __restrict__ order,
+                                                       uint64_t n, uint32_t W, uint32_t k, int first,
+                                                       float* __restrict__ best_d, uint32_t* __restrict__ best_i) {
+  extern __shared__ float rows[];          // (kTile + 2W) x kRow
+  const uint64_t q0 = (uint64_t)blockIdx.x * kTile;
+  const int64_t lo = (int64_t)q0 - (int64_t)W;
+  const uint32_t nrows = kTile + 2 * W;
+  for (uint32_t t = threadIdx.x; t < nrows * kDim; t += kTile) {
+    const uint32_t rr = t / kDim, d = t % kDim;
+    const int64_t q = lo + (int64_t)rr;
+    rows[rr * kRow + d] = (q >= 0 && q < (int64_t)n) ? e[(uint64_t)order[q] * kDim + d] : 0.f;
+  }
+  __syncthreads();
+  const uint64_t q = q0 + threadIdx.x;
+  if (q >= n) return;
+  const uint32_t x = order[q];
+  float bd[kMaxK];
+  uint32_t bi[kMaxK];
+#pragma unroll
+  for (int t = 0; t < kMaxK; ++t) {
+    bd[t] = first || t >= (int)k ? __builtin_inff() : best_d[(uint64_t)x * k + t];
+    bi[t] = first || t >= (int)k ? 0xFFFFFFFFu : best_i[(uint64_t)x * k + t];
+  }
+  const float* ex = &rows[(threadIdx.x + W) * kRow];
+  for (int64_t r = (int64_t)q - (int64_t)W; r <= (int64_t)q + (int64_t)W; ++r) {
+    if (r < 0 || r >= (int64_t)n || r == (int64_t)q) continue;
+    const uint32_t y = order[r];
+    const float* ey = &rows[(uint32_t)(r - lo) * kRow];
+    float s = 0.f;
+#pragma unroll
+    for (int d = 0; d < kDim; ++d) { const float t = ex[d] - ey[d]; const float sq = t * t; s = s + sq; }
+    // (s, y) < worst kept?
+    float wd = bd[0];
+    uint32_t wi = bi[0];
+#pragma unroll
+    for (int t = 1; t < kMaxK; ++t)
+      if (t == (int)k - 1) { wd = bd[t]; wi = bi[t]; }
+    if (!(s < wd || (s == wd && y < wi))) continue;
+    bool dup = false;
+#pragma unroll
+    for (int t = 0; t < kMaxK; ++t) dup |= (t < (int)k) && bi[t] == y;
+    if (dup) continue;
+    // insert keeping (distance, index) ascending
+    float cd = s;
+    uint32_t ci = y;
+#pragma unroll
+    for (int t = 0; t < kMaxK; ++t) {
+      if (t < (int)k && (cd < bd[t] || (cd == bd[t] && ci < bi[t]))) {
+        const float td = bd[t]; const uint32_t ti = bi[t];
+        bd[t] = cd; bi[t] = ci;
+        cd = td; ci = ti;
+      }
+    }
+  }
+#pragma unroll
+  for (int t = 0; t < kMaxK; ++t)
+    if (t < (int)k) { best_d[(uint64_t)x * k + t] = bd[t]; best_i[(uint64_t)x * k + t] = bi[t]; }
+}
+
+}  // namespace
+
+void knn_lsh(svd_ctx* c, const float* emb, uint64_t n, uint32_t k, uint32_t tables, uint32_t bits, uint32_t window,
+             uint64_t seed, uint32_t* out) {
+  if (k < 1 || k > (uint32_t)kMaxK) throw Error{SVD_ERR_ARG, "knn: k must be 1..32"};
+  if (bits < 1 || bits > 24 || tables < 1 || window < 1 || window > 96) throw Error{SVD_ERR_ARG, "knn: bits 1..24, tables >= 1, window 1..96"};
+  if (n >= 0xFFFFFFFFull) throw Error{SVD_ERR_CAPACITY, "knn: too many points"};
+  if (n == 0) return;
+  // hyperplanes of every table
+  std::vector<float> h((size_t)tables * bits * kDim);
+  uint64_t s = seed;
+  for (auto& v : h) {
+    s += 0x9E3779B97F4A7C15ull;
+    uint64_t z = s;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    v = (float)(z >> 40) * (1.0f / 16777216.0f) * 2.0f - 1.0f;
+  }
+  float* dh = dalloc<float>(c, h.size());
+  SVD_HIP(hipMemcpyAsync(dh, h.data(), h.size() * 4, hipMemcpyHostToDevice, c->stream));
+  uint32_t* keys = dalloc<uint32_t>(c, n);
+  uint32_t* keys2 = dalloc<uint32_t>(c, n);
+  uint32_t* vals = dalloc<uint32_t>(c, n);
+  uint32_t* order = dalloc<uint32_t>(c, n);
+  float* bd = dalloc<float>(c, n * k);
+  size_t bytes = 0;
+  SVD_HIP(rocprim::radix_sort_pairs(nullptr, bytes, keys, keys2, vals, order, (size_t)n, 0, bits, c->stream));
+  void* tmp = c->arena->get(bytes + 16);
+  const size_t lds = (size_t)(kTile + 2 * window) * kRow * sizeof(float);
+  for (uint32_t t = 0; t < tables; ++t) {
+    k_lsh_keys<<<div_up(n, 256), 256, 0, c->stream>>>(emb, n, dh + (size_t)t * bits * kDim, bits, keys, vals);
+    SVD_HIP(rocprim::radix_sort_pairs(tmp, bytes, keys, keys2, vals, order, (size_t)n, 0, bits, c->stream));
+    k_window_topk<<<div_up(n, kTile), kTile, lds, c->stream>>>(emb, order, n, window, k, t == 0, bd, out);
+    check_launch("knn table");
+  }
+  c->arena->put(tmp);
+  dfree(c, dh); dfree(c, keys); dfree(c, keys2); dfree(c, vals); dfree(c, order); dfree(c, bd);
+}
+
+}  // namespace svd

@@ -83,6 +83,18 @@ struct svd_octree {
   svd::Encoded enc[4];
 };
 
+// Learned candidate encoder: fp32 master parameters (flat, SVD_MLP_PARAMS),
+// Adam moments, gradients and the bf16 weight copies the MFMA kernels read.
+struct svd_mlp {
+  svd_ctx* ctx = nullptr;
+  float* params = nullptr;
+  float* grads = nullptr;
+  float* m = nullptr;
+  float* v = nullptr;
+  uint16_t* wb = nullptr;          // bf16 W1 | W2 | W3 (same row-major layouts)
+  uint64_t steps = 0;              // Adam steps taken
+};
+
 struct svd_scene {
   svd_ctx* ctx = nullptr;
   int fmt = 0;
@@ -135,6 +147,7 @@ void fill_u8(svd_ctx* c, uint8_t* d, uint8_t v, uint64_t n);
 // ---- build / reduction (build.hip) -----------------------------------------
 void build_svo_from_morton(svd_octree* o, const uint64_t* dcodes, uint64_t n);
 void reduce_dag(svd_octree* o);
+uint64_t dedup_all(svd_ctx* c, uint32_t L, const std::vector<Level>& src, std::vector<Level>& dst);
 void reduce_sym(svd_octree* o);
 void cross_level_merge(svd_octree* o);
 
@@ -152,6 +165,21 @@ void encode(svd_octree* o, int fmt);
 
 // ---- traversal (trace.hip) -----------------------------------------------------
 void trace_frame(svd_scene* s, const svd_trace_params* p, svd_ray_result* prim, svd_ray_result* shad);
+
+// ---- lossy DAG (lossy.hip), learned encoder (mlp.hip), kNN (knn.hip) ----------
+void sort_u64(svd_ctx* c, uint64_t* keys, uint64_t n);
+void lossy_signatures(svd_octree* o, uint32_t lev, uint32_t* sig);
+void lossy_merge(svd_octree* o, const svd_lossy_params* p, std::vector<svd_lossy_level_stats>& out);
+void mlp_embed(svd_mlp* m, const uint32_t* sig, uint64_t n, float* emb);
+void mlp_alloc(svd_mlp* m);
+void mlp_release(svd_mlp* m);
+void mlp_init(svd_mlp* m, uint64_t seed);
+void mlp_set_params(svd_mlp* m, const float* host);
+void mlp_get_params(svd_mlp* m, float* host);
+void mlp_loss_grad(svd_mlp* m, const uint32_t* X, const uint32_t* labels, uint64_t P, float* lossHost, float* gradsHost);
+void mlp_train(svd_mlp* m, svd_octree* o, const svd_mlp_train_params* p, svd_mlp_train_stats* st);
+void knn_lsh(svd_ctx* c, const float* emb, uint64_t n, uint32_t k, uint32_t tables, uint32_t bits, uint32_t window,
+             uint64_t seed, uint32_t* out);
 
 // ---- timing helpers ------------------------------------------------------------
 struct Timer {

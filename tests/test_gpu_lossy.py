@@ -1,0 +1,241 @@
+"""GPU parity of the lossy DAG (LSVDAG) row and its learned proposer.
+
+* exhaustive proposer: every level array, every per-level statistic and the
+  encoded .svdag of the lossy DAG are bit-exact against the oracle;
+* learned proposer: given the kNN lists the GPU used (recorded), the oracle
+  reproduces the whole lossy DAG bit-exactly; the kNN itself is bit-exact
+  against the oracle's sequential restatement on the same embeddings;
+* MLP forward / gradients against the NumPy oracle (bf16 rounding at the
+  same points), with the tolerances written below: the GPU accumulates in a
+  different order (MFMA), so an intermediate bf16 rounding can land one ulp
+  (2^-8 relative) apart.
+"""
+import numpy as np
+import pytest
+import torch
+
+import mlp_oracle as M
+import oracle as O
+from decode import decode_words
+
+import svdag_amd as S
+import svdag_amd.workloads as W
+from test_oracle import clustered_codes, random_codes
+
+pytestmark = pytest.mark.gpu
+
+DEV = "cuda:0"
+
+SCENES = {
+    "terrain7": lambda: (W.terrain_codes(7, seed=1), 7),
+    "caves7": lambda: (W.cave_codes(7, seed=3), 7),
+    "random6": lambda: (random_codes(6, 20000, 3), 6),
+    "clustered7": lambda: (clustered_codes(7, 2), 7),
+}
+
+STAT_FIELDS = ("level", "bound", "nodes_before", "nodes_after", "candidates", "listed", "verified", "merged",
+               "error_bound")
+
+
+def gpu_dag(ctx, codes, L):
+    g = S.Octree.from_morton(ctx, codes, L)
+    g.to_dag()
+    return g
+
+
+def oracle_dag(codes, L):
+    o = O.OracleOctree.from_morton(codes, L)
+    o.to_dag()
+    return o
+
+
+def assert_same_dag(g, o, L):
+    for l in range(L):
+        np.testing.assert_array_equal(g.level(S.VIEW_DAG, l), o.level(l), err_msg=f"level {l}")
+    assert g.counts()[0] == o.n_nodes
+
+
+def assert_same_stats(st_g, st_o):
+    for f in STAT_FIELDS:
+        np.testing.assert_array_equal(st_g[f], st_o[f], err_msg=f)
+
+
+@pytest.mark.parametrize("name", list(SCENES))
+@pytest.mark.parametrize("mode", ["max10", "factor1", "refs3"])
+def test_lossy_exhaustive_bit_exact(ctx, name, mode):
+    codes, L = SCENES[name]()
+    kw = {"max10": dict(max_diff=10), "factor1": dict(max_diff=0, diff_factor=1.0),
+          "refs3": dict(max_diff=6, max_refs=3, list_cap=5)}[mode]
+    print(f"[case] {name} {mode} L={L} voxels={codes.size}", flush=True)
+    g, o = gpu_dag(ctx, codes, L), oracle_dag(codes, L)
+    st_g = g.lossy_merge(1, L - 2, **kw)
+    st_o = o.lossy_merge(1, L - 2, **kw)
+    assert_same_stats(st_g, st_o)
+    assert_same_dag(g, o, L)
+    assert st_g["merged"].sum() > 0 or mode == "refs3"
+    g.encode(S.FMT_SVDAG)
+    o.init_child_levels()
+    assert g.encoded_bytes(S.FMT_SVDAG) == o.encode(0)
+    # the lossy DAG feeds the symmetry pass and the SSVDAG encoder as usual
+    g.to_sdag()
+    g.encode(S.FMT_SSVDAG)
+    o.to_sdag()
+    assert g.encoded_bytes(S.FMT_SSVDAG) == o.encode(2)
+
+
+def test_lossy_error_within_accounted_bound(ctx):
+    codes, L = W.cave_codes(7, seed=3), 7
+    g = gpu_dag(ctx, codes, L)
+    st = g.lossy_merge(1, L - 2, max_diff=10)
+    g.encode(S.FMT_SVDAG)
+    vox = decode_words(g.encoded_bytes(S.FMT_SVDAG), False)
+    diff = np.setxor1d(vox, codes).size
+    assert 0 < diff <= int(st["error_bound"].sum()), (diff, st["error_bound"])
+
+
+def test_lossy_signatures_match_oracle(ctx):
+    codes, L = W.terrain_codes(7, seed=1), 7
+    g, o = gpu_dag(ctx, codes, L), oracle_dag(codes, L)
+    for lev in range(1, L - 1):
+        n = g.level_size(S.VIEW_DAG, lev)
+        sig = torch.zeros(n * 16, dtype=torch.int32, device=DEV)
+        g.lossy_signatures(lev, sig.data_ptr())
+        np.testing.assert_array_equal(sig.cpu().numpy().view(np.uint32).reshape(n, 16), o.lossy_signatures(lev))
+
+
+def test_lossy_argument_errors(ctx):
+    codes, L = W.terrain_codes(6, seed=1), 6
+    g = gpu_dag(ctx, codes, L)
+    for bad in (dict(lev_lo=0, lev_hi=3), dict(lev_lo=2, lev_hi=5), dict(lev_lo=3, lev_hi=2)):
+        with pytest.raises(S.SvdError):
+            g.lossy_merge(bad["lev_lo"], bad["lev_hi"], max_diff=4)
+    with pytest.raises(S.SvdError):
+        g.lossy_merge(1, 4, max_diff=4, list_cap=65)
+    sv = S.Octree.from_morton(ctx, codes, L)          # still an SVO
+    with pytest.raises(S.SvdError):
+        sv.lossy_merge(1, 4, max_diff=4)
+
+
+# ---------------------------------------------------------------- MLP
+def _rand_sigs(n, seed, density=0.15):
+    rng = np.random.default_rng(seed)
+    bits = (rng.random((n, 16, 32)) < density).astype(np.uint64)
+    return (bits << np.arange(32, dtype=np.uint64)).sum(axis=2).astype(np.uint32)
+
+
+def _dev_u32(a):
+    return torch.from_numpy(np.ascontiguousarray(a).view(np.int32)).to(DEV)
+
+
+def test_mlp_init_matches_oracle(ctx):
+    m = S.MLP(ctx, seed=5)
+    np.testing.assert_array_equal(m.params(), M.init_params(5))
+
+
+def test_mlp_forward_matches_oracle(ctx):
+    m = S.MLP(ctx, seed=3)
+    p = M.init_params(3)
+    p[M.split(p)[0].size:M.split(p)[0].size + 256] = np.linspace(-0.1, 0.1, 256, dtype=np.float32)   # b1
+    m.set_params(p)
+    n = 1000
+    sig = _rand_sigs(n, 1)
+    emb = torch.zeros(n * 32, dtype=torch.float32, device=DEV)
+    m.embed(_dev_u32(sig).data_ptr(), n, emb.data_ptr())
+    e_g = emb.cpu().numpy().reshape(n, 32)
+    e_o, _ = M.forward(p, M.unpack_bits(sig))
+    scale = np.abs(e_o).max()
+    # one bf16 ulp (2^-8) of a hidden activation moves e by far less than 1% of its range
+    assert np.abs(e_g - e_o).max() <= 1e-2 * scale, np.abs(e_g - e_o).max() / scale
+    assert np.corrcoef(e_g.ravel(), e_o.ravel())[0, 1] > 0.9999
+
+
+def test_mlp_gradients_match_oracle(ctx):
+    m = S.MLP(ctx, seed=9)
+    p = M.init_params(9)
+    P = 256
+    sig = _rand_sigs(2 * P, 2)
+    sig[1::2] = sig[0::2] ^ (_rand_sigs(P, 3, density=0.02))     # near pairs
+    H = M.hamming(sig[0::2], sig[1::2])
+    loss_g, grad_g = m.loss_grad(_dev_u32(sig).data_ptr(), _dev_u32(H.astype(np.uint32)).data_ptr(), P)
+    loss_o, grad_o = M.loss_and_grads(p, M.unpack_bits(sig), H)
+    assert abs(loss_g - loss_o) <= 1e-3 * abs(loss_o)
+    off = 0
+    for name, t in zip(("W1", "b1", "W2", "b2", "W3", "b3"), M.split(p)):
+        a, b = grad_g[off:off + t.size], grad_o[off:off + t.size]
+        off += t.size
+        rel = np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30)
+        assert rel <= 2e-2, (name, rel)
+
+
+def test_mlp_training_reduces_loss(ctx):
+    codes, L = W.cave_codes(7, seed=3), 7
+    g = gpu_dag(ctx, codes, L)
+    m = S.MLP(ctx, seed=1)
+    st = m.train(g, 2, L - 2, pairs_per_step=512, steps=300, lr=2e-3, seed=5)
+    print(f"loss {st.first_loss:.3f} -> {st.last_loss:.3f} over {st.pairs} pairs, {st.ms_train:.1f} ms", flush=True)
+    assert st.last_loss < 0.5 * st.first_loss
+
+
+# ---------------------------------------------------------------- kNN
+def test_knn_matches_oracle(ctx):
+    rng = np.random.default_rng(4)
+    n = 3000
+    e = (rng.standard_normal((n, 32)) * np.linspace(2, 0.1, 32)).astype(np.float32)
+    e[100] = e[7]                       # an exact duplicate (distance 0, tie by index)
+    de = torch.from_numpy(e).to(DEV)
+    out = torch.zeros(n * 16, dtype=torch.int32, device=DEV)
+    ctx.knn(de.data_ptr(), n, 16, 4, 16, 32, 7, out.data_ptr())
+    got = out.cpu().numpy().view(np.uint32).reshape(n, 16)
+    np.testing.assert_array_equal(got, O.knn_lsh(e, 16, 4, 16, 32, 7))
+    assert got[7, 0] == 100 and got[100, 0] == 7
+
+
+def test_knn_full_window_is_brute_force(ctx):
+    rng = np.random.default_rng(5)
+    n = 90                               # every point within +-96 positions of every other
+    e = rng.standard_normal((n, 32)).astype(np.float32)
+    de = torch.from_numpy(e).to(DEV)
+    out = torch.zeros(n * 8, dtype=torch.int32, device=DEV)
+    ctx.knn(de.data_ptr(), n, 8, 1, 4, 96, 3, out.data_ptr())
+    got = out.cpu().numpy().view(np.uint32).reshape(n, 8)
+    d = np.zeros((n, n), np.float32)           # the same sequential fp32 sum as the kernel
+    for k in range(32):
+        t = e[:, None, k] - e[None, :, k]
+        d = d + t * t
+    np.fill_diagonal(d, np.inf)
+    want = np.argsort(d, axis=1, kind="stable")[:, :8]
+    np.testing.assert_array_equal(got, want)
+
+
+# ---------------------------------------------------------------- learned lossy
+def test_lossy_learned_matches_oracle_given_knn(ctx):
+    codes, L = W.cave_codes(7, seed=3), 7
+    g, o = gpu_dag(ctx, codes, L), oracle_dag(codes, L)
+    m = S.MLP(ctx, seed=1)
+    m.train(g, 1, L - 2, pairs_per_step=512, steps=200, lr=2e-3, seed=5)
+    st_g, rec = g.lossy_merge(1, L - 2, max_diff=10, model=m, record_knn=True)
+    st_o = o.lossy_merge(1, L - 2, max_diff=10, proposals=rec, k=16, list_cap=32)
+    assert_same_stats(st_g, st_o)
+    assert_same_dag(g, o, L)
+    assert st_g["merged"].sum() > 0
+
+
+def test_learned_proposer_gains_over_untrained(ctx):
+    """Compression with the trained encoder vs the same proposer with the
+    untrained (random-init) encoder and vs exhaustive search."""
+    codes, L = W.cave_codes(8, seed=3), 8
+    res = {}
+    for tag in ("exhaustive", "untrained", "trained"):
+        g = gpu_dag(ctx, codes, L)
+        m = None
+        if tag != "exhaustive":
+            m = S.MLP(ctx, seed=1)
+            if tag == "trained":
+                m.train(g, 2, L - 2, pairs_per_step=1024, steps=400, lr=2e-3, seed=5)
+        st = g.lossy_merge(2, L - 2, max_diff=10, model=m)
+        res[tag] = (g.counts()[0], int(st["merged"].sum()))
+    print(res, flush=True)
+    exh, unt, tr = res["exhaustive"][0], res["untrained"][0], res["trained"][0]
+    assert tr < unt                      # the trained encoder finds more merges
+    assert res["trained"][1] > res["untrained"][1]
+    assert exh <= tr
