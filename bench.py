@@ -75,6 +75,57 @@ def cpu_baseline(W, levels_full, sample_levels):
                        f"std::map / std::sort, single thread, {t2 - t0:.2f} s")
 
 
+LOSSY_BYTES_PER_NODE = 1284   # 64 B occupancy + 64 B embedding write + 64 B read + 16 x (64 + 4) B verify + 4 B remap
+
+
+def lossy_bench(S, ctx, levels, log):
+    """Lossy DAG on the cave volume: learned proposer (MLP embeddings + kNN,
+    every candidate verified exactly) against the exhaustive proposer, same
+    voxel bound, levels L-6..L-3, nodes with at most one effective reference."""
+    lo, hi, bound = levels - 6, levels - 3, 10
+    t0 = time.perf_counter()
+    codes = ctx.caves(levels, seed=3)
+    log(f"[bench] caves {1 << levels}^3: {codes.n} voxels ({time.perf_counter() - t0:.1f} s)")
+
+    def fresh():
+        g = S.Octree.from_morton(ctx, (codes.ptr, codes.n), levels)
+        g.to_dag()
+        return g
+
+    g = fresh()
+    nodes_dag = g.counts()[0]
+    m = S.MLP(ctx, seed=1)
+    pairs_per_step, steps = 4096, 1024
+    tr = m.train(g, lo, hi, pairs_per_step=pairs_per_step, steps=steps, lr=2e-3, seed=5)
+    log(f"[bench] trained on {tr.pairs} pairs: loss {tr.first_loss:.2f} -> {tr.last_loss:.2f} ({tr.ms_train:.0f} ms)")
+    g.free()
+    out = {"workload": f"caves {1 << levels}^3 ({levels} levels, seed 3, {codes.n} voxels): lossy DAG merge of "
+                       f"levels {lo}-{hi}, bound {bound} voxels, nodes with <= 1 effective reference; "
+                       f"learned = MLP 512-256-128-32 embeddings + LSH kNN k=16, every candidate verified exactly",
+           "nodes_dag": int(nodes_dag),
+           "training": {"pairs": int(tr.pairs), "ms": round(tr.ms_train, 1), "first_loss": round(tr.first_loss, 3),
+                        "last_loss": round(tr.last_loss, 3)}}
+    for tag, model in (("learned", m), ("exhaustive", None)):
+        g = fresh()
+        st = g.lossy_merge(lo, hi, max_diff=bound, model=model)
+        cand = int(st["candidates"].sum())
+        ms = float(st["ms_total"].sum())
+        out[tag] = {"nodes_after": int(g.counts()[0]), "merged": int(st["merged"].sum()), "candidates": cand,
+                    "ms": round(ms, 3), "nodes_s": round(cand / (ms * 1e-3), 1),
+                    "ms_phases": {k: round(float(st[k].sum()), 3) for k in
+                                  ("ms_embed", "ms_knn", "ms_lists", "ms_verify", "ms_merge")}}
+        log(f"[bench] lossy {tag}: {out[tag]}")
+        g.free()
+    codes.free()
+    le, ex = out["learned"], out["exhaustive"]
+    out["compression_gap"] = round((le["nodes_after"] - ex["nodes_after"]) / ex["nodes_after"], 5)
+    out["speedup_vs_exhaustive"] = round(le["nodes_s"] / ex["nodes_s"], 2)
+    achieved = le["candidates"] * LOSSY_BYTES_PER_NODE / (le["ms"] * 1e-3) / 1e9
+    out["roofline"] = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                       "frac": round(achieved / HBM_PEAK_GBS, 5), "bytes_per_node": LOSSY_BYTES_PER_NODE}
+    return out
+
+
 def measured_traffic(L, args):
     """HBM bytes per trace launch from the committed counter summary (two
     rocprofv3 --pmc runs of this workload, FETCH_SIZE and WRITE_SIZE, each
@@ -99,6 +150,8 @@ def main():
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-levels", type=int, default=11)
+    ap.add_argument("--lossy-levels", type=int, default=12,
+                    help="grid of the lossy-DAG measurement (caves 2^L); 0 skips it")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB three times (HBM counter calibration)")
     args = ap.parse_args()
@@ -201,6 +254,8 @@ def main():
                      "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": measured_traffic(L, args),
                      "kernel": "k_trace<SSVDAG>", "algorithmic_bytes": bytes_fetched},
     }
+    if args.lossy_levels:
+        out["lossy"] = lossy_bench(S, ctx, args.lossy_levels, log)
     if not args.no_cpu_baseline:
         log("[bench] cpu baseline (oracle) ...")
         out["cpu_baseline"] = cpu_baseline(W, L, args.cpu_sample_levels)

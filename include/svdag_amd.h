@@ -248,6 +248,18 @@ int svd_mlp_train(svd_mlp* m, svd_octree* oct, const svd_mlp_train_params* p, sv
 int svd_knn(svd_ctx* ctx, const float* emb_dev, uint64_t n, uint32_t k, uint32_t tables, uint32_t bits,
             uint32_t window, uint64_t seed, uint32_t* out_dev);
 
+/* ---- synthetic scenes (benchmark and test inputs; no reference counterpart)
+ * Cave volume: surface voxels (solid with an empty 6-neighbour) of
+ * {fbm3(p) - 0.5 > threshold}, p = (i + 0.5) * base_cells / 2^levels, value
+ * noise with a SplitMix64 lattice hash, lacunarity 2, gain 0.5 -- the float64
+ * definition of workloads.cave_codes, bit for bit.  *out_dev receives a
+ * device array of *n sorted Morton codes; release it with svd_device_free.   */
+int svd_workload_caves(svd_ctx* ctx, uint32_t levels, uint64_t seed, uint32_t octaves, double threshold,
+                       double base_cells, uint64_t** out_dev, uint64_t* n);
+int svd_device_free(svd_ctx* ctx, void* dev_ptr);
+/* Copy bytes from a device array to a HOST buffer (synchronous). */
+int svd_device_read(svd_ctx* ctx, void* host_dst, const void* dev_src, uint64_t bytes);
+
 /* ---- reference-count order ------------------------------------------------
  * The permutation std::sort(hist, [](a,b){ return a.second > b.second; })
  * produces in EncodedSSVDAG::encode (encoded_ssvdag.cpp:273-275), computed on

@@ -140,6 +140,9 @@ def lib():
         "svd_mlp_loss_grad": (i32, [vp, vp, vp, u64, P(C.c_float), vp]),
         "svd_mlp_train": (i32, [vp, vp, P(TrainParams), P(TrainStats)]),
         "svd_knn": (i32, [vp, vp, u64, u32, u32, u32, u32, u64, vp]),
+        "svd_workload_caves": (i32, [vp, u32, u64, u32, C.c_double, C.c_double, P(vp), P(u64)]),
+        "svd_device_free": (i32, [vp, vp]),
+        "svd_device_read": (i32, [vp, vp, vp, u64]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -156,7 +159,8 @@ EXPORTED = ["svd_ctx_create", "svd_ctx_destroy", "svd_last_error", "svd_ctx_sync
             "svd_encoded_device", "svd_scene_create", "svd_scene_from_octree", "svd_scene_free", "svd_trace",
             "svd_last_kernel_ms", "svd_refcount_order", "svd_attr_bittree_encode", "svd_lossy_merge",
             "svd_lossy_signatures", "svd_mlp_create", "svd_mlp_free", "svd_mlp_get_params", "svd_mlp_set_params",
-            "svd_mlp_embed", "svd_mlp_loss_grad", "svd_mlp_train", "svd_knn"]
+            "svd_mlp_embed", "svd_mlp_loss_grad", "svd_mlp_train", "svd_knn", "svd_workload_caves",
+            "svd_device_free", "svd_device_read"]
 
 
 class Context:
@@ -214,6 +218,14 @@ class Context:
         """k nearest neighbours of n 32-d fp32 points (device pointers)."""
         self.check(self.L.svd_knn(self.h, C.c_void_p(emb_ptr), n, k, tables, bits, window, seed, C.c_void_p(out_ptr)))
 
+    def caves(self, levels: int, seed: int = 3, octaves: int = 5, threshold: float = 0.1,
+              base_cells: float = 4.0) -> "DeviceCodes":
+        """Cave-volume surface voxels generated on the device (same definition as workloads.cave_codes)."""
+        ptr, n = C.c_void_p(), C.c_uint64()
+        self.check(self.L.svd_workload_caves(self.h, levels, seed, octaves, threshold, base_cells, C.byref(ptr),
+                                             C.byref(n)))
+        return DeviceCodes(self, ptr.value, n.value)
+
     def close(self):
         if getattr(self, "h", None):
             self.L.svd_ctx_destroy(self.h)
@@ -222,6 +234,31 @@ class Context:
     def __del__(self):
         try:
             self.close()
+        except Exception:
+            pass
+
+
+class DeviceCodes:
+    """Sorted Morton codes in device memory owned by the library."""
+
+    def __init__(self, ctx: Context, ptr: int, n: int):
+        self.ctx, self.ptr, self.n = ctx, ptr, n
+
+    def numpy(self) -> np.ndarray:
+        out = np.zeros(self.n, np.uint64)
+        if self.n:
+            self.ctx.check(self.ctx.L.svd_device_read(self.ctx.h, out.ctypes.data_as(C.c_void_p),
+                                                      C.c_void_p(self.ptr), self.n * 8))
+        return out
+
+    def free(self):
+        if self.ptr:
+            self.ctx.L.svd_device_free(self.ctx.h, C.c_void_p(self.ptr))
+            self.ptr = 0
+
+    def __del__(self):
+        try:
+            self.free()
         except Exception:
             pass
 

@@ -621,3 +621,26 @@ int svd_knn(svd_ctx* c, const float* emb, uint64_t n, uint32_t k, uint32_t table
     knn_lsh(c, emb, n, k, tables, bits, window, seed, out);
   });
 }
+
+// ---- synthetic scenes ----------------------------------------------------------
+int svd_workload_caves(svd_ctx* c, uint32_t levels, uint64_t seed, uint32_t octaves, double threshold,
+                       double base_cells, uint64_t** out, uint64_t* n) {
+  if (!c || !out || !n || octaves < 1 || octaves > 16) return SVD_ERR_ARG;
+  return guarded(c, [&] {
+    Timer t(c, "workload");
+    *n = gen_caves(c, levels, seed, octaves, threshold, base_cells, out);
+  });
+}
+
+int svd_device_free(svd_ctx* c, void* p) {
+  if (!c) return SVD_ERR_ARG;
+  return guarded(c, [&] { if (p) SVD_HIP(hipFree(p)); });
+}
+
+int svd_device_read(svd_ctx* c, void* dst, const void* src, uint64_t bytes) {
+  if (!c || (bytes && (!dst || !src))) return SVD_ERR_ARG;
+  return guarded(c, [&] {
+    if (bytes) SVD_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
+    SVD_HIP(hipStreamSynchronize(c->stream));
+  });
+}

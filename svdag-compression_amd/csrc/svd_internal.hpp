@@ -181,6 +181,10 @@ void mlp_train(svd_mlp* m, svd_octree* o, const svd_mlp_train_params* p, svd_mlp
 void knn_lsh(svd_ctx* c, const float* emb, uint64_t n, uint32_t k, uint32_t tables, uint32_t bits, uint32_t window,
              uint64_t seed, uint32_t* out);
 
+// ---- synthetic scenes (workload.hip) ---------------------------------------------
+uint64_t gen_caves(svd_ctx* c, uint32_t levels, uint64_t seed, uint32_t octaves, double threshold, double baseCells,
+                   uint64_t** out);
+
 // ---- timing helpers ------------------------------------------------------------
 struct Timer {
   svd_ctx* c;

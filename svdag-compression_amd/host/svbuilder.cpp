@@ -109,10 +109,6 @@ int main(int argc, char** argv) {
     }
   }
   printf("Lossy: %d, Cross-level: %d, Hidden geom: %d\n", lossy, multiLevel, 0);
-  if (lossy) {
-    printf("Lossy (LSVDAG) merging is available through the Python API (svdag_amd.lossy); building lossless.\n");
-    lossy = false;
-  }
 
   const svd::aabox3f ab = scene.getAABB();
   svd::aabox3d sceneBBoxD(svd::point3d(ab[0][0], ab[0][1], ab[0][2]), svd::point3d(ab[1][0], ab[1][1], ab[1][2]));
@@ -138,11 +134,26 @@ int main(int argc, char** argv) {
   octree.initChildLevels();
 
   const std::string basePath = dir_of(inputFile) + "/" + base_without_ext(inputFile) + "_" + std::to_string(nLevels);
+  // main.cpp:213-217: "-l_<infl>_<factor>_<refs>" with one decimal kept
+  auto trim1 = [](float v) { std::string t = std::to_string(v); return t.substr(0, t.find(".") + 2); };
+  const std::string paramStr = "-l_" + trim1(lossyInflation) + "_" + trim1(allowedLossyDiffFactor) + "_" +
+                               std::to_string(includedNodeRefCount);
   EncodedSVDAG svdag2;
   svdag2.encode(octree);
   svdag2.save(basePath + ".svdag");
 
-  if (multiLevel) octree.mergeAcrossAllLevels();
+  if (lossy) {                         // main.cpp:225-229
+    EncodedSSVDAG esvdag2;
+    esvdag2.encode(octree);
+    esvdag2.save(basePath + ".esvdag");
+  }
+  if (lossy) {                         // main.cpp:231-238
+    const double origDagTime = octree.getStats().toDAGTime;
+    octree.toLossyDag(lossyInflation, allowedLossyDiffFactor, includedNodeRefCount);
+    octree.getStats().toDAGTime = origDagTime;
+  } else if (multiLevel) {
+    octree.mergeAcrossAllLevels();
+  }
 
   EncodedSVDAG svdag;
   svdag.encode(octree);
@@ -154,7 +165,7 @@ int main(int argc, char** argv) {
   EncodedSSVDAG ssvdag;
   if (!multiLevel) ssvdag.encode(octree);
 
-  std::string infix = multiLevel ? "-multi" : "";
+  std::string infix = lossy ? paramStr : (multiLevel ? "-multi" : "");
   svdag.save(basePath + infix + ".svdag");
   if (!multiLevel) {
     ussvdag.save(basePath + infix + ".ussvdag");

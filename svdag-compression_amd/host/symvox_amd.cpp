@@ -517,6 +517,41 @@ unsigned int GeomOctree::mergeAcrossAllLevels() {
   return (unsigned)n;
 }
 
+void GeomOctree::toLossyDag(float lossyInflation, float allowedLossyDiffFactor, int includedNodeRefCount) {
+  (void)lossyInflation;
+  if (_state != S_DAG) { printf("ERROR! This is not a DAG!\n"); return; }
+  if (_levels < 3) { printf("ERROR! A lossy DAG needs at least 3 levels\n"); return; }
+  printf("* Transforming DAG -> Lossy DAG3 ... \n");
+  svd_lossy_params p = {};
+  p.lev_lo = 1;
+  p.lev_hi = _levels - 2;
+  p.max_diff = 0;
+  p.diff_factor = allowedLossyDiffFactor;
+  p.max_refs = (uint32_t)std::max(1, includedNodeRefCount);
+  p.list_cap = 32;
+  p.proposer = SVD_PROPOSER_EXHAUSTIVE;
+  std::vector<svd_lossy_level_stats> st(_levels);
+  const double t0 = now_s();
+  const int rc = svd_lossy_merge(_oct, &p, st.data(), (uint32_t)st.size());
+  _stats.toLSVDAGTime = now_s() - t0;
+  if (rc != SVD_OK) { printf("\t\t- %s\n", svd_last_error(default_context())); return; }
+  size_t err = 0, merged = 0;
+  for (unsigned i = 0; i + 2 < _levels; ++i) {
+    printf("Level %u: max diff %u, %zu -> %zu nodes (%zu merged)\n", (unsigned)st[i].level, (unsigned)st[i].bound,
+           (size_t)st[i].nodes_before, (size_t)st[i].nodes_after, (size_t)st[i].merged);
+    err += st[i].error_bound;
+    merged += st[i].merged;
+  }
+  const size_t before = _stats.nNodesDAG;
+  uint64_t nn = 0;
+  svd_octree_counts(_oct, &nn, nullptr);
+  _stats.nNodesDAG = nn;
+  _stats.totalLossyVoxelDifference = err;
+  _stats.nClusteredNodes = merged;
+  _cacheState = -1;
+  printf("Lossy compressed from %zu to %zu nodes.\n", before, (size_t)nn);
+}
+
 size_t GeomOctree::getNNodes() const {
   uint64_t nn = 0;
   if (_oct) svd_octree_counts(_oct, &nn, nullptr);

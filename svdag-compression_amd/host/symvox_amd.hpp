@@ -137,6 +137,12 @@ class GeomOctree {
   void toDAG(bool internalCall = false);
   void toSDAG(bool internalCall = false, bool skipSymmetry = false);
   unsigned int mergeAcrossAllLevels();
+  // geom_octree_extension.cpp:797-1179 on the GPU: levels L-2..1, bound
+  // round((L-lev)^2 * allowedLossyDiffFactor), nodes with at most
+  // includedNodeRefCount effective references.  lossyInflation steered the
+  // reference's MCL run, which this build replaces by its deterministic
+  // first-representative rule; it is accepted and ignored.
+  void toLossyDag(float lossyInflation, float allowedLossyDiffFactor, int includedNodeRefCount);
   void initChildLevels() {}   // child levels default to lev+1 on the device
 
   // getters (geom_octree.hpp:111-117, octree.hpp:101-114)

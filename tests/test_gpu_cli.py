@@ -65,6 +65,25 @@ def test_svbuilder_files_equal_oracle(mesh, step):
     assert os.path.exists(d / "stats.txt")
 
 
+@pytest.mark.parametrize("largs,infix,factor,refs", [([], "-l_2.0_1.0_1", 1.0, 1),
+                                                     (["1.5", "0.5", "2"], "-l_1.5_0.5_2", 0.5, 2)])
+def test_svbuilder_lossy_equals_oracle(mesh, largs, infix, factor, refs):
+    """svbuilder model.obj L 0 -l [infl factor refs] (main.cpp:116-137, 225-264)."""
+    d, tris, bmin, bmax = mesh
+    L = 7
+    out = run([os.path.join(BIN, "svbuilder"), "spheres.obj", str(L), "0", "-l"] + largs, d)
+    assert "Lossy compressed from" in out
+    o = O.OracleOctree.from_mesh(tris, L, bmin, bmax)
+    o.to_dag()
+    o.init_child_levels()
+    assert open(d / f"spheres_{L}.esvdag", "rb").read() == o.encode(2)      # saved before the lossy pass
+    st = o.lossy_merge(1, L - 2, max_diff=0, diff_factor=factor, max_refs=refs, list_cap=32)
+    assert st["merged"].sum() > 0
+    want = oracle_files(o)
+    for ext, data in want.items():
+        assert open(d / f"spheres_{L}{infix}{ext}", "rb").read() == data, ext
+
+
 def test_svbuilder_ply_input_equals_obj(mesh):
     d, tris, bmin, bmax = mesh
     run([os.path.join(BIN, "svbuilder"), "spheresply.ply", "6", "0"], d)
