@@ -248,6 +248,30 @@ int svd_mlp_train(svd_mlp* m, svd_octree* oct, const svd_mlp_train_params* p, sv
 int svd_knn(svd_ctx* ctx, const float* emb_dev, uint64_t n, uint32_t k, uint32_t tables, uint32_t bits,
             uint32_t window, uint64_t seed, uint32_t* out_dev);
 
+/* ---- batches: many scenes per launch (no reference counterpart: the
+ * reference builds one model per svbuilder process) ---------------------------
+ * svd_batch_build: n_scenes voxel lists (each sorted, unique, at least one
+ * voxel, below 2^(3*levels[s])) concatenated in `codes`; scene s is
+ * codes[offsets[s] .. offsets[s+1]) (offsets: HOST array of n_scenes+1),
+ * depth levels[s] (HOST array, 2..13, uneven depths allowed).  All scenes are
+ * reduced to DAGs and encoded as .svdag images in shared launches; every
+ * scene's image equals its own svd_octree_from_morton + svd_reduce_dag +
+ * svd_encode(SVD_FMT_SVDAG) byte for byte.                                   */
+typedef struct svd_batch svd_batch;
+int svd_batch_build(svd_ctx* ctx, uint32_t n_scenes, const uint64_t* codes, const uint64_t* offsets,
+                    const uint32_t* levels, int codes_on_device, svd_batch** out);
+int svd_batch_free(svd_batch* b);
+int svd_batch_scene_info(svd_batch* b, uint32_t scene, uint32_t* levels, uint64_t* n_voxels, uint64_t* n_nodes,
+                         uint64_t* svdag_bytes);
+/* The .svdag image of one scene: device pointer, or a copy into a HOST buffer. */
+int svd_batch_svdag(svd_batch* b, uint32_t scene, const void** dptr, uint64_t* bytes);
+int svd_batch_svdag_copy(svd_batch* b, uint32_t scene, void* dst, uint64_t cap);
+/* n_cams cameras (HOST array, one resolution for all) over the batch's scenes
+ * in one launch; camera k looks at scene cam_scene[k] (HOST array).  prim /
+ * shadow: DEVICE arrays of n_cams * width * height results, camera-major.    */
+int svd_batch_trace(svd_batch* b, uint32_t n_cams, const svd_trace_params* cams, const uint32_t* cam_scene,
+                    svd_ray_result* primary, svd_ray_result* shadow);
+
 /* ---- synthetic scenes (benchmark and test inputs; no reference counterpart)
  * Cave volume: surface voxels (solid with an empty 6-neighbour) of
  * {fbm3(p) - 0.5 > threshold}, p = (i + 0.5) * base_cells / 2^levels, value
@@ -256,7 +280,19 @@ int svd_knn(svd_ctx* ctx, const float* emb_dev, uint64_t n, uint32_t k, uint32_t
  * device array of *n sorted Morton codes; release it with svd_device_free.   */
 int svd_workload_caves(svd_ctx* ctx, uint32_t levels, uint64_t seed, uint32_t octaves, double threshold,
                        double base_cells, uint64_t** out_dev, uint64_t* n);
+/* Union of balls (cx, cy, cz, r per ball, HOST array of 4*n_balls doubles):
+ * surface voxels of {|p - c| <= r}, p = (i + 0.5) / 2^levels.              */
+int svd_workload_balls(svd_ctx* ctx, uint32_t levels, const double* balls, uint32_t n_balls, uint64_t** out_dev,
+                       uint64_t* n);
+/* Menger sponge of the given order filling the grid: its surface voxels.    */
+int svd_workload_menger(svd_ctx* ctx, uint32_t levels, uint32_t order, uint64_t** out_dev, uint64_t* n);
+/* fBm heightfield terrain (workloads.terrain_codes): every column filled from
+ * the lowest of its 4-neighbour heights up to its own height.               */
+int svd_workload_terrain(svd_ctx* ctx, uint32_t levels, uint64_t seed, uint32_t octaves, double hmin, double hmax,
+                         double base_cells, uint64_t** out_dev, uint64_t* n);
 int svd_device_free(svd_ctx* ctx, void* dev_ptr);
+/* Copy bytes between two device arrays (on the context stream, synchronous). */
+int svd_device_copy(svd_ctx* ctx, void* dev_dst, const void* dev_src, uint64_t bytes);
 /* Copy bytes from a device array to a HOST buffer (synchronous). */
 int svd_device_read(svd_ctx* ctx, void* host_dst, const void* dev_src, uint64_t bytes);
 

@@ -143,6 +143,16 @@ def lib():
         "svd_workload_caves": (i32, [vp, u32, u64, u32, C.c_double, C.c_double, P(vp), P(u64)]),
         "svd_device_free": (i32, [vp, vp]),
         "svd_device_read": (i32, [vp, vp, vp, u64]),
+        "svd_workload_balls": (i32, [vp, u32, vp, u32, P(vp), P(u64)]),
+        "svd_workload_menger": (i32, [vp, u32, u32, P(vp), P(u64)]),
+        "svd_workload_terrain": (i32, [vp, u32, u64, u32, C.c_double, C.c_double, C.c_double, P(vp), P(u64)]),
+        "svd_batch_build": (i32, [vp, u32, vp, P(u64), P(u32), i32, P(vp)]),
+        "svd_batch_free": (i32, [vp]),
+        "svd_batch_scene_info": (i32, [vp, u32, P(u32), P(u64), P(u64), P(u64)]),
+        "svd_batch_svdag": (i32, [vp, u32, P(vp), P(u64)]),
+        "svd_batch_svdag_copy": (i32, [vp, u32, vp, u64]),
+        "svd_batch_trace": (i32, [vp, u32, vp, P(u32), vp, vp]),
+        "svd_device_copy": (i32, [vp, vp, vp, u64]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -160,7 +170,9 @@ EXPORTED = ["svd_ctx_create", "svd_ctx_destroy", "svd_last_error", "svd_ctx_sync
             "svd_last_kernel_ms", "svd_refcount_order", "svd_attr_bittree_encode", "svd_lossy_merge",
             "svd_lossy_signatures", "svd_mlp_create", "svd_mlp_free", "svd_mlp_get_params", "svd_mlp_set_params",
             "svd_mlp_embed", "svd_mlp_loss_grad", "svd_mlp_train", "svd_knn", "svd_workload_caves",
-            "svd_device_free", "svd_device_read"]
+            "svd_device_free", "svd_device_read", "svd_workload_balls", "svd_workload_menger",
+            "svd_workload_terrain", "svd_batch_build", "svd_batch_free", "svd_batch_scene_info", "svd_batch_svdag",
+            "svd_batch_svdag_copy", "svd_batch_trace", "svd_device_copy"]
 
 
 class Context:
@@ -226,6 +238,34 @@ class Context:
                                              C.byref(n)))
         return DeviceCodes(self, ptr.value, n.value)
 
+    def balls(self, levels: int, balls) -> "DeviceCodes":
+        b = np.ascontiguousarray(balls, dtype=np.float64).reshape(-1, 4)
+        ptr, n = C.c_void_p(), C.c_uint64()
+        self.check(self.L.svd_workload_balls(self.h, levels, b.ctypes.data_as(C.c_void_p), b.shape[0], C.byref(ptr),
+                                             C.byref(n)))
+        return DeviceCodes(self, ptr.value, n.value)
+
+    def menger(self, levels: int, order: int) -> "DeviceCodes":
+        ptr, n = C.c_void_p(), C.c_uint64()
+        self.check(self.L.svd_workload_menger(self.h, levels, order, C.byref(ptr), C.byref(n)))
+        return DeviceCodes(self, ptr.value, n.value)
+
+    def terrain(self, levels: int, seed: int = 1, octaves: int = 6, hmin: float = 0.2, hmax: float = 0.6,
+                base_cells: float = 8.0) -> "DeviceCodes":
+        ptr, n = C.c_void_p(), C.c_uint64()
+        self.check(self.L.svd_workload_terrain(self.h, levels, seed, octaves, hmin, hmax, base_cells, C.byref(ptr),
+                                               C.byref(n)))
+        return DeviceCodes(self, ptr.value, n.value)
+
+    def spec_codes(self, spec) -> "DeviceCodes":
+        """Device voxels of one batch scene (workloads.batch_specs)."""
+        from .workloads import sphere_balls
+        if spec["kind"] == "spheres":
+            return self.balls(spec["levels"], sphere_balls(spec["n_spheres"], spec["seed"]))
+        if spec["kind"] == "terrain":
+            return self.terrain(spec["levels"], seed=spec["seed"])
+        return self.menger(spec["levels"], spec["order"])
+
     def close(self):
         if getattr(self, "h", None):
             self.L.svd_ctx_destroy(self.h)
@@ -252,15 +292,85 @@ class DeviceCodes:
         return out
 
     def free(self):
-        if self.ptr:
+        if self.ptr and self.ctx.h:
             self.ctx.L.svd_device_free(self.ctx.h, C.c_void_p(self.ptr))
-            self.ptr = 0
+        self.ptr = 0
 
     def __del__(self):
         try:
             self.free()
         except Exception:
             pass
+
+
+class Batch:
+    """Many scenes reduced to DAGs and encoded as .svdag images in shared launches (svd_batch)."""
+
+    def __init__(self, ctx: Context, codes, offsets, levels, on_device: bool = False):
+        """codes: concatenated per-scene sorted codes (numpy uint64, or a device
+        pointer with on_device=True); offsets: n_scenes + 1 starts; levels: depths."""
+        self.ctx = ctx
+        off = np.ascontiguousarray(offsets, dtype=np.uint64)
+        lev = np.ascontiguousarray(levels, dtype=np.uint32)
+        self.S = lev.size
+        h = C.c_void_p()
+        if on_device:
+            src = C.c_void_p(codes)
+        else:
+            self._codes = np.ascontiguousarray(codes, dtype=np.uint64)
+            src = self._codes.ctypes.data_as(C.c_void_p)
+        ctx.check(ctx.L.svd_batch_build(ctx.h, self.S, src, off.ctypes.data_as(P_U64), lev.ctypes.data_as(P_U32),
+                                        int(on_device), C.byref(h)))
+        self.h = h
+        self._codes = None
+
+    @classmethod
+    def from_device_scenes(cls, ctx: Context, scenes, levels):
+        """scenes: list of DeviceCodes; packs them into one device buffer first."""
+        import torch
+        sizes = [d.n for d in scenes]
+        off = np.concatenate([[0], np.cumsum(sizes)]).astype(np.uint64)
+        buf = torch.empty(int(off[-1]), dtype=torch.int64, device="cuda:0")
+        for d, o in zip(scenes, off[:-1]):
+            ctx.check(ctx.L.svd_device_copy(ctx.h, C.c_void_p(buf.data_ptr() + 8 * int(o)), C.c_void_p(d.ptr),
+                                            8 * d.n))
+        b = cls(ctx, buf.data_ptr(), off, levels, on_device=True)
+        del buf
+        return b
+
+    def info(self, s: int):
+        L, nv, nn, nb = C.c_uint32(), C.c_uint64(), C.c_uint64(), C.c_uint64()
+        self.ctx.check(self.ctx.L.svd_batch_scene_info(self.h, s, C.byref(L), C.byref(nv), C.byref(nn), C.byref(nb)))
+        return dict(levels=L.value, voxels=nv.value, nodes=nn.value, bytes=nb.value)
+
+    def svdag_bytes(self, s: int) -> bytes:
+        n = self.info(s)["bytes"]
+        buf = np.zeros(n, np.uint8)
+        self.ctx.check(self.ctx.L.svd_batch_svdag_copy(self.h, s, buf.ctypes.data_as(C.c_void_p), n))
+        return buf.tobytes()
+
+    def trace(self, cams, cam_scene, prim_ptr: int, shad_ptr: int | None):
+        """cams: list of TraceParams (one resolution); results camera-major in device buffers."""
+        arr = (TraceParams * len(cams))(*cams)
+        cs = np.ascontiguousarray(cam_scene, dtype=np.uint32)
+        self.ctx.check(self.ctx.L.svd_batch_trace(self.h, len(cams), C.cast(arr, C.c_void_p),
+                                                  cs.ctypes.data_as(P_U32), C.c_void_p(prim_ptr),
+                                                  C.c_void_p(shad_ptr) if shad_ptr else None))
+
+    def free(self):
+        if getattr(self, "h", None) and self.ctx.h:      # a closed context released everything
+            self.ctx.L.svd_batch_free(self.h)
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+P_U64 = C.POINTER(C.c_uint64)
+P_U32 = C.POINTER(C.c_uint32)
 
 
 class Octree:
@@ -295,9 +405,9 @@ class Octree:
         return cls(ctx, h)
 
     def free(self):
-        if self.h:
+        if self.h and self.ctx.h:
             self.ctx.L.svd_octree_free(self.h)
-            self.h = None
+        self.h = None
 
     def __del__(self):
         try:
@@ -405,9 +515,9 @@ class MLP:
         self.h = h
 
     def free(self):
-        if getattr(self, "h", None):
+        if getattr(self, "h", None) and self.ctx.h:
             self.ctx.L.svd_mlp_free(self.h)
-            self.h = None
+        self.h = None
 
     def __del__(self):
         try:
@@ -462,9 +572,9 @@ class Scene:
         return cls(octree.ctx, h, fmt, keep=octree)
 
     def free(self):
-        if self.h:
+        if self.h and self.ctx.h:
             self.ctx.L.svd_scene_free(self.h)
-            self.h = None
+        self.h = None
 
     def __del__(self):
         try:

@@ -644,3 +644,118 @@ int svd_device_read(svd_ctx* c, void* dst, const void* src, uint64_t bytes) {
     SVD_HIP(hipStreamSynchronize(c->stream));
   });
 }
+
+// ---- batches -------------------------------------------------------------------
+int svd_batch_build(svd_ctx* c, uint32_t S, const uint64_t* codes, const uint64_t* offsets, const uint32_t* levels,
+                    int on_device, svd_batch** out) {
+  if (!c || !offsets || !levels || !out || S == 0 || S > (1u << 20)) return SVD_ERR_ARG;
+  *out = nullptr;
+  const uint64_t n = offsets[S];
+  if (n && !codes) return SVD_ERR_ARG;
+  svd_batch* b = new svd_batch();
+  b->ctx = c;
+  b->S = S;
+  const int rc = guarded(c, [&] {
+    Timer t(c, "batch_build");
+    b->L.assign(levels, levels + S);
+    b->voxels.resize(S);
+    std::vector<uint64_t> off(offsets, offsets + S + 1);
+    if (off[0] != 0) throw Error{SVD_ERR_ARG, "offsets[0] must be 0"};
+    for (uint32_t s = 0; s < S; ++s) {
+      if (levels[s] < 2 || levels[s] > 13) throw Error{SVD_ERR_LEVELS, "batched scenes need 2..13 levels"};
+      if (off[s + 1] <= off[s]) throw Error{SVD_ERR_ARG, "every batched scene needs at least one voxel"};
+      b->voxels[s] = off[s + 1] - off[s];
+      b->Lmax = std::max(b->Lmax, levels[s]);
+    }
+    const uint64_t* d = codes;
+    uint64_t* tmp = nullptr;
+    if (!on_device) {
+      tmp = dalloc<uint64_t>(c, n);
+      SVD_HIP(hipMemcpyAsync(tmp, codes, 8 * n, hipMemcpyHostToDevice, c->stream));
+      d = tmp;
+    }
+    batch_build(b, d, n, off);
+    dfree(c, tmp);
+  });
+  if (rc != SVD_OK) { (void)svd_batch_free(b); return rc; }
+  *out = b;
+  return SVD_OK;
+}
+
+int svd_batch_free(svd_batch* b) {
+  if (!b) return SVD_ERR_ARG;
+  const int rc = guarded(b->ctx, [&] {
+    for (auto& lv : b->dag) dfree(b->ctx, lv.rec);
+    dfree(b->ctx, b->files);
+  });
+  delete b;
+  return rc;
+}
+
+int svd_batch_scene_info(svd_batch* b, uint32_t s, uint32_t* levels, uint64_t* nv, uint64_t* nn, uint64_t* bytes) {
+  if (!b || s >= b->S) return SVD_ERR_ARG;
+  if (levels) *levels = b->L[s];
+  if (nv) *nv = b->voxels[s];
+  if (nn) *nn = b->nodes[s];
+  if (bytes) *bytes = b->fileBytes[s];
+  return SVD_OK;
+}
+
+int svd_batch_svdag(svd_batch* b, uint32_t s, const void** dptr, uint64_t* bytes) {
+  if (!b || s >= b->S || !dptr) return SVD_ERR_ARG;
+  *dptr = b->files + b->fileOff[s];
+  if (bytes) *bytes = b->fileBytes[s];
+  return SVD_OK;
+}
+
+int svd_batch_svdag_copy(svd_batch* b, uint32_t s, void* dst, uint64_t cap) {
+  if (!b || s >= b->S || !dst || cap < b->fileBytes[s]) return SVD_ERR_ARG;
+  return guarded(b->ctx, [&] {
+    SVD_HIP(hipMemcpyAsync(dst, b->files + b->fileOff[s], b->fileBytes[s], hipMemcpyDeviceToHost, b->ctx->stream));
+    SVD_HIP(hipStreamSynchronize(b->ctx->stream));
+  });
+}
+
+int svd_batch_trace(svd_batch* b, uint32_t n_cams, const svd_trace_params* cams, const uint32_t* cam_scene,
+                    svd_ray_result* prim, svd_ray_result* shad) {
+  if (!b || (n_cams && (!cams || !cam_scene || !prim))) return SVD_ERR_ARG;
+  return guarded(b->ctx, [&] {
+    Timer t(b->ctx, "trace");
+    std::vector<svd_trace_params> cv(cams, cams + n_cams);
+    std::vector<uint32_t> cs(cam_scene, cam_scene + n_cams);
+    trace_batch_svdag(b->ctx, b->scenes, cv, cs, prim, shad);
+  });
+}
+
+int svd_workload_balls(svd_ctx* c, uint32_t levels, const double* balls, uint32_t nb, uint64_t** out, uint64_t* n) {
+  if (!c || !out || !n || (nb && !balls)) return SVD_ERR_ARG;
+  return guarded(c, [&] {
+    Timer t(c, "workload");
+    *n = gen_balls(c, levels, balls, nb, out);
+  });
+}
+
+int svd_workload_menger(svd_ctx* c, uint32_t levels, uint32_t order, uint64_t** out, uint64_t* n) {
+  if (!c || !out || !n) return SVD_ERR_ARG;
+  return guarded(c, [&] {
+    Timer t(c, "workload");
+    *n = gen_menger(c, levels, order, out);
+  });
+}
+
+int svd_workload_terrain(svd_ctx* c, uint32_t levels, uint64_t seed, uint32_t octaves, double hmin, double hmax,
+                         double base_cells, uint64_t** out, uint64_t* n) {
+  if (!c || !out || !n || octaves < 1 || octaves > 16) return SVD_ERR_ARG;
+  return guarded(c, [&] {
+    Timer t(c, "workload");
+    *n = gen_terrain(c, levels, seed, octaves, hmin, hmax, base_cells, out);
+  });
+}
+
+int svd_device_copy(svd_ctx* c, void* dst, const void* src, uint64_t bytes) {
+  if (!c || (bytes && (!dst || !src))) return SVD_ERR_ARG;
+  return guarded(c, [&] {
+    if (bytes) SVD_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, c->stream));
+    SVD_HIP(hipStreamSynchronize(c->stream));
+  });
+}

@@ -448,49 +448,73 @@ void build_svo_from_morton(svd_octree* o, const uint64_t* dcodes, uint64_t n) {
     SVD_HIP(hipStreamSynchronize(c->stream));
     return;
   }
-  uint32_t* scanbuf = dalloc<uint32_t>(c, n + 1);
   uint32_t* total = dalloc<uint32_t>(c, 1);
   fill_u32(c, total, 0u, 1);
   k_check_codes<<<div_up(n, kB), kB, 0, c->stream>>>(dcodes, n, 1ull << (3 * L), total);
   check_launch("k_check_codes");
-  if (read_u32(c, total) != 0) {
-    dfree(c, scanbuf); dfree(c, total);
-    throw Error{SVD_ERR_ARG, "Morton codes must be strictly ascending and below 2^(3*levels)"};
-  }
+  const uint64_t bad = read_u32(c, total);
+  dfree(c, total);
+  if (bad) throw Error{SVD_ERR_ARG, "Morton codes must be strictly ascending and below 2^(3*levels)"};
+  const uint64_t nNodes = svo_levels_from_codes(c, dcodes, n, L, o->svo, nullptr);
+  o->nNodesSVO = nNodes;   // non-root nodes created (GeomOctree::_nNodes after buildSVO)
+  o->state = SVD_S_SVO;
+}
+
+// SVO levels from strictly ascending codes (the caller checked them).  Node
+// order per level = order of first arrival.  levelCodes (optional) receives,
+// per level, a device copy of every node's code (code >> 3*(L-lev)).
+// Returns the number of non-root nodes.
+uint64_t svo_levels_from_codes(svd_ctx* c, const uint64_t* dcodes, uint64_t n, uint32_t L, std::vector<Level>& svo,
+                               std::vector<uint64_t*>* levelCodes) {
+  svo.assign(L, Level());
+  if (levelCodes) levelCodes->assign(L, nullptr);
+  uint32_t* scanbuf = dalloc<uint32_t>(c, n + 1);
+  uint32_t* total = dalloc<uint32_t>(c, 1);
   uint64_t* codeA = dalloc<uint64_t>(c, n);
   uint64_t* codeB = dalloc<uint64_t>(c, n);
+  auto keep = [&](uint32_t lev, const uint64_t* codes, uint64_t m) {
+    if (!levelCodes) return;
+    (*levelCodes)[lev] = dalloc<uint64_t>(c, m + 1);
+    if (m) SVD_HIP(hipMemcpyAsync((*levelCodes)[lev], codes, m * 8, hipMemcpyDeviceToDevice, c->stream));
+  };
   // finest level
   k_heads<<<div_up(n, kB), kB, 0, c->stream>>>(dcodes, n, scanbuf);
   scan_exclusive(c, scanbuf, scanbuf, n, total);
   uint64_t nl = read_u32(c, total);
-  Level& leaf = o->svo[L - 1];
+  Level& leaf = svo[L - 1];
   leaf.leaf = true;
   leaf.n = nl;
   leaf.rec = dalloc<uint32_t>(c, nl);
   k_leaf_level<<<div_up(n, kB), kB, 0, c->stream>>>(dcodes, n, scanbuf, leaf.rec, codeA);
   check_launch("k_leaf_level");
+  keep(L - 1, codeA, nl);
   uint64_t nNodes = (L >= 2) ? nl : 0;
   uint64_t nc = nl;
   for (int lev = (int)L - 2; lev >= 0; --lev) {
     k_heads<<<div_up(nc, kB), kB, 0, c->stream>>>(codeA, nc, scanbuf);
     scan_exclusive(c, scanbuf, scanbuf, nc, total);
     uint64_t np = read_u32(c, total);
-    Level& lv = o->svo[lev];
+    Level& lv = svo[lev];
     lv.n = np;
     lv.rec = dalloc<uint32_t>(c, np * kRecWords);
     k_inner_level<<<div_up(nc, kB), kB, 0, c->stream>>>(codeA, nc, scanbuf, lv.rec, codeB);
     check_launch("k_inner_level");
+    keep((uint32_t)lev, codeB, np);
     std::swap(codeA, codeB);
     nc = np;
     if (lev > 0) nNodes += np;
   }
-  o->nNodesSVO = nNodes;   // non-root nodes created (GeomOctree::_nNodes after buildSVO)
   dfree(c, scanbuf); dfree(c, total); dfree(c, codeA); dfree(c, codeB);
-  o->state = SVD_S_SVO;
+  return nNodes;
+}
+
+void remap_children(svd_ctx* c, const uint32_t* in, uint64_t n, const uint32_t* remap, uint32_t* out) {
+  if (n) k_remap_children<<<div_up(n, kB), kB, 0, c->stream>>>(in, n, remap, out);
+  check_launch("k_remap_children");
 }
 
 // Generic dedup of one inner level's records -> remap + compacted records.
-static uint64_t dedup_level(svd_ctx* c, const uint32_t* rec, uint64_t n, uint32_t* remap, uint32_t** outRec) {
+uint64_t dedup_level(svd_ctx* c, const uint32_t* rec, uint64_t n, uint32_t* remap, uint32_t** outRec) {
   const uint32_t cap = table_cap(n);
   uint32_t* table = dalloc<uint32_t>(c, cap);
   uint32_t* slot = dalloc<uint32_t>(c, n);

@@ -109,6 +109,19 @@ struct svd_scene {
   uint32_t nLevelOffsets = 0;
 };
 
+// Many scenes built together (batch.hip): the shared DAG levels and the
+// SVDAG image of every scene in one device buffer.
+struct svd_batch {
+  svd_ctx* ctx = nullptr;
+  uint32_t S = 0, Lmax = 0;
+  std::vector<uint32_t> L;                     // depth of every scene
+  std::vector<uint64_t> voxels, nodes;         // per scene
+  std::vector<uint64_t> fileOff, fileBytes;    // SVDAG image of scene s at files + fileOff[s]
+  std::vector<svd::Level> dag;                 // batched DAG levels (scene-major)
+  uint8_t* files = nullptr;
+  std::vector<svd_scene> scenes;               // traversal views of the images
+};
+
 namespace svd {
 
 // ---- error helpers ----------------------------------------------------------
@@ -146,6 +159,11 @@ void fill_u8(svd_ctx* c, uint8_t* d, uint8_t v, uint64_t n);
 
 // ---- build / reduction (build.hip) -----------------------------------------
 void build_svo_from_morton(svd_octree* o, const uint64_t* dcodes, uint64_t n);
+uint64_t svo_levels_from_codes(svd_ctx* c, const uint64_t* dcodes, uint64_t n, uint32_t L, std::vector<Level>& svo,
+                               std::vector<uint64_t*>* levelCodes);
+// one inner level: exact-key dedup, first occurrence wins -> remap + compacted records
+uint64_t dedup_level(svd_ctx* c, const uint32_t* rec, uint64_t n, uint32_t* remap, uint32_t** outRec);
+void remap_children(svd_ctx* c, const uint32_t* in, uint64_t n, const uint32_t* remap, uint32_t* out);
 void reduce_dag(svd_octree* o);
 uint64_t dedup_all(svd_ctx* c, uint32_t L, const std::vector<Level>& src, std::vector<Level>& dst);
 void reduce_sym(svd_octree* o);
@@ -181,9 +199,18 @@ void mlp_train(svd_mlp* m, svd_octree* o, const svd_mlp_train_params* p, svd_mlp
 void knn_lsh(svd_ctx* c, const float* emb, uint64_t n, uint32_t k, uint32_t tables, uint32_t bits, uint32_t window,
              uint64_t seed, uint32_t* out);
 
+// ---- batches (batch.hip, trace.hip) ---------------------------------------------
+void batch_build(svd_batch* b, const uint64_t* dcodes, uint64_t n, const std::vector<uint64_t>& off);
+void trace_batch_svdag(svd_ctx* c, const std::vector<svd_scene>& scenes, const std::vector<svd_trace_params>& cams,
+                       const std::vector<uint32_t>& camScene, svd_ray_result* prim, svd_ray_result* shad);
+
 // ---- synthetic scenes (workload.hip) ---------------------------------------------
 uint64_t gen_caves(svd_ctx* c, uint32_t levels, uint64_t seed, uint32_t octaves, double threshold, double baseCells,
                    uint64_t** out);
+uint64_t gen_balls(svd_ctx* c, uint32_t levels, const double* balls, uint32_t nBalls, uint64_t** out);
+uint64_t gen_menger(svd_ctx* c, uint32_t levels, uint32_t order, uint64_t** out);
+uint64_t gen_terrain(svd_ctx* c, uint32_t levels, uint64_t seed, uint32_t octaves, double hmin, double hmax,
+                     double baseCells, uint64_t** out);
 
 // ---- timing helpers ------------------------------------------------------------
 struct Timer {

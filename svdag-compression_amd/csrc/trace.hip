@@ -321,15 +321,13 @@ __device__ __forceinline__ void normalize3(float v[3]) {
   v[0] = v[0] / l; v[1] = v[1] / l; v[2] = v[2] / l;
 }
 
+// One pixel of one camera: primary ray, then the shadow ray of a hit.
 template <int FMT>
-__global__ void __launch_bounds__(kBT) k_trace(Dev s, svd_trace_params p, svd_ray_result* __restrict__ prim,
-                                               svd_ray_result* __restrict__ shad) {
-  extern __shared__ uint32_t stackMem[];
+__device__ __forceinline__ void trace_pixel(const Dev& s, const svd_trace_params& p, uint32_t px, uint32_t py,
+                                            uint32_t* stackMem, svd_ray_result* __restrict__ prim,
+                                            svd_ray_result* __restrict__ shad) {
   const int depth = s.innerLevels + 1;
   const int lane = threadIdx.y * kTW + threadIdx.x;
-  const uint32_t px = blockIdx.x * kTW + threadIdx.x;
-  const uint32_t py = blockIdx.y * kTH + threadIdx.y;
-  if (px >= p.width || py >= p.height) return;
   Stack stk{stackMem, stackMem + depth * kBT, stackMem + 2 * depth * kBT, lane, 0};
   Tracer<FMT> tr(s, stk);
   // computeCameraRay (octree_dda.frag.glsl:587-603)
@@ -381,6 +379,34 @@ __global__ void __launch_bounds__(kBT) k_trace(Dev s, svd_trace_params p, svd_ra
   shad[k] = sr;
 }
 
+template <int FMT>
+__global__ void __launch_bounds__(kBT) k_trace(Dev s, svd_trace_params p, svd_ray_result* __restrict__ prim,
+                                               svd_ray_result* __restrict__ shad) {
+  extern __shared__ uint32_t stackMem[];
+  const uint32_t px = blockIdx.x * kTW + threadIdx.x;
+  const uint32_t py = blockIdx.y * kTH + threadIdx.y;
+  if (px >= p.width || py >= p.height) return;
+  trace_pixel<FMT>(s, p, px, py, stackMem, prim, shad);
+}
+
+// Many cameras over many scenes in one launch: blockIdx.z is the camera;
+// every camera has the same resolution, results are [camera][row][column].
+template <int FMT>
+__global__ void __launch_bounds__(kBT) k_trace_batch(const Dev* __restrict__ scenes,
+                                                     const svd_trace_params* __restrict__ cams,
+                                                     const uint32_t* __restrict__ camScene,
+                                                     svd_ray_result* __restrict__ prim, svd_ray_result* __restrict__ shad) {
+  extern __shared__ uint32_t stackMem[];
+  const uint32_t cam = blockIdx.z;
+  const svd_trace_params p = cams[cam];
+  const uint32_t px = blockIdx.x * kTW + threadIdx.x;
+  const uint32_t py = blockIdx.y * kTH + threadIdx.y;
+  if (px >= p.width || py >= p.height) return;
+  const Dev s = scenes[camScene[cam]];
+  const size_t off = (size_t)cam * p.width * p.height;
+  trace_pixel<FMT>(s, p, px, py, stackMem, prim + off, shad ? shad + off : nullptr);
+}
+
 }  // namespace
 
 void trace_frame(svd_scene* sc, const svd_trace_params* p, svd_ray_result* prim, svd_ray_result* shad) {
@@ -408,6 +434,55 @@ void trace_frame(svd_scene* sc, const svd_trace_params* p, svd_ray_result* prim,
     default: k_trace<SVD_FMT_SSVDAG><<<grid, block, shmem, c->stream>>>(d, *p, prim, shad); break;
   }
   check_launch("k_trace");
+}
+
+Dev scene_dev(const svd_scene* sc) {
+  Dev d{};
+  d.nodes = sc->nodes;
+  d.inner = sc->inner;
+  d.leaves = (const uint2*)sc->leaves;
+  d.levelOffsets = sc->levelOffsets;
+  d.nWords = sc->nWords;
+  d.nInner = sc->nInner;
+  d.nLeaves = sc->nLeaves;
+  if (sc->fmt == SVD_FMT_SSVDAG || sc->fmt == SVD_FMT_ESVDAG) { d.innerLevels = (int)sc->levels - 2; d.leafSize = 4; }
+  else { d.innerLevels = (int)sc->levels - 1; d.leafSize = 2; }
+  for (int a = 0; a < 3; ++a) { d.bmin[a] = sc->bmin[a]; d.bmax[a] = sc->bmax[a]; d.center[a] = sc->center[a]; }
+  d.rootHalfSide = sc->rootHalfSide;
+  return d;
+}
+
+// SVDAG scenes (nodes, nWords, levels, box) x cameras in one launch.
+void trace_batch_svdag(svd_ctx* c, const std::vector<svd_scene>& scenes, const std::vector<svd_trace_params>& cams,
+                       const std::vector<uint32_t>& camScene, svd_ray_result* prim, svd_ray_result* shad) {
+  if (cams.empty()) return;
+  const uint32_t W = cams[0].width, H = cams[0].height;
+  int maxInner = 1;
+  std::vector<Dev> devs(scenes.size());
+  for (size_t i = 0; i < scenes.size(); ++i) {
+    if (scenes[i].fmt != SVD_FMT_SVDAG) throw Error{SVD_ERR_ARG, "batched trace: SVDAG scenes only"};
+    devs[i] = scene_dev(&scenes[i]);
+    maxInner = devs[i].innerLevels > maxInner ? devs[i].innerLevels : maxInner;
+  }
+  for (size_t k = 0; k < cams.size(); ++k) {
+    if (cams[k].width != W || cams[k].height != H) throw Error{SVD_ERR_ARG, "batched trace: one resolution for all cameras"};
+    if (cams[k].draw_level == 0) throw Error{SVD_ERR_ARG, "draw_level must be >= 1"};
+    if (camScene[k] >= scenes.size()) throw Error{SVD_ERR_ARG, "camera scene index out of range"};
+  }
+  if (W == 0 || H == 0) return;
+  Dev* dDev = dalloc<Dev>(c, devs.size());
+  svd_trace_params* dCam = dalloc<svd_trace_params>(c, cams.size());
+  uint32_t* dMap = dalloc<uint32_t>(c, camScene.size());
+  SVD_HIP(hipMemcpyAsync(dDev, devs.data(), devs.size() * sizeof(Dev), hipMemcpyHostToDevice, c->stream));
+  SVD_HIP(hipMemcpyAsync(dCam, cams.data(), cams.size() * sizeof(svd_trace_params), hipMemcpyHostToDevice, c->stream));
+  SVD_HIP(hipMemcpyAsync(dMap, camScene.data(), camScene.size() * 4, hipMemcpyHostToDevice, c->stream));
+  const dim3 grid(div_up(W, kTW), div_up(H, kTH), (uint32_t)cams.size());
+  const dim3 block(kTW, kTH);
+  const size_t shmem = (size_t)3 * (maxInner + 1) * kBT * sizeof(uint32_t);
+  k_trace_batch<SVD_FMT_SVDAG><<<grid, block, shmem, c->stream>>>(dDev, dCam, dMap, prim, shad);
+  check_launch("k_trace_batch");
+  SVD_HIP(hipStreamSynchronize(c->stream));
+  dfree(c, dDev); dfree(c, dCam); dfree(c, dMap);
 }
 
 }  // namespace svd

@@ -289,6 +289,75 @@ def menger_codes(levels: int, order: int) -> np.ndarray:
     return sorted_codes(x, y, z)
 
 
+def _surface(solid: np.ndarray) -> np.ndarray:
+    """Voxels of a boolean grid with an empty 6-neighbour (outside is empty)."""
+    P = np.pad(solid, 1)
+    full = (P[:-2, 1:-1, 1:-1] & P[2:, 1:-1, 1:-1] & P[1:-1, :-2, 1:-1] & P[1:-1, 2:, 1:-1]
+            & P[1:-1, 1:-1, :-2] & P[1:-1, 1:-1, 2:])
+    return solid & ~full
+
+
+def menger_surface_codes(levels: int, order: int) -> np.ndarray:
+    """Surface voxels of the Menger sponge of the given order filling the 2^levels grid."""
+    n = 1 << levels
+    c = np.arange(n)
+    X, Y, Z = np.meshgrid(c, c, c, indexing="ij")
+    solid = np.ones((n, n, n), dtype=bool)
+    for k in range(order):
+        s = 3 ** (k + 1)
+        mx = ((X * s) // n) % 3 == 1
+        my = ((Y * s) // n) % 3 == 1
+        mz = ((Z * s) // n) % 3 == 1
+        solid &= ~((mx & my) | (my & mz) | (mx & mz))
+    return sorted_codes(*np.nonzero(_surface(solid)))
+
+
+def sphere_balls(n_spheres: int, seed: int, rmin: float = 0.02, rmax: float = 0.15) -> np.ndarray:
+    """(cx, cy, cz, r) of the icosphere scene's spheres (the same draws as spheres_mesh)."""
+    u = uniforms(seed, 4 * n_spheres).reshape(n_spheres, 4)
+    return np.column_stack([0.1 + 0.8 * u[:, :3], rmin + (rmax - rmin) * u[:, 3]]).astype(np.float64)
+
+
+def balls_codes(levels: int, balls: np.ndarray) -> np.ndarray:
+    """Surface voxels of a union of balls, voxel centres p = (i + 0.5) / 2^levels."""
+    n = 1 << levels
+    c = (np.arange(n, dtype=np.float64) + 0.5) / n
+    X, Y, Z = np.meshgrid(c, c, c, indexing="ij")
+    solid = np.zeros((n, n, n), dtype=bool)
+    for cx, cy, cz, r in balls:
+        dx, dy, dz = X - cx, Y - cy, Z - cz
+        solid |= dx * dx + dy * dy + dz * dz <= r * r
+    return sorted_codes(*np.nonzero(_surface(solid)))
+
+
+def batch_specs(n_scenes: int = 256, seed: int = 0, depth=(6, 10)):
+    """The multi-scene batch: depth uniform in depth[0]..depth[1]; the kind is
+    drawn per scene: spheres (8-128 icospheres' balls), fBm terrain, or a
+    Menger sponge of order 3-5 (surface voxels)."""
+    u = uniforms(seed * 7919 + 17, 4 * n_scenes).reshape(n_scenes, 4)
+    specs = []
+    for i in range(n_scenes):
+        levels = depth[0] + int(u[i, 0] * (depth[1] - depth[0] + 1))
+        kind = ("spheres", "terrain", "menger")[int(u[i, 1] * 3)]
+        sd = seed * 100003 + i
+        if kind == "spheres":
+            specs.append(dict(kind=kind, levels=levels, seed=sd, n_spheres=8 + int(u[i, 2] * 121)))
+        elif kind == "terrain":
+            specs.append(dict(kind=kind, levels=levels, seed=sd))
+        else:
+            specs.append(dict(kind=kind, levels=levels, order=3 + int(u[i, 2] * 3)))
+    return specs
+
+
+def spec_codes(spec) -> np.ndarray:
+    """Host (NumPy) voxels of one batch scene (small depths only)."""
+    if spec["kind"] == "spheres":
+        return balls_codes(spec["levels"], sphere_balls(spec["n_spheres"], spec["seed"]))
+    if spec["kind"] == "terrain":
+        return terrain_codes(spec["levels"], seed=spec["seed"])
+    return menger_surface_codes(spec["levels"], spec["order"])
+
+
 # ---------------------------------------------------------------- cameras
 def _look_at(eye, center, up):
     f = center - eye

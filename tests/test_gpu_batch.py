@@ -1,0 +1,112 @@
+"""Batched multi-scene build + multi-camera traversal, and the device scene
+generators, against the oracle / the NumPy definitions / single-scene runs."""
+import numpy as np
+import pytest
+import torch
+
+import oracle as O
+
+import svdag_amd as S
+import svdag_amd.workloads as W
+
+pytestmark = pytest.mark.gpu
+
+DEV = "cuda:0"
+
+
+@pytest.mark.parametrize("kind,levels,arg", [("balls", 6, 20), ("balls", 5, 8), ("menger", 5, 3), ("menger", 6, 4),
+                                             ("terrain", 6, 1), ("terrain", 7, 5)])
+def test_generators_match_numpy(ctx, kind, levels, arg):
+    if kind == "balls":
+        b = W.sphere_balls(arg, 3)
+        d, want = ctx.balls(levels, b), W.balls_codes(levels, b)
+    elif kind == "menger":
+        d, want = ctx.menger(levels, arg), W.menger_surface_codes(levels, arg)
+    else:
+        d, want = ctx.terrain(levels, seed=arg), W.terrain_codes(levels, seed=arg)
+    got = d.numpy()
+    d.free()
+    assert want.size > 0
+    np.testing.assert_array_equal(got, want)
+
+
+def _scenes(n, seed, depth):
+    specs = W.batch_specs(n, seed, depth)
+    return specs, [W.spec_codes(s) for s in specs]
+
+
+def _pack(codes):
+    off = np.concatenate([[0], np.cumsum([c.size for c in codes])]).astype(np.uint64)
+    return np.concatenate(codes).astype(np.uint64), off
+
+
+def test_batch_build_equals_oracle(ctx):
+    specs, codes = _scenes(24, 1, (3, 6))
+    # uneven extremes: a 2-level scene and a single-voxel scene
+    codes += [W.sorted_codes(np.array([0, 3]), np.array([1, 2]), np.array([3, 0])),
+              W.sorted_codes(np.array([5]), np.array([9]), np.array([2]))]
+    levels = [s["levels"] for s in specs] + [2, 5]
+    allc, off = _pack(codes)
+    print(f"[case] batch of {len(codes)} scenes, {allc.size} voxels, depths {sorted(set(levels))}", flush=True)
+    b = S.Batch(ctx, allc, off, levels)
+    for s, (c, L) in enumerate(zip(codes, levels)):
+        o = O.OracleOctree.from_morton(c, L)
+        o.to_dag()
+        o.init_child_levels()
+        info = b.info(s)
+        assert info["nodes"] == o.n_nodes and info["voxels"] == c.size and info["levels"] == L
+        assert b.svdag_bytes(s) == o.encode(0), f"scene {s} ({L} levels)"
+
+
+def test_batch_device_scenes_equal_single_builds(ctx):
+    specs = W.batch_specs(16, 2, (6, 8))
+    dev = [ctx.spec_codes(sp) for sp in specs]
+    levels = [sp["levels"] for sp in specs]
+    b = S.Batch.from_device_scenes(ctx, dev, levels)
+    for s, (d, L) in enumerate(zip(dev, levels)):
+        g = S.Octree.from_morton(ctx, (d.ptr, d.n), L)
+        g.to_dag()
+        g.encode(S.FMT_SVDAG)
+        assert b.svdag_bytes(s) == g.encoded_bytes(S.FMT_SVDAG), f"scene {s} {specs[s]}"
+        g.free()
+    for d in dev:
+        d.free()
+
+
+def test_batch_trace_equals_single_traces(ctx):
+    specs, codes = _scenes(6, 3, (5, 7))
+    levels = [s["levels"] for s in specs]
+    allc, off = _pack(codes)
+    b = S.Batch(ctx, allc, off, levels)
+    W_, H_ = 96, 64
+    cams, cam_scene = [], []
+    for k in range(10):
+        s = k % len(codes)
+        cams.append(W.trace_params(W_, H_, seed=40 + k, levels=levels[s], shadows=True))
+        cam_scene.append(s)
+    n = W_ * H_
+    prim = torch.zeros(len(cams) * n * 6, dtype=torch.int32, device=DEV)
+    shad = torch.zeros(len(cams) * n * 6, dtype=torch.int32, device=DEV)
+    b.trace(cams, cam_scene, prim.data_ptr(), shad.data_ptr())
+    pr = prim.cpu().numpy().view(S.RAY_DTYPE).reshape(len(cams), n)
+    sh = shad.cpu().numpy().view(S.RAY_DTYPE).reshape(len(cams), n)
+    one_p = torch.zeros(n * 6, dtype=torch.int32, device=DEV)
+    one_s = torch.zeros(n * 6, dtype=torch.int32, device=DEV)
+    for k, s in enumerate(cam_scene):
+        sc = S.Scene.from_bytes(ctx, S.FMT_SVDAG, b.svdag_bytes(s))
+        sc.trace_device(cams[k], one_p.data_ptr(), one_s.data_ptr())
+        ctx.sync()
+        np.testing.assert_array_equal(pr[k], one_p.cpu().numpy().view(S.RAY_DTYPE), err_msg=f"camera {k}")
+        np.testing.assert_array_equal(sh[k], one_s.cpu().numpy().view(S.RAY_DTYPE), err_msg=f"camera {k}")
+        sc.free()
+    assert (pr["t"] >= 0).mean() > 0.05
+
+
+def test_batch_rejects_bad_input(ctx):
+    c = W.terrain_codes(5, seed=1)
+    with pytest.raises(S.SvdError):                     # an empty scene
+        S.Batch(ctx, c, np.array([0, c.size, c.size], np.uint64), [5, 5])
+    with pytest.raises(S.SvdError):                     # codes beyond the scene's depth
+        S.Batch(ctx, c, np.array([0, c.size], np.uint64), [4])
+    with pytest.raises(S.SvdError):                     # not ascending within a scene
+        S.Batch(ctx, c[::-1].copy(), np.array([0, c.size], np.uint64), [5])
