@@ -218,7 +218,7 @@ typedef struct svd_mlp_train_params {
   uint32_t pairs_per_step;     /* P (multiple of 16)                           */
   uint32_t steps;
   float lr;                    /* Adam learning rate                           */
-  uint32_t pad;
+  uint32_t near_pct;           /* percent of pairs drawn as near pairs (0 = 50) */
   uint64_t seed;               /* pair sampling seed                           */
 } svd_mlp_train_params;
 

@@ -1551,6 +1551,7 @@ static int lossyMerge(Octree& o, const LossyParams& p, const uint32_t* proposals
         for (uint32_t t = 0; t < k; ++t) {
           const uint32_t y = proposals[propOff + x * k + t];
           if (y == 0xFFFFFFFFu || y == x) continue;
+          if (y >= N) return -1;
           const size_t lo = std::min<size_t>(x, y), hi = std::max<size_t>(x, y);
           prop[hi].push_back((id_t)lo);
         }

@@ -63,7 +63,7 @@ LOSSY_STATS_DTYPE = np.dtype([(f, "<u8") for f in ("level", "bound", "nodes_befo
 
 class TrainParams(C.Structure):
     _fields_ = [("lev_lo", C.c_uint32), ("lev_hi", C.c_uint32), ("pairs_per_step", C.c_uint32),
-                ("steps", C.c_uint32), ("lr", C.c_float), ("pad", C.c_uint32), ("seed", C.c_uint64)]
+                ("steps", C.c_uint32), ("lr", C.c_float), ("near_pct", C.c_uint32), ("seed", C.c_uint64)]
 
 
 class TrainStats(C.Structure):
@@ -545,8 +545,8 @@ class MLP:
         return loss.value, g
 
     def train(self, octree: Octree, lev_lo: int, lev_hi: int, pairs_per_step: int = 2048, steps: int = 1024,
-              lr: float = 1e-3, seed: int = 11) -> TrainStats:
-        p = TrainParams(lev_lo, lev_hi, pairs_per_step, steps, lr, 0, seed)
+              lr: float = 1e-3, seed: int = 11, near_pct: int = 50) -> TrainStats:
+        p = TrainParams(lev_lo, lev_hi, pairs_per_step, steps, lr, near_pct, seed)
         st = TrainStats()
         self.ctx.check(self.ctx.L.svd_mlp_train(self.h, octree.h, C.byref(p), C.byref(st)))
         return st

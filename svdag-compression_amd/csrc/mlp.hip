@@ -216,11 +216,12 @@ __global__ void k_inverse_perm(const uint32_t* __restrict__ order, uint64_t n, u
   if (q < n) pos[order[q]] = (uint32_t)q;
 }
 
-// Batch of P pairs: even pairs random, odd pairs neighbours (+-1..8) in the
-// near-pair ordering.  Rows 2p, 2p+1 get the two signatures; label = Hamming.
+// Batch of P pairs: near_pct percent are neighbours (+-1..8) in the near-pair
+// ordering, the rest random.  Rows 2p, 2p+1 get the two signatures; label =
+// Hamming distance.
 __global__ void k_sample_batch(const uint32_t* __restrict__ sig, uint64_t n, const uint32_t* __restrict__ order,
                                const uint32_t* __restrict__ pos, uint64_t seed, const uint32_t* __restrict__ step,
-                               uint64_t P, uint32_t* __restrict__ X, uint32_t* __restrict__ labels) {
+                               uint64_t P, uint32_t nearPct, uint32_t* __restrict__ X, uint32_t* __restrict__ labels) {
   const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= P) return;
   const uint64_t gidx = (uint64_t)(*step) * P + p;
@@ -228,7 +229,7 @@ __global__ void k_sample_batch(const uint32_t* __restrict__ sig, uint64_t n, con
   const uint64_t z2 = sm_next(z1);
   const uint64_t a = z1 % n;
   uint64_t b;
-  if ((p & 1) == 0) {
+  if ((p * 37u) % 100u >= nearPct) {
     b = z2 % n;
   } else {
     int64_t d = (int64_t)(z2 % 16) - 8;
@@ -422,6 +423,7 @@ void mlp_train(svd_mlp* m, svd_octree* o, const svd_mlp_train_params* p, svd_mlp
   if (n < 2) throw Error{SVD_ERR_ARG, "not enough nodes to sample pairs from"};
   if (n >= 0xFFFFFFFFull) throw Error{SVD_ERR_CAPACITY, "too many nodes"};
   const uint64_t P = p->pairs_per_step, R = 2 * P;
+  const uint32_t nearPct = p->near_pct ? (p->near_pct > 100 ? 100 : p->near_pct) : 50;
   uint32_t* sig = dalloc<uint32_t>(c, n * 16);
   {
     uint64_t off = 0;
@@ -475,7 +477,7 @@ void mlp_train(svd_mlp* m, svd_octree* o, const svd_mlp_train_params* p, svd_mlp
   SVD_HIP(hipStreamSynchronize(c->stream));
 
   auto enqueue_step = [&]() {
-    k_sample_batch<<<div_up(P, 256), 256, 0, c->stream>>>(sig, n, order, pos, p->seed, dstep, P, X, labels);
+    k_sample_batch<<<div_up(P, 256), 256, 0, c->stream>>>(sig, n, order, pos, p->seed, dstep, P, nearPct, X, labels);
     forward(m, X, R, w.h1, w.h2, w.e);
     k_pair_loss<<<div_up(P, 256), 256, 0, c->stream>>>(w.e, labels, P, w.dE, nullptr, dstep, lossHist);
     backward(m, X, w);

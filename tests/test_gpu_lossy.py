@@ -140,7 +140,8 @@ def test_mlp_forward_matches_oracle(ctx):
     n = 1000
     sig = _rand_sigs(n, 1)
     emb = torch.zeros(n * 32, dtype=torch.float32, device=DEV)
-    m.embed(_dev_u32(sig).data_ptr(), n, emb.data_ptr())
+    dsig = _dev_u32(sig)                 # keep the tensor alive while the library reads it
+    m.embed(dsig.data_ptr(), n, emb.data_ptr())
     e_g = emb.cpu().numpy().reshape(n, 32)
     e_o, _ = M.forward(p, M.unpack_bits(sig))
     scale = np.abs(e_o).max()
@@ -156,7 +157,9 @@ def test_mlp_gradients_match_oracle(ctx):
     sig = _rand_sigs(2 * P, 2)
     sig[1::2] = sig[0::2] ^ (_rand_sigs(P, 3, density=0.02))     # near pairs
     H = M.hamming(sig[0::2], sig[1::2])
-    loss_g, grad_g = m.loss_grad(_dev_u32(sig).data_ptr(), _dev_u32(H.astype(np.uint32)).data_ptr(), P)
+    # both tensors stay referenced: a freed temporary's block is reused by the next allocation
+    dsig, dlab = _dev_u32(sig), _dev_u32(H.astype(np.uint32))
+    loss_g, grad_g = m.loss_grad(dsig.data_ptr(), dlab.data_ptr(), P)
     loss_o, grad_o = M.loss_and_grads(p, M.unpack_bits(sig), H)
     errs = {"loss": abs(loss_g - loss_o) / abs(loss_o)}
     off = 0
@@ -165,9 +168,10 @@ def test_mlp_gradients_match_oracle(ctx):
         off += t.size
         errs[name] = np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30)
     print({k: f"{v:.2e}" for k, v in errs.items()}, flush=True)
-    # one bf16 ulp (2^-8) of an activation or delta that rounds the other way
-    # moves the loss and the gradients by well under 1% / 3%
-    assert errs["loss"] <= 1e-2
+    # the GPU accumulates in MFMA order: an activation or delta whose bf16
+    # rounding lands one ulp (2^-8) apart moves the loss by ~1e-5 and the
+    # gradients by ~1e-2 relative (near pairs cancel in e_a - e_b)
+    assert errs["loss"] <= 1e-3
     for name in ("W1", "b1", "W2", "b2", "W3", "b3"):
         assert errs[name] <= 3e-2, (name, errs[name])
 
