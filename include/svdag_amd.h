@@ -146,6 +146,18 @@ int svd_scene_free(svd_scene* sc);
  * DEVICE arrays of width*height svd_ray_result; shadow may be NULL.          */
 int svd_trace(svd_scene* sc, const svd_trace_params* p, svd_ray_result* primary, svd_ray_result* shadow);
 
+/* ---- autotuning (no reference counterpart) ----------------------------------
+ * The traversal workgroup covers a tile of tile_w x (256 / tile_w) pixels,
+ * tile_w in {8, 16, 32, 64} (default 16).  Results do not depend on it.
+ * svd_autotune_trace times every tile shape on one frame (1 warm-up + `reps`
+ * timed launches each, HIP events), keeps the fastest in the context and
+ * reports it; ms[4] (may be NULL) receives the mean kernel time per shape in
+ * the order 8, 16, 32, 64.                                                    */
+int svd_set_trace_tile(svd_ctx* ctx, uint32_t tile_w);
+int svd_get_trace_tile(svd_ctx* ctx, uint32_t* tile_w);
+int svd_autotune_trace(svd_scene* sc, const svd_trace_params* p, svd_ray_result* primary, svd_ray_result* shadow,
+                       uint32_t reps, uint32_t* best_tile_w, double* ms);
+
 /* ---- Gray-code attribute bit-tree --------------------------------------------
  * The reference lists the feature (readme.md:8) but has no code for it; the
  * encoding is the one documented with this library: per-voxel u8 attributes

@@ -153,6 +153,9 @@ def lib():
         "svd_batch_svdag_copy": (i32, [vp, u32, vp, u64]),
         "svd_batch_trace": (i32, [vp, u32, vp, P(u32), vp, vp]),
         "svd_device_copy": (i32, [vp, vp, vp, u64]),
+        "svd_set_trace_tile": (i32, [vp, u32]),
+        "svd_get_trace_tile": (i32, [vp, P(u32)]),
+        "svd_autotune_trace": (i32, [vp, P(TraceParams), vp, vp, u32, P(u32), P(C.c_double)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -172,7 +175,8 @@ EXPORTED = ["svd_ctx_create", "svd_ctx_destroy", "svd_last_error", "svd_ctx_sync
             "svd_mlp_embed", "svd_mlp_loss_grad", "svd_mlp_train", "svd_knn", "svd_workload_caves",
             "svd_device_free", "svd_device_read", "svd_workload_balls", "svd_workload_menger",
             "svd_workload_terrain", "svd_batch_build", "svd_batch_free", "svd_batch_scene_info", "svd_batch_svdag",
-            "svd_batch_svdag_copy", "svd_batch_trace", "svd_device_copy"]
+            "svd_batch_svdag_copy", "svd_batch_trace", "svd_device_copy",
+            "svd_set_trace_tile", "svd_get_trace_tile", "svd_autotune_trace"]
 
 
 class Context:
@@ -265,6 +269,48 @@ class Context:
         if spec["kind"] == "terrain":
             return self.terrain(spec["levels"], seed=spec["seed"])
         return self.menger(spec["levels"], spec["order"])
+
+    @property
+    def trace_tile(self) -> int:
+        w = C.c_uint32()
+        self.check(self.L.svd_get_trace_tile(self.h, C.byref(w)))
+        return w.value
+
+    @trace_tile.setter
+    def trace_tile(self, w: int):
+        self.check(self.L.svd_set_trace_tile(self.h, w))
+
+    def autotune_trace(self, scene: "Scene", params: TraceParams, prim_ptr: int, shad_ptr: int | None,
+                       reps: int = 3, cache: str | None = None, key: str | None = None) -> int:
+        """Pick the fastest traversal tile shape for this frame and keep it in
+        the context.  With a cache file (JSON) and a key, a stored choice is
+        reused and a new one is stored."""
+        import json
+        if cache and key and os.path.exists(cache):
+            try:
+                stored = json.load(open(cache)).get(key)
+            except (OSError, ValueError):
+                stored = None
+            if stored:
+                self.trace_tile = int(stored["tile_w"])
+                return self.trace_tile
+        best = C.c_uint32()
+        ms = (C.c_double * 4)()
+        self.check(self.L.svd_autotune_trace(scene.h, C.byref(params), C.c_void_p(prim_ptr),
+                                             C.c_void_p(shad_ptr) if shad_ptr else None, reps, C.byref(best), ms))
+        self.last_autotune = {"tile_w": best.value, "ms": {w: round(ms[i], 4) for i, w in enumerate((8, 16, 32, 64))}}
+        if cache and key:
+            data = {}
+            if os.path.exists(cache):
+                try:
+                    data = json.load(open(cache))
+                except (OSError, ValueError):
+                    data = {}
+            data[key] = self.last_autotune
+            os.makedirs(os.path.dirname(os.path.abspath(cache)), exist_ok=True)
+            with open(cache, "w") as f:
+                json.dump(data, f, indent=1)
+        return best.value
 
     def close(self):
         if getattr(self, "h", None):

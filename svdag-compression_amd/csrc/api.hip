@@ -759,3 +759,44 @@ int svd_device_copy(svd_ctx* c, void* dst, const void* src, uint64_t bytes) {
     SVD_HIP(hipStreamSynchronize(c->stream));
   });
 }
+
+// ---- autotuning --------------------------------------------------------------
+int svd_set_trace_tile(svd_ctx* c, uint32_t w) {
+  if (!c || (w != 8 && w != 16 && w != 32 && w != 64)) return SVD_ERR_ARG;
+  c->traceTileW = w;
+  return SVD_OK;
+}
+
+int svd_get_trace_tile(svd_ctx* c, uint32_t* w) {
+  if (!c || !w) return SVD_ERR_ARG;
+  *w = c->traceTileW;
+  return SVD_OK;
+}
+
+int svd_autotune_trace(svd_scene* s, const svd_trace_params* p, svd_ray_result* prim, svd_ray_result* shad,
+                       uint32_t reps, uint32_t* best, double* ms) {
+  if (!s || !p || !prim || reps < 1) return SVD_ERR_ARG;
+  svd_ctx* c = s->ctx;
+  return guarded(c, [&] {
+    const uint32_t shapes[4] = {8, 16, 32, 64};
+    double t[4];
+    const uint32_t keep = c->traceTileW;
+    for (int k = 0; k < 4; ++k) {
+      c->traceTileW = shapes[k];
+      trace_frame(s, p, prim, shad);              // warm-up
+      SVD_HIP(hipEventRecord(c->ev[0], c->stream));
+      for (uint32_t r = 0; r < reps; ++r) trace_frame(s, p, prim, shad);
+      SVD_HIP(hipEventRecord(c->ev[1], c->stream));
+      SVD_HIP(hipEventSynchronize(c->ev[1]));
+      float e = 0.f;
+      SVD_HIP(hipEventElapsedTime(&e, c->ev[0], c->ev[1]));
+      t[k] = (double)e / reps;
+    }
+    int b = 0;
+    for (int k = 1; k < 4; ++k) if (t[k] < t[b]) b = k;
+    c->traceTileW = shapes[b];
+    (void)keep;
+    if (best) *best = shapes[b];
+    if (ms) for (int k = 0; k < 4; ++k) ms[k] = t[k];
+  });
+}

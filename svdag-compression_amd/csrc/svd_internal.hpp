@@ -45,6 +45,7 @@ struct svd_ctx {
   uint64_t* pinned = nullptr;                // small pinned staging for size read-backs
   hipEvent_t ev[2] = {nullptr, nullptr};
   std::map<std::string, double> kernel_ms;   // last measured per family
+  uint32_t traceTileW = 16;                  // traversal workgroup tile: traceTileW x (256 / traceTileW) pixels
 };
 
 namespace svd {

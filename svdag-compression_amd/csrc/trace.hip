@@ -10,7 +10,7 @@ namespace svd {
 
 namespace {
 
-constexpr int kTW = 16, kTH = 16, kBT = kTW * kTH;
+constexpr int kBT = 256;                    // threads per workgroup (any tile shape of 256)
 
 struct Dev {
   const uint32_t* nodes;
@@ -327,7 +327,7 @@ __device__ __forceinline__ void trace_pixel(const Dev& s, const svd_trace_params
                                             uint32_t* stackMem, svd_ray_result* __restrict__ prim,
                                             svd_ray_result* __restrict__ shad) {
   const int depth = s.innerLevels + 1;
-  const int lane = threadIdx.y * kTW + threadIdx.x;
+  const int lane = threadIdx.y * blockDim.x + threadIdx.x;
   Stack stk{stackMem, stackMem + depth * kBT, stackMem + 2 * depth * kBT, lane, 0};
   Tracer<FMT> tr(s, stk);
   // computeCameraRay (octree_dda.frag.glsl:587-603)
@@ -383,8 +383,8 @@ template <int FMT>
 __global__ void __launch_bounds__(kBT) k_trace(Dev s, svd_trace_params p, svd_ray_result* __restrict__ prim,
                                                svd_ray_result* __restrict__ shad) {
   extern __shared__ uint32_t stackMem[];
-  const uint32_t px = blockIdx.x * kTW + threadIdx.x;
-  const uint32_t py = blockIdx.y * kTH + threadIdx.y;
+  const uint32_t px = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t py = blockIdx.y * blockDim.y + threadIdx.y;
   if (px >= p.width || py >= p.height) return;
   trace_pixel<FMT>(s, p, px, py, stackMem, prim, shad);
 }
@@ -399,8 +399,8 @@ __global__ void __launch_bounds__(kBT) k_trace_batch(const Dev* __restrict__ sce
   extern __shared__ uint32_t stackMem[];
   const uint32_t cam = blockIdx.z;
   const svd_trace_params p = cams[cam];
-  const uint32_t px = blockIdx.x * kTW + threadIdx.x;
-  const uint32_t py = blockIdx.y * kTH + threadIdx.y;
+  const uint32_t px = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t py = blockIdx.y * blockDim.y + threadIdx.y;
   if (px >= p.width || py >= p.height) return;
   const Dev s = scenes[camScene[cam]];
   const size_t off = (size_t)cam * p.width * p.height;
@@ -425,8 +425,9 @@ void trace_frame(svd_scene* sc, const svd_trace_params* p, svd_ray_result* prim,
   d.rootHalfSide = sc->rootHalfSide;
   if (p->width == 0 || p->height == 0) return;
   if (p->draw_level == 0) throw Error{SVD_ERR_ARG, "draw_level must be >= 1"};
-  const dim3 grid(div_up(p->width, kTW), div_up(p->height, kTH));
-  const dim3 block(kTW, kTH);
+  const uint32_t tw = c->traceTileW, th = kBT / tw;
+  const dim3 grid(div_up(p->width, tw), div_up(p->height, th));
+  const dim3 block(tw, th);
   const size_t shmem = (size_t)3 * (d.innerLevels + 1) * kBT * sizeof(uint32_t);
   switch (sc->fmt) {
     case SVD_FMT_SVDAG: k_trace<SVD_FMT_SVDAG><<<grid, block, shmem, c->stream>>>(d, *p, prim, shad); break;
@@ -476,8 +477,9 @@ void trace_batch_svdag(svd_ctx* c, const std::vector<svd_scene>& scenes, const s
   SVD_HIP(hipMemcpyAsync(dDev, devs.data(), devs.size() * sizeof(Dev), hipMemcpyHostToDevice, c->stream));
   SVD_HIP(hipMemcpyAsync(dCam, cams.data(), cams.size() * sizeof(svd_trace_params), hipMemcpyHostToDevice, c->stream));
   SVD_HIP(hipMemcpyAsync(dMap, camScene.data(), camScene.size() * 4, hipMemcpyHostToDevice, c->stream));
-  const dim3 grid(div_up(W, kTW), div_up(H, kTH), (uint32_t)cams.size());
-  const dim3 block(kTW, kTH);
+  const uint32_t tw = c->traceTileW, th = kBT / tw;
+  const dim3 grid(div_up(W, tw), div_up(H, th), (uint32_t)cams.size());
+  const dim3 block(tw, th);
   const size_t shmem = (size_t)3 * (maxInner + 1) * kBT * sizeof(uint32_t);
   k_trace_batch<SVD_FMT_SVDAG><<<grid, block, shmem, c->stream>>>(dDev, dCam, dMap, prim, shad);
   check_launch("k_trace_batch");
