@@ -96,12 +96,14 @@ def lossy_bench(S, ctx, levels, log):
     nodes_dag = g.counts()[0]
     m = S.MLP(ctx, seed=1)
     pairs_per_step, steps = 4096, 1024
-    tr = m.train(g, lo, hi, pairs_per_step=pairs_per_step, steps=steps, lr=2e-3, seed=5)
+    tr = m.train(g, lo, hi, pairs_per_step=pairs_per_step, steps=steps, lr=2e-3, seed=5, near_pct=80)
     log(f"[bench] trained on {tr.pairs} pairs: loss {tr.first_loss:.2f} -> {tr.last_loss:.2f} ({tr.ms_train:.0f} ms)")
     g.free()
     out = {"workload": f"caves {1 << levels}^3 ({levels} levels, seed 3, {codes.n} voxels): lossy DAG merge of "
                        f"levels {lo}-{hi}, bound {bound} voxels, nodes with <= 1 effective reference; "
-                       f"learned = MLP 512-256-128-32 embeddings + LSH kNN k=16, every candidate verified exactly",
+                       f"learned = MLP 512-256-128-32 embeddings -> LSH orderings (8 tables x 16 bits, +-64 window) "
+                       f"+ 2 neighbour-of-neighbour rounds, candidates ranked and verified exactly (XOR-popcount, "
+                       f"voxel difference)",
            "nodes_dag": int(nodes_dag),
            "training": {"pairs": int(tr.pairs), "ms": round(tr.ms_train, 1), "first_loss": round(tr.first_loss, 3),
                         "last_loss": round(tr.last_loss, 3)}}

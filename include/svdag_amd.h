@@ -197,8 +197,8 @@ typedef struct svd_lossy_params {
   float diff_factor;           /* allowedLossyDiffFactor (main.cpp:111)        */
   uint32_t proposer;           /* SVD_PROPOSER_*                               */
   uint32_t knn_k;              /* learned: neighbours per node (16)            */
-  uint32_t knn_tables, knn_bits, knn_window;  /* learned: LSH search shape (4, 16, 32) */
-  uint32_t pad;
+  uint32_t knn_tables, knn_bits, knn_window;  /* learned: LSH search shape (8, 16, 64) */
+  uint32_t knn_refine;         /* learned: neighbour-of-neighbour rounds (0..16) */
   uint64_t knn_seed;           /* learned: LSH hyperplane seed                 */
   svd_mlp* model;              /* learned: trained encoder                     */
   uint32_t* knn_record;        /* optional HOST buffer: the kNN lists used, level by level */
@@ -254,11 +254,14 @@ int svd_mlp_loss_grad(svd_mlp* m, const uint32_t* sig_dev, const uint32_t* label
 /* Train on node pairs of oct's DAG levels with exact Hamming labels (Adam);
  * the step is captured once in a HIP graph and replayed.                     */
 int svd_mlp_train(svd_mlp* m, svd_octree* oct, const svd_mlp_train_params* p, svd_mlp_train_stats* st);
-/* k nearest neighbours (squared L2) of n points of dim 32 (DEVICE), searched
- * in `tables` LSH orderings of `bits` sign bits within +-window positions;
- * out_dev: n x k indices, ascending by (distance, index), 0xFFFFFFFF-padded. */
+/* k nearest neighbours of n points of dim 32 (DEVICE), searched in `tables`
+ * LSH orderings of `bits` sign bits within +-window positions; distance =
+ * squared L2 of the embeddings, or, when sig_dev (n x 16 uint32 signatures,
+ * DEVICE) is given, the Hamming distance of the signatures (the embeddings
+ * then only decide which points meet).  out_dev: n x k indices, ascending by
+ * (distance, index), 0xFFFFFFFF-padded.                                      */
 int svd_knn(svd_ctx* ctx, const float* emb_dev, uint64_t n, uint32_t k, uint32_t tables, uint32_t bits,
-            uint32_t window, uint64_t seed, uint32_t* out_dev);
+            uint32_t window, uint64_t seed, uint32_t* out_dev, const uint32_t* sig_dev);
 
 /* ---- batches: many scenes per launch (no reference counterpart: the
  * reference builds one model per svbuilder process) ---------------------------

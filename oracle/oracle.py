@@ -73,7 +73,8 @@ def lib():
             "orc_lossy_signatures": (i32, [vp, u32, vp]),
             "orc_effective_refcounts": (i32, [vp, u32, vp]),
             "orc_subtree_diff": (u64, [vp, u32, u32, u32, u64]),
-            "orc_knn_lsh": (None, [vp, u64, u32, u32, u32, u32, u32, u64, vp]),
+            "orc_knn_lsh": (None, [vp, u64, u32, u32, u32, u32, u32, u64, vp, vp]),
+            "orc_knn_refine": (None, [vp, u64, u32, vp, vp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -262,10 +263,25 @@ def attr_bittree_decode(blob: bytes, n: int) -> np.ndarray:
     return out[:n]
 
 
-def knn_lsh(emb: np.ndarray, k: int = 16, tables: int = 4, bits: int = 16, window: int = 32, seed: int = 7):
-    """The learned proposer's LSH neighbour search, restated sequentially."""
+def knn_lsh(emb: np.ndarray, k: int = 16, tables: int = 4, bits: int = 16, window: int = 32, seed: int = 7,
+            sig: np.ndarray | None = None):
+    """The learned proposer's LSH neighbour search, restated sequentially.
+    With sig (n x 16 u32), candidates are scored by signature Hamming distance."""
     e = np.ascontiguousarray(emb, dtype=np.float32)
     n, dim = e.shape
     out = np.zeros((n, k), np.uint32)
-    lib().orc_knn_lsh(_p(e), n, dim, k, tables, bits, window, seed, _p(out))
+    s = None if sig is None else np.ascontiguousarray(sig, dtype=np.uint32)
+    lib().orc_knn_lsh(_p(e), n, dim, k, tables, bits, window, seed, _p(out), _p(s) if s is not None else None)
     return out
+
+
+def knn_refine(sig: np.ndarray, lists: np.ndarray, rounds: int = 1) -> np.ndarray:
+    """Neighbour-of-neighbour rounds on the signature Hamming metric."""
+    s = np.ascontiguousarray(sig, dtype=np.uint32)
+    cur = np.ascontiguousarray(lists, dtype=np.uint32)
+    n, k = cur.shape
+    for _ in range(rounds):
+        out = np.zeros_like(cur)
+        lib().orc_knn_refine(_p(s), n, k, _p(cur), _p(out))
+        cur = out
+    return cur

@@ -1434,8 +1434,9 @@ int orc_attr_bittree_decode(const uint8_t* f, uint64_t len, uint8_t* a, uint64_t
 //     (voxels for level L-3); Hamming(sig) <= d, so it is an exact pre-filter;
 //   * node j (effective refcount <= max_refs) lists the first list_cap
 //     earlier nodes i < j that pass the pre-filter (exhaustive), or the
-//     proposed ones (learned), keeps those with d <= bound, and merges into
-//     the first of them that stays a representative.
+//     proposed ones (learned), keeps those with d <= bound, and walks them in
+//     order: a representative becomes its target; an entry that merged
+//     earlier offers its own target, taken when within the bound of j.
 // ============================================================================
 namespace orc {
 
@@ -1578,17 +1579,25 @@ static int lossyMerge(Octree& o, const LossyParams& p, const uint32_t* proposals
       }
       st.verified += lists[j].size();
     }
-    // sequential resolution in index order
+    // sequential resolution in index order: the first verified entry that is
+    // a representative, or whose representative (it merged earlier) is within
+    // the bound of j, becomes j's target
     std::vector<char> rep(N, 1);
     std::vector<id_t> target(N);
     for (size_t j = 0; j < N; ++j) {
       target[j] = (id_t)j;
-      for (const auto& e : lists[j])
-        if (rep[e.first]) {
-          rep[j] = 0; target[j] = e.first;
-          st.merged += 1; st.error_bound += e.second * refs[lev][j];
-          break;
+      for (const auto& e : lists[j]) {
+        id_t t = e.first;
+        uint64_t d = e.second;
+        if (!rep[t]) {
+          t = target[t];
+          d = subtreeDiff(o, vox, (unsigned)lev, t, (id_t)j, bound, 0);
+          if (d > bound) continue;
         }
+        rep[j] = 0; target[j] = t;
+        st.merged += 1; st.error_bound += d * refs[lev][j];
+        break;
+      }
     }
     std::vector<id_t> newId(N, 0);
     std::vector<Node> kept;
@@ -1629,13 +1638,16 @@ static void lshHyperplanes(uint64_t seed, uint32_t tables, uint32_t bits, uint32
   }
 }
 
+// sig (optional): candidates are scored by the Hamming distance of the
+// 512-bit signatures (16 u32 per point) instead of the embedding distance.
 static void knnLSH(const float* e, size_t n, uint32_t dim, uint32_t k, uint32_t tables, uint32_t bits, uint32_t W,
-                   uint64_t seed, uint32_t* out) {
+                   uint64_t seed, uint32_t* out, const uint32_t* sig = nullptr) {
   std::vector<float> h;
   lshHyperplanes(seed, tables, bits, dim, h);
   std::vector<std::vector<std::pair<float, uint32_t>>> best(n);
   std::vector<uint32_t> key(n), order(n), pos(n);
   auto dist = [&](size_t a, size_t b) {
+    if (sig) return (float)hamming512(sig + 16 * a, sig + 16 * b);
     float s = 0.f;
     for (uint32_t d = 0; d < dim; ++d) { const float t = e[a * dim + d] - e[b * dim + d]; const float q = t * t; s = s + q; }
     return s;
@@ -1672,9 +1684,35 @@ static void knnLSH(const float* e, size_t n, uint32_t dim, uint32_t k, uint32_t 
   }
 }
 
+// One neighbour-of-neighbour round on the signature Hamming metric: x keeps
+// the k smallest distinct (distance, index) among its list and its
+// neighbours' lists (kNone-padded).
+static void knnRefine(const uint32_t* sig, size_t n, uint32_t k, const uint32_t* in, uint32_t* out) {
+  for (size_t x = 0; x < n; ++x) {
+    std::vector<std::pair<uint32_t, uint32_t>> c;
+    auto add = [&](uint32_t z) {
+      if (z == 0xFFFFFFFFu || z == x || z >= n) return;
+      c.push_back({hamming512(sig + 16 * x, sig + 16 * (size_t)z), z});
+    };
+    for (uint32_t t = 0; t < k; ++t) {
+      const uint32_t y = in[x * k + t];
+      add(y);
+      if (y == 0xFFFFFFFFu || y >= n) continue;
+      for (uint32_t u = 0; u < k; ++u) add(in[(size_t)y * k + u]);
+    }
+    std::sort(c.begin(), c.end());
+    c.erase(std::unique(c.begin(), c.end()), c.end());
+    for (uint32_t t = 0; t < k; ++t) out[x * k + t] = t < c.size() ? c[t].second : 0xFFFFFFFFu;
+  }
+}
+
 }  // namespace orc
 
 extern "C" {
+
+void orc_knn_refine(const uint32_t* sig, uint64_t n, uint32_t k, const uint32_t* in, uint32_t* out) {
+  orc::knnRefine(sig, (size_t)n, k, in, out);
+}
 
 int orc_lossy_merge(void* h, const orc::LossyParams* p, const uint32_t* proposals, uint32_t k,
                     orc::LossyLevelStats* stats, uint32_t stats_cap) {
@@ -1709,8 +1747,8 @@ uint64_t orc_subtree_diff(void* h, uint32_t lev, uint32_t a, uint32_t b, uint64_
 }
 
 void orc_knn_lsh(const float* e, uint64_t n, uint32_t dim, uint32_t k, uint32_t tables, uint32_t bits, uint32_t window,
-                 uint64_t seed, uint32_t* out) {
-  orc::knnLSH(e, (size_t)n, dim, k, tables, bits, window, seed, out);
+                 uint64_t seed, uint32_t* out, const uint32_t* sig) {
+  orc::knnLSH(e, (size_t)n, dim, k, tables, bits, window, seed, out, sig);
 }
 
 }  // extern "C"

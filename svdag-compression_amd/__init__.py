@@ -51,7 +51,7 @@ class LossyParams(C.Structure):
     _fields_ = [("lev_lo", C.c_uint32), ("lev_hi", C.c_uint32), ("max_diff", C.c_uint32), ("max_refs", C.c_uint32),
                 ("list_cap", C.c_uint32), ("diff_factor", C.c_float), ("proposer", C.c_uint32),
                 ("knn_k", C.c_uint32), ("knn_tables", C.c_uint32), ("knn_bits", C.c_uint32),
-                ("knn_window", C.c_uint32), ("pad", C.c_uint32), ("knn_seed", C.c_uint64), ("model", C.c_void_p),
+                ("knn_window", C.c_uint32), ("knn_refine", C.c_uint32), ("knn_seed", C.c_uint64), ("model", C.c_void_p),
                 ("knn_record", C.c_void_p), ("knn_record_cap", C.c_uint64)]
 
 
@@ -139,7 +139,7 @@ def lib():
         "svd_mlp_embed": (i32, [vp, vp, u64, vp]),
         "svd_mlp_loss_grad": (i32, [vp, vp, vp, u64, P(C.c_float), vp]),
         "svd_mlp_train": (i32, [vp, vp, P(TrainParams), P(TrainStats)]),
-        "svd_knn": (i32, [vp, vp, u64, u32, u32, u32, u32, u64, vp]),
+        "svd_knn": (i32, [vp, vp, u64, u32, u32, u32, u32, u64, vp, vp]),
         "svd_workload_caves": (i32, [vp, u32, u64, u32, C.c_double, C.c_double, P(vp), P(u64)]),
         "svd_device_free": (i32, [vp, vp]),
         "svd_device_read": (i32, [vp, vp, vp, u64]),
@@ -230,9 +230,11 @@ class Context:
         return buf.tobytes()
 
     def knn(self, emb_ptr: int, n: int, k: int = 16, tables: int = 4, bits: int = 16, window: int = 32,
-            seed: int = 7, out_ptr: int = 0):
-        """k nearest neighbours of n 32-d fp32 points (device pointers)."""
-        self.check(self.L.svd_knn(self.h, C.c_void_p(emb_ptr), n, k, tables, bits, window, seed, C.c_void_p(out_ptr)))
+            seed: int = 7, out_ptr: int = 0, sig_ptr: int = 0):
+        """k nearest neighbours of n 32-d fp32 points (device pointers); with
+        sig_ptr the candidates are scored by signature Hamming distance."""
+        self.check(self.L.svd_knn(self.h, C.c_void_p(emb_ptr), n, k, tables, bits, window, seed, C.c_void_p(out_ptr),
+                                  C.c_void_p(sig_ptr) if sig_ptr else None))
 
     def caves(self, levels: int, seed: int = 3, octaves: int = 5, threshold: float = 0.1,
               base_cells: float = 4.0) -> "DeviceCodes":
@@ -528,13 +530,13 @@ class Octree:
         return ptr.value, n.value
 
     def lossy_merge(self, lev_lo: int, lev_hi: int, max_diff: int = 0, diff_factor: float = 1.0, max_refs: int = 1,
-                    list_cap: int = 32, model: "MLP | None" = None, k: int = 16, tables: int = 4, bits: int = 16,
-                    window: int = 32, knn_seed: int = 7, record_knn: bool = False):
+                    list_cap: int = 32, model: "MLP | None" = None, k: int = 16, tables: int = 8, bits: int = 16,
+                    window: int = 64, refine: int = 2, knn_seed: int = 7, record_knn: bool = False):
         """GeomOctree::toLossyDag on the device.  model=None: exhaustive
         proposer; else the learned one.  Returns per-level stats (and the kNN
         lists used, concatenated level by level, when record_knn)."""
         p = LossyParams(lev_lo, lev_hi, max_diff, max_refs, list_cap, diff_factor,
-                        PROPOSER_LEARNED if model is not None else PROPOSER_EXHAUSTIVE, k, tables, bits, window, 0,
+                        PROPOSER_LEARNED if model is not None else PROPOSER_EXHAUSTIVE, k, tables, bits, window, refine,
                         knn_seed, model.h if model is not None else None, None, 0)
         rec = None
         if record_knn and model is not None:

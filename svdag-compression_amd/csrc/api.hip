@@ -614,11 +614,11 @@ int svd_mlp_train(svd_mlp* m, svd_octree* o, const svd_mlp_train_params* p, svd_
 }
 
 int svd_knn(svd_ctx* c, const float* emb, uint64_t n, uint32_t k, uint32_t tables, uint32_t bits, uint32_t window,
-            uint64_t seed, uint32_t* out) {
+            uint64_t seed, uint32_t* out, const uint32_t* sig) {
   if (!c || (n && (!emb || !out))) return SVD_ERR_ARG;
   return guarded(c, [&] {
     Timer t(c, "knn");
-    knn_lsh(c, emb, n, k, tables, bits, window, seed, out);
+    knn_lsh(c, emb, n, k, tables, bits, window, seed, out, sig);
   });
 }
 

@@ -7,6 +7,9 @@
 //   (component = ((z >> 40) * 2^-24) * 2 - 1);
 //   stable sort by key; distance = sum_d (e_x[d] - e_y[d])^2;
 //   result = the k smallest distinct (distance, index) pairs over all tables.
+// With signatures given, the candidates are scored by the exact Hamming
+// distance of the 512-bit occupancies instead (the embedding only decides
+// which nodes meet in a window): the learned proposer's re-ranking.
 #include "svd_internal.hpp"
 #include <rocprim/device/device_radix_sort.hpp>
 
@@ -40,54 +43,81 @@ __global__ void k_lsh_keys(const float* __restrict__ e, uint64_t n, const float*
 
 // Block = 256 consecutive sorted positions; the window's rows are staged in
 // LDS.  Each lane keeps its point's running top-k (registers) across tables
-// through best_d / best_i in HBM.
-__global__ void __launch_bounds__(kTile) k_window_topk(const float* __restrict__ e, const uint32_t* __restrict__ order,
-                                                       uint64_t n, uint32_t W, uint32_t k, int first,
-                                                       float* __restrict__ best_d, uint32_t* __restrict__ best_i) {
-  extern __shared__ float rows[];          // (kTile + 2W) x kRow
+// through best_d / best_i in HBM.  HAM = 0: squared L2 of the embeddings;
+// HAM = 1: Hamming distance of the 512-bit signatures.
+template <int HAM, int KM>
+__global__ void __launch_bounds__(kTile) k_window_topk(const float* __restrict__ e, const uint32_t* __restrict__ sig,
+                                                       const uint32_t* __restrict__ order, uint64_t n, uint32_t W,
+                                                       uint32_t k, int first, float* __restrict__ best_d,
+                                                       uint32_t* __restrict__ best_i) {
+  constexpr int kWords = HAM ? 16 : kDim;
+  constexpr int kR = kWords + 1;            // padded LDS row (no bank conflicts)
+  extern __shared__ float rows[];           // (kTile + 2W) x kR
+  uint32_t* urows = reinterpret_cast<uint32_t*>(rows);
   const uint64_t q0 = (uint64_t)blockIdx.x * kTile;
   const int64_t lo = (int64_t)q0 - (int64_t)W;
   const uint32_t nrows = kTile + 2 * W;
-  for (uint32_t t = threadIdx.x; t < nrows * kDim; t += kTile) {
-    const uint32_t rr = t / kDim, d = t % kDim;
+  for (uint32_t t = threadIdx.x; t < nrows * kWords; t += kTile) {
+    const uint32_t rr = t / kWords, d = t % kWords;
     const int64_t q = lo + (int64_t)rr;
-    rows[rr * kRow + d] = (q >= 0 && q < (int64_t)n) ? e[(uint64_t)order[q] * kDim + d] : 0.f;
+    const bool in = q >= 0 && q < (int64_t)n;
+    if (HAM) urows[rr * kR + d] = in ? sig[(uint64_t)order[q] * 16 + d] : 0u;
+    else rows[rr * kR + d] = in ? e[(uint64_t)order[q] * kDim + d] : 0.f;
   }
   __syncthreads();
   const uint64_t q = q0 + threadIdx.x;
   if (q >= n) return;
   const uint32_t x = order[q];
-  float bd[kMaxK];
-  uint32_t bi[kMaxK];
+  float bd[KM];
+  uint32_t bi[KM];
 #pragma unroll
-  for (int t = 0; t < kMaxK; ++t) {
+  for (int t = 0; t < KM; ++t) {
     bd[t] = first || t >= (int)k ? __builtin_inff() : best_d[(uint64_t)x * k + t];
     bi[t] = first || t >= (int)k ? 0xFFFFFFFFu : best_i[(uint64_t)x * k + t];
   }
-  const float* ex = &rows[(threadIdx.x + W) * kRow];
+  const uint32_t xr = (threadIdx.x + W) * kR;
+  // the query row lives in registers
+  uint32_t xs[HAM ? 16 : 1];
+  float xe[HAM ? 1 : kDim];
+  if (HAM) {
+#pragma unroll
+    for (int d = 0; d < (HAM ? 16 : 1); ++d) xs[d] = urows[xr + d];
+  } else {
+#pragma unroll
+    for (int d = 0; d < (HAM ? 1 : kDim); ++d) xe[d] = rows[xr + d];
+  }
   for (int64_t r = (int64_t)q - (int64_t)W; r <= (int64_t)q + (int64_t)W; ++r) {
     if (r < 0 || r >= (int64_t)n || r == (int64_t)q) continue;
-    const uint32_t y = order[r];
-    const float* ey = &rows[(uint32_t)(r - lo) * kRow];
+    const uint32_t yr = (uint32_t)(r - lo) * kR;
     float s = 0.f;
+    if (HAM) {
+      uint32_t h = 0;
 #pragma unroll
-    for (int d = 0; d < kDim; ++d) { const float t = ex[d] - ey[d]; const float sq = t * t; s = s + sq; }
+      for (int d = 0; d < (HAM ? 16 : 1); ++d) h += __popc(xs[d] ^ urows[yr + d]);
+      s = (float)h;
+    } else {
+#pragma unroll
+      for (int d = 0; d < (HAM ? 1 : kDim); ++d) { const float t = xe[d] - rows[yr + d]; const float sq = t * t; s = s + sq; }
+    }
     // (s, y) < worst kept?
-    float wd = bd[0];
-    uint32_t wi = bi[0];
+    float wd = bd[KM - 1];
+    uint32_t wi = bi[KM - 1];
+    if (KM != (int)k) {
 #pragma unroll
-    for (int t = 1; t < kMaxK; ++t)
-      if (t == (int)k - 1) { wd = bd[t]; wi = bi[t]; }
+      for (int t = 0; t < KM; ++t)
+        if (t == (int)k - 1) { wd = bd[t]; wi = bi[t]; }
+    }
+    const uint32_t y = order[r];
     if (!(s < wd || (s == wd && y < wi))) continue;
     bool dup = false;
 #pragma unroll
-    for (int t = 0; t < kMaxK; ++t) dup |= (t < (int)k) && bi[t] == y;
+    for (int t = 0; t < KM; ++t) dup |= (t < (int)k) && bi[t] == y;
     if (dup) continue;
     // insert keeping (distance, index) ascending
     float cd = s;
     uint32_t ci = y;
 #pragma unroll
-    for (int t = 0; t < kMaxK; ++t) {
+    for (int t = 0; t < KM; ++t) {
       if (t < (int)k && (cd < bd[t] || (cd == bd[t] && ci < bi[t]))) {
         const float td = bd[t]; const uint32_t ti = bi[t];
         bd[t] = cd; bi[t] = ci;
@@ -96,14 +126,83 @@ __global__ void __launch_bounds__(kTile) k_window_topk(const float* __restrict__
     }
   }
 #pragma unroll
-  for (int t = 0; t < kMaxK; ++t)
+  for (int t = 0; t < KM; ++t)
     if (t < (int)k) { best_d[(uint64_t)x * k + t] = bd[t]; best_i[(uint64_t)x * k + t] = bi[t]; }
+}
+
+// One neighbour-of-neighbour round (NN-descent) on the signature Hamming
+// metric: node x keeps the k smallest distinct (distance, index) among its
+// own list and the lists of its neighbours.  Reads `in`, writes `out`.
+template <int KM>
+__global__ void __launch_bounds__(256) k_knn_refine(const uint32_t* __restrict__ sig, uint64_t n, uint32_t k,
+                                                    const uint32_t* __restrict__ in, uint32_t* __restrict__ out) {
+  const uint64_t x = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= n) return;
+  uint32_t sx[16];
+  const uint4* px = reinterpret_cast<const uint4*>(sig + x * 16);
+#pragma unroll
+  for (int q = 0; q < 4; ++q) { const uint4 t = px[q]; sx[4 * q] = t.x; sx[4 * q + 1] = t.y; sx[4 * q + 2] = t.z; sx[4 * q + 3] = t.w; }
+  uint32_t bd[KM], bi[KM];
+#pragma unroll
+  for (int t = 0; t < KM; ++t) { bd[t] = 0xFFFFFFFFu; bi[t] = 0xFFFFFFFFu; }
+  auto consider = [&](uint32_t z) {
+    if (z == 0xFFFFFFFFu || (uint64_t)z == x || (uint64_t)z >= n) return;
+    const uint4* pz = reinterpret_cast<const uint4*>(sig + (uint64_t)z * 16);
+    uint32_t h = 0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const uint4 t = pz[q];
+      h += __popc(t.x ^ sx[4 * q]) + __popc(t.y ^ sx[4 * q + 1]) + __popc(t.z ^ sx[4 * q + 2]) + __popc(t.w ^ sx[4 * q + 3]);
+    }
+    uint32_t wd = bd[KM - 1], wi = bi[KM - 1];
+    if (KM != (int)k) {
+#pragma unroll
+      for (int t = 0; t < KM; ++t)
+        if (t == (int)k - 1) { wd = bd[t]; wi = bi[t]; }
+    }
+    if (!(h < wd || (h == wd && z < wi))) return;
+    bool dup = false;
+#pragma unroll
+    for (int t = 0; t < KM; ++t) dup |= (t < (int)k) && bi[t] == z;
+    if (dup) return;
+    uint32_t cd = h, ci = z;
+#pragma unroll
+    for (int t = 0; t < KM; ++t) {
+      if (t < (int)k && (cd < bd[t] || (cd == bd[t] && ci < bi[t]))) {
+        const uint32_t td = bd[t], ti = bi[t];
+        bd[t] = cd; bi[t] = ci;
+        cd = td; ci = ti;
+      }
+    }
+  };
+  for (uint32_t t = 0; t < k; ++t) {
+    const uint32_t y = in[x * k + t];
+    consider(y);
+    if (y == 0xFFFFFFFFu || (uint64_t)y >= n) continue;
+    for (uint32_t u = 0; u < k; ++u) consider(in[(uint64_t)y * k + u]);
+  }
+#pragma unroll
+  for (int t = 0; t < KM; ++t)
+    if (t < (int)k) out[x * k + t] = bi[t];
 }
 
 }  // namespace
 
+void knn_refine(svd_ctx* c, const uint32_t* sig, uint64_t n, uint32_t k, uint32_t rounds, uint32_t* knn) {
+  if (!rounds || !n) return;
+  if (k < 1 || k > (uint32_t)kMaxK) throw Error{SVD_ERR_ARG, "knn: k must be 1..32"};
+  uint32_t* tmp = dalloc<uint32_t>(c, n * k);
+  for (uint32_t r = 0; r < rounds; ++r) {
+    if (k == 16) k_knn_refine<16><<<div_up(n, 256), 256, 0, c->stream>>>(sig, n, k, knn, tmp);
+    else k_knn_refine<kMaxK><<<div_up(n, 256), 256, 0, c->stream>>>(sig, n, k, knn, tmp);
+    SVD_HIP(hipMemcpyAsync(knn, tmp, n * k * 4, hipMemcpyDeviceToDevice, c->stream));
+  }
+  check_launch("k_knn_refine");
+  dfree(c, tmp);
+}
+
 void knn_lsh(svd_ctx* c, const float* emb, uint64_t n, uint32_t k, uint32_t tables, uint32_t bits, uint32_t window,
-             uint64_t seed, uint32_t* out) {
+             uint64_t seed, uint32_t* out, const uint32_t* sig) {
   if (k < 1 || k > (uint32_t)kMaxK) throw Error{SVD_ERR_ARG, "knn: k must be 1..32"};
   if (bits < 1 || bits > 24 || tables < 1 || window < 1 || window > 96) throw Error{SVD_ERR_ARG, "knn: bits 1..24, tables >= 1, window 1..96"};
   if (n >= 0xFFFFFFFFull) throw Error{SVD_ERR_CAPACITY, "knn: too many points"};
@@ -129,11 +228,15 @@ void knn_lsh(svd_ctx* c, const float* emb, uint64_t n, uint32_t k, uint32_t tabl
   size_t bytes = 0;
   SVD_HIP(rocprim::radix_sort_pairs(nullptr, bytes, keys, keys2, vals, order, (size_t)n, 0, bits, c->stream));
   void* tmp = c->arena->get(bytes + 16);
-  const size_t lds = (size_t)(kTile + 2 * window) * kRow * sizeof(float);
+  const size_t lds = (size_t)(kTile + 2 * window) * (sig ? 17 : kRow) * sizeof(float);
   for (uint32_t t = 0; t < tables; ++t) {
     k_lsh_keys<<<div_up(n, 256), 256, 0, c->stream>>>(emb, n, dh + (size_t)t * bits * kDim, bits, keys, vals);
     SVD_HIP(rocprim::radix_sort_pairs(tmp, bytes, keys, keys2, vals, order, (size_t)n, 0, bits, c->stream));
-    k_window_topk<<<div_up(n, kTile), kTile, lds, c->stream>>>(emb, order, n, window, k, t == 0, bd, out);
+    const dim3 g(div_up(n, kTile));
+    if (sig && k == 16) k_window_topk<1, 16><<<g, kTile, lds, c->stream>>>(emb, sig, order, n, window, k, t == 0, bd, out);
+    else if (sig) k_window_topk<1, kMaxK><<<g, kTile, lds, c->stream>>>(emb, sig, order, n, window, k, t == 0, bd, out);
+    else if (k == 16) k_window_topk<0, 16><<<g, kTile, lds, c->stream>>>(emb, sig, order, n, window, k, t == 0, bd, out);
+    else k_window_topk<0, kMaxK><<<g, kTile, lds, c->stream>>>(emb, sig, order, n, window, k, t == 0, bd, out);
     check_launch("knn table");
   }
   c->arena->put(tmp);

@@ -12,9 +12,11 @@
 //      MLP embeddings, sorted and de-duplicated);
 //   3. verify: exact voxel difference of the two subtrees by a joint descent
 //      with early abort (the Hamming distance above is a lower bound of it);
-//   4. resolve: j merges into the first verified list entry that stays a
-//      representative; resolved in monotone rounds (each round settles at
-//      least the smallest unresolved node, so the loop ends);
+//   4. resolve: j walks its verified entries in order; a representative
+//      becomes its target, an entry that merged earlier offers its own target
+//      (taken when the exact difference to j is within the bound); resolved
+//      in monotone rounds (each round settles at least the smallest
+//      unresolved node, so the loop ends);
 //   5. compact the level, remap the parent level's child pointers.
 // Finally the whole DAG is re-deduplicated exactly (:1176).
 #include "svd_internal.hpp"
@@ -251,54 +253,74 @@ __global__ void k_verify(LevelView v, uint32_t lev, const uint32_t* __restrict__
   if (d <= bound) atomicAdd(&stats[1], 1ull);
 }
 
+// Node state word: state << 32 | target node (one 64-bit word, so a reader
+// that sees "merged" also sees the target).
+constexpr uint64_t kStRep = (uint64_t)kRep << 32, kStMerged = (uint64_t)kMerged << 32;
+
 __global__ void k_resolve_init(uint64_t n, const uint64_t* __restrict__ refs, uint64_t maxRefs,
-                               uint8_t* __restrict__ state, unsigned long long* __restrict__ stats) {
+                               unsigned long long* __restrict__ st, unsigned long long* __restrict__ stats) {
   uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= n) return;
   const bool cand = refs[j] <= maxRefs;
-  state[j] = cand ? kUnres : kRep;
+  st[j] = cand ? (unsigned long long)kUnres << 32 : (unsigned long long)(kStRep | j);
   if (cand) atomicAdd(&stats[2], 1ull);
 }
 
-// One round: an unresolved j walks its verified entries in order; the first
-// representative is its target, merged entries are skipped, an unresolved
-// entry makes it wait.  States only move away from kUnres, so a stale read
-// delays a node by one round and never changes its outcome.
-__global__ void k_resolve(const uint32_t* __restrict__ lists, const uint32_t* __restrict__ dv,
-                          const uint32_t* __restrict__ counts, uint64_t n, uint32_t cap, uint32_t bound,
-                          uint8_t* state, uint32_t* __restrict__ target, uint32_t* __restrict__ unresolved) {
+// One round: an unresolved j walks its verified entries in order.  A
+// representative becomes its target; an entry that merged offers its own
+// target (a representative), taken when the exact difference to j is within
+// the bound; an unresolved entry makes j wait.  States only move away from
+// "unresolved", so a stale read delays a node by one round and never changes
+// its outcome.
+__global__ void k_resolve(LevelView v, uint32_t lev, const uint32_t* __restrict__ lists,
+                          const uint32_t* __restrict__ dv, const uint32_t* __restrict__ counts, uint64_t n,
+                          uint32_t cap, uint32_t bound, unsigned long long* st, uint32_t* __restrict__ tdist,
+                          uint32_t* __restrict__ unresolved) {
   uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= n || state[j] != kUnres) return;
+  if (j >= n) return;
+  if ((__atomic_load_n(&st[j], __ATOMIC_RELAXED) >> 32) != kUnres) return;
   const uint32_t cnt = counts[j];
   for (uint32_t t = 0; t < cnt; ++t) {
     const uint64_t e = j * cap + t;
     if (dv[e] > bound) continue;
     const uint32_t i = lists[e];
-    const uint8_t s = __atomic_load_n(&state[i], __ATOMIC_RELAXED);
+    const unsigned long long w = __atomic_load_n(&st[i], __ATOMIC_RELAXED);
+    const uint32_t s = (uint32_t)(w >> 32);
     if (s == kRep) {
-      target[j] = (uint32_t)t;
-      __atomic_store_n(&state[j], kMerged, __ATOMIC_RELAXED);
+      tdist[j] = dv[e];
+      __atomic_store_n(&st[j], (unsigned long long)(kStMerged | i), __ATOMIC_RELAXED);
       return;
     }
-    if (s == kUnres) { atomicAdd(unresolved, 1u); return; }
+    if (s == kMerged) {
+      const uint32_t r = (uint32_t)w;
+      const uint32_t d = subtree_diff(v, lev, r, (uint32_t)j, bound);
+      if (d <= bound) {
+        tdist[j] = d;
+        __atomic_store_n(&st[j], (unsigned long long)(kStMerged | r), __ATOMIC_RELAXED);
+        return;
+      }
+      continue;
+    }
+    atomicAdd(unresolved, 1u);
+    return;
   }
-  __atomic_store_n(&state[j], kRep, __ATOMIC_RELAXED);
+  __atomic_store_n(&st[j], (unsigned long long)(kStRep | j), __ATOMIC_RELAXED);
 }
 
-__global__ void k_rep_flag(const uint8_t* __restrict__ state, uint64_t n, uint32_t* __restrict__ flag) {
+__global__ void k_rep_flag(const unsigned long long* __restrict__ st, uint64_t n, uint32_t* __restrict__ flag) {
   uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (j < n) flag[j] = state[j] == kRep ? 1u : 0u;
+  if (j < n) flag[j] = (st[j] >> 32) == kRep ? 1u : 0u;
 }
 
 __global__ void k_lossy_compact(const uint32_t* __restrict__ rec, const uint64_t* __restrict__ vox,
-                                const uint8_t* __restrict__ state, const uint32_t* __restrict__ target,
-                                const uint32_t* __restrict__ lists, const uint32_t* __restrict__ dv,
+                                const unsigned long long* __restrict__ st, const uint32_t* __restrict__ tdist,
                                 const uint32_t* __restrict__ pos, const uint64_t* __restrict__ refs, uint64_t n,
-                                uint32_t cap, uint32_t* __restrict__ out, uint64_t* __restrict__ outVox,
+                                uint32_t* __restrict__ out, uint64_t* __restrict__ outVox,
                                 uint32_t* __restrict__ corr, unsigned long long* __restrict__ stats) {
   uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= n) return;
-  if (state[j] == kRep) {
+  const unsigned long long w = st[j];
+  if ((w >> 32) == kRep) {
     const uint32_t p = pos[j];
     const uint32_t* s = rec + j * kRecWords;
     uint32_t* d = out + (uint64_t)p * kRecWords;
@@ -307,10 +329,9 @@ __global__ void k_lossy_compact(const uint32_t* __restrict__ rec, const uint64_t
     outVox[p] = vox[j];
     corr[j] = p;
   } else {
-    const uint64_t e = j * cap + target[j];
-    corr[j] = pos[lists[e]];
+    corr[j] = pos[(uint32_t)w];
     atomicAdd(&stats[3], 1ull);
-    atomicAdd(&stats[4], (unsigned long long)dv[e] * refs[j]);
+    atomicAdd(&stats[4], (unsigned long long)tdist[j] * refs[j]);
   }
 }
 
@@ -380,7 +401,7 @@ void lossy_merge(svd_octree* o, const svd_lossy_params* p, std::vector<svd_lossy
   if (p->list_cap < 1 || p->list_cap > 64) throw Error{SVD_ERR_ARG, "list_cap must be 1..64"};
   const bool learned = p->proposer == SVD_PROPOSER_LEARNED;
   if (p->proposer > SVD_PROPOSER_LEARNED) throw Error{SVD_ERR_ARG, "unknown proposer"};
-  if (learned && (!p->model || p->knn_k < 1 || p->knn_k > 32 || p->knn_tables < 1 || p->knn_bits < 1 ||
+  if (learned && (!p->model || p->knn_k < 1 || p->knn_k > 32 || p->knn_tables < 1 || p->knn_bits < 1 || p->knn_refine > 16 ||
                   p->knn_bits > 24 || p->knn_window < 1 || p->knn_window > 96))
     throw Error{SVD_ERR_ARG, "learned proposer needs a model, 1<=k<=32, tables>=1, 1<=bits<=24, 1<=window<=96"};
   const uint32_t cap = p->list_cap;
@@ -432,8 +453,8 @@ void lossy_merge(svd_octree* o, const svd_lossy_params* p, std::vector<svd_lossy
     uint32_t* lists = dalloc<uint32_t>(c, (n + 1) * cap);
     uint32_t* counts = dalloc<uint32_t>(c, n + 1);
     uint32_t* dv = dalloc<uint32_t>(c, (n + 1) * cap);
-    uint8_t* state = dalloc<uint8_t>(c, n + 1);
-    uint32_t* target = dalloc<uint32_t>(c, n + 1);
+    unsigned long long* state = dalloc<unsigned long long>(c, n + 1);
+    uint32_t* tdist = dalloc<uint32_t>(c, n + 1);
     uint32_t* pos = dalloc<uint32_t>(c, n + 1);
     uint32_t* total = dalloc<uint32_t>(c, 1);
     uint32_t* corr = dalloc<uint32_t>(c, n + 1);
@@ -453,7 +474,8 @@ void lossy_merge(svd_octree* o, const svd_lossy_params* p, std::vector<svd_lossy
       uint32_t* knn = dalloc<uint32_t>(c, (n + 1) * k);
       if (n) mlp_embed(p->model, sig, n, emb);
       clk.mark(2);
-      if (n) knn_lsh(c, emb, n, k, p->knn_tables, p->knn_bits, p->knn_window, p->knn_seed, knn);
+      if (n) knn_lsh(c, emb, n, k, p->knn_tables, p->knn_bits, p->knn_window, p->knn_seed, knn, sig);
+      if (n) knn_refine(c, sig, n, k, p->knn_refine, knn);
       if (p->knn_record) {
         if (recUsed + n * k > p->knn_record_cap) throw Error{SVD_ERR_CAPACITY, "knn_record buffer too small"};
         if (n) SVD_HIP(hipMemcpyAsync(p->knn_record + recUsed, knn, n * k * 4, hipMemcpyDeviceToHost, c->stream));
@@ -481,7 +503,8 @@ void lossy_merge(svd_octree* o, const svd_lossy_params* p, std::vector<svd_lossy
     check_launch("k_resolve_init");
     for (uint64_t round = 0; n; ++round) {
       fill_u32(c, unresolved, 0u, 1);
-      k_resolve<<<div_up(n, kB), kB, 0, c->stream>>>(lists, dv, counts, n, cap, bound, state, target, unresolved);
+      k_resolve<<<div_up(n, kB), kB, 0, c->stream>>>(v, (uint32_t)lev, lists, dv, counts, n, cap, bound, state, tdist,
+                                                     unresolved);
       check_launch("k_resolve");
       if (read_u32(c, unresolved) == 0) break;
       if (round > n) throw Error{SVD_ERR_DEVICE, "lossy resolution did not converge"};
@@ -491,8 +514,8 @@ void lossy_merge(svd_octree* o, const svd_lossy_params* p, std::vector<svd_lossy
     const uint64_t kept = n ? read_u32(c, total) : 0;
     uint32_t* newRec = dalloc<uint32_t>(c, (kept + 1) * kRecWords);
     uint64_t* newVox = dalloc<uint64_t>(c, kept + 1);
-    if (n) k_lossy_compact<<<div_up(n, kB), kB, 0, c->stream>>>(lv.rec, vox[lev], state, target, lists, dv, pos,
-                                                              refs[lev], n, cap, newRec, newVox, corr, dstats);
+    if (n) k_lossy_compact<<<div_up(n, kB), kB, 0, c->stream>>>(lv.rec, vox[lev], state, tdist, pos, refs[lev], n,
+                                                              newRec, newVox, corr, dstats);
     check_launch("k_lossy_compact");
     const uint64_t np = o->dag[lev - 1].n;
     if (np) k_remap_inplace<<<div_up(np, kB), kB, 0, c->stream>>>(o->dag[lev - 1].rec, np, corr);
@@ -519,7 +542,7 @@ void lossy_merge(svd_octree* o, const svd_lossy_params* p, std::vector<svd_lossy
     lv.n = kept;
     dfree(c, vox[lev]);
     vox[lev] = newVox;
-    dfree(c, sig); dfree(c, lists); dfree(c, counts); dfree(c, dv); dfree(c, state); dfree(c, target);
+    dfree(c, sig); dfree(c, lists); dfree(c, counts); dfree(c, dv); dfree(c, state); dfree(c, tdist);
     dfree(c, pos); dfree(c, total); dfree(c, corr);
   }
   for (auto* r : refs) dfree(c, r);

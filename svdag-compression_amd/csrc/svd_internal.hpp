@@ -197,8 +197,11 @@ void mlp_set_params(svd_mlp* m, const float* host);
 void mlp_get_params(svd_mlp* m, float* host);
 void mlp_loss_grad(svd_mlp* m, const uint32_t* X, const uint32_t* labels, uint64_t P, float* lossHost, float* gradsHost);
 void mlp_train(svd_mlp* m, svd_octree* o, const svd_mlp_train_params* p, svd_mlp_train_stats* st);
+// sig != null: candidates scored by the Hamming distance of the signatures
 void knn_lsh(svd_ctx* c, const float* emb, uint64_t n, uint32_t k, uint32_t tables, uint32_t bits, uint32_t window,
-             uint64_t seed, uint32_t* out);
+             uint64_t seed, uint32_t* out, const uint32_t* sig = nullptr);
+// neighbour-of-neighbour rounds on the signature Hamming metric (in place)
+void knn_refine(svd_ctx* c, const uint32_t* sig, uint64_t n, uint32_t k, uint32_t rounds, uint32_t* knn);
 
 // ---- batches (batch.hip, trace.hip) ---------------------------------------------
 void batch_build(svd_batch* b, const uint64_t* dcodes, uint64_t n, const std::vector<uint64_t>& off);

@@ -197,6 +197,12 @@ def test_knn_matches_oracle(ctx):
     got = out.cpu().numpy().view(np.uint32).reshape(n, 16)
     np.testing.assert_array_equal(got, O.knn_lsh(e, 16, 4, 16, 32, 7))
     assert got[7, 0] == 100 and got[100, 0] == 7
+    # signature re-ranking
+    sig = _rand_sigs(n, 6, density=0.1)
+    dsig = _dev_u32(sig)
+    ctx.knn(de.data_ptr(), n, 16, 4, 12, 48, 9, out.data_ptr(), sig_ptr=dsig.data_ptr())
+    np.testing.assert_array_equal(out.cpu().numpy().view(np.uint32).reshape(n, 16),
+                                  O.knn_lsh(e, 16, 4, 12, 48, 9, sig=sig))
 
 
 def test_knn_full_window_is_brute_force(ctx):
@@ -222,7 +228,18 @@ def test_lossy_learned_matches_oracle_given_knn(ctx):
     g, o = gpu_dag(ctx, codes, L), oracle_dag(codes, L)
     m = S.MLP(ctx, seed=1)
     m.train(g, 1, L - 2, pairs_per_step=512, steps=200, lr=2e-3, seed=5)
+    # the first processed level (L-2) sees the input DAG: its kNN lists are
+    # checked against the oracle's search on the same signatures / embeddings
+    n0 = g.level_size(S.VIEW_DAG, L - 2)
+    dsig = torch.zeros(n0 * 16, dtype=torch.int32, device=DEV)
+    g.lossy_signatures(L - 2, dsig.data_ptr())
+    demb = torch.zeros(n0 * 32, dtype=torch.float32, device=DEV)
+    m.embed(dsig.data_ptr(), n0, demb.data_ptr())
+    sig0 = dsig.cpu().numpy().view(np.uint32).reshape(n0, 16)
+    emb0 = demb.cpu().numpy().reshape(n0, 32)
     st_g, rec = g.lossy_merge(1, L - 2, max_diff=10, model=m, record_knn=True)
+    lsh = O.knn_lsh(emb0, 16, 8, 16, 64, 7, sig=sig0)
+    np.testing.assert_array_equal(rec[:n0 * 16].reshape(n0, 16), O.knn_refine(sig0, lsh, 2))
     st_o = o.lossy_merge(1, L - 2, max_diff=10, proposals=rec, k=16, list_cap=32)
     assert_same_stats(st_g, st_o)
     assert_same_dag(g, o, L)
@@ -230,21 +247,23 @@ def test_lossy_learned_matches_oracle_given_knn(ctx):
 
 
 def test_learned_proposer_gains_over_untrained(ctx):
-    """Compression with the trained encoder vs the same proposer with the
-    untrained (random-init) encoder and vs exhaustive search."""
+    """What the trained encoder adds: with the neighbour search alone (no
+    neighbour-of-neighbour rounds) the trained embedding finds more merges
+    than the untrained one; the full learned pipeline stays within 1% of the
+    exhaustive proposer's compression."""
     codes, L = W.cave_codes(8, seed=3), 8
     res = {}
-    for tag in ("exhaustive", "untrained", "trained"):
+    g = gpu_dag(ctx, codes, L)
+    st = g.lossy_merge(2, L - 2, max_diff=10)
+    res["exhaustive"] = (g.counts()[0], int(st["merged"].sum()))
+    trained = S.MLP(ctx, seed=1)
+    trained.train(gpu_dag(ctx, codes, L), 2, L - 2, pairs_per_step=1024, steps=400, lr=2e-3, seed=5, near_pct=80)
+    untrained = S.MLP(ctx, seed=1)
+    for tag, m, refine in (("untrained-r0", untrained, 0), ("trained-r0", trained, 0), ("trained-r2", trained, 2)):
         g = gpu_dag(ctx, codes, L)
-        m = None
-        if tag != "exhaustive":
-            m = S.MLP(ctx, seed=1)
-            if tag == "trained":
-                m.train(g, 2, L - 2, pairs_per_step=1024, steps=400, lr=2e-3, seed=5)
-        st = g.lossy_merge(2, L - 2, max_diff=10, model=m)
+        st = g.lossy_merge(2, L - 2, max_diff=10, model=m, refine=refine)
         res[tag] = (g.counts()[0], int(st["merged"].sum()))
     print(res, flush=True)
-    exh, unt, tr = res["exhaustive"][0], res["untrained"][0], res["trained"][0]
-    assert tr < unt                      # the trained encoder finds more merges
-    assert res["trained"][1] > res["untrained"][1]
-    assert exh <= tr
+    assert res["trained-r0"][0] < res["untrained-r0"][0]
+    assert res["trained-r0"][1] > res["untrained-r0"][1]
+    assert res["trained-r2"][0] <= 1.01 * res["exhaustive"][0]

@@ -195,6 +195,38 @@ def test_knn_lsh_full_window_is_brute_force():
     np.testing.assert_array_equal(got, np.argsort(_f32_sqdist(e), axis=1, kind="stable")[:, :8])
 
 
+def test_knn_lsh_hamming_full_window_is_brute_force():
+    """Signature re-ranking: with the window covering every point the result
+    is the exact Hamming top-k (ties by index)."""
+    rng = np.random.default_rng(8)
+    e = rng.standard_normal((50, 32)).astype(np.float32)
+    sig = (rng.integers(0, 2 ** 32, (50, 16), dtype=np.uint64)
+           & rng.integers(0, 2 ** 32, (50, 16), dtype=np.uint64)).astype(np.uint32)
+    got = O.knn_lsh(e, k=8, tables=2, bits=4, window=96, seed=3, sig=sig)
+    H = np.array([[M.hamming(sig[i:i + 1], sig[j:j + 1])[0] for j in range(50)] for i in range(50)])
+    np.fill_diagonal(H, 10 ** 6)
+    np.testing.assert_array_equal(got, np.argsort(H, axis=1, kind="stable")[:, :8])
+
+
+def test_knn_refine_improves_monotonically_and_is_exact_on_complete_lists():
+    rng = np.random.default_rng(9)
+    n, k = 60, 8
+    sig = (rng.integers(0, 2 ** 32, (n, 16), dtype=np.uint64)
+           & rng.integers(0, 2 ** 32, (n, 16), dtype=np.uint64)).astype(np.uint32)
+    H = np.array([[M.hamming(sig[i:i + 1], sig[j:j + 1])[0] for j in range(n)] for i in range(n)])
+    np.fill_diagonal(H, 10 ** 6)
+    lists = np.array([rng.choice([j for j in range(n) if j != i], k, replace=False) for i in range(n)], np.uint32)
+    before = np.sort(np.take_along_axis(H, lists.astype(np.int64), 1), axis=1)
+    after = O.knn_refine(sig, lists, 2)
+    d_after = np.take_along_axis(H, after.astype(np.int64), 1)
+    assert (np.diff(d_after, axis=1) >= 0).all()                 # ascending
+    assert (d_after <= before).all()                              # never worse at any rank
+    full = np.array([[j for j in range(n) if j != i] for i in range(n)], np.uint32)
+    # complete lists (k = n - 1): one round returns the exact order
+    got = O.knn_refine(sig, full, 1)
+    np.testing.assert_array_equal(got, np.argsort(H, axis=1, kind="stable")[:, :n - 1])
+
+
 def test_knn_lsh_recall_on_clustered_points():
     rng = np.random.default_rng(4)
     centers = rng.standard_normal((40, 32)) * 4
