@@ -128,6 +128,69 @@ def lossy_bench(S, ctx, levels, log):
     return out
 
 
+def batch_bench(S, W, ctx, n_scenes, log):
+    """Config-3 batch: n_scenes scenes of uneven depth (6..10), kinds drawn per
+    scene (icosphere balls, fBm terrain, Menger sponge surfaces), generated on
+    the device; one batched build (SVO -> DAG -> .svdag images for all scenes)
+    against the same scenes built one by one, then 64 cameras of 640x360
+    spread over the scenes traced in one launch."""
+    import torch
+    specs = W.batch_specs(n_scenes, 0)
+    t0 = time.perf_counter()
+    dev = [ctx.spec_codes(sp) for sp in specs]
+    levels = [sp["levels"] for sp in specs]
+    off = np.concatenate([[0], np.cumsum([d.n for d in dev])]).astype(np.uint64)
+    nvox = int(off[-1])
+    log(f"[bench] batch: {n_scenes} scenes, {nvox} voxels generated ({time.perf_counter() - t0:.1f} s)")
+    buf = torch.empty(nvox, dtype=torch.int64, device="cuda:0")
+    for d, o in zip(dev, off[:-1]):
+        ctx.check(ctx.L.svd_device_copy(ctx.h, S.C.c_void_p(buf.data_ptr() + 8 * int(o)), S.C.c_void_p(d.ptr),
+                                        8 * d.n))
+    tb = []
+    b = None
+    for r in range(3):
+        if b is not None:
+            b.free()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        b = S.Batch(ctx, buf.data_ptr(), off, levels, on_device=True)
+        ctx.sync()
+        tb.append(time.perf_counter() - t)
+    t = time.perf_counter()
+    for d, L in zip(dev, levels):
+        g = S.Octree.from_morton(ctx, (d.ptr, d.n), L)
+        g.to_dag()
+        g.encode(S.FMT_SVDAG)
+        g.free()
+    ctx.sync()
+    t_seq = time.perf_counter() - t
+    for d in dev:
+        d.free()
+    n_cam, cw, ch = 64, 640, 360
+    cam_scene = [k * n_scenes // n_cam for k in range(n_cam)]
+    cams = [W.trace_params(cw, ch, seed=100 + k, levels=levels[s], shadows=True) for k, s in enumerate(cam_scene)]
+    nr = n_cam * cw * ch
+    prim = torch.zeros(nr * 6, dtype=torch.int32, device="cuda:0")
+    shad = torch.zeros(nr * 6, dtype=torch.int32, device="cuda:0")
+    b.trace(cams, cam_scene, prim.data_ptr(), shad.data_ptr())          # warm-up
+    tt = []
+    for _ in range(3):
+        b.trace(cams, cam_scene, prim.data_ptr(), shad.data_ptr())
+        tt.append(ctx.kernel_ms("trace"))
+    pr = prim.cpu().numpy().view(S.RAY_DTYPE)
+    rays = nr + int((pr["t"] >= 0).sum())
+    t_b = float(np.median(tb))
+    out = {"workload": f"{n_scenes} scenes, depth 6-10, {nvox} voxels (spheres / fBm terrain / Menger sponge "
+                       f"surfaces); {n_cam} cameras x {cw}x{ch} primary + shadow rays",
+           "build_ms": round(t_b * 1e3, 2), "build_gvox_s": round(nvox / t_b / 1e9, 4),
+           "sequential_build_ms": round(t_seq * 1e3, 2), "speedup_vs_sequential": round(t_seq / t_b, 2),
+           "trace_ms": round(float(np.median(tt)), 3), "rays": rays,
+           "trace_mrays_s": round(rays / (float(np.median(tt)) * 1e-3) / 1e6, 1)}
+    b.free()
+    log(f"[bench] batch: {out}")
+    return out
+
+
 def measured_traffic(L, args):
     """HBM bytes per trace launch from the committed counter summary (two
     rocprofv3 --pmc runs of this workload, FETCH_SIZE and WRITE_SIZE, each
@@ -152,6 +215,8 @@ def main():
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-levels", type=int, default=11)
+    ap.add_argument("--batch-scenes", type=int, default=256,
+                    help="scenes of the multi-scene batch measurement; 0 skips it")
     ap.add_argument("--lossy-levels", type=int, default=12,
                     help="grid of the lossy-DAG measurement (caves 2^L); 0 skips it")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -258,6 +323,8 @@ def main():
     }
     if args.lossy_levels:
         out["lossy"] = lossy_bench(S, ctx, args.lossy_levels, log)
+    if args.batch_scenes:
+        out["batch"] = batch_bench(S, W, ctx, args.batch_scenes, log)
     if not args.no_cpu_baseline:
         log("[bench] cpu baseline (oracle) ...")
         out["cpu_baseline"] = cpu_baseline(W, L, args.cpu_sample_levels)

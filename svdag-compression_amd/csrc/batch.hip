@@ -55,11 +55,30 @@ __global__ void k_scene_starts(const uint32_t* __restrict__ sid, uint64_t n, uin
   if (i == 0 || sid[i] != sid[i - 1]) start[sid[i]] = (uint32_t)i;
 }
 
-__global__ void k_leaf_firstocc_scene(const uint32_t* __restrict__ mask, const uint32_t* __restrict__ sid, uint64_t n,
-                                      uint32_t* __restrict__ firstocc) {
-  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  atomicMin(&firstocc[(uint64_t)sid[i] * 256 + (mask[i] & 0xFFu)], (uint32_t)i);
+// Each block folds a contiguous chunk into an LDS table for the chunk's first
+// scene (the array is scene-major, so other scenes appear only at chunk
+// boundaries and go straight to the global table).
+constexpr int kFoBlocks = 4096;
+__global__ void __launch_bounds__(256) k_leaf_firstocc_scene(const uint32_t* __restrict__ mask,
+                                                             const uint32_t* __restrict__ sid, uint64_t n,
+                                                             uint32_t* __restrict__ firstocc) {
+  __shared__ uint32_t fo[256];
+  fo[threadIdx.x] = kNone;
+  __syncthreads();
+  const uint64_t chunk = (n + gridDim.x - 1) / gridDim.x;
+  const uint64_t lo = (uint64_t)blockIdx.x * chunk, hi = lo + chunk < n ? lo + chunk : n;
+  const uint32_t s0 = lo < n ? sid[lo] : 0u;
+  for (uint64_t i = lo + threadIdx.x; i < hi; i += blockDim.x) {
+    const uint32_t m = mask[i] & 0xFFu, s = sid[i];
+    if (s == s0) {
+      if (fo[m] > (uint32_t)i) atomicMin(&fo[m], (uint32_t)i);
+    } else {
+      atomicMin(&firstocc[(uint64_t)s * 256 + m], (uint32_t)i);
+    }
+  }
+  __syncthreads();
+  const uint32_t v = fo[threadIdx.x];
+  if (lo < n && v != kNone) atomicMin(&firstocc[(uint64_t)s0 * 256 + threadIdx.x], v);
 }
 
 __global__ void k_leaf_repflag(const uint32_t* __restrict__ mask, const uint32_t* __restrict__ sid, uint64_t n,
@@ -204,7 +223,10 @@ void batch_build(svd_batch* b, const uint64_t* dcodes, uint64_t n, const std::ve
     uint32_t* pos = dalloc<uint32_t>(c, m + 1);
     uint32_t* total = dalloc<uint32_t>(c, 1);
     fill_u32(c, fo, kNone, (uint64_t)S * 256);
-    k_leaf_firstocc_scene<<<div_up(m, kB), kB, 0, c->stream>>>(sl.rec, sid[Lm - 1], m, fo);
+    if (m) {
+      const uint64_t nb = div_up(m, 256) < (uint64_t)kFoBlocks ? div_up(m, 256) : (uint64_t)kFoBlocks;
+      k_leaf_firstocc_scene<<<nb, 256, 0, c->stream>>>(sl.rec, sid[Lm - 1], m, fo);
+    }
     k_leaf_repflag<<<div_up(m, kB), kB, 0, c->stream>>>(sl.rec, sid[Lm - 1], m, fo, pos);
     scan_exclusive(c, pos, pos, m, total);
     const uint64_t d = read_u32(c, total);
