@@ -191,6 +191,73 @@ def batch_bench(S, W, ctx, n_scenes, log):
     return out
 
 
+def attr_bench(S, W, ctx, levels, log):
+    """Config-5 scene: fBm terrain (seed 1) union caves (seed 3) at 2^levels,
+    per-voxel u8 materials (value noise seed 4, 16 levels).  Cross-level merge
+    of the DAG, Gray vs binary attribute bit-trees, and a 3840x2160 frame
+    (primary + shadow rays) on the scene's lossless SSVDAG."""
+    import torch
+    t0 = time.perf_counter()
+    terr, cav = ctx.terrain(levels, seed=1), ctx.caves(levels, seed=3)
+    u = ctx.union(terr, cav)
+    terr.free()
+    cav.free()
+    log(f"[bench] attr scene {1 << levels}^3: {u.n} voxels ({time.perf_counter() - t0:.1f} s)")
+    out = {"workload": f"fBm terrain (seed 1) + caves (seed 3) {1 << levels}^3 ({levels} levels, {u.n} voxels), "
+                       f"u8 materials (value noise seed 4, 16 levels)", "voxels": int(u.n)}
+    g = S.Octree.from_morton(ctx, (u.ptr, u.n), levels)
+    g.to_dag()
+    out["nodes_dag"] = int(g.counts()[0])
+    g.encode(S.FMT_SVDAG)
+    out["svdag_bytes"] = int(g.encoded_device(S.FMT_SVDAG)[1])
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    elim = g.merge_cross_levels()
+    ctx.sync()
+    out["cross_level_ms"] = round((time.perf_counter() - t) * 1e3, 2)
+    out["cross_level_eliminated"] = int(elim)
+    g.encode(S.FMT_SVDAG)
+    out["csvdag_bytes"] = int(g.encoded_device(S.FMT_SVDAG)[1])
+    g.free()
+    mat = torch.zeros(max(u.n, 1), dtype=torch.uint8, device="cuda:0")
+    ctx.materials(u, levels, mat.data_ptr(), seed=4, n_values=16)
+    for tag, gray in (("gray", True), ("binary", False)):
+        blob = ctx.attr_bittree((mat.data_ptr(), u.n), gray)        # warm-up
+        ms = []
+        for _ in range(3):
+            blob = ctx.attr_bittree((mat.data_ptr(), u.n), gray)
+            ms.append(ctx.kernel_ms("attr"))
+        out[f"bittree_{tag}_bytes"] = len(blob)
+        out[f"bittree_{tag}_bits_per_voxel"] = round(8 * len(blob) / u.n, 4)
+        out[f"bittree_{tag}_ms"] = round(float(np.median(ms)), 3)
+    # traversal of the lossless SSVDAG of the same scene
+    g = S.Octree.from_morton(ctx, (u.ptr, u.n), levels)
+    g.to_dag()
+    g.to_sdag()
+    g.encode(S.FMT_SSVDAG)
+    sc = S.Scene.from_octree(g, S.FMT_SSVDAG)
+    w, h = 3840, 2160
+    p = W.trace_params(w, h, seed=5, levels=levels, shadows=True)
+    prim = torch.zeros(w * h * 6, dtype=torch.int32, device="cuda:0")
+    shad = torch.zeros(w * h * 6, dtype=torch.int32, device="cuda:0")
+    tile = ctx.autotune_trace(sc, p, prim.data_ptr(), shad.data_ptr(), reps=2)
+    tk = []
+    for _ in range(3):
+        sc.trace_device(p, prim.data_ptr(), shad.data_ptr())
+        ctx.sync()
+        tk.append(ctx.kernel_ms("trace"))
+    pr = prim.cpu().numpy().view(S.RAY_DTYPE)
+    rays = w * h + int((pr["t"] >= 0).sum())
+    out["trace_4k_ms"] = round(float(np.median(tk)), 3)
+    out["trace_4k_mrays_s"] = round(rays / (float(np.median(tk)) * 1e-3) / 1e6, 1)
+    out["trace_tile"] = f"{tile}x{256 // tile} (autotuned: {ctx.last_autotune['ms']})"
+    sc.free()
+    g.free()
+    u.free()
+    log(f"[bench] attr: {out}")
+    return out
+
+
 def measured_traffic(L, args):
     """HBM bytes per trace launch from the committed counter summary (two
     rocprofv3 --pmc runs of this workload, FETCH_SIZE and WRITE_SIZE, each
@@ -215,6 +282,8 @@ def main():
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-levels", type=int, default=11)
+    ap.add_argument("--attr-levels", type=int, default=12,
+                    help="grid of the cross-level + attribute bit-tree + 4K trace measurement; 0 skips it")
     ap.add_argument("--batch-scenes", type=int, default=256,
                     help="scenes of the multi-scene batch measurement; 0 skips it")
     ap.add_argument("--lossy-levels", type=int, default=12,
@@ -325,6 +394,8 @@ def main():
         out["lossy"] = lossy_bench(S, ctx, args.lossy_levels, log)
     if args.batch_scenes:
         out["batch"] = batch_bench(S, W, ctx, args.batch_scenes, log)
+    if args.attr_levels:
+        out["cross_level_attr"] = attr_bench(S, W, ctx, args.attr_levels, log)
     if not args.no_cpu_baseline:
         log("[bench] cpu baseline (oracle) ...")
         out["cpu_baseline"] = cpu_baseline(W, L, args.cpu_sample_levels)

@@ -305,6 +305,15 @@ int svd_workload_menger(svd_ctx* ctx, uint32_t levels, uint32_t order, uint64_t*
  * the lowest of its 4-neighbour heights up to its own height.               */
 int svd_workload_terrain(svd_ctx* ctx, uint32_t levels, uint64_t seed, uint32_t octaves, double hmin, double hmax,
                          double base_cells, uint64_t** out_dev, uint64_t* n);
+/* Per-voxel u8 material of n Morton codes (DEVICE) into out_dev (DEVICE):
+ * floor(value_noise3((voxel + 0.5) * base_cells / 2^levels) * n_values),
+ * clamped to n_values - 1 (workloads.material_attrs).                       */
+int svd_workload_materials(svd_ctx* ctx, const uint64_t* codes_dev, uint64_t n, uint32_t levels, uint64_t seed,
+                           uint32_t n_values, double base_cells, uint8_t* out_dev);
+/* Sorted union of two sorted unique code arrays (DEVICE); *out_dev receives a
+ * device array of *n codes (release with svd_device_free).                  */
+int svd_codes_union(svd_ctx* ctx, const uint64_t* a_dev, uint64_t na, const uint64_t* b_dev, uint64_t nb,
+                    uint64_t** out_dev, uint64_t* n);
 int svd_device_free(svd_ctx* ctx, void* dev_ptr);
 /* Copy bytes between two device arrays (on the context stream, synchronous). */
 int svd_device_copy(svd_ctx* ctx, void* dev_dst, const void* dev_src, uint64_t bytes);

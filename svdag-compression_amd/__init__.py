@@ -153,6 +153,8 @@ def lib():
         "svd_batch_svdag_copy": (i32, [vp, u32, vp, u64]),
         "svd_batch_trace": (i32, [vp, u32, vp, P(u32), vp, vp]),
         "svd_device_copy": (i32, [vp, vp, vp, u64]),
+        "svd_workload_materials": (i32, [vp, vp, u64, u32, u64, u32, C.c_double, vp]),
+        "svd_codes_union": (i32, [vp, vp, u64, vp, u64, P(vp), P(u64)]),
         "svd_set_trace_tile": (i32, [vp, u32]),
         "svd_get_trace_tile": (i32, [vp, P(u32)]),
         "svd_autotune_trace": (i32, [vp, P(TraceParams), vp, vp, u32, P(u32), P(C.c_double)]),
@@ -176,7 +178,8 @@ EXPORTED = ["svd_ctx_create", "svd_ctx_destroy", "svd_last_error", "svd_ctx_sync
             "svd_device_free", "svd_device_read", "svd_workload_balls", "svd_workload_menger",
             "svd_workload_terrain", "svd_batch_build", "svd_batch_free", "svd_batch_scene_info", "svd_batch_svdag",
             "svd_batch_svdag_copy", "svd_batch_trace", "svd_device_copy",
-            "svd_set_trace_tile", "svd_get_trace_tile", "svd_autotune_trace"]
+            "svd_set_trace_tile", "svd_get_trace_tile", "svd_autotune_trace", "svd_workload_materials",
+            "svd_codes_union"]
 
 
 class Context:
@@ -261,6 +264,18 @@ class Context:
         ptr, n = C.c_void_p(), C.c_uint64()
         self.check(self.L.svd_workload_terrain(self.h, levels, seed, octaves, hmin, hmax, base_cells, C.byref(ptr),
                                                C.byref(n)))
+        return DeviceCodes(self, ptr.value, n.value)
+
+    def materials(self, codes: "DeviceCodes", levels: int, out_ptr: int, seed: int = 4, n_values: int = 16,
+                  base_cells: float = 8.0):
+        """Per-voxel u8 materials of device codes into a device buffer (workloads.material_attrs)."""
+        self.check(self.L.svd_workload_materials(self.h, C.c_void_p(codes.ptr), codes.n, levels, seed, n_values,
+                                                 base_cells, C.c_void_p(out_ptr)))
+
+    def union(self, a: "DeviceCodes", b: "DeviceCodes") -> "DeviceCodes":
+        ptr, n = C.c_void_p(), C.c_uint64()
+        self.check(self.L.svd_codes_union(self.h, C.c_void_p(a.ptr), a.n, C.c_void_p(b.ptr), b.n, C.byref(ptr),
+                                          C.byref(n)))
         return DeviceCodes(self, ptr.value, n.value)
 
     def spec_codes(self, spec) -> "DeviceCodes":

@@ -800,3 +800,15 @@ int svd_autotune_trace(svd_scene* s, const svd_trace_params* p, svd_ray_result* 
     if (ms) for (int k = 0; k < 4; ++k) ms[k] = t[k];
   });
 }
+
+int svd_workload_materials(svd_ctx* c, const uint64_t* codes, uint64_t n, uint32_t levels, uint64_t seed,
+                           uint32_t nValues, double baseCells, uint8_t* out) {
+  if (!c || (n && (!codes || !out)) || nValues < 1 || nValues > 256 || levels < 1 || levels > 21) return SVD_ERR_ARG;
+  return guarded(c, [&] { gen_materials(c, codes, n, levels, seed, nValues, baseCells, out); });
+}
+
+int svd_codes_union(svd_ctx* c, const uint64_t* a, uint64_t na, const uint64_t* b, uint64_t nb, uint64_t** out,
+                    uint64_t* n) {
+  if (!c || !out || !n || (na && !a) || (nb && !b)) return SVD_ERR_ARG;
+  return guarded(c, [&] { *n = codes_union(c, a, na, b, nb, out); });
+}

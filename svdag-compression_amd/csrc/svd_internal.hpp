@@ -212,6 +212,9 @@ void trace_batch_svdag(svd_ctx* c, const std::vector<svd_scene>& scenes, const s
 uint64_t gen_caves(svd_ctx* c, uint32_t levels, uint64_t seed, uint32_t octaves, double threshold, double baseCells,
                    uint64_t** out);
 uint64_t gen_balls(svd_ctx* c, uint32_t levels, const double* balls, uint32_t nBalls, uint64_t** out);
+void gen_materials(svd_ctx* c, const uint64_t* codes, uint64_t n, uint32_t levels, uint64_t seed, uint32_t nValues,
+                   double baseCells, uint8_t* out);
+uint64_t codes_union(svd_ctx* c, const uint64_t* a, uint64_t na, const uint64_t* b, uint64_t nb, uint64_t** out);
 uint64_t gen_menger(svd_ctx* c, uint32_t levels, uint32_t order, uint64_t** out);
 uint64_t gen_terrain(svd_ctx* c, uint32_t levels, uint64_t seed, uint32_t octaves, double hmin, double hmax,
                      double baseCells, uint64_t** out);

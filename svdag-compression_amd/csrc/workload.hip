@@ -17,6 +17,8 @@
 // voxel, and the wave ballots pack the brick's 512 surface bits (64 B); a
 // popcount scan then emits the codes in ascending Morton order.
 #include "svd_internal.hpp"
+#include <rocprim/device/device_merge.hpp>
+#include <rocprim/device/device_select.hpp>
 
 namespace svd {
 
@@ -288,7 +290,58 @@ __global__ void k_brick_emit(const uint64_t* __restrict__ masks, uint64_t brick0
   }
 }
 
+// workloads.material_attrs: quantized 3D value noise at the voxel centre.
+__global__ void k_materials(const uint64_t* __restrict__ codes, uint64_t n, double step, uint64_t seedMix,
+                            uint32_t nValues, uint8_t* __restrict__ out) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint64_t c = codes[i];
+  uint32_t x = 0, y = 0, z = 0;
+  for (int b = 0; b < 21; ++b) {
+    x |= (uint32_t)((c >> (3 * b + 2)) & 1u) << b;
+    y |= (uint32_t)((c >> (3 * b + 1)) & 1u) << b;
+    z |= (uint32_t)((c >> (3 * b)) & 1u) << b;
+  }
+  const double v = value_noise3(((double)x + 0.5) * step, ((double)y + 0.5) * step, ((double)z + 0.5) * step, seedMix);
+  double q = floor(v * (double)nValues);
+  if (q > (double)(nValues - 1)) q = (double)(nValues - 1);
+  out[i] = (uint8_t)q;
+}
+
 }  // namespace
+
+void gen_materials(svd_ctx* c, const uint64_t* codes, uint64_t n, uint32_t levels, uint64_t seed, uint32_t nValues,
+                   double baseCells, uint8_t* out) {
+  if (!n) return;
+  const double step = baseCells / (double)(1ull << levels);
+  k_materials<<<div_up(n, 256), 256, 0, c->stream>>>(codes, n, step, seed * 0x9E3779B97F4A7C15ull, nValues, out);
+  check_launch("k_materials");
+  SVD_HIP(hipStreamSynchronize(c->stream));
+}
+
+// Sorted union of two sorted unique code lists (rocPRIM merge + unique).
+uint64_t codes_union(svd_ctx* c, const uint64_t* a, uint64_t na, const uint64_t* b, uint64_t nb, uint64_t** out) {
+  uint64_t* merged = dalloc<uint64_t>(c, na + nb + 1);
+  size_t bytes = 0;
+  SVD_HIP(rocprim::merge(nullptr, bytes, a, b, merged, (size_t)na, (size_t)nb, rocprim::less<uint64_t>(), c->stream));
+  void* tmp = c->arena->get(bytes + 16);
+  SVD_HIP(rocprim::merge(tmp, bytes, a, b, merged, (size_t)na, (size_t)nb, rocprim::less<uint64_t>(), c->stream));
+  c->arena->put(tmp);
+  uint64_t* res = nullptr;
+  SVD_HIP(hipMalloc(&res, (na + nb + 1) * 8));
+  uint64_t* cnt = dalloc<uint64_t>(c, 1);
+  bytes = 0;
+  SVD_HIP(rocprim::unique(nullptr, bytes, merged, res, cnt, (size_t)(na + nb), rocprim::equal_to<uint64_t>(), c->stream));
+  tmp = c->arena->get(bytes + 16);
+  SVD_HIP(rocprim::unique(tmp, bytes, merged, res, cnt, (size_t)(na + nb), rocprim::equal_to<uint64_t>(), c->stream));
+  c->arena->put(tmp);
+  uint64_t n = 0;
+  SVD_HIP(hipMemcpyAsync(&n, cnt, 8, hipMemcpyDeviceToHost, c->stream));
+  SVD_HIP(hipStreamSynchronize(c->stream));
+  dfree(c, merged); dfree(c, cnt);
+  *out = res;
+  return n;
+}
 
 // Walk all bricks of the 2^levels grid with `launch(b0, nb, masks)` writing
 // their surface bits; returns a device array (hipMalloc; free with hipFree)
