@@ -325,6 +325,11 @@ int svd_device_read(svd_ctx* ctx, void* host_dst, const void* dev_src, uint64_t 
  * produces in EncodedSSVDAG::encode (encoded_ssvdag.cpp:273-275), computed on
  * the device.  idx / val are HOST arrays of n pairs, sorted in place.        */
 int svd_refcount_order(svd_ctx* ctx, uint32_t* idx, uint32_t* val, uint64_t n);
+/* The same order applied to nr disjoint ranges [first[r], last[r]) of one pair
+ * array at once, each range its own std::sort call (the per-level sorts of
+ * encoded_ssvdag.cpp:257-275 run together).  Arrays on the HOST.            */
+int svd_refcount_order_ranges(svd_ctx* ctx, uint32_t* idx, uint32_t* val, uint64_t n, const uint64_t* first,
+                              const uint64_t* last, uint64_t nr);
 
 /* ---- instrumentation ------------------------------------------------------ */
 /* Average duration in milliseconds of the last launch of the named kernel

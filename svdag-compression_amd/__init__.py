@@ -129,6 +129,7 @@ def lib():
         "svd_trace": (i32, [vp, P(TraceParams), vp, vp]),
         "svd_last_kernel_ms": (i32, [vp, C.c_char_p, P(C.c_double)]),
         "svd_refcount_order": (i32, [vp, vp, vp, u64]),
+        "svd_refcount_order_ranges": (i32, [vp, vp, vp, u64, vp, vp, u64]),
         "svd_attr_bittree_encode": (i32, [vp, vp, u64, i32, i32, vp, u64, P(u64)]),
         "svd_lossy_merge": (i32, [vp, P(LossyParams), vp, u32]),
         "svd_lossy_signatures": (i32, [vp, u32, vp]),
@@ -172,7 +173,7 @@ EXPORTED = ["svd_ctx_create", "svd_ctx_destroy", "svd_last_error", "svd_ctx_sync
             "svd_reduce_sym", "svd_cross_level_merge", "svd_octree_state", "svd_octree_levels",
             "svd_octree_counts", "svd_level_size", "svd_get_level", "svd_encode", "svd_encoded_copy",
             "svd_encoded_device", "svd_scene_create", "svd_scene_from_octree", "svd_scene_free", "svd_trace",
-            "svd_last_kernel_ms", "svd_refcount_order", "svd_attr_bittree_encode", "svd_lossy_merge",
+            "svd_last_kernel_ms", "svd_refcount_order", "svd_refcount_order_ranges", "svd_attr_bittree_encode", "svd_lossy_merge",
             "svd_lossy_signatures", "svd_mlp_create", "svd_mlp_free", "svd_mlp_get_params", "svd_mlp_set_params",
             "svd_mlp_embed", "svd_mlp_loss_grad", "svd_mlp_train", "svd_knn", "svd_workload_caves",
             "svd_device_free", "svd_device_read", "svd_workload_balls", "svd_workload_menger",
@@ -215,6 +216,17 @@ class Context:
         v = np.ascontiguousarray(val, dtype=np.uint32).copy()
         self.check(self.L.svd_refcount_order(self.h, i.ctypes.data_as(C.c_void_p), v.ctypes.data_as(C.c_void_p),
                                              i.size))
+        return i, v
+
+    def refcount_order_ranges(self, idx, val, ranges):
+        """refcount_order applied to each [first, last) range independently, in one device sort."""
+        i = np.ascontiguousarray(idx, dtype=np.uint32).copy()
+        v = np.ascontiguousarray(val, dtype=np.uint32).copy()
+        f = np.ascontiguousarray([r[0] for r in ranges], dtype=np.uint64)
+        l = np.ascontiguousarray([r[1] for r in ranges], dtype=np.uint64)
+        self.check(self.L.svd_refcount_order_ranges(self.h, i.ctypes.data_as(C.c_void_p), v.ctypes.data_as(C.c_void_p),
+                                                    i.size, f.ctypes.data_as(C.c_void_p),
+                                                    l.ctypes.data_as(C.c_void_p), f.size))
         return i, v
 
     def attr_bittree(self, attrs, gray: bool) -> bytes:

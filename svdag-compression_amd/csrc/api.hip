@@ -507,6 +507,24 @@ int svd_refcount_order(svd_ctx* c, uint32_t* idx, uint32_t* val, uint64_t n) {
   });
 }
 
+int svd_refcount_order_ranges(svd_ctx* c, uint32_t* idx, uint32_t* val, uint64_t n, const uint64_t* first,
+                              const uint64_t* last, uint64_t nr) {
+  if (!c || (n && (!idx || !val)) || (nr && (!first || !last))) return SVD_ERR_ARG;
+  return guarded(c, [&] {
+    if (!n) return;
+    uint32_t* di = dalloc<uint32_t>(c, n);
+    uint32_t* dv = dalloc<uint32_t>(c, n);
+    SVD_HIP(hipMemcpyAsync(di, idx, 4 * n, hipMemcpyHostToDevice, c->stream));
+    SVD_HIP(hipMemcpyAsync(dv, val, 4 * n, hipMemcpyHostToDevice, c->stream));
+    refsort_ranges(c, di, dv, n, first, last, (size_t)nr);
+    SVD_HIP(hipMemcpyAsync(idx, di, 4 * n, hipMemcpyDeviceToHost, c->stream));
+    SVD_HIP(hipMemcpyAsync(val, dv, 4 * n, hipMemcpyDeviceToHost, c->stream));
+    SVD_HIP(hipStreamSynchronize(c->stream));
+    dfree(c, di);
+    dfree(c, dv);
+  });
+}
+
 int svd_attr_bittree_encode(svd_ctx* c, const uint8_t* attrs, uint64_t n, int gray, int on_device, void* out,
                             uint64_t cap, uint64_t* bytes) {
   if (!c || (n && !attrs) || !bytes) return SVD_ERR_ARG;

@@ -49,18 +49,44 @@ __global__ void k_write_words(const uint32_t* __restrict__ rec, uint64_t n, int 
 }
 
 // ---------------------------------------------------------------- SSVDAG
-__global__ void k_refcount(const uint32_t* __restrict__ parent, uint64_t np, uint32_t* __restrict__ hist,
-                           uint64_t nChild, uint32_t* __restrict__ err) {
-  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= np) return;
-  const uint32_t* r = parent + i * kRecWords;
+// Reference counts of the children of a level.  A few children (common
+// bricks) take most references, so each block first aggregates in an LDS
+// hash table (1024 slots, at most 8 probes; a child that finds no slot goes
+// straight to the global histogram) and flushes one add per distinct child.
+constexpr int kRcSlots = 1024;
+constexpr int kRcBlocks = 2048;
+__global__ void __launch_bounds__(256) k_refcount(const uint32_t* __restrict__ parent, uint64_t np,
+                                                  uint32_t* __restrict__ hist, uint64_t nChild,
+                                                  uint32_t* __restrict__ err) {
+  __shared__ uint32_t key[kRcSlots], cnt[kRcSlots];
+  for (int t = threadIdx.x; t < kRcSlots; t += blockDim.x) { key[t] = kNone; cnt[t] = 0; }
+  __syncthreads();
+  const uint64_t chunk = (np + gridDim.x - 1) / gridDim.x;
+  const uint64_t lo = (uint64_t)blockIdx.x * chunk, hi = lo + chunk < np ? lo + chunk : np;
+  for (uint64_t i = lo + threadIdx.x; i < hi; i += blockDim.x) {
+    const uint32_t* r = parent + i * kRecWords;
 #pragma unroll
-  for (int c = 0; c < 8; ++c) {
-    const uint32_t ch = r[1 + c];
-    if (ch == kNull) continue;
-    if (ch < nChild) atomicAdd(&hist[ch], 1u);
-    else atomicAdd(err, 1u);
+    for (int c = 0; c < 8; ++c) {
+      const uint32_t ch = r[1 + c];
+      if (ch == kNull) continue;
+      if (ch >= nChild) { atomicAdd(err, 1u); continue; }
+      uint32_t h = (ch * 2654435761u) >> 22;                 // 10-bit slot
+      bool done = false;
+#pragma unroll 1
+      for (int p = 0; p < 8 && !done; ++p, h = (h + 1) & (kRcSlots - 1)) {
+        uint32_t k = key[h];
+        if (k == kNone) {
+          const uint32_t old = atomicCAS(&key[h], kNone, ch);
+          k = old == kNone ? ch : old;
+        }
+        if (k == ch) { atomicAdd(&cnt[h], 1u); done = true; }
+      }
+      if (!done) atomicAdd(&hist[ch], 1u);
+    }
   }
+  __syncthreads();
+  for (int t = threadIdx.x; t < kRcSlots; t += blockDim.x)
+    if (key[t] != kNone && cnt[t]) atomicAdd(&hist[key[t]], cnt[t]);
 }
 
 __global__ void k_scatter_rank(const uint32_t* __restrict__ perm, uint64_t n, uint32_t* __restrict__ ind) {
@@ -246,24 +272,37 @@ void encode_ssvdag(svd_octree* o, int fmt) {
     if (lv[l].n > (1ull << 30)) throw Error{SVD_ERR_CAPACITY, "level too big for encoding in 30 bits"};
   std::vector<uint32_t*> perm(L, nullptr), ind(L, nullptr), offs(L, nullptr);
   std::vector<uint32_t> innerSize(L, 0);
-  uint32_t* err = dalloc<uint32_t>(c, 1);
-  uint32_t* total = dalloc<uint32_t>(c, 1);
-  fill_u32(c, err, 0u, 1);
-  // Phase 1, bottom-up: order of each level and its children addresses.
-  for (int lev = L - 2; lev >= 0; --lev) {
+  // dev[0..L-1]: inner-size totals per level, dev[L]: pointer-size error flag
+  uint32_t* dev = dalloc<uint32_t>(c, L + 1);
+  uint32_t* err = dev + L;
+  fill_u32(c, dev, 0u, L + 1);
+  // Phase 1a: the reference counts of every level land in one array, and one
+  // segmented sort orders all levels (each level is its own std::sort call).
+  std::vector<uint64_t> off(L, 0), rfirst, rlast;
+  uint64_t tot = 0;
+  for (int lev = 0; lev <= L - 2; ++lev) { off[lev] = tot; tot += lv[lev].n; }
+  uint32_t* permAll = dalloc<uint32_t>(c, tot + 1);
+  uint32_t* hist = dalloc<uint32_t>(c, tot + 1);
+  fill_u32(c, hist, 0u, tot);
+  for (int lev = 0; lev <= L - 2; ++lev) {
     const uint64_t n = lv[lev].n;
-    perm[lev] = dalloc<uint32_t>(c, n + 1);
-    ind[lev] = dalloc<uint32_t>(c, n + 1);
+    perm[lev] = permAll + off[lev];
     iota_u32(c, perm[lev], n);
     if (lev > 0 && n > 0) {
-      uint32_t* hist = dalloc<uint32_t>(c, n + 1);
-      fill_u32(c, hist, 0u, n);
       if (lv[lev - 1].n)
-        k_refcount<<<div_up(lv[lev - 1].n, kB), kB, 0, c->stream>>>(lv[lev - 1].rec, lv[lev - 1].n, hist, n, err);
+        k_refcount<<<std::min<uint64_t>(div_up(lv[lev - 1].n, kB), kRcBlocks), kB, 0, c->stream>>>(
+            lv[lev - 1].rec, lv[lev - 1].n, hist + off[lev], n, err);
       check_launch("k_refcount");
-      refsort(c, perm[lev], hist, n);
-      dfree(c, hist);
+      rfirst.push_back(off[lev]);
+      rlast.push_back(off[lev] + n);
     }
+  }
+  refsort_ranges(c, permAll, hist, tot, rfirst.data(), rlast.data(), rfirst.size());
+  dfree(c, hist);
+  // Phase 1b, bottom-up: children addresses of each level in its order.
+  for (int lev = L - 2; lev >= 0; --lev) {
+    const uint64_t n = lv[lev].n;
+    ind[lev] = dalloc<uint32_t>(c, n + 1);
     if (lev == L - 2) {
       if (n) k_scatter_rank<<<div_up(n, kB), kB, 0, c->stream>>>(perm[lev], n, ind[lev]);
       check_launch("k_scatter_rank");
@@ -273,13 +312,17 @@ void encode_ssvdag(svd_octree* o, int fmt) {
         k_inner_sizes<<<div_up(n, kB), kB, 0, c->stream>>>(lv[lev].rec, perm[lev], n, ind[lev + 1], offs[lev], err);
         check_launch("k_inner_sizes");
       }
-      scan_exclusive(c, offs[lev], offs[lev], n, total);
+      scan_exclusive(c, offs[lev], offs[lev], n, dev + lev);
       if (n) k_offsets_to_ind<<<div_up(n, kB), kB, 0, c->stream>>>(perm[lev], offs[lev], n, ind[lev]);
       check_launch("k_offsets_to_ind");
-      innerSize[lev] = (uint32_t)read_u32(c, total);
     }
   }
-  if (read_u32(c, err)) throw Error{SVD_ERR_CAPACITY, "unsupported pointer size (>= 2^30)"};
+  {
+    std::vector<uint32_t> h(L + 1);
+    read_u32s(c, dev, h.data(), L + 1);
+    if (h[L]) throw Error{SVD_ERR_CAPACITY, "unsupported pointer size (>= 2^30)"};
+    for (int lev = 0; lev < L - 2; ++lev) innerSize[lev] = h[lev];
+  }
   // Layout of the file image.
   std::vector<uint32_t> levelOffsets(L - 2, 0);
   for (int i = 1; i < L - 2; ++i) levelOffsets[i] = innerSize[i - 1] + levelOffsets[i - 1];
@@ -320,8 +363,8 @@ void encode_ssvdag(svd_octree* o, int fmt) {
                                                               (uint16_t*)(e.d + leafAt));
   check_launch("ssvdag write");
   SVD_HIP(hipStreamSynchronize(c->stream));   // host staging vectors go out of scope
-  for (int l = 0; l < L; ++l) { dfree(c, perm[l]); dfree(c, ind[l]); dfree(c, offs[l]); }
-  dfree(c, err); dfree(c, total);
+  for (int l = 0; l < L; ++l) { dfree(c, ind[l]); dfree(c, offs[l]); }
+  dfree(c, permAll); dfree(c, dev);
   e.valid = true;
 }
 

@@ -14,6 +14,7 @@
 // together.  The final insertion sort never moves an element across a
 // partition boundary, so it reduces to a stable insertion sort of each
 // <=16-element leaf segment.
+#include <algorithm>
 #include "svd_internal.hpp"
 #include "block_scan.hpp"
 
@@ -137,11 +138,15 @@ __device__ __forceinline__ void classify(const Lists& L, uint32_t first, uint32_
 // ---------------------------------------------------------------- device-wide rounds
 struct SegState { uint32_t pivot, K, nL, nR, blk0, nblk, cut, pad; };
 
-__global__ void k_g_median(const Seg* __restrict__ segs, uint32_t ns, uint32_t* val, uint32_t* idx,
-                           SegState* __restrict__ st, uint32_t* __restrict__ nblk) {
+// nblk has cap+1 entries; every entry at or past the live count is zeroed so
+// that a scan over all of them ends in the total tile count.
+__global__ void k_g_median(const Seg* __restrict__ segs, const uint32_t* __restrict__ nsp, uint32_t cap,
+                           uint32_t* val, uint32_t* idx, SegState* __restrict__ st, uint32_t* __restrict__ nblk,
+                           uint32_t* __restrict__ nnext) {
   uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s > ns) return;
-  if (s == ns) { nblk[s] = 0; return; }
+  if (s == 0) *nnext = 0;   // the list this round's splits append to
+  if (s > cap) return;
+  if (s >= *nsp) { nblk[s] = 0; return; }
   const Seg g = segs[s];
   auto vf = [&](int64_t k) { return val[k]; };
   const int64_t m = median_pos(vf, g.first, g.last);
@@ -166,13 +171,16 @@ __device__ __forceinline__ int find_seg(const uint32_t* __restrict__ blk0, uint3
 }
 
 // Per-tile counts of left/right stoppers.
-__global__ void __launch_bounds__(kBT) k_g_count(const Seg* __restrict__ segs, uint32_t ns,
+__global__ void __launch_bounds__(kBT) k_g_count(const Seg* __restrict__ segs, const uint32_t* __restrict__ nsp,
                                                  const uint32_t* __restrict__ blk0, const SegState* __restrict__ st,
                                                  const uint32_t* __restrict__ val, uint32_t* __restrict__ cntL,
                                                  uint32_t* __restrict__ cntR) {
   __shared__ uint32_t lds[kBT / 64 + 1];
-  const int s = find_seg(blk0, ns, blockIdx.x);
-  if (s < 0) return;
+  const int s = find_seg(blk0, *nsp, blockIdx.x);
+  if (s < 0) {   // tiles past the live ones (and the trailing entry) count zero
+    if (threadIdx.x == 0) { cntL[blockIdx.x] = 0; cntR[blockIdx.x] = 0; }
+    return;
+  }
   const Seg g = segs[s];
   const uint32_t p = st[s].pivot;
   const uint64_t base = (uint64_t)g.first + (uint64_t)(blockIdx.x - blk0[s]) * kTile;
@@ -191,13 +199,13 @@ __global__ void __launch_bounds__(kBT) k_g_count(const Seg* __restrict__ segs, u
 }
 
 // Rank every stopper inside its segment; Lpos[first+k] / Rpos[first+k] = k-th stop.
-__global__ void __launch_bounds__(kBT) k_g_scatter(const Seg* __restrict__ segs, uint32_t ns,
+__global__ void __launch_bounds__(kBT) k_g_scatter(const Seg* __restrict__ segs, const uint32_t* __restrict__ nsp,
                                                    const uint32_t* __restrict__ blk0, SegState* __restrict__ st,
                                                    const uint32_t* __restrict__ val, const uint32_t* __restrict__ exL,
                                                    const uint32_t* __restrict__ exR, uint32_t* __restrict__ Lpos,
                                                    uint32_t* __restrict__ Rpos) {
   __shared__ uint32_t lds[kBT / 64 + 1];
-  const int s = find_seg(blk0, ns, blockIdx.x);
+  const int s = find_seg(blk0, *nsp, blockIdx.x);
   if (s < 0) return;
   const Seg g = segs[s];
   const uint32_t p = st[s].pivot;
@@ -226,11 +234,11 @@ __global__ void __launch_bounds__(kBT) k_g_scatter(const Seg* __restrict__ segs,
 }
 
 // Swap the k-th left stop with the k-th right stop while L_k < R_k; record K.
-__global__ void __launch_bounds__(kBT) k_g_swap(const Seg* __restrict__ segs, uint32_t ns,
+__global__ void __launch_bounds__(kBT) k_g_swap(const Seg* __restrict__ segs, const uint32_t* __restrict__ nsp,
                                                 const uint32_t* __restrict__ blk0, SegState* __restrict__ st,
                                                 uint32_t* val, uint32_t* idx, const uint32_t* __restrict__ Lpos,
                                                 const uint32_t* __restrict__ Rpos) {
-  const int s = find_seg(blk0, ns, blockIdx.x);
+  const int s = find_seg(blk0, *nsp, blockIdx.x);
   if (s < 0) return;
   const Seg g = segs[s];
   const uint32_t m = min(st[s].nL, st[s].nR);
@@ -249,10 +257,11 @@ __global__ void __launch_bounds__(kBT) k_g_swap(const Seg* __restrict__ segs, ui
   }
 }
 
-__global__ void k_g_split(const Seg* __restrict__ segs, uint32_t ns, const SegState* __restrict__ st,
-                          const uint32_t* __restrict__ Lpos, const uint32_t* __restrict__ Rpos, Lists L) {
+__global__ void k_g_split(const Seg* __restrict__ segs, const uint32_t* __restrict__ nsp,
+                          const SegState* __restrict__ st, const uint32_t* __restrict__ Lpos,
+                          const uint32_t* __restrict__ Rpos, Lists L) {
   uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= ns) return;
+  if (s >= *nsp) return;
   const Seg g = segs[s];
   const SegState z = st[s];
   const uint32_t rprev = z.K ? Rpos[g.first + z.K - 1] : g.last;
@@ -430,12 +439,27 @@ __global__ void __launch_bounds__(kBT) k_block_sort(const Seg* __restrict__ segs
 
 }  // namespace
 
-void refsort(svd_ctx* c, uint32_t* idx, uint32_t* val, uint64_t n) {
-  if (n <= 1) return;
-  if (n > 0xFFFFFFF0ull) throw Error{SVD_ERR_CAPACITY, "refsort: level too large"};
-  int lg = 63 - __builtin_clzll(n);
-  const uint64_t capBig = n / kLdsN + 2;
-  const uint64_t capSeg = n + 2;
+void refsort_ranges(svd_ctx* c, uint32_t* idx, uint32_t* val, uint64_t n, const uint64_t* rfirst,
+                    const uint64_t* rlast, size_t nr) {
+  if (n > 0xFFFFFFF0ull) throw Error{SVD_ERR_CAPACITY, "refsort: array too large"};
+  // Each range is its own std::sort call: own depth budget 2*floor(log2 size).
+  std::vector<Seg> roots[4];   // big, small, (heap), leaf
+  std::vector<std::pair<uint64_t, uint64_t>> chk(nr);
+  for (size_t r = 0; r < nr; ++r) chk[r] = {rfirst[r], rlast[r]};
+  std::sort(chk.begin(), chk.end());
+  for (size_t r = 0; r < nr; ++r)
+    if (chk[r].first > chk[r].second || chk[r].second > n || (r && chk[r].first < chk[r - 1].second))
+      throw Error{SVD_ERR_ARG, "refsort: ranges must be disjoint and inside the array"};
+  for (size_t r = 0; r < nr; ++r) {
+    const uint64_t size = rlast[r] - rfirst[r];
+    if (size <= 1) continue;
+    const int lg = 63 - __builtin_clzll(size);
+    const Seg s{(uint32_t)rfirst[r], (uint32_t)rlast[r], 2 * lg, 0};
+    roots[size <= (uint64_t)kThreshold ? 3 : size <= (uint64_t)kLdsN ? 1 : 0].push_back(s);
+  }
+  if (roots[0].empty() && roots[1].empty() && roots[3].empty()) return;
+  const uint64_t capBig = n / kLdsN + nr + 2;
+  const uint64_t capSeg = n + nr + 2;
   Lists L;
   Seg* bigA = dalloc<Seg>(c, capBig);
   Seg* bigB = dalloc<Seg>(c, capBig);
@@ -444,44 +468,53 @@ void refsort(svd_ctx* c, uint32_t* idx, uint32_t* val, uint64_t n) {
   L.leaf = dalloc<Seg>(c, capSeg);
   uint32_t* counters = dalloc<uint32_t>(c, 8);
   L.nbig = counters + 0; L.nsmall = counters + 1; L.nheap = counters + 2; L.nleaf = counters + 3;
-  fill_u32(c, counters, 0u, 8);
-  // root segment classification (host knows n)
-  Seg root{0, (uint32_t)n, 2 * lg, 0};
-  uint32_t hcnt[4] = {0, 0, 0, 0};
-  Seg* dst = nullptr;
-  if (n <= (uint64_t)kThreshold) { dst = L.leaf; hcnt[3] = 1; }
-  else if (n <= (uint64_t)kLdsN) { dst = L.small; hcnt[1] = 1; }
-  else { dst = bigA; hcnt[0] = 1; }
-  SVD_HIP(hipMemcpyAsync(dst, &root, sizeof(Seg), hipMemcpyHostToDevice, c->stream));
+  uint32_t hcnt[8] = {(uint32_t)roots[0].size(), (uint32_t)roots[1].size(), 0, (uint32_t)roots[3].size(), 0, 0, 0, 0};
+  Seg* dstOf[4] = {bigA, L.small, L.heap, L.leaf};
+  for (int k = 0; k < 4; ++k)
+    if (!roots[k].empty())
+      SVD_HIP(hipMemcpyAsync(dstOf[k], roots[k].data(), sizeof(Seg) * roots[k].size(), hipMemcpyHostToDevice,
+                             c->stream));
   SVD_HIP(hipMemcpyAsync(counters, hcnt, sizeof(hcnt), hipMemcpyHostToDevice, c->stream));
-  uint32_t nbig = hcnt[0];
-  if (nbig) {
+  if (hcnt[0]) {
     uint32_t* Lpos = dalloc<uint32_t>(c, n);
     uint32_t* Rpos = dalloc<uint32_t>(c, n);
     SegState* st = dalloc<SegState>(c, capBig);
     uint32_t* blk0 = dalloc<uint32_t>(c, capBig + 1);
-    const uint64_t maxBlocks = n / kTile + capBig + 2;
-    uint32_t* cntL = dalloc<uint32_t>(c, maxBlocks + 1);
-    uint32_t* cntR = dalloc<uint32_t>(c, maxBlocks + 1);
+    // tiles of one round: sum of ceil(size / kTile) <= n / kTile + live segments
+    const uint32_t G = (uint32_t)(n / kTile + capBig + 1);
+    uint32_t* cntL = dalloc<uint32_t>(c, G + 1);
+    uint32_t* cntR = dalloc<uint32_t>(c, G + 1);
+    const uint32_t capB = (uint32_t)capBig;
     Seg* cur = bigA;
     Seg* nxt = bigB;
-    while (nbig) {
-      L.big = nxt;
-      fill_u32(c, L.nbig, 0u, 1);
-      k_g_median<<<div_up(nbig + 1, 256), 256, 0, c->stream>>>(cur, nbig, val, idx, st, blk0);
-      scan_exclusive(c, blk0, blk0, nbig + 1, nullptr);
-      const uint32_t grid = (uint32_t)(n / kTile + nbig + 1);
-      fill_u32(c, cntL, 0u, grid + 1);
-      fill_u32(c, cntR, 0u, grid + 1);
-      k_g_count<<<grid, kBT, 0, c->stream>>>(cur, nbig, blk0, st, val, cntL, cntR);
-      scan_exclusive(c, cntL, cntL, grid + 1, nullptr);
-      scan_exclusive(c, cntR, cntR, grid + 1, nullptr);
-      k_g_scatter<<<grid, kBT, 0, c->stream>>>(cur, nbig, blk0, st, val, cntL, cntR, Lpos, Rpos);
-      k_g_swap<<<grid, kBT, 0, c->stream>>>(cur, nbig, blk0, st, val, idx, Lpos, Rpos);
-      k_g_split<<<div_up(nbig, 256), 256, 0, c->stream>>>(cur, nbig, st, Lpos, Rpos, L);
-      check_launch("refsort round");
-      nbig = (uint32_t)read_u32(c, L.nbig);
-      std::swap(cur, nxt);
+    uint32_t* ncur = counters + 0;
+    uint32_t* nnxt = counters + 4;
+    // Every round leaves a piece of at least half the largest segment, so the
+    // first ceil(log2(max / kLdsN)) rounds are all needed: run them without
+    // looking at the list size, then check it every second round.  A round
+    // over an empty list changes nothing.
+    uint64_t maxRoot = 0;
+    for (const Seg& r : roots[0]) maxRoot = std::max<uint64_t>(maxRoot, r.last - r.first);
+    int batch = 0;
+    while (((uint64_t)kLdsN << batch) < maxRoot) ++batch;
+    while (true) {
+      for (int r = 0; r < batch; ++r) {
+        L.big = nxt;
+        L.nbig = nnxt;
+        k_g_median<<<div_up(capB + 1, 256), 256, 0, c->stream>>>(cur, ncur, capB, val, idx, st, blk0, nnxt);
+        scan_exclusive(c, blk0, blk0, capB + 1, nullptr);
+        k_g_count<<<G, kBT, 0, c->stream>>>(cur, ncur, blk0, st, val, cntL, cntR);
+        scan_exclusive(c, cntL, cntL, G, nullptr);
+        scan_exclusive(c, cntR, cntR, G, nullptr);
+        k_g_scatter<<<G, kBT, 0, c->stream>>>(cur, ncur, blk0, st, val, cntL, cntR, Lpos, Rpos);
+        k_g_swap<<<G, kBT, 0, c->stream>>>(cur, ncur, blk0, st, val, idx, Lpos, Rpos);
+        k_g_split<<<div_up(capB, 256), 256, 0, c->stream>>>(cur, ncur, st, Lpos, Rpos, L);
+        check_launch("refsort round");
+        std::swap(cur, nxt);
+        std::swap(ncur, nnxt);
+      }
+      if (!read_u32(c, ncur)) break;
+      batch = 2;
     }
     dfree(c, Lpos); dfree(c, Rpos); dfree(c, st); dfree(c, blk0); dfree(c, cntL); dfree(c, cntR);
   }
@@ -493,6 +526,11 @@ void refsort(svd_ctx* c, uint32_t* idx, uint32_t* val, uint64_t n) {
   if (h[3]) k_leaf_sort<<<div_up(h[3], 64), 64, 0, c->stream>>>(L.leaf, L.nleaf, val, idx);
   check_launch("refsort finish");
   dfree(c, bigA); dfree(c, bigB); dfree(c, L.small); dfree(c, L.heap); dfree(c, L.leaf); dfree(c, counters);
+}
+
+void refsort(svd_ctx* c, uint32_t* idx, uint32_t* val, uint64_t n) {
+  const uint64_t f = 0;
+  refsort_ranges(c, idx, val, n, &f, &n, 1);
 }
 
 }  // namespace svd

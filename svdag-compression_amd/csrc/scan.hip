@@ -63,6 +63,37 @@ __global__ void __launch_bounds__(kThreads) k_scan_down(const uint32_t* __restri
   }
 }
 
+// Small inputs: one 1024-thread workgroup walks the array in chunks of 8192
+// with a running carry (one launch instead of three).
+constexpr int kSmallThreads = 1024;
+constexpr uint64_t kSmallMax = 8192;
+__global__ void __launch_bounds__(kSmallThreads) k_scan_small(const uint32_t* __restrict__ in, uint32_t* __restrict__ out,
+                                                              uint64_t n, uint32_t* __restrict__ total) {
+  __shared__ uint32_t lds[kSmallThreads / 64 + 1];
+  uint32_t carry = 0;
+  for (uint64_t base = 0; base < n; base += (uint64_t)kSmallThreads * kItems) {
+    const uint64_t b = base + (uint64_t)threadIdx.x * kItems;
+    uint32_t v[kItems];
+    uint32_t s = 0;
+#pragma unroll
+    for (int k = 0; k < kItems; ++k) {
+      const uint64_t i = b + k;
+      v[k] = i < n ? in[i] : 0u;
+      s += v[k];
+    }
+    uint32_t agg;
+    uint32_t ex = block_exclusive_sum(s, lds, agg) + carry;
+#pragma unroll
+    for (int k = 0; k < kItems; ++k) {
+      const uint64_t i = b + k;
+      if (i < n) out[i] = ex;
+      ex += v[k];
+    }
+    carry += agg;
+  }
+  if (threadIdx.x == 0 && total) *total = carry;
+}
+
 __global__ void k_fill(uint32_t* d, uint32_t v, uint64_t n) {
   uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) d[i] = v;
@@ -81,6 +112,11 @@ __global__ void k_iota(uint32_t* d, uint64_t n) {
 void scan_exclusive(svd_ctx* c, const uint32_t* in, uint32_t* out, uint64_t n, uint32_t* total) {
   if (n == 0) {
     if (total) fill_u32(c, total, 0u, 1);
+    return;
+  }
+  if (n <= kSmallMax) {
+    k_scan_small<<<1, kSmallThreads, 0, c->stream>>>(in, out, n, total);
+    check_launch("scan_exclusive");
     return;
   }
   const uint32_t nb = div_up(n, kTile);

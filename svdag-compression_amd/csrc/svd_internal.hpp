@@ -175,6 +175,10 @@ void cross_level_merge(svd_octree* o);
 // std::sort(first, last, [](a, b) { return a.second > b.second; })
 // (encoded_ssvdag.cpp:273-275).
 void refsort(svd_ctx* c, uint32_t* idx, uint32_t* val, uint64_t n);
+// Several independent std::sort calls at once: range r is [rfirst[r], rlast[r])
+// of the same arrays (n elements in all).  The ranges share every round.
+void refsort_ranges(svd_ctx* c, uint32_t* idx, uint32_t* val, uint64_t n, const uint64_t* rfirst,
+                    const uint64_t* rlast, size_t nr);
 
 // ---- attribute bit-tree (attr.hip) ---------------------------------------------
 uint64_t attr_bittree(svd_ctx* c, const uint8_t* dattrs, uint64_t V, int gray, uint8_t* dst, uint64_t cap);
