@@ -99,7 +99,9 @@ __global__ void k_insert(const uint32_t* __restrict__ rec, uint64_t n, uint32_t*
       v = old;
     }
     if (rec_eq(rec, v, w)) {
-      atomicMin(&table[h], (uint32_t)i);
+      // the slot only decreases, so a reader above the value it saw has
+      // nothing to add: hot keys cost one atomic per decrease, not per node
+      if ((uint32_t)i < v) atomicMin(&table[h], (uint32_t)i);
       slot[i] = h;
       return;
     }

@@ -27,6 +27,8 @@ constexpr int kLdsN = 2048;           // segments up to this size finish in one 
 constexpr int kBT = 256;              // threads per workgroup
 constexpr int kPer = kLdsN / kBT;     // elements per thread in the workgroup phase
 constexpr int kTile = 2048;           // elements per workgroup in the device-wide rounds
+constexpr uint32_t kLeafTag = 0x8000u; // workgroup phase: segOf tags
+constexpr uint32_t kDone = 0xFFFFu;
 
 struct Seg { uint32_t first, last; int32_t depth; uint32_t pad; };
 
@@ -301,8 +303,11 @@ __global__ void __launch_bounds__(kBT) k_block_sort(const Seg* __restrict__ segs
   __shared__ int8_t aDepth[2][128];
   __shared__ uint32_t aPivot[128];
   __shared__ uint16_t aK[128], aCut[128], aLeft[128], aRight[128];
+  __shared__ uint16_t lfFirst[kLdsN / 2], lfLast[kLdsN / 2];   // leaves of 2..16 elements
   __shared__ uint32_t lds[kBT / 64 + 1];
   __shared__ uint32_t nActive[2];
+  __shared__ uint32_t nLeaf;
+  // segOf: < kLeafTag active segment slot, kLeafTag | q leaf q, kDone finished
 
   if (blockIdx.x >= *nsp) return;
   const Seg g = segs[blockIdx.x];
@@ -311,6 +316,7 @@ __global__ void __launch_bounds__(kBT) k_block_sort(const Seg* __restrict__ segs
   for (uint32_t k = tid; k < n; k += kBT) { sv[k] = val[g.first + k]; si[k] = idx[g.first + k]; segOf[k] = 0; }
   if (tid == 0) {
     nActive[0] = 0;
+    nLeaf = 0;
     if (g.depth == 0) {
       LArr a{sv, si};
       heap_sort(a, n);
@@ -344,7 +350,7 @@ __global__ void __launch_bounds__(kBT) k_block_sort(const Seg* __restrict__ segs
       fl[k] = fr[k] = 0;
       if (e < n) {
         const uint32_t s = segOf[e];
-        if (s != 0xFFFFu) {
+        if (s < kLeafTag) {
           const uint32_t p = aPivot[s];
           fl[k] = (e > aFirst[cur][s] && sv[e] <= p) ? 1u : 0u;
           fr[k] = (sv[e] >= p) ? 1u : 0u;
@@ -369,7 +375,7 @@ __global__ void __launch_bounds__(kBT) k_block_sort(const Seg* __restrict__ segs
       const uint32_t e = tid * kPer + k;
       if (e >= n) continue;
       const uint32_t s = segOf[e];
-      if (s == 0xFFFFu) continue;
+      if (s >= kLeafTag) continue;
       const uint32_t f = aFirst[cur][s], l = aLast[cur][s];
       if (fl[k]) Lp[f + (exL[e] - exL[f])] = (uint16_t)e;
       if (fr[k]) Rp[f + (exR[l] - 1 - exR[e])] = (uint16_t)e;
@@ -381,7 +387,7 @@ __global__ void __launch_bounds__(kBT) k_block_sort(const Seg* __restrict__ segs
       const uint32_t e = tid * kPer + k;
       if (e >= n) continue;
       const uint32_t s = segOf[e];
-      if (s == 0xFFFFu) continue;
+      if (s >= kLeafTag) continue;
       const uint32_t f = aFirst[cur][s], l = aLast[cur][s];
       const uint32_t nL = exL[l] - exL[f], nR = exR[l] - exR[f];
       const uint32_t m = min(nL, nR);
@@ -409,10 +415,16 @@ __global__ void __launch_bounds__(kBT) k_block_sort(const Seg* __restrict__ segs
       const uint32_t cf[2] = {f, cut}, cl[2] = {cut, l};
       for (int h = 0; h < 2; ++h) {
         const uint32_t sz = cl[h] - cf[h];
-        ids[h] = 0xFFFFu;
+        ids[h] = kDone;
         LArr a{sv + cf[h], si + cf[h]};
-        if (sz <= (uint32_t)kThreshold) insertion_sort(a, sz);
-        else if (d == 0) heap_sort(a, sz);
+        if (sz <= (uint32_t)kThreshold) {
+          if (sz > 1) {   // sorted after the rounds, all leaves at once
+            const uint32_t q = atomicAdd(&nLeaf, 1u);
+            lfFirst[q] = (uint16_t)cf[h];
+            lfLast[q] = (uint16_t)cl[h];
+            ids[h] = (uint16_t)(kLeafTag | q);
+          }
+        } else if (d == 0) heap_sort(a, sz);
         else {
           const uint32_t slotn = atomicAdd(&nActive[nxt], 1u);
           aFirst[nxt][slotn] = (uint16_t)cf[h]; aLast[nxt][slotn] = (uint16_t)cl[h]; aDepth[nxt][slotn] = (int8_t)d;
@@ -428,12 +440,40 @@ __global__ void __launch_bounds__(kBT) k_block_sort(const Seg* __restrict__ segs
       const uint32_t e = tid * kPer + k;
       if (e >= n) continue;
       const uint32_t s = segOf[e];
-      if (s == 0xFFFFu) continue;
+      if (s >= kLeafTag) continue;
       segOf[e] = e < aCut[s] ? aLeft[s] : aRight[s];
     }
     cur = nxt;
     __syncthreads();
   }
+  // The final insertion sort only acts inside leaves and is stable: every
+  // element's place is the number of larger elements of its leaf plus the
+  // equal ones before it.
+  uint32_t rv[kPer], ri[kPer];
+  uint16_t rp[kPer];
+#pragma unroll
+  for (int k = 0; k < kPer; ++k) {
+    const uint32_t e = tid * kPer + k;
+    rp[k] = kDone;
+    if (e >= n) continue;
+    const uint32_t s = segOf[e];
+    if (s < kLeafTag || s == kDone) continue;
+    const uint32_t f = lfFirst[s & ~kLeafTag], l = lfLast[s & ~kLeafTag];
+    const uint32_t v = sv[e];
+    uint32_t r = 0;
+    for (uint32_t j = f; j < l; ++j) {
+      const uint32_t u = sv[j];
+      r += (comp(u, v) || (u == v && j < e)) ? 1u : 0u;
+    }
+    rv[k] = v;
+    ri[k] = si[e];
+    rp[k] = (uint16_t)(f + r);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < kPer; ++k)
+    if (rp[k] != kDone) { sv[rp[k]] = rv[k]; si[rp[k]] = ri[k]; }
+  __syncthreads();
   for (uint32_t k = tid; k < n; k += kBT) { val[g.first + k] = sv[k]; idx[g.first + k] = si[k]; }
 }
 

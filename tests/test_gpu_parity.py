@@ -162,6 +162,38 @@ def test_refcount_order_heap_fallback(ctx):
         np.testing.assert_array_equal(gi, wi)
 
 
+def test_refcount_order_ranges_equal_separate_sorts(ctx):
+    """Ranges of every class (leaf, workgroup, device-wide, heap fallback, empty,
+    single) sorted together equal one std::sort per range."""
+    rng = np.random.default_rng(12)
+    parts = [np.ones(70000, np.uint32),
+             np.minimum(rng.geometric(0.55, 300000), 60).astype(np.uint32),
+             rng.integers(0, 4, 1500).astype(np.uint32),
+             _killer(9000),
+             np.zeros(0, np.uint32),
+             np.array([5], np.uint32),
+             rng.integers(0, 3, 16).astype(np.uint32),
+             np.minimum(rng.geometric(0.3, 5000), 9).astype(np.uint32),
+             np.full(2049, 2, np.uint32)]
+    vals = np.concatenate(parts)
+    idx = rng.permutation(vals.size).astype(np.uint32)
+    bounds = np.cumsum([0] + [p.size for p in parts])
+    ranges = list(zip(bounds[:-1], bounds[1:]))
+    gi, gv = ctx.refcount_order_ranges(idx, vals, ranges)
+    for a, b in ranges:
+        wi, wv = O.std_sort_refcount(idx[a:b], vals[a:b])
+        np.testing.assert_array_equal(gv[a:b], wv)
+        np.testing.assert_array_equal(gi[a:b], wi)
+
+
+def test_refcount_order_ranges_bad_range(ctx):
+    idx = np.arange(10, dtype=np.uint32)
+    with pytest.raises(S.SvdError):
+        ctx.refcount_order_ranges(idx, idx, [(0, 11)])
+    with pytest.raises(S.SvdError):
+        ctx.refcount_order_ranges(idx, idx, [(6, 5)])
+
+
 @pytest.mark.parametrize("fmt", [S.FMT_SVDAG, S.FMT_USSVDAG, S.FMT_SSVDAG])
 def test_trace_matches_shader_port(ctx, fmt):
     codes, L = W.terrain_codes(8, seed=1), 8
@@ -276,12 +308,30 @@ def test_cross_level_merge_bit_exact(ctx, name):
     assert g.encoded_bytes(S.FMT_SVDAG) == o.encode(0)
 
 
-@pytest.mark.parametrize("n", [0, 1, 3, 1000, 65536, 300001])
+@pytest.mark.parametrize("n", [0, 1, 2, 3, 63, 64, 65, 1000, 2048, 2049, 65536, 300001, 3_000_017])
 @pytest.mark.parametrize("gray", [False, True])
 def test_attr_bittree_bit_exact(ctx, n, gray):
     rng = np.random.default_rng(n + gray)
     a = np.repeat(rng.integers(0, 16, max(n // 7, 1)), 7)[:n].astype(np.uint8)
     assert ctx.attr_bittree(a, gray) == O.attr_bittree_encode(a, gray)
+
+
+@pytest.mark.parametrize("kind", ["random", "constant", "ramp", "sparse"])
+def test_attr_bittree_patterns(ctx, kind):
+    """Fully mixed planes (random bytes), a constant, a ramp and sparse flips."""
+    rng = np.random.default_rng(7)
+    n = 100_003
+    if kind == "random":
+        a = rng.integers(0, 256, n).astype(np.uint8)
+    elif kind == "constant":
+        a = np.full(n, 0xA5, np.uint8)
+    elif kind == "ramp":
+        a = (np.arange(n) // 391 % 256).astype(np.uint8)
+    else:
+        a = np.zeros(n, np.uint8)
+        a[rng.integers(0, n, 40)] = rng.integers(1, 256, 40).astype(np.uint8)
+    for gray in (False, True):
+        assert ctx.attr_bittree(a, gray) == O.attr_bittree_encode(a, gray)
 
 
 def test_attr_bittree_full_size(ctx):
