@@ -2,17 +2,27 @@
 //
 // The reference walks a top-down frontier: every frontier node A of level levA
 // is compared (compareSubtrees, :238-290) with the nodes of the levels above
-// that share its depth-d hash (computeNodeHash, :34-74); the first equal one in
+// that share its hash (computeNodeHash, :34-74); the first equal one in
 // (levB, index) order wins, A's whole subtree is recorded as corresponding to
 // B's, and unmatched nodes send their children to the next frontier.  Then the
 // levels are compacted bottom-up and pointers redirected (:1428-1498).
 //
-// Here, per level: all (frontier node, candidate) pairs with equal hashes are
-// compared in parallel and the first match is an atomicMin over
-// (levB << 32 | index); subtree correspondences get the priority the
-// reference's map semantics give them (earlier match wins between matches,
-// last pre-order visit wins inside one comparison); the compaction runs the
-// reference's two pointer-update loops level by level.
+// compareSubtrees(A, B) is equality of the two subtrees cut at the depth of
+// A's leaves: every pair of visited nodes has equal child masks.  Here that
+// equality is a class number: cls_0(x) = mask(x), and cls_d(x) = the hash-table
+// slot of (mask(x), cls_{d-1}(children)) over all nodes of levels 1..L-1-d, so
+// cls_d(A) == cls_d(B) exactly when the depth-d subtrees are equal.  Equal
+// subtrees have equal hashes (the hash depth never exceeds the compared
+// depth), so the hash pre-filter drops nothing and is not needed.  The first
+// match is then an atomicMin of (levB << 32 | index) per class.
+//
+// Recording: inside one comparison the reference's map keeps the last
+// pre-order visit of a node; between matches of one round the earliest
+// frontier node wins.  The last pre-order visit of x is its lexicographically
+// largest child-slot path from A, so a top-down pass per level keeps, per
+// node, the smallest matching frontier index and then the largest path
+// (path * 8 + slot), and the winning parent writes B's descendant on the same
+// path.  No subtree is walked per thread.
 #include "svd_internal.hpp"
 
 namespace svd {
@@ -35,137 +45,129 @@ __device__ __forceinline__ uint32_t node_child(const LevelPtrs& P, uint32_t lev,
   return lev == P.leafLevel ? kNull : P.rec[lev][(uint64_t)i * kRecWords + 1 + c];
 }
 
-// computeNodeHash with childLevels = lev+1 (std::hash<uint64_t> is the identity).
-__global__ void k_hash(LevelPtrs P, uint32_t lev, uint32_t depth, const uint64_t* __restrict__ childHash,
-                       uint64_t* __restrict__ out) {
-  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+// ---- truncated-subtree classes ---------------------------------------------
+// keys of level lev for depth d >= 1: mask, then the children's cls_{d-1}
+// (clsChild == nullptr: d == 1, the children's masks)
+__global__ void k_class_keys(LevelPtrs P, uint32_t lev, const uint32_t* __restrict__ clsChild,
+                             uint32_t* __restrict__ keys) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= P.n[lev]) return;
-  uint64_t key = 0;
-  if (depth == 0) {
-    key = node_mask(P, lev, i);
-  } else if (depth == 1) {
-    for (int c = 0; c < 8; ++c) {
-      key <<= 8;
+  const uint32_t m = node_mask(P, lev, i);
+  uint32_t* k = keys + (uint64_t)i * kRecWords;
+  k[0] = m;
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    uint32_t v = kNull;
+    if ((m >> c) & 1u) {
       const uint32_t ch = node_child(P, lev, i, c);
-      if (ch != kNull) key += node_mask(P, lev + 1, ch);
+      v = clsChild ? clsChild[ch] : node_mask(P, lev + 1, ch);
     }
-  } else {
-    for (int c = 0; c < 8; ++c) {
-      const uint64_t orig = key;
-      uint64_t h = orig << (c + 1);
-      const uint32_t ch = node_child(P, lev, i, c);
-      if (ch != kNull) h = childHash[ch];
-      key = (orig << 1) ^ h;
-      key = key ^ orig;
-    }
+    k[1 + c] = v;
   }
-  out[i] = key;
 }
 
-// Chained table of the frontier nodes of level levA, keyed by hash.
-__global__ void k_frontier_table(const uint8_t* __restrict__ front, uint32_t n, const uint64_t* __restrict__ hash,
-                                 uint32_t* __restrict__ head, uint32_t cmask, uint32_t* __restrict__ next) {
-  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n || !front[i]) return;
-  const uint32_t b = (uint32_t)mix64(hash[i]) & cmask;
-  next[i] = atomicExch(&head[b], i);
-}
-
-// compareSubtrees (geom_octree_extension.cpp:238-290) as an iterative DFS.
-// record = 0: only decide equality.  record = 1: also emit, in pre-order,
-// every (A-descendant -> B-descendant) correspondence with its visit number.
-struct Frame { uint32_t a, b, c; };
-
-template <bool kRecord>
-__device__ bool compare_subtrees(const LevelPtrs& P, uint32_t L, uint32_t levA, uint32_t a, uint32_t levB, uint32_t b,
-                                 uint64_t idA, unsigned long long* const* prio, uint32_t round) {
-  // depth of the walk is at most L-1-levA <= 20
-  Frame st[kMaxL];
-  int sp = 0;
-  uint32_t visit = 0;
-  auto check_node = [&](uint32_t la, uint32_t x, uint32_t lb, uint32_t y) -> int {
-    // returns 0 = unequal, 1 = equal leaf (stop), 2 = descend
-    const uint32_t ma = node_mask(P, la, x), mb = node_mask(P, lb, y);
-    if (ma != 0 && mb == 0) return 0;
-    if (la == L - 1) return ma == mb ? 1 : 0;
-    return 2;
-  };
-  int r = check_node(levA, a, levB, b);
-  if (r != 2) return r == 1;
-  st[sp++] = Frame{a, b, 0};
-  uint32_t la = levA, lb = levB;
-  while (sp > 0) {
-    Frame& f = st[sp - 1];
-    if (f.c >= 8) { --sp; --la; --lb; continue; }
-    const int i = (int)f.c++;
-    const uint32_t ma = node_mask(P, la, f.a), mb = node_mask(P, lb, f.b);
-    const bool ea = (ma >> i) & 1u, eb = (mb >> i) & 1u;
-    if (ea != eb) return false;
-    if (!ea) continue;
-    const uint32_t ca = node_child(P, la, f.a, i), cb = node_child(P, lb, f.b, i);
-    if (kRecord) {
-      // operator[] inside one comparison: the last visit wins -> larger visit number wins
-      const unsigned long long pr = ((unsigned long long)idA << 32) | (0xFFFFFFFFu - visit);
-      atomicMin(&prio[la + 1][ca], pr);
-      (void)round;
+// cls[i] = the slot that holds key i's value (one slot per distinct key)
+__global__ void k_class_insert(const uint32_t* __restrict__ keys, uint64_t n, uint32_t* __restrict__ table,
+                               uint32_t cmask, uint32_t* __restrict__ cls) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint32_t w[kRecWords];
+#pragma unroll
+  for (int k = 0; k < kRecWords; ++k) w[k] = keys[i * kRecWords + k];
+  uint32_t h = (uint32_t)hash_rec(w) & cmask;
+  while (true) {
+    uint32_t v = __atomic_load_n(&table[h], __ATOMIC_RELAXED);
+    if (v == kNone) {
+      const uint32_t old = atomicCAS(&table[h], kNone, (uint32_t)i);
+      if (old == kNone) { cls[i] = h; return; }
+      v = old;
     }
-    ++visit;
-    r = check_node(la + 1, ca, lb + 1, cb);
-    if (r == 0) return false;
-    if (r == 2) { st[sp++] = Frame{ca, cb, 0}; ++la; ++lb; }
+    bool eq = true;
+#pragma unroll
+    for (int k = 0; k < kRecWords; ++k) eq = eq && keys[(uint64_t)v * kRecWords + k] == w[k];
+    if (eq) { cls[i] = h; return; }
+    h = (h + 1) & cmask;
   }
-  return true;
 }
 
-// For every node B of level levB: walk the chain of frontier nodes with the
-// same hash; an equal subtree lowers best[A] to (levB, B).
-__global__ void k_match(LevelPtrs P, uint32_t L, uint32_t levA, uint32_t levB, const uint64_t* __restrict__ hashB,
-                        const uint64_t* __restrict__ hashA, const uint32_t* __restrict__ head, uint32_t cmask,
-                        const uint32_t* __restrict__ next, unsigned long long* __restrict__ best) {
-  uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ uint32_t class_of(const LevelPtrs& P, uint32_t lev, uint32_t i,
+                                             const uint32_t* __restrict__ cls) {
+  return cls ? cls[i] : node_mask(P, lev, i);
+}
+
+// bestOf[class] = min over the nodes of level levB of (levB << 32 | index)
+__global__ void k_class_best(LevelPtrs P, uint32_t levB, const uint32_t* __restrict__ cls,
+                             unsigned long long* __restrict__ bestOf) {
+  const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= P.n[levB]) return;
-  const uint64_t k = hashB[j];
-  uint32_t a = head[(uint32_t)mix64(k) & cmask];
-  while (a != kNone) {
-    if (hashA[a] == k) {
-      const unsigned long long cand = ((unsigned long long)levB << 32) | j;
-      if (cand < best[a] && compare_subtrees<false>(P, L, levA, a, levB, j, 0, nullptr, 0)) atomicMin(&best[a], cand);
-    }
-    a = next[a];
+  const unsigned long long key = ((unsigned long long)levB << 32) | j;
+  unsigned long long* b = &bestOf[class_of(P, levB, j, cls)];
+  if (key < __atomic_load_n(b, __ATOMIC_RELAXED)) atomicMin(b, key);
+}
+
+// Frontier nodes of levA take their class's first match; matched ones start
+// the recording state (frontier index, empty path, B).
+__global__ void k_frontier_best(LevelPtrs P, uint32_t levA, const uint8_t* __restrict__ front,
+                                const uint32_t* __restrict__ cls, const unsigned long long* __restrict__ bestOf,
+                                unsigned long long* __restrict__ best, uint32_t* __restrict__ sa,
+                                uint64_t* __restrict__ sp, uint64_t* __restrict__ sb, uint32_t* __restrict__ nMatched) {
+  const uint32_t a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= P.n[levA]) return;
+  unsigned long long b = ~0ull;
+  if (front[a]) b = bestOf[class_of(P, levA, a, cls)];
+  best[a] = b;
+  sa[a] = b != ~0ull ? a : kNone;
+  sp[a] = 0;
+  sb[a] = b;
+  if (b != ~0ull) atomicAdd(nMatched, 1u);
+}
+
+// One level of the recording pass: children (level lev+1) of recorded nodes.
+__global__ void k_prop_min(LevelPtrs P, uint32_t lev, const uint32_t* __restrict__ sa, uint32_t* __restrict__ na) {
+  const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= P.n[lev]) return;
+  const uint32_t a = sa[x];
+  if (a == kNone) return;
+  const uint32_t m = node_mask(P, lev, x);
+  for (int c = 0; c < 8; ++c) {
+    if (!((m >> c) & 1u)) continue;
+    uint32_t* t = &na[node_child(P, lev, x, c)];
+    if (a < __atomic_load_n(t, __ATOMIC_RELAXED)) atomicMin(t, a);
   }
 }
 
-// Matched frontier nodes record their subtree correspondences (priority pass).
-__global__ void k_record_prio(LevelPtrs P, uint32_t L, uint32_t levA, const uint8_t* __restrict__ front,
-                              const unsigned long long* __restrict__ best, unsigned long long* const* prio) {
-  uint32_t a = blockIdx.x * blockDim.x + threadIdx.x;
-  if (a >= P.n[levA] || !front[a] || best[a] == ~0ull) return;
-  const uint32_t levB = (uint32_t)(best[a] >> 32), b = (uint32_t)best[a];
-  compare_subtrees<true>(P, L, levA, a, levB, b, a, prio, 0);
+__global__ void k_prop_path(LevelPtrs P, uint32_t lev, const uint32_t* __restrict__ sa, const uint64_t* __restrict__ sp,
+                            const uint32_t* __restrict__ na, unsigned long long* __restrict__ np) {
+  const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= P.n[lev]) return;
+  const uint32_t a = sa[x];
+  if (a == kNone) return;
+  const uint32_t m = node_mask(P, lev, x);
+  const unsigned long long base = (unsigned long long)sp[x] << 3;
+  for (int c = 0; c < 8; ++c) {
+    if (!((m >> c) & 1u)) continue;
+    const uint32_t y = node_child(P, lev, x, c);
+    if (na[y] != a) continue;
+    const unsigned long long v = base | (unsigned long long)c;
+    if (v > __atomic_load_n(&np[y], __ATOMIC_RELAXED)) atomicMax(&np[y], v);
+  }
 }
 
-// Second pass: the winning visit writes its correspondence (levB', index).
-__global__ void k_record_write(LevelPtrs P, uint32_t L, uint32_t levA, const uint8_t* __restrict__ front,
-                               const unsigned long long* __restrict__ best, unsigned long long* const* prio,
-                               uint64_t* const* cand) {
-  uint32_t a = blockIdx.x * blockDim.x + threadIdx.x;
-  if (a >= P.n[levA] || !front[a] || best[a] == ~0ull) return;
-  const uint32_t levB = (uint32_t)(best[a] >> 32), b = (uint32_t)best[a];
-  Frame st[kMaxL];
-  int sp = 0;
-  uint32_t visit = 0, la = levA, lb = levB;
-  if (levA == L - 1) return;
-  st[sp++] = Frame{a, b, 0};
-  while (sp > 0) {
-    Frame& f = st[sp - 1];
-    if (f.c >= 8) { --sp; --la; --lb; continue; }
-    const int i = (int)f.c++;
-    if (!((node_mask(P, la, f.a) >> i) & 1u)) continue;
-    const uint32_t ca = node_child(P, la, f.a, i), cb = node_child(P, lb, f.b, i);
-    const unsigned long long pr = ((unsigned long long)a << 32) | (0xFFFFFFFFu - visit);
-    if (prio[la + 1][ca] == pr) cand[la + 1][ca] = ((uint64_t)(lb + 1) << 32) | cb;
-    ++visit;
-    if (la + 1 < L - 1) { st[sp++] = Frame{ca, cb, 0}; ++la; ++lb; }
+__global__ void k_prop_write(LevelPtrs P, uint32_t lev, const uint32_t* __restrict__ sa, const uint64_t* __restrict__ sp,
+                             const uint64_t* __restrict__ sb, const uint32_t* __restrict__ na,
+                             const uint64_t* __restrict__ np, uint64_t* __restrict__ nb) {
+  const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= P.n[lev]) return;
+  const uint32_t a = sa[x];
+  if (a == kNone) return;
+  const uint32_t m = node_mask(P, lev, x);
+  const uint64_t base = sp[x] << 3;
+  const uint32_t lb = (uint32_t)(sb[x] >> 32), b = (uint32_t)sb[x];
+  for (int c = 0; c < 8; ++c) {
+    if (!((m >> c) & 1u)) continue;
+    const uint32_t y = node_child(P, lev, x, c);
+    if (na[y] == a && np[y] == (base | (uint64_t)c))
+      nb[y] = ((uint64_t)(lb + 1) << 32) | node_child(P, lb, b, c);
   }
 }
 
@@ -279,72 +281,76 @@ void cross_level_merge(svd_octree* o) {
     P.n[l] = (uint32_t)W[l].n;
   }
   // per-level state
-  std::vector<uint64_t*> hash(L, nullptr), hashTmp(L, nullptr), mlc(L, nullptr), cand(L, nullptr);
-  std::vector<unsigned long long*> prio(L, nullptr);
+  std::vector<uint64_t*> mlc(L, nullptr), cand(L, nullptr), sp(L, nullptr);
+  std::vector<uint32_t*> sa(L, nullptr);
   std::vector<uint8_t*> front(L, nullptr);
   for (uint32_t l = 0; l < L; ++l) {
     const uint64_t n = W[l].n + 1;
-    hash[l] = dalloc<uint64_t>(c, n);
-    hashTmp[l] = dalloc<uint64_t>(c, n);
     mlc[l] = dalloc<uint64_t>(c, n);
     cand[l] = dalloc<uint64_t>(c, n);
-    prio[l] = dalloc<unsigned long long>(c, n);
+    sp[l] = dalloc<uint64_t>(c, n);
+    sa[l] = dalloc<uint32_t>(c, n);
     front[l] = dalloc<uint8_t>(c, n);
     k_fill64<<<div_up(n, kB), kB, 0, c->stream>>>(mlc[l], ~0ull, n);
     k_fill8<<<div_up(n, kB), kB, 0, c->stream>>>(front[l], 0, n);
   }
-  unsigned long long** dPrio = dalloc<unsigned long long*>(c, L);
-  uint64_t** dCand = dalloc<uint64_t*>(c, L);
-  SVD_HIP(hipMemcpyAsync(dPrio, prio.data(), sizeof(void*) * L, hipMemcpyHostToDevice, c->stream));
-  SVD_HIP(hipMemcpyAsync(dCand, cand.data(), sizeof(void*) * L, hipMemcpyHostToDevice, c->stream));
-  SVD_HIP(hipStreamSynchronize(c->stream));   // pointer tables come from host vectors
-
-  auto compute_hashes = [&](uint32_t depth) {
-    // hash_d(level l) needs hash_{d-1}(level l+1): sweep d = 0..depth
-    for (uint32_t d = 0; d <= depth; ++d) {
-      for (uint32_t l = 1; l < L; ++l) {
-        if (!P.n[l]) continue;
-        const uint64_t* ch = (l + 1 < L) ? hash[l + 1] : nullptr;
-        k_hash<<<div_up(P.n[l], kB), kB, 0, c->stream>>>(P, l, d, ch, hashTmp[l]);
-      }
-      std::swap(hash, hashTmp);
+  // classes: cls[d] covers levels 1..L-1-d, level l at base[l]
+  std::vector<uint64_t> base(L + 1, 0);
+  for (uint32_t l = 1; l < L; ++l) base[l + 1] = base[l] + P.n[l];
+  std::vector<uint32_t*> cls(L, nullptr);
+  std::vector<uint32_t> capOf(L, 256);
+  if (L >= 3) {
+    uint32_t* keys = dalloc<uint32_t>(c, (base[L - 1] + 1) * kRecWords);
+    for (uint32_t d = 1; d + 2 <= L; ++d) {
+      const uint32_t top = L - 1 - d;   // deepest level with d levels below it
+      const uint64_t nd = base[top + 1];
+      uint64_t cap = 1024;
+      while (cap < 2 * nd) cap <<= 1;
+      if (cap > (1ull << 31)) throw Error{SVD_ERR_CAPACITY, "cross-level merge: too many nodes"};
+      capOf[d] = (uint32_t)cap;
+      cls[d] = dalloc<uint32_t>(c, nd + 1);
+      for (uint32_t l = 1; l <= top; ++l)
+        if (P.n[l])
+          k_class_keys<<<div_up(P.n[l], kB), kB, 0, c->stream>>>(P, l, d == 1 ? nullptr : cls[d - 1] + base[l + 1],
+                                                                 keys + base[l] * kRecWords);
+      uint32_t* table = dalloc<uint32_t>(c, cap);
+      fill_u32(c, table, kNone, cap);
+      if (nd) k_class_insert<<<div_up(nd, kB), kB, 0, c->stream>>>(keys, nd, table, (uint32_t)(cap - 1), cls[d]);
+      check_launch("k_class");
+      dfree(c, table);
     }
-    check_launch("k_hash");
-  };
+    dfree(c, keys);
+  }
+  auto clsAt = [&](uint32_t d, uint32_t l) -> const uint32_t* { return d == 0 ? nullptr : cls[d] + base[l]; };
 
-  uint32_t curDepth = L / 2;
-  compute_hashes(curDepth);
   if (P.n[1]) k_fill8<<<div_up(P.n[1], kB), kB, 0, c->stream>>>(front[1], 1, P.n[1]);
-  unsigned long long* best = nullptr;
-  uint32_t* head = nullptr;
-  uint32_t* nextA = nullptr;
+  uint32_t* nMatched = dalloc<uint32_t>(c, 1);
   for (uint32_t levA = 1; levA < L; ++levA) {
-    const uint32_t maxDepth = L - levA - 1;
-    if (maxDepth < curDepth) { curDepth = maxDepth; compute_hashes(curDepth); }
+    const uint32_t D = L - 1 - levA;   // levels below A
     const uint32_t nA = P.n[levA];
     if (!nA) continue;
-    uint32_t cap = 1024;
-    while (cap < 2 * nA) cap <<= 1;
-    best = dalloc<unsigned long long>(c, nA);
-    head = dalloc<uint32_t>(c, cap);
-    nextA = dalloc<uint32_t>(c, nA);
-    k_fill64<<<div_up(nA, kB), kB, 0, c->stream>>>((uint64_t*)best, ~0ull, nA);
-    fill_u32(c, head, kNone, cap);
-    k_frontier_table<<<div_up(nA, kB), kB, 0, c->stream>>>(front[levA], nA, hash[levA], head, cap - 1, nextA);
-    for (uint32_t levB = 1; levB < levA; ++levB) {
-      if (!P.n[levB]) continue;
-      k_match<<<div_up(P.n[levB], kB), kB, 0, c->stream>>>(P, L, levA, levB, hash[levB], hash[levA], head, cap - 1,
-                                                             nextA, best);
-    }
-    check_launch("k_match");
-    if (levA + 1 < L) {
-      for (uint32_t l = levA + 1; l < L; ++l) {
-        const uint64_t n = P.n[l] + 1;
-        k_fill64<<<div_up(n, kB), kB, 0, c->stream>>>((uint64_t*)prio[l], ~0ull, n);
-        k_fill64<<<div_up(n, kB), kB, 0, c->stream>>>(cand[l], ~0ull, n);
+    unsigned long long* best = dalloc<unsigned long long>(c, nA);
+    unsigned long long* bestOf = dalloc<unsigned long long>(c, capOf[D]);
+    k_fill64<<<div_up(capOf[D], kB), kB, 0, c->stream>>>((uint64_t*)bestOf, ~0ull, capOf[D]);
+    for (uint32_t levB = 1; levB < levA; ++levB)
+      if (P.n[levB]) k_class_best<<<div_up(P.n[levB], kB), kB, 0, c->stream>>>(P, levB, clsAt(D, levB), bestOf);
+    fill_u32(c, nMatched, 0u, 1);
+    k_frontier_best<<<div_up(nA, kB), kB, 0, c->stream>>>(P, levA, front[levA], clsAt(D, levA), bestOf, best,
+                                                            sa[levA], sp[levA], cand[levA], nMatched);
+    check_launch("k_frontier_best");
+    if (levA + 1 < L && read_u32(c, nMatched)) {
+      for (uint32_t l = levA; l + 1 < L; ++l) {
+        const uint32_t nc = P.n[l + 1];
+        fill_u32(c, sa[l + 1], kNone, nc);
+        k_fill64<<<div_up(nc + 1, kB), kB, 0, c->stream>>>(sp[l + 1], 0ull, nc + 1);
+        k_fill64<<<div_up(nc + 1, kB), kB, 0, c->stream>>>(cand[l + 1], ~0ull, nc + 1);
+        if (!P.n[l]) continue;
+        const uint32_t g = div_up(P.n[l], kB);
+        k_prop_min<<<g, kB, 0, c->stream>>>(P, l, sa[l], sa[l + 1]);
+        k_prop_path<<<g, kB, 0, c->stream>>>(P, l, sa[l], sp[l], sa[l + 1], (unsigned long long*)sp[l + 1]);
+        k_prop_write<<<g, kB, 0, c->stream>>>(P, l, sa[l], sp[l], cand[l], sa[l + 1], sp[l + 1], cand[l + 1]);
+        check_launch("cross-level record");
       }
-      k_record_prio<<<div_up(nA, kB), kB, 0, c->stream>>>(P, L, levA, front[levA], best, dPrio);
-      k_record_write<<<div_up(nA, kB), kB, 0, c->stream>>>(P, L, levA, front[levA], best, dPrio, dCand);
       for (uint32_t l = levA + 1; l < L; ++l)
         if (P.n[l]) k_merge_sub<<<div_up(P.n[l], kB), kB, 0, c->stream>>>(P.n[l], cand[l], mlc[l]);
     }
@@ -352,8 +358,10 @@ void cross_level_merge(svd_octree* o) {
                                                         levA + 1 < L ? front[levA + 1] : nullptr, W[levA].rec,
                                                         levA == L - 1 ? 1 : 0);
     check_launch("cross-level round");
-    dfree(c, best); dfree(c, head); dfree(c, nextA);
+    dfree(c, best); dfree(c, bestOf);
   }
+  dfree(c, nMatched);
+  for (auto* p : cls) dfree(c, p);
 
   // compaction, bottom-up
   uint32_t* missing = dalloc<uint32_t>(c, 1);
@@ -401,9 +409,9 @@ void cross_level_merge(svd_octree* o) {
   o->nNodesX = nNodes;
   o->crossMerged = merged;
   for (uint32_t l = 0; l < L; ++l) {
-    dfree(c, hash[l]); dfree(c, hashTmp[l]); dfree(c, mlc[l]); dfree(c, cand[l]); dfree(c, prio[l]); dfree(c, front[l]);
+    dfree(c, mlc[l]); dfree(c, cand[l]); dfree(c, sp[l]); dfree(c, sa[l]); dfree(c, front[l]);
   }
-  dfree(c, dPrio); dfree(c, dCand); dfree(c, missing); dfree(c, total);
+  dfree(c, missing); dfree(c, total);
   for (auto& e : o->enc) e.valid = false;
   if (nMissing)
     throw Error{SVD_ERR_POINTER, "cross-level merge: " + std::to_string(nMissing) +

@@ -286,9 +286,15 @@ def test_errors_are_reported(ctx):
         g.to_sdag()
 
 
-@pytest.mark.parametrize("name", ["terrain7", "clustered7", "caves6", "random6"])
+MERGE_SCENES = dict(SCENES, terrain9=lambda: (W.terrain_codes(9, seed=1), 9),
+                    caves8=lambda: (W.cave_codes(8, seed=3), 8),
+                    terrain10=lambda: (W.terrain_codes(10, seed=1), 10))
+
+
+@pytest.mark.parametrize("name", ["terrain7", "clustered7", "caves6", "random6", "single", "full8", "terrain9",
+                                  "caves8", "terrain10"])
 def test_cross_level_merge_bit_exact(ctx, name):
-    codes, L = SCENES[name]()
+    codes, L = MERGE_SCENES[name]()
     print(f"[case] csvdag {name} L={L}", flush=True)
     o = O.OracleOctree.from_morton(codes, L)
     o.to_dag()
