@@ -71,6 +71,16 @@ void* svd_ctx_stream(svd_ctx* ctx);
 int svd_octree_from_morton(svd_ctx* ctx, const uint64_t* codes, uint64_t n, uint32_t levels,
                            int codes_on_device, svd_octree** out);
 
+/* svd_octree_from_svdag: an .svdag file image (host or device bytes) decoded
+ * into a DAG-state octree, as EncodedSVDAG::load + EncodedSVDAG::decode do
+ * for svbuilder's .svdag input (encoded_svdag.cpp:43-66, 200-281;
+ * svbuilder/main.cpp:97-101).  Node order per level = file order.  Like the
+ * reference the octree reports the header's node count and 0 voxels (load
+ * does not read a voxel count).  Files whose levels are not tiled exactly by
+ * the referenced nodes (unreferenced or overlapping words, pointers into
+ * other levels as in -multi files) fail with SVD_ERR_FORMAT.               */
+int svd_octree_from_svdag(svd_ctx* ctx, const void* file, uint64_t bytes, int file_on_device, svd_octree** out);
+
 /* svd_octree_from_svo: SVO given level by level on the host, as produced by
  * GeomOctree::buildSVO (geom_octree.cpp:171-280).  words[lev] holds
  * 9*level_sizes[lev] uint32: word 0 = childrenBitmask (bits 0-7), words
@@ -147,16 +157,29 @@ int svd_scene_free(svd_scene* sc);
 int svd_trace(svd_scene* sc, const svd_trace_params* p, svd_ray_result* primary, svd_ray_result* shadow);
 
 /* ---- autotuning (no reference counterpart) ----------------------------------
- * The traversal workgroup covers a tile of tile_w x (256 / tile_w) pixels,
- * tile_w in {8, 16, 32, 64} (default 16).  Results do not depend on it.
- * svd_autotune_trace times every tile shape on one frame (1 warm-up + `reps`
- * timed launches each, HIP events), keeps the fastest in the context and
- * reports it; ms[4] (may be NULL) receives the mean kernel time per shape in
- * the order 8, 16, 32, 64.                                                    */
+ * Launch shape of the traversal; results never depend on it.
+ *  - SVD_TRACE_TILED (default): one workgroup per tile of tile_w x (256 / tile_w) pixels.
+ *  - SVD_TRACE_PERSISTENT: as many workgroups as the device holds;
+ *    a lane that finishes a ray takes the next one from a device-wide queue
+ *    of rays ordered by the same tiles.
+ * tile_w in {8, 16, 32, 64} (default 16).
+ * svd_autotune_trace times every tile shape in the current mode on one frame
+ * (1 warm-up + `reps` timed launches each, HIP events), keeps the fastest in
+ * the context and reports it; ms[4] (may be NULL) receives the mean kernel
+ * time per shape in the order 8, 16, 32, 64.
+ * svd_autotune_trace_modes does the same over both modes: ms[8] (may be NULL)
+ * = tiled 8..64, then persistent 8..64.                                       */
+#define SVD_TRACE_TILED       0
+#define SVD_TRACE_PERSISTENT  1
 int svd_set_trace_tile(svd_ctx* ctx, uint32_t tile_w);
 int svd_get_trace_tile(svd_ctx* ctx, uint32_t* tile_w);
+int svd_set_trace_mode(svd_ctx* ctx, uint32_t mode);
+int svd_get_trace_mode(svd_ctx* ctx, uint32_t* mode);
 int svd_autotune_trace(svd_scene* sc, const svd_trace_params* p, svd_ray_result* primary, svd_ray_result* shadow,
                        uint32_t reps, uint32_t* best_tile_w, double* ms);
+int svd_autotune_trace_modes(svd_scene* sc, const svd_trace_params* p, svd_ray_result* primary,
+                             svd_ray_result* shadow, uint32_t reps, uint32_t* best_tile_w, uint32_t* best_mode,
+                             double* ms);
 
 /* ---- Gray-code attribute bit-tree --------------------------------------------
  * The reference lists the feature (readme.md:8) but has no code for it; the

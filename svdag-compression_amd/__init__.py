@@ -30,6 +30,9 @@ NULL_NODE = 0xFFFFFFFE
 FMT_EXT = {FMT_SVDAG: ".svdag", FMT_USSVDAG: ".ussvdag", FMT_SSVDAG: ".ssvdag", FMT_ESVDAG: ".esvdag"}
 
 
+TRACE_MODES = ("tiled", "persistent")   # SVD_TRACE_TILED, SVD_TRACE_PERSISTENT
+
+
 class SvdError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__(f"svd error {code}: {msg}")
@@ -111,6 +114,7 @@ def lib():
         "svd_ctx_stream": (vp, [vp]),
         "svd_octree_from_morton": (i32, [vp, vp, u64, u32, i32, P(vp)]),
         "svd_octree_from_svo": (i32, [vp, u32, P(u64), P(vp), P(C.c_float), C.c_double, P(vp)]),
+        "svd_octree_from_svdag": (i32, [vp, vp, u64, i32, P(vp)]),
         "svd_octree_free": (i32, [vp]),
         "svd_reduce_dag": (i32, [vp]),
         "svd_reduce_sym": (i32, [vp]),
@@ -159,6 +163,9 @@ def lib():
         "svd_set_trace_tile": (i32, [vp, u32]),
         "svd_get_trace_tile": (i32, [vp, P(u32)]),
         "svd_autotune_trace": (i32, [vp, P(TraceParams), vp, vp, u32, P(u32), P(C.c_double)]),
+        "svd_set_trace_mode": (i32, [vp, u32]),
+        "svd_get_trace_mode": (i32, [vp, P(u32)]),
+        "svd_autotune_trace_modes": (i32, [vp, P(TraceParams), vp, vp, u32, P(u32), P(u32), P(C.c_double)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -169,7 +176,7 @@ def lib():
 
 
 EXPORTED = ["svd_ctx_create", "svd_ctx_destroy", "svd_last_error", "svd_ctx_synchronize", "svd_ctx_stream",
-            "svd_octree_from_morton", "svd_octree_from_svo", "svd_octree_free", "svd_reduce_dag",
+            "svd_octree_from_morton", "svd_octree_from_svo", "svd_octree_from_svdag", "svd_octree_free", "svd_reduce_dag",
             "svd_reduce_sym", "svd_cross_level_merge", "svd_octree_state", "svd_octree_levels",
             "svd_octree_counts", "svd_level_size", "svd_get_level", "svd_encode", "svd_encoded_copy",
             "svd_encoded_device", "svd_scene_create", "svd_scene_from_octree", "svd_scene_free", "svd_trace",
@@ -179,7 +186,8 @@ EXPORTED = ["svd_ctx_create", "svd_ctx_destroy", "svd_last_error", "svd_ctx_sync
             "svd_device_free", "svd_device_read", "svd_workload_balls", "svd_workload_menger",
             "svd_workload_terrain", "svd_batch_build", "svd_batch_free", "svd_batch_scene_info", "svd_batch_svdag",
             "svd_batch_svdag_copy", "svd_batch_trace", "svd_device_copy",
-            "svd_set_trace_tile", "svd_get_trace_tile", "svd_autotune_trace", "svd_workload_materials",
+            "svd_set_trace_tile", "svd_get_trace_tile", "svd_autotune_trace", "svd_set_trace_mode",
+            "svd_get_trace_mode", "svd_autotune_trace_modes", "svd_workload_materials",
             "svd_codes_union"]
 
 
@@ -309,25 +317,41 @@ class Context:
     def trace_tile(self, w: int):
         self.check(self.L.svd_set_trace_tile(self.h, w))
 
+    @property
+    def trace_mode(self) -> str:
+        """"tiled" (one workgroup per pixel tile) or "persistent" (ray queue)."""
+        m = C.c_uint32()
+        self.check(self.L.svd_get_trace_mode(self.h, C.byref(m)))
+        return TRACE_MODES[m.value]
+
+    @trace_mode.setter
+    def trace_mode(self, mode: str):
+        self.check(self.L.svd_set_trace_mode(self.h, TRACE_MODES.index(mode)))
+
     def autotune_trace(self, scene: "Scene", params: TraceParams, prim_ptr: int, shad_ptr: int | None,
                        reps: int = 3, cache: str | None = None, key: str | None = None) -> int:
-        """Pick the fastest traversal tile shape for this frame and keep it in
-        the context.  With a cache file (JSON) and a key, a stored choice is
-        reused and a new one is stored."""
+        """Pick the fastest traversal launch (mode x tile shape) for this frame
+        and keep it in the context; returns the tile width.  With a cache file
+        (JSON) and a key, a stored choice is reused and a new one is stored."""
         import json
         if cache and key and os.path.exists(cache):
             try:
                 stored = json.load(open(cache)).get(key)
             except (OSError, ValueError):
                 stored = None
-            if stored:
+            if stored and stored.get("mode") in TRACE_MODES:
                 self.trace_tile = int(stored["tile_w"])
+                self.trace_mode = stored["mode"]
+                self.last_autotune = stored
                 return self.trace_tile
-        best = C.c_uint32()
-        ms = (C.c_double * 4)()
-        self.check(self.L.svd_autotune_trace(scene.h, C.byref(params), C.c_void_p(prim_ptr),
-                                             C.c_void_p(shad_ptr) if shad_ptr else None, reps, C.byref(best), ms))
-        self.last_autotune = {"tile_w": best.value, "ms": {w: round(ms[i], 4) for i, w in enumerate((8, 16, 32, 64))}}
+        best, mode = C.c_uint32(), C.c_uint32()
+        ms = (C.c_double * 8)()
+        self.check(self.L.svd_autotune_trace_modes(scene.h, C.byref(params), C.c_void_p(prim_ptr),
+                                                   C.c_void_p(shad_ptr) if shad_ptr else None, reps, C.byref(best),
+                                                   C.byref(mode), ms))
+        self.last_autotune = {"tile_w": best.value, "mode": TRACE_MODES[mode.value],
+                              "ms": {m: {w: round(ms[4 * i + j], 4) for j, w in enumerate((8, 16, 32, 64))}
+                                     for i, m in enumerate(TRACE_MODES)}}
         if cache and key:
             data = {}
             if os.path.exists(cache):
@@ -465,6 +489,18 @@ class Octree:
             codes = np.ascontiguousarray(codes, dtype=np.uint64)
             ctx.check(ctx.L.svd_octree_from_morton(ctx.h, codes.ctypes.data_as(C.c_void_p), codes.size, levels, 0,
                                                    C.byref(h)))
+        return cls(ctx, h)
+
+    @classmethod
+    def from_svdag(cls, ctx: Context, file):
+        """Decode an .svdag file image (bytes, or (device_ptr, n_bytes)) into a DAG-state octree."""
+        h = C.c_void_p()
+        if isinstance(file, tuple):
+            ptr, n = file
+            ctx.check(ctx.L.svd_octree_from_svdag(ctx.h, C.c_void_p(ptr), n, 1, C.byref(h)))
+        else:
+            buf = np.frombuffer(bytes(file), dtype=np.uint8)
+            ctx.check(ctx.L.svd_octree_from_svdag(ctx.h, buf.ctypes.data_as(C.c_void_p), buf.size, 0, C.byref(h)))
         return cls(ctx, h)
 
     @classmethod

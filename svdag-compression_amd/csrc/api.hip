@@ -134,6 +134,7 @@ int svd_ctx_create(int device, svd_ctx** out) {
     return SVD_ERR_DEVICE;
   }
   c->arena = new Arena(device);
+  if (hipDeviceGetAttribute(&c->numCU, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) c->numCU = 256;
   *out = c;
   return SVD_OK;
 }
@@ -238,6 +239,20 @@ int svd_octree_from_svo(svd_ctx* c, uint32_t levels, const uint64_t* sizes, cons
     o->state = SVD_S_SVO;
   });
   if (rc) { free_levels(c, o->svo); delete o; return rc; }
+  *out = o;
+  return SVD_OK;
+}
+
+int svd_octree_from_svdag(svd_ctx* c, const void* file, uint64_t bytes, int on_device, svd_octree** out) {
+  if (!c || !file || !out) return SVD_ERR_ARG;
+  *out = nullptr;
+  svd_octree* o = new svd_octree();
+  o->ctx = c;
+  int rc = guarded(c, [&] {
+    Timer t(c, "decode");
+    octree_from_svdag(o, (const uint8_t*)file, bytes, on_device != 0);
+  });
+  if (rc) { delete o; return rc; }
   *out = o;
   return SVD_OK;
 }
@@ -791,6 +806,33 @@ int svd_get_trace_tile(svd_ctx* c, uint32_t* w) {
   return SVD_OK;
 }
 
+int svd_set_trace_mode(svd_ctx* c, uint32_t mode) {
+  if (!c || (mode != SVD_TRACE_TILED && mode != SVD_TRACE_PERSISTENT)) return SVD_ERR_ARG;
+  c->traceMode = mode;
+  return SVD_OK;
+}
+
+int svd_get_trace_mode(svd_ctx* c, uint32_t* mode) {
+  if (!c || !mode) return SVD_ERR_ARG;
+  *mode = c->traceMode;
+  return SVD_OK;
+}
+
+namespace {
+// mean kernel time of `reps` frames (after one warm-up) with the context's current shape
+double time_frames(svd_ctx* c, svd_scene* s, const svd_trace_params* p, svd_ray_result* prim, svd_ray_result* shad,
+                   uint32_t reps) {
+  trace_frame(s, p, prim, shad);
+  SVD_HIP(hipEventRecord(c->ev[0], c->stream));
+  for (uint32_t r = 0; r < reps; ++r) trace_frame(s, p, prim, shad);
+  SVD_HIP(hipEventRecord(c->ev[1], c->stream));
+  SVD_HIP(hipEventSynchronize(c->ev[1]));
+  float e = 0.f;
+  SVD_HIP(hipEventElapsedTime(&e, c->ev[0], c->ev[1]));
+  return (double)e / reps;
+}
+}  // namespace
+
 int svd_autotune_trace(svd_scene* s, const svd_trace_params* p, svd_ray_result* prim, svd_ray_result* shad,
                        uint32_t reps, uint32_t* best, double* ms) {
   if (!s || !p || !prim || reps < 1) return SVD_ERR_ARG;
@@ -798,24 +840,38 @@ int svd_autotune_trace(svd_scene* s, const svd_trace_params* p, svd_ray_result* 
   return guarded(c, [&] {
     const uint32_t shapes[4] = {8, 16, 32, 64};
     double t[4];
-    const uint32_t keep = c->traceTileW;
     for (int k = 0; k < 4; ++k) {
       c->traceTileW = shapes[k];
-      trace_frame(s, p, prim, shad);              // warm-up
-      SVD_HIP(hipEventRecord(c->ev[0], c->stream));
-      for (uint32_t r = 0; r < reps; ++r) trace_frame(s, p, prim, shad);
-      SVD_HIP(hipEventRecord(c->ev[1], c->stream));
-      SVD_HIP(hipEventSynchronize(c->ev[1]));
-      float e = 0.f;
-      SVD_HIP(hipEventElapsedTime(&e, c->ev[0], c->ev[1]));
-      t[k] = (double)e / reps;
+      t[k] = time_frames(c, s, p, prim, shad, reps);
     }
     int b = 0;
     for (int k = 1; k < 4; ++k) if (t[k] < t[b]) b = k;
     c->traceTileW = shapes[b];
-    (void)keep;
     if (best) *best = shapes[b];
     if (ms) for (int k = 0; k < 4; ++k) ms[k] = t[k];
+  });
+}
+
+int svd_autotune_trace_modes(svd_scene* s, const svd_trace_params* p, svd_ray_result* prim, svd_ray_result* shad,
+                             uint32_t reps, uint32_t* best_tile, uint32_t* best_mode, double* ms) {
+  if (!s || !p || !prim || reps < 1) return SVD_ERR_ARG;
+  svd_ctx* c = s->ctx;
+  return guarded(c, [&] {
+    const uint32_t shapes[4] = {8, 16, 32, 64};
+    double t[8];
+    for (int m = 0; m < 2; ++m)
+      for (int k = 0; k < 4; ++k) {
+        c->traceMode = (uint32_t)m;
+        c->traceTileW = shapes[k];
+        t[4 * m + k] = time_frames(c, s, p, prim, shad, reps);
+      }
+    int b = 0;
+    for (int k = 1; k < 8; ++k) if (t[k] < t[b]) b = k;
+    c->traceMode = (uint32_t)(b / 4);
+    c->traceTileW = shapes[b % 4];
+    if (best_tile) *best_tile = c->traceTileW;
+    if (best_mode) *best_mode = c->traceMode;
+    if (ms) for (int k = 0; k < 8; ++k) ms[k] = t[k];
   });
 }
 

@@ -46,6 +46,9 @@ struct svd_ctx {
   hipEvent_t ev[2] = {nullptr, nullptr};
   std::map<std::string, double> kernel_ms;   // last measured per family
   uint32_t traceTileW = 16;                  // traversal workgroup tile: traceTileW x (256 / traceTileW) pixels
+  uint32_t traceMode = SVD_TRACE_TILED;      // SVD_TRACE_TILED or SVD_TRACE_PERSISTENT
+  int numCU = 0;                             // compute units of the device
+  std::map<uint64_t, int> occupancy;         // (kernel id, LDS bytes) -> resident workgroups per CU
 };
 
 namespace svd {
@@ -185,6 +188,8 @@ uint64_t attr_bittree(svd_ctx* c, const uint8_t* dattrs, uint64_t V, int gray, u
 
 // ---- encoders (encode.hip) -----------------------------------------------------
 void encode(svd_octree* o, int fmt);
+// .svdag file image -> DAG-state octree (decode.hip)
+void octree_from_svdag(svd_octree* o, const uint8_t* file, uint64_t bytes, bool onDevice);
 
 // ---- traversal (trace.hip) -----------------------------------------------------
 void trace_frame(svd_scene* s, const svd_trace_params* p, svd_ray_result* prim, svd_ray_result* shad);

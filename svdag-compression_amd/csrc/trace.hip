@@ -4,6 +4,7 @@
 // FP32 throughout; this file is compiled with -ffp-contract=off so every
 // product / sum rounds exactly like the shader's individual GLSL operations.
 // min/max follow the GLSL definitions: min(x,y) = y<x ? y : x.
+#include <algorithm>
 #include "svd_internal.hpp"
 
 namespace svd {
@@ -214,20 +215,31 @@ struct Tracer {
     path = (path ^ ts.level) * 16777619u;
   }
 
-  __device__ void trace(const float oin[3], const float din[3], float tmin, float tmax, float projf,
-                        uint32_t maxIters, uint32_t drawLevel, float out[4], int step[3]) {
+  // Per-ray state between iterations of the traversal loop.
+  struct Ray {
+    TS ts;
+    float o[3], d[3];
+    float tmax, projf;
+    int it;
+    uint32_t maxLevel, maxIters;
+    int step[3];
+  };
+
+  // Setup of traceRay; returns true when the ray is already finished (out set).
+  __device__ __forceinline__ bool begin(Ray& r, const float oin[3], const float din[3], float tmin, float tmax,
+                                        float projf, uint32_t maxIters, uint32_t drawLevel, float out[4]) {
     st.size = 0;
-    step[0] = step[1] = step[2] = 0;
+    r.step[0] = r.step[1] = r.step[2] = 0;
     const float eps = 1e-4f;
     const float scale = 1.0f / (2.0f * s.rootHalfSide);
-    float o[3], d[3], omin[3];
+    float omin[3];
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
       omin[a] = s.center[a] - s.rootHalfSide;
-      o[a] = (oin[a] - omin[a]) * scale;
+      r.o[a] = (oin[a] - omin[a]) * scale;
       const float sg = gsign(din[a]);
-      d[a] = din[a];
-      if (d[a] * sg < eps) d[a] = sg * eps;
+      r.d[a] = din[a];
+      if (r.d[a] * sg < eps) r.d[a] = sg * eps;
     }
     tmin = tmin * scale;
     tmax = tmax * scale;
@@ -236,21 +248,21 @@ struct Tracer {
     for (int a = 0; a < 3; ++a) {
       const float cmn = (s.bmin[a] - omin[a]) * scale;
       const float cmx = (s.bmax[a] - omin[a]) * scale;
-      const float t1 = (cmn - o[a]) / d[a];
-      const float t2 = (cmx - o[a]) / d[a];
+      const float t1 = (cmn - r.o[a]) / r.d[a];
+      const float t2 = (cmx - r.o[a]) / r.d[a];
       tMin[a] = gmin(t1, t2);
       tMax[a] = gmax(t1, t2);
     }
     const float tix = gmax(gmax(tMin[0], 0.0f), gmax(tMin[1], tMin[2]));
     const float tiy = gmin(tMax[0], gmin(tMax[1], tMax[2]));
     tmin = gmax(tix, tmin + 1e-10f);
-    tmax = gmin(tiy, tmax);
-    if (!(tix < tiy)) { out[0] = -4.f; out[1] = 0.f; out[2] = 0.f; out[3] = -1.f; return; }
-    const float oscale = 2.0f * s.rootHalfSide;
-    TS ts = {};
+    r.tmax = gmin(tiy, tmax);
+    if (!(tix < tiy)) { out[0] = -4.f; out[1] = 0.f; out[2] = 0.f; out[3] = -1.f; return true; }
+    TS& ts = r.ts;
+    ts = TS{};
     ts.t_current = tmin;
 #pragma unroll
-    for (int a = 0; a < 3; ++a) { ts.inv[a] = 1.0f / d[a]; ts.di[a] = isign(d[a]); }
+    for (int a = 0; a < 3; ++a) { ts.inv[a] = 1.0f / r.d[a]; ts.di[a] = isign(r.d[a]); }
     ts.mm0 = ts.mm1 = ts.mm2 = 0;
     ts.level = 0;
     ts.cell = 0.5f;
@@ -260,10 +272,10 @@ struct Tracer {
       const float tt = ts.t_current + voxel_eps;
 #pragma unroll
       for (int a = 0; a < 3; ++a) {
-        const float pa = o[a] + tt * d[a];
+        const float pa = r.o[a] + tt * r.d[a];
         ts.idx[a] = (int)(pa / ts.cell);
       }
-      next_crossing(ts, o);
+      next_crossing(ts, r.o);
 #pragma unroll
       for (int a = 0; a < 3; ++a) ts.li[a] = ts.idx[a] % 2;
     }
@@ -271,30 +283,48 @@ struct Tracer {
     ts.node_index = 0;
     fetch_data(ts);
     ts.cli = lin_idx(ts, ts.cns);
-    int it = 0;
-    const uint32_t maxLevel = min((uint32_t)s.innerLevels, drawLevel - 1);
-    do {
-      const bool full = fetch_voxel_bit(ts);
-      if (!full) {
-        dda_next(ts, step);
-        if (!in_bounds(ts.li, ts.cns)) {
-          if (st.size == 0) { out[0] = -1.f; out[1] = 0.f; out[2] = (float)it; out[3] = -1.f; return; }
-          up_in(o, ts);
-        }
-      } else {
-        const bool hit = (ts.level >= maxLevel) || ((ts.cell * projf) < ts.t_current);
-        if (hit) {
-          out[0] = ts.t_current * oscale; out[1] = (float)ts.level; out[2] = (float)it; out[3] = (float)ts.node_index;
-          return;
-        }
-        down_in(o, d, ts);
-        fetch_data(ts);
+    r.it = 0;
+    r.projf = projf;
+    r.maxIters = maxIters;
+    r.maxLevel = min((uint32_t)s.innerLevels, drawLevel - 1);
+    return false;
+  }
+
+  // One iteration of traceRay's loop; returns true when the ray is finished.
+  __device__ __forceinline__ bool step(Ray& r, float out[4]) {
+    TS& ts = r.ts;
+    const bool full = fetch_voxel_bit(ts);
+    if (!full) {
+      dda_next(ts, r.step);
+      if (!in_bounds(ts.li, ts.cns)) {
+        if (st.size == 0) { out[0] = -1.f; out[1] = 0.f; out[2] = (float)r.it; out[3] = -1.f; return true; }
+        up_in(r.o, ts);
       }
-      ts.cli = lin_idx(ts, ts.cns);
-      ++it;
-    } while ((ts.t_current < tmax) && ((uint32_t)it < maxIters));
-    if ((uint32_t)it >= maxIters) { out[0] = -3.f; out[1] = 0.f; out[2] = (float)it; out[3] = -1.f; return; }
-    out[0] = -2.f; out[1] = 0.f; out[2] = (float)it; out[3] = -1.f;
+    } else {
+      const bool hit = (ts.level >= r.maxLevel) || ((ts.cell * r.projf) < ts.t_current);
+      if (hit) {
+        out[0] = ts.t_current * (2.0f * s.rootHalfSide); out[1] = (float)ts.level; out[2] = (float)r.it;
+        out[3] = (float)ts.node_index;
+        return true;
+      }
+      down_in(r.o, r.d, ts);
+      fetch_data(ts);
+    }
+    ts.cli = lin_idx(ts, ts.cns);
+    ++r.it;
+    if ((ts.t_current < r.tmax) && ((uint32_t)r.it < r.maxIters)) return false;
+    if ((uint32_t)r.it >= r.maxIters) { out[0] = -3.f; out[1] = 0.f; out[2] = (float)r.it; out[3] = -1.f; return true; }
+    out[0] = -2.f; out[1] = 0.f; out[2] = (float)r.it; out[3] = -1.f;
+    return true;
+  }
+
+  __device__ void trace(const float oin[3], const float din[3], float tmin, float tmax, float projf,
+                        uint32_t maxIters, uint32_t drawLevel, float out[4], int step_out[3]) {
+    Ray r;
+    if (!begin(r, oin, din, tmin, tmax, projf, maxIters, drawLevel, out)) {
+      while (!step(r, out)) {}
+    }
+    step_out[0] = r.step[0]; step_out[1] = r.step[1]; step_out[2] = r.step[2];
   }
 };
 
@@ -321,16 +351,8 @@ __device__ __forceinline__ void normalize3(float v[3]) {
   v[0] = v[0] / l; v[1] = v[1] / l; v[2] = v[2] / l;
 }
 
-// One pixel of one camera: primary ray, then the shadow ray of a hit.
-template <int FMT>
-__device__ __forceinline__ void trace_pixel(const Dev& s, const svd_trace_params& p, uint32_t px, uint32_t py,
-                                            uint32_t* stackMem, svd_ray_result* __restrict__ prim,
-                                            svd_ray_result* __restrict__ shad) {
-  const int depth = s.innerLevels + 1;
-  const int lane = threadIdx.y * blockDim.x + threadIdx.x;
-  Stack stk{stackMem, stackMem + depth * kBT, stackMem + 2 * depth * kBT, lane, 0};
-  Tracer<FMT> tr(s, stk);
-  // computeCameraRay (octree_dda.frag.glsl:587-603)
+// computeCameraRay (octree_dda.frag.glsl:587-603)
+__device__ __forceinline__ void camera_ray(const svd_trace_params& p, uint32_t px, uint32_t py, float o[3], float d[3]) {
   const float fx = (float)px + 0.5f, fy = (float)py + 0.5f;
   const float sx = (fx / (float)p.width) * 2.0f - 1.0f;
   const float sy = (fy / (float)p.height) * 2.0f - 1.0f;
@@ -344,9 +366,43 @@ __device__ __forceinline__ void trace_pixel(const Dev& s, const svd_trace_params
   float op[4], ep[4];
   mat4_mul(p.view_inv, z4, op);
   mat4_mul(p.view_inv, d4, ep);
-  const float o[3] = {op[0], op[1], op[2]};
-  float d[3] = {ep[0] - op[0], ep[1] - op[1], ep[2] - op[2]};
+  o[0] = op[0]; o[1] = op[1]; o[2] = op[2];
+  d[0] = ep[0] - op[0]; d[1] = ep[1] - op[1]; d[2] = ep[2] - op[2];
   normalize3(d);
+}
+
+// Shadow ray of a primary hit (octree_dda.frag.glsl:686-688, 753-754, 803-815, normSamples = 0).
+struct ShadowRay { float o[3], d[3], tmin, tmax, projf; };
+__device__ __forceinline__ ShadowRay shadow_ray(const Dev& s, const svd_trace_params& p, const float o[3],
+                                                const float d[3], const float res[4], const int step[3]) {
+  ShadowRay r;
+  const float hit[3] = {o[0] + res[0] * d[0], o[1] + res[0] * d[1], o[2] + res[0] * d[2]};
+  const float cellSize = (2.0f * s.rootHalfSide) / ldexpf(1.0f, (int)res[1]);
+  float hn[3] = {-(float)step[0], -(float)step[1], -(float)step[2]};
+  normalize3(hn);
+  const float pp[3] = {hit[0] + (hn[0] * cellSize) * 0.5f, hit[1] + (hn[1] * cellSize) * 0.5f,
+                       hit[2] + (hn[2] * cellSize) * 0.5f};
+  const float l2p[3] = {pp[0] - p.light_pos[0], pp[1] - p.light_pos[1], pp[2] - p.light_pos[2]};
+  const float len = sqrtf(dot3(l2p, l2p));
+  r.projf = len / cellSize;
+  r.d[0] = -(l2p[0] / len); r.d[1] = -(l2p[1] / len); r.d[2] = -(l2p[2] / len);
+  r.o[0] = pp[0]; r.o[1] = pp[1]; r.o[2] = pp[2];
+  r.tmin = 1e-3f * 0.5f;
+  r.tmax = len * 0.5f;
+  return r;
+}
+
+// One pixel of one camera: primary ray, then the shadow ray of a hit.
+template <int FMT>
+__device__ __forceinline__ void trace_pixel(const Dev& s, const svd_trace_params& p, uint32_t px, uint32_t py,
+                                            uint32_t* stackMem, svd_ray_result* __restrict__ prim,
+                                            svd_ray_result* __restrict__ shad) {
+  const int depth = s.innerLevels + 1;
+  const int lane = threadIdx.y * blockDim.x + threadIdx.x;
+  Stack stk{stackMem, stackMem + depth * kBT, stackMem + 2 * depth * kBT, lane, 0};
+  Tracer<FMT> tr(s, stk);
+  float o[3], d[3];
+  camera_ray(p, px, py, o, d);
   float res[4];
   int step[3];
   tr.trace(o, d, 0.0f, 1e30f, p.projection_factor, p.max_iters, p.draw_level, res, step);
@@ -356,27 +412,125 @@ __device__ __forceinline__ void trace_pixel(const Dev& s, const svd_trace_params
   if (!shad) return;
   svd_ray_result sr{-5.f, 0.f, 0.f, -1.f, 0u, 0u};
   if (p.shadows && res[0] >= 0.f) {
-    // octree_dda.frag.glsl:686-688, 753-754, 803-815 (normSamples = 0)
-    const float hit[3] = {o[0] + res[0] * d[0], o[1] + res[0] * d[1], o[2] + res[0] * d[2]};
-    const float cellSize = (2.0f * s.rootHalfSide) / ldexpf(1.0f, (int)res[1]);
-    float hn[3] = {-(float)step[0], -(float)step[1], -(float)step[2]};
-    normalize3(hn);
-    const float pp[3] = {hit[0] + (hn[0] * cellSize) * 0.5f, hit[1] + (hn[1] * cellSize) * 0.5f,
-                         hit[2] + (hn[2] * cellSize) * 0.5f};
-    const float l2p[3] = {pp[0] - p.light_pos[0], pp[1] - p.light_pos[1], pp[2] - p.light_pos[2]};
-    const float len = sqrtf(dot3(l2p, l2p));
-    const float sProj = len / cellSize;
-    const float sd[3] = {-(l2p[0] / len), -(l2p[1] / len), -(l2p[2] / len)};
+    const ShadowRay sh = shadow_ray(s, p, o, d, res, step);
     float sres[4];
     int sstep[3];
     tr.path = 2166136261u;
     tr.bytes = 0;
     tr.bad = false;
-    tr.trace(pp, sd, 1e-3f * 0.5f, len * 0.5f, sProj, p.max_iters, p.draw_level, sres, sstep);
+    tr.trace(sh.o, sh.d, sh.tmin, sh.tmax, sh.projf, p.max_iters, p.draw_level, sres, sstep);
     if (tr.bad) { sres[0] = -6.f; sres[1] = 0.f; sres[3] = -1.f; }
     sr = svd_ray_result{sres[0], sres[1], sres[2], sres[3], tr.path, tr.bytes};
   }
   shad[k] = sr;
+}
+
+// Persistent form: lanes that finished their rays take new ones from a
+// queue, so a wavefront is not held by its slowest ray.
+//  - Refill only once kRefill lanes (or all) are idle: the setup of a new
+//    ray is paid by the whole wavefront, so it is batched.
+//  - Each wavefront owns a chunk of kChunk consecutive rays and takes a new
+//    chunk with one atomic when it runs out (no hot device-wide counter).
+//  - Ray r is pixel (tile-major tiles of tw x 256/tw, pixel-major inside a
+//    tile), so a wavefront's rays stay spatially coherent.
+//  - A lane that finishes a primary hit continues with its shadow ray.
+// Every wavefront leaves once the queue is exhausted and its lanes are idle;
+// each ray ends within max_iters iterations.
+#ifndef SVD_TRACE_REFILL
+#define SVD_TRACE_REFILL 32
+#endif
+constexpr uint32_t kRefill = SVD_TRACE_REFILL;
+constexpr uint32_t kChunk = 1024;
+
+template <int FMT>
+__global__ void __launch_bounds__(kBT) __attribute__((amdgpu_waves_per_eu(5)))
+k_trace_persist(Dev s, svd_trace_params p, svd_ray_result* __restrict__ prim, svd_ray_result* __restrict__ shad,
+                uint32_t* __restrict__ queue, uint32_t tw, uint32_t tilesX, uint32_t total) {
+  enum : uint32_t { kIdle = 0, kStartPrimary, kStartShadow, kPrimary, kShadow };
+  extern __shared__ uint32_t stackMem[];
+  const int depth = s.innerLevels + 1;
+  const uint32_t lane = threadIdx.x & 63u;
+  Stack stk{stackMem, stackMem + depth * kBT, stackMem + 2 * depth * kBT, (int)threadIdx.x, 0};
+  Tracer<FMT> tr(s, stk);
+  typename Tracer<FMT>::Ray ray;
+  const uint32_t th = kBT / tw;
+  uint32_t state = kIdle, px = 0, py = 0;
+  uint32_t qNext = 0, qEnd = 0;   // this wavefront's remaining rays [qNext, qEnd)
+  bool exhausted = false;
+  float out[4] = {0.f, 0.f, 0.f, 0.f};
+  while (true) {
+    const uint64_t idle = __ballot(state == kIdle);
+    const uint32_t nIdle = (uint32_t)__popcll(idle);
+    if (!exhausted && (nIdle >= kRefill || nIdle == 64u)) {
+      const uint32_t avail = qEnd - qNext;
+      uint32_t newBase = 0;
+      if (avail < nIdle) {
+        uint32_t b = 0;
+        if (lane == 0) b = atomicAdd(queue, kChunk);
+        newBase = __shfl(b, 0);
+      }
+      if (state == kIdle) {
+        const uint32_t rank = (uint32_t)__popcll(idle & ((1ull << lane) - 1ull));
+        const uint32_t r = rank < avail ? qNext + rank : (newBase < total ? newBase + (rank - avail) : total);
+        if (r < total) {
+          const uint32_t tile = r / kBT, in = r % kBT;
+          px = (tile % tilesX) * tw + in % tw;
+          py = (tile / tilesX) * th + in / tw;
+          if (px < p.width && py < p.height) state = kStartPrimary;
+        }
+      }
+      if (avail >= nIdle) {
+        qNext += nIdle;
+      } else if (newBase >= total) {
+        qNext = qEnd;
+        exhausted = true;
+      } else {
+        qNext = newBase + (nIdle - avail);
+        qEnd = newBase + kChunk;
+      }
+    }
+    if (!__ballot(state != kIdle)) {
+      if (exhausted) break;
+      continue;
+    }
+    if (state == kIdle) continue;
+    bool done;
+    if (state == kStartPrimary || state == kStartShadow) {
+      float o[3], d[3];
+      camera_ray(p, px, py, o, d);
+      float tmin = 0.0f, tmax = 1e30f, projf = p.projection_factor;
+      if (state == kStartShadow) {   // from the primary's result and steps
+        const ShadowRay sh = shadow_ray(s, p, o, d, out, ray.step);
+#pragma unroll
+        for (int a = 0; a < 3; ++a) { o[a] = sh.o[a]; d[a] = sh.d[a]; }
+        tmin = sh.tmin;
+        tmax = sh.tmax;
+        projf = sh.projf;
+      }
+      state = state == kStartPrimary ? kPrimary : kShadow;
+      tr.path = 2166136261u;
+      tr.bytes = 0;
+      tr.bad = false;
+      done = tr.begin(ray, o, d, tmin, tmax, projf, p.max_iters, p.draw_level, out);
+    } else {
+      done = tr.step(ray, out);
+    }
+    if (!done) continue;
+    if (tr.bad) { out[0] = -6.f; out[1] = 0.f; out[3] = -1.f; }
+    const size_t k = (size_t)py * p.width + px;
+    const svd_ray_result res{out[0], out[1], out[2], out[3], tr.path, tr.bytes};
+    if (state == kShadow) {
+      shad[k] = res;
+      state = kIdle;
+    } else {
+      prim[k] = res;
+      state = kIdle;
+      if (shad) {
+        if (p.shadows && out[0] >= 0.f) state = kStartShadow;
+        else shad[k] = svd_ray_result{-5.f, 0.f, 0.f, -1.f, 0u, 0u};
+      }
+    }
+  }
 }
 
 template <int FMT>
@@ -427,8 +581,45 @@ void trace_frame(svd_scene* sc, const svd_trace_params* p, svd_ray_result* prim,
   if (p->draw_level == 0) throw Error{SVD_ERR_ARG, "draw_level must be >= 1"};
   const uint32_t tw = c->traceTileW, th = kBT / tw;
   const dim3 grid(div_up(p->width, tw), div_up(p->height, th));
-  const dim3 block(tw, th);
   const size_t shmem = (size_t)3 * (d.innerLevels + 1) * kBT * sizeof(uint32_t);
+  if (c->traceMode == SVD_TRACE_PERSISTENT) {
+    const uint64_t total = (uint64_t)grid.x * grid.y * kBT;
+    if (total > 0xF0000000ull) throw Error{SVD_ERR_ARG, "frame too large"};
+    const void* fn = sc->fmt == SVD_FMT_SVDAG     ? (const void*)k_trace_persist<SVD_FMT_SVDAG>
+                     : sc->fmt == SVD_FMT_USSVDAG ? (const void*)k_trace_persist<SVD_FMT_USSVDAG>
+                                                  : (const void*)k_trace_persist<SVD_FMT_SSVDAG>;
+    // the occupancy query costs milliseconds of host time: ask once per (format, LDS size)
+    const uint64_t key = ((uint64_t)sc->fmt << 32) | (uint64_t)shmem;
+    auto it = c->occupancy.find(key);
+    if (it == c->occupancy.end()) {
+      int q = 0;
+      SVD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&q, fn, kBT, shmem));
+      it = c->occupancy.emplace(key, q < 1 ? 1 : q).first;
+    }
+    const int perCU = it->second;
+    const uint64_t want = (uint64_t)perCU * (uint64_t)(c->numCU > 0 ? c->numCU : 256);
+    const uint32_t blocks = (uint32_t)std::min<uint64_t>(want, div_up(total, kBT));
+    uint32_t* queue = dalloc<uint32_t>(c, 1);
+    fill_u32(c, queue, 0u, 1);
+    switch (sc->fmt) {
+      case SVD_FMT_SVDAG:
+        k_trace_persist<SVD_FMT_SVDAG><<<blocks, kBT, shmem, c->stream>>>(d, *p, prim, shad, queue, tw, grid.x,
+                                                                          (uint32_t)total);
+        break;
+      case SVD_FMT_USSVDAG:
+        k_trace_persist<SVD_FMT_USSVDAG><<<blocks, kBT, shmem, c->stream>>>(d, *p, prim, shad, queue, tw, grid.x,
+                                                                            (uint32_t)total);
+        break;
+      default:
+        k_trace_persist<SVD_FMT_SSVDAG><<<blocks, kBT, shmem, c->stream>>>(d, *p, prim, shad, queue, tw, grid.x,
+                                                                           (uint32_t)total);
+        break;
+    }
+    check_launch("k_trace_persist");
+    dfree(c, queue);
+    return;
+  }
+  const dim3 block(tw, th);
   switch (sc->fmt) {
     case SVD_FMT_SVDAG: k_trace<SVD_FMT_SVDAG><<<grid, block, shmem, c->stream>>>(d, *p, prim, shad); break;
     case SVD_FMT_USSVDAG: k_trace<SVD_FMT_USSVDAG><<<grid, block, shmem, c->stream>>>(d, *p, prim, shad); break;

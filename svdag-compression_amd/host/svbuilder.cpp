@@ -79,10 +79,15 @@ int main(int argc, char** argv) {
 
   Scene scene;
   GeomOctree octree(&scene);
+  bool isInputDAG = false;
   if (strstr(inputFile.c_str(), ".obj") || strstr(inputFile.c_str(), ".OBJ")) {
     scene.loadObj(inputFile, true, false, false, false, true);
   } else if (strstr(inputFile.c_str(), ".ply") || strstr(inputFile.c_str(), ".PLY")) {
     if (!scene.loadPly(inputFile)) exit(1);
+  } else if (strstr(inputFile.c_str(), ".svdag")) {   // main.cpp:97-101
+    EncodedSVDAG inputSVDAG;
+    if (!inputSVDAG.load(inputFile) || !inputSVDAG.decode(octree)) exit(1);
+    isInputDAG = true;
   } else {
     printf("Can't read input file '%s'. Only supported ASCII Obj (or PLY) files.\n", inputFile.c_str());
     exit(1);
@@ -112,14 +117,16 @@ int main(int argc, char** argv) {
 
   const svd::aabox3f ab = scene.getAABB();
   svd::aabox3d sceneBBoxD(svd::point3d(ab[0][0], ab[0][1], ab[0][2]), svd::point3d(ab[1][0], ab[1][1], ab[1][2]));
-  if (levelStep == 0) {
-    octree.buildSVO(nLevels, sceneBBoxD, false);
-    octree.toDAG();
-  } else {
-    octree.buildDAG(nLevels, levelStep, sceneBBoxD, true);
+  if (!isInputDAG) {   // main.cpp:147-202
+    if (levelStep == 0) {
+      octree.buildSVO(nLevels, sceneBBoxD, false);
+      octree.toDAG();
+    } else {
+      octree.buildDAG(nLevels, levelStep, sceneBBoxD, true);
+    }
   }
   // main.cpp:177-193: normalize very large / very small boxes
-  {
+  if (!isInputDAG) {
     const float dx = ab[1][0] - ab[0][0], dy = ab[1][1] - ab[0][1], dz = ab[1][2] - ab[0][2];
     const float bboxSize = std::sqrt(dx * dx + dy * dy + dz * dz);
     const float maxBboxSize = 100000.f;

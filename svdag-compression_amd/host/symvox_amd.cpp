@@ -450,6 +450,35 @@ bool GeomOctree::buildFromMorton(const uint64_t* codes, size_t n, unsigned int l
   return true;
 }
 
+bool GeomOctree::fromSVDAG(const std::vector<uint8_t>& file) {
+  if (_oct) { svd_octree_free(_oct); _oct = nullptr; }
+  printf("Decoding nodes...\n");
+  int rc = svd_octree_from_svdag(default_context(), file.data(), file.size(), 0, &_oct);
+  if (rc != SVD_OK) { printf("ERROR: %s\n", svd_last_error(default_context())); _state = S_EMPTY; return false; }
+  printf("Updating pointers...\n");
+  float bb[6], rs = 0;
+  uint32_t lv = 0;
+  memcpy(bb, file.data(), 24);
+  memcpy(&rs, file.data() + 24, 4);
+  memcpy(&lv, file.data() + 28, 4);
+  _bbox = aabox3f(point3f(bb[0], bb[1], bb[2]), point3f(bb[3], bb[4], bb[5]));
+  _rootSide = rs;
+  _levels = lv;
+  uint64_t nn = 0, nv = 0;
+  svd_octree_counts(_oct, &nn, &nv);
+  _stats = Stats();
+  _stats.nTotalVoxels = nv;     // 0: EncodedSVDAG::load reads no voxel count
+  _stats.nNodesDAG = nn;        // the header's node count
+  _state = S_DAG;
+  _cacheState = -1;
+  return true;
+}
+
+bool EncodedSVDAG::decode(GeomOctree& octree) const {
+  if (_file.size() < 44) { printf("ERROR: no .svdag loaded\n"); return false; }
+  return octree.fromSVDAG(_file);
+}
+
 void GeomOctree::toDAG(bool internalCall) {
   if (_state == S_SVO || _state == S_DAG) {
     if (!internalCall) printf("* Transforming SVO -> DAG ... ");

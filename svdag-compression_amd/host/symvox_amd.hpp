@@ -134,6 +134,8 @@ class GeomOctree {
   void buildDAG(unsigned int levels, unsigned int stepLevel, svd::aabox3d bbox, bool verbose = false);
   // Voxel list entry (Morton codes, svd_octree_from_morton).
   bool buildFromMorton(const uint64_t* codes, size_t n, unsigned int levels, bool codesOnDevice = false);
+  // An .svdag file image decoded on the device (svd_octree_from_svdag); used by EncodedSVDAG::decode.
+  bool fromSVDAG(const std::vector<uint8_t>& file);
   void toDAG(bool internalCall = false);
   void toSDAG(bool internalCall = false, bool skipSymmetry = false);
   unsigned int mergeAcrossAllLevels();
@@ -211,6 +213,10 @@ class EncodedSVDAG : public EncodedOctree {
   uint32_t getFirstLeafPtr() const { return _firstLeafPtr; }
   const uint32_t* getDataPtr() const { return _file.empty() ? nullptr : (const uint32_t*)(_file.data() + 44); }
   int format() const override { return SVD_FMT_SVDAG; }
+  // encoded_svdag.cpp:216-281.  The reference returns a GeomOctree by value;
+  // here the decoded DAG goes into an existing octree (state S_DAG), because
+  // the octree owns a device handle and is not copyable.
+  bool decode(GeomOctree& octree) const;
 
  protected:
   void parse() override;

@@ -112,36 +112,55 @@ def test_batch_rejects_bad_input(ctx):
         S.Batch(ctx, c[::-1].copy(), np.array([0, c.size], np.uint64), [5])
 
 
-def test_trace_tile_shapes_give_identical_rays(ctx, tmp_path):
-    """The autotuner's choices never change results: every tile shape traces
-    the same rays bit for bit, single-scene and batched."""
+@pytest.mark.parametrize("fmt", [S.FMT_SVDAG, S.FMT_USSVDAG, S.FMT_SSVDAG])
+def test_trace_launch_shapes_give_identical_rays(ctx, tmp_path, fmt):
+    """The autotuner's choices never change results: both launch modes (tiled,
+    persistent ray queue) and every tile shape trace the same rays bit for bit,
+    on a frame whose size is not a multiple of any tile."""
     codes, L = W.terrain_codes(7, seed=1), 7
     g = S.Octree.from_morton(ctx, codes, L)
     g.to_dag()
-    g.to_sdag()
-    g.encode(S.FMT_SSVDAG)
-    sc = S.Scene.from_octree(g, S.FMT_SSVDAG)
-    p = W.trace_params(200, 120, seed=3, levels=L, shadows=True)
-    n = 200 * 120
+    if fmt != S.FMT_SVDAG:
+        g.to_sdag()
+    g.encode(fmt)
+    sc = S.Scene.from_octree(g, fmt)
+    p = W.trace_params(200, 121, seed=3, levels=L, shadows=True)
+    n = 200 * 121
     prim = torch.zeros(n * 6, dtype=torch.int32, device=DEV)
     shad = torch.zeros(n * 6, dtype=torch.int32, device=DEV)
     ref = None
-    for w in (8, 16, 32, 64):
-        ctx.trace_tile = w
-        sc.trace_device(p, prim.data_ptr(), shad.data_ptr())
-        ctx.sync()
-        got = (prim.cpu().numpy().copy(), shad.cpu().numpy().copy())
-        if ref is None:
-            ref = got
-        else:
-            np.testing.assert_array_equal(got[0], ref[0], err_msg=f"tile {w}")
-            np.testing.assert_array_equal(got[1], ref[1], err_msg=f"tile {w}")
+    for mode in S.TRACE_MODES:
+        ctx.trace_mode = mode
+        for w in (8, 16, 32, 64):
+            ctx.trace_tile = w
+            prim.fill_(-7)
+            shad.fill_(-7)
+            sc.trace_device(p, prim.data_ptr(), shad.data_ptr())
+            ctx.sync()
+            got = (prim.cpu().numpy().copy(), shad.cpu().numpy().copy())
+            if ref is None:
+                ref = got
+            else:
+                np.testing.assert_array_equal(got[0], ref[0], err_msg=f"{mode} tile {w}")
+                np.testing.assert_array_equal(got[1], ref[1], err_msg=f"{mode} tile {w}")
+    for r in ref:   # every pixel written
+        assert not (r.reshape(n, 6) == -7).all(axis=1).any()
+    if fmt != S.FMT_SSVDAG:
+        ctx.trace_mode = "tiled"
+        return
     cache = str(tmp_path / "autotune.json")
-    best = ctx.autotune_trace(sc, p, prim.data_ptr(), shad.data_ptr(), reps=2, cache=cache, key="terrain7-200x120")
+    best = ctx.autotune_trace(sc, p, prim.data_ptr(), shad.data_ptr(), reps=2, cache=cache, key="terrain7-200x121")
     assert best in (8, 16, 32, 64) and ctx.trace_tile == best
-    assert set(ctx.last_autotune["ms"]) == {8, 16, 32, 64}
+    assert set(ctx.last_autotune["ms"]) == set(S.TRACE_MODES)
+    assert all(set(v) == {8, 16, 32, 64} for v in ctx.last_autotune["ms"].values())
+    chosen = ctx.trace_mode
     ctx.trace_tile = 16
-    assert ctx.autotune_trace(sc, p, prim.data_ptr(), shad.data_ptr(), cache=cache, key="terrain7-200x120") == best
+    ctx.trace_mode = "tiled" if chosen == "persistent" else "persistent"
+    assert ctx.autotune_trace(sc, p, prim.data_ptr(), shad.data_ptr(), cache=cache, key="terrain7-200x121") == best
+    assert ctx.trace_mode == chosen
     with pytest.raises(S.SvdError):
         ctx.trace_tile = 12
+    with pytest.raises(ValueError):
+        ctx.trace_mode = "other"
     ctx.trace_tile = 16
+    ctx.trace_mode = "tiled"

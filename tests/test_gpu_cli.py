@@ -113,3 +113,26 @@ def test_svviewer_headless(mesh):
                    "--frames", "2", "--image", f"img{ext}.ppm"], d)
         assert "Rays/s" in out
         assert os.path.getsize(d / f"img{ext}.ppm") > 320 * 240 * 3
+
+
+def test_svbuilder_svdag_input(mesh, tmp_path):
+    """svbuilder model.svdag L 0 (main.cpp:97-101, 147): the decoded DAG skips
+    the build and goes through the same passes as a mesh input; with -c the
+    cross-level merge runs on the decoded DAG."""
+    d, tris, bmin, bmax = mesh
+    L = 7
+    o = O.OracleOctree.from_mesh(tris, L, bmin, bmax)
+    o.to_dag()
+    o.init_child_levels()
+    (tmp_path / "in.svdag").write_bytes(o.encode(0))
+    out = run([os.path.join(BIN, "svbuilder"), "in.svdag", str(L), "0"], tmp_path)
+    assert "Decoding nodes" in out and "RESULTS" in out
+    want = oracle_files(o)
+    for ext, data in want.items():
+        assert open(tmp_path / f"in_{L}{ext}", "rb").read() == data, ext
+    o = O.OracleOctree.from_mesh(tris, L, bmin, bmax)
+    o.to_dag()
+    o.init_child_levels()
+    assert o.merge_cross_levels() == 0
+    run([os.path.join(BIN, "svbuilder"), "in.svdag", str(L), "0", "-c"], tmp_path)
+    assert open(tmp_path / f"in_{L}-multi.svdag", "rb").read() == o.encode(0)
