@@ -440,13 +440,13 @@ __device__ __forceinline__ void trace_pixel(const Dev& s, const svd_trace_params
 #define SVD_TRACE_REFILL 32
 #endif
 constexpr uint32_t kRefill = SVD_TRACE_REFILL;
-constexpr uint32_t kChunk = 1024;
+constexpr uint32_t kChunk = 64;
 
 template <int FMT>
 __global__ void __launch_bounds__(kBT) __attribute__((amdgpu_waves_per_eu(5)))
 k_trace_persist(Dev s, svd_trace_params p, svd_ray_result* __restrict__ prim, svd_ray_result* __restrict__ shad,
                 uint32_t* __restrict__ queue, uint32_t tw, uint32_t tilesX, uint32_t total) {
-  enum : uint32_t { kIdle = 0, kStartPrimary, kStartShadow, kPrimary, kShadow };
+  enum : uint32_t { kIdle = 0, kWantShadow, kPrimary, kShadow };
   extern __shared__ uint32_t stackMem[];
   const int depth = s.innerLevels + 1;
   const uint32_t lane = threadIdx.x & 63u;
@@ -459,9 +459,11 @@ k_trace_persist(Dev s, svd_trace_params p, svd_ray_result* __restrict__ prim, sv
   bool exhausted = false;
   float out[4] = {0.f, 0.f, 0.f, 0.f};
   while (true) {
+    // ---- refill: idle lanes take new pixels, finished primaries start shadows
     const uint64_t idle = __ballot(state == kIdle);
     const uint32_t nIdle = (uint32_t)__popcll(idle);
-    if (!exhausted && (nIdle >= kRefill || nIdle == 64u)) {
+    bool start = state == kWantShadow;
+    if (nIdle && !exhausted) {
       const uint32_t avail = qEnd - qNext;
       uint32_t newBase = 0;
       if (avail < nIdle) {
@@ -476,7 +478,7 @@ k_trace_persist(Dev s, svd_trace_params p, svd_ray_result* __restrict__ prim, sv
           const uint32_t tile = r / kBT, in = r % kBT;
           px = (tile % tilesX) * tw + in % tw;
           py = (tile / tilesX) * th + in / tw;
-          if (px < p.width && py < p.height) state = kStartPrimary;
+          start = px < p.width && py < p.height;
         }
       }
       if (avail >= nIdle) {
@@ -489,17 +491,12 @@ k_trace_persist(Dev s, svd_trace_params p, svd_ray_result* __restrict__ prim, sv
         qEnd = newBase + kChunk;
       }
     }
-    if (!__ballot(state != kIdle)) {
-      if (exhausted) break;
-      continue;
-    }
-    if (state == kIdle) continue;
-    bool done;
-    if (state == kStartPrimary || state == kStartShadow) {
+    bool done = false;
+    if (start) {
       float o[3], d[3];
       camera_ray(p, px, py, o, d);
       float tmin = 0.0f, tmax = 1e30f, projf = p.projection_factor;
-      if (state == kStartShadow) {   // from the primary's result and steps
+      if (state == kWantShadow) {   // from the primary's result and steps
         const ShadowRay sh = shadow_ray(s, p, o, d, out, ray.step);
 #pragma unroll
         for (int a = 0; a < 3; ++a) { o[a] = sh.o[a]; d[a] = sh.d[a]; }
@@ -507,28 +504,40 @@ k_trace_persist(Dev s, svd_trace_params p, svd_ray_result* __restrict__ prim, sv
         tmax = sh.tmax;
         projf = sh.projf;
       }
-      state = state == kStartPrimary ? kPrimary : kShadow;
+      state = state == kWantShadow ? kShadow : kPrimary;
       tr.path = 2166136261u;
       tr.bytes = 0;
       tr.bad = false;
       done = tr.begin(ray, o, d, tmin, tmax, projf, p.max_iters, p.draw_level, out);
-    } else {
-      done = tr.step(ray, out);
     }
-    if (!done) continue;
-    if (tr.bad) { out[0] = -6.f; out[1] = 0.f; out[3] = -1.f; }
-    const size_t k = (size_t)py * p.width + px;
-    const svd_ray_result res{out[0], out[1], out[2], out[3], tr.path, tr.bytes};
-    if (state == kShadow) {
-      shad[k] = res;
-      state = kIdle;
-    } else {
-      prim[k] = res;
-      state = kIdle;
-      if (shad) {
-        if (p.shadows && out[0] >= 0.f) state = kStartShadow;
-        else shad[k] = svd_ray_result{-5.f, 0.f, 0.f, -1.f, 0u, 0u};
+    if (!__ballot(state != kIdle)) {
+      if (exhausted) break;
+      continue;
+    }
+    // ---- traverse until kRefill lanes (or all) wait for work
+    while (true) {
+      if (state == kPrimary || state == kShadow) {
+        if (!done) done = tr.step(ray, out);
+        if (done) {
+          done = false;
+          if (tr.bad) { out[0] = -6.f; out[1] = 0.f; out[3] = -1.f; }
+          const size_t k = (size_t)py * p.width + px;
+          const svd_ray_result res{out[0], out[1], out[2], out[3], tr.path, tr.bytes};
+          if (state == kShadow) {
+            shad[k] = res;
+            state = kIdle;
+          } else {
+            prim[k] = res;
+            state = kIdle;
+            if (shad) {
+              if (p.shadows && out[0] >= 0.f) state = kWantShadow;
+              else shad[k] = svd_ray_result{-5.f, 0.f, 0.f, -1.f, 0u, 0u};
+            }
+          }
+        }
       }
+      const uint32_t waiting = (uint32_t)__popcll(__ballot(state == kIdle || state == kWantShadow));
+      if (waiting >= kRefill || waiting == 64u) break;
     }
   }
 }
