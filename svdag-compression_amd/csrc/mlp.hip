@@ -11,6 +11,7 @@
 //   Pair p = batch rows (2p, 2p+1); d = |e_a - e_b|^2; y = min(H, 32);
 //   loss = sum (d - y)^2 / P.  Each layer's delta is rounded to bf16 once and
 //   feeds both its weight gradient and its bias gradient.
+#include <algorithm>
 #include "svd_internal.hpp"
 #include <rocprim/device/device_radix_sort.hpp>
 #include <cmath>
@@ -48,6 +49,7 @@ struct GemmArgs {
   const float* bias;
   const uint16_t* mask; int64_t mask_sm;   // E_MASK_BF16: keep C[m][n] where mask[m*mask_sm + n] > 0
   uint16_t* outb; float* outf; int64_t o_sm;
+  int kPer; int64_t o_sz;          // split-K: blockIdx.z covers k in [z*kPer, (z+1)*kPer), writes outf + z*o_sz
 };
 
 // One wavefront computes one 32x32 tile of C = A(MxK) B(KxN) with
@@ -65,7 +67,9 @@ __global__ void __launch_bounds__(64) k_gemm(GemmArgs g) {
   f32x16_t acc;
 #pragma unroll
   for (int v = 0; v < 16; ++v) acc[v] = 0.f;
-  for (int k0 = 0; k0 < g.K; k0 += 8) {
+  const int kBeg = (int)blockIdx.z * g.kPer;
+  const int kEnd = min(g.K, kBeg + g.kPer);
+  for (int k0 = kBeg; k0 < kEnd; k0 += 8) {
     bf16x4_t a, b;
     const int kb = k0 + kq;
     if (AM == A_BITS) {
@@ -101,7 +105,7 @@ __global__ void __launch_bounds__(64) k_gemm(GemmArgs g) {
       const uint16_t mk = g.mask[(int64_t)m * g.mask_sm + n];
       g.outb[o] = f2bf(bf2f(mk) > 0.f ? x : 0.f);
     } else {
-      g.outf[o] = x;
+      g.outf[(int64_t)blockIdx.z * g.o_sz + o] = x;
     }
   }
 }
@@ -110,8 +114,53 @@ template <int AM, int EP>
 void gemm(svd_ctx* c, uint64_t M, int N, int K, const GemmArgs& g0) {
   GemmArgs g = g0;
   g.N = N; g.K = K;
+  g.kPer = K;
+  g.o_sz = 0;
   dim3 grid(div_up(M, 32), N / 32);
   k_gemm<AM, EP><<<grid, 64, 0, c->stream>>>(g);
+}
+
+// out[m][n] = sum over splits s = 0..S-1 (in this order) of part[s][m][n]
+__global__ void k_split_sum(const float* __restrict__ part, int S, uint64_t MN, int N, float* __restrict__ out,
+                            int64_t o_sm) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= MN) return;
+  float t = 0.f;
+  for (int s = 0; s < S; ++s) t += part[(uint64_t)s * MN + i];
+  out[(int64_t)(i / N) * o_sm + (int64_t)(i % N)] = t;
+}
+
+// Weight gradients: M x N is small and K (the rows of the batch) is long, so
+// the K range is split over enough workgroups to fill the device; the partial
+// tiles are summed in a fixed order (deterministic, no atomics).
+constexpr int kSplitTiles = 2048;     // workgroups to aim for
+constexpr int kMinKPer = 128;         // at least 16 MFMA steps per workgroup
+uint64_t split_part_floats(uint64_t M, int N, int K) {
+  const uint64_t tiles = div_up(M, 32) * (uint64_t)(N / 32);
+  uint64_t S = std::max<uint64_t>(1, kSplitTiles / tiles);
+  S = std::min<uint64_t>(S, std::max(1, K / kMinKPer));
+  return S * M * N;
+}
+
+template <int AM>
+void gemm_wgrad(svd_ctx* c, uint64_t M, int N, int K, const GemmArgs& g0, float* part) {
+  GemmArgs g = g0;
+  const uint64_t tiles = div_up(M, 32) * (uint64_t)(N / 32);
+  int S = (int)std::max<uint64_t>(1, kSplitTiles / tiles);
+  S = std::min(S, std::max(1, K / kMinKPer));
+  int kPer = (K + S - 1) / S;
+  kPer = (kPer + 7) / 8 * 8;
+  S = (K + kPer - 1) / kPer;
+  float* out = g.outf;
+  const int64_t osm = g.o_sm;
+  g.N = N; g.K = K;
+  g.kPer = kPer;
+  g.outf = part;
+  g.o_sm = N;
+  g.o_sz = (int64_t)M * N;
+  dim3 grid(div_up(M, 32), N / 32, S);
+  k_gemm<AM, E_F32><<<grid, 64, 0, c->stream>>>(g);
+  k_split_sum<<<div_up(M * N, 256), 256, 0, c->stream>>>(part, S, M * N, N, out, osm);
 }
 
 // Column sums of a bf16 [R x N] matrix (bias gradients), one block per
@@ -257,13 +306,19 @@ struct Work {
   uint64_t R = 0;
   uint16_t *h1 = nullptr, *h2 = nullptr, *dE = nullptr, *dz2 = nullptr, *dz1 = nullptr;
   float* e = nullptr;
+  float* part = nullptr;   // split-K partial weight gradients
   void alloc(svd_ctx* c, uint64_t rows) {
     R = rows;
     h1 = dalloc<uint16_t>(c, R * D1); h2 = dalloc<uint16_t>(c, R * D2);
     dE = dalloc<uint16_t>(c, R * D3); dz2 = dalloc<uint16_t>(c, R * D2); dz1 = dalloc<uint16_t>(c, R * D1);
     e = dalloc<float>(c, R * D3);
+    const uint64_t np = std::max(split_part_floats(D0, D1, (int)R),
+                                 std::max(split_part_floats(D1, D2, (int)R), split_part_floats(D2, D3, (int)R)));
+    part = dalloc<float>(c, np);
   }
-  void release(svd_ctx* c) { dfree(c, h1); dfree(c, h2); dfree(c, dE); dfree(c, dz2); dfree(c, dz1); dfree(c, e); }
+  void release(svd_ctx* c) {
+    dfree(c, h1); dfree(c, h2); dfree(c, dE); dfree(c, dz2); dfree(c, dz1); dfree(c, e); dfree(c, part);
+  }
 };
 
 void forward(svd_mlp* m, const uint32_t* X, uint64_t rows, uint16_t* h1, uint16_t* h2, float* e) {
@@ -292,7 +347,7 @@ void backward(svd_mlp* m, const uint32_t* X, const Work& w) {
   // dW3 = h2^T dE  (128 x 32), K = R
   g.mValid = D2; g.a = w.h2; g.a_sm = 1; g.a_sk = D2; g.b = w.dE; g.b_sk = D3; g.b_sn = 1;
   g.outf = m->grads + OFF_W3; g.o_sm = D3;
-  gemm<A_BF16, E_F32>(c, D2, D3, (int)R, g);
+  gemm_wgrad<A_BF16>(c, D2, D3, (int)R, g, w.part);
   k_colsum_bf16<<<D3, 256, 0, c->stream>>>(w.dE, R, D3, m->grads + OFF_B3);
   // dz2 = (dE W3^T) * (h2 > 0)  (R x 128), K = 32
   g = {};
@@ -303,7 +358,7 @@ void backward(svd_mlp* m, const uint32_t* X, const Work& w) {
   g = {};
   g.mValid = D1; g.a = w.h1; g.a_sm = 1; g.a_sk = D1; g.b = w.dz2; g.b_sk = D2; g.b_sn = 1;
   g.outf = m->grads + OFF_W2; g.o_sm = D2;
-  gemm<A_BF16, E_F32>(c, D1, D2, (int)R, g);
+  gemm_wgrad<A_BF16>(c, D1, D2, (int)R, g, w.part);
   k_colsum_bf16<<<D2, 256, 0, c->stream>>>(w.dz2, R, D2, m->grads + OFF_B2);
   // dz1 = (dz2 W2^T) * (h1 > 0)  (R x 256), K = 128
   g = {};
@@ -313,7 +368,7 @@ void backward(svd_mlp* m, const uint32_t* X, const Work& w) {
   // dW1 = x^T dz1  (512 x 256), K = R
   g = {};
   g.mValid = D0; g.abits = X; g.b = w.dz1; g.b_sk = D1; g.b_sn = 1; g.outf = m->grads + OFF_W1; g.o_sm = D1;
-  gemm<A_BITS_T, E_F32>(c, D0, D1, (int)R, g);
+  gemm_wgrad<A_BITS_T>(c, D0, D1, (int)R, g, w.part);
   k_colsum_bf16<<<D1, 256, 0, c->stream>>>(w.dz1, R, D1, m->grads + OFF_B1);
 }
 

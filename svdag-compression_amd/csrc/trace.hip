@@ -45,13 +45,25 @@ struct TS {
   uint32_t level;
 };
 
+// Descent stack in LDS, [depth][lane]: the node, and (mirror bits, level,
+// header) packed in one word when the header fits 16 bits (SSVDAG inner
+// headers, SVDAG masks); USSVDAG headers carry mirror masks and keep a word of
+// their own.
+template <int FMT>
+constexpr int kStackWords = FMT == SVD_FMT_USSVDAG ? 3 : 2;
+
 struct Stack {
   uint32_t* node;   // [depth * kBT + lane]
-  uint32_t* hdr;
-  uint32_t* msk;
+  uint32_t* aux;    // mirror bits | level << 3 | header << 16 (header word apart for USSVDAG)
+  uint32_t* hdr;    // USSVDAG only
   int lane;
   uint32_t size;
 };
+
+template <int FMT>
+__device__ __forceinline__ Stack make_stack(uint32_t* mem, int depth, int lane) {
+  return Stack{mem, mem + depth * kBT, kStackWords<FMT> == 3 ? mem + 2 * depth * kBT : nullptr, lane, 0};
+}
 
 template <int FMT>
 struct Tracer {
@@ -74,20 +86,24 @@ struct Tracer {
     return 0u;
   }
 
+  // SSVDAG inner headers are 16 bits; of an SVDAG header word only the mask
+  // (bits 0-7) is ever read, so both fit beside the mirror bits and level.
   __device__ __forceinline__ void push(int node, uint32_t hdr, int m0, int m1, int m2, uint32_t level) {
-    const int mask = m0 | (m1 << 1) | (m2 << 2) | (int)(level << 3);
+    const uint32_t mask = (uint32_t)(m0 | (m1 << 1) | (m2 << 2)) | (level << 3);
     const uint32_t o = st.size * kBT + st.lane;
-    st.node[o] = (uint32_t)node; st.hdr[o] = hdr; st.msk[o] = (uint32_t)mask;
+    st.node[o] = (uint32_t)node;
+    if (kStackWords<FMT> == 3) { st.aux[o] = mask; st.hdr[o] = hdr; }
+    else st.aux[o] = mask | ((FMT == SVD_FMT_SSVDAG ? hdr : (hdr & 0xFFu)) << 16);
     ++st.size;
   }
   __device__ __forceinline__ void pop(TS& ts) {
     --st.size;
     const uint32_t o = st.size * kBT + st.lane;
     ts.node_index = (int)st.node[o];
-    ts.hdr = st.hdr[o];
-    const int mask = (int)st.msk[o];
-    ts.mm0 = mask & 1; ts.mm1 = (mask >> 1) & 1; ts.mm2 = (mask >> 2) & 1;
-    ts.level = (uint32_t)(mask >> 3) & 255u;
+    const uint32_t a = st.aux[o];
+    ts.hdr = kStackWords<FMT> == 3 ? st.hdr[o] : (a >> 16);
+    ts.mm0 = (int)(a & 1u); ts.mm1 = (int)((a >> 1) & 1u); ts.mm2 = (int)((a >> 2) & 1u);
+    ts.level = (a >> 3) & 255u;
   }
   __device__ __forceinline__ static uint32_t lin_idx(const TS& ts, int sz) {
     const int i0 = (1 - 2 * ts.mm0) * ts.li[0] + ts.mm0 * (sz - 1);
@@ -399,7 +415,7 @@ __device__ __forceinline__ void trace_pixel(const Dev& s, const svd_trace_params
                                             svd_ray_result* __restrict__ shad) {
   const int depth = s.innerLevels + 1;
   const int lane = threadIdx.y * blockDim.x + threadIdx.x;
-  Stack stk{stackMem, stackMem + depth * kBT, stackMem + 2 * depth * kBT, lane, 0};
+  Stack stk = make_stack<FMT>(stackMem, depth, lane);
   Tracer<FMT> tr(s, stk);
   float o[3], d[3];
   camera_ray(p, px, py, o, d);
@@ -450,7 +466,7 @@ k_trace_persist(Dev s, svd_trace_params p, svd_ray_result* __restrict__ prim, sv
   extern __shared__ uint32_t stackMem[];
   const int depth = s.innerLevels + 1;
   const uint32_t lane = threadIdx.x & 63u;
-  Stack stk{stackMem, stackMem + depth * kBT, stackMem + 2 * depth * kBT, (int)threadIdx.x, 0};
+  Stack stk = make_stack<FMT>(stackMem, depth, (int)threadIdx.x);
   Tracer<FMT> tr(s, stk);
   typename Tracer<FMT>::Ray ray;
   const uint32_t th = kBT / tw;
@@ -542,8 +558,15 @@ k_trace_persist(Dev s, svd_trace_params p, svd_ray_result* __restrict__ prim, sv
   }
 }
 
+// Six wavefronts per SIMD (80 VGPRs, a 16-byte spill) measured 5% faster on
+// the 1920x1080 and 3840x2160 terrain frames than the compiler's five (86
+// VGPRs); seven and eight spill more and gain nothing at 1080p.
+#ifndef SVD_TRACE_WAVES_PER_EU
+#define SVD_TRACE_WAVES_PER_EU 6
+#endif
+#define SVD_TRACE_OCC __attribute__((amdgpu_waves_per_eu(SVD_TRACE_WAVES_PER_EU)))
 template <int FMT>
-__global__ void __launch_bounds__(kBT) k_trace(Dev s, svd_trace_params p, svd_ray_result* __restrict__ prim,
+__global__ void __launch_bounds__(kBT) SVD_TRACE_OCC k_trace(Dev s, svd_trace_params p, svd_ray_result* __restrict__ prim,
                                                svd_ray_result* __restrict__ shad) {
   extern __shared__ uint32_t stackMem[];
   const uint32_t px = blockIdx.x * blockDim.x + threadIdx.x;
@@ -555,7 +578,7 @@ __global__ void __launch_bounds__(kBT) k_trace(Dev s, svd_trace_params p, svd_ra
 // Many cameras over many scenes in one launch: blockIdx.z is the camera;
 // every camera has the same resolution, results are [camera][row][column].
 template <int FMT>
-__global__ void __launch_bounds__(kBT) k_trace_batch(const Dev* __restrict__ scenes,
+__global__ void __launch_bounds__(kBT) SVD_TRACE_OCC k_trace_batch(const Dev* __restrict__ scenes,
                                                      const svd_trace_params* __restrict__ cams,
                                                      const uint32_t* __restrict__ camScene,
                                                      svd_ray_result* __restrict__ prim, svd_ray_result* __restrict__ shad) {
@@ -590,7 +613,8 @@ void trace_frame(svd_scene* sc, const svd_trace_params* p, svd_ray_result* prim,
   if (p->draw_level == 0) throw Error{SVD_ERR_ARG, "draw_level must be >= 1"};
   const uint32_t tw = c->traceTileW, th = kBT / tw;
   const dim3 grid(div_up(p->width, tw), div_up(p->height, th));
-  const size_t shmem = (size_t)3 * (d.innerLevels + 1) * kBT * sizeof(uint32_t);
+  const size_t shmem = (size_t)(sc->fmt == SVD_FMT_USSVDAG ? kStackWords<SVD_FMT_USSVDAG> : 2) *
+                       (d.innerLevels + 1) * kBT * sizeof(uint32_t);
   if (c->traceMode == SVD_TRACE_PERSISTENT) {
     const uint64_t total = (uint64_t)grid.x * grid.y * kBT;
     if (total > 0xF0000000ull) throw Error{SVD_ERR_ARG, "frame too large"};
@@ -680,7 +704,7 @@ void trace_batch_svdag(svd_ctx* c, const std::vector<svd_scene>& scenes, const s
   const uint32_t tw = c->traceTileW, th = kBT / tw;
   const dim3 grid(div_up(W, tw), div_up(H, th), (uint32_t)cams.size());
   const dim3 block(tw, th);
-  const size_t shmem = (size_t)3 * (maxInner + 1) * kBT * sizeof(uint32_t);
+  const size_t shmem = (size_t)kStackWords<SVD_FMT_SVDAG> * (maxInner + 1) * kBT * sizeof(uint32_t);
   k_trace_batch<SVD_FMT_SVDAG><<<grid, block, shmem, c->stream>>>(dDev, dCam, dMap, prim, shad);
   check_launch("k_trace_batch");
   SVD_HIP(hipStreamSynchronize(c->stream));
