@@ -81,6 +81,17 @@ int svd_octree_from_morton(svd_ctx* ctx, const uint64_t* codes, uint64_t n, uint
  * other levels as in -multi files) fail with SVD_ERR_FORMAT.               */
 int svd_octree_from_svdag(svd_ctx* ctx, const void* file, uint64_t bytes, int file_on_device, svd_octree** out);
 
+/* svd_octree_from_mesh: GeomOctree::buildSVO (geom_octree.cpp:171-280) +
+ * cleanEmptyNodes (:437-456) on the device, for a triangle soup of ntris
+ * triangles (9 floats each: v0 xyz, v1 xyz, v2 xyz, Scene::getTrianglePtr
+ * order) inside bbox = min xyz, max xyz (double, svbuilder's sceneBBoxD).
+ * The triangle/box overlap test is test_triangle_box.cpp:105-184 in FP64 and
+ * the node order per level is the reference's creation order, so the SVO
+ * equals the reference's node for node.  tris on the host (on_device = 0) or
+ * on the device.                                                             */
+int svd_octree_from_mesh(svd_ctx* ctx, const float* tris, uint64_t ntris, uint32_t levels, const double bbox[6],
+                         int on_device, svd_octree** out);
+
 /* svd_octree_from_svo: SVO given level by level on the host, as produced by
  * GeomOctree::buildSVO (geom_octree.cpp:171-280).  words[lev] holds
  * 9*level_sizes[lev] uint32: word 0 = childrenBitmask (bits 0-7), words

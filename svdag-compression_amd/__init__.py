@@ -115,6 +115,7 @@ def lib():
         "svd_octree_from_morton": (i32, [vp, vp, u64, u32, i32, P(vp)]),
         "svd_octree_from_svo": (i32, [vp, u32, P(u64), P(vp), P(C.c_float), C.c_double, P(vp)]),
         "svd_octree_from_svdag": (i32, [vp, vp, u64, i32, P(vp)]),
+        "svd_octree_from_mesh": (i32, [vp, vp, u64, u32, P(C.c_double), i32, P(vp)]),
         "svd_octree_free": (i32, [vp]),
         "svd_reduce_dag": (i32, [vp]),
         "svd_reduce_sym": (i32, [vp]),
@@ -176,7 +177,7 @@ def lib():
 
 
 EXPORTED = ["svd_ctx_create", "svd_ctx_destroy", "svd_last_error", "svd_ctx_synchronize", "svd_ctx_stream",
-            "svd_octree_from_morton", "svd_octree_from_svo", "svd_octree_from_svdag", "svd_octree_free", "svd_reduce_dag",
+            "svd_octree_from_morton", "svd_octree_from_svo", "svd_octree_from_svdag", "svd_octree_from_mesh", "svd_octree_free", "svd_reduce_dag",
             "svd_reduce_sym", "svd_cross_level_merge", "svd_octree_state", "svd_octree_levels",
             "svd_octree_counts", "svd_level_size", "svd_get_level", "svd_encode", "svd_encoded_copy",
             "svd_encoded_device", "svd_scene_create", "svd_scene_from_octree", "svd_scene_free", "svd_trace",
@@ -489,6 +490,20 @@ class Octree:
             codes = np.ascontiguousarray(codes, dtype=np.uint64)
             ctx.check(ctx.L.svd_octree_from_morton(ctx.h, codes.ctypes.data_as(C.c_void_p), codes.size, levels, 0,
                                                    C.byref(h)))
+        return cls(ctx, h)
+
+    @classmethod
+    def from_mesh(cls, ctx: Context, tris, levels: int, bbox_min, bbox_max):
+        """buildSVO on the device: tris float32 [T, 3, 3] (host array) or (device_ptr, T)."""
+        h = C.c_void_p()
+        bb = (C.c_double * 6)(*[float(x) for x in list(bbox_min) + list(bbox_max)])
+        if isinstance(tris, tuple):
+            ptr, n = tris
+            ctx.check(ctx.L.svd_octree_from_mesh(ctx.h, C.c_void_p(ptr), n, levels, bb, 1, C.byref(h)))
+        else:
+            t = np.ascontiguousarray(tris, dtype=np.float32).reshape(-1, 9)
+            ctx.check(ctx.L.svd_octree_from_mesh(ctx.h, t.ctypes.data_as(C.c_void_p), t.shape[0], levels, bb, 0,
+                                                 C.byref(h)))
         return cls(ctx, h)
 
     @classmethod

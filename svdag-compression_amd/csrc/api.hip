@@ -186,6 +186,40 @@ int svd_octree_from_morton(svd_ctx* c, const uint64_t* codes, uint64_t n, uint32
   return SVD_OK;
 }
 
+int svd_octree_from_mesh(svd_ctx* c, const float* tris, uint64_t ntris, uint32_t levels, const double bbox[6],
+                         int on_device, svd_octree** out) {
+  if (!c || !out || !bbox || (ntris && !tris)) return SVD_ERR_ARG;
+  if (levels < 2 || levels > kMaxLevels) return SVD_ERR_LEVELS;
+  *out = nullptr;
+  svd_octree* o = new svd_octree();
+  o->ctx = c;
+  o->levels = levels;
+  int rc = guarded(c, [&] {
+    Timer t(c, "build_svo");
+    const float* d = tris;
+    float* tmp = nullptr;
+    if (!on_device && ntris) {
+      tmp = dalloc<float>(c, 9 * ntris);
+      SVD_HIP(hipMemcpyAsync(tmp, tris, 36 * ntris, hipMemcpyHostToDevice, c->stream));
+      d = tmp;
+    }
+    build_svo_from_mesh(o, d, ntris, bbox, bbox + 3);
+    dfree(c, tmp);
+    // nVoxels: set bits of the leaf level
+    const Level& leaf = o->svo[levels - 1];
+    std::vector<uint32_t> m(leaf.n);
+    if (leaf.n) SVD_HIP(hipMemcpyAsync(m.data(), leaf.rec, 4 * leaf.n, hipMemcpyDeviceToHost, c->stream));
+    SVD_HIP(hipStreamSynchronize(c->stream));
+    uint64_t v = 0;
+    for (uint32_t x : m) v += (uint64_t)__builtin_popcount(x & 0xFFu);
+    o->nVoxels = v;
+    o->state = SVD_S_SVO;
+  });
+  if (rc) { free_levels(c, o->svo); delete o; return rc; }
+  *out = o;
+  return SVD_OK;
+}
+
 int svd_octree_from_svo(svd_ctx* c, uint32_t levels, const uint64_t* sizes, const uint32_t* const* words,
                         const float bbox[6], double rootSide, svd_octree** out) {
   if (!out || !sizes || !words || !bbox) return SVD_ERR_ARG;

@@ -163,6 +163,8 @@ void fill_u8(svd_ctx* c, uint8_t* d, uint8_t v, uint64_t n);
 
 // ---- build / reduction (build.hip) -----------------------------------------
 void build_svo_from_morton(svd_octree* o, const uint64_t* dcodes, uint64_t n);
+// buildSVO of a triangle soup (voxelize.hip); dtris = 9 floats per triangle on the device
+void build_svo_from_mesh(svd_octree* o, const float* dtris, uint64_t ntris, const double bmin[3], const double bmax[3]);
 uint64_t svo_levels_from_codes(svd_ctx* c, const uint64_t* dcodes, uint64_t n, uint32_t L, std::vector<Level>& svo,
                                std::vector<uint64_t*>* levelCodes);
 // one inner level: exact-key dedup, first occurrence wins -> remap + compacted records

@@ -1,0 +1,374 @@
+// Mesh voxelization into the reference's SVO: GeomOctree::buildSVO
+// (geom_octree.cpp:171-280) + cleanEmptyNodes (:437-456), with the FP64
+// triangle/box overlap test of test_triangle_box.cpp:105-184.
+//
+// The reference walks the triangles in order; for each it runs a DFS from the
+// root with an explicit stack.  Popping node p tests p's 8 child boxes, slot 7
+// first: a hit sets p's mask bit and, if the child does not exist yet, appends
+// it to the next level; every hit child is pushed, so the DFS visits children
+// in ascending slot order.  A level's node order is therefore the order of
+// creation events: by the first triangle that reaches the child, then within
+// that triangle's DFS by the parent's pre-order position (its path from the
+// root, ascending) and by descending slot.
+//
+// Device form, one level at a time: the (triangle, node) pairs of a level, in
+// triangle order, test their 8 child boxes (one lane per pair); the hits are
+// the next level's pairs.  A stable radix sort of the hits by (parent, slot)
+// groups the visits of each child; the first of a group is its first triangle.
+// A second pair of stable sorts orders the children by (first triangle,
+// parent path * 8 + 7 - slot), which is the creation order.  Child centres are
+// the parent's centre plus or minus k, the same FP64 operations as the
+// reference, so every box test sees the reference's numbers.
+#include <rocprim/device/device_radix_sort.hpp>
+#include "svd_internal.hpp"
+
+namespace svd {
+
+namespace {
+
+constexpr int kB = 256;
+
+// ---- test_triangle_box.cpp:38-184 (Akenine-Moller), in double ---------------
+__device__ __forceinline__ bool plane_box_overlap(const double n[3], const double v[3], double h) {
+  double vmin[3], vmax[3];
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    if (n[q] > 0.0f) { vmin[q] = -h - v[q]; vmax[q] = h - v[q]; }
+    else { vmin[q] = h - v[q]; vmax[q] = -h - v[q]; }
+  }
+  if (n[0] * vmin[0] + n[1] * vmin[1] + n[2] * vmin[2] > 0.0) return false;
+  if (n[0] * vmax[0] + n[1] * vmax[1] + n[2] * vmax[2] >= 0.0) return true;
+  return false;
+}
+
+// one separating-axis family test: p_a = s0*a*p[i0] + s1*b*p[i1] for two vertices
+__device__ __forceinline__ bool sat_axis(double a, double b, double fa, double fb, const double* p, const double* q,
+                                         int i0, int i1, double s0, double s1, double h) {
+  const double pa = s0 * a * p[i0] + s1 * b * p[i1];
+  const double pb = s0 * a * q[i0] + s1 * b * q[i1];
+  double mn, mx;
+  if (pa < pb) { mn = pa; mx = pb; } else { mn = pb; mx = pa; }
+  const double rad = (fa + fb) * h;
+  return !(mn > rad || mx < -rad);
+}
+
+__device__ __forceinline__ bool sat_z(const double* e, double fex, double fey, const double* v1, const double* v2,
+                                      double h) {
+  const double p1 = e[1] * v1[0] - e[0] * v1[1], p2 = e[1] * v2[0] - e[0] * v2[1];
+  double mn, mx;
+  if (p2 < p1) { mn = p2; mx = p1; } else { mn = p1; mx = p2; }
+  const double rad = (fey + fex) * h;
+  return !(mn > rad || mx < -rad);
+}
+
+__device__ bool tri_box(const double c[3], double h, const float* tri) {
+  double v0[3], v1[3], v2[3], e0[3], e1[3], e2[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    v0[i] = (double)tri[i] - c[i];
+    v1[i] = (double)tri[3 + i] - c[i];
+    v2[i] = (double)tri[6 + i] - c[i];
+  }
+#pragma unroll
+  for (int i = 0; i < 3; ++i) { e0[i] = v1[i] - v0[i]; e1[i] = v2[i] - v1[i]; e2[i] = v0[i] - v2[i]; }
+  double fex = fabs(e0[0]), fey = fabs(e0[1]), fez = fabs(e0[2]);
+  if (!sat_axis(e0[2], e0[1], fez, fey, v0, v2, 1, 2, 1, -1, h)) return false;   // AXISTEST_X01
+  if (!sat_axis(e0[2], e0[0], fez, fex, v0, v2, 0, 2, -1, 1, h)) return false;   // AXISTEST_Y02
+  if (!sat_z(e0, fex, fey, v1, v2, h)) return false;                             // AXISTEST_Z12
+  fex = fabs(e1[0]); fey = fabs(e1[1]); fez = fabs(e1[2]);
+  if (!sat_axis(e1[2], e1[1], fez, fey, v0, v2, 1, 2, 1, -1, h)) return false;   // AXISTEST_X01
+  if (!sat_axis(e1[2], e1[0], fez, fex, v0, v2, 0, 2, -1, 1, h)) return false;   // AXISTEST_Y02
+  if (!sat_axis(e1[1], e1[0], fey, fex, v0, v1, 0, 1, 1, -1, h)) return false;   // AXISTEST_Z0
+  fex = fabs(e2[0]); fey = fabs(e2[1]); fez = fabs(e2[2]);
+  if (!sat_axis(e2[2], e2[1], fez, fey, v0, v1, 1, 2, 1, -1, h)) return false;   // AXISTEST_X2
+  if (!sat_axis(e2[2], e2[0], fez, fex, v0, v1, 0, 2, -1, 1, h)) return false;   // AXISTEST_Y1
+  if (!sat_z(e2, fex, fey, v1, v2, h)) return false;                             // AXISTEST_Z12
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    double mn = v0[a], mx = v0[a];
+    if (v1[a] < mn) mn = v1[a];
+    if (v1[a] > mx) mx = v1[a];
+    if (v2[a] < mn) mn = v2[a];
+    if (v2[a] > mx) mx = v2[a];
+    if (mn > h || mx < -h) return false;
+  }
+  const double nrm[3] = {e0[1] * e1[2] - e0[2] * e1[1], e0[2] * e1[0] - e0[0] * e1[2], e0[0] * e1[1] - e0[1] * e1[0]};
+  return plane_box_overlap(nrm, v0, h);
+}
+
+struct NodeGeo { double* cx; double* cy; double* cz; uint64_t* path; };
+
+// 1. every (triangle, node) pair tests the node's 8 child boxes; hit bits go
+//    to the node's mask; the number of hits is the pair's visit count
+__global__ void k_vox_test(const float* __restrict__ tris, const uint32_t* __restrict__ pt,
+                           const uint32_t* __restrict__ pn, uint64_t np, NodeGeo g, double k,
+                           uint32_t* __restrict__ mask, uint32_t* __restrict__ cnt, uint8_t* __restrict__ hits) {
+  const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= np) return;
+  const float* tri = tris + 9ull * pt[j];
+  const uint32_t p = pn[j];
+  const double c0 = g.cx[p], c1 = g.cy[p], c2 = g.cz[p];
+  uint32_t m = 0;
+  for (int i = 7; i >= 0; --i) {
+    const double cc[3] = {c0 + ((i & 4) ? +k : -k), c1 + ((i & 2) ? +k : -k), c2 + ((i & 1) ? +k : -k)};
+    if (tri_box(cc, k, tri)) m |= 1u << i;
+  }
+  if (m) atomicOr(&mask[p], m);
+  if (cnt) { cnt[j] = (uint32_t)__popc(m); hits[j] = (uint8_t)m; }
+}
+
+// 2. the hits of each pair, in pair (triangle) order: key = parent * 8 + slot
+__global__ void k_vox_visits(const uint32_t* __restrict__ pt, const uint32_t* __restrict__ pn, uint64_t np,
+                             const uint8_t* __restrict__ hits, const uint32_t* __restrict__ off,
+                             uint64_t* __restrict__ vkey, uint32_t* __restrict__ vt, uint32_t* __restrict__ vpos) {
+  const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= np) return;
+  const uint32_t p = pn[j], m = hits[j];
+  uint32_t o = off[j];
+  for (int i = 0; i < 8; ++i) {
+    if (!((m >> i) & 1u)) continue;
+    vkey[o] = (uint64_t)p * 8 + (uint64_t)i;
+    vt[o] = pt[j];
+    vpos[o] = o;
+    ++o;
+  }
+}
+
+// 3. group heads of the (parent, slot)-sorted visits
+__global__ void k_vox_heads(const uint64_t* __restrict__ sk, uint64_t nv, uint32_t* __restrict__ head) {
+  const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < nv) head[j] = (j == 0 || sk[j] != sk[j - 1]) ? 1u : 0u;
+}
+
+// 4. per child (group): first triangle and creation key; per visit: its group
+//    (gid = exclusive count of heads, so a visit's group is gid + head - 1)
+__global__ void k_vox_groups(const uint64_t* __restrict__ sk, const uint32_t* __restrict__ sv,
+                             const uint32_t* __restrict__ vt, const uint32_t* __restrict__ head,
+                             const uint32_t* __restrict__ gid, uint64_t nv, const uint64_t* __restrict__ ppath,
+                             uint32_t* __restrict__ gKey, uint64_t* __restrict__ gPath, uint32_t* __restrict__ gVal,
+                             uint64_t* __restrict__ gChild, uint32_t* __restrict__ visitGroup) {
+  const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nv) return;
+  const uint32_t gi = gid[j] + head[j] - 1u;
+  visitGroup[sv[j]] = gi;
+  if (!head[j]) return;
+  const uint64_t key = sk[j];
+  const uint32_t p = (uint32_t)(key >> 3), slot = (uint32_t)(key & 7u);
+  gKey[gi] = vt[sv[j]];                        // first triangle (stable sort of a triangle-ordered list)
+  gPath[gi] = ppath[p] * 8 + (7u - slot);      // DFS position of the creation inside that triangle
+  gVal[gi] = gi;
+  gChild[gi] = key;
+}
+
+__global__ void k_gather_u32(const uint32_t* __restrict__ src, const uint32_t* __restrict__ idx, uint64_t n,
+                             uint32_t* __restrict__ dst) {
+  const uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < n) dst[r] = src[idx[r]];
+}
+
+__global__ void k_inverse(const uint32_t* __restrict__ order, uint64_t n, uint32_t* __restrict__ rank) {
+  const uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < n) rank[order[r]] = (uint32_t)r;
+}
+
+// 5. node records of the parent level and geometry of the new level
+__global__ void k_vox_children(const uint64_t* __restrict__ gChild, const uint32_t* __restrict__ rank, uint64_t ng,
+                               uint32_t* __restrict__ rec, NodeGeo pg, NodeGeo cg, double k) {
+  const uint64_t gi = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gi >= ng) return;
+  const uint64_t key = gChild[gi];
+  const uint32_t p = (uint32_t)(key >> 3), i = (uint32_t)(key & 7u);
+  const uint32_t x = rank[gi];
+  rec[(uint64_t)p * kRecWords + 1 + i] = x;
+  cg.cx[x] = pg.cx[p] + ((i & 4) ? +k : -k);
+  cg.cy[x] = pg.cy[p] + ((i & 2) ? +k : -k);
+  cg.cz[x] = pg.cz[p] + ((i & 1) ? +k : -k);
+  cg.path[x] = pg.path[p] * 8 + i;
+}
+
+__global__ void k_vox_next_pairs(const uint32_t* __restrict__ vt, const uint32_t* __restrict__ visitGroup,
+                                 const uint32_t* __restrict__ rank, uint64_t nv, uint32_t* __restrict__ pt,
+                                 uint32_t* __restrict__ pn) {
+  const uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= nv) return;
+  pt[v] = vt[v];
+  pn[v] = rank[visitGroup[v]];
+}
+
+__global__ void k_vox_init_rec(uint32_t* __restrict__ rec, const uint32_t* __restrict__ mask, uint64_t n) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint32_t* r = rec + i * kRecWords;
+  r[0] = mask[i];
+#pragma unroll
+  for (int c = 0; c < 8; ++c) r[1 + c] = kNull;
+}
+
+// cleanEmptyNodes: parents drop children without children
+__global__ void k_vox_clean(uint32_t* __restrict__ rec, uint64_t n, const uint32_t* __restrict__ child, int childLeaf) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint32_t* r = rec + i * kRecWords;
+  for (int c = 0; c < 8; ++c) {
+    const uint32_t ch = r[1 + c];
+    if (ch == kNull) continue;
+    const uint32_t m = childLeaf ? child[ch] : child[(uint64_t)ch * kRecWords];
+    if ((m & 0xFFu) == 0) { r[0] &= ~(1u << c); r[1 + c] = kNull; }
+  }
+}
+
+template <typename K>
+void stable_sort_pairs(svd_ctx* c, K* keys, uint32_t* vals, uint64_t n, int bits) {
+  if (n < 2 || bits <= 0) return;
+  K* k2 = dalloc<K>(c, n);
+  uint32_t* v2 = dalloc<uint32_t>(c, n);
+  size_t bytes = 0;
+  SVD_HIP(rocprim::radix_sort_pairs(nullptr, bytes, keys, k2, vals, v2, (size_t)n, 0, bits, c->stream));
+  void* tmp = c->arena->get(bytes + 16);
+  SVD_HIP(rocprim::radix_sort_pairs(tmp, bytes, keys, k2, vals, v2, (size_t)n, 0, bits, c->stream));
+  SVD_HIP(hipMemcpyAsync(keys, k2, sizeof(K) * n, hipMemcpyDeviceToDevice, c->stream));
+  SVD_HIP(hipMemcpyAsync(vals, v2, 4 * n, hipMemcpyDeviceToDevice, c->stream));
+  c->arena->put(tmp);
+  dfree(c, k2);
+  dfree(c, v2);
+}
+
+int bits_for(uint64_t v) {
+  int b = 1;
+  while (b < 64 && (v >> b)) ++b;
+  return b;
+}
+
+NodeGeo alloc_geo(svd_ctx* c, uint64_t n) {
+  return NodeGeo{dalloc<double>(c, n + 1), dalloc<double>(c, n + 1), dalloc<double>(c, n + 1),
+                 dalloc<uint64_t>(c, n + 1)};
+}
+void free_geo(svd_ctx* c, NodeGeo& g) { dfree(c, g.cx); dfree(c, g.cy); dfree(c, g.cz); dfree(c, g.path); }
+
+}  // namespace
+
+void build_svo_from_mesh(svd_octree* o, const float* dtris, uint64_t ntris, const double bmin[3],
+                         const double bmax[3]) {
+  svd_ctx* c = o->ctx;
+  const uint32_t L = o->levels;
+  if (ntris >= 0xFFFFFFFFull) throw Error{SVD_ERR_CAPACITY, "too many triangles"};
+  // buildSVO:176-184: float box from the double one; rootSide = max side in float
+  float bb[6];
+  for (int i = 0; i < 3; ++i) { bb[i] = (float)bmin[i]; bb[3 + i] = (float)bmax[i]; }
+  float side = 0.f;
+  for (int i = 0; i < 3; ++i) {
+    const float s = ((bb[3 + i] - bb[i]) * 0.5f) * 2.0f;
+    side = i == 0 ? s : std::max(side, s);
+  }
+  for (int i = 0; i < 6; ++i) o->bbox[i] = bb[i];
+  o->rootSide = side;
+  const double rootSide = side;
+  o->svo.assign(L, Level());
+  // level 0: the root, reached by every triangle
+  uint64_t np = ntris, n = 1, nNodes = 0;
+  uint32_t* pt = dalloc<uint32_t>(c, np + 1);
+  uint32_t* pn = dalloc<uint32_t>(c, np + 1);
+  iota_u32(c, pt, np);
+  fill_u32(c, pn, 0u, np);
+  NodeGeo geo = alloc_geo(c, 1);
+  {
+    const double ctr[3] = {(bmin[0] + bmax[0]) * 0.5, (bmin[1] + bmax[1]) * 0.5, (bmin[2] + bmax[2]) * 0.5};
+    const uint64_t zero = 0;
+    SVD_HIP(hipMemcpyAsync(geo.cx, &ctr[0], 8, hipMemcpyHostToDevice, c->stream));
+    SVD_HIP(hipMemcpyAsync(geo.cy, &ctr[1], 8, hipMemcpyHostToDevice, c->stream));
+    SVD_HIP(hipMemcpyAsync(geo.cz, &ctr[2], 8, hipMemcpyHostToDevice, c->stream));
+    SVD_HIP(hipMemcpyAsync(geo.path, &zero, 8, hipMemcpyHostToDevice, c->stream));
+    SVD_HIP(hipStreamSynchronize(c->stream));   // host staging goes out of scope
+  }
+  std::vector<uint32_t*> masks(L, nullptr);
+  for (uint32_t l = 0; l < L; ++l) {
+    const double k = rootSide / double(1ull << (l + 2));   // getHalfSideD(level + 1)
+    masks[l] = dalloc<uint32_t>(c, n + 1);
+    fill_u32(c, masks[l], 0u, n);
+    Level& lv = o->svo[l];
+    lv.n = n;
+    lv.leaf = (l == L - 1);
+    if (l > 0) nNodes += n;
+    if (lv.leaf) {   // voxel bits of the leaf-level nodes
+      if (np) k_vox_test<<<div_up(np, kB), kB, 0, c->stream>>>(dtris, pt, pn, np, geo, k, masks[l], nullptr, nullptr);
+      check_launch("k_vox_test");
+      lv.rec = masks[l];
+      masks[l] = nullptr;
+      break;
+    }
+    uint32_t* cnt = dalloc<uint32_t>(c, np + 1);
+    uint8_t* hits = dalloc<uint8_t>(c, np + 1);
+    uint32_t* total = dalloc<uint32_t>(c, 1);
+    if (np) k_vox_test<<<div_up(np, kB), kB, 0, c->stream>>>(dtris, pt, pn, np, geo, k, masks[l], cnt, hits);
+    check_launch("k_vox_test");
+    scan_exclusive(c, cnt, cnt, np, total);
+    const uint64_t nv = read_u32(c, total);
+    if (nv >= 0xFFFFFFF0ull) throw Error{SVD_ERR_CAPACITY, "too many triangle/node visits in one level"};
+    lv.rec = dalloc<uint32_t>(c, n * kRecWords + 1);
+    if (n) k_vox_init_rec<<<div_up(n, kB), kB, 0, c->stream>>>(lv.rec, masks[l], n);
+    uint64_t* vkey = dalloc<uint64_t>(c, nv + 1);
+    uint32_t* vt = dalloc<uint32_t>(c, nv + 1);
+    uint32_t* vpos = dalloc<uint32_t>(c, nv + 1);
+    if (np) k_vox_visits<<<div_up(np, kB), kB, 0, c->stream>>>(pt, pn, np, hits, cnt, vkey, vt, vpos);
+    check_launch("k_vox_visits");
+    dfree(c, cnt);
+    dfree(c, hits);
+    // group the visits by child (parent, slot); the sort is stable, so the
+    // first visit of a group comes from the child's first triangle
+    stable_sort_pairs<uint64_t>(c, vkey, vpos, nv, 3 + bits_for(n));
+    uint32_t* head = dalloc<uint32_t>(c, nv + 1);
+    uint32_t* gid = dalloc<uint32_t>(c, nv + 1);
+    if (nv) k_vox_heads<<<div_up(nv, kB), kB, 0, c->stream>>>(vkey, nv, head);
+    scan_exclusive(c, head, gid, nv, total);
+    const uint64_t ng = read_u32(c, total);   // children of this level = nodes of the next
+    if (ng >= 0xFFFFFFF0ull) throw Error{SVD_ERR_CAPACITY, "level too large"};
+    uint32_t* gKey = dalloc<uint32_t>(c, ng + 1);
+    uint64_t* gPath = dalloc<uint64_t>(c, ng + 1);
+    uint32_t* gVal = dalloc<uint32_t>(c, ng + 1);
+    uint64_t* gChild = dalloc<uint64_t>(c, ng + 1);
+    uint32_t* visitGroup = dalloc<uint32_t>(c, nv + 1);
+    if (nv)
+      k_vox_groups<<<div_up(nv, kB), kB, 0, c->stream>>>(vkey, vpos, vt, head, gid, nv, geo.path, gKey, gPath, gVal,
+                                                          gChild, visitGroup);
+    check_launch("k_vox_groups");
+    // creation order: by first triangle, then by DFS position inside it
+    stable_sort_pairs<uint64_t>(c, gPath, gVal, ng, 3 * (int)(l + 1));
+    uint32_t* gKey2 = dalloc<uint32_t>(c, ng + 1);
+    uint32_t* rank = dalloc<uint32_t>(c, ng + 1);
+    if (ng) k_gather_u32<<<div_up(ng, kB), kB, 0, c->stream>>>(gKey, gVal, ng, gKey2);
+    stable_sort_pairs<uint32_t>(c, gKey2, gVal, ng, bits_for(ntris));
+    if (ng) k_inverse<<<div_up(ng, kB), kB, 0, c->stream>>>(gVal, ng, rank);
+    NodeGeo next = alloc_geo(c, ng);
+    if (ng) k_vox_children<<<div_up(ng, kB), kB, 0, c->stream>>>(gChild, rank, ng, lv.rec, geo, next, k);
+    dfree(c, pt);
+    dfree(c, pn);
+    pt = dalloc<uint32_t>(c, nv + 1);
+    pn = dalloc<uint32_t>(c, nv + 1);
+    if (nv) k_vox_next_pairs<<<div_up(nv, kB), kB, 0, c->stream>>>(vt, visitGroup, rank, nv, pt, pn);
+    check_launch("k_vox_children");
+    free_geo(c, geo);
+    geo = next;
+    dfree(c, vkey); dfree(c, vt); dfree(c, vpos); dfree(c, head); dfree(c, gid); dfree(c, gKey); dfree(c, gKey2); dfree(c, gPath);
+    dfree(c, gVal); dfree(c, gChild); dfree(c, visitGroup); dfree(c, rank); dfree(c, total);
+    dfree(c, masks[l]);
+    masks[l] = nullptr;
+    np = nv;
+    n = ng;
+  }
+  dfree(c, pt);
+  dfree(c, pn);
+  free_geo(c, geo);
+  // cleanEmptyNodes, bottom-up
+  for (uint32_t l = L - 1; l > 0; --l) {
+    const Level& child = o->svo[l];
+    const Level& par = o->svo[l - 1];
+    if (par.n && child.n)
+      k_vox_clean<<<div_up(par.n, kB), kB, 0, c->stream>>>(par.rec, par.n, child.rec, child.leaf ? 1 : 0);
+  }
+  check_launch("k_vox_clean");
+  SVD_HIP(hipStreamSynchronize(c->stream));
+  o->nNodesSVO = nNodes;
+  o->nVoxels = 0;   // set by the caller from the leaf masks
+}
+
+}  // namespace svd

@@ -339,22 +339,33 @@ void GeomOctree::uploadSVO(NodeData& data) {
   _cacheState = -1;
 }
 
+static_assert(sizeof(point3f) == 12, "triangles are handed over as packed float triples");
+
+// geom_octree.cpp:171-280 on the device (svd_octree_from_mesh): the same
+// creation order and FP64 box tests as the host restatement above.
 void GeomOctree::buildSVO(unsigned int levels, aabox3d bbox, bool internalCall) {
   if (!internalCall) printf("* Building SVO... ");
   fflush(stdout);
   const double t0 = now_s();
-  VoxOut v;
-  voxelize_scene(*_scene, levels, bbox, v, nullptr);
-  _bbox = v.bbox;
-  _rootSide = v.rootSide;
+  if (_oct) { svd_octree_free(_oct); _oct = nullptr; }
+  const double bb[6] = {bbox[0][0], bbox[0][1], bbox[0][2], bbox[1][0], bbox[1][1], bbox[1][2]};
+  const size_t nt = _scene->getNRawTriangles();
+  const float* tris = nt ? &_scene->getTrianglePtr(0)[0][0] : nullptr;
+  check(svd_octree_from_mesh(default_context(), tris, nt, levels, bb, 0, &_oct), "buildSVO");
+  _bbox = aabox3f(point3f((float)bb[0], (float)bb[1], (float)bb[2]), point3f((float)bb[3], (float)bb[4], (float)bb[5]));
+  const point3f sides = _bbox.half_side_lengths();
+  _rootSide = std::max(std::max(sides[0] * 2.0f, sides[1] * 2.0f), sides[2] * 2.0f);
   _levels = levels;
   _stats.buildSVOTime = now_s() - t0;
-  _stats.nNodesSVO = v.nNodes;
-  _stats.nNodesLastLevSVO = v.data[levels - 1].size();
+  uint64_t nn = 0, nv = 0, nl = 0;
+  svd_octree_counts(_oct, &nn, &nv);
+  svd_level_size(_oct, SVD_VIEW_SVO, levels - 1, &nl);
+  _stats.nNodesSVO = nn;
+  _stats.nNodesLastLevSVO = nl;
   _stats.simulatedEncodedSVOSize = (_stats.nNodesSVO - _stats.nNodesLastLevSVO) * 4;
-  _stats.nTotalVoxels = v.nVoxels;
-  uploadSVO(v.data);
+  _stats.nTotalVoxels = nv;
   _state = S_SVO;
+  _cacheState = -1;
   if (!internalCall) printf("OK! [%.3f s]\n", _stats.buildSVOTime);
 }
 
