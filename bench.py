@@ -191,6 +191,61 @@ def batch_bench(S, W, ctx, n_scenes, log):
     return out
 
 
+def mesh_bench(S, W, ctx, levels, log, cpu=True):
+    """Mesh input (the svbuilder case): 64 icospheres (subdivision 3, 81,920
+    triangles, seed 0) voxelized at 2^levels by the device buildSVO, reduced and
+    encoded to the four images.  The oracle runs the same region (buildSVO in
+    triangle-major DFS order + toDAG + encoders + toSDAG + encoders) on one
+    core when cpu is set (this section's CPU baseline leg)."""
+    V, F = W.spheres_mesh(n_spheres=64, subdiv=3, seed=0)
+    tris = np.ascontiguousarray(V[F], np.float32)
+    bmin, bmax = W.mesh_bbox(V)
+    bmin, bmax = bmin.astype(np.float64), bmax.astype(np.float64)
+
+    def gpu_once():
+        t0 = time.perf_counter()
+        g = S.Octree.from_mesh(ctx, tris, levels, bmin, bmax)
+        t1 = time.perf_counter()
+        g.to_dag()
+        for f in (S.FMT_SVDAG, S.FMT_ESVDAG):
+            g.encode(f)
+        g.to_sdag()
+        for f in (S.FMT_USSVDAG, S.FMT_SSVDAG):
+            g.encode(f)
+        ctx.sync()
+        t2 = time.perf_counter()
+        return g, t1 - t0, t2 - t0
+
+    gpu_once()
+    runs = [gpu_once() for _ in range(3)]
+    g = runs[-1][0]
+    nvox = g.counts()[1]
+    t_vox = float(np.median([r[1] for r in runs]))
+    t_all = float(np.median([r[2] for r in runs]))
+    out = {"workload": f"64 icospheres (subdivision 3, {tris.shape[0]} triangles, seed 0) at {1 << levels}^3 "
+                       f"({levels} levels, {nvox} voxels): buildSVO + toDAG + .svdag/.esvdag, toSDAG + .ussvdag/.ssvdag",
+           "voxels": nvox, "svo_ms": round(t_vox * 1e3, 3), "total_ms": round(t_all * 1e3, 3)}
+    if cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+        t0 = time.perf_counter()
+        o = O.OracleOctree.from_mesh(tris, levels, bmin, bmax)
+        t1 = time.perf_counter()
+        o.to_dag()
+        o.init_child_levels()
+        o.encode(0)
+        o.encode(2)
+        o.to_sdag()
+        o.encode(1)
+        o.encode(2)
+        t2 = time.perf_counter()
+        out["cpu_oracle_svo_ms"] = round((t1 - t0) * 1e3, 1)
+        out["cpu_oracle_total_ms"] = round((t2 - t0) * 1e3, 1)
+        out["speedup_total"] = round((t2 - t0) / t_all, 1)
+    log(f"[bench] mesh: {out}")
+    return out
+
+
 def attr_bench(S, W, ctx, levels, log):
     """Config-5 scene: fBm terrain (seed 1) union caves (seed 3) at 2^levels,
     per-voxel u8 materials (value noise seed 4, 16 levels).  Cross-level merge
@@ -282,6 +337,8 @@ def main():
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-levels", type=int, default=11)
+    ap.add_argument("--mesh-levels", type=int, default=8,
+                    help="mesh (svbuilder input) section: 64 icospheres at 2^L; 0 = skip")
     ap.add_argument("--attr-levels", type=int, default=12,
                     help="grid of the cross-level + attribute bit-tree + 4K trace measurement; 0 skips it")
     ap.add_argument("--batch-scenes", type=int, default=256,
@@ -396,6 +453,8 @@ def main():
         out["batch"] = batch_bench(S, W, ctx, args.batch_scenes, log)
     if args.attr_levels:
         out["cross_level_attr"] = attr_bench(S, W, ctx, args.attr_levels, log)
+    if args.mesh_levels:
+        out["mesh"] = mesh_bench(S, W, ctx, args.mesh_levels, log, cpu=not args.no_cpu_baseline)
     if not args.no_cpu_baseline:
         log("[bench] cpu baseline (oracle) ...")
         out["cpu_baseline"] = cpu_baseline(W, L, args.cpu_sample_levels)
