@@ -91,6 +91,17 @@ int svd_octree_from_svdag(svd_ctx* ctx, const void* file, uint64_t bytes, int fi
  * on the device.                                                             */
 int svd_octree_from_mesh(svd_ctx* ctx, const float* tris, uint64_t ntris, uint32_t levels, const double bbox[6],
                          int on_device, svd_octree** out);
+/* svd_octree_from_mesh_subtrees: the SVO that GeomOctree::buildDAG
+ * (geom_octree.cpp:289-435) reduces: a top SVO of step_level + 1 levels, then
+ * one subtree per set voxel of its finest level, each voxelized from all
+ * triangles in its own box, joined under the top in the reference's order.
+ * The result is in the SVO state (svd_reduce_dag gives buildDAG's DAG); the
+ * node count is the reference's (subtree roots not counted).  1 <= step_level
+ * and step_level + 1 < levels (SVD_ERR_ARG otherwise: with step 0 the
+ * reference's top has no finest-node centres, at levels - 1 no subtree level
+ * is left; svbuilder only calls buildDAG for steps > 0).                    */
+int svd_octree_from_mesh_subtrees(svd_ctx* ctx, const float* tris, uint64_t ntris, uint32_t levels,
+                                  uint32_t step_level, const double bbox[6], int on_device, svd_octree** out);
 
 /* svd_octree_from_svo: SVO given level by level on the host, as produced by
  * GeomOctree::buildSVO (geom_octree.cpp:171-280).  words[lev] holds
@@ -166,6 +177,18 @@ int svd_scene_free(svd_scene* sc);
 /* One frame: width*height primary rays (+ shadow rays).  primary / shadow are
  * DEVICE arrays of width*height svd_ray_result; shadow may be NULL.          */
 int svd_trace(svd_scene* sc, const svd_trace_params* p, svd_ray_result* primary, svd_ray_result* shadow);
+/* svd_trace_beam: the same frame with the viewer's minimum-depth optimisation
+ * (OctreeDDARenderer::draw with _useMinDepthOptimization, its default,
+ * octree_dda_renderer.cpp:389-408): a DEPTH_MODE pre-pass over a
+ * (width/8) x (height/8) texture with its own projection factor
+ * (getProjectionFactor(1, 8), :503-507; screenRes = width/8, height/8 as
+ * floats) stores the hit distance of each coarse ray (0 on a miss); every
+ * primary ray then starts at getMinT(8) - 1e-3 (octree_dda.frag.glsl:607-617,
+ * 681).  Texels outside the texture read as 0 (GL leaves them undefined).
+ * min_depth (optional DEVICE array of (width/8)*(height/8) floats) receives
+ * the texture.  Always uses the tiled launch.                                */
+int svd_trace_beam(svd_scene* sc, const svd_trace_params* p, float depth_projection_factor, svd_ray_result* primary,
+                   svd_ray_result* shadow, float* min_depth);
 
 /* ---- autotuning (no reference counterpart) ----------------------------------
  * Launch shape of the traversal; results never depend on it.

@@ -67,6 +67,7 @@ def lib():
             "orc_encode": (C.c_int64, [vp, i32, vp, u64]),
             "orc_std_sort_refcount": (None, [vp, vp, u64]),
             "orc_trace_frame": (i32, [vp, u64, i32, C.POINTER(TraceParams), vp, vp, u32, u32]),
+            "orc_trace_frame_beam": (i32, [vp, u64, i32, C.POINTER(TraceParams), C.c_float, vp, vp, u32, u32, vp]),
             "orc_attr_bittree_encode": (C.c_int64, [vp, u64, i32, vp, u64]),
             "orc_attr_bittree_decode": (i32, [vp, u64, vp, u64]),
             "orc_lossy_merge": (i32, [vp, C.POINTER(LossyParams), vp, u32, vp, u32]),
@@ -244,6 +245,23 @@ def trace_frame(file_bytes: bytes, fmt: int, params, shadows: bool = True, row0:
                                row0, p.height if nrows is None else nrows)
     assert rc == 0, rc
     return prim, shad
+
+
+def trace_frame_beam(file_bytes: bytes, fmt: int, params, depth_projection_factor: float, shadows: bool = True):
+    """trace_frame with the viewer's minimum-depth pre-pass (octree_dda_renderer.cpp:389-408):
+    returns (primary, shadow, min-depth texture [height/8, width/8])."""
+    p = TraceParams()
+    C.memmove(C.addressof(p), C.addressof(params), C.sizeof(TraceParams))
+    n = p.width * p.height
+    prim = np.full(n, 0, RAY_DTYPE)
+    shad = np.full(n, 0, RAY_DTYPE) if shadows else None
+    tex = np.zeros((p.height // 8, p.width // 8), np.float32)
+    buf = np.frombuffer(file_bytes, np.uint8)
+    f = 2 if fmt == 3 else fmt
+    rc = lib().orc_trace_frame_beam(_p(buf), buf.size, f, C.byref(p), C.c_float(depth_projection_factor), _p(prim),
+                                    _p(shad) if shad is not None else None, 0, p.height, _p(tex))
+    assert rc == 0, rc
+    return prim, shad, tex
 
 
 def attr_bittree_encode(attrs: np.ndarray, gray: bool) -> bytes:

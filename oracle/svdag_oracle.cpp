@@ -1090,11 +1090,12 @@ static inline void mat4_mul(const float m[16], const float v[4], float r[4]) {
   }
 }
 
-// computeCameraRay (octree_dda.frag.glsl:587-603) for pixel (px, py).
-static void camera_ray(const TraceParams& p, uint32_t px, uint32_t py, V3& o, V3& d) {
+// computeCameraRay (octree_dda.frag.glsl:587-603) for pixel (px, py);
+// (sw, sh) = the screenRes uniform.
+static void camera_ray(const TraceParams& p, uint32_t px, uint32_t py, float sw, float sh, V3& o, V3& d) {
   const float fx = (float)px + 0.5f, fy = (float)py + 0.5f;
-  const float sx = (fx / (float)p.width) * 2.0f - 1.0f;
-  const float sy = (fy / (float)p.height) * 2.0f - 1.0f;
+  const float sx = (fx / sw) * 2.0f - 1.0f;
+  const float sy = (fy / sh) * 2.0f - 1.0f;
   const float s0[4] = {sx, sy, 0.f, 1.f}, s1[4] = {sx, sy, 1.f, 1.f};
   float w0[4], w1[4];
   mat4_mul(p.projInv, s0, w0); mat4_mul(p.projInv, s1, w1);
@@ -1243,7 +1244,9 @@ int orc_build_mesh(void* h, const float* tris, uint64_t ntris, uint32_t levels, 
 
 int orc_build_dag_mesh(void* h, const float* tris, uint64_t ntris, uint32_t levels, uint32_t stepLevel,
                        const double bbox[6]) {
-  if (levels < 2 || levels > 21 || stepLevel + 1 >= levels) return -2;
+  // stepLevel 0 leaves a one-level top with no finest-node centres (the
+  // reference's buildDAG then indexes an empty leavesCenters): rejected here
+  if (levels < 2 || levels > 21 || stepLevel == 0 || stepLevel + 1 >= levels) return -2;
   buildDAGMesh(*(Octree*)h, tris, (size_t)ntris, levels, stepLevel, bbox, bbox + 3);
   return 0;
 }
@@ -1326,8 +1329,40 @@ void orc_std_sort_refcount(uint32_t* idx, uint32_t* val, uint64_t n) {
 // Trace one frame over an encoded file (bytes of .svdag/.ussvdag/.ssvdag).
 // primary[w*h] and shadow[w*h] results; shadow entries are filled only for
 // primary hits when params->shadows != 0 (others get t = -5).
+// The viewer's minimum-depth pre-pass (octree_dda_renderer.cpp:389-408,
+// DEPTH_MODE octree_dda.frag.glsl:862-872) and getMinT(8) (:607-617); texels
+// outside the texture read as 0 (GL leaves them undefined).
+struct MinDepthTex { const float* t = nullptr; uint32_t w = 0, h = 0; };
+static float depth_texel(const MinDepthTex& m, int x, int y) {
+  return (x >= 0 && y >= 0 && (uint32_t)x < m.w && (uint32_t)y < m.h) ? m.t[(size_t)y * m.w + x] : 0.f;
+}
+static float min_t(const MinDepthTex& m, uint32_t px, uint32_t py) {
+  const int x = (int)((((float)px + 0.5f) - 4.0f) / 8.0f);   // ivec2((gl_FragCoord.xy - 8/2) / 8)
+  const int y = (int)((((float)py + 0.5f) - 4.0f) / 8.0f);
+  const float tl = depth_texel(m, x, y), tr = depth_texel(m, x + 1, y);
+  const float bl = depth_texel(m, x, y + 1), br = depth_texel(m, x + 1, y + 1);
+  return gmin(gmin(tl, tr), gmin(bl, br));
+}
+
+static int trace_frame_impl(const uint8_t* file, uint64_t len, int fmt, const TraceParams* p,
+                            OrcTraceOut* primary, OrcTraceOut* shadow, uint32_t row0, uint32_t nrows,
+                            const float* depthProjf, float* minDepthOut);
+
 int orc_trace_frame(const uint8_t* file, uint64_t len, int fmt, const TraceParams* p,
                     OrcTraceOut* primary, OrcTraceOut* shadow, uint32_t row0, uint32_t nrows) {
+  return trace_frame_impl(file, len, fmt, p, primary, shadow, row0, nrows, nullptr, nullptr);
+}
+
+// Beam-optimised frame: minDepthOut receives the (width/8) x (height/8) texture.
+int orc_trace_frame_beam(const uint8_t* file, uint64_t len, int fmt, const TraceParams* p, float depthProjf,
+                         OrcTraceOut* primary, OrcTraceOut* shadow, uint32_t row0, uint32_t nrows,
+                         float* minDepthOut) {
+  return trace_frame_impl(file, len, fmt, p, primary, shadow, row0, nrows, &depthProjf, minDepthOut);
+}
+
+static int trace_frame_impl(const uint8_t* file, uint64_t len, int fmt, const TraceParams* p,
+                            OrcTraceOut* primary, OrcTraceOut* shadow, uint32_t row0, uint32_t nrows,
+                            const float* depthProjf, float* minDepthOut) {
   Scene s;
   s.fmt = fmt;
   size_t off = 0;
@@ -1364,15 +1399,34 @@ int orc_trace_frame(const uint8_t* file, uint64_t len, int fmt, const TraceParam
   s.rootHalfSide = rs / 2.0f;   // EncodedOctree::getHalfSide(0) (encoded_octree.hpp:68-70)
   for (int a = 0; a < 3; ++a) s.center[a] = (s.bmin[a] + s.bmax[a]) * 0.5f;
   Tracer tr(s);
+  MinDepthTex md;
+  std::vector<float> tex;
+  if (depthProjf) {
+    md.w = p->width / 8;
+    md.h = p->height / 8;
+    tex.assign((size_t)md.w * md.h, 0.f);
+    const float sw = (float)p->width / 8.0f, sh = (float)p->height / 8.0f;
+    for (uint32_t y = 0; y < md.h; ++y)
+      for (uint32_t x = 0; x < md.w; ++x) {
+        V3 o, d;
+        camera_ray(*p, x, y, sw, sh, o, d);
+        float res[4]; int step[3];
+        tr.trace(o, d, 0.0f, 1e30f, *depthProjf, p->maxIters, p->drawLevel, res, step);
+        tex[(size_t)y * md.w + x] = res[0] > 0.f ? res[0] : 0.f;
+      }
+    md.t = tex.data();
+    if (minDepthOut && !tex.empty()) memcpy(minDepthOut, tex.data(), 4 * tex.size());
+  }
   const uint32_t rowEnd = std::min(p->height, row0 + nrows);
   for (uint32_t py = row0; py < rowEnd; ++py)
     for (uint32_t px = 0; px < p->width; ++px) {
       const size_t k = (size_t)py * p->width + px;
       V3 o, d;
-      camera_ray(*p, px, py, o, d);
+      camera_ray(*p, px, py, (float)p->width, (float)p->height, o, d);
       float res[4]; int step[3];
       tr.path = 2166136261u; tr.bytes = 0;
-      tr.trace(o, d, 0.0f, 1e30f, p->projectionFactor, p->maxIters, p->drawLevel, res, step);
+      const float tmin = md.t ? min_t(md, px, py) - 1e-3f : 0.0f;   // octree_dda.frag.glsl:681
+      tr.trace(o, d, tmin, 1e30f, p->projectionFactor, p->maxIters, p->drawLevel, res, step);
       primary[k] = OrcTraceOut{res[0], res[1], res[2], res[3], tr.path, tr.bytes};
       if (!shadow) continue;
       shadow[k] = OrcTraceOut{-5.f, 0, 0, -1.f, 0, 0};

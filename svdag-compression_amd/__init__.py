@@ -116,6 +116,7 @@ def lib():
         "svd_octree_from_svo": (i32, [vp, u32, P(u64), P(vp), P(C.c_float), C.c_double, P(vp)]),
         "svd_octree_from_svdag": (i32, [vp, vp, u64, i32, P(vp)]),
         "svd_octree_from_mesh": (i32, [vp, vp, u64, u32, P(C.c_double), i32, P(vp)]),
+        "svd_octree_from_mesh_subtrees": (i32, [vp, vp, u64, u32, u32, P(C.c_double), i32, P(vp)]),
         "svd_octree_free": (i32, [vp]),
         "svd_reduce_dag": (i32, [vp]),
         "svd_reduce_sym": (i32, [vp]),
@@ -132,6 +133,7 @@ def lib():
         "svd_scene_from_octree": (i32, [vp, i32, P(vp)]),
         "svd_scene_free": (i32, [vp]),
         "svd_trace": (i32, [vp, P(TraceParams), vp, vp]),
+        "svd_trace_beam": (i32, [vp, P(TraceParams), C.c_float, vp, vp, vp]),
         "svd_last_kernel_ms": (i32, [vp, C.c_char_p, P(C.c_double)]),
         "svd_refcount_order": (i32, [vp, vp, vp, u64]),
         "svd_refcount_order_ranges": (i32, [vp, vp, vp, u64, vp, vp, u64]),
@@ -177,10 +179,11 @@ def lib():
 
 
 EXPORTED = ["svd_ctx_create", "svd_ctx_destroy", "svd_last_error", "svd_ctx_synchronize", "svd_ctx_stream",
-            "svd_octree_from_morton", "svd_octree_from_svo", "svd_octree_from_svdag", "svd_octree_from_mesh", "svd_octree_free", "svd_reduce_dag",
+            "svd_octree_from_morton", "svd_octree_from_svo", "svd_octree_from_svdag", "svd_octree_from_mesh", "svd_octree_from_mesh_subtrees",
+            "svd_octree_free", "svd_reduce_dag",
             "svd_reduce_sym", "svd_cross_level_merge", "svd_octree_state", "svd_octree_levels",
             "svd_octree_counts", "svd_level_size", "svd_get_level", "svd_encode", "svd_encoded_copy",
-            "svd_encoded_device", "svd_scene_create", "svd_scene_from_octree", "svd_scene_free", "svd_trace",
+            "svd_encoded_device", "svd_scene_create", "svd_scene_from_octree", "svd_scene_free", "svd_trace", "svd_trace_beam",
             "svd_last_kernel_ms", "svd_refcount_order", "svd_refcount_order_ranges", "svd_attr_bittree_encode", "svd_lossy_merge",
             "svd_lossy_signatures", "svd_mlp_create", "svd_mlp_free", "svd_mlp_get_params", "svd_mlp_set_params",
             "svd_mlp_embed", "svd_mlp_loss_grad", "svd_mlp_train", "svd_knn", "svd_workload_caves",
@@ -507,6 +510,16 @@ class Octree:
         return cls(ctx, h)
 
     @classmethod
+    def from_mesh_subtrees(cls, ctx: Context, tris, levels: int, step_level: int, bbox_min, bbox_max):
+        """buildDAG's joined SVO (top + per-voxel subtrees) on the device; to_dag() completes buildDAG."""
+        h = C.c_void_p()
+        bb = (C.c_double * 6)(*[float(x) for x in list(bbox_min) + list(bbox_max)])
+        t = np.ascontiguousarray(tris, dtype=np.float32).reshape(-1, 9)
+        ctx.check(ctx.L.svd_octree_from_mesh_subtrees(ctx.h, t.ctypes.data_as(C.c_void_p), t.shape[0], levels,
+                                                      step_level, bb, 0, C.byref(h)))
+        return cls(ctx, h)
+
+    @classmethod
     def from_svdag(cls, ctx: Context, file):
         """Decode an .svdag file image (bytes, or (device_ptr, n_bytes)) into a DAG-state octree."""
         h = C.c_void_p()
@@ -707,6 +720,13 @@ class Scene:
             self.free()
         except Exception:
             pass
+
+    def trace_beam_device(self, params: TraceParams, depth_projection_factor: float, prim_ptr: int,
+                          shad_ptr: int | None, min_depth_ptr: int | None = None):
+        """Beam-optimised frame (depth pre-pass at 1/8 resolution) into device buffers."""
+        self.ctx.check(self.ctx.L.svd_trace_beam(self.h, C.byref(params), float(depth_projection_factor),
+                                                 C.c_void_p(prim_ptr), C.c_void_p(shad_ptr) if shad_ptr else None,
+                                                 C.c_void_p(min_depth_ptr) if min_depth_ptr else None))
 
     def trace_device(self, params: TraceParams, prim_ptr: int, shad_ptr: int | None):
         """Trace one frame into device buffers (e.g. torch tensors' data_ptr())."""

@@ -186,8 +186,27 @@ int svd_octree_from_morton(svd_ctx* c, const uint64_t* codes, uint64_t n, uint32
   return SVD_OK;
 }
 
+namespace {
+int mesh_octree(svd_ctx* c, const float* tris, uint64_t ntris, uint32_t levels, int stepLevel, const double bbox[6],
+                int on_device, svd_octree** out);
+}
+
 int svd_octree_from_mesh(svd_ctx* c, const float* tris, uint64_t ntris, uint32_t levels, const double bbox[6],
                          int on_device, svd_octree** out) {
+  return mesh_octree(c, tris, ntris, levels, -1, bbox, on_device, out);
+}
+
+int svd_octree_from_mesh_subtrees(svd_ctx* c, const float* tris, uint64_t ntris, uint32_t levels,
+                                  uint32_t step_level, const double bbox[6], int on_device, svd_octree** out) {
+  if (step_level == 0 || step_level >= 64) return SVD_ERR_ARG;
+  return mesh_octree(c, tris, ntris, levels, (int)step_level, bbox, on_device, out);
+}
+
+}  // extern "C"
+
+namespace {
+int mesh_octree(svd_ctx* c, const float* tris, uint64_t ntris, uint32_t levels, int stepLevel, const double bbox[6],
+                int on_device, svd_octree** out) {
   if (!c || !out || !bbox || (ntris && !tris)) return SVD_ERR_ARG;
   if (levels < 2 || levels > kMaxLevels) return SVD_ERR_LEVELS;
   *out = nullptr;
@@ -203,7 +222,8 @@ int svd_octree_from_mesh(svd_ctx* c, const float* tris, uint64_t ntris, uint32_t
       SVD_HIP(hipMemcpyAsync(tmp, tris, 36 * ntris, hipMemcpyHostToDevice, c->stream));
       d = tmp;
     }
-    build_svo_from_mesh(o, d, ntris, bbox, bbox + 3);
+    if (stepLevel < 0) build_svo_from_mesh(o, d, ntris, bbox, bbox + 3);
+    else build_svo_from_mesh_subtrees(o, d, ntris, bbox, bbox + 3, (uint32_t)stepLevel);
     dfree(c, tmp);
     // nVoxels: set bits of the leaf level
     const Level& leaf = o->svo[levels - 1];
@@ -219,6 +239,10 @@ int svd_octree_from_mesh(svd_ctx* c, const float* tris, uint64_t ntris, uint32_t
   *out = o;
   return SVD_OK;
 }
+
+}  // namespace
+
+extern "C" {
 
 int svd_octree_from_svo(svd_ctx* c, uint32_t levels, const uint64_t* sizes, const uint32_t* const* words,
                         const float bbox[6], double rootSide, svd_octree** out) {
@@ -533,6 +557,15 @@ int svd_trace(svd_scene* s, const svd_trace_params* p, svd_ray_result* prim, svd
   return guarded(s->ctx, [&] {
     Timer t(s->ctx, "trace");
     trace_frame(s, p, prim, shad);
+  });
+}
+
+int svd_trace_beam(svd_scene* s, const svd_trace_params* p, float depth_projection_factor, svd_ray_result* prim,
+                   svd_ray_result* shad, float* min_depth) {
+  if (!s || !p || !prim) return SVD_ERR_ARG;
+  return guarded(s->ctx, [&] {
+    Timer t(s->ctx, "trace");
+    trace_frame_beam(s, p, depth_projection_factor, prim, shad, min_depth);
   });
 }
 

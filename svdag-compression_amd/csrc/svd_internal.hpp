@@ -165,6 +165,9 @@ void fill_u8(svd_ctx* c, uint8_t* d, uint8_t v, uint64_t n);
 void build_svo_from_morton(svd_octree* o, const uint64_t* dcodes, uint64_t n);
 // buildSVO of a triangle soup (voxelize.hip); dtris = 9 floats per triangle on the device
 void build_svo_from_mesh(svd_octree* o, const float* dtris, uint64_t ntris, const double bmin[3], const double bmax[3]);
+// buildDAG's joined SVO: top of stepLevel+1 levels + one voxelized subtree per set voxel of its finest level
+void build_svo_from_mesh_subtrees(svd_octree* o, const float* dtris, uint64_t ntris, const double bmin[3],
+                                  const double bmax[3], uint32_t stepLevel);
 uint64_t svo_levels_from_codes(svd_ctx* c, const uint64_t* dcodes, uint64_t n, uint32_t L, std::vector<Level>& svo,
                                std::vector<uint64_t*>* levelCodes);
 // one inner level: exact-key dedup, first occurrence wins -> remap + compacted records
@@ -195,6 +198,9 @@ void octree_from_svdag(svd_octree* o, const uint8_t* file, uint64_t bytes, bool 
 
 // ---- traversal (trace.hip) -----------------------------------------------------
 void trace_frame(svd_scene* s, const svd_trace_params* p, svd_ray_result* prim, svd_ray_result* shad);
+// beam optimisation: depth pre-pass at 1/8 resolution, then the frame from getMinT(8)
+void trace_frame_beam(svd_scene* s, const svd_trace_params* p, float depthProjf, svd_ray_result* prim,
+                      svd_ray_result* shad, float* minDepthOut);
 
 // ---- lossy DAG (lossy.hip), learned encoder (mlp.hip), kNN (knn.hip) ----------
 void sort_u64(svd_ctx* c, uint64_t* keys, uint64_t n);

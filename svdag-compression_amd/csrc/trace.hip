@@ -368,10 +368,13 @@ __device__ __forceinline__ void normalize3(float v[3]) {
 }
 
 // computeCameraRay (octree_dda.frag.glsl:587-603)
-__device__ __forceinline__ void camera_ray(const svd_trace_params& p, uint32_t px, uint32_t py, float o[3], float d[3]) {
+// (sw, sh) = the shader's screenRes uniform: the frame size, or a fraction of
+// it for the depth pre-pass
+__device__ __forceinline__ void camera_ray(const svd_trace_params& p, uint32_t px, uint32_t py, float sw, float sh,
+                                           float o[3], float d[3]) {
   const float fx = (float)px + 0.5f, fy = (float)py + 0.5f;
-  const float sx = (fx / (float)p.width) * 2.0f - 1.0f;
-  const float sy = (fy / (float)p.height) * 2.0f - 1.0f;
+  const float sx = (fx / sw) * 2.0f - 1.0f;
+  const float sy = (fy / sh) * 2.0f - 1.0f;
   const float s0[4] = {sx, sy, 0.f, 1.f}, s1[4] = {sx, sy, 1.f, 1.f};
   float w0[4], w1[4];
   mat4_mul(p.proj_inv, s0, w0);
@@ -408,20 +411,43 @@ __device__ __forceinline__ ShadowRay shadow_ray(const Dev& s, const svd_trace_pa
   return r;
 }
 
-// One pixel of one camera: primary ray, then the shadow ray of a hit.
+// The minimum-depth pre-pass ("beam optimisation", octree_dda_renderer.cpp:389-408):
+// a texture of (width/8) x (height/8) texels holding the hit distance of a
+// coarse ray per texel (0 where the ray does not hit).  getMinT(8)
+// (octree_dda.frag.glsl:607-617) reads the 2x2 texels around the pixel; GL
+// leaves reads outside the texture undefined, this build reads them as 0.
+struct MinDepth { const float* t; uint32_t w, h; };
+
+__device__ __forceinline__ float depth_texel(const MinDepth& md, int x, int y) {
+  return (x >= 0 && y >= 0 && (uint32_t)x < md.w && (uint32_t)y < md.h) ? md.t[(uint32_t)y * md.w + (uint32_t)x] : 0.f;
+}
+
+__device__ __forceinline__ float min_t(const MinDepth& md, uint32_t px, uint32_t py) {
+  // ivec2((gl_FragCoord.xy - delta / 2) / delta), delta = 8: truncation toward zero
+  const int x = (int)((((float)px + 0.5f) - 4.0f) / 8.0f);
+  const int y = (int)((((float)py + 0.5f) - 4.0f) / 8.0f);
+  const float tl = depth_texel(md, x, y), tr = depth_texel(md, x + 1, y);
+  const float bl = depth_texel(md, x, y + 1), br = depth_texel(md, x + 1, y + 1);
+  return gmin(gmin(tl, tr), gmin(bl, br));
+}
+
+// One pixel of one camera: primary ray, then the shadow ray of a hit.  With a
+// minimum-depth texture the primary ray starts at getMinT(8) - 1e-3
+// (octree_dda.frag.glsl:681).
 template <int FMT>
 __device__ __forceinline__ void trace_pixel(const Dev& s, const svd_trace_params& p, uint32_t px, uint32_t py,
                                             uint32_t* stackMem, svd_ray_result* __restrict__ prim,
-                                            svd_ray_result* __restrict__ shad) {
+                                            svd_ray_result* __restrict__ shad, MinDepth md = MinDepth{nullptr, 0, 0}) {
   const int depth = s.innerLevels + 1;
   const int lane = threadIdx.y * blockDim.x + threadIdx.x;
   Stack stk = make_stack<FMT>(stackMem, depth, lane);
   Tracer<FMT> tr(s, stk);
   float o[3], d[3];
-  camera_ray(p, px, py, o, d);
+  camera_ray(p, px, py, (float)p.width, (float)p.height, o, d);
   float res[4];
   int step[3];
-  tr.trace(o, d, 0.0f, 1e30f, p.projection_factor, p.max_iters, p.draw_level, res, step);
+  const float tmin = md.t ? min_t(md, px, py) - 1e-3f : 0.0f;
+  tr.trace(o, d, tmin, 1e30f, p.projection_factor, p.max_iters, p.draw_level, res, step);
   if (tr.bad) { res[0] = -6.f; res[1] = 0.f; res[3] = -1.f; }
   const size_t k = (size_t)py * p.width + px;
   prim[k] = svd_ray_result{res[0], res[1], res[2], res[3], tr.path, tr.bytes};
@@ -510,7 +536,7 @@ k_trace_persist(Dev s, svd_trace_params p, svd_ray_result* __restrict__ prim, sv
     bool done = false;
     if (start) {
       float o[3], d[3];
-      camera_ray(p, px, py, o, d);
+      camera_ray(p, px, py, (float)p.width, (float)p.height, o, d);
       float tmin = 0.0f, tmax = 1e30f, projf = p.projection_factor;
       if (state == kWantShadow) {   // from the primary's result and steps
         const ShadowRay sh = shadow_ray(s, p, o, d, out, ray.step);
@@ -567,12 +593,33 @@ k_trace_persist(Dev s, svd_trace_params p, svd_ray_result* __restrict__ prim, sv
 #define SVD_TRACE_OCC __attribute__((amdgpu_waves_per_eu(SVD_TRACE_WAVES_PER_EU)))
 template <int FMT>
 __global__ void __launch_bounds__(kBT) SVD_TRACE_OCC k_trace(Dev s, svd_trace_params p, svd_ray_result* __restrict__ prim,
-                                               svd_ray_result* __restrict__ shad) {
+                                               svd_ray_result* __restrict__ shad, MinDepth md) {
   extern __shared__ uint32_t stackMem[];
   const uint32_t px = blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t py = blockIdx.y * blockDim.y + threadIdx.y;
   if (px >= p.width || py >= p.height) return;
-  trace_pixel<FMT>(s, p, px, py, stackMem, prim, shad);
+  trace_pixel<FMT>(s, p, px, py, stackMem, prim, shad, md);
+}
+
+// DEPTH_MODE pre-pass (octree_dda.frag.glsl:862-872) over the (w x h)
+// texture: screenRes = (sw, sh), its own projection factor; a texel keeps the
+// hit distance when it is > 0 and the cleared 0 otherwise.
+template <int FMT>
+__global__ void __launch_bounds__(kBT) SVD_TRACE_OCC k_trace_depth(Dev s, svd_trace_params p, float sw, float sh,
+                                                                     float projf, uint32_t w, uint32_t h,
+                                                                     float* __restrict__ depth) {
+  extern __shared__ uint32_t stackMem[];
+  const uint32_t px = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t py = blockIdx.y * blockDim.y + threadIdx.y;
+  if (px >= w || py >= h) return;
+  const int lane = threadIdx.y * blockDim.x + threadIdx.x;
+  Tracer<FMT> tr(s, make_stack<FMT>(stackMem, s.innerLevels + 1, lane));
+  float o[3], d[3];
+  camera_ray(p, px, py, sw, sh, o, d);
+  float res[4];
+  int step[3];
+  tr.trace(o, d, 0.0f, 1e30f, projf, p.max_iters, p.draw_level, res, step);
+  depth[(size_t)py * w + px] = res[0] > 0.f ? res[0] : 0.f;
 }
 
 // Many cameras over many scenes in one launch: blockIdx.z is the camera;
@@ -595,7 +642,23 @@ __global__ void __launch_bounds__(kBT) SVD_TRACE_OCC k_trace_batch(const Dev* __
 
 }  // namespace
 
+void trace_frame_impl(svd_scene* sc, const svd_trace_params* p, svd_ray_result* prim, svd_ray_result* shad,
+                      const float* depthProjf, float* minDepthOut);
+
 void trace_frame(svd_scene* sc, const svd_trace_params* p, svd_ray_result* prim, svd_ray_result* shad) {
+  trace_frame_impl(sc, p, prim, shad, nullptr, nullptr);
+}
+
+// Beam-optimised frame: the DEPTH_MODE pre-pass at (width/8) x (height/8)
+// with its own projection factor, then the frame with every primary ray
+// starting at getMinT(8) - 1e-3 (octree_dda_renderer.cpp:389-408).
+void trace_frame_beam(svd_scene* sc, const svd_trace_params* p, float depthProjf, svd_ray_result* prim,
+                      svd_ray_result* shad, float* minDepthOut) {
+  trace_frame_impl(sc, p, prim, shad, &depthProjf, minDepthOut);
+}
+
+void trace_frame_impl(svd_scene* sc, const svd_trace_params* p, svd_ray_result* prim, svd_ray_result* shad,
+                      const float* depthProjf, float* minDepthOut) {
   svd_ctx* c = sc->ctx;
   Dev d{};
   d.nodes = sc->nodes;
@@ -615,7 +678,33 @@ void trace_frame(svd_scene* sc, const svd_trace_params* p, svd_ray_result* prim,
   const dim3 grid(div_up(p->width, tw), div_up(p->height, th));
   const size_t shmem = (size_t)(sc->fmt == SVD_FMT_USSVDAG ? kStackWords<SVD_FMT_USSVDAG> : 2) *
                        (d.innerLevels + 1) * kBT * sizeof(uint32_t);
-  if (c->traceMode == SVD_TRACE_PERSISTENT) {
+  MinDepth md{nullptr, 0, 0};
+  float* depth = nullptr;
+  if (depthProjf) {
+    // the pre-pass texture is (int)width / 8 x (int)height / 8; its screenRes
+    // uniform is the float width / 8, height / 8
+    md.w = p->width / 8;
+    md.h = p->height / 8;
+    depth = minDepthOut ? minDepthOut : dalloc<float>(c, (uint64_t)md.w * md.h + 1);
+    md.t = depth;
+    if (md.w && md.h) {
+      const float sw = (float)p->width / 8.0f, sh = (float)p->height / 8.0f;
+      const dim3 dg(div_up(md.w, tw), div_up(md.h, th)), db(tw, th);
+      switch (sc->fmt) {
+        case SVD_FMT_SVDAG:
+          k_trace_depth<SVD_FMT_SVDAG><<<dg, db, shmem, c->stream>>>(d, *p, sw, sh, *depthProjf, md.w, md.h, depth);
+          break;
+        case SVD_FMT_USSVDAG:
+          k_trace_depth<SVD_FMT_USSVDAG><<<dg, db, shmem, c->stream>>>(d, *p, sw, sh, *depthProjf, md.w, md.h, depth);
+          break;
+        default:
+          k_trace_depth<SVD_FMT_SSVDAG><<<dg, db, shmem, c->stream>>>(d, *p, sw, sh, *depthProjf, md.w, md.h, depth);
+          break;
+      }
+      check_launch("k_trace_depth");
+    }
+  }
+  if (c->traceMode == SVD_TRACE_PERSISTENT && !depthProjf) {
     const uint64_t total = (uint64_t)grid.x * grid.y * kBT;
     if (total > 0xF0000000ull) throw Error{SVD_ERR_ARG, "frame too large"};
     const void* fn = sc->fmt == SVD_FMT_SVDAG     ? (const void*)k_trace_persist<SVD_FMT_SVDAG>
@@ -654,11 +743,12 @@ void trace_frame(svd_scene* sc, const svd_trace_params* p, svd_ray_result* prim,
   }
   const dim3 block(tw, th);
   switch (sc->fmt) {
-    case SVD_FMT_SVDAG: k_trace<SVD_FMT_SVDAG><<<grid, block, shmem, c->stream>>>(d, *p, prim, shad); break;
-    case SVD_FMT_USSVDAG: k_trace<SVD_FMT_USSVDAG><<<grid, block, shmem, c->stream>>>(d, *p, prim, shad); break;
-    default: k_trace<SVD_FMT_SSVDAG><<<grid, block, shmem, c->stream>>>(d, *p, prim, shad); break;
+    case SVD_FMT_SVDAG: k_trace<SVD_FMT_SVDAG><<<grid, block, shmem, c->stream>>>(d, *p, prim, shad, md); break;
+    case SVD_FMT_USSVDAG: k_trace<SVD_FMT_USSVDAG><<<grid, block, shmem, c->stream>>>(d, *p, prim, shad, md); break;
+    default: k_trace<SVD_FMT_SSVDAG><<<grid, block, shmem, c->stream>>>(d, *p, prim, shad, md); break;
   }
   check_launch("k_trace");
+  if (depth && depth != minDepthOut) dfree(c, depth);
 }
 
 Dev scene_dev(const svd_scene* sc) {

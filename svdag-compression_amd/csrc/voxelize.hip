@@ -96,17 +96,21 @@ __device__ bool tri_box(const double c[3], double h, const float* tri) {
   return plane_box_overlap(nrm, v0, h);
 }
 
-struct NodeGeo { double* cx; double* cy; double* cz; uint64_t* path; };
+// per node: centre, path from its root (3 bits per level) and root index
+struct NodeGeo { double* cx; double* cy; double* cz; uint64_t* path; uint32_t* root; };
 
 // 1. every (triangle, node) pair tests the node's 8 child boxes; hit bits go
 //    to the node's mask; the number of hits is the pair's visit count
+// k = getHalfSideD(level + 1) of the node's root: rootSide / 2^(level + 2)
 __global__ void k_vox_test(const float* __restrict__ tris, const uint32_t* __restrict__ pt,
-                           const uint32_t* __restrict__ pn, uint64_t np, NodeGeo g, double k,
-                           uint32_t* __restrict__ mask, uint32_t* __restrict__ cnt, uint8_t* __restrict__ hits) {
+                           const uint32_t* __restrict__ pn, uint64_t np, NodeGeo g, const double* __restrict__ side,
+                           double scale, uint32_t* __restrict__ mask, uint32_t* __restrict__ cnt,
+                           uint8_t* __restrict__ hits) {
   const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= np) return;
   const float* tri = tris + 9ull * pt[j];
   const uint32_t p = pn[j];
+  const double k = side[g.root[p]] / scale;
   const double c0 = g.cx[p], c1 = g.cy[p], c2 = g.cz[p];
   uint32_t m = 0;
   for (int i = 7; i >= 0; --i) {
@@ -173,17 +177,54 @@ __global__ void k_inverse(const uint32_t* __restrict__ order, uint64_t n, uint32
 
 // 5. node records of the parent level and geometry of the new level
 __global__ void k_vox_children(const uint64_t* __restrict__ gChild, const uint32_t* __restrict__ rank, uint64_t ng,
-                               uint32_t* __restrict__ rec, NodeGeo pg, NodeGeo cg, double k) {
+                               uint32_t* __restrict__ rec, NodeGeo pg, NodeGeo cg, const double* __restrict__ side,
+                               double scale) {
   const uint64_t gi = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (gi >= ng) return;
   const uint64_t key = gChild[gi];
   const uint32_t p = (uint32_t)(key >> 3), i = (uint32_t)(key & 7u);
   const uint32_t x = rank[gi];
+  const uint32_t r = pg.root[p];
+  const double k = side[r] / scale;
   rec[(uint64_t)p * kRecWords + 1 + i] = x;
   cg.cx[x] = pg.cx[p] + ((i & 4) ? +k : -k);
   cg.cy[x] = pg.cy[p] + ((i & 2) ? +k : -k);
   cg.cz[x] = pg.cz[p] + ((i & 1) ? +k : -k);
   cg.path[x] = pg.path[p] * 8 + i;
+  cg.root[x] = r;
+}
+
+// roots' key for the last (most significant) creation-order sort
+__global__ void k_gather_root(const uint64_t* __restrict__ gChild, const uint32_t* __restrict__ order, uint64_t ng,
+                              const uint32_t* __restrict__ proot, uint32_t* __restrict__ key) {
+  const uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < ng) key[r] = proot[(uint32_t)(gChild[order[r]] >> 3)];
+}
+
+// level-0 pairs of R roots, triangle-major: pair t*R + r = (t, r)
+__global__ void k_vox_root_pairs(uint64_t ntris, uint32_t R, uint32_t* __restrict__ pt, uint32_t* __restrict__ pn) {
+  const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= ntris * R) return;
+  pt[j] = (uint32_t)(j / R);
+  pn[j] = (uint32_t)(j % R);
+}
+
+// The top SVO's finest level becomes an inner level whose children are the
+// subtree roots, numbered i ascending, slot 7..0 (geom_octree.cpp:397-417).
+__global__ void k_vox_join_top(const uint32_t* __restrict__ mask, const uint32_t* __restrict__ base, uint64_t n,
+                               uint32_t* __restrict__ rec) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t m = mask[i] & 0xFFu;
+  uint32_t* r = rec + i * kRecWords;
+  r[0] = m;
+  for (int j = 0; j < 8; ++j)
+    r[1 + j] = ((m >> j) & 1u) ? base[i] + (uint32_t)__popc(m >> (j + 1)) : kNull;
+}
+
+__global__ void k_popc_mask(const uint32_t* __restrict__ mask, uint64_t n, uint32_t* __restrict__ out) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = (uint32_t)__popc(mask[i] & 0xFFu);
 }
 
 __global__ void k_vox_next_pairs(const uint32_t* __restrict__ vt, const uint32_t* __restrict__ visitGroup,
@@ -241,71 +282,87 @@ int bits_for(uint64_t v) {
 
 NodeGeo alloc_geo(svd_ctx* c, uint64_t n) {
   return NodeGeo{dalloc<double>(c, n + 1), dalloc<double>(c, n + 1), dalloc<double>(c, n + 1),
-                 dalloc<uint64_t>(c, n + 1)};
+                 dalloc<uint64_t>(c, n + 1), dalloc<uint32_t>(c, n + 1)};
 }
-void free_geo(svd_ctx* c, NodeGeo& g) { dfree(c, g.cx); dfree(c, g.cy); dfree(c, g.cz); dfree(c, g.path); }
+void free_geo(svd_ctx* c, NodeGeo& g) {
+  dfree(c, g.cx); dfree(c, g.cy); dfree(c, g.cz); dfree(c, g.path); dfree(c, g.root);
+}
 
-}  // namespace
-
-void build_svo_from_mesh(svd_octree* o, const float* dtris, uint64_t ntris, const double bmin[3],
-                         const double bmax[3]) {
-  svd_ctx* c = o->ctx;
-  const uint32_t L = o->levels;
-  if (ntris >= 0xFFFFFFFFull) throw Error{SVD_ERR_CAPACITY, "too many triangles"};
-  // buildSVO:176-184: float box from the double one; rootSide = max side in float
-  float bb[6];
-  for (int i = 0; i < 3; ++i) { bb[i] = (float)bmin[i]; bb[3 + i] = (float)bmax[i]; }
+// One root box of buildSVO (geom_octree.cpp:176-184): the float box of the
+// double one, rootSide = its largest side in float, centre of the double box.
+struct VoxRoot { double c[3]; double side; float bb[6]; };
+VoxRoot root_of_box(const double bmin[3], const double bmax[3]) {
+  VoxRoot r;
+  for (int i = 0; i < 3; ++i) { r.bb[i] = (float)bmin[i]; r.bb[3 + i] = (float)bmax[i]; }
   float side = 0.f;
   for (int i = 0; i < 3; ++i) {
-    const float s = ((bb[3 + i] - bb[i]) * 0.5f) * 2.0f;
+    const float s = ((r.bb[3 + i] - r.bb[i]) * 0.5f) * 2.0f;
     side = i == 0 ? s : std::max(side, s);
   }
-  for (int i = 0; i < 6; ++i) o->bbox[i] = bb[i];
-  o->rootSide = side;
-  const double rootSide = side;
-  o->svo.assign(L, Level());
-  // level 0: the root, reached by every triangle
-  uint64_t np = ntris, n = 1, nNodes = 0;
+  r.side = side;
+  for (int i = 0; i < 3; ++i) r.c[i] = (bmin[i] + bmax[i]) * 0.5;
+  return r;
+}
+
+// R independent SVOs of L levels (one per root box) over the same triangles,
+// joined level by level in root order (each root's nodes in its creation
+// order, child pointers into the joined next level), cleanEmptyNodes applied
+// inside every SVO.  Returns the nodes created below the roots; lastCenters
+// (optional) receives the centres of the finest level's nodes.
+uint64_t voxelize_roots(svd_ctx* c, const float* dtris, uint64_t ntris, uint32_t L, const std::vector<VoxRoot>& roots,
+                        std::vector<Level>& out, std::vector<double>* lastCenters) {
+  const uint64_t R = roots.size();
+  if (ntris >= 0xFFFFFFFFull || R == 0 || ntris * R >= 0xFFFFFFF0ull)
+    throw Error{SVD_ERR_CAPACITY, "too many triangles (x roots)"};
+  out.assign(L, Level());
+  uint64_t np = ntris * R, n = R, nNodes = 0;
   uint32_t* pt = dalloc<uint32_t>(c, np + 1);
   uint32_t* pn = dalloc<uint32_t>(c, np + 1);
-  iota_u32(c, pt, np);
-  fill_u32(c, pn, 0u, np);
-  NodeGeo geo = alloc_geo(c, 1);
+  if (np) k_vox_root_pairs<<<div_up(np, kB), kB, 0, c->stream>>>(ntris, (uint32_t)R, pt, pn);
+  NodeGeo geo = alloc_geo(c, R);
+  double* side = dalloc<double>(c, R + 1);
   {
-    const double ctr[3] = {(bmin[0] + bmax[0]) * 0.5, (bmin[1] + bmax[1]) * 0.5, (bmin[2] + bmax[2]) * 0.5};
-    const uint64_t zero = 0;
-    SVD_HIP(hipMemcpyAsync(geo.cx, &ctr[0], 8, hipMemcpyHostToDevice, c->stream));
-    SVD_HIP(hipMemcpyAsync(geo.cy, &ctr[1], 8, hipMemcpyHostToDevice, c->stream));
-    SVD_HIP(hipMemcpyAsync(geo.cz, &ctr[2], 8, hipMemcpyHostToDevice, c->stream));
-    SVD_HIP(hipMemcpyAsync(geo.path, &zero, 8, hipMemcpyHostToDevice, c->stream));
+    std::vector<double> cx(R), cy(R), cz(R), sd(R);
+    std::vector<uint64_t> path(R, 0);
+    std::vector<uint32_t> rid(R);
+    for (uint64_t r = 0; r < R; ++r) {
+      cx[r] = roots[r].c[0]; cy[r] = roots[r].c[1]; cz[r] = roots[r].c[2]; sd[r] = roots[r].side;
+      rid[r] = (uint32_t)r;
+    }
+    SVD_HIP(hipMemcpyAsync(geo.cx, cx.data(), 8 * R, hipMemcpyHostToDevice, c->stream));
+    SVD_HIP(hipMemcpyAsync(geo.cy, cy.data(), 8 * R, hipMemcpyHostToDevice, c->stream));
+    SVD_HIP(hipMemcpyAsync(geo.cz, cz.data(), 8 * R, hipMemcpyHostToDevice, c->stream));
+    SVD_HIP(hipMemcpyAsync(geo.path, path.data(), 8 * R, hipMemcpyHostToDevice, c->stream));
+    SVD_HIP(hipMemcpyAsync(geo.root, rid.data(), 4 * R, hipMemcpyHostToDevice, c->stream));
+    SVD_HIP(hipMemcpyAsync(side, sd.data(), 8 * R, hipMemcpyHostToDevice, c->stream));
     SVD_HIP(hipStreamSynchronize(c->stream));   // host staging goes out of scope
   }
-  std::vector<uint32_t*> masks(L, nullptr);
+  uint32_t* total = dalloc<uint32_t>(c, 1);
   for (uint32_t l = 0; l < L; ++l) {
-    const double k = rootSide / double(1ull << (l + 2));   // getHalfSideD(level + 1)
-    masks[l] = dalloc<uint32_t>(c, n + 1);
-    fill_u32(c, masks[l], 0u, n);
-    Level& lv = o->svo[l];
+    const double scale = double(1ull << (l + 2));   // k = rootSide / 2^(l+2) = getHalfSideD(level + 1)
+    uint32_t* mask = dalloc<uint32_t>(c, n + 1);
+    fill_u32(c, mask, 0u, n);
+    Level& lv = out[l];
     lv.n = n;
     lv.leaf = (l == L - 1);
     if (l > 0) nNodes += n;
     if (lv.leaf) {   // voxel bits of the leaf-level nodes
-      if (np) k_vox_test<<<div_up(np, kB), kB, 0, c->stream>>>(dtris, pt, pn, np, geo, k, masks[l], nullptr, nullptr);
+      if (np) k_vox_test<<<div_up(np, kB), kB, 0, c->stream>>>(dtris, pt, pn, np, geo, side, scale, mask, nullptr,
+                                                                 nullptr);
       check_launch("k_vox_test");
-      lv.rec = masks[l];
-      masks[l] = nullptr;
+      lv.rec = mask;
       break;
     }
     uint32_t* cnt = dalloc<uint32_t>(c, np + 1);
     uint8_t* hits = dalloc<uint8_t>(c, np + 1);
-    uint32_t* total = dalloc<uint32_t>(c, 1);
-    if (np) k_vox_test<<<div_up(np, kB), kB, 0, c->stream>>>(dtris, pt, pn, np, geo, k, masks[l], cnt, hits);
+    if (np) k_vox_test<<<div_up(np, kB), kB, 0, c->stream>>>(dtris, pt, pn, np, geo, side, scale, mask, cnt, hits);
     check_launch("k_vox_test");
     scan_exclusive(c, cnt, cnt, np, total);
     const uint64_t nv = read_u32(c, total);
     if (nv >= 0xFFFFFFF0ull) throw Error{SVD_ERR_CAPACITY, "too many triangle/node visits in one level"};
     lv.rec = dalloc<uint32_t>(c, n * kRecWords + 1);
-    if (n) k_vox_init_rec<<<div_up(n, kB), kB, 0, c->stream>>>(lv.rec, masks[l], n);
+    if (n) k_vox_init_rec<<<div_up(n, kB), kB, 0, c->stream>>>(lv.rec, mask, n);
+    dfree(c, mask);
     uint64_t* vkey = dalloc<uint64_t>(c, nv + 1);
     uint32_t* vt = dalloc<uint32_t>(c, nv + 1);
     uint32_t* vpos = dalloc<uint32_t>(c, nv + 1);
@@ -331,15 +388,19 @@ void build_svo_from_mesh(svd_octree* o, const float* dtris, uint64_t ntris, cons
       k_vox_groups<<<div_up(nv, kB), kB, 0, c->stream>>>(vkey, vpos, vt, head, gid, nv, geo.path, gKey, gPath, gVal,
                                                           gChild, visitGroup);
     check_launch("k_vox_groups");
-    // creation order: by first triangle, then by DFS position inside it
+    // creation order: by root, then first triangle, then DFS position inside it
     stable_sort_pairs<uint64_t>(c, gPath, gVal, ng, 3 * (int)(l + 1));
     uint32_t* gKey2 = dalloc<uint32_t>(c, ng + 1);
     uint32_t* rank = dalloc<uint32_t>(c, ng + 1);
     if (ng) k_gather_u32<<<div_up(ng, kB), kB, 0, c->stream>>>(gKey, gVal, ng, gKey2);
     stable_sort_pairs<uint32_t>(c, gKey2, gVal, ng, bits_for(ntris));
+    if (R > 1) {
+      if (ng) k_gather_root<<<div_up(ng, kB), kB, 0, c->stream>>>(gChild, gVal, ng, geo.root, gKey2);
+      stable_sort_pairs<uint32_t>(c, gKey2, gVal, ng, bits_for(R));
+    }
     if (ng) k_inverse<<<div_up(ng, kB), kB, 0, c->stream>>>(gVal, ng, rank);
     NodeGeo next = alloc_geo(c, ng);
-    if (ng) k_vox_children<<<div_up(ng, kB), kB, 0, c->stream>>>(gChild, rank, ng, lv.rec, geo, next, k);
+    if (ng) k_vox_children<<<div_up(ng, kB), kB, 0, c->stream>>>(gChild, rank, ng, lv.rec, geo, next, side, scale);
     dfree(c, pt);
     dfree(c, pn);
     pt = dalloc<uint32_t>(c, nv + 1);
@@ -348,27 +409,109 @@ void build_svo_from_mesh(svd_octree* o, const float* dtris, uint64_t ntris, cons
     check_launch("k_vox_children");
     free_geo(c, geo);
     geo = next;
-    dfree(c, vkey); dfree(c, vt); dfree(c, vpos); dfree(c, head); dfree(c, gid); dfree(c, gKey); dfree(c, gKey2); dfree(c, gPath);
-    dfree(c, gVal); dfree(c, gChild); dfree(c, visitGroup); dfree(c, rank); dfree(c, total);
-    dfree(c, masks[l]);
-    masks[l] = nullptr;
+    dfree(c, vkey); dfree(c, vt); dfree(c, vpos); dfree(c, head); dfree(c, gid); dfree(c, gKey); dfree(c, gKey2);
+    dfree(c, gPath); dfree(c, gVal); dfree(c, gChild); dfree(c, visitGroup); dfree(c, rank);
     np = nv;
     n = ng;
   }
+  if (lastCenters) {
+    const uint64_t nl = out[L - 1].n;
+    lastCenters->assign(3 * nl, 0.0);
+    std::vector<double> tmp(nl);
+    const double* srcs[3] = {geo.cx, geo.cy, geo.cz};
+    for (int a = 0; a < 3; ++a) {
+      if (nl) SVD_HIP(hipMemcpyAsync(tmp.data(), srcs[a], 8 * nl, hipMemcpyDeviceToHost, c->stream));
+      SVD_HIP(hipStreamSynchronize(c->stream));
+      for (uint64_t i = 0; i < nl; ++i) (*lastCenters)[3 * i + a] = tmp[i];
+    }
+  }
   dfree(c, pt);
   dfree(c, pn);
+  dfree(c, side);
+  dfree(c, total);
   free_geo(c, geo);
-  // cleanEmptyNodes, bottom-up
+  // cleanEmptyNodes inside every SVO, bottom-up (parents never leave their SVO)
   for (uint32_t l = L - 1; l > 0; --l) {
-    const Level& child = o->svo[l];
-    const Level& par = o->svo[l - 1];
+    const Level& child = out[l];
+    const Level& par = out[l - 1];
     if (par.n && child.n)
       k_vox_clean<<<div_up(par.n, kB), kB, 0, c->stream>>>(par.rec, par.n, child.rec, child.leaf ? 1 : 0);
   }
   check_launch("k_vox_clean");
   SVD_HIP(hipStreamSynchronize(c->stream));
-  o->nNodesSVO = nNodes;
-  o->nVoxels = 0;   // set by the caller from the leaf masks
+  return nNodes;
+}
+
+}  // namespace
+
+void build_svo_from_mesh(svd_octree* o, const float* dtris, uint64_t ntris, const double bmin[3],
+                         const double bmax[3]) {
+  const VoxRoot root = root_of_box(bmin, bmax);
+  for (int i = 0; i < 6; ++i) o->bbox[i] = root.bb[i];
+  o->rootSide = root.side;
+  o->nNodesSVO = voxelize_roots(o->ctx, dtris, ntris, o->levels, {root}, o->svo, nullptr);
+}
+
+// GeomOctree::buildDAG's SVO (geom_octree.cpp:289-435): a top SVO of
+// stepLevel + 1 levels; one subtree per set voxel of its finest nodes,
+// voxelized from all triangles inside the voxel's own box (:331-378; the box
+// spans the parent centre and the voxel's far corner); the subtrees joined
+// under the top in order (finest node ascending, slot 7..0, :397-417).  The
+// node count is the reference's: the top's created nodes plus each subtree's,
+// without the subtree roots.
+void build_svo_from_mesh_subtrees(svd_octree* o, const float* dtris, uint64_t ntris, const double bmin[3],
+                                  const double bmax[3], uint32_t stepLevel) {
+  svd_ctx* c = o->ctx;
+  const uint32_t L = o->levels, stepLevels = stepLevel + 1;
+  if (stepLevels >= L) throw Error{SVD_ERR_ARG, "buildDAG: step level must be below levels - 1"};
+  const VoxRoot top = root_of_box(bmin, bmax);
+  for (int i = 0; i < 6; ++i) o->bbox[i] = top.bb[i];
+  o->rootSide = top.side;
+  std::vector<Level> T;
+  std::vector<double> centers;
+  const uint64_t topNodes = voxelize_roots(c, dtris, ntris, stepLevels, {top}, T, &centers);
+  Level& fin = T[stepLevels - 1];
+  std::vector<uint32_t> masks(fin.n);
+  if (fin.n) SVD_HIP(hipMemcpyAsync(masks.data(), fin.rec, 4 * fin.n, hipMemcpyDeviceToHost, c->stream));
+  SVD_HIP(hipStreamSynchronize(c->stream));
+  const double lhs = top.side / double(1u << stepLevels);   // getHalfSideD(stepLevels - 1)
+  std::vector<VoxRoot> roots;
+  for (uint64_t i = 0; i < fin.n; ++i)
+    for (int j = 7; j >= 0; --j) {
+      if (!((masks[i] >> j) & 1u)) continue;
+      const double* cc = &centers[3 * i];
+      double lo[3], hi[3];
+      for (int a = 0; a < 3; ++a) {
+        const double corner = cc[a] + ((j & (4 >> a)) ? lhs : -lhs);
+        lo[a] = std::min(cc[a], corner);
+        hi[a] = std::max(cc[a], corner);
+      }
+      roots.push_back(root_of_box(lo, hi));
+    }
+  std::vector<Level> sub;
+  uint64_t subNodes = 0;
+  if (!roots.empty()) subNodes = voxelize_roots(c, dtris, ntris, L - stepLevels, roots, sub, nullptr);
+  // join: the top's finest level points at the subtree roots
+  uint32_t* joined = dalloc<uint32_t>(c, fin.n * kRecWords + 1);
+  uint32_t* base = dalloc<uint32_t>(c, fin.n + 1);
+  if (fin.n) {
+    k_popc_mask<<<div_up(fin.n, kB), kB, 0, c->stream>>>(fin.rec, fin.n, base);
+    scan_exclusive(c, base, base, fin.n, nullptr);
+    k_vox_join_top<<<div_up(fin.n, kB), kB, 0, c->stream>>>(fin.rec, base, fin.n, joined);
+  }
+  check_launch("k_vox_join_top");
+  SVD_HIP(hipStreamSynchronize(c->stream));
+  dfree(c, base);
+  dfree(c, fin.rec);
+  fin.rec = joined;
+  fin.leaf = false;
+  o->svo.assign(L, Level());
+  for (uint32_t l = 0; l < stepLevels; ++l) o->svo[l] = T[l];
+  for (uint32_t l = stepLevels; l < L; ++l) {
+    if (roots.empty()) { o->svo[l].n = 0; o->svo[l].leaf = (l == L - 1); o->svo[l].rec = dalloc<uint32_t>(c, 1); }
+    else o->svo[l] = sub[l - stepLevels];
+  }
+  o->nNodesSVO = topNodes + subNodes;
 }
 
 }  // namespace svd

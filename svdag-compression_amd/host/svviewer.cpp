@@ -1,6 +1,6 @@
 // svviewer -- headless form of the reference viewer (src/svviewer/main.cpp):
 //   svviewer file.{svdag,ussvdag,ssvdag,esvdag} [--width W] [--height H]
-//            [--frames N] [--seed S] [--no-shadows] [--image out.ppm]
+//            [--frames N] [--seed S] [--no-shadows] [--no-beam] [--image out.ppm]
 // Loads the file (EncodedOctree::load), traces frames with the octree-DDA
 // kernel from a seeded orbit camera (renderer defaults: FOV 60, maxIters 300,
 // pixel tolerance 1, drawLevel = levels, light at the box max) and prints
@@ -84,13 +84,13 @@ void orbit_camera(uint64_t seed, const svd::aabox3f& box, uint32_t w, uint32_t h
 int main(int argc, char** argv) {
   if (argc < 2) {
     printf("Usage: svviewer file.{svdag,ussvdag,ssvdag,esvdag} [--width W] [--height H] [--frames N] [--seed S] "
-           "[--no-shadows] [--image out.ppm]\n");
+           "[--no-shadows] [--no-beam] [--image out.ppm]\n");
     return 1;
   }
   std::string file = argv[1];
   uint32_t W = 1280, H = 720, frames = 10;
   uint64_t seed = 1;
-  bool shadows = true;
+  bool shadows = true, beam = true;   // the viewer's minimum-depth optimisation is on by default
   std::string image;
   for (int i = 2; i < argc; ++i) {
     std::string a = argv[i];
@@ -99,6 +99,7 @@ int main(int argc, char** argv) {
     else if (a == "--frames" && i + 1 < argc) frames = atoi(argv[++i]);
     else if (a == "--seed" && i + 1 < argc) seed = strtoull(argv[++i], nullptr, 10);
     else if (a == "--no-shadows") shadows = false;
+    else if (a == "--no-beam") beam = false;
     else if (a == "--image" && i + 1 < argc) image = argv[++i];
   }
   EncodedOctree* eo = nullptr;
@@ -122,6 +123,8 @@ int main(int argc, char** argv) {
   // getProjectionFactor (octree_dda_renderer.cpp:503-507), pixel tolerance 1
   const float fovy = 60.0f * 3.14159265359f / 180.0f;
   p.projection_factor = (1.0f / (2.0f * std::tan(0.5f * fovy))) / (1.0f / (float)H);
+  // getProjectionFactor(1, 8) of the depth pre-pass
+  const float beamProjection = beam ? (1.0f / (2.0f * std::tan(0.5f * fovy))) / (1.0f / ((float)H / 8.0f)) : 0.f;
   const svd::aabox3f bb = eo->getSceneBBox();
   p.light_pos[0] = bb[1][0]; p.light_pos[1] = bb[1][1]; p.light_pos[2] = bb[1][2];
   p.shadows = shadows ? 1 : 0;
@@ -129,7 +132,7 @@ int main(int argc, char** argv) {
   double best = 1e30;
   for (uint32_t f = 0; f < frames; ++f) {
     HeadlessTraceResult r;
-    if (!traceHeadless(*eo, p, r, f + 1 == frames ? &prim : nullptr)) return 1;
+    if (!traceHeadless(*eo, p, r, f + 1 == frames ? &prim : nullptr, beamProjection)) return 1;
     best = std::min(best, r.kernelMs);
     printf("frame %u: %.3f ms, %lu primary + %lu shadow rays, %.1f Mrays/s (hits %lu)\n", f, r.kernelMs,
            (unsigned long)r.primaryRays, (unsigned long)r.shadowRays, r.mraysPerSec, (unsigned long)r.hits);

@@ -164,156 +164,6 @@ void Scene::buildTriVector(bool clearOtherData) {
   if (clearOtherData) { _vertices.clear(); _vertices.shrink_to_fit(); _indexed.clear(); _indexed.shrink_to_fit(); }
 }
 
-// ============================================================================ voxelizer
-namespace {
-
-// Akenine-Moller triangle / box overlap in double (test_triangle_box.cpp:38-184)
-bool planeBoxOverlap(const double n[3], const double v[3], double h) {
-  double vmin[3], vmax[3];
-  for (int q = 0; q < 3; ++q) {
-    if (n[q] > 0.0f) { vmin[q] = -h - v[q]; vmax[q] = h - v[q]; }
-    else { vmin[q] = h - v[q]; vmax[q] = -h - v[q]; }
-  }
-  if (n[0] * vmin[0] + n[1] * vmin[1] + n[2] * vmin[2] > 0.0) return false;
-  if (n[0] * vmax[0] + n[1] * vmax[1] + n[2] * vmax[2] >= 0.0) return true;
-  return false;
-}
-
-bool triBox(const double c[3], double h, const point3f* tri) {
-  double v0[3], v1[3], v2[3], e0[3], e1[3], e2[3];
-  for (int i = 0; i < 3; ++i) {
-    v0[i] = (double)tri[0][i] - c[i];
-    v1[i] = (double)tri[1][i] - c[i];
-    v2[i] = (double)tri[2][i] - c[i];
-  }
-  for (int i = 0; i < 3; ++i) { e0[i] = v1[i] - v0[i]; e1[i] = v2[i] - v1[i]; e2[i] = v0[i] - v2[i]; }
-  auto axis = [&](double a, double b, double fa, double fb, const double* p, const double* q, int i0, int i1,
-                  double s0, double s1) {
-    // p_k = s0*a*p[i0] + s1*b*p[i1] restated per SAT axis family
-    const double pa = s0 * a * p[i0] + s1 * b * p[i1];
-    const double pb = s0 * a * q[i0] + s1 * b * q[i1];
-    double mn, mx;
-    if (pa < pb) { mn = pa; mx = pb; } else { mn = pb; mx = pa; }
-    const double rad = (fa + fb) * h;
-    return !(mn > rad || mx < -rad);
-  };
-  // X-tests: p = a*v[Y] - b*v[Z]
-  // Y-tests: p = -a*v[X] + b*v[Z]
-  // Z-tests: p = a*v[X] - b*v[Y]
-  double fex = fabs(e0[0]), fey = fabs(e0[1]), fez = fabs(e0[2]);
-  if (!axis(e0[2], e0[1], fez, fey, v0, v2, 1, 2, 1, -1)) return false;   // X01
-  if (!axis(e0[2], e0[0], fez, fex, v0, v2, 0, 2, -1, 1)) return false;   // Y02
-  {                                                                        // Z12 (v1, v2; p2<p1 ordering)
-    const double p1 = e0[1] * v1[0] - e0[0] * v1[1], p2 = e0[1] * v2[0] - e0[0] * v2[1];
-    double mn, mx;
-    if (p2 < p1) { mn = p2; mx = p1; } else { mn = p1; mx = p2; }
-    const double rad = (fey + fex) * h;
-    if (mn > rad || mx < -rad) return false;
-  }
-  fex = fabs(e1[0]); fey = fabs(e1[1]); fez = fabs(e1[2]);
-  if (!axis(e1[2], e1[1], fez, fey, v0, v2, 1, 2, 1, -1)) return false;   // X01
-  if (!axis(e1[2], e1[0], fez, fex, v0, v2, 0, 2, -1, 1)) return false;   // Y02
-  if (!axis(e1[1], e1[0], fey, fex, v0, v1, 0, 1, 1, -1)) return false;   // Z0
-  fex = fabs(e2[0]); fey = fabs(e2[1]); fez = fabs(e2[2]);
-  if (!axis(e2[2], e2[1], fez, fey, v0, v1, 1, 2, 1, -1)) return false;   // X2
-  if (!axis(e2[2], e2[0], fez, fex, v0, v1, 0, 2, -1, 1)) return false;   // Y1
-  {                                                                        // Z12
-    const double p1 = e2[1] * v1[0] - e2[0] * v1[1], p2 = e2[1] * v2[0] - e2[0] * v2[1];
-    double mn, mx;
-    if (p2 < p1) { mn = p2; mx = p1; } else { mn = p1; mx = p2; }
-    const double rad = (fey + fex) * h;
-    if (mn > rad || mx < -rad) return false;
-  }
-  for (int a = 0; a < 3; ++a) {
-    double mn = v0[a], mx = v0[a];
-    if (v1[a] < mn) mn = v1[a];
-    if (v1[a] > mx) mx = v1[a];
-    if (v2[a] < mn) mn = v2[a];
-    if (v2[a] > mx) mx = v2[a];
-    if (mn > h || mx < -h) return false;
-  }
-  const double nrm[3] = {e0[1] * e1[2] - e0[2] * e1[1], e0[2] * e1[0] - e0[0] * e1[2], e0[0] * e1[1] - e0[1] * e1[0]};
-  return planeBoxOverlap(nrm, v0, h);
-}
-
-struct VoxOut {
-  GeomOctree::NodeData data;
-  aabox3f bbox;
-  double rootSide = 0;
-  size_t nNodes = 0, nVoxels = 0;
-};
-
-// GeomOctree::buildSVO (geom_octree.cpp:171-280) + cleanEmptyNodes (:437-456)
-void voxelize_scene(const Scene& scene, unsigned levels, const aabox3d& bb, VoxOut& out,
-                    std::vector<point3d>* leavesCenters) {
-  out.bbox = aabox3f(point3f((float)bb[0][0], (float)bb[0][1], (float)bb[0][2]),
-                     point3f((float)bb[1][0], (float)bb[1][1], (float)bb[1][2]));
-  const point3f sides = out.bbox.half_side_lengths();
-  out.rootSide = std::max(std::max(sides[0] * 2.0f, sides[1] * 2.0f), sides[2] * 2.0f);
-  auto& D = out.data;
-  D.assign(levels, {});
-  D[0].push_back(GeomOctree::Node());
-  out.nNodes = 0;
-  struct QI { uint32_t id; uint8_t level; double c[3]; };
-  std::stack<QI> st;
-  const point3d ctr = bb.center();
-  for (size_t t = 0; t < scene.getNRawTriangles(); ++t) {
-    const point3f* tri = scene.getTrianglePtr(t);
-    st.push(QI{0, 0, {ctr[0], ctr[1], ctr[2]}});
-    while (!st.empty()) {
-      QI qi = st.top();
-      st.pop();
-      const double k = out.rootSide / double(1u << (qi.level + 2));
-      double cc[8][3];
-      for (int c = 0; c < 8; ++c) {
-        cc[c][0] = qi.c[0] + ((c & 4) ? +k : -k);
-        cc[c][1] = qi.c[1] + ((c & 2) ? +k : -k);
-        cc[c][2] = qi.c[2] + ((c & 1) ? +k : -k);
-      }
-      for (int i = 7; i >= 0; --i) {
-        if (!triBox(cc[i], k, tri)) continue;
-        GeomOctree::Node& node = D[qi.level][qi.id];
-        node.childrenBitmask |= (uint8_t)(1u << i);
-        if (!node.existsChildPointer(i) && qi.level < levels - 1) {
-          node.children[i] = (uint32_t)D[qi.level + 1].size();
-          D[qi.level + 1].emplace_back();
-          out.nNodes++;
-          if (leavesCenters && qi.level == levels - 2) leavesCenters->push_back(point3d(cc[i][0], cc[i][1], cc[i][2]));
-        }
-        if (qi.level + 1u < levels) {
-          const uint32_t ch = D[qi.level][qi.id].children[i];
-          st.push(QI{ch, (uint8_t)(qi.level + 1), {cc[i][0], cc[i][1], cc[i][2]}});
-        }
-      }
-    }
-  }
-  for (int lev = (int)levels - 1; lev > 0; --lev) {
-    std::set<uint32_t> empty;
-    for (uint32_t i = 0; i < D[lev].size(); ++i) if (!D[lev][i].hasChildren()) empty.insert(i);
-    if (empty.empty()) continue;
-    for (auto& n : D[lev - 1])
-      for (int j = 0; j < 8; ++j)
-        if (empty.count(n.children[j])) { n.childrenBitmask &= (uint8_t)~(1u << j); n.children[j] = GeomOctree::nullNode; }
-  }
-  out.nVoxels = 0;
-  for (auto& n : D[levels - 1]) out.nVoxels += n.getNChildren();
-}
-
-std::vector<std::vector<uint32_t>> to_words(const GeomOctree::NodeData& D) {
-  std::vector<std::vector<uint32_t>> w(D.size());
-  for (size_t l = 0; l < D.size(); ++l) {
-    w[l].resize(9 * D[l].size());
-    for (size_t i = 0; i < D[l].size(); ++i) {
-      const auto& n = D[l][i];
-      w[l][9 * i] = n.childrenBitmask;
-      for (int c = 0; c < 8; ++c) w[l][9 * i + 1 + c] = n.children[c];
-    }
-  }
-  return w;
-}
-
-}  // namespace
-
 // ============================================================================ GeomOctree
 GeomOctree::GeomOctree(Scene* scene) : _scene(scene) { _bbox.to_empty(); }
 
@@ -327,22 +177,10 @@ void GeomOctree::resizeSceneBbox(aabox3f bbox) {
   _rootSide = std::max(std::max(s[0] * 2.0f, s[1] * 2.0f), s[2] * 2.0f);
 }
 
-void GeomOctree::uploadSVO(NodeData& data) {
-  auto w = to_words(data);
-  std::vector<uint64_t> sizes(data.size());
-  std::vector<const uint32_t*> ptrs(data.size());
-  for (size_t l = 0; l < data.size(); ++l) { sizes[l] = data[l].size(); ptrs[l] = w[l].data(); }
-  const float bb[6] = {_bbox[0][0], _bbox[0][1], _bbox[0][2], _bbox[1][0], _bbox[1][1], _bbox[1][2]};
-  if (_oct) { svd_octree_free(_oct); _oct = nullptr; }
-  check(svd_octree_from_svo(default_context(), (uint32_t)data.size(), sizes.data(), ptrs.data(), bb, _rootSide, &_oct),
-        "svd_octree_from_svo");
-  _cacheState = -1;
-}
-
 static_assert(sizeof(point3f) == 12, "triangles are handed over as packed float triples");
 
-// geom_octree.cpp:171-280 on the device (svd_octree_from_mesh): the same
-// creation order and FP64 box tests as the host restatement above.
+// geom_octree.cpp:171-280 on the device (svd_octree_from_mesh): the
+// reference's creation order and FP64 triangle/box tests.
 void GeomOctree::buildSVO(unsigned int levels, aabox3d bbox, bool internalCall) {
   if (!internalCall) printf("* Building SVO... ");
   fflush(stdout);
@@ -369,77 +207,36 @@ void GeomOctree::buildSVO(unsigned int levels, aabox3d bbox, bool internalCall) 
   if (!internalCall) printf("OK! [%.3f s]\n", _stats.buildSVOTime);
 }
 
+// geom_octree.cpp:289-435 on the device (svd_octree_from_mesh_subtrees): the
+// top SVO, one subtree per set voxel of its finest level voxelized in its own
+// box, joined in the reference's order; one GPU reduction of the joined
+// subtrees gives the same DAG as the reference's per-subtree toDAG followed by
+// the final toDAG, because both keep the first occurrence of every key in join
+// order.
 void GeomOctree::buildDAG(unsigned int levels, unsigned int stepLevel, aabox3d bbox, bool verbose) {
   printf("* Building DAG [stepLevel: %i]\n", stepLevel);
   fflush(stdout);
   const double t0 = now_s();
-  const unsigned stepLevels = stepLevel + 1;
-  if (stepLevels >= levels) { buildSVO(levels, bbox, true); toDAG(true); return; }
-  // top SVO of stepLevels levels (geom_octree.cpp:302)
-  std::vector<point3d> leavesCenters;
-  VoxOut top;
-  voxelize_scene(*_scene, stepLevels, bbox, top, &leavesCenters);
-  _bbox = top.bbox;
-  _rootSide = top.rootSide;
-  const double lhs = top.rootSide / double(1u << stepLevels);   // getHalfSideD(stepLevels-1)
-  auto& T = top.data;
-  const size_t nTop = T[stepLevels - 1].size();
-  // subtrees, one per set bit of the top SVO's finest nodes (geom_octree.cpp:331-378)
-  std::vector<VoxOut> sub(nTop * 8);
-  std::vector<std::pair<size_t, int>> jobs;
-  for (size_t i = 0; i < nTop; ++i)
-    for (int j = 7; j >= 0; --j)
-      if (T[stepLevels - 1][i].existsChild(j)) jobs.push_back({i, j});
-  printf("\t- Building %zu subtrees (host voxelization) ", jobs.size());
-#ifdef _OPENMP
-  printf("[%d threads]... ", omp_get_max_threads());
-#endif
-  fflush(stdout);
-#pragma omp parallel for schedule(dynamic, 2)
-  for (long k = 0; k < (long)jobs.size(); ++k) {
-    const size_t i = jobs[k].first;
-    const int j = jobs[k].second;
-    const point3d c = leavesCenters[i];
-    const point3d corner(c[0] + ((j & 4) ? lhs : -lhs), c[1] + ((j & 2) ? lhs : -lhs), c[2] + ((j & 1) ? lhs : -lhs));
-    aabox3d lb;
-    lb.merge(c);
-    lb.merge(corner);
-    voxelize_scene(*_scene, levels - stepLevels, lb, sub[i * 8 + j], nullptr);
-  }
-  printf("OK! [%.3f s]\n", now_s() - t0);
-  // join in the reference's order (i ascending, j = 7..0; geom_octree.cpp:397-417)
-  NodeData J(levels);
-  for (unsigned l = 0; l < stepLevels; ++l) J[l] = T[l];
-  size_t nNodes = top.nNodes, nVox = 0, lastLev = 0;
-  for (size_t i = 0; i < nTop; ++i)
-    for (int j = 7; j >= 0; --j) {
-      if (!T[stepLevels - 1][i].existsChild(j)) continue;
-      VoxOut& s = sub[i * 8 + j];
-      J[stepLevels - 1][i].children[j] = (uint32_t)J[stepLevels].size();
-      for (unsigned k = stepLevels; k < levels; ++k) {
-        auto lvl = s.data[k - stepLevels];
-        if (k < levels - 1) {
-          const uint32_t off = (uint32_t)J[k + 1].size();
-          for (auto& n : lvl)
-            for (int c = 0; c < 8; ++c)
-              if (n.existsChildPointer(c)) n.children[c] += off;
-        }
-        J[k].insert(J[k].end(), lvl.begin(), lvl.end());
-      }
-      nNodes += s.nNodes;
-      nVox += s.nVoxels;
-      lastLev += s.data[levels - 1 - stepLevels].size();
-    }
+  // no subtrees (a top of one level, or of every level): build the SVO and reduce it
+  if (stepLevel == 0 || stepLevel + 1 >= levels) { buildSVO(levels, bbox, true); toDAG(true); return; }
+  if (_oct) { svd_octree_free(_oct); _oct = nullptr; }
+  const double bb[6] = {bbox[0][0], bbox[0][1], bbox[0][2], bbox[1][0], bbox[1][1], bbox[1][2]};
+  const size_t nt = _scene->getNRawTriangles();
+  const float* tris = nt ? &_scene->getTrianglePtr(0)[0][0] : nullptr;
+  check(svd_octree_from_mesh_subtrees(default_context(), tris, nt, levels, stepLevel, bb, 0, &_oct), "buildDAG");
+  _bbox = aabox3f(point3f((float)bb[0], (float)bb[1], (float)bb[2]), point3f((float)bb[3], (float)bb[4], (float)bb[5]));
+  const point3f sides = _bbox.half_side_lengths();
+  _rootSide = std::max(std::max(sides[0] * 2.0f, sides[1] * 2.0f), sides[2] * 2.0f);
   _levels = levels;
-  _stats.nNodesSVO = nNodes;
-  _stats.nNodesLastLevSVO = lastLev;
-  _stats.simulatedEncodedSVOSize = (nNodes - lastLev) * 4;
-  _stats.nTotalVoxels = nVox;
-  uploadSVO(J);
+  uint64_t nn = 0, nv = 0, nl = 0;
+  svd_octree_counts(_oct, &nn, &nv);
+  svd_level_size(_oct, SVD_VIEW_SVO, levels - 1, &nl);
+  _stats.nNodesSVO = nn;
+  _stats.nNodesLastLevSVO = nl;
+  _stats.simulatedEncodedSVOSize = (nn - nl) * 4;
+  _stats.nTotalVoxels = nv;
   _state = S_SVO;
-  // one GPU reduction over the joined subtrees gives the same DAG as the
-  // reference's per-subtree toDAG followed by the final toDAG: both keep the
-  // first occurrence of every key in join order.
+  _cacheState = -1;
   toDAG(true);
   _stats.buildDAGTime = now_s() - t0;
   if (verbose) printf("\t- Finished! Total time [%.3f s]\n", _stats.buildDAGTime);
@@ -726,7 +523,7 @@ void EncodedSSVDAG::encode(const GeomOctree& octree) {
 
 // ============================================================================ traversal
 bool traceHeadless(const EncodedOctree& eo, const svd_trace_params& p, HeadlessTraceResult& out,
-                   std::vector<svd_ray_result>* primary) {
+                   std::vector<svd_ray_result>* primary, float beamProjection) {
   svd_ctx* c = default_context();
   const std::vector<uint8_t>& f = eo.fileImage();
   int fmt = eo.format();
@@ -744,7 +541,7 @@ bool traceHeadless(const EncodedOctree& eo, const svd_trace_params& p, HeadlessT
     svd_scene_free(sc);
     return false;
   }
-  int rc = svd_trace(sc, &p, dp, ds);
+  int rc = beamProjection > 0.f ? svd_trace_beam(sc, &p, beamProjection, dp, ds, nullptr) : svd_trace(sc, &p, dp, ds);
   double ms = 0;
   svd_last_kernel_ms(c, "trace", &ms);
   std::vector<svd_ray_result> hp(n), hs(n);

@@ -126,11 +126,10 @@ class GeomOctree {
   GeomOctree& operator=(const GeomOctree&) = delete;
   ~GeomOctree();
 
-  // geom_octree.cpp:171-280 (host voxelization, reference order) + upload
+  // geom_octree.cpp:171-280: voxelization on the device in the reference's node order
   void buildSVO(unsigned int levels, svd::aabox3d bbox, bool internalCall = false);
-  // geom_octree.cpp:289-435: the top SVO and every subtree are voxelized on
-  // the host (OpenMP over subtrees, each with its own box), the subtrees are
-  // joined in the reference's order and reduced on the GPU.
+  // geom_octree.cpp:289-435: the top SVO and every subtree (each in its own
+  // box) voxelized on the device, joined in the reference's order and reduced.
   void buildDAG(unsigned int levels, unsigned int stepLevel, svd::aabox3d bbox, bool verbose = false);
   // Voxel list entry (Morton codes, svd_octree_from_morton).
   bool buildFromMorton(const uint64_t* codes, size_t n, unsigned int levels, bool codesOnDevice = false);
@@ -161,8 +160,6 @@ class GeomOctree {
   svd_octree* handle() const { return _oct; }
 
  private:
-  void uploadSVO(NodeData& data);
-  void voxelize(NodeData& data, unsigned int levels, const svd::aabox3d& bbox, std::vector<svd::point3d>* leaves) const;
 
   Scene* _scene;
   svd_octree* _oct = nullptr;
@@ -257,5 +254,7 @@ struct HeadlessTraceResult {
   double kernelMs = 0, mraysPerSec = 0;
   uint64_t primaryRays = 0, shadowRays = 0, hits = 0;
 };
+// beamProjection > 0: the viewer's minimum-depth pre-pass with that projection
+// factor (svd_trace_beam); 0: every primary ray starts at the camera.
 bool traceHeadless(const EncodedOctree& eo, const svd_trace_params& params, HeadlessTraceResult& out,
-                   std::vector<svd_ray_result>* primary = nullptr);
+                   std::vector<svd_ray_result>* primary = nullptr, float beamProjection = 0.f);

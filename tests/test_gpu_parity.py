@@ -217,6 +217,40 @@ def test_trace_matches_shader_port(ctx, fmt):
         np.testing.assert_array_equal(got_s[k], want_s[k], err_msg=f"shadow {k}")
 
 
+@pytest.mark.parametrize("fmt", [S.FMT_SVDAG, S.FMT_USSVDAG, S.FMT_SSVDAG])
+@pytest.mark.parametrize("size", [(256, 192), (203, 121)])
+def test_trace_beam_matches_shader_port(ctx, fmt, size):
+    """The viewer's minimum-depth optimisation: the 1/8-resolution depth
+    pre-pass texture and every primary/shadow ray equal the CPU port's, also
+    when the frame is not a multiple of 8 (edge texels read as 0)."""
+    import torch
+    codes, L = W.terrain_codes(8, seed=1), 8
+    _, _, _, files = oracle_pipeline(codes, L)
+    buf = files[fmt]
+    w, h = size
+    p = W.trace_params(w, h, seed=4, levels=L, shadows=True, light=(1.0, 0.6, 1.0))
+    dp = W.projection_factor(h, screen_divisor=8.0)
+    want_p, want_s, want_tex = O.trace_frame_beam(buf, fmt, p, dp, shadows=True)
+    sc = S.Scene.from_bytes(ctx, fmt, buf)
+    n = w * h
+    prim = torch.zeros(n * 6, dtype=torch.int32, device="cuda")
+    shad = torch.zeros(n * 6, dtype=torch.int32, device="cuda")
+    tex = torch.zeros((h // 8) * (w // 8), dtype=torch.float32, device="cuda")
+    torch.cuda.synchronize()
+    sc.trace_beam_device(p, dp, prim.data_ptr(), shad.data_ptr(), tex.data_ptr())
+    ctx.sync()
+    np.testing.assert_array_equal(tex.cpu().numpy().reshape(h // 8, w // 8), want_tex)
+    assert (want_tex > 0).mean() > 0.1
+    got_p = prim.cpu().numpy().view(S.RAY_DTYPE)
+    got_s = shad.cpu().numpy().view(S.RAY_DTYPE)
+    for k in ("t", "level", "iters", "node", "path", "bytes"):
+        np.testing.assert_array_equal(got_p[k], want_p[k], err_msg=f"primary {k}")
+        np.testing.assert_array_equal(got_s[k], want_s[k], err_msg=f"shadow {k}")
+    # the pre-pass only moves the start of primary rays: fewer iterations
+    plain_p, _ = O.trace_frame(buf, fmt, p, shadows=False)
+    assert want_p["iters"].mean() < plain_p["iters"].mean()
+
+
 def test_trace_no_lod_all_formats(ctx):
     codes, L = clustered_codes(6, 8), 6
     _, _, _, files = oracle_pipeline(codes, L)

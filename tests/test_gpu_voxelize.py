@@ -72,3 +72,32 @@ def test_mesh_svo_empty_and_bad_args(ctx):
     assert g.counts()[1] == 0
     with pytest.raises(S.SvdError):
         S.Octree.from_mesh(ctx, np.zeros((1, 3, 3), np.float32), 1, np.zeros(3), np.ones(3))
+
+
+@pytest.mark.parametrize("L,step", [(7, 1), (7, 2), (8, 3), (6, 4)])
+def test_mesh_subtrees_equal_oracle_builddag(ctx, L, step):
+    """buildDAG (geom_octree.cpp:289-435): top SVO + per-voxel subtrees, each
+    voxelized in its own box, joined and reduced: the same DAG and counts as
+    the oracle's literal restatement (step L - 2: subtrees of one level)."""
+    V, F = W.spheres_mesh(n_spheres=16, subdiv=2, seed=0)
+    bmin, bmax = W.mesh_bbox(V)
+    bmin, bmax = bmin.astype(np.float64), bmax.astype(np.float64)
+    tris = np.ascontiguousarray(V[F], np.float32)
+    print(f"[case] builddag L={L} step={step}", flush=True)
+    o = O.OracleOctree.from_mesh_subtrees(tris, L, step, bmin, bmax)
+    g = S.Octree.from_mesh_subtrees(ctx, tris, L, step, bmin, bmax)
+    assert g.counts()[1] == o.n_voxels
+    g.to_dag()
+    for l in range(L):
+        np.testing.assert_array_equal(g.level(S.VIEW_DAG, l), o.level(l), err_msg=f"level {l}")
+    assert g.counts()[0] == o.n_nodes
+    o.init_child_levels()
+    g.encode(S.FMT_SVDAG)
+    assert g.encoded_bytes(S.FMT_SVDAG) == o.encode(0)
+
+
+def test_mesh_subtrees_reject_step_without_subtrees(ctx):
+    tris = np.array([[[0.1, 0.2, 0.3], [0.8, 0.2, 0.3], [0.1, 0.9, 0.6]]], np.float32)
+    for step in (0, 5, 6):
+        with pytest.raises(S.SvdError):
+            S.Octree.from_mesh_subtrees(ctx, tris, 6, step, np.zeros(3), np.ones(3))
