@@ -190,6 +190,47 @@ int svd_trace(svd_scene* sc, const svd_trace_params* p, svd_ray_result* primary,
 int svd_trace_beam(svd_scene* sc, const svd_trace_params* p, float depth_projection_factor, svd_ray_result* primary,
                    svd_ray_result* shadow, float* min_depth);
 
+/* ---- viewer shading ---------------------------------------------------------
+ * The uniforms of the viewer program (octree_dda.frag.glsl:52-60, 81-83;
+ * defaults octree_dda_renderer.cpp:41-45, 73).                               */
+typedef struct svd_view_params {
+  int32_t render_mode;         /* viewerRenderMode: 0 iterations (default), 1 depth, 2 voxel levels, 3 pretty */
+  int32_t norm_samples;        /* normSamples: extra rays that smooth the normal in mode 3 (0..64; default 0) */
+  uint32_t selected_voxel;     /* selectedVoxelIndex, drawn blue; 0xFFFFFFFF = none (the default -1) */
+  uint32_t random_colors;      /* randomColors (default 1) */
+  uint32_t freq_colors;        /* freqColors (default 1); SSVDAG / ESVDAG scenes only */
+  uint32_t pad;
+} svd_view_params;
+/* svd_render: the colour VIEWER_MODE main() writes per pixel
+ * (octree_dda.frag.glsl:671-858): the primary ray, the render mode's
+ * intensity, the pretty mode's normal samples and diffuse / fall-off terms,
+ * highlight, frequency or random colours, and the shadow ray when
+ * p->shadows.  beam_projection_factor > 0 runs the minimum-depth pre-pass
+ * first, as svd_trace_beam does (<= 0: off).  rgb: DEVICE array of
+ * 3 * width * height floats (unclamped; a discarded fragment gets 0);
+ * primary (optional, DEVICE) receives the primary rays.
+ *  - GLSL leaves sin, cos, log2 and pow to the implementation; this build
+ *    defines them (Cody-Waite reduction and Taylor polynomials, the atanh
+ *    series, pow(x, 2) = x * x) so results are reproducible bit for bit.
+ *  - freqColors reads the SSVDAG texture sizes as TBOs (textureSize(leafNodes)
+ *    = leaf count, textureSize(innerNodes) = u16 count / 4).  The reference's
+ *    USSVDAG viewer program does not compile with that branch (it names
+ *    SSVDAG-only uniforms); here USSVDAG and SVDAG scenes fall through to
+ *    randomColors.                                                          */
+int svd_render(svd_scene* sc, const svd_trace_params* p, const svd_view_params* v, float beam_projection_factor,
+               float* rgb, svd_ray_result* primary);
+/* svd_trace_ao: AO_MODE main() (octree_dda.frag.glsl:884-919) with num_rays
+ * (<= 256; the renderer's default is 16) hemisphere rays of length 0.5 per
+ * pixel, over the 256 cosine-weighted Halton samples of
+ * OctreeDDARenderer::generateHSSamples (octree_dda_renderer.cpp:510-531).
+ * The reference's AO program reads a G-buffer that nothing in the reference
+ * writes (and svviewer never selects the mode); here it is defined from each
+ * pixel's primary ray: hit position, -stepDir as the normal, cell size
+ * 2 rootHalfSide / 2^level.  visibility: DEVICE array of width * height
+ * floats = (1 - hits / num_rays)^2, -1 where the primary ray has no hit.    */
+int svd_trace_ao(svd_scene* sc, const svd_trace_params* p, uint32_t num_rays, float* visibility,
+                 svd_ray_result* primary);
+
 /* ---- autotuning (no reference counterpart) ----------------------------------
  * Launch shape of the traversal; results never depend on it.
  *  - SVD_TRACE_TILED (default): one workgroup per tile of tile_w x (256 / tile_w) pixels.

@@ -40,6 +40,12 @@ class LossyParams(C.Structure):
 LOSSY_STATS_DTYPE = np.dtype([(f, "<u8") for f in ("level", "bound", "nodes_before", "nodes_after", "candidates",
                                                    "listed", "verified", "merged", "error_bound")])
 
+class ViewParams(C.Structure):
+    """Layout of svd_view_params (render mode, normal samples, selection, colour switches)."""
+    _fields_ = [("render_mode", C.c_int32), ("norm_samples", C.c_int32), ("selected_voxel", C.c_uint32),
+                ("random_colors", C.c_uint32), ("freq_colors", C.c_uint32), ("pad", C.c_uint32)]
+
+
 RAY_DTYPE = np.dtype([("t", "<f4"), ("level", "<f4"), ("iters", "<f4"), ("node", "<f4"),
                       ("path", "<u4"), ("bytes", "<u4")])
 
@@ -68,6 +74,11 @@ def lib():
             "orc_std_sort_refcount": (None, [vp, vp, u64]),
             "orc_trace_frame": (i32, [vp, u64, i32, C.POINTER(TraceParams), vp, vp, u32, u32]),
             "orc_trace_frame_beam": (i32, [vp, u64, i32, C.POINTER(TraceParams), C.c_float, vp, vp, u32, u32, vp]),
+            "orc_render_frame": (i32, [vp, u64, i32, C.POINTER(TraceParams), C.POINTER(ViewParams), C.c_float,
+                                       vp, vp, u32, u32]),
+            "orc_trace_ao": (i32, [vp, u64, i32, C.POINTER(TraceParams), u32, vp, vp, u32, u32]),
+            "orc_hemisphere_samples": (None, [vp]),
+            "orc_vm_math": (None, [vp, u64, vp, vp, vp]),
             "orc_attr_bittree_encode": (C.c_int64, [vp, u64, i32, vp, u64]),
             "orc_attr_bittree_decode": (i32, [vp, u64, vp, u64]),
             "orc_lossy_merge": (i32, [vp, C.POINTER(LossyParams), vp, u32, vp, u32]),
@@ -262,6 +273,59 @@ def trace_frame_beam(file_bytes: bytes, fmt: int, params, depth_projection_facto
                                     _p(shad) if shad is not None else None, 0, p.height, _p(tex))
     assert rc == 0, rc
     return prim, shad, tex
+
+
+def _copy_params(params) -> TraceParams:
+    p = TraceParams()
+    C.memmove(C.addressof(p), C.addressof(params), C.sizeof(TraceParams))
+    return p
+
+
+def render_frame(file_bytes: bytes, fmt: int, params, view, beam_projection_factor: float = 0.0,
+                 row0: int = 0, nrows: int | None = None):
+    """VIEWER_MODE colour of rows [row0, row0 + nrows) (octree_dda.frag.glsl:671-858):
+    returns (rgb [rows, width, 3] float32, primary rays)."""
+    p = _copy_params(params)
+    v = ViewParams()
+    C.memmove(C.addressof(v), C.addressof(view), C.sizeof(ViewParams))
+    rows = p.height - row0 if nrows is None else min(nrows, p.height - row0)
+    rgb = np.zeros((rows, p.width, 3), np.float32)
+    prim = np.zeros(rows * p.width, RAY_DTYPE)
+    buf = np.frombuffer(file_bytes, np.uint8)
+    f = 2 if fmt == 3 else fmt
+    rc = lib().orc_render_frame(_p(buf), buf.size, f, C.byref(p), C.byref(v), C.c_float(beam_projection_factor),
+                                _p(rgb), _p(prim), row0, rows)
+    assert rc == 0, rc
+    return rgb, prim
+
+
+def trace_ao(file_bytes: bytes, fmt: int, params, num_rays: int, row0: int = 0, nrows: int | None = None):
+    """AO_MODE visibility of rows [row0, row0 + nrows) over the primary-ray G-buffer:
+    returns (visibility [rows, width] float32, primary rays)."""
+    p = _copy_params(params)
+    rows = p.height - row0 if nrows is None else min(nrows, p.height - row0)
+    vis = np.zeros((rows, p.width), np.float32)
+    prim = np.zeros(rows * p.width, RAY_DTYPE)
+    buf = np.frombuffer(file_bytes, np.uint8)
+    f = 2 if fmt == 3 else fmt
+    rc = lib().orc_trace_ao(_p(buf), buf.size, f, C.byref(p), int(num_rays), _p(vis), _p(prim), row0, rows)
+    assert rc == 0, rc
+    return vis, prim
+
+
+def hemisphere_samples() -> np.ndarray:
+    """The 256 cosine-weighted Halton directions of OctreeDDARenderer::generateHSSamples, [256, 3]."""
+    out = np.zeros((256, 3), np.float32)
+    lib().orc_hemisphere_samples(_p(out))
+    return out
+
+
+def vm_math(x: np.ndarray):
+    """The oracle's sin, cos and log2 definitions over a float32 array."""
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    sn, cs, lg = (np.zeros_like(x) for _ in range(3))
+    lib().orc_vm_math(_p(x), x.size, _p(sn), _p(cs), _p(lg))
+    return sn, cs, lg
 
 
 def attr_bittree_encode(attrs: np.ndarray, gray: bool) -> bytes:

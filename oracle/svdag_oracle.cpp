@@ -1092,10 +1092,8 @@ static inline void mat4_mul(const float m[16], const float v[4], float r[4]) {
 
 // computeCameraRay (octree_dda.frag.glsl:587-603) for pixel (px, py);
 // (sw, sh) = the screenRes uniform.
-static void camera_ray(const TraceParams& p, uint32_t px, uint32_t py, float sw, float sh, V3& o, V3& d) {
-  const float fx = (float)px + 0.5f, fy = (float)py + 0.5f;
-  const float sx = (fx / sw) * 2.0f - 1.0f;
-  const float sy = (fy / sh) * 2.0f - 1.0f;
+// camera_ray_sc: the same from the shader's screenCoords (sx, sy).
+static void camera_ray_sc(const TraceParams& p, float sx, float sy, V3& o, V3& d) {
   const float s0[4] = {sx, sy, 0.f, 1.f}, s1[4] = {sx, sy, 1.f, 1.f};
   float w0[4], w1[4];
   mat4_mul(p.projInv, s0, w0); mat4_mul(p.projInv, s1, w1);
@@ -1107,6 +1105,12 @@ static void camera_ray(const TraceParams& p, uint32_t px, uint32_t py, float sw,
   mat4_mul(p.viewInv, z4, op); mat4_mul(p.viewInv, d4, ep);
   o = v3(op[0], op[1], op[2]);
   d = normalize3(v3(ep[0] - op[0], ep[1] - op[1], ep[2] - op[2]));
+}
+
+static float screen_coord(uint32_t px, float sw) { return (((float)px + 0.5f) / sw) * 2.0f - 1.0f; }
+
+static void camera_ray(const TraceParams& p, uint32_t px, uint32_t py, float sw, float sh, V3& o, V3& d) {
+  camera_ray_sc(p, screen_coord(px, sw), screen_coord(py, sh), o, d);
 }
 
 // --------------------------------------------------------------------------
@@ -1360,10 +1364,16 @@ int orc_trace_frame_beam(const uint8_t* file, uint64_t len, int fmt, const Trace
   return trace_frame_impl(file, len, fmt, p, primary, shadow, row0, nrows, &depthProjf, minDepthOut);
 }
 
-static int trace_frame_impl(const uint8_t* file, uint64_t len, int fmt, const TraceParams* p,
-                            OrcTraceOut* primary, OrcTraceOut* shadow, uint32_t row0, uint32_t nrows,
-                            const float* depthProjf, float* minDepthOut) {
+// An encoded file image (.svdag / .ussvdag / .ssvdag bytes) as a traversal scene.
+struct LoadedScene {
   Scene s;
+  std::vector<uint32_t> nodes;
+  std::vector<uint16_t> inner;
+  std::vector<uint64_t> leaves;
+};
+
+static int load_scene(const uint8_t* file, uint64_t len, int fmt, LoadedScene& ls) {
+  Scene& s = ls.s;
   s.fmt = fmt;
   size_t off = 0;
   auto rd = [&](void* dst, size_t n) { if (off + n > len) return false; memcpy(dst, file + off, n); off += n; return true; };
@@ -1371,33 +1381,39 @@ static int trace_frame_impl(const uint8_t* file, uint64_t len, int fmt, const Tr
   uint32_t lv, nn;
   if (!rd(s.bmin, 12) || !rd(s.bmax, 12) || !rd(&rs, 4) || !rd(&lv, 4) || !rd(&nn, 4)) return -4;
   s.levels = lv;
-  std::vector<uint32_t> nodes;
-  std::vector<uint16_t> inner;
-  std::vector<uint64_t> leaves;
   if (fmt == 0 || fmt == 1) {
     uint32_t flp, cnt;
     if (!rd(&flp, 4) || !rd(&cnt, 4)) return -4;
-    nodes.resize(cnt);
-    if (!rd(nodes.data(), 4ull * cnt)) return -4;
+    ls.nodes.resize(cnt);
+    if (!rd(ls.nodes.data(), 4ull * cnt)) return -4;
     s.innerLevels = (int)lv - 1; s.leafSize = 2;
-    s.nodes = nodes.data(); s.nNodesWords = cnt;
+    s.nodes = ls.nodes.data(); s.nNodesWords = cnt;
   } else {
     uint32_t cnt;
     if (!rd(&cnt, 4)) return -4;
-    inner.resize(cnt);
-    if (!rd(inner.data(), 2ull * cnt)) return -4;
+    ls.inner.resize(cnt);
+    if (!rd(ls.inner.data(), 2ull * cnt)) return -4;
     if (!rd(&cnt, 4)) return -4;
-    leaves.resize(cnt / 8);
-    if (!rd(leaves.data(), cnt)) return -4;
+    ls.leaves.resize(cnt / 8);
+    if (!rd(ls.leaves.data(), cnt)) return -4;
     if (!rd(&cnt, 4)) return -4;
     s.levelOffsets.resize(cnt);
     if (!rd(s.levelOffsets.data(), 4ull * cnt)) return -4;
     s.innerLevels = (int)lv - 2; s.leafSize = 4;
-    s.inner = inner.data(); s.nInner = inner.size();
-    s.leaves = leaves.data(); s.nLeaves = leaves.size();
+    s.inner = ls.inner.data(); s.nInner = ls.inner.size();
+    s.leaves = ls.leaves.data(); s.nLeaves = ls.leaves.size();
   }
   s.rootHalfSide = rs / 2.0f;   // EncodedOctree::getHalfSide(0) (encoded_octree.hpp:68-70)
   for (int a = 0; a < 3; ++a) s.center[a] = (s.bmin[a] + s.bmax[a]) * 0.5f;
+  return 0;
+}
+
+static int trace_frame_impl(const uint8_t* file, uint64_t len, int fmt, const TraceParams* p,
+                            OrcTraceOut* primary, OrcTraceOut* shadow, uint32_t row0, uint32_t nrows,
+                            const float* depthProjf, float* minDepthOut) {
+  LoadedScene ls;
+  if (int rc = load_scene(file, len, fmt, ls)) return rc;
+  const Scene& s = ls.s;
   Tracer tr(s);
   MinDepthTex md;
   std::vector<float> tex;
@@ -1448,6 +1464,303 @@ static int trace_frame_impl(const uint8_t* file, uint64_t len, int fmt, const Tr
       }
     }
   return 0;
+}
+
+
+// ---- viewer shading: VIEWER_MODE main() (octree_dda.frag.glsl:671-858) and
+// AO_MODE main() (:884-919) --------------------------------------------------
+// GLSL leaves sin, cos, log2 and pow to the implementation.  The definitions
+// restated here are fixed sequences of float operations:
+//  - sin/cos: k = floor(x * 2/pi + 0.5); r = x - k*P1 - k*P2 - k*P3 (pi/2 in
+//    two 12-bit parts and a remainder); Taylor polynomials in r^2 on
+//    [-pi/4, pi/4] (sin to r^11, cos to r^12); quadrant k mod 4.
+//  - log2: x = m 2^e, m in [sqrt(1/2), sqrt(2)); ln m = 2 atanh((m-1)/(m+1))
+//    by its series to the 13th power; log2 x = e + ln m / ln 2.
+//  - mod(x, y) = x - y floor(x / y) (GLSL); pow(x, 2) = x * x.
+// They are pinned against float64 sin / cos / log2 (tests/test_oracle_viewer.py).
+static void vm_sincos(float x, float& sn, float& cs) {
+  const float k = floorf(x * 0.636619747f + 0.5f);
+  float r = x - k * 1.5703125f;
+  r = r - k * 4.8375129699707031e-4f;
+  r = r - k * 7.5497901264043321e-8f;
+  const float r2 = r * r;
+  float ps = -2.50521084e-8f;
+  ps = ps * r2 + 2.75573192e-6f;
+  ps = ps * r2 + -1.98412698e-4f;
+  ps = ps * r2 + 8.33333333e-3f;
+  ps = ps * r2 + -1.66666667e-1f;
+  const float sv = r + (r * r2) * ps;
+  float pc = 2.08767570e-9f;
+  pc = pc * r2 + -2.75573192e-7f;
+  pc = pc * r2 + 2.48015873e-5f;
+  pc = pc * r2 + -1.38888889e-3f;
+  pc = pc * r2 + 4.16666667e-2f;
+  const float cv = (1.0f - 0.5f * r2) + (r2 * r2) * pc;
+  const int q = (int)k & 3;
+  sn = q == 0 ? sv : q == 1 ? cv : q == 2 ? -sv : -cv;
+  cs = q == 0 ? cv : q == 1 ? -sv : q == 2 ? -cv : sv;
+}
+
+static float vm_log2(float x) {
+  if (x != x || x < 0.f) return NAN;
+  if (x == 0.f) return -INFINITY;
+  if (x == INFINITY) return x;
+  int e;
+  float m = frexpf(x, &e);
+  if (m < 0.707106769f) { m = m * 2.0f; e = e - 1; }
+  const float f = (m - 1.0f) / (m + 1.0f);
+  const float q = f * f;
+  float p = 7.69230769e-2f;
+  p = p * q + 9.09090909e-2f;
+  p = p * q + 1.11111111e-1f;
+  p = p * q + 1.42857143e-1f;
+  p = p * q + 2.00000000e-1f;
+  p = p * q + 3.33333333e-1f;
+  const float f2 = 2.0f * f;
+  const float ln = f2 + (f2 * q) * p;
+  return (float)e + ln * 1.44269502f;
+}
+
+static float vm_mod(float x, float y) { return x - y * floorf(x / y); }
+
+struct ViewParams {            // identical layout to svd_view_params (include/svdag_amd.h)
+  int32_t renderMode, normSamples;
+  uint32_t selectedVoxel, randomColors, freqColors, pad;
+};
+
+// The DEPTH_MODE pre-pass texture (octree_dda_renderer.cpp:389-408).
+static MinDepthTex min_depth_texture(Tracer& tr, const TraceParams& p, float projf, std::vector<float>& tex) {
+  MinDepthTex md;
+  md.w = p.width / 8;
+  md.h = p.height / 8;
+  tex.assign((size_t)md.w * md.h, 0.f);
+  const float sw = (float)p.width / 8.0f, sh = (float)p.height / 8.0f;
+  for (uint32_t y = 0; y < md.h; ++y)
+    for (uint32_t x = 0; x < md.w; ++x) {
+      V3 o, d;
+      camera_ray(p, x, y, sw, sh, o, d);
+      float res[4]; int step[3];
+      tr.trace(o, d, 0.0f, 1e30f, projf, p.maxIters, p.drawLevel, res, step);
+      tex[(size_t)y * md.w + x] = res[0] > 0.f ? res[0] : 0.f;
+    }
+  md.t = tex.data();
+  return md;
+}
+
+// One fragment of VIEWER_MODE main().
+static void render_pixel(Tracer& tr, const Scene& s, const TraceParams& p, const ViewParams& v,
+                         const MinDepthTex& md, uint32_t px, uint32_t py, float col[3], OrcTraceOut* prim) {
+  const float sw = (float)p.width, sh = (float)p.height;
+  const float scx = screen_coord(px, sw), scy = screen_coord(py, sh);          // :672
+  V3 o, d;
+  camera_ray_sc(p, scx, scy, o, d);                                             // :673
+  const float tmin = md.t ? min_t(md, px, py) - 1e-3f : 0.0f;                   // :680-681
+  float res[4]; int step[3];
+  tr.path = 2166136261u; tr.bytes = 0;
+  tr.trace(o, d, tmin, 1e30f, p.projectionFactor, p.maxIters, p.drawLevel, res, step);
+  if (prim) *prim = OrcTraceOut{res[0], res[1], res[2], res[3], tr.path, tr.bytes};
+  if (res[0] >= 0.f) {                                                          // :694
+    const V3 hitPos = v3(o.x + res[0] * d.x, o.y + res[0] * d.y, o.z + res[0] * d.z);   // :686
+    const float cellSize = (2.0f * s.rootHalfSide) / ldexpf(1.0f, (int)res[1]);         // :687
+    const uint32_t nodeIndex = (uint32_t)res[3];                                          // :688
+    V3 hitNorm = v3(-(float)step[0], -(float)step[1], -(float)step[2]);                  // trace_ray :566
+    const float diag = length3(v3(s.bmax[0] - s.bmin[0], s.bmax[1] - s.bmin[1], s.bmax[2] - s.bmin[2]));
+    float t = 0.f;
+    if (v.renderMode == 0) {
+      t = 1.0f - (res[2] / (float)p.maxIters);                                  // :697-698
+    } else if (v.renderMode == 1) {
+      t = res[0] / diag;                                                        // :699-700
+    } else if (v.renderMode == 2) {
+      t = vm_log2((2.0f * s.rootHalfSide) / res[1]) / (float)s.levels;           // :701-702
+    } else if (v.renderMode == 3) {                                             // :703-764
+      t = 1.0f;
+      const float normalRayLength = 64.0f * cellSize;
+      V3 smooth = hitNorm;
+      const float normSampleRot = (3.14159f * 2.0f) / (float)v.normSamples;
+      const float ni = (float)nodeIndex;
+      const float initAngleOffset = vm_mod(((res[0] * scx) * ni) * 0.01f + (ni * 0.001f) * scy, 1.0f);
+      for (int i = 0; i < v.normSamples; ++i) {
+        const float rad = 0.1f + vm_mod((((float)i / (float)v.normSamples) * ni) * 0.001f, 0.9f);
+        const float angle = (float)i * normSampleRot + initAngleOffset;
+        float sn, cs;
+        vm_sincos(angle, sn, cs);
+        const float offx = (rad * cs) / sw, offy = (rad * sn) / sh;
+        V3 so, sd;
+        camera_ray_sc(p, scx + offx, scy + offy, so, sd);
+        float sres[4]; int sstep[3];
+        tr.trace(so, sd, res[0] - normalRayLength * 0.5f, res[0] + normalRayLength * 0.5f, p.projectionFactor,
+                 p.maxIters, p.drawLevel, sres, sstep);
+        if (sres[0] >= 0.f && fabsf(sres[0] - res[0]) <= normalRayLength)
+          smooth = v3(smooth.x + -(float)sstep[0], smooth.y + -(float)sstep[1], smooth.z + -(float)sstep[2]);
+      }
+      hitNorm = normalize3(smooth);
+      const V3 lightDir = normalize3(v3(p.lightPos[0] - hitPos.x, p.lightPos[1] - hitPos.y, p.lightPos[2] - hitPos.z));
+      t = t * (0.5f + 0.5f * gmax(dot3(hitNorm, lightDir), 0.0f));
+      const float dist = length3(v3(hitPos.x - p.lightPos[0], hitPos.y - p.lightPos[1], hitPos.z - p.lightPos[2]));
+      t = t - (0.2f * dist) / diag;
+    }
+    col[0] = col[1] = col[2] = t;                                               // :767
+    if (nodeIndex == v.selectedVoxel) {                                         // :771-776
+      col[0] = col[0] * 0.5f; col[1] = col[1] * 0.5f; col[2] = 1.0f;
+    } else if (s.fmt == 2 && v.freqColors) {                                    // :777-792 (SSVDAG)
+      const int hitLev = (int)res[1];
+      const int L = (int)s.levels;
+      const size_t nOff = s.levelOffsets.size();
+      float levSize = 0.f;
+      if (hitLev > L - 3) levSize = (float)(uint32_t)s.nLeaves;                 // textureSize(leafNodes)
+      else if (hitLev == L - 3) {
+        if ((size_t)hitLev < nOff) levSize = (float)((uint32_t)(s.nInner / 4) * 4u - s.levelOffsets[hitLev]);
+      } else if ((size_t)hitLev + 1 < nOff) {
+        levSize = (float)(s.levelOffsets[hitLev + 1] - s.levelOffsets[hitLev]);
+      }
+      const float indexInLevel = (float)nodeIndex / levSize;
+      const float u = 1.0f - indexInLevel;
+      const float a = 1.0f - u * u;
+      // mix(vec3(1), vec3(0, 0.2, 0.8), a) = x (1 - a) + y a
+      col[0] = col[0] * (1.0f * (1.0f - a) + 0.0f * a);
+      col[1] = col[1] * (1.0f * (1.0f - a) + 0.2f * a);
+      col[2] = col[2] * (1.0f * (1.0f - a) + 0.8f * a);
+    } else if (v.randomColors) {                                                // :795-802
+      const V3 rc = normalize3(v3((float)(nodeIndex % 100u) / 100.0f, (float)((3u * nodeIndex) % 200u) / 200.0f,
+                                  (float)((2u * nodeIndex) % 300u) / 300.0f));
+      col[0] = col[0] * rc.x; col[1] = col[1] * rc.y; col[2] = col[2] * rc.z;
+    }
+    if (p.shadows) {                                                            // :805-820
+      const V3 sp = v3(hitPos.x + (hitNorm.x * cellSize) * 0.5f, hitPos.y + (hitNorm.y * cellSize) * 0.5f,
+                       hitPos.z + (hitNorm.z * cellSize) * 0.5f);
+      const V3 l2p = v3(sp.x - p.lightPos[0], sp.y - p.lightPos[1], sp.z - p.lightPos[2]);
+      const float l2pLen = length3(l2p);
+      const float sProj = l2pLen / cellSize;
+      const V3 nd = normalize3(l2p);
+      float sres[4]; int sstep[3];
+      tr.trace(sp, v3(-nd.x, -nd.y, -nd.z), 1e-3f * 0.5f, l2pLen * 0.5f, sProj, p.maxIters, p.drawLevel, sres, sstep);
+      const float f = sres[0] > 0.f ? 0.8f : 1.0f;
+      col[0] = col[0] * f; col[1] = col[1] * f; col[2] = col[2] * f;
+    }
+  } else if (res[0] == -1.f || res[0] == -2.f || res[0] == -4.f) {             // :823-835
+    col[0] = col[1] = col[2] = 1.0f;
+  } else if (res[0] == -3.f) {                                                  // :837-840
+    col[0] = 1.0f; col[1] = 0.0f; col[2] = 0.0f;
+  } else {                                                                      // discard
+    col[0] = col[1] = col[2] = 0.0f;
+  }
+}
+
+// rows [row0, row0 + nrows) of the viewer's frame; rgb holds 3 floats per
+// pixel of those rows (row-major from row0); primary (optional) likewise.
+int orc_render_frame(const uint8_t* file, uint64_t len, int fmt, const TraceParams* p, const ViewParams* v,
+                     float beamProjf, float* rgb, OrcTraceOut* primary, uint32_t row0, uint32_t nrows) {
+  LoadedScene ls;
+  if (int rc = load_scene(file, len, fmt, ls)) return rc;
+  if (v->normSamples < 0 || v->normSamples > 64 || p->drawLevel == 0) return -2;
+  Tracer tr(ls.s);
+  std::vector<float> tex;
+  MinDepthTex md;
+  if (beamProjf > 0.f) md = min_depth_texture(tr, *p, beamProjf, tex);
+  const uint32_t rowEnd = std::min(p->height, row0 + nrows);
+  for (uint32_t py = row0; py < rowEnd; ++py)
+    for (uint32_t px = 0; px < p->width; ++px) {
+      const size_t k = (size_t)(py - row0) * p->width + px;
+      render_pixel(tr, ls.s, *p, *v, md, px, py, rgb + 3 * k, primary ? primary + k : nullptr);
+    }
+  return 0;
+}
+
+// OctreeDDARenderer::generateHSSamples(256) (octree_dda_renderer.cpp:510-531)
+// over HaltonSequence(2) (halton.cpp:24-40, 110-122: bases 2 and 3, index
+// from 1); cos / sin evaluated in double and rounded to float.
+static std::vector<float> hemisphere_samples() {
+  std::vector<float> out;
+  for (int i = 1; i <= 256; ++i) {
+    float h[2];
+    const int base[2] = {2, 3};
+    for (int b = 0; b < 2; ++b) {
+      float result = 0.f, fraction = 1.0f / (float)base[b];
+      int idx = i;
+      while (idx > 0) {
+        int digit = idx % base[b];
+        result += fraction * (float)digit;
+        idx = (idx - digit) / base[b];
+        fraction /= (float)base[b];
+      }
+      h[b] = result;
+    }
+    const float s1 = sqrtf(h[0]);
+    const float l = sqrtf(1.0f - s1 * s1);
+    const float ang = 2.0f * h[1] * 3.141592654f;
+    out.push_back(l * (float)cos((double)ang));
+    out.push_back(s1);
+    out.push_back(l * (float)sin((double)ang));
+  }
+  return out;
+}
+
+void orc_hemisphere_samples(float* out) {
+  const std::vector<float> hs = hemisphere_samples();
+  memcpy(out, hs.data(), hs.size() * sizeof(float));
+}
+
+// AO_MODE over the G-buffer of the primary rays (hit position, -stepDir,
+// cell size 2 rootHalfSide / 2^level; a pixel without a hit has position 0
+// and is discarded: -1).  Sample rays whose direction has a NaN component
+// count as misses (the bounding-box test of transform_ray rejects them).
+int orc_trace_ao(const uint8_t* file, uint64_t len, int fmt, const TraceParams* p, uint32_t numRays, float* vis,
+                 OrcTraceOut* primary, uint32_t row0, uint32_t nrows) {
+  LoadedScene ls;
+  if (int rc = load_scene(file, len, fmt, ls)) return rc;
+  if (numRays > 256 || p->drawLevel == 0) return -2;
+  const Scene& s = ls.s;
+  const std::vector<float> hs = hemisphere_samples();
+  Tracer tr(s);
+  const uint32_t rowEnd = std::min(p->height, row0 + nrows);
+  for (uint32_t py = row0; py < rowEnd; ++py)
+    for (uint32_t px = 0; px < p->width; ++px) {
+      const size_t k = (size_t)(py - row0) * p->width + px;
+      V3 o, d;
+      camera_ray(*p, px, py, (float)p->width, (float)p->height, o, d);
+      float res[4]; int step[3];
+      tr.path = 2166136261u; tr.bytes = 0;
+      tr.trace(o, d, 0.0f, 1e30f, p->projectionFactor, p->maxIters, p->drawLevel, res, step);
+      if (primary) primary[k] = OrcTraceOut{res[0], res[1], res[2], res[3], tr.path, tr.bytes};
+      V3 hitPos = v3(0.f, 0.f, 0.f);
+      if (res[0] >= 0.f) hitPos = v3(o.x + res[0] * d.x, o.y + res[0] * d.y, o.z + res[0] * d.z);
+      if (hitPos.x == 0.f && hitPos.y == 0.f && hitPos.z == 0.f) { vis[k] = -1.0f; continue; }   // :888
+      const V3 n = v3(-(float)step[0], -(float)step[1], -(float)step[2]);
+      const float cellSize = (2.0f * s.rootHalfSide) / ldexpf(1.0f, (int)res[1]);
+      hitPos = v3(hitPos.x + n.x * 1e-3f, hitPos.y + n.y * 1e-3f, hitPos.z + n.z * 1e-3f);            // :891
+      const V3 rvec = v3(0.f, 1.f, 0.f);                                                            // :897
+      const float dn = dot3(rvec, n);
+      const V3 tangent = normalize3(v3(rvec.x - n.x * dn, rvec.y - n.y * dn, rvec.z - n.z * dn));  // :900
+      // cross(n, tangent) (GLSL: x = a.y b.z - b.y a.z, ...)
+      const V3 bitangent = v3(n.y * tangent.z - tangent.y * n.z, n.z * tangent.x - tangent.z * n.x,
+                              n.x * tangent.y - tangent.x * n.y);
+      const float largo = 0.5f;
+      const float projFactor = largo / cellSize;
+      const int base = (int)(((float)px + 0.5f) * ((float)py + 0.5f));
+      float hits = 0.f;
+      for (uint32_t i = 0; i < numRays; ++i) {                                                     // :911-916
+        const float* sm = &hs[3 * (((int)i + base) % 256)];
+        // mat3(tangent, hitNorm, bitangent) * sample
+        const V3 dir = normalize3(v3((tangent.x * sm[0] + n.x * sm[1]) + bitangent.x * sm[2],
+                                     (tangent.y * sm[0] + n.y * sm[1]) + bitangent.y * sm[2],
+                                     (tangent.z * sm[0] + n.z * sm[1]) + bitangent.z * sm[2]));
+        if (dir.x != dir.x || dir.y != dir.y || dir.z != dir.z) continue;
+        float ares[4]; int astep[3];
+        tr.trace(hitPos, dir, 0.0f, largo, projFactor, p->maxIters, p->drawLevel, ares, astep);
+        if (ares[0] > 0.f) hits = hits + 1.0f;
+      }
+      const float visibility = numRays > 0 ? 1.0f - (hits / (float)numRays) : 1.0f;
+      vis[k] = visibility * visibility;
+    }
+  return 0;
+}
+
+// Pinning hooks for the math definitions above.
+void orc_vm_math(const float* x, uint64_t n, float* sn, float* cs, float* lg) {
+  for (uint64_t i = 0; i < n; ++i) {
+    vm_sincos(x[i], sn[i], cs[i]);
+    lg[i] = vm_log2(x[i]);
+  }
 }
 
 }  // extern "C"

@@ -46,6 +46,19 @@ class TraceParams(C.Structure):
                 ("light_pos", C.c_float * 3), ("shadows", C.c_uint32), ("pad", C.c_uint32)]
 
 
+class ViewParams(C.Structure):
+    """svd_view_params: the viewer program's uniforms (octree_dda_renderer.cpp:41-45, 73 defaults)."""
+    _fields_ = [("render_mode", C.c_int32), ("norm_samples", C.c_int32), ("selected_voxel", C.c_uint32),
+                ("random_colors", C.c_uint32), ("freq_colors", C.c_uint32), ("pad", C.c_uint32)]
+
+    @classmethod
+    def defaults(cls, **kw) -> "ViewParams":
+        v = cls(0, 0, 0xFFFFFFFF, 1, 1, 0)
+        for k, x in kw.items():
+            setattr(v, k, x)
+        return v
+
+
 PROPOSER_EXHAUSTIVE, PROPOSER_LEARNED = 0, 1
 MLP_PARAMS = 168352
 
@@ -134,6 +147,8 @@ def lib():
         "svd_scene_free": (i32, [vp]),
         "svd_trace": (i32, [vp, P(TraceParams), vp, vp]),
         "svd_trace_beam": (i32, [vp, P(TraceParams), C.c_float, vp, vp, vp]),
+        "svd_render": (i32, [vp, P(TraceParams), P(ViewParams), C.c_float, vp, vp]),
+        "svd_trace_ao": (i32, [vp, P(TraceParams), C.c_uint32, vp, vp]),
         "svd_last_kernel_ms": (i32, [vp, C.c_char_p, P(C.c_double)]),
         "svd_refcount_order": (i32, [vp, vp, vp, u64]),
         "svd_refcount_order_ranges": (i32, [vp, vp, vp, u64, vp, vp, u64]),
@@ -184,6 +199,7 @@ EXPORTED = ["svd_ctx_create", "svd_ctx_destroy", "svd_last_error", "svd_ctx_sync
             "svd_reduce_sym", "svd_cross_level_merge", "svd_octree_state", "svd_octree_levels",
             "svd_octree_counts", "svd_level_size", "svd_get_level", "svd_encode", "svd_encoded_copy",
             "svd_encoded_device", "svd_scene_create", "svd_scene_from_octree", "svd_scene_free", "svd_trace", "svd_trace_beam",
+            "svd_render", "svd_trace_ao",
             "svd_last_kernel_ms", "svd_refcount_order", "svd_refcount_order_ranges", "svd_attr_bittree_encode", "svd_lossy_merge",
             "svd_lossy_signatures", "svd_mlp_create", "svd_mlp_free", "svd_mlp_get_params", "svd_mlp_set_params",
             "svd_mlp_embed", "svd_mlp_loss_grad", "svd_mlp_train", "svd_knn", "svd_workload_caves",
@@ -727,6 +743,17 @@ class Scene:
         self.ctx.check(self.ctx.L.svd_trace_beam(self.h, C.byref(params), float(depth_projection_factor),
                                                  C.c_void_p(prim_ptr), C.c_void_p(shad_ptr) if shad_ptr else None,
                                                  C.c_void_p(min_depth_ptr) if min_depth_ptr else None))
+
+    def render_device(self, params: TraceParams, view: ViewParams, rgb_ptr: int, beam_projection_factor: float = 0.0,
+                      prim_ptr: int | None = None):
+        """The viewer's colour per pixel (3 floats) into a device buffer; beam pre-pass when the factor is > 0."""
+        self.ctx.check(self.ctx.L.svd_render(self.h, C.byref(params), C.byref(view), float(beam_projection_factor),
+                                             C.c_void_p(rgb_ptr), C.c_void_p(prim_ptr) if prim_ptr else None))
+
+    def ao_device(self, params: TraceParams, num_rays: int, vis_ptr: int, prim_ptr: int | None = None):
+        """AO visibility per pixel (float, -1 without a primary hit) into a device buffer."""
+        self.ctx.check(self.ctx.L.svd_trace_ao(self.h, C.byref(params), int(num_rays), C.c_void_p(vis_ptr),
+                                               C.c_void_p(prim_ptr) if prim_ptr else None))
 
     def trace_device(self, params: TraceParams, prim_ptr: int, shad_ptr: int | None):
         """Trace one frame into device buffers (e.g. torch tensors' data_ptr())."""

@@ -569,6 +569,24 @@ int svd_trace_beam(svd_scene* s, const svd_trace_params* p, float depth_projecti
   });
 }
 
+int svd_render(svd_scene* s, const svd_trace_params* p, const svd_view_params* v, float beam_projection_factor,
+               float* rgb, svd_ray_result* primary) {
+  if (!s || !p || !v || !rgb) return SVD_ERR_ARG;
+  return guarded(s->ctx, [&] {
+    Timer t(s->ctx, "trace");
+    render_frame(s, p, v, beam_projection_factor, rgb, primary);
+  });
+}
+
+int svd_trace_ao(svd_scene* s, const svd_trace_params* p, uint32_t num_rays, float* visibility,
+                 svd_ray_result* primary) {
+  if (!s || !p || !visibility) return SVD_ERR_ARG;
+  return guarded(s->ctx, [&] {
+    Timer t(s->ctx, "trace");
+    ao_frame(s, p, num_rays, visibility, primary);
+  });
+}
+
 int svd_refcount_order(svd_ctx* c, uint32_t* idx, uint32_t* val, uint64_t n) {
   if (!c || (n && (!idx || !val))) return SVD_ERR_ARG;
   return guarded(c, [&] {

@@ -201,6 +201,10 @@ void trace_frame(svd_scene* s, const svd_trace_params* p, svd_ray_result* prim, 
 // beam optimisation: depth pre-pass at 1/8 resolution, then the frame from getMinT(8)
 void trace_frame_beam(svd_scene* s, const svd_trace_params* p, float depthProjf, svd_ray_result* prim,
                       svd_ray_result* shad, float* minDepthOut);
+// viewer colour per pixel (beamProjf > 0: after the depth pre-pass) and the AO pass
+void render_frame(svd_scene* s, const svd_trace_params* p, const svd_view_params* v, float beamProjf, float* rgb,
+                  svd_ray_result* prim);
+void ao_frame(svd_scene* s, const svd_trace_params* p, uint32_t nrays, float* vis, svd_ray_result* prim);
 
 // ---- lossy DAG (lossy.hip), learned encoder (mlp.hip), kNN (knn.hip) ----------
 void sort_u64(svd_ctx* c, uint64_t* keys, uint64_t n);

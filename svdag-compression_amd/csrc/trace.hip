@@ -12,6 +12,7 @@ namespace svd {
 namespace {
 
 constexpr int kBT = 256;                    // threads per workgroup (any tile shape of 256)
+constexpr int kAOSamples = 256;             // NUM_MAX_HS_SAMPLES (octree_dda_renderer.cpp:32)
 
 struct Dev {
   const uint32_t* nodes;
@@ -370,11 +371,8 @@ __device__ __forceinline__ void normalize3(float v[3]) {
 // computeCameraRay (octree_dda.frag.glsl:587-603)
 // (sw, sh) = the shader's screenRes uniform: the frame size, or a fraction of
 // it for the depth pre-pass
-__device__ __forceinline__ void camera_ray(const svd_trace_params& p, uint32_t px, uint32_t py, float sw, float sh,
-                                           float o[3], float d[3]) {
-  const float fx = (float)px + 0.5f, fy = (float)py + 0.5f;
-  const float sx = (fx / sw) * 2.0f - 1.0f;
-  const float sy = (fy / sh) * 2.0f - 1.0f;
+// camera_ray_sc takes the shader's screenCoords (sx, sy) in [-1, 1].
+__device__ __forceinline__ void camera_ray_sc(const svd_trace_params& p, float sx, float sy, float o[3], float d[3]) {
   const float s0[4] = {sx, sy, 0.f, 1.f}, s1[4] = {sx, sy, 1.f, 1.f};
   float w0[4], w1[4];
   mat4_mul(p.proj_inv, s0, w0);
@@ -390,15 +388,18 @@ __device__ __forceinline__ void camera_ray(const svd_trace_params& p, uint32_t p
   normalize3(d);
 }
 
-// Shadow ray of a primary hit (octree_dda.frag.glsl:686-688, 753-754, 803-815, normSamples = 0).
+__device__ __forceinline__ float screen_coord(uint32_t px, float sw) { return (((float)px + 0.5f) / sw) * 2.0f - 1.0f; }
+
+__device__ __forceinline__ void camera_ray(const svd_trace_params& p, uint32_t px, uint32_t py, float sw, float sh,
+                                           float o[3], float d[3]) {
+  camera_ray_sc(p, screen_coord(px, sw), screen_coord(py, sh), o, d);
+}
+
+// Shadow ray from a hit position, its normal and cell size (octree_dda.frag.glsl:803-815).
 struct ShadowRay { float o[3], d[3], tmin, tmax, projf; };
-__device__ __forceinline__ ShadowRay shadow_ray(const Dev& s, const svd_trace_params& p, const float o[3],
-                                                const float d[3], const float res[4], const int step[3]) {
+__device__ __forceinline__ ShadowRay shadow_from(const svd_trace_params& p, const float hit[3], const float hn[3],
+                                                 float cellSize) {
   ShadowRay r;
-  const float hit[3] = {o[0] + res[0] * d[0], o[1] + res[0] * d[1], o[2] + res[0] * d[2]};
-  const float cellSize = (2.0f * s.rootHalfSide) / ldexpf(1.0f, (int)res[1]);
-  float hn[3] = {-(float)step[0], -(float)step[1], -(float)step[2]};
-  normalize3(hn);
   const float pp[3] = {hit[0] + (hn[0] * cellSize) * 0.5f, hit[1] + (hn[1] * cellSize) * 0.5f,
                        hit[2] + (hn[2] * cellSize) * 0.5f};
   const float l2p[3] = {pp[0] - p.light_pos[0], pp[1] - p.light_pos[1], pp[2] - p.light_pos[2]};
@@ -409,6 +410,17 @@ __device__ __forceinline__ ShadowRay shadow_ray(const Dev& s, const svd_trace_pa
   r.tmin = 1e-3f * 0.5f;
   r.tmax = len * 0.5f;
   return r;
+}
+
+// Shadow ray of a primary hit as the pretty mode casts it with no normal
+// samples: hitNorm = normalize(-stepDir) (octree_dda.frag.glsl:686-688, 753-754, 803-815).
+__device__ __forceinline__ ShadowRay shadow_ray(const Dev& s, const svd_trace_params& p, const float o[3],
+                                                const float d[3], const float res[4], const int step[3]) {
+  const float hit[3] = {o[0] + res[0] * d[0], o[1] + res[0] * d[1], o[2] + res[0] * d[2]};
+  const float cellSize = (2.0f * s.rootHalfSide) / ldexpf(1.0f, (int)res[1]);
+  float hn[3] = {-(float)step[0], -(float)step[1], -(float)step[2]};
+  normalize3(hn);
+  return shadow_from(p, hit, hn, cellSize);
 }
 
 // The minimum-depth pre-pass ("beam optimisation", octree_dda_renderer.cpp:389-408):
@@ -465,6 +477,247 @@ __device__ __forceinline__ void trace_pixel(const Dev& s, const svd_trace_params
     sr = svd_ray_result{sres[0], sres[1], sres[2], sres[3], tr.path, tr.bytes};
   }
   shad[k] = sr;
+}
+
+// ---- viewer shading ---------------------------------------------------------
+// GLSL leaves sin, cos and log2 to the implementation.  This build defines
+// them with plain float operations in a fixed order, so that a CPU
+// restatement reproduces every bit:
+//  - sin/cos: k = floor(x * 2/pi + 0.5); r = x - k*P1 - k*P2 - k*P3 with pi/2
+//    split into two 12-bit parts and a remainder (k*P1 and k*P2 are exact for
+//    |k| < 2^12); Taylor polynomials in r^2 on [-pi/4, pi/4] (sin to r^11,
+//    cos to r^12); the quadrant k mod 4 picks the sign and function.
+//  - log2: x = m * 2^e with m in [sqrt(1/2), sqrt(2)); ln m = 2 atanh f with
+//    f = (m - 1) / (m + 1), series to f^13; log2 x = e + ln m / ln 2.
+//  - mod(x, y) = x - y * floor(x / y) (the GLSL definition).
+__device__ __forceinline__ void vm_sincos(float x, float& sn, float& cs) {
+  const float k = floorf(x * 0.636619747f + 0.5f);
+  float r = x - k * 1.5703125f;
+  r = r - k * 4.8375129699707031e-4f;
+  r = r - k * 7.5497901264043321e-8f;
+  const float r2 = r * r;
+  float ps = -2.50521084e-8f;          // -1/11!
+  ps = ps * r2 + 2.75573192e-6f;       //  1/9!
+  ps = ps * r2 + -1.98412698e-4f;      // -1/7!
+  ps = ps * r2 + 8.33333333e-3f;       //  1/5!
+  ps = ps * r2 + -1.66666667e-1f;      // -1/3!
+  const float s = r + (r * r2) * ps;
+  float pc = 2.08767570e-9f;           //  1/12!
+  pc = pc * r2 + -2.75573192e-7f;      // -1/10!
+  pc = pc * r2 + 2.48015873e-5f;       //  1/8!
+  pc = pc * r2 + -1.38888889e-3f;      // -1/6!
+  pc = pc * r2 + 4.16666667e-2f;       //  1/4!
+  const float c = (1.0f - 0.5f * r2) + (r2 * r2) * pc;
+  const int q = (int)k & 3;
+  sn = q == 0 ? s : q == 1 ? c : q == 2 ? -s : -c;
+  cs = q == 0 ? c : q == 1 ? -s : q == 2 ? -c : s;
+}
+
+__device__ __forceinline__ float vm_log2(float x) {
+  if (x != x || x < 0.f) return __builtin_nanf("");
+  if (x == 0.f) return -__builtin_inff();
+  if (x == __builtin_inff()) return x;
+  int e;
+  float m = frexpf(x, &e);
+  if (m < 0.707106769f) { m = m * 2.0f; e = e - 1; }
+  const float f = (m - 1.0f) / (m + 1.0f);
+  const float s = f * f;
+  float p = 7.69230769e-2f;            // 1/13
+  p = p * s + 9.09090909e-2f;          // 1/11
+  p = p * s + 1.11111111e-1f;          // 1/9
+  p = p * s + 1.42857143e-1f;          // 1/7
+  p = p * s + 2.00000000e-1f;          // 1/5
+  p = p * s + 3.33333333e-1f;          // 1/3
+  const float f2 = 2.0f * f;
+  const float ln = f2 + (f2 * s) * p;
+  return (float)e + ln * 1.44269502f;
+}
+
+__device__ __forceinline__ float vm_mod(float x, float y) { return x - y * floorf(x / y); }
+
+// Uniforms of the viewer program beyond the traversal's.
+struct ViewArgs {
+  int mode, normSamples;
+  uint32_t selected, randomColors, freqColors;
+  uint32_t levels;                  // LEVELS
+  uint32_t leafTexels, innerTexels; // textureSize(leafNodes), textureSize(innerNodes) (SSVDAG)
+  uint32_t nOffsets;                // entries of levelOffsets
+};
+
+// VIEWER_MODE main() for one pixel (octree_dda.frag.glsl:671-858): primary
+// ray (from getMinT(8) - 1e-3 with a depth texture), the render mode's
+// intensity, the normal samples of the pretty mode, the highlight / frequency /
+// random colouring and the shadow ray.  rgb gets the colour the fragment
+// writes; a fragment the shader discards gets 0.
+template <int FMT>
+__device__ __forceinline__ void render_pixel(const Dev& s, const svd_trace_params& p, const ViewArgs& va,
+                                             uint32_t px, uint32_t py, uint32_t* stackMem, float* __restrict__ rgb,
+                                             svd_ray_result* __restrict__ prim, MinDepth md) {
+  const int lane = threadIdx.y * blockDim.x + threadIdx.x;
+  Tracer<FMT> tr(s, make_stack<FMT>(stackMem, s.innerLevels + 1, lane));
+  const float sw = (float)p.width, sh = (float)p.height;
+  const float scx = screen_coord(px, sw), scy = screen_coord(py, sh);
+  float o[3], d[3];
+  camera_ray_sc(p, scx, scy, o, d);
+  float res[4];
+  int step[3];
+  const float tmin = md.t ? min_t(md, px, py) - 1e-3f : 0.0f;
+  tr.trace(o, d, tmin, 1e30f, p.projection_factor, p.max_iters, p.draw_level, res, step);
+  if (tr.bad) { res[0] = -6.f; res[1] = 0.f; res[3] = -1.f; }
+  const size_t k = (size_t)py * p.width + px;
+  if (prim) prim[k] = svd_ray_result{res[0], res[1], res[2], res[3], tr.path, tr.bytes};
+  float col[3];
+  if (res[0] >= 0.f) {
+    const float hit[3] = {o[0] + res[0] * d[0], o[1] + res[0] * d[1], o[2] + res[0] * d[2]};
+    const float cellSize = (2.0f * s.rootHalfSide) / ldexpf(1.0f, (int)res[1]);
+    const uint32_t nodeIndex = (uint32_t)res[3];
+    float hn[3] = {-(float)step[0], -(float)step[1], -(float)step[2]};
+    const float ext[3] = {s.bmax[0] - s.bmin[0], s.bmax[1] - s.bmin[1], s.bmax[2] - s.bmin[2]};
+    const float diag = sqrtf(dot3(ext, ext));
+    float t = 0.f;
+    if (va.mode == 0) {
+      t = 1.0f - (res[2] / (float)p.max_iters);
+    } else if (va.mode == 1) {
+      t = res[0] / diag;
+    } else if (va.mode == 2) {
+      t = vm_log2((2.0f * s.rootHalfSide) / res[1]) / (float)va.levels;
+    } else if (va.mode == 3) {
+      t = 1.0f;
+      const float nrl = 64.0f * cellSize;
+      const float fni = (float)nodeIndex;
+      const float rot = (3.14159f * 2.0f) / (float)va.normSamples;
+      const float off = vm_mod(((res[0] * scx) * fni) * 0.01f + (fni * 0.001f) * scy, 1.0f);
+      for (int i = 0; i < va.normSamples; ++i) {
+        const float rad = 0.1f + vm_mod((((float)i / (float)va.normSamples) * fni) * 0.001f, 0.9f);
+        const float ang = (float)i * rot + off;
+        float sn, cs;
+        vm_sincos(ang, sn, cs);
+        float so[3], sd[3];
+        camera_ray_sc(p, scx + (rad * cs) / sw, scy + (rad * sn) / sh, so, sd);
+        float sres[4];
+        int sstep[3];
+        tr.trace(so, sd, res[0] - nrl * 0.5f, res[0] + nrl * 0.5f, p.projection_factor, p.max_iters, p.draw_level,
+                 sres, sstep);
+        if (sres[0] >= 0.f && fabsf(sres[0] - res[0]) <= nrl) {
+          hn[0] = hn[0] + -(float)sstep[0];
+          hn[1] = hn[1] + -(float)sstep[1];
+          hn[2] = hn[2] + -(float)sstep[2];
+        }
+      }
+      normalize3(hn);
+      float ld[3] = {p.light_pos[0] - hit[0], p.light_pos[1] - hit[1], p.light_pos[2] - hit[2]};
+      normalize3(ld);
+      t = t * (0.5f + 0.5f * gmax(dot3(hn, ld), 0.0f));
+      const float dl[3] = {hit[0] - p.light_pos[0], hit[1] - p.light_pos[1], hit[2] - p.light_pos[2]};
+      t = t - (0.2f * sqrtf(dot3(dl, dl))) / diag;
+    }
+    col[0] = col[1] = col[2] = t;
+    if (nodeIndex == va.selected) {
+      col[0] = col[0] * 0.5f;
+      col[1] = col[1] * 0.5f;
+      col[2] = 1.0f;
+    } else if (FMT == SVD_FMT_SSVDAG && va.freqColors) {
+      const int hitLev = (int)res[1];
+      float levSize = 0.f;
+      if (hitLev > (int)va.levels - 3) {
+        levSize = (float)va.leafTexels;
+      } else if (hitLev == (int)va.levels - 3) {
+        if ((uint32_t)hitLev < va.nOffsets) levSize = (float)(va.innerTexels * 4u - s.levelOffsets[hitLev]);
+      } else if ((uint32_t)hitLev + 1 < va.nOffsets) {
+        levSize = (float)(s.levelOffsets[hitLev + 1] - s.levelOffsets[hitLev]);
+      }
+      const float u = 1.0f - (float)nodeIndex / levSize;
+      const float a = 1.0f - u * u;
+      col[0] = col[0] * (1.0f * (1.0f - a) + 0.0f * a);
+      col[1] = col[1] * (1.0f * (1.0f - a) + 0.2f * a);
+      col[2] = col[2] * (1.0f * (1.0f - a) + 0.8f * a);
+    } else if (va.randomColors) {
+      float rc[3] = {(float)(nodeIndex % 100u) / 100.0f, (float)((3u * nodeIndex) % 200u) / 200.0f,
+                     (float)((2u * nodeIndex) % 300u) / 300.0f};
+      normalize3(rc);
+      col[0] = col[0] * rc[0];
+      col[1] = col[1] * rc[1];
+      col[2] = col[2] * rc[2];
+    }
+    if (p.shadows) {
+      const ShadowRay sh = shadow_from(p, hit, hn, cellSize);
+      float sres[4];
+      int sstep[3];
+      tr.trace(sh.o, sh.d, sh.tmin, sh.tmax, sh.projf, p.max_iters, p.draw_level, sres, sstep);
+      const float f = sres[0] > 0.f ? 0.8f : 1.0f;
+      col[0] = col[0] * f;
+      col[1] = col[1] * f;
+      col[2] = col[2] * f;
+    }
+  } else if (res[0] == -1.f || res[0] == -2.f || res[0] == -4.f) {
+    col[0] = col[1] = col[2] = 1.0f;
+  } else if (res[0] == -3.f) {
+    col[0] = 1.0f; col[1] = 0.0f; col[2] = 0.0f;
+  } else {
+    col[0] = col[1] = col[2] = 0.0f;
+  }
+  rgb[3 * k + 0] = col[0];
+  rgb[3 * k + 1] = col[1];
+  rgb[3 * k + 2] = col[2];
+}
+
+// AO_MODE main() for one pixel (octree_dda.frag.glsl:884-919).  The
+// reference's AO program reads a G-buffer (hitPosTex, hitNormTex, depthTex)
+// that nothing in the reference writes; this build defines it from the
+// pixel's primary ray: hitPos = o + t d of a hit and (0, 0, 0) otherwise,
+// hitNorm = -stepDir as trace_ray returns it, cell size 2 rootHalfSide / 2^level.
+// hs = the 256 cosine-weighted hemisphere samples (3 floats each).  A
+// discarded fragment (hitPos = 0) gets -1.  A sample ray whose direction has
+// a NaN component counts as a miss (the box test rejects such rays).
+template <int FMT>
+__device__ __forceinline__ void ao_pixel(const Dev& s, const svd_trace_params& p, const float* __restrict__ hs,
+                                         uint32_t nrays, uint32_t px, uint32_t py, uint32_t* stackMem,
+                                         float* __restrict__ vis, svd_ray_result* __restrict__ prim) {
+  const int lane = threadIdx.y * blockDim.x + threadIdx.x;
+  Tracer<FMT> tr(s, make_stack<FMT>(stackMem, s.innerLevels + 1, lane));
+  float o[3], d[3];
+  camera_ray(p, px, py, (float)p.width, (float)p.height, o, d);
+  float res[4];
+  int step[3];
+  tr.trace(o, d, 0.0f, 1e30f, p.projection_factor, p.max_iters, p.draw_level, res, step);
+  if (tr.bad) { res[0] = -6.f; res[1] = 0.f; res[3] = -1.f; }
+  const size_t k = (size_t)py * p.width + px;
+  if (prim) prim[k] = svd_ray_result{res[0], res[1], res[2], res[3], tr.path, tr.bytes};
+  float hit[3] = {0.f, 0.f, 0.f};
+  if (res[0] >= 0.f) {
+    hit[0] = o[0] + res[0] * d[0];
+    hit[1] = o[1] + res[0] * d[1];
+    hit[2] = o[2] + res[0] * d[2];
+  }
+  if (hit[0] == 0.f && hit[1] == 0.f && hit[2] == 0.f) { vis[k] = -1.0f; return; }
+  const float hn[3] = {-(float)step[0], -(float)step[1], -(float)step[2]};
+  const float cellSize = (2.0f * s.rootHalfSide) / ldexpf(1.0f, (int)res[1]);
+  hit[0] = hit[0] + hn[0] * 1e-3f;
+  hit[1] = hit[1] + hn[1] * 1e-3f;
+  hit[2] = hit[2] + hn[2] * 1e-3f;
+  const float rv[3] = {0.f, 1.f, 0.f};
+  const float dn = dot3(rv, hn);
+  float tg[3] = {rv[0] - hn[0] * dn, rv[1] - hn[1] * dn, rv[2] - hn[2] * dn};
+  normalize3(tg);
+  const float bt[3] = {hn[1] * tg[2] - tg[1] * hn[2], hn[2] * tg[0] - tg[2] * hn[0], hn[0] * tg[1] - tg[0] * hn[1]};
+  const float largo = 0.5f;
+  const float projf = largo / cellSize;
+  const int base = (int)((((float)px + 0.5f)) * (((float)py + 0.5f)));
+  float kk = 0.f;
+  for (uint32_t i = 0; i < nrays; ++i) {
+    const float* sm = hs + 3 * (((int)i + base) % kAOSamples);
+    float rd[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) rd[a] = (tg[a] * sm[0] + hn[a] * sm[1]) + bt[a] * sm[2];
+    normalize3(rd);
+    if (rd[0] != rd[0] || rd[1] != rd[1] || rd[2] != rd[2]) continue;
+    float ares[4];
+    int astep[3];
+    tr.trace(hit, rd, 0.0f, largo, projf, p.max_iters, p.draw_level, ares, astep);
+    if (ares[0] > 0.f) kk = kk + 1.0f;
+  }
+  const float v = nrays > 0 ? 1.0f - (kk / (float)nrays) : 1.0f;
+  vis[k] = v * v;
 }
 
 // Persistent form: lanes that finished their rays take new ones from a
@@ -640,6 +893,29 @@ __global__ void __launch_bounds__(kBT) SVD_TRACE_OCC k_trace_batch(const Dev* __
   trace_pixel<FMT>(s, p, px, py, stackMem, prim + off, shad ? shad + off : nullptr);
 }
 
+template <int FMT>
+__global__ void __launch_bounds__(kBT) SVD_TRACE_OCC k_render(Dev s, svd_trace_params p, ViewArgs va,
+                                                                float* __restrict__ rgb,
+                                                                svd_ray_result* __restrict__ prim, MinDepth md) {
+  extern __shared__ uint32_t stackMem[];
+  const uint32_t px = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t py = blockIdx.y * blockDim.y + threadIdx.y;
+  if (px >= p.width || py >= p.height) return;
+  render_pixel<FMT>(s, p, va, px, py, stackMem, rgb, prim, md);
+}
+
+template <int FMT>
+__global__ void __launch_bounds__(kBT) SVD_TRACE_OCC k_trace_ao(Dev s, svd_trace_params p,
+                                                                  const float* __restrict__ hs, uint32_t nrays,
+                                                                  float* __restrict__ vis,
+                                                                  svd_ray_result* __restrict__ prim) {
+  extern __shared__ uint32_t stackMem[];
+  const uint32_t px = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t py = blockIdx.y * blockDim.y + threadIdx.y;
+  if (px >= p.width || py >= p.height) return;
+  ao_pixel<FMT>(s, p, hs, nrays, px, py, stackMem, vis, prim);
+}
+
 }  // namespace
 
 void trace_frame_impl(svd_scene* sc, const svd_trace_params* p, svd_ray_result* prim, svd_ray_result* shad,
@@ -657,9 +933,7 @@ void trace_frame_beam(svd_scene* sc, const svd_trace_params* p, float depthProjf
   trace_frame_impl(sc, p, prim, shad, &depthProjf, minDepthOut);
 }
 
-void trace_frame_impl(svd_scene* sc, const svd_trace_params* p, svd_ray_result* prim, svd_ray_result* shad,
-                      const float* depthProjf, float* minDepthOut) {
-  svd_ctx* c = sc->ctx;
+Dev scene_dev(const svd_scene* sc) {
   Dev d{};
   d.nodes = sc->nodes;
   d.inner = sc->inner;
@@ -672,37 +946,59 @@ void trace_frame_impl(svd_scene* sc, const svd_trace_params* p, svd_ray_result* 
   else { d.innerLevels = (int)sc->levels - 1; d.leafSize = 2; }
   for (int a = 0; a < 3; ++a) { d.bmin[a] = sc->bmin[a]; d.bmax[a] = sc->bmax[a]; d.center[a] = sc->center[a]; }
   d.rootHalfSide = sc->rootHalfSide;
+  return d;
+}
+
+namespace {
+
+// LDS of one workgroup: the descent stacks of its kBT lanes
+size_t stack_bytes(const svd_scene* sc, const Dev& d) {
+  return (size_t)(sc->fmt == SVD_FMT_USSVDAG ? kStackWords<SVD_FMT_USSVDAG> : 2) * (d.innerLevels + 1) * kBT *
+         sizeof(uint32_t);
+}
+
+// The DEPTH_MODE pre-pass into `depth` ((width/8) x (height/8) floats).  The
+// texture is (int)width / 8 x (int)height / 8; its screenRes uniform is the
+// float width / 8, height / 8.
+MinDepth depth_prepass(svd_scene* sc, const Dev& d, const svd_trace_params* p, float projf, float* depth) {
+  svd_ctx* c = sc->ctx;
+  MinDepth md{depth, p->width / 8, p->height / 8};
+  if (!md.w || !md.h) return md;
+  const uint32_t tw = c->traceTileW, th = kBT / tw;
+  const size_t shmem = stack_bytes(sc, d);
+  const float sw = (float)p->width / 8.0f, sh = (float)p->height / 8.0f;
+  const dim3 dg(div_up(md.w, tw), div_up(md.h, th)), db(tw, th);
+  switch (sc->fmt) {
+    case SVD_FMT_SVDAG:
+      k_trace_depth<SVD_FMT_SVDAG><<<dg, db, shmem, c->stream>>>(d, *p, sw, sh, projf, md.w, md.h, depth);
+      break;
+    case SVD_FMT_USSVDAG:
+      k_trace_depth<SVD_FMT_USSVDAG><<<dg, db, shmem, c->stream>>>(d, *p, sw, sh, projf, md.w, md.h, depth);
+      break;
+    default:
+      k_trace_depth<SVD_FMT_SSVDAG><<<dg, db, shmem, c->stream>>>(d, *p, sw, sh, projf, md.w, md.h, depth);
+      break;
+  }
+  check_launch("k_trace_depth");
+  return md;
+}
+
+}  // namespace
+
+void trace_frame_impl(svd_scene* sc, const svd_trace_params* p, svd_ray_result* prim, svd_ray_result* shad,
+                      const float* depthProjf, float* minDepthOut) {
+  svd_ctx* c = sc->ctx;
+  const Dev d = scene_dev(sc);
   if (p->width == 0 || p->height == 0) return;
   if (p->draw_level == 0) throw Error{SVD_ERR_ARG, "draw_level must be >= 1"};
   const uint32_t tw = c->traceTileW, th = kBT / tw;
   const dim3 grid(div_up(p->width, tw), div_up(p->height, th));
-  const size_t shmem = (size_t)(sc->fmt == SVD_FMT_USSVDAG ? kStackWords<SVD_FMT_USSVDAG> : 2) *
-                       (d.innerLevels + 1) * kBT * sizeof(uint32_t);
+  const size_t shmem = stack_bytes(sc, d);
   MinDepth md{nullptr, 0, 0};
   float* depth = nullptr;
   if (depthProjf) {
-    // the pre-pass texture is (int)width / 8 x (int)height / 8; its screenRes
-    // uniform is the float width / 8, height / 8
-    md.w = p->width / 8;
-    md.h = p->height / 8;
-    depth = minDepthOut ? minDepthOut : dalloc<float>(c, (uint64_t)md.w * md.h + 1);
-    md.t = depth;
-    if (md.w && md.h) {
-      const float sw = (float)p->width / 8.0f, sh = (float)p->height / 8.0f;
-      const dim3 dg(div_up(md.w, tw), div_up(md.h, th)), db(tw, th);
-      switch (sc->fmt) {
-        case SVD_FMT_SVDAG:
-          k_trace_depth<SVD_FMT_SVDAG><<<dg, db, shmem, c->stream>>>(d, *p, sw, sh, *depthProjf, md.w, md.h, depth);
-          break;
-        case SVD_FMT_USSVDAG:
-          k_trace_depth<SVD_FMT_USSVDAG><<<dg, db, shmem, c->stream>>>(d, *p, sw, sh, *depthProjf, md.w, md.h, depth);
-          break;
-        default:
-          k_trace_depth<SVD_FMT_SSVDAG><<<dg, db, shmem, c->stream>>>(d, *p, sw, sh, *depthProjf, md.w, md.h, depth);
-          break;
-      }
-      check_launch("k_trace_depth");
-    }
+    depth = minDepthOut ? minDepthOut : dalloc<float>(c, (uint64_t)(p->width / 8) * (p->height / 8) + 1);
+    md = depth_prepass(sc, d, p, *depthProjf, depth);
   }
   if (c->traceMode == SVD_TRACE_PERSISTENT && !depthProjf) {
     const uint64_t total = (uint64_t)grid.x * grid.y * kBT;
@@ -751,20 +1047,108 @@ void trace_frame_impl(svd_scene* sc, const svd_trace_params* p, svd_ray_result* 
   if (depth && depth != minDepthOut) dfree(c, depth);
 }
 
-Dev scene_dev(const svd_scene* sc) {
-  Dev d{};
-  d.nodes = sc->nodes;
-  d.inner = sc->inner;
-  d.leaves = (const uint2*)sc->leaves;
-  d.levelOffsets = sc->levelOffsets;
-  d.nWords = sc->nWords;
-  d.nInner = sc->nInner;
-  d.nLeaves = sc->nLeaves;
-  if (sc->fmt == SVD_FMT_SSVDAG || sc->fmt == SVD_FMT_ESVDAG) { d.innerLevels = (int)sc->levels - 2; d.leafSize = 4; }
-  else { d.innerLevels = (int)sc->levels - 1; d.leafSize = 2; }
-  for (int a = 0; a < 3; ++a) { d.bmin[a] = sc->bmin[a]; d.bmax[a] = sc->bmax[a]; d.center[a] = sc->center[a]; }
-  d.rootHalfSide = sc->rootHalfSide;
-  return d;
+// The viewer's frame (VIEWER_MODE, octree_dda.frag.glsl:671-858), optionally
+// after the minimum-depth pre-pass (beamProjf > 0).  rgb: 3 floats per pixel.
+void render_frame(svd_scene* sc, const svd_trace_params* p, const svd_view_params* v, float beamProjf, float* rgb,
+                  svd_ray_result* prim) {
+  svd_ctx* c = sc->ctx;
+  const Dev d = scene_dev(sc);
+  if (v->norm_samples < 0 || v->norm_samples > 64) throw Error{SVD_ERR_ARG, "norm_samples must be in [0, 64]"};
+  if (p->width == 0 || p->height == 0) return;
+  if (p->draw_level == 0) throw Error{SVD_ERR_ARG, "draw_level must be >= 1"};
+  if (!rgb) throw Error{SVD_ERR_ARG, "rgb is NULL"};
+  ViewArgs va{};
+  va.mode = v->render_mode;
+  va.normSamples = v->norm_samples;
+  va.selected = v->selected_voxel;
+  va.randomColors = v->random_colors;
+  va.freqColors = v->freq_colors;
+  va.levels = sc->levels;
+  va.leafTexels = sc->nLeaves;          // RG32UI texels: one per 8-byte leaf
+  va.innerTexels = sc->nInner / 4;      // RGBA16UI texels: four u16 each
+  va.nOffsets = sc->nLevelOffsets;
+  const uint32_t tw = c->traceTileW, th = kBT / tw;
+  const dim3 grid(div_up(p->width, tw), div_up(p->height, th)), block(tw, th);
+  const size_t shmem = stack_bytes(sc, d);
+  MinDepth md{nullptr, 0, 0};
+  float* depth = nullptr;
+  if (beamProjf > 0.f) {
+    depth = dalloc<float>(c, (uint64_t)(p->width / 8) * (p->height / 8) + 1);
+    md = depth_prepass(sc, d, p, beamProjf, depth);
+  }
+  switch (sc->fmt) {
+    case SVD_FMT_SVDAG: k_render<SVD_FMT_SVDAG><<<grid, block, shmem, c->stream>>>(d, *p, va, rgb, prim, md); break;
+    case SVD_FMT_USSVDAG:
+      k_render<SVD_FMT_USSVDAG><<<grid, block, shmem, c->stream>>>(d, *p, va, rgb, prim, md);
+      break;
+    default: k_render<SVD_FMT_SSVDAG><<<grid, block, shmem, c->stream>>>(d, *p, va, rgb, prim, md); break;
+  }
+  check_launch("k_render");
+  dfree(c, depth);
+}
+
+namespace {
+// The 256 hemisphere samples of OctreeDDARenderer::generateHSSamples
+// (octree_dda_renderer.cpp:510-531): Halton points (bases 2 and 3,
+// halton.cpp:24-40) mapped to cosine-weighted directions around +y.
+const std::vector<float>& hemisphere_samples() {
+  static const std::vector<float> hs = [] {
+    std::vector<float> out;
+    out.reserve(3 * kAOSamples);
+    for (int i = 1; i <= kAOSamples; ++i) {
+      float h[2];
+      const int bases[2] = {2, 3};
+      for (int b = 0; b < 2; ++b) {
+        float r = 0.f, frac = 1.0f / (float)bases[b];
+        int idx = i;
+        while (idx > 0) {
+          const int digit = idx % bases[b];
+          r += frac * (float)digit;
+          idx = (idx - digit) / bases[b];
+          frac /= (float)bases[b];
+        }
+        h[b] = r;
+      }
+      const float y = std::sqrt(h[0]);
+      const float l = std::sqrt(1.0f - y * y);
+      const float ang = 2.0f * h[1] * 3.141592654f;
+      // cos / sin in double, rounded once to float
+      out.push_back(l * (float)std::cos((double)ang));
+      out.push_back(y);
+      out.push_back(l * (float)std::sin((double)ang));
+    }
+    return out;
+  }();
+  return hs;
+}
+}  // namespace
+
+// AO_MODE (octree_dda.frag.glsl:884-919) over the G-buffer of the frame's
+// primary rays: visibility^2 per pixel, -1 where the pixel has no hit.
+void ao_frame(svd_scene* sc, const svd_trace_params* p, uint32_t nrays, float* vis, svd_ray_result* prim) {
+  svd_ctx* c = sc->ctx;
+  const Dev d = scene_dev(sc);
+  if (nrays > (uint32_t)kAOSamples) throw Error{SVD_ERR_ARG, "num_rays must be <= 256"};
+  if (p->width == 0 || p->height == 0) return;
+  if (p->draw_level == 0) throw Error{SVD_ERR_ARG, "draw_level must be >= 1"};
+  if (!vis) throw Error{SVD_ERR_ARG, "visibility is NULL"};
+  const std::vector<float>& hs = hemisphere_samples();
+  float* dhs = dalloc<float>(c, hs.size());
+  SVD_HIP(hipMemcpyAsync(dhs, hs.data(), hs.size() * sizeof(float), hipMemcpyHostToDevice, c->stream));
+  const uint32_t tw = c->traceTileW, th = kBT / tw;
+  const dim3 grid(div_up(p->width, tw), div_up(p->height, th)), block(tw, th);
+  const size_t shmem = stack_bytes(sc, d);
+  switch (sc->fmt) {
+    case SVD_FMT_SVDAG:
+      k_trace_ao<SVD_FMT_SVDAG><<<grid, block, shmem, c->stream>>>(d, *p, dhs, nrays, vis, prim);
+      break;
+    case SVD_FMT_USSVDAG:
+      k_trace_ao<SVD_FMT_USSVDAG><<<grid, block, shmem, c->stream>>>(d, *p, dhs, nrays, vis, prim);
+      break;
+    default: k_trace_ao<SVD_FMT_SSVDAG><<<grid, block, shmem, c->stream>>>(d, *p, dhs, nrays, vis, prim); break;
+  }
+  check_launch("k_trace_ao");
+  dfree(c, dhs);
 }
 
 // SVDAG scenes (nodes, nWords, levels, box) x cameras in one launch.

@@ -1,10 +1,17 @@
 // svviewer -- headless form of the reference viewer (src/svviewer/main.cpp):
 //   svviewer file.{svdag,ussvdag,ssvdag,esvdag} [--width W] [--height H]
 //            [--frames N] [--seed S] [--no-shadows] [--no-beam] [--image out.ppm]
+//            [--mode M] [--norm-samples K] [--select NODE] [--no-random-colors]
+//            [--no-freq-colors] [--ao RAYS]
 // Loads the file (EncodedOctree::load), traces frames with the octree-DDA
 // kernel from a seeded orbit camera (renderer defaults: FOV 60, maxIters 300,
 // pixel tolerance 1, drawLevel = levels, light at the box max) and prints
 // Rays/s like the viewer's window title (svviewer/main.cpp:166-183).
+// --image writes the last frame as the viewer draws it (svd_render: render
+// mode M = viewerRenderMode 0 iterations / 1 depth / 2 levels / 3 pretty, K
+// normal samples, the selected node, random and frequency colours; the
+// framebuffer's clamp to [0, 1]); with --ao it writes the AO pass's visibility
+// of RAYS hemisphere rays instead (svd_trace_ao).
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -84,7 +91,8 @@ void orbit_camera(uint64_t seed, const svd::aabox3f& box, uint32_t w, uint32_t h
 int main(int argc, char** argv) {
   if (argc < 2) {
     printf("Usage: svviewer file.{svdag,ussvdag,ssvdag,esvdag} [--width W] [--height H] [--frames N] [--seed S] "
-           "[--no-shadows] [--no-beam] [--image out.ppm]\n");
+           "[--no-shadows] [--no-beam] [--image out.ppm] [--mode M] [--norm-samples K] [--select NODE] "
+           "[--no-random-colors] [--no-freq-colors] [--ao RAYS]\n");
     return 1;
   }
   std::string file = argv[1];
@@ -92,6 +100,9 @@ int main(int argc, char** argv) {
   uint64_t seed = 1;
   bool shadows = true, beam = true;   // the viewer's minimum-depth optimisation is on by default
   std::string image;
+  // OctreeDDARenderer defaults (octree_dda_renderer.cpp:41-45, 73)
+  svd_view_params view{0, 0, 0xFFFFFFFFu, 1, 1, 0};
+  int aoRays = -1;
   for (int i = 2; i < argc; ++i) {
     std::string a = argv[i];
     if (a == "--width" && i + 1 < argc) W = atoi(argv[++i]);
@@ -101,6 +112,12 @@ int main(int argc, char** argv) {
     else if (a == "--no-shadows") shadows = false;
     else if (a == "--no-beam") beam = false;
     else if (a == "--image" && i + 1 < argc) image = argv[++i];
+    else if (a == "--mode" && i + 1 < argc) view.render_mode = atoi(argv[++i]);
+    else if (a == "--norm-samples" && i + 1 < argc) view.norm_samples = atoi(argv[++i]);
+    else if (a == "--select" && i + 1 < argc) view.selected_voxel = (uint32_t)strtoul(argv[++i], nullptr, 10);
+    else if (a == "--no-random-colors") view.random_colors = 0;
+    else if (a == "--no-freq-colors") view.freq_colors = 0;
+    else if (a == "--ao" && i + 1 < argc) aoRays = atoi(argv[++i]);
   }
   EncodedOctree* eo = nullptr;
   EncodedSVDAG svdag;
@@ -128,32 +145,43 @@ int main(int argc, char** argv) {
   const svd::aabox3f bb = eo->getSceneBBox();
   p.light_pos[0] = bb[1][0]; p.light_pos[1] = bb[1][1]; p.light_pos[2] = bb[1][2];
   p.shadows = shadows ? 1 : 0;
-  std::vector<svd_ray_result> prim;
   double best = 1e30;
   for (uint32_t f = 0; f < frames; ++f) {
     HeadlessTraceResult r;
-    if (!traceHeadless(*eo, p, r, f + 1 == frames ? &prim : nullptr, beamProjection)) return 1;
+    if (!traceHeadless(*eo, p, r, nullptr, beamProjection)) return 1;
     best = std::min(best, r.kernelMs);
     printf("frame %u: %.3f ms, %lu primary + %lu shadow rays, %.1f Mrays/s (hits %lu)\n", f, r.kernelMs,
            (unsigned long)r.primaryRays, (unsigned long)r.shadowRays, r.mraysPerSec, (unsigned long)r.hits);
   }
   printf("Rays/s = %.1f M (best frame %.3f ms)\n", (double)W * H / (best * 1e-3) / 1e6, best);
-  if (!image.empty() && !prim.empty()) {
-    FILE* fp = fopen(image.c_str(), "wb");
-    if (fp) {
-      fprintf(fp, "P6\n%u %u\n255\n", W, H);
-      for (int y = (int)H - 1; y >= 0; --y)
-        for (uint32_t x = 0; x < W; ++x) {
-          const svd_ray_result& r = prim[(size_t)y * W + x];
-          uint8_t g = 255;
-          if (r.t >= 0) g = (uint8_t)std::max(0.0f, std::min(255.0f, 255.f * (1.f - r.iters / 300.f)));
-          else if (r.t == -3.f) g = 0;
-          const uint8_t px[3] = {g, g, g};
-          fwrite(px, 1, 3, fp);
-        }
-      fclose(fp);
-      printf("* wrote %s\n", image.c_str());
+  if (!image.empty()) {
+    // the framebuffer stores clamp(c, 0, 1) in 8 bits; rows bottom-up in GL
+    std::vector<float> px;
+    double ms = 0;
+    int ch = 3;
+    if (aoRays >= 0) {
+      if (!aoHeadless(*eo, p, (uint32_t)aoRays, px, &ms)) return 1;
+      ch = 1;
+      printf("AO pass (%d rays per pixel): %.3f ms\n", aoRays, ms);
+    } else {
+      if (!renderHeadless(*eo, p, view, beamProjection, px, &ms)) return 1;
+      printf("viewer frame (mode %d, %d normal samples): %.3f ms\n", view.render_mode, view.norm_samples, ms);
     }
+    FILE* fp = fopen(image.c_str(), "wb");
+    if (!fp) { printf("ERROR: cannot write %s\n", image.c_str()); return 1; }
+    fprintf(fp, "P6\n%u %u\n255\n", W, H);
+    for (int y = (int)H - 1; y >= 0; --y)
+      for (uint32_t x = 0; x < W; ++x) {
+        uint8_t out[3];
+        for (int k = 0; k < 3; ++k) {
+          float v = px[((size_t)y * W + x) * ch + (ch == 3 ? k : 0)];
+          v = v == v ? std::min(1.0f, std::max(0.0f, v)) : 0.0f;
+          out[k] = (uint8_t)std::lrint(v * 255.0f);
+        }
+        fwrite(out, 1, 3, fp);
+      }
+    fclose(fp);
+    printf("* wrote %s\n", image.c_str());
   }
   return 0;
 }

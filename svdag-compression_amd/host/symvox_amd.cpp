@@ -565,3 +565,69 @@ bool traceHeadless(const EncodedOctree& eo, const svd_trace_params& p, HeadlessT
   if (primary) *primary = hp;
   return true;
 }
+
+namespace {
+
+// Scene of an encoded file on the default context (ESVDAG traces as SSVDAG).
+svd_scene* headless_scene(const EncodedOctree& eo) {
+  svd_ctx* c = default_context();
+  const std::vector<uint8_t>& f = eo.fileImage();
+  int fmt = eo.format();
+  if (fmt == SVD_FMT_ESVDAG) fmt = SVD_FMT_SSVDAG;
+  svd_scene* sc = nullptr;
+  if (svd_scene_create(c, fmt, f.data(), f.size(), 0, &sc) != SVD_OK) {
+    printf("ERROR: %s\n", svd_last_error(c));
+    return nullptr;
+  }
+  return sc;
+}
+
+}  // namespace
+
+bool renderHeadless(const EncodedOctree& eo, const svd_trace_params& p, const svd_view_params& v,
+                    float beamProjection, std::vector<float>& rgb, double* kernelMs) {
+  svd_ctx* c = default_context();
+  svd_scene* sc = headless_scene(eo);
+  if (!sc) return false;
+  const size_t n = (size_t)p.width * p.height;
+  float* d = nullptr;
+  if (hipMalloc((void**)&d, 3 * n * sizeof(float) + 4) != hipSuccess) {
+    printf("ERROR: device allocation failed\n");
+    svd_scene_free(sc);
+    return false;
+  }
+  const int rc = svd_render(sc, &p, &v, beamProjection, d, nullptr);
+  if (rc == SVD_OK) {
+    rgb.resize(3 * n);
+    (void)hipMemcpy(rgb.data(), d, 3 * n * sizeof(float), hipMemcpyDeviceToHost);
+    if (kernelMs) svd_last_kernel_ms(c, "trace", kernelMs);
+  }
+  (void)hipFree(d);
+  svd_scene_free(sc);
+  if (rc != SVD_OK) { printf("ERROR: %s\n", svd_last_error(c)); return false; }
+  return true;
+}
+
+bool aoHeadless(const EncodedOctree& eo, const svd_trace_params& p, uint32_t numRays, std::vector<float>& vis,
+                double* kernelMs) {
+  svd_ctx* c = default_context();
+  svd_scene* sc = headless_scene(eo);
+  if (!sc) return false;
+  const size_t n = (size_t)p.width * p.height;
+  float* d = nullptr;
+  if (hipMalloc((void**)&d, n * sizeof(float) + 4) != hipSuccess) {
+    printf("ERROR: device allocation failed\n");
+    svd_scene_free(sc);
+    return false;
+  }
+  const int rc = svd_trace_ao(sc, &p, numRays, d, nullptr);
+  if (rc == SVD_OK) {
+    vis.resize(n);
+    (void)hipMemcpy(vis.data(), d, n * sizeof(float), hipMemcpyDeviceToHost);
+    if (kernelMs) svd_last_kernel_ms(c, "trace", kernelMs);
+  }
+  (void)hipFree(d);
+  svd_scene_free(sc);
+  if (rc != SVD_OK) { printf("ERROR: %s\n", svd_last_error(c)); return false; }
+  return true;
+}

@@ -258,3 +258,10 @@ struct HeadlessTraceResult {
 // factor (svd_trace_beam); 0: every primary ray starts at the camera.
 bool traceHeadless(const EncodedOctree& eo, const svd_trace_params& params, HeadlessTraceResult& out,
                    std::vector<svd_ray_result>* primary = nullptr, float beamProjection = 0.f);
+// The viewer's colour per pixel (3 floats, VIEWER_MODE main) with its render
+// mode, normal samples and colour switches in `view` (svd_render).
+bool renderHeadless(const EncodedOctree& eo, const svd_trace_params& params, const svd_view_params& view,
+                    float beamProjection, std::vector<float>& rgb, double* kernelMs = nullptr);
+// AO_MODE visibility per pixel over the primary rays' hits (svd_trace_ao).
+bool aoHeadless(const EncodedOctree& eo, const svd_trace_params& params, uint32_t numRays, std::vector<float>& vis,
+                double* kernelMs = nullptr);

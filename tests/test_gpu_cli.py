@@ -115,6 +115,50 @@ def test_svviewer_headless(mesh):
         assert os.path.getsize(d / f"img{ext}.ppm") > 320 * 240 * 3
 
 
+def read_ppm(path):
+    data = open(path, "rb").read()
+    head, w, h, mx = data.split(maxsplit=3)[:4]
+    assert head == b"P6" and mx.startswith(b"255")
+    w, h = int(w), int(h)
+    return np.frombuffer(data[-w * h * 3:], np.uint8).reshape(h, w, 3)
+
+
+def viewer_params(buf, w, h, seed):
+    """svviewer's frame: orbit camera around the file's box, renderer defaults, light at the box max."""
+    bb = np.frombuffer(buf[:24], np.float32).astype(np.float64)
+    levels = int(np.frombuffer(buf[28:32], np.uint32)[0])
+    center = tuple((bb[:3] + bb[3:]) / 2)
+    radius = 1.5 * float(np.sqrt(((bb[3:] - bb[:3]) ** 2).sum())) / 2.0
+    p = W.trace_params(w, h, seed=seed, levels=levels, shadows=True, light=tuple(float(x) for x in bb[3:].astype(np.float32)),
+                       center=center, radius=radius)
+    return p
+
+
+def test_svviewer_image_is_the_viewer_frame(mesh):
+    """svviewer --image: the viewer's colour (pretty mode, normal samples,
+    shadows, beam pre-pass on as in the reference) equals the oracle's frame
+    after the framebuffer's clamp to [0, 1] and 8-bit rounding; with --ao the
+    image is the AO pass's visibility."""
+    d, *_ = mesh
+    run([os.path.join(BIN, "svbuilder"), "spheres.obj", "6", "0"], d)
+    w, h = 160, 120
+    for ext, fmt in ((".svdag", S.FMT_SVDAG), (".ssvdag", S.FMT_SSVDAG)):
+        buf = open(d / f"spheres_6{ext}", "rb").read()
+        p = viewer_params(buf, w, h, seed=1)
+        run([os.path.join(BIN, "svviewer"), f"spheres_6{ext}", "--width", str(w), "--height", str(h), "--frames", "1",
+             "--mode", "3", "--norm-samples", "2", "--image", f"view{ext}.ppm"], d)
+        rgb, prim = O.render_frame(buf, fmt, p, S.ViewParams.defaults(render_mode=3, norm_samples=2),
+                                   W.projection_factor(h, screen_divisor=8.0))
+        assert (prim["t"] >= 0).mean() > 0.1
+        q = np.rint(np.clip(np.nan_to_num(rgb, nan=0.0), 0, 1) * 255).astype(np.uint8)[::-1]
+        np.testing.assert_array_equal(read_ppm(d / f"view{ext}.ppm"), q, err_msg=ext)
+        run([os.path.join(BIN, "svviewer"), f"spheres_6{ext}", "--width", str(w), "--height", str(h), "--frames", "1",
+             "--ao", "8", "--image", f"ao{ext}.ppm"], d)
+        vis, _ = O.trace_ao(buf, fmt, p, 8)
+        q = np.rint(np.clip(vis, 0, 1) * 255).astype(np.uint8)[::-1]
+        np.testing.assert_array_equal(read_ppm(d / f"ao{ext}.ppm"), np.repeat(q[..., None], 3, axis=2), err_msg=ext)
+
+
 def test_svbuilder_svdag_input(mesh, tmp_path):
     """svbuilder model.svdag L 0 (main.cpp:97-101, 147): the decoded DAG skips
     the build and goes through the same passes as a mesh input; with -c the
