@@ -441,6 +441,25 @@ def main():
             "mean_primary_iters_no_prepass": round(float(pr["iters"].mean()), 3),
             "same_hit_t_frac": round(float((pb["t"] == pr["t"]).mean()), 6)}
     log(f"[bench] beam: {beam}")
+
+    # the viewer's shaded frame (pretty mode, 4 normal samples, shadows, beam pre-pass) and its AO pass
+    rgb = torch.empty(nray * 3, dtype=torch.float32, device="cuda")
+    vis = torch.empty(nray, dtype=torch.float32, device="cuda")
+    vp = S.ViewParams.defaults(render_mode=3, norm_samples=4)
+    trm, tao = [], []
+    for _ in range(1 + max(args.steps, 3)):
+        sc.render_device(params, vp, rgb.data_ptr(), dpf)
+        ctx.sync()
+        trm.append(ctx.kernel_ms("trace"))
+        sc.ao_device(params, 16, vis.data_ptr())
+        ctx.sync()
+        tao.append(ctx.kernel_ms("trace"))
+    hits_b = int((pb["t"] >= 0).sum())
+    render_ms, ao_ms = float(np.median(trm[1:])), float(np.median(tao[1:]))
+    viewer = {"frame": f"{args.width}x{args.height} pretty mode, 4 normal samples, shadows, 1/8 depth pre-pass",
+              "ms": round(render_ms, 4), "fps": round(1e3 / render_ms, 1),
+              "rays_upper_bound": nray + hits_b * 5, "ao_rays_per_pixel": 16, "ao_ms": round(ao_ms, 4)}
+    log(f"[bench] viewer: {viewer}")
     sc.free()
     g.free()
 
@@ -465,6 +484,7 @@ def main():
         "traversal_mrays_s": round(mrays, 2),
         "trace_kernel_ms": round(trace_kernel_ms, 4),
         "traversal_beam": beam,
+        "viewer": viewer,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": measured_traffic(L, args),
                      "kernel": "k_trace<SSVDAG>", "algorithmic_bytes": bytes_fetched},

@@ -20,10 +20,31 @@ __device__ __forceinline__ bool head64(const uint64_t* code, uint64_t i) {
 }
 
 // Input contract: strictly ascending codes below 2^(3*levels).
-__global__ void k_check_codes(const uint64_t* __restrict__ code, uint64_t n, uint64_t limit, uint32_t* __restrict__ bad) {
-  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  if (code[i] >= limit || (i > 0 && code[i] <= code[i - 1])) atomicAdd(bad, 1u);
+// The input check and every level's node count in one pass.  Code i starts
+// a node of the level k steps above the voxels (prefix code >> 3k) exactly
+// when k <= K_i = floor(msb(code[i] ^ code[i-1]) / 3) (K_0 = L), so the
+// level k steps up has #{i : K_i >= k} nodes.  hist[0] = bad codes,
+// hist[1 + K] = #{i : K_i = K}.
+__global__ void k_check_and_count(const uint64_t* __restrict__ code, uint64_t n, uint32_t L,
+                                  uint32_t* __restrict__ hist) {
+  __shared__ uint32_t h[kMaxLevels + 2];
+  if (threadIdx.x < kMaxLevels + 2) h[threadIdx.x] = 0;
+  __syncthreads();
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    const uint64_t v = code[i];
+    uint32_t K = L;
+    bool bad = L < 22 && (v >> (3 * L)) != 0;
+    if (i > 0) {
+      const uint64_t u = code[i - 1];
+      if (v <= u) bad = true;
+      else K = min(L, (uint32_t)(63 - __clzll((long long)(v ^ u))) / 3u);
+    }
+    if (bad) atomicAdd(&h[0], 1u);
+    else atomicAdd(&h[1 + K], 1u);
+  }
+  __syncthreads();
+  if (threadIdx.x < kMaxLevels + 2 && h[threadIdx.x]) atomicAdd(&hist[threadIdx.x], h[threadIdx.x]);
 }
 
 __global__ void k_heads(const uint64_t* __restrict__ code, uint64_t n, uint32_t* __restrict__ head) {
@@ -357,14 +378,32 @@ __global__ void k_state_flags(const uint32_t* __restrict__ state, uint64_t n, ui
 // representative's identity key.  Representatives are compacted in order.
 __global__ void k_sdag_outcome(const uint32_t* __restrict__ key, uint64_t n, const uint32_t* __restrict__ table,
                                uint32_t cmask, const uint32_t* __restrict__ slot, const uint32_t* __restrict__ state,
-                               const uint32_t* __restrict__ pos, const uint8_t* __restrict__ childInv,
-                               const uint8_t* __restrict__ invIn, uint32_t* __restrict__ map,
-                               uint32_t* __restrict__ out, uint8_t* __restrict__ invOut, uint32_t* __restrict__ err) {
+                               const uint32_t* __restrict__ pos, const uint32_t* __restrict__ cand,
+                               const uint8_t* __restrict__ childInv, const uint8_t* __restrict__ invIn,
+                               uint32_t* __restrict__ map, uint32_t* __restrict__ out, uint8_t* __restrict__ invOut,
+                               uint32_t* __restrict__ err) {
   uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
+  const uint32_t st = state[i];
+  if (st != kRep && table[slot[i]] == (uint32_t)i) {
+    // the first node of its key: k_sdag_cands already looked up transforms
+    // 1..7 and kept the earlier nodes found; the identity finds itself
+    const uint32_t* cd = cand + i * 7;
+    for (int t = 1; t < 8; ++t) {
+      const uint32_t r = cd[t - 1];
+      if (r != kNone && state[r] == kRep) {
+        const uint32_t f = kFlip[t];
+        map[i] = pos[r] | (((f >> 2) & 1u) << 29) | (((f >> 1) & 1u) << 30) | ((f & 1u) << 31);
+        return;
+      }
+    }
+    map[i] = kNone;
+    atomicAdd(err, 1u);
+    return;
+  }
   uint32_t k0[kRecWords];
   load_rec(key, i, k0);
-  if (state[i] == kRep) {
+  if (st == kRep) {
     const uint32_t p = pos[i];
     map[i] = p;
     uint32_t* o = out + (uint64_t)p * kRecWords;
@@ -450,14 +489,22 @@ void build_svo_from_morton(svd_octree* o, const uint64_t* dcodes, uint64_t n) {
     SVD_HIP(hipStreamSynchronize(c->stream));
     return;
   }
-  uint32_t* total = dalloc<uint32_t>(c, 1);
-  fill_u32(c, total, 0u, 1);
-  k_check_codes<<<div_up(n, kB), kB, 0, c->stream>>>(dcodes, n, 1ull << (3 * L), total);
-  check_launch("k_check_codes");
-  const uint64_t bad = read_u32(c, total);
-  dfree(c, total);
-  if (bad) throw Error{SVD_ERR_ARG, "Morton codes must be strictly ascending and below 2^(3*levels)"};
-  const uint64_t nNodes = svo_levels_from_codes(c, dcodes, n, L, o->svo, nullptr);
+  uint32_t* hist = dalloc<uint32_t>(c, kMaxLevels + 2);
+  fill_u32(c, hist, 0u, kMaxLevels + 2);
+  k_check_and_count<<<div_up(n, kB), kB, 0, c->stream>>>(dcodes, n, L, hist);
+  check_launch("k_check_and_count");
+  uint32_t h[kMaxLevels + 2];
+  read_u32s(c, hist, h, L + 2);
+  dfree(c, hist);
+  if (h[0]) throw Error{SVD_ERR_ARG, "Morton codes must be strictly ascending and below 2^(3*levels)"};
+  // level lev is L - lev steps above the voxels: #{i : K_i >= L - lev}
+  std::vector<uint64_t> sizes(L);
+  uint64_t atLeast = 0;
+  for (uint32_t k = L; k >= 1; --k) {
+    atLeast += h[1 + k];
+    sizes[L - k] = atLeast;
+  }
+  const uint64_t nNodes = svo_levels_from_codes(c, dcodes, n, L, o->svo, nullptr, sizes.data());
   o->nNodesSVO = nNodes;   // non-root nodes created (GeomOctree::_nNodes after buildSVO)
   o->state = SVD_S_SVO;
 }
@@ -467,7 +514,7 @@ void build_svo_from_morton(svd_octree* o, const uint64_t* dcodes, uint64_t n) {
 // per level, a device copy of every node's code (code >> 3*(L-lev)).
 // Returns the number of non-root nodes.
 uint64_t svo_levels_from_codes(svd_ctx* c, const uint64_t* dcodes, uint64_t n, uint32_t L, std::vector<Level>& svo,
-                               std::vector<uint64_t*>* levelCodes) {
+                               std::vector<uint64_t*>* levelCodes, const uint64_t* sizes) {
   svo.assign(L, Level());
   if (levelCodes) levelCodes->assign(L, nullptr);
   uint32_t* scanbuf = dalloc<uint32_t>(c, n + 1);
@@ -482,7 +529,7 @@ uint64_t svo_levels_from_codes(svd_ctx* c, const uint64_t* dcodes, uint64_t n, u
   // finest level
   k_heads<<<div_up(n, kB), kB, 0, c->stream>>>(dcodes, n, scanbuf);
   scan_exclusive(c, scanbuf, scanbuf, n, total);
-  uint64_t nl = read_u32(c, total);
+  uint64_t nl = sizes ? sizes[L - 1] : read_u32(c, total);
   Level& leaf = svo[L - 1];
   leaf.leaf = true;
   leaf.n = nl;
@@ -495,7 +542,7 @@ uint64_t svo_levels_from_codes(svd_ctx* c, const uint64_t* dcodes, uint64_t n, u
   for (int lev = (int)L - 2; lev >= 0; --lev) {
     k_heads<<<div_up(nc, kB), kB, 0, c->stream>>>(codeA, nc, scanbuf);
     scan_exclusive(c, scanbuf, scanbuf, nc, total);
-    uint64_t np = read_u32(c, total);
+    uint64_t np = sizes ? sizes[lev] : read_u32(c, total);
     Level& lv = svo[lev];
     lv.n = np;
     lv.rec = dalloc<uint32_t>(c, np * kRecWords);
@@ -516,23 +563,31 @@ void remap_children(svd_ctx* c, const uint32_t* in, uint64_t n, const uint32_t* 
 }
 
 // Generic dedup of one inner level's records -> remap + compacted records.
-uint64_t dedup_level(svd_ctx* c, const uint32_t* rec, uint64_t n, uint32_t* remap, uint32_t** outRec) {
+// The unique count goes to the device word *total; the output holds n
+// records (the upper bound), so no host read is needed.
+void dedup_level_dev(svd_ctx* c, const uint32_t* rec, uint64_t n, uint32_t* remap, uint32_t** outRec,
+                     uint32_t* total) {
   const uint32_t cap = table_cap(n);
   uint32_t* table = dalloc<uint32_t>(c, cap);
   uint32_t* slot = dalloc<uint32_t>(c, n);
   uint32_t* pos = dalloc<uint32_t>(c, n + 1);
-  uint32_t* total = dalloc<uint32_t>(c, 1);
   fill_u32(c, table, kNone, cap);
   k_insert<<<div_up(n, kB), kB, 0, c->stream>>>(rec, n, table, cap - 1, slot);
   k_rep_flags<<<div_up(n, kB), kB, 0, c->stream>>>(slot, table, n, pos);
   check_launch("k_insert");
   scan_exclusive(c, pos, pos, n, total);
-  const uint64_t d = read_u32(c, total);
-  uint32_t* out = dalloc<uint32_t>(c, (d ? d : 1) * kRecWords);
+  uint32_t* out = dalloc<uint32_t>(c, (n ? n : 1) * kRecWords);
   k_compact<<<div_up(n, kB), kB, 0, c->stream>>>(rec, slot, table, pos, n, out, remap);
   check_launch("k_compact");
-  dfree(c, table); dfree(c, slot); dfree(c, pos); dfree(c, total);
+  dfree(c, table); dfree(c, slot); dfree(c, pos);
   *outRec = out;
+}
+
+uint64_t dedup_level(svd_ctx* c, const uint32_t* rec, uint64_t n, uint32_t* remap, uint32_t** outRec) {
+  uint32_t* total = dalloc<uint32_t>(c, 1);
+  dedup_level_dev(c, rec, n, remap, outRec, total);
+  const uint64_t d = read_u32(c, total);
+  dfree(c, total);
   return d;
 }
 
@@ -547,13 +602,14 @@ uint64_t dedup_all(svd_ctx* c, uint32_t L, const std::vector<Level>& src, std::v
     return 1;
   }
   uint64_t nNodes = 1;
+  // unique counts of every reduced level, read back once at the end
+  uint32_t* counts = dalloc<uint32_t>(c, L);
   // finest level: dedup of 8-bit masks by first occurrence
   const Level& sl = src[L - 1];
   uint32_t* remap = dalloc<uint32_t>(c, sl.n + 1);
   {
     uint32_t* firstocc = dalloc<uint32_t>(c, 256);
     uint32_t* idOfMask = dalloc<uint32_t>(c, 256);
-    uint32_t* count = dalloc<uint32_t>(c, 1);
     Level& dl = dst[L - 1];
     dl.leaf = true;
     dl.rec = dalloc<uint32_t>(c, 256);
@@ -562,12 +618,10 @@ uint64_t dedup_all(svd_ctx* c, uint32_t L, const std::vector<Level>& src, std::v
       const uint64_t nb = div_up(sl.n, 256) < (uint64_t)kFoBlocks ? div_up(sl.n, 256) : (uint64_t)kFoBlocks;
       k_leaf_firstocc<<<nb, 256, 0, c->stream>>>(sl.rec, sl.n, firstocc);
     }
-    k_leaf_rank<<<1, 256, 0, c->stream>>>(firstocc, idOfMask, dl.rec, count);
+    k_leaf_rank<<<1, 256, 0, c->stream>>>(firstocc, idOfMask, dl.rec, counts + (L - 1));
     if (sl.n) k_leaf_remap<<<div_up(sl.n, kB), kB, 0, c->stream>>>(sl.rec, sl.n, idOfMask, remap);
     check_launch("leaf dedup");
-    dl.n = read_u32(c, count);
-    nNodes += dl.n;
-    dfree(c, firstocc); dfree(c, idOfMask); dfree(c, count);
+    dfree(c, firstocc); dfree(c, idOfMask);
   }
   for (int lev = (int)L - 2; lev >= 0; --lev) {
     const Level& s = src[lev];
@@ -581,14 +635,20 @@ uint64_t dedup_all(svd_ctx* c, uint32_t L, const std::vector<Level>& src, std::v
       d.n = s.n;
     } else {
       uint32_t* nremap = dalloc<uint32_t>(c, s.n + 1);
-      d.n = dedup_level(c, keyed, s.n, nremap, &d.rec);
+      dedup_level_dev(c, keyed, s.n, nremap, &d.rec, counts + lev);
       dfree(c, keyed);
       dfree(c, remap);
       remap = nremap;
-      nNodes += d.n;
     }
   }
   dfree(c, remap);
+  std::vector<uint32_t> h(L);
+  read_u32s(c, counts, h.data(), L);
+  dfree(c, counts);
+  for (uint32_t lev = 1; lev < L; ++lev) {
+    dst[lev].n = h[lev];
+    nNodes += h[lev];
+  }
   return nNodes;
 }
 
@@ -613,6 +673,8 @@ void reduce_sym(svd_octree* o) {
   uint64_t nNodes = 0;
   uint32_t* count = dalloc<uint32_t>(c, 1);
   uint32_t* err = dalloc<uint32_t>(c, 1);
+  // representative counts of every level, read back once at the end
+  uint32_t* counts = dalloc<uint32_t>(c, L);
   fill_u32(c, err, 0u, 1);
   {
     const Level& dl = o->dag[L - 1];
@@ -621,10 +683,9 @@ void reduce_sym(svd_octree* o) {
     sl.rec = dalloc<uint32_t>(c, 256);
     sl.inv = dalloc<uint8_t>(c, 256);
     o->sdagMap[L - 1] = dalloc<uint32_t>(c, dl.n + 1);
-    k_sdag_leaves<<<1, 256, 0, c->stream>>>(dl.rec, (uint32_t)dl.n, o->sdagMap[L - 1], sl.rec, sl.inv, count);
+    k_sdag_leaves<<<1, 256, 0, c->stream>>>(dl.rec, (uint32_t)dl.n, o->sdagMap[L - 1], sl.rec, sl.inv,
+                                            counts + (L - 1));
     check_launch("k_sdag_leaves");
-    sl.n = read_u32(c, count);
-    nNodes += sl.n;
   }
   for (int lev = (int)L - 2; lev >= 0; --lev) {
     const Level& d = o->dag[lev];
@@ -662,19 +723,25 @@ void reduce_sym(svd_octree* o) {
       if (round > (int)n + 2) throw Error{SVD_ERR_DEVICE, "toSDAG: resolution did not converge"};
     }
     k_state_flags<<<div_up(n, kB), kB, 0, c->stream>>>(state, n, pos);
-    scan_exclusive(c, pos, pos, n, count);
-    s.n = read_u32(c, count);
-    s.rec = dalloc<uint32_t>(c, (s.n ? s.n : 1) * kRecWords);
-    s.inv = dalloc<uint8_t>(c, s.n + 1);
+    scan_exclusive(c, pos, pos, n, counts + lev);
+    // n records: the upper bound of the representative count, known on the host
+    s.rec = dalloc<uint32_t>(c, (n ? n : 1) * kRecWords);
+    s.inv = dalloc<uint8_t>(c, n + 1);
     o->sdagMap[lev] = dalloc<uint32_t>(c, n + 1);
-    k_sdag_outcome<<<div_up(n, kB), kB, 0, c->stream>>>(key, n, table, cap - 1, slot, state, pos, childInv, inv,
-                                                         o->sdagMap[lev], s.rec, s.inv, err);
+    k_sdag_outcome<<<div_up(n, kB), kB, 0, c->stream>>>(key, n, table, cap - 1, slot, state, pos, cand, childInv,
+                                                         inv, o->sdagMap[lev], s.rec, s.inv, err);
     check_launch("k_sdag_outcome");
-    nNodes += s.n;
     dfree(c, key); dfree(c, inv); dfree(c, table); dfree(c, slot); dfree(c, cand); dfree(c, state); dfree(c, pos);
   }
-  const uint64_t nerr = read_u32(c, err);
-  dfree(c, count); dfree(c, err);
+  std::vector<uint32_t> h(L + 1);
+  read_u32s(c, counts, h.data(), L);
+  read_u32s(c, err, h.data() + L, 1);
+  const uint64_t nerr = h[L];
+  for (uint32_t lev = 1; lev < L; ++lev) {
+    o->sdag[lev].n = h[lev];
+    nNodes += h[lev];
+  }
+  dfree(c, count); dfree(c, err); dfree(c, counts);
   if (nerr) throw Error{SVD_ERR_DEVICE, "toSDAG: node without correspondence"};
   o->nNodesSDAG = nNodes;
   o->state = SVD_S_SDAG;

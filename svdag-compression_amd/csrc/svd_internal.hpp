@@ -168,8 +168,9 @@ void build_svo_from_mesh(svd_octree* o, const float* dtris, uint64_t ntris, cons
 // buildDAG's joined SVO: top of stepLevel+1 levels + one voxelized subtree per set voxel of its finest level
 void build_svo_from_mesh_subtrees(svd_octree* o, const float* dtris, uint64_t ntris, const double bmin[3],
                                   const double bmax[3], uint32_t stepLevel);
+// sizes (optional, host): the node count of every level, known in advance
 uint64_t svo_levels_from_codes(svd_ctx* c, const uint64_t* dcodes, uint64_t n, uint32_t L, std::vector<Level>& svo,
-                               std::vector<uint64_t*>* levelCodes);
+                               std::vector<uint64_t*>* levelCodes, const uint64_t* sizes = nullptr);
 // one inner level: exact-key dedup, first occurrence wins -> remap + compacted records
 uint64_t dedup_level(svd_ctx* c, const uint32_t* rec, uint64_t n, uint32_t* remap, uint32_t** outRec);
 void remap_children(svd_ctx* c, const uint32_t* in, uint64_t n, const uint32_t* remap, uint32_t* out);
