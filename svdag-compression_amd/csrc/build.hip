@@ -24,27 +24,45 @@ __device__ __forceinline__ bool head64(const uint64_t* code, uint64_t i) {
 // a node of the level k steps above the voxels (prefix code >> 3k) exactly
 // when k <= K_i = floor(msb(code[i] ^ code[i-1]) / 3) (K_0 = L), so the
 // level k steps up has #{i : K_i >= k} nodes.  hist[0] = bad codes,
-// hist[1 + K] = #{i : K_i = K}.
-__global__ void k_check_and_count(const uint64_t* __restrict__ code, uint64_t n, uint32_t L,
-                                  uint32_t* __restrict__ hist) {
-  __shared__ uint32_t h[kMaxLevels + 2];
-  if (threadIdx.x < kMaxLevels + 2) h[threadIdx.x] = 0;
-  __syncthreads();
-  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) {
+// hist[k] = #{i : K_i >= k} (1 <= k <= L).  Counts stay in registers over a
+// grid-stride loop and are reduced per workgroup: one global atomic per
+// workgroup and counter.
+constexpr int kCountBlocks = 1024;
+__global__ void __launch_bounds__(256) k_check_and_count(const uint64_t* __restrict__ code, uint64_t n, uint32_t L,
+                                                         uint32_t* __restrict__ hist) {
+  uint32_t cnt[kMaxLevels + 1];
+#pragma unroll
+  for (int k = 0; k <= (int)kMaxLevels; ++k) cnt[k] = 0;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
     const uint64_t v = code[i];
     uint32_t K = L;
-    bool bad = L < 22 && (v >> (3 * L)) != 0;
+    bool bad = (v >> (3 * L)) != 0;   // L <= 21: the shift is at most 63
     if (i > 0) {
       const uint64_t u = code[i - 1];
       if (v <= u) bad = true;
       else K = min(L, (uint32_t)(63 - __clzll((long long)(v ^ u))) / 3u);
     }
-    if (bad) atomicAdd(&h[0], 1u);
-    else atomicAdd(&h[1 + K], 1u);
+    cnt[0] += bad ? 1u : 0u;
+    if (!bad) {
+#pragma unroll
+      for (int k = 1; k <= (int)kMaxLevels; ++k) cnt[k] += K >= (uint32_t)k ? 1u : 0u;
+    }
+  }
+  __shared__ uint32_t part[kMaxLevels + 1][4];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k <= (int)kMaxLevels; ++k) {
+    uint32_t x = cnt[k];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off, 64);
+    if (lane == 0) part[k][wave] = x;
   }
   __syncthreads();
-  if (threadIdx.x < kMaxLevels + 2 && h[threadIdx.x]) atomicAdd(&hist[threadIdx.x], h[threadIdx.x]);
+  if (threadIdx.x <= kMaxLevels) {
+    const uint32_t x = part[threadIdx.x][0] + part[threadIdx.x][1] + part[threadIdx.x][2] + part[threadIdx.x][3];
+    if (x) atomicAdd(&hist[threadIdx.x], x);
+  }
 }
 
 __global__ void k_heads(const uint64_t* __restrict__ code, uint64_t n, uint32_t* __restrict__ head) {
@@ -101,28 +119,38 @@ __device__ __forceinline__ bool rec_eq(const uint32_t* __restrict__ rec, uint32_
   return true;
 }
 
+// Open-addressing table of node indices.  An entry is (fingerprint << 32 |
+// index), the fingerprint being the key hash's upper 32 bits, published by one
+// 64-bit CAS: a probe compares fingerprints in registers and reads the 36-byte
+// key only when they agree.  Empty slots are all ones (two kNone words).
+using TEntry = unsigned long long;
+constexpr TEntry kEmpty = ~0ull;
+
 // Insert node i; the table slot ends holding the smallest index of all nodes
-// with an equal key (atomicMin).  Empty nodes (mask 0) are skipped, like the
-// reference's `if (!n.hasChildren()) continue;`.
-__global__ void k_insert(const uint32_t* __restrict__ rec, uint64_t n, uint32_t* __restrict__ table, uint32_t cmask,
+// with an equal key (atomicMin: equal keys share the fingerprint).  Empty
+// nodes (mask 0) are skipped, like the reference's `if (!n.hasChildren()) continue;`.
+__global__ void k_insert(const uint32_t* __restrict__ rec, uint64_t n, TEntry* __restrict__ table, uint32_t cmask,
                          uint32_t* __restrict__ slot) {
   uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   uint32_t w[kRecWords];
   load_rec(rec, i, w);
   if ((w[0] & 0xFFu) == 0) { slot[i] = kNone; return; }
-  uint32_t h = (uint32_t)hash_rec(w) & cmask;
+  const uint64_t hv = hash_rec(w);
+  const uint32_t fp = (uint32_t)(hv >> 32);
+  const TEntry mine = ((TEntry)fp << 32) | (TEntry)(uint32_t)i;
+  uint32_t h = (uint32_t)hv & cmask;
   while (true) {
-    uint32_t v = __atomic_load_n(&table[h], __ATOMIC_RELAXED);
-    if (v == kNone) {
-      uint32_t old = atomicCAS(&table[h], kNone, (uint32_t)i);
-      if (old == kNone) { slot[i] = h; return; }
-      v = old;
+    TEntry e = __atomic_load_n(&table[h], __ATOMIC_RELAXED);
+    if (e == kEmpty) {
+      const TEntry old = atomicCAS(&table[h], kEmpty, mine);
+      if (old == kEmpty) { slot[i] = h; return; }
+      e = old;
     }
-    if (rec_eq(rec, v, w)) {
+    if ((uint32_t)(e >> 32) == fp && rec_eq(rec, (uint32_t)e, w)) {
       // the slot only decreases, so a reader above the value it saw has
       // nothing to add: hot keys cost one atomic per decrease, not per node
-      if ((uint32_t)i < v) atomicMin(&table[h], (uint32_t)i);
+      if (mine < e) atomicMin(&table[h], mine);
       slot[i] = h;
       return;
     }
@@ -130,28 +158,32 @@ __global__ void k_insert(const uint32_t* __restrict__ rec, uint64_t n, uint32_t*
   }
 }
 
+__device__ __forceinline__ uint32_t entry_index(TEntry e) { return (uint32_t)e; }
+
 // Probe for a key without inserting; returns the stored (minimum) index.
-__device__ __forceinline__ uint32_t lookup(const uint32_t* __restrict__ rec, const uint32_t* __restrict__ table,
+__device__ __forceinline__ uint32_t lookup(const uint32_t* __restrict__ rec, const TEntry* __restrict__ table,
                                            uint32_t cmask, const uint32_t w[kRecWords]) {
-  uint32_t h = (uint32_t)hash_rec(w) & cmask;
+  const uint64_t hv = hash_rec(w);
+  const uint32_t fp = (uint32_t)(hv >> 32);
+  uint32_t h = (uint32_t)hv & cmask;
   while (true) {
-    uint32_t v = table[h];
-    if (v == kNone) return kNone;
-    if (rec_eq(rec, v, w)) return v;
+    const TEntry e = table[h];
+    if (e == kEmpty) return kNone;
+    if ((uint32_t)(e >> 32) == fp && rec_eq(rec, (uint32_t)e, w)) return (uint32_t)e;
     h = (h + 1) & cmask;
   }
 }
 
-__global__ void k_rep_flags(const uint32_t* __restrict__ slot, const uint32_t* __restrict__ table, uint64_t n,
+__global__ void k_rep_flags(const uint32_t* __restrict__ slot, const TEntry* __restrict__ table, uint64_t n,
                             uint32_t* __restrict__ flag) {
   uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const uint32_t s = slot[i];
-  flag[i] = (s != kNone && table[s] == (uint32_t)i) ? 1u : 0u;
+  flag[i] = (s != kNone && entry_index(table[s]) == (uint32_t)i) ? 1u : 0u;
 }
 
 __global__ void k_compact(const uint32_t* __restrict__ rec, const uint32_t* __restrict__ slot,
-                          const uint32_t* __restrict__ table, const uint32_t* __restrict__ pos, uint64_t n,
+                          const TEntry* __restrict__ table, const uint32_t* __restrict__ pos, uint64_t n,
                           uint32_t* __restrict__ out, uint32_t* __restrict__ remap) {
   uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -159,7 +191,7 @@ __global__ void k_compact(const uint32_t* __restrict__ rec, const uint32_t* __re
   // an empty (skipped) node keeps correspondence 0, like the reference's
   // zero-initialised correspondences vector (geom_octree.cpp:489-495)
   if (s == kNone) { remap[i] = 0; return; }
-  const uint32_t r = table[s];
+  const uint32_t r = entry_index(table[s]);
   remap[i] = pos[r];
   if (r == (uint32_t)i) {
     const uint32_t* src = rec + i * kRecWords;
@@ -322,12 +354,12 @@ enum : uint32_t { kUndecided = 0, kRep = 1, kNonRep = 2 };
 // First-occurrence nodes look up their 7 mirrored variants; an earlier node
 // carrying that key is a candidate that, if it is a representative, takes
 // this node.  Later duplicates of a key are never representatives.
-__global__ void k_sdag_cands(const uint32_t* __restrict__ key, uint64_t n, const uint32_t* __restrict__ table,
+__global__ void k_sdag_cands(const uint32_t* __restrict__ key, uint64_t n, const TEntry* __restrict__ table,
                              uint32_t cmask, const uint32_t* __restrict__ slot, const uint8_t* __restrict__ childInv,
                              uint32_t* __restrict__ cand, uint32_t* __restrict__ state) {
   uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const bool first = table[slot[i]] == (uint32_t)i;
+  const bool first = entry_index(table[slot[i]]) == (uint32_t)i;
   uint32_t* cd = cand + i * 7;
   if (!first) {
 #pragma unroll
@@ -340,7 +372,11 @@ __global__ void k_sdag_cands(const uint32_t* __restrict__ key, uint64_t n, const
   bool any = false;
   for (int t = 1; t < 8; ++t) {
     make_variant(k0, t, childInv, v);
-    uint32_t r = lookup(key, table, cmask, v);
+    // a variant equal to the node's own key finds the node itself
+    bool self = true;
+#pragma unroll
+    for (int k = 0; k < kRecWords; ++k) self = self && v[k] == k0[k];
+    const uint32_t r = self ? (uint32_t)i : lookup(key, table, cmask, v);
     if (r != kNone && r < (uint32_t)i) { cd[t - 1] = r; any = true; }
     else cd[t - 1] = kNone;
   }
@@ -376,7 +412,7 @@ __global__ void k_state_flags(const uint32_t* __restrict__ state, uint64_t n, ui
 // Correspondence of every DAG node: itself if a representative, else the
 // first transform (in reference order) whose key is an earlier
 // representative's identity key.  Representatives are compacted in order.
-__global__ void k_sdag_outcome(const uint32_t* __restrict__ key, uint64_t n, const uint32_t* __restrict__ table,
+__global__ void k_sdag_outcome(const uint32_t* __restrict__ key, uint64_t n, const TEntry* __restrict__ table,
                                uint32_t cmask, const uint32_t* __restrict__ slot, const uint32_t* __restrict__ state,
                                const uint32_t* __restrict__ pos, const uint32_t* __restrict__ cand,
                                const uint8_t* __restrict__ childInv, const uint8_t* __restrict__ invIn,
@@ -385,7 +421,7 @@ __global__ void k_sdag_outcome(const uint32_t* __restrict__ key, uint64_t n, con
   uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const uint32_t st = state[i];
-  if (st != kRep && table[slot[i]] == (uint32_t)i) {
+  if (st != kRep && entry_index(table[slot[i]]) == (uint32_t)i) {
     // the first node of its key: k_sdag_cands already looked up transforms
     // 1..7 and kept the earlier nodes found; the identity finds itself
     const uint32_t* cd = cand + i * 7;
@@ -415,7 +451,7 @@ __global__ void k_sdag_outcome(const uint32_t* __restrict__ key, uint64_t n, con
   uint32_t v[kRecWords];
   for (int t = 0; t < 8; ++t) {
     uint32_t r;
-    if (t == 0) r = table[slot[i]];
+    if (t == 0) r = entry_index(table[slot[i]]);
     else { make_variant(k0, t, childInv, v); r = lookup(key, table, cmask, v); }
     if (r != kNone && r < (uint32_t)i && state[r] == kRep) {
       const uint32_t f = kFlip[t];
@@ -489,21 +525,18 @@ void build_svo_from_morton(svd_octree* o, const uint64_t* dcodes, uint64_t n) {
     SVD_HIP(hipStreamSynchronize(c->stream));
     return;
   }
-  uint32_t* hist = dalloc<uint32_t>(c, kMaxLevels + 2);
-  fill_u32(c, hist, 0u, kMaxLevels + 2);
-  k_check_and_count<<<div_up(n, kB), kB, 0, c->stream>>>(dcodes, n, L, hist);
+  uint32_t* hist = dalloc<uint32_t>(c, kMaxLevels + 1);
+  fill_u32(c, hist, 0u, kMaxLevels + 1);
+  const uint32_t nb = std::min<uint64_t>(div_up(n, kB), (uint64_t)kCountBlocks);
+  k_check_and_count<<<nb, kB, 0, c->stream>>>(dcodes, n, L, hist);
   check_launch("k_check_and_count");
-  uint32_t h[kMaxLevels + 2];
-  read_u32s(c, hist, h, L + 2);
+  uint32_t h[kMaxLevels + 1];
+  read_u32s(c, hist, h, L + 1);
   dfree(c, hist);
   if (h[0]) throw Error{SVD_ERR_ARG, "Morton codes must be strictly ascending and below 2^(3*levels)"};
-  // level lev is L - lev steps above the voxels: #{i : K_i >= L - lev}
+  // level lev is L - lev steps above the voxels
   std::vector<uint64_t> sizes(L);
-  uint64_t atLeast = 0;
-  for (uint32_t k = L; k >= 1; --k) {
-    atLeast += h[1 + k];
-    sizes[L - k] = atLeast;
-  }
+  for (uint32_t lev = 0; lev < L; ++lev) sizes[lev] = h[L - lev];
   const uint64_t nNodes = svo_levels_from_codes(c, dcodes, n, L, o->svo, nullptr, sizes.data());
   o->nNodesSVO = nNodes;   // non-root nodes created (GeomOctree::_nNodes after buildSVO)
   o->state = SVD_S_SVO;
@@ -568,10 +601,10 @@ void remap_children(svd_ctx* c, const uint32_t* in, uint64_t n, const uint32_t* 
 void dedup_level_dev(svd_ctx* c, const uint32_t* rec, uint64_t n, uint32_t* remap, uint32_t** outRec,
                      uint32_t* total) {
   const uint32_t cap = table_cap(n);
-  uint32_t* table = dalloc<uint32_t>(c, cap);
+  TEntry* table = dalloc<TEntry>(c, cap);
   uint32_t* slot = dalloc<uint32_t>(c, n);
   uint32_t* pos = dalloc<uint32_t>(c, n + 1);
-  fill_u32(c, table, kNone, cap);
+  fill_u32(c, (uint32_t*)table, kNone, 2ull * cap);
   k_insert<<<div_up(n, kB), kB, 0, c->stream>>>(rec, n, table, cap - 1, slot);
   k_rep_flags<<<div_up(n, kB), kB, 0, c->stream>>>(slot, table, n, pos);
   check_launch("k_insert");
@@ -706,12 +739,12 @@ void reduce_sym(svd_octree* o) {
       break;
     }
     const uint32_t cap = table_cap(n);
-    uint32_t* table = dalloc<uint32_t>(c, cap);
+    TEntry* table = dalloc<TEntry>(c, cap);
     uint32_t* slot = dalloc<uint32_t>(c, n);
     uint32_t* cand = dalloc<uint32_t>(c, n * 7);
     uint32_t* state = dalloc<uint32_t>(c, n);
     uint32_t* pos = dalloc<uint32_t>(c, n + 1);
-    fill_u32(c, table, kNone, cap);
+    fill_u32(c, (uint32_t*)table, kNone, 2ull * cap);
     k_insert<<<div_up(n, kB), kB, 0, c->stream>>>(key, n, table, cap - 1, slot);
     k_sdag_cands<<<div_up(n, kB), kB, 0, c->stream>>>(key, n, table, cap - 1, slot, childInv, cand, state);
     check_launch("k_sdag_cands");
