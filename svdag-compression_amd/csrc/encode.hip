@@ -12,26 +12,58 @@ namespace {
 
 constexpr int kB = 256;
 
+// ---------------------------------------------------------------- level sets
+// All levels of an octree in one launch: node j of level l is element
+// start[l] + j of the concatenation (start[L] = total), and per-level arrays
+// travel in the kernel arguments.
+struct LevelSet {
+  const uint32_t* rec[kMaxLevels];
+  const uint32_t* aux[kMaxLevels];   // childLevels (SVDAG), or a per-level array of the kernel
+  uint32_t start[kMaxLevels + 1];
+  uint32_t L;
+  uint32_t leafBits;                 // bit l: level l holds leaf masks
+};
+
+__device__ __forceinline__ uint32_t level_of(const LevelSet& a, uint64_t i) {
+  uint32_t l = 0;
+  while (l + 1 < a.L && i >= a.start[l + 1]) ++l;
+  return l;
+}
+
+// Up to 128 bytes written at dst + off[k] from the kernel arguments (file
+// headers): no host staging buffer has to outlive the launch.
+struct Bytes {
+  uint8_t b[128];
+  uint32_t n;
+};
+__global__ void k_put_bytes(Bytes src, uint8_t* __restrict__ dst) {
+  const uint32_t t = threadIdx.x;
+  if (t < src.n) dst[t] = src.b[t];
+}
+
 // ---------------------------------------------------------------- SVDAG / USSVDAG
-__global__ void k_word_counts(const uint32_t* __restrict__ rec, uint64_t n, int leaf, uint32_t* __restrict__ out) {
+__global__ void k_word_counts(LevelSet a, uint64_t n, uint32_t* __restrict__ out) {
   uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  out[i] = leaf ? 1u : 1u + (uint32_t)__popc(rec[i * kRecWords] & 0xFFu);
+  const uint32_t l = level_of(a, i);
+  const uint64_t j = i - a.start[l];
+  out[i] = ((a.leafBits >> l) & 1u) ? 1u : 1u + (uint32_t)__popc(a.rec[l][j * kRecWords] & 0xFFu);
 }
 
 // ussv = 0: header word = mask; child pointers follow childLevels
 //           (offset = first node of that level), c = 7..0.
 // ussv = 1: header word = mZ<<24 | mY<<16 | mX<<8 | mask; pointers into the
 //           next level (childLevels ignored, encoded_ussvdag.cpp:137).
-__global__ void k_write_words(const uint32_t* __restrict__ rec, uint64_t n, int leaf, int ussv,
-                              const uint32_t* __restrict__ childLv, uint32_t lev, const uint32_t* __restrict__ levStart,
-                              const uint32_t* __restrict__ truePtrs, uint32_t base, uint64_t totalNodes,
+__global__ void k_write_words(LevelSet a, int ussv, const uint32_t* __restrict__ truePtrs, uint64_t totalNodes,
                               uint32_t* __restrict__ words, uint32_t* __restrict__ err) {
   uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const uint32_t at = truePtrs[base + i];
-  if (leaf) { words[at] = rec[i] & 0xFFu; return; }
-  const uint32_t* r = rec + i * kRecWords;
+  if (i >= totalNodes) return;
+  const uint32_t lev = level_of(a, i);
+  const uint64_t j = i - a.start[lev];
+  const uint32_t at = truePtrs[i];
+  if ((a.leafBits >> lev) & 1u) { words[at] = a.rec[lev][j] & 0xFFu; return; }
+  const uint32_t* r = a.rec[lev] + j * kRecWords;
+  const uint32_t* childLv = a.aux[lev];
   const uint32_t w0 = r[0];
   words[at] = ussv ? w0 : (w0 & 0xFFu);
   uint32_t k = at + 1;
@@ -41,11 +73,17 @@ __global__ void k_write_words(const uint32_t* __restrict__ rec, uint64_t n, int 
     const uint32_t ch = r[1 + c];
     if (ch == kNull) continue;
     if (k >= kEnd) { atomicAdd(err, 1u); break; }
-    const uint32_t cl = (!ussv && childLv) ? childLv[i * 8 + c] : lev + 1;
-    const uint64_t g = (uint64_t)ch + levStart[cl];
+    const uint32_t cl = (!ussv && childLv) ? childLv[j * 8 + c] : lev + 1;
+    const uint64_t g = cl < a.L ? (uint64_t)ch + a.start[cl] : totalNodes;
     if (g >= totalNodes) { atomicAdd(err, 1u); words[k++] = 0u; continue; }
     words[k++] = truePtrs[g];
   }
+}
+
+// The two header words that depend on the word count: _firstLeafPtr and the
+// array length (both = count, see encode_words).
+__global__ void k_put_count(const uint32_t* __restrict__ total, uint32_t* __restrict__ dst) {
+  if (threadIdx.x < 2) dst[threadIdx.x] = *total;
 }
 
 // ---------------------------------------------------------------- SSVDAG
@@ -55,9 +93,15 @@ __global__ void k_write_words(const uint32_t* __restrict__ rec, uint64_t n, int 
 // straight to the global histogram) and flushes one add per distinct child.
 constexpr int kRcSlots = 1024;
 constexpr int kRcBlocks = 2048;
-__global__ void __launch_bounds__(256) k_refcount(const uint32_t* __restrict__ parent, uint64_t np,
-                                                  uint32_t* __restrict__ hist, uint64_t nChild,
+// Levels 1..L-2 in one launch: blockIdx.y = parent level l, whose children
+// are level l+1 (element start[l+1] + child of the concatenated histogram).
+__global__ void __launch_bounds__(256) k_refcount(LevelSet a, uint32_t* __restrict__ histAll,
                                                   uint32_t* __restrict__ err) {
+  const uint32_t l = blockIdx.y;
+  const uint32_t* __restrict__ parent = a.rec[l];
+  const uint64_t np = a.start[l + 1] - a.start[l];
+  uint32_t* __restrict__ hist = histAll + a.start[l + 1];
+  const uint64_t nChild = a.start[l + 2] - a.start[l + 1];
   __shared__ uint32_t key[kRcSlots], cnt[kRcSlots];
   for (int t = threadIdx.x; t < kRcSlots; t += blockDim.x) { key[t] = kNone; cnt[t] = 0; }
   __syncthreads();
@@ -87,6 +131,12 @@ __global__ void __launch_bounds__(256) k_refcount(const uint32_t* __restrict__ p
   __syncthreads();
   for (int t = threadIdx.x; t < kRcSlots; t += blockDim.x)
     if (key[t] != kNone && cnt[t]) atomicAdd(&hist[key[t]], cnt[t]);
+}
+
+// perm[i] = position of element i within its level (the identity order of every level)
+__global__ void k_level_iota(LevelSet a, uint64_t n, uint32_t* __restrict__ perm) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) perm[i] = (uint32_t)(i - a.start[level_of(a, i)]);
 }
 
 __global__ void k_scatter_rank(const uint32_t* __restrict__ perm, uint64_t n, uint32_t* __restrict__ ind) {
@@ -120,16 +170,30 @@ __global__ void k_offsets_to_ind(const uint32_t* __restrict__ perm, const uint32
   if (i < n) ind[perm[i]] = offs[i];
 }
 
+// Inner levels 0..L-3 of the SSVDAG image in one launch.
+struct InnerLevels {
+  const uint32_t* rec[kMaxLevels];
+  const uint32_t* indChild[kMaxLevels];   // addresses of the next level's nodes
+  const uint32_t* offs[kMaxLevels];       // u16 offset of each sorted node in its level
+  uint32_t outOff[kMaxLevels];            // levelOffsets
+  uint32_t start[kMaxLevels + 1];         // sorted positions, concatenated (= permAll layout)
+  uint32_t L;
+};
+
 // Variable-length inner node (encoded_ssvdag.cpp:352-430).
-__global__ void k_inner_write(const uint32_t* __restrict__ rec, const uint32_t* __restrict__ perm, uint64_t n,
-                              const uint32_t* __restrict__ indChild, const uint32_t* __restrict__ offs,
-                              uint16_t* __restrict__ out) {
-  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const uint32_t* r = rec + (uint64_t)perm[i] * kRecWords;
+__global__ void k_inner_write(InnerLevels a, const uint32_t* __restrict__ permAll, uint64_t n,
+                              uint16_t* __restrict__ inner) {
+  uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= n) return;
+  uint32_t l = 0;
+  while (l + 1 < a.L && g >= a.start[l + 1]) ++l;
+  const uint64_t i = g - a.start[l];
+  const uint32_t* __restrict__ indChild = a.indChild[l];
+  uint16_t* out = inner + a.outOff[l];
+  const uint32_t* r = a.rec[l] + (uint64_t)permAll[g] * kRecWords;
   const uint32_t w0 = r[0];
   const uint32_t m = w0 & 0xFFu, mx = (w0 >> 8) & 0xFFu, my = (w0 >> 16) & 0xFFu, mz = (w0 >> 24) & 0xFFu;
-  uint16_t* o = out + offs[i];
+  uint16_t* o = out + a.offs[l][i];
   uint32_t hdr = 0;
   uint32_t k = 1;
 #pragma unroll
@@ -209,10 +273,31 @@ std::vector<uint8_t> base_header(const svd_octree* o, uint64_t nNodes) {
   return h;
 }
 
-void upload(svd_ctx* c, void* dst, const std::vector<uint8_t>& src) {
+LevelSet level_set(const std::vector<Level>& lv, uint32_t L, bool withChildLevels) {
+  if (L > kMaxLevels) throw Error{SVD_ERR_LEVELS, "too many levels"};
+  LevelSet a{};
+  a.L = L;
+  uint64_t at = 0;
+  for (uint32_t l = 0; l < L; ++l) {
+    a.start[l] = (uint32_t)at;
+    a.rec[l] = lv[l].rec;
+    a.aux[l] = withChildLevels ? lv[l].childLevels : nullptr;
+    if (lv[l].leaf) a.leafBits |= 1u << l;
+    at += lv[l].n;
+  }
+  if (at >= 0xFFFFFFFFull) throw Error{SVD_ERR_CAPACITY, "too many nodes for 32-bit word pointers"};
+  a.start[L] = (uint32_t)at;
+  return a;
+}
+
+void put_bytes(svd_ctx* c, uint8_t* dst, const std::vector<uint8_t>& src) {
   if (src.empty()) return;
-  SVD_HIP(hipMemcpyAsync(dst, src.data(), src.size(), hipMemcpyHostToDevice, c->stream));
-  SVD_HIP(hipStreamSynchronize(c->stream));   // src is pageable host memory
+  if (src.size() > sizeof(Bytes::b)) throw Error{SVD_ERR_DEVICE, "header too long"};
+  Bytes b{};
+  memcpy(b.b, src.data(), src.size());
+  b.n = (uint32_t)src.size();
+  k_put_bytes<<<1, 128, 0, c->stream>>>(b, dst);
+  check_launch("k_put_bytes");
 }
 
 void encode_words(svd_octree* o, int fmt) {
@@ -220,46 +305,37 @@ void encode_words(svd_octree* o, int fmt) {
   const bool ussv = fmt == SVD_FMT_USSVDAG;
   const std::vector<Level>& lv = ussv ? o->sdag : (o->hasX ? o->xdag : o->dag);
   const uint32_t L = o->levels;
-  std::vector<uint32_t> levStart(L + 1, 0);
-  for (uint32_t l = 0; l < L; ++l) levStart[l + 1] = levStart[l] + (uint32_t)lv[l].n;
-  const uint64_t N = levStart[L];
-  uint32_t* dLevStart = dalloc<uint32_t>(c, L + 1);
-  SVD_HIP(hipMemcpyAsync(dLevStart, levStart.data(), 4 * (L + 1), hipMemcpyHostToDevice, c->stream));
+  const LevelSet a = level_set(lv, L, !ussv);
+  const uint64_t N = a.start[L];
   uint32_t* ptrs = dalloc<uint32_t>(c, N + 1);
-  uint32_t* total = dalloc<uint32_t>(c, 1);
-  uint32_t* err = dalloc<uint32_t>(c, 1);
-  fill_u32(c, err, 0u, 1);
-  for (uint32_t l = 0; l < L; ++l)
-    if (lv[l].n) k_word_counts<<<div_up(lv[l].n, kB), kB, 0, c->stream>>>(lv[l].rec, lv[l].n, lv[l].leaf, ptrs + levStart[l]);
+  uint32_t* dev = dalloc<uint32_t>(c, 2);   // word count, error flags
+  fill_u32(c, dev, 0u, 2);
+  if (N) k_word_counts<<<div_up(N, kB), kB, 0, c->stream>>>(a, N, ptrs);
   check_launch("k_word_counts");
-  scan_exclusive(c, ptrs, ptrs, N, total);
-  uint32_t h2[2];
-  read_u32s(c, total, h2, 1);
-  const uint64_t count = h2[0];
+  scan_exclusive(c, ptrs, ptrs, N, dev);
   // The reference assigns _firstLeafPtr after the loop over ALL levels
   // (encoded_svdag.cpp:129-137), so the stored value is the total word count.
-  const uint32_t firstLeaf = (uint32_t)count;
+  // Every node takes at most 9 words: the image is allocated for that bound,
+  // the header's two count words are written on the device and the count is
+  // read back once, with the error flags, at the end.
+  uint64_t maxWords = 0;
+  for (uint32_t l = 0; l < L; ++l) maxWords += lv[l].leaf ? lv[l].n : lv[l].n * kRecWords;
   const uint64_t nNodes = ussv ? o->nNodesSDAG : (o->hasX ? o->nNodesX : o->nNodesDAG);
   std::vector<uint8_t> hdr = base_header(o, nNodes);
-  put(hdr, &firstLeaf, 4);
-  const uint32_t cnt32 = (uint32_t)count;
-  put(hdr, &cnt32, 4);
+  const uint64_t hdrBytes = hdr.size() + 8;   // + firstLeafPtr + count
   Encoded& e = o->enc[fmt];
   dfree(c, e.d);
-  e.bytes = hdr.size() + 4 * count;
-  e.d = dalloc<uint8_t>(c, e.bytes);
-  upload(c, e.d, hdr);
-  uint32_t* words = (uint32_t*)(e.d + hdr.size());
-  for (uint32_t l = 0; l < L; ++l) {
-    if (!lv[l].n) continue;
-    k_write_words<<<div_up(lv[l].n, kB), kB, 0, c->stream>>>(lv[l].rec, lv[l].n, lv[l].leaf, ussv ? 1 : 0,
-                                                              lv[l].childLevels, l, dLevStart, ptrs, levStart[l], N,
-                                                              words, err);
-  }
+  e.d = dalloc<uint8_t>(c, hdrBytes + 4 * maxWords);
+  put_bytes(c, e.d, hdr);
+  k_put_count<<<1, 64, 0, c->stream>>>(dev, (uint32_t*)(e.d + hdr.size()));
+  uint32_t* words = (uint32_t*)(e.d + hdrBytes);
+  if (N) k_write_words<<<div_up(N, kB), kB, 0, c->stream>>>(a, ussv ? 1 : 0, ptrs, N, words, dev + 1);
   check_launch("k_write_words");
-  const uint64_t nerr = read_u32(c, err);
-  dfree(c, dLevStart); dfree(c, ptrs); dfree(c, total); dfree(c, err);
-  if (nerr) throw Error{SVD_ERR_POINTER, "encode: child pointer without a target node"};
+  uint32_t h[2];
+  read_u32s(c, dev, h, 2);
+  dfree(c, ptrs); dfree(c, dev);
+  if (h[1]) throw Error{SVD_ERR_POINTER, "encode: child pointer without a target node"};
+  e.bytes = hdrBytes + 4ull * h[0];
   e.valid = true;
 }
 
@@ -284,19 +360,25 @@ void encode_ssvdag(svd_octree* o, int fmt) {
   uint32_t* permAll = dalloc<uint32_t>(c, tot + 1);
   uint32_t* hist = dalloc<uint32_t>(c, tot + 1);
   fill_u32(c, hist, 0u, tot);
+  // levels 0..L-2 as one concatenation (start = off)
+  std::vector<Level> upper(lv.begin(), lv.begin() + (L - 1));
+  const LevelSet ua = level_set(upper, (uint32_t)(L - 1), false);
+  uint64_t maxParent = 0;
   for (int lev = 0; lev <= L - 2; ++lev) {
-    const uint64_t n = lv[lev].n;
     perm[lev] = permAll + off[lev];
-    iota_u32(c, perm[lev], n);
-    if (lev > 0 && n > 0) {
-      if (lv[lev - 1].n)
-        k_refcount<<<std::min<uint64_t>(div_up(lv[lev - 1].n, kB), kRcBlocks), kB, 0, c->stream>>>(
-            lv[lev - 1].rec, lv[lev - 1].n, hist + off[lev], n, err);
-      check_launch("k_refcount");
+    if (lev > 0 && lv[lev].n > 0) {
       rfirst.push_back(off[lev]);
-      rlast.push_back(off[lev] + n);
+      rlast.push_back(off[lev] + lv[lev].n);
     }
+    if (lev <= L - 3) maxParent = std::max<uint64_t>(maxParent, lv[lev].n);
   }
+  if (tot) k_level_iota<<<div_up(tot, kB), kB, 0, c->stream>>>(ua, tot, permAll);
+  check_launch("k_level_iota");
+  if (maxParent) {
+    const dim3 rg((uint32_t)std::min<uint64_t>(div_up(maxParent, kB), kRcBlocks), (uint32_t)(L - 2));
+    k_refcount<<<rg, kB, 0, c->stream>>>(ua, hist, err);
+  }
+  check_launch("k_refcount");
   refsort_ranges(c, permAll, hist, tot, rfirst.data(), rlast.data(), rfirst.size());
   dfree(c, hist);
   // Phase 1b, bottom-up: children addresses of each level in its order.
@@ -347,22 +429,28 @@ void encode_ssvdag(svd_octree* o, int fmt) {
   dfree(c, e.d);
   e.bytes = tailAt + tail.size();
   e.d = dalloc<uint8_t>(c, e.bytes);
-  SVD_HIP(hipMemcpyAsync(e.d, head.data(), head.size(), hipMemcpyHostToDevice, c->stream));
-  SVD_HIP(hipMemcpyAsync(e.d + leafCntAt, mid.data(), mid.size(), hipMemcpyHostToDevice, c->stream));
-  SVD_HIP(hipMemcpyAsync(e.d + tailAt, tail.data(), tail.size(), hipMemcpyHostToDevice, c->stream));
-  // Phase 2: write every level into the image.
+  put_bytes(c, e.d, head);
+  put_bytes(c, e.d + leafCntAt, mid);
+  put_bytes(c, e.d + tailAt, tail);
+  // Phase 2: write every level into the image (inner levels in one launch).
   uint16_t* inner = (uint16_t*)(e.d + innerAt);
-  for (int lev = 0; lev <= L - 3; ++lev) {
-    const uint64_t n = lv[lev].n;
-    if (!n) continue;
-    k_inner_write<<<div_up(n, kB), kB, 0, c->stream>>>(lv[lev].rec, perm[lev], n, ind[lev + 1], offs[lev],
-                                                        inner + levelOffsets[lev]);
+  if (off[L - 2]) {
+    InnerLevels ia{};
+    ia.L = (uint32_t)(L - 2);
+    for (int lev = 0; lev <= L - 3; ++lev) {
+      ia.rec[lev] = lv[lev].rec;
+      ia.indChild[lev] = ind[lev + 1];
+      ia.offs[lev] = offs[lev];
+      ia.outOff[lev] = levelOffsets[lev];
+      ia.start[lev] = (uint32_t)off[lev];
+    }
+    ia.start[L - 2] = (uint32_t)off[L - 2];
+    k_inner_write<<<div_up(off[L - 2], kB), kB, 0, c->stream>>>(ia, permAll, off[L - 2], inner);
   }
   if (nBricks)
     k_brick_write<<<div_up(nBricks, kB), kB, 0, c->stream>>>(lv[L - 2].rec, perm[L - 2], nBricks, lv[L - 1].rec,
                                                               (uint16_t*)(e.d + leafAt));
   check_launch("ssvdag write");
-  SVD_HIP(hipStreamSynchronize(c->stream));   // host staging vectors go out of scope
   for (int l = 0; l < L; ++l) { dfree(c, ind[l]); dfree(c, offs[l]); }
   dfree(c, permAll); dfree(c, dev);
   e.valid = true;
