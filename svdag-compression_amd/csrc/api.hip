@@ -1,6 +1,7 @@
 // C ABI of libsvdag_amd: handle management, error mapping, host <-> device
 // transfers.  Every entry point converts internal svd::Error exceptions into
 // an int status plus a message stored in the context.
+#include <cstdlib>
 #include "svd_internal.hpp"
 #include <algorithm>
 #include <new>
@@ -135,6 +136,10 @@ int svd_ctx_create(int device, svd_ctx** out) {
   }
   c->arena = new Arena(device);
   if (hipDeviceGetAttribute(&c->numCU, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) c->numCU = 256;
+  {
+    const char* e = std::getenv("SVD_TOSDAG_CHECKED");
+    c->sdagChecked = e && e[0] && !(e[0] == '0' && e[1] == 0);
+  }
   *out = c;
   return SVD_OK;
 }

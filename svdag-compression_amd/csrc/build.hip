@@ -27,7 +27,7 @@ __device__ __forceinline__ bool head64(const uint64_t* code, uint64_t i) {
 // hist[k] = #{i : K_i >= k} (1 <= k <= L).  Counts stay in registers over a
 // grid-stride loop and are reduced per workgroup: one global atomic per
 // workgroup and counter.
-constexpr int kCountBlocks = 1024;
+constexpr int kCountBlocks = 4096;
 __global__ void __launch_bounds__(256) k_check_and_count(const uint64_t* __restrict__ code, uint64_t n, uint32_t L,
                                                          uint32_t* __restrict__ hist) {
   uint32_t cnt[kMaxLevels + 1];
@@ -463,39 +463,64 @@ __global__ void k_sdag_outcome(const uint32_t* __restrict__ key, uint64_t n, con
   atomicAdd(err, 1u);
 }
 
-// Finest level (at most 255 distinct masks): the reference loop run
-// sequentially by one lane over an LDS table.
-__global__ void k_sdag_leaves(const uint32_t* __restrict__ dagMask, uint32_t n, uint32_t* __restrict__ map,
-                              uint32_t* __restrict__ sdagMask, uint8_t* __restrict__ inv, uint32_t* __restrict__ count) {
-  __shared__ uint32_t repOf[256];
-  repOf[threadIdx.x] = kNone;
+// Finest level.  The reference loop walks the nodes in order and makes a node
+// a representative unless one of its eight mirrored masks (kFlip order) is an
+// earlier representative's.  Mirroring a leaf mask only permutes its bits by
+// c -> c ^ f, and these permutations form a group, so the mirrored masks of a
+// node are its orbit: a node is a representative exactly when it is the first
+// node of its orbit, and any other node maps to that first node through the
+// first transform, in kFlip order, that turns its mask into the
+// representative's.  Representatives are numbered in node order.  One
+// workgroup of 256 threads, any n.
+__global__ void __launch_bounds__(256) k_sdag_leaves(const uint32_t* __restrict__ dagMask, uint32_t n,
+                                                     uint32_t* __restrict__ map, uint32_t* __restrict__ sdagMask,
+                                                     uint8_t* __restrict__ inv, uint32_t* __restrict__ count) {
+  __shared__ uint32_t firstOf[256], rankOf[256];
+  const uint32_t t = threadIdx.x;
+  firstOf[t] = kNone;
   __syncthreads();
-  if (threadIdx.x != 0) return;
-  uint32_t cnt = 0;
-  for (uint32_t i = 0; i < n; ++i) {
+  auto canon = [](uint32_t m) {
+    uint32_t c = m;
+#pragma unroll
+    for (uint32_t f = 1; f < 8; ++f) c = min(c, perm8(m, f));
+    return c;
+  };
+  for (uint32_t i = t; i < n; i += 256) atomicMin(&firstOf[canon(dagMask[i] & 0xFFu)], i);
+  __syncthreads();
+  // rank of orbit t's first node among all orbits' first nodes
+  uint32_t r = 0;
+  const uint32_t me = firstOf[t];
+  for (int k = 0; k < 256; ++k) r += firstOf[k] < me ? 1u : 0u;
+  rankOf[t] = r;
+  __syncthreads();
+  if (t == 0) {
+    uint32_t cnt = 0;
+    for (int k = 0; k < 256; ++k) cnt += firstOf[k] != kNone;
+    *count = cnt;
+  }
+  for (uint32_t i = t; i < n; i += 256) {
     const uint32_t m = dagMask[i] & 0xFFu;
-    uint32_t b = 0;
-    if (perm8(m, 4) == m) b |= 1;
-    if (perm8(m, 2) == m) b |= 2;
-    if (perm8(m, 1) == m) b |= 4;
-    bool found = false;
-    for (int t = 0; t < 8 && !found; ++t) {
-      const uint32_t f = kFlip[t];
-      const uint32_t r = repOf[perm8(m, f)];
-      if (r != kNone) {
-        map[i] = r | (((f >> 2) & 1u) << 29) | (((f >> 1) & 1u) << 30) | ((f & 1u) << 31);
-        found = true;
+    const uint32_t c = canon(m);
+    const uint32_t first = firstOf[c], rank = rankOf[c];
+    if (first == i) {
+      uint32_t b = 0;
+      if (perm8(m, 4) == m) b |= 1;
+      if (perm8(m, 2) == m) b |= 2;
+      if (perm8(m, 1) == m) b |= 4;
+      sdagMask[rank] = m;
+      inv[rank] = (uint8_t)b;
+      map[i] = rank;
+      continue;
+    }
+    const uint32_t mr = dagMask[first] & 0xFFu;
+    for (int k = 0; k < 8; ++k) {
+      const uint32_t f = kFlip[k];
+      if (perm8(m, f) == mr) {
+        map[i] = rank | (((f >> 2) & 1u) << 29) | (((f >> 1) & 1u) << 30) | ((f & 1u) << 31);
+        break;
       }
     }
-    if (!found) {
-      repOf[m] = cnt;
-      sdagMask[cnt] = m;
-      inv[cnt] = (uint8_t)b;
-      map[i] = cnt;
-      ++cnt;
-    }
   }
-  *count = cnt;
 }
 
 uint32_t table_cap(uint64_t n) {
@@ -694,21 +719,30 @@ void reduce_dag(svd_octree* o) {
   o->state = SVD_S_DAG;
 }
 
-void reduce_sym(svd_octree* o) {
+namespace {
+
+// One toSDAG pass.  speculative: every level runs two resolution rounds
+// without looking at the host, and the nodes still undecided after the
+// second are counted per level and read back at the end with the level
+// sizes; the pass returns false (the octree's SDAG levels are dropped) when
+// any level had not converged, and the caller reruns it with a convergence
+// check after every round.
+bool reduce_sym_pass(svd_octree* o, bool speculative) {
   svd_ctx* c = o->ctx;
-  if (o->state != SVD_S_DAG && o->state != SVD_S_SDAG) throw Error{SVD_ERR_STATE, "toSDAG: octree is not a DAG"};
   const uint32_t L = o->levels;
   for (auto& lv : o->sdag) { dfree(c, lv.rec); dfree(c, lv.inv); }
   for (auto* m : o->sdagMap) dfree(c, m);
   o->sdag.assign(L, Level());
   o->sdagMap.assign(L, nullptr);
-  if (L == 1) throw Error{SVD_ERR_LEVELS, "toSDAG needs at least 2 levels"};
   uint64_t nNodes = 0;
   uint32_t* count = dalloc<uint32_t>(c, 1);
-  uint32_t* err = dalloc<uint32_t>(c, 1);
-  // representative counts of every level, read back once at the end
-  uint32_t* counts = dalloc<uint32_t>(c, L);
-  fill_u32(c, err, 0u, 1);
+  // [0, L): representative counts of every level; [L, 2L): nodes undecided
+  // after the speculative rounds; [2L]: nodes without correspondence
+  uint32_t* dev = dalloc<uint32_t>(c, 2 * L + 1);
+  uint32_t* counts = dev;
+  uint32_t* undecided = dev + L;
+  uint32_t* err = dev + 2 * L;
+  fill_u32(c, dev, 0u, 2 * L + 1);
   {
     const Level& dl = o->dag[L - 1];
     Level& sl = o->sdag[L - 1];
@@ -748,12 +782,18 @@ void reduce_sym(svd_octree* o) {
     k_insert<<<div_up(n, kB), kB, 0, c->stream>>>(key, n, table, cap - 1, slot);
     k_sdag_cands<<<div_up(n, kB), kB, 0, c->stream>>>(key, n, table, cap - 1, slot, childInv, cand, state);
     check_launch("k_sdag_cands");
-    for (int round = 0;; ++round) {
-      fill_u32(c, count, 0u, 1);
+    if (speculative) {
       k_sdag_resolve<<<div_up(n, kB), kB, 0, c->stream>>>(cand, n, state, count);
+      k_sdag_resolve<<<div_up(n, kB), kB, 0, c->stream>>>(cand, n, state, undecided + lev);
       check_launch("k_sdag_resolve");
-      if (read_u32(c, count) == 0) break;
-      if (round > (int)n + 2) throw Error{SVD_ERR_DEVICE, "toSDAG: resolution did not converge"};
+    } else {
+      for (int round = 0;; ++round) {
+        fill_u32(c, count, 0u, 1);
+        k_sdag_resolve<<<div_up(n, kB), kB, 0, c->stream>>>(cand, n, state, count);
+        check_launch("k_sdag_resolve");
+        if (read_u32(c, count) == 0) break;
+        if (round > (int)n + 2) throw Error{SVD_ERR_DEVICE, "toSDAG: resolution did not converge"};
+      }
     }
     k_state_flags<<<div_up(n, kB), kB, 0, c->stream>>>(state, n, pos);
     scan_exclusive(c, pos, pos, n, counts + lev);
@@ -766,18 +806,28 @@ void reduce_sym(svd_octree* o) {
     check_launch("k_sdag_outcome");
     dfree(c, key); dfree(c, inv); dfree(c, table); dfree(c, slot); dfree(c, cand); dfree(c, state); dfree(c, pos);
   }
-  std::vector<uint32_t> h(L + 1);
-  read_u32s(c, counts, h.data(), L);
-  read_u32s(c, err, h.data() + L, 1);
-  const uint64_t nerr = h[L];
+  std::vector<uint32_t> h(2 * L + 1);
+  read_u32s(c, dev, h.data(), 2 * L + 1);
+  dfree(c, count); dfree(c, dev);
+  for (uint32_t lev = 0; lev < L; ++lev)
+    if (h[L + lev]) return false;
+  const uint64_t nerr = h[2 * L];
   for (uint32_t lev = 1; lev < L; ++lev) {
     o->sdag[lev].n = h[lev];
     nNodes += h[lev];
   }
-  dfree(c, count); dfree(c, err); dfree(c, counts);
   if (nerr) throw Error{SVD_ERR_DEVICE, "toSDAG: node without correspondence"};
   o->nNodesSDAG = nNodes;
   o->state = SVD_S_SDAG;
+  return true;
+}
+
+}  // namespace
+
+void reduce_sym(svd_octree* o) {
+  if (o->state != SVD_S_DAG && o->state != SVD_S_SDAG) throw Error{SVD_ERR_STATE, "toSDAG: octree is not a DAG"};
+  if (o->levels == 1) throw Error{SVD_ERR_LEVELS, "toSDAG needs at least 2 levels"};
+  if (o->ctx->sdagChecked || !reduce_sym_pass(o, true)) reduce_sym_pass(o, false);
 }
 
 }  // namespace svd

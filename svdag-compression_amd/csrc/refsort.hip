@@ -544,8 +544,7 @@ void refsort_ranges(svd_ctx* c, uint32_t* idx, uint32_t* val, uint64_t n, const 
         k_g_median<<<div_up(capB + 1, 256), 256, 0, c->stream>>>(cur, ncur, capB, val, idx, st, blk0, nnxt);
         scan_exclusive(c, blk0, blk0, capB + 1, nullptr);
         k_g_count<<<G, kBT, 0, c->stream>>>(cur, ncur, blk0, st, val, cntL, cntR);
-        scan_exclusive(c, cntL, cntL, G, nullptr);
-        scan_exclusive(c, cntR, cntR, G, nullptr);
+        scan_exclusive2(c, cntL, cntR, G);
         k_g_scatter<<<G, kBT, 0, c->stream>>>(cur, ncur, blk0, st, val, cntL, cntR, Lpos, Rpos);
         k_g_swap<<<G, kBT, 0, c->stream>>>(cur, ncur, blk0, st, val, idx, Lpos, Rpos);
         k_g_split<<<div_up(capB, 256), 256, 0, c->stream>>>(cur, ncur, st, Lpos, Rpos, L);

@@ -64,11 +64,22 @@ __global__ void __launch_bounds__(kThreads) k_scan_down(const uint32_t* __restri
 }
 
 // Small inputs: one 1024-thread workgroup walks the array in chunks of 8192
-// with a running carry (one launch instead of three).
+// with a running carry (one launch instead of three).  Workgroup k scans
+// job k, so two arrays of the same length share a launch.
 constexpr int kSmallThreads = 1024;
 constexpr uint64_t kSmallMax = 8192;
-__global__ void __launch_bounds__(kSmallThreads) k_scan_small(const uint32_t* __restrict__ in, uint32_t* __restrict__ out,
-                                                              uint64_t n, uint32_t* __restrict__ total) {
+struct ScanJob {
+  const uint32_t* in;
+  uint32_t* out;
+  uint32_t* total;
+};
+struct ScanJobs {
+  ScanJob j[2];
+};
+__global__ void __launch_bounds__(kSmallThreads) k_scan_small(ScanJobs jobs, uint64_t n) {
+  const uint32_t* __restrict__ in = jobs.j[blockIdx.x].in;
+  uint32_t* __restrict__ out = jobs.j[blockIdx.x].out;
+  uint32_t* __restrict__ total = jobs.j[blockIdx.x].total;
   __shared__ uint32_t lds[kSmallThreads / 64 + 1];
   uint32_t carry = 0;
   for (uint64_t base = 0; base < n; base += (uint64_t)kSmallThreads * kItems) {
@@ -94,6 +105,38 @@ __global__ void __launch_bounds__(kSmallThreads) k_scan_small(const uint32_t* __
   if (threadIdx.x == 0 && total) *total = carry;
 }
 
+// Down-sweep for up to kFuseMax tiles that also replaces the partials pass:
+// each workgroup sums the partial sums of the tiles before it (they sit in
+// L2), and the last one writes the total.
+constexpr uint32_t kFuseMax = 1024;
+__global__ void __launch_bounds__(kThreads) k_scan_down_fused(const uint32_t* __restrict__ in, uint32_t* __restrict__ out,
+                                                              uint64_t n, const uint32_t* __restrict__ partial,
+                                                              uint32_t* __restrict__ total) {
+  __shared__ uint32_t red[kThreads / 64];
+  __shared__ uint32_t lds[kThreads / 64 + 1];
+  uint32_t p = 0;
+  for (uint32_t k = threadIdx.x; k < blockIdx.x; k += kThreads) p += partial[k];
+  const uint32_t prefix = block_reduce_sum(p, red);
+  const uint64_t base = (uint64_t)blockIdx.x * kTile + (uint64_t)threadIdx.x * kItems;
+  uint32_t v[kItems];
+  uint32_t s = 0;
+#pragma unroll
+  for (int k = 0; k < kItems; ++k) {
+    uint64_t i = base + k;
+    v[k] = i < n ? in[i] : 0u;
+    s += v[k];
+  }
+  uint32_t agg;
+  uint32_t ex = block_exclusive_sum(s, lds, agg) + prefix;
+#pragma unroll
+  for (int k = 0; k < kItems; ++k) {
+    uint64_t i = base + k;
+    if (i < n) out[i] = ex;
+    ex += v[k];
+  }
+  if (total && blockIdx.x + 1 == gridDim.x && threadIdx.x == 0) *total = prefix + agg;
+}
+
 __global__ void k_fill(uint32_t* d, uint32_t v, uint64_t n) {
   uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) d[i] = v;
@@ -115,17 +158,37 @@ void scan_exclusive(svd_ctx* c, const uint32_t* in, uint32_t* out, uint64_t n, u
     return;
   }
   if (n <= kSmallMax) {
-    k_scan_small<<<1, kSmallThreads, 0, c->stream>>>(in, out, n, total);
+    ScanJobs jobs{};
+    jobs.j[0] = ScanJob{in, out, total};
+    k_scan_small<<<1, kSmallThreads, 0, c->stream>>>(jobs, n);
     check_launch("scan_exclusive");
     return;
   }
   const uint32_t nb = div_up(n, kTile);
   uint32_t* partial = dalloc<uint32_t>(c, nb);
   k_scan_reduce<<<nb, kThreads, 0, c->stream>>>(in, n, partial);
-  k_scan_partials<<<1, kThreads, 0, c->stream>>>(partial, nb, total);
-  k_scan_down<<<nb, kThreads, 0, c->stream>>>(in, out, n, partial);
+  if (nb <= kFuseMax) {
+    k_scan_down_fused<<<nb, kThreads, 0, c->stream>>>(in, out, n, partial, total);
+  } else {
+    k_scan_partials<<<1, kThreads, 0, c->stream>>>(partial, nb, total);
+    k_scan_down<<<nb, kThreads, 0, c->stream>>>(in, out, n, partial);
+  }
   check_launch("scan_exclusive");
   dfree(c, partial);
+}
+
+void scan_exclusive2(svd_ctx* c, uint32_t* a, uint32_t* b, uint64_t n) {
+  if (n == 0) return;
+  if (n > kSmallMax) {
+    scan_exclusive(c, a, a, n, nullptr);
+    scan_exclusive(c, b, b, n, nullptr);
+    return;
+  }
+  ScanJobs jobs{};
+  jobs.j[0] = ScanJob{a, a, nullptr};
+  jobs.j[1] = ScanJob{b, b, nullptr};
+  k_scan_small<<<2, kSmallThreads, 0, c->stream>>>(jobs, n);
+  check_launch("scan_exclusive2");
 }
 
 void fill_u32(svd_ctx* c, uint32_t* d, uint32_t v, uint64_t n) {

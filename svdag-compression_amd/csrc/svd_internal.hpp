@@ -48,6 +48,7 @@ struct svd_ctx {
   uint32_t traceTileW = 16;                  // traversal workgroup tile: traceTileW x (256 / traceTileW) pixels
   uint32_t traceMode = SVD_TRACE_TILED;      // SVD_TRACE_TILED or SVD_TRACE_PERSISTENT
   int numCU = 0;                             // compute units of the device
+  bool sdagChecked = false;                  // toSDAG: check convergence after every round (SVD_TOSDAG_CHECKED=1)
   std::map<uint64_t, int> occupancy;         // (kernel id, LDS bytes) -> resident workgroups per CU
 };
 
@@ -157,6 +158,8 @@ void read_u32s(svd_ctx* c, const uint32_t* d, uint32_t* h, size_t n);
 // Exclusive prefix sum of n u32 values; out may alias in.  *total (device) gets
 // the sum (may be null).  Scratch comes from the context arena.
 void scan_exclusive(svd_ctx* c, const uint32_t* in, uint32_t* out, uint64_t n, uint32_t* total);
+// in-place exclusive scans of two arrays of n elements (one launch when small)
+void scan_exclusive2(svd_ctx* c, uint32_t* a, uint32_t* b, uint64_t n);
 void fill_u32(svd_ctx* c, uint32_t* d, uint32_t v, uint64_t n);
 void iota_u32(svd_ctx* c, uint32_t* d, uint64_t n);
 void fill_u8(svd_ctx* c, uint8_t* d, uint8_t v, uint64_t n);

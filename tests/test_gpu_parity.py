@@ -86,6 +86,30 @@ def test_build_levels_and_files_bit_exact(ctx, name):
         assert got == want, f"format {fmt} differs at byte {next(i for i in range(len(got)) if got[i] != want[i])}"
 
 
+def test_tosdag_checked_rounds_equal_speculative(ctx, monkeypatch):
+    """toSDAG runs two resolution rounds per level without host reads and
+    checks convergence once at the end; SVD_TOSDAG_CHECKED=1 (read when a
+    context is created) checks after every round instead.  Both give the
+    oracle's SDAG and files."""
+    codes, L = W.terrain_codes(9, seed=2), 9
+    _, _, sdag_o, files_o = oracle_pipeline(codes, L)
+    monkeypatch.setenv("SVD_TOSDAG_CHECKED", "1")
+    checked = S.Context(0)
+    monkeypatch.delenv("SVD_TOSDAG_CHECKED")
+    try:
+        for c in (ctx, checked):
+            g, _, _, sdag_g, files_g = gpu_pipeline(c, codes, L)
+            g.free()
+            for l in range(L):
+                np.testing.assert_array_equal(sdag_g[l][0], sdag_o[l][0], err_msg=f"SDAG level {l}")
+                if l > 0:
+                    np.testing.assert_array_equal(sdag_g[l][1], sdag_o[l][1], err_msg=f"SDAG inv level {l}")
+            for f in (S.FMT_USSVDAG, S.FMT_SSVDAG):
+                assert files_g[f] == files_o[f]
+    finally:
+        checked.close()
+
+
 def test_mesh_svo_reduction_bit_exact(ctx):
     """Config-1 style input: icosphere mesh voxelized in the reference's
     triangle-major order (oracle buildSVO), reduced and encoded on the GPU."""
