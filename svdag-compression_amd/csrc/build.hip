@@ -279,9 +279,32 @@ __device__ __forceinline__ uint32_t perm8(uint32_t m, uint32_t flip) {
   return r;
 }
 
+// Invariance bits of a node's children, one 8-bit mask per axis (bit c =
+// child c in the node's own order), loaded once per node; zero without a
+// child level (childInv == nullptr).
+struct ChildInv {
+  uint32_t x, y, z;
+};
+__device__ __forceinline__ ChildInv child_inv(const uint32_t k0[kRecWords], const uint8_t* __restrict__ childInv) {
+  ChildInv r{0u, 0u, 0u};
+  if (!childInv) return r;
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    if (k0[1 + c] == kNull) continue;
+    const uint32_t ci = childInv[k0[1 + c]];
+    r.x |= (ci & 1u) << c;
+    r.y |= ((ci >> 1) & 1u) << c;
+    r.z |= ((ci >> 2) & 1u) << c;
+  }
+  return r;
+}
+
 // Node::mirror with applyToChildren (octree_node.cpp:88-195) by transform t,
-// then (if childInv) GeomOctree::invertInvs (geom_octree.cpp:822-831).
-__device__ __forceinline__ void make_variant(const uint32_t k0[kRecWords], int t, const uint8_t* __restrict__ childInv,
+// then GeomOctree::invertInvs (geom_octree.cpp:822-831): a mirrored child
+// that is invariant under the axis keeps its mirror bit clear.  Child c of
+// the variant is child c ^ flip of the node, so the invariance masks are
+// permuted like the children.
+__device__ __forceinline__ void make_variant(const uint32_t k0[kRecWords], int t, const ChildInv& inv,
                                              uint32_t v[kRecWords]) {
   const uint32_t flip = kFlip[t];
   const uint32_t tx = (flip >> 2) & 1u, ty = (flip >> 1) & 1u, tz = flip & 1u;
@@ -297,16 +320,9 @@ __device__ __forceinline__ void make_variant(const uint32_t k0[kRecWords], int t
   if (tx) mx ^= ex;
   if (ty) my ^= ex;
   if (tz) mz ^= ex;
-  if (childInv) {
-#pragma unroll
-    for (int c = 0; c < 8; ++c) {
-      if (!((ex >> c) & 1u)) continue;
-      const uint32_t ci = childInv[v[1 + c]];
-      if (tx && (ci & 1u)) mx &= ~(1u << c);
-      if (ty && (ci & 2u)) my &= ~(1u << c);
-      if (tz && (ci & 4u)) mz &= ~(1u << c);
-    }
-  }
+  if (tx) mx &= ~perm8(inv.x, flip);
+  if (ty) my &= ~perm8(inv.y, flip);
+  if (tz) mz &= ~perm8(inv.z, flip);
   v[0] = mask | (mx << 8) | (my << 16) | (mz << 24);
 }
 
@@ -342,9 +358,10 @@ __global__ void k_sdag_keys(const uint32_t* __restrict__ dag, uint64_t n, const 
   for (int k = 0; k < kRecWords; ++k) o[k] = w[k];
   if (inv) {
     uint32_t b = 0, v[kRecWords];
-    make_variant(w, 1, nullptr, v); b |= same_rec(v, w) ? 1u : 0u;
-    make_variant(w, 2, nullptr, v); b |= same_rec(v, w) ? 2u : 0u;
-    make_variant(w, 3, nullptr, v); b |= same_rec(v, w) ? 4u : 0u;
+    const ChildInv none{0u, 0u, 0u};
+    make_variant(w, 1, none, v); b |= same_rec(v, w) ? 1u : 0u;
+    make_variant(w, 2, none, v); b |= same_rec(v, w) ? 2u : 0u;
+    make_variant(w, 3, none, v); b |= same_rec(v, w) ? 4u : 0u;
     inv[i] = (uint8_t)b;
   }
 }
@@ -369,9 +386,10 @@ __global__ void k_sdag_cands(const uint32_t* __restrict__ key, uint64_t n, const
   }
   uint32_t k0[kRecWords], v[kRecWords];
   load_rec(key, i, k0);
+  const ChildInv ci = child_inv(k0, childInv);
   bool any = false;
   for (int t = 1; t < 8; ++t) {
-    make_variant(k0, t, childInv, v);
+    make_variant(k0, t, ci, v);
     // a variant equal to the node's own key finds the node itself
     bool self = true;
 #pragma unroll
@@ -449,10 +467,11 @@ __global__ void k_sdag_outcome(const uint32_t* __restrict__ key, uint64_t n, con
     return;
   }
   uint32_t v[kRecWords];
+  const ChildInv ci = child_inv(k0, childInv);
   for (int t = 0; t < 8; ++t) {
     uint32_t r;
     if (t == 0) r = entry_index(table[slot[i]]);
-    else { make_variant(k0, t, childInv, v); r = lookup(key, table, cmask, v); }
+    else { make_variant(k0, t, ci, v); r = lookup(key, table, cmask, v); }
     if (r != kNone && r < (uint32_t)i && state[r] == kRep) {
       const uint32_t f = kFlip[t];
       map[i] = pos[r] | (((f >> 2) & 1u) << 29) | (((f >> 1) & 1u) << 30) | ((f & 1u) << 31);
