@@ -27,7 +27,7 @@ __device__ __forceinline__ bool head64(const uint64_t* code, uint64_t i) {
 // hist[k] = #{i : K_i >= k} (1 <= k <= L).  Counts stay in registers over a
 // grid-stride loop and are reduced per workgroup: one global atomic per
 // workgroup and counter.
-constexpr int kCountBlocks = 4096;
+constexpr int kCountBlocks = 1024;
 __global__ void __launch_bounds__(256) k_check_and_count(const uint64_t* __restrict__ code, uint64_t n, uint32_t L,
                                                          uint32_t* __restrict__ hist) {
   uint32_t cnt[kMaxLevels + 1];
