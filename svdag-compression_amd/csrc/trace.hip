@@ -661,18 +661,33 @@ __device__ __forceinline__ void render_pixel(const Dev& s, const svd_trace_param
   rgb[3 * k + 2] = col[2];
 }
 
-// AO_MODE main() for one pixel (octree_dda.frag.glsl:884-919).  The
-// reference's AO program reads a G-buffer (hitPosTex, hitNormTex, depthTex)
-// that nothing in the reference writes; this build defines it from the
-// pixel's primary ray: hitPos = o + t d of a hit and (0, 0, 0) otherwise,
-// hitNorm = -stepDir as trace_ray returns it, cell size 2 rootHalfSide / 2^level.
-// hs = the 256 cosine-weighted hemisphere samples (3 floats each).  A
-// discarded fragment (hitPos = 0) gets -1.  A sample ray whose direction has
-// a NaN component counts as a miss (the box test rejects such rays).
+// AO_MODE main() (octree_dda.frag.glsl:884-919), in three passes so that
+// the hemisphere rays of the pixels with a hit fill every lane:
+//  1. ao_gbuffer: per pixel, the primary ray and the G-buffer entry.  The
+//     reference's AO program reads a G-buffer (hitPosTex, hitNormTex,
+//     depthTex) that nothing in the reference writes; this build defines it
+//     from the primary ray: hitPos = o + t d of a hit and (0, 0, 0)
+//     otherwise, hitNorm = -stepDir as trace_ray returns it, cell size
+//     2 rootHalfSide / 2^level.  A discarded fragment (hitPos = 0) gets -1;
+//     the others are appended to a list.
+//  2. k_ao_rays: one (entry, ray) pair per lane over the list; a ray that
+//     hits adds 1 to its pixel's count.  A sample direction with a NaN
+//     component counts as a miss (the box test rejects such rays).
+//  3. k_ao_finish: visibility (1 - hits / numAORays)^2.  The count is an
+//     integer, so the order of the additions does not matter.
+struct AOEntry {
+  float hit[3];       // hit position moved 1e-3 along the normal
+  float projf;        // 0.5 / cell size
+  int8_t step[4];     // stepDir: x, y, z (the normal is -stepDir)
+  int base;           // int(gl_FragCoord.x * gl_FragCoord.y)
+  uint32_t pixel;
+};
+
 template <int FMT>
-__device__ __forceinline__ void ao_pixel(const Dev& s, const svd_trace_params& p, const float* __restrict__ hs,
-                                         uint32_t nrays, uint32_t px, uint32_t py, uint32_t* stackMem,
-                                         float* __restrict__ vis, svd_ray_result* __restrict__ prim) {
+__device__ __forceinline__ void ao_gbuffer(const Dev& s, const svd_trace_params& p, uint32_t px, uint32_t py,
+                                           uint32_t* stackMem, float* __restrict__ vis, uint32_t* __restrict__ cnt,
+                                           AOEntry* __restrict__ entries, uint32_t* __restrict__ nEntries,
+                                           svd_ray_result* __restrict__ prim) {
   const int lane = threadIdx.y * blockDim.x + threadIdx.x;
   Tracer<FMT> tr(s, make_stack<FMT>(stackMem, s.innerLevels + 1, lane));
   float o[3], d[3];
@@ -692,32 +707,20 @@ __device__ __forceinline__ void ao_pixel(const Dev& s, const svd_trace_params& p
   if (hit[0] == 0.f && hit[1] == 0.f && hit[2] == 0.f) { vis[k] = -1.0f; return; }
   const float hn[3] = {-(float)step[0], -(float)step[1], -(float)step[2]};
   const float cellSize = (2.0f * s.rootHalfSide) / ldexpf(1.0f, (int)res[1]);
-  hit[0] = hit[0] + hn[0] * 1e-3f;
-  hit[1] = hit[1] + hn[1] * 1e-3f;
-  hit[2] = hit[2] + hn[2] * 1e-3f;
-  const float rv[3] = {0.f, 1.f, 0.f};
-  const float dn = dot3(rv, hn);
-  float tg[3] = {rv[0] - hn[0] * dn, rv[1] - hn[1] * dn, rv[2] - hn[2] * dn};
-  normalize3(tg);
-  const float bt[3] = {hn[1] * tg[2] - tg[1] * hn[2], hn[2] * tg[0] - tg[2] * hn[0], hn[0] * tg[1] - tg[0] * hn[1]};
-  const float largo = 0.5f;
-  const float projf = largo / cellSize;
-  const int base = (int)((((float)px + 0.5f)) * (((float)py + 0.5f)));
-  float kk = 0.f;
-  for (uint32_t i = 0; i < nrays; ++i) {
-    const float* sm = hs + 3 * (((int)i + base) % kAOSamples);
-    float rd[3];
-#pragma unroll
-    for (int a = 0; a < 3; ++a) rd[a] = (tg[a] * sm[0] + hn[a] * sm[1]) + bt[a] * sm[2];
-    normalize3(rd);
-    if (rd[0] != rd[0] || rd[1] != rd[1] || rd[2] != rd[2]) continue;
-    float ares[4];
-    int astep[3];
-    tr.trace(hit, rd, 0.0f, largo, projf, p.max_iters, p.draw_level, ares, astep);
-    if (ares[0] > 0.f) kk = kk + 1.0f;
-  }
-  const float v = nrays > 0 ? 1.0f - (kk / (float)nrays) : 1.0f;
-  vis[k] = v * v;
+  AOEntry e;
+  e.hit[0] = hit[0] + hn[0] * 1e-3f;
+  e.hit[1] = hit[1] + hn[1] * 1e-3f;
+  e.hit[2] = hit[2] + hn[2] * 1e-3f;
+  e.projf = 0.5f / cellSize;
+  e.step[0] = (int8_t)step[0];
+  e.step[1] = (int8_t)step[1];
+  e.step[2] = (int8_t)step[2];
+  e.step[3] = 0;
+  e.base = (int)((((float)px + 0.5f)) * (((float)py + 0.5f)));
+  e.pixel = (uint32_t)k;
+  vis[k] = 0.0f;
+  cnt[k] = 0u;
+  entries[atomicAdd(nEntries, 1u)] = e;
 }
 
 // Persistent form: lanes that finished their rays take new ones from a
@@ -905,15 +908,56 @@ __global__ void __launch_bounds__(kBT) SVD_TRACE_OCC k_render(Dev s, svd_trace_p
 }
 
 template <int FMT>
-__global__ void __launch_bounds__(kBT) SVD_TRACE_OCC k_trace_ao(Dev s, svd_trace_params p,
-                                                                  const float* __restrict__ hs, uint32_t nrays,
-                                                                  float* __restrict__ vis,
-                                                                  svd_ray_result* __restrict__ prim) {
+__global__ void __launch_bounds__(kBT) SVD_TRACE_OCC k_ao_gbuffer(Dev s, svd_trace_params p, float* __restrict__ vis,
+                                                                    uint32_t* __restrict__ cnt,
+                                                                    AOEntry* __restrict__ entries,
+                                                                    uint32_t* __restrict__ nEntries,
+                                                                    svd_ray_result* __restrict__ prim) {
   extern __shared__ uint32_t stackMem[];
   const uint32_t px = blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t py = blockIdx.y * blockDim.y + threadIdx.y;
   if (px >= p.width || py >= p.height) return;
-  ao_pixel<FMT>(s, p, hs, nrays, px, py, stackMem, vis, prim);
+  ao_gbuffer<FMT>(s, p, px, py, stackMem, vis, cnt, entries, nEntries, prim);
+}
+
+// Grid-stride over the (entry, ray) pairs; consecutive lanes take the rays
+// of one entry.  The pair count is read once, so every lane's loop ends.
+template <int FMT>
+__global__ void __launch_bounds__(kBT) SVD_TRACE_OCC k_ao_rays(Dev s, svd_trace_params p, const float* __restrict__ hs,
+                                                                 uint32_t nrays, const AOEntry* __restrict__ entries,
+                                                                 const uint32_t* __restrict__ nEntries,
+                                                                 uint32_t* __restrict__ cnt) {
+  extern __shared__ uint32_t stackMem[];
+  Tracer<FMT> tr(s, make_stack<FMT>(stackMem, s.innerLevels + 1, threadIdx.x));
+  const uint64_t total = (uint64_t)(*nEntries) * nrays;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; w < total; w += stride) {
+    const uint32_t i = (uint32_t)(w % nrays);
+    const AOEntry e = entries[w / nrays];
+    const float hn[3] = {-(float)e.step[0], -(float)e.step[1], -(float)e.step[2]};
+    const float rv[3] = {0.f, 1.f, 0.f};
+    const float dn = dot3(rv, hn);
+    float tg[3] = {rv[0] - hn[0] * dn, rv[1] - hn[1] * dn, rv[2] - hn[2] * dn};
+    normalize3(tg);
+    const float bt[3] = {hn[1] * tg[2] - tg[1] * hn[2], hn[2] * tg[0] - tg[2] * hn[0], hn[0] * tg[1] - tg[0] * hn[1]};
+    const float* sm = hs + 3 * (((int)i + e.base) % kAOSamples);
+    float rd[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) rd[a] = (tg[a] * sm[0] + hn[a] * sm[1]) + bt[a] * sm[2];
+    normalize3(rd);
+    if (rd[0] != rd[0] || rd[1] != rd[1] || rd[2] != rd[2]) continue;
+    float ares[4];
+    int astep[3];
+    tr.trace(e.hit, rd, 0.0f, 0.5f, e.projf, p.max_iters, p.draw_level, ares, astep);
+    if (ares[0] > 0.f) atomicAdd(&cnt[e.pixel], 1u);
+  }
+}
+
+__global__ void k_ao_finish(float* __restrict__ vis, const uint32_t* __restrict__ cnt, uint64_t n, uint32_t nrays) {
+  const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n || vis[k] == -1.0f) return;
+  const float v = nrays > 0 ? 1.0f - ((float)cnt[k] / (float)nrays) : 1.0f;
+  vis[k] = v * v;
 }
 
 }  // namespace
@@ -1132,23 +1176,38 @@ void ao_frame(svd_scene* sc, const svd_trace_params* p, uint32_t nrays, float* v
   if (p->width == 0 || p->height == 0) return;
   if (p->draw_level == 0) throw Error{SVD_ERR_ARG, "draw_level must be >= 1"};
   if (!vis) throw Error{SVD_ERR_ARG, "visibility is NULL"};
+  const uint64_t npix = (uint64_t)p->width * p->height;
   const std::vector<float>& hs = hemisphere_samples();
   float* dhs = dalloc<float>(c, hs.size());
   SVD_HIP(hipMemcpyAsync(dhs, hs.data(), hs.size() * sizeof(float), hipMemcpyHostToDevice, c->stream));
+  uint32_t* cnt = dalloc<uint32_t>(c, npix);
+  AOEntry* entries = dalloc<AOEntry>(c, npix);
+  uint32_t* nEntries = dalloc<uint32_t>(c, 1);
+  fill_u32(c, nEntries, 0u, 1);
   const uint32_t tw = c->traceTileW, th = kBT / tw;
   const dim3 grid(div_up(p->width, tw), div_up(p->height, th)), block(tw, th);
   const size_t shmem = stack_bytes(sc, d);
+  // as many workgroups as run at once at six wavefronts per SIMD
+  const uint32_t rayBlocks = (uint32_t)std::max(1, c->numCU > 0 ? c->numCU : 256) * 6u;
   switch (sc->fmt) {
     case SVD_FMT_SVDAG:
-      k_trace_ao<SVD_FMT_SVDAG><<<grid, block, shmem, c->stream>>>(d, *p, dhs, nrays, vis, prim);
+      k_ao_gbuffer<SVD_FMT_SVDAG><<<grid, block, shmem, c->stream>>>(d, *p, vis, cnt, entries, nEntries, prim);
+      if (nrays) k_ao_rays<SVD_FMT_SVDAG><<<rayBlocks, kBT, shmem, c->stream>>>(d, *p, dhs, nrays, entries, nEntries, cnt);
       break;
     case SVD_FMT_USSVDAG:
-      k_trace_ao<SVD_FMT_USSVDAG><<<grid, block, shmem, c->stream>>>(d, *p, dhs, nrays, vis, prim);
+      k_ao_gbuffer<SVD_FMT_USSVDAG><<<grid, block, shmem, c->stream>>>(d, *p, vis, cnt, entries, nEntries, prim);
+      if (nrays)
+        k_ao_rays<SVD_FMT_USSVDAG><<<rayBlocks, kBT, shmem, c->stream>>>(d, *p, dhs, nrays, entries, nEntries, cnt);
       break;
-    default: k_trace_ao<SVD_FMT_SSVDAG><<<grid, block, shmem, c->stream>>>(d, *p, dhs, nrays, vis, prim); break;
+    default:
+      k_ao_gbuffer<SVD_FMT_SSVDAG><<<grid, block, shmem, c->stream>>>(d, *p, vis, cnt, entries, nEntries, prim);
+      if (nrays)
+        k_ao_rays<SVD_FMT_SSVDAG><<<rayBlocks, kBT, shmem, c->stream>>>(d, *p, dhs, nrays, entries, nEntries, cnt);
+      break;
   }
-  check_launch("k_trace_ao");
-  dfree(c, dhs);
+  k_ao_finish<<<div_up(npix, 256), 256, 0, c->stream>>>(vis, cnt, npix, nrays);
+  check_launch("ao passes");
+  dfree(c, dhs); dfree(c, cnt); dfree(c, entries); dfree(c, nEntries);
 }
 
 // SVDAG scenes (nodes, nWords, levels, box) x cameras in one launch.

@@ -74,6 +74,32 @@ def test_render_matches_oracle(ctx, scene_files, fmt):
         assert (a != b).any()
 
 
+@pytest.mark.parametrize("fmt", [S.FMT_SVDAG, S.FMT_USSVDAG, S.FMT_SSVDAG])
+def test_render_iteration_limit_and_draw_level(ctx, scene_files, fmt):
+    """maxIters small enough that rays run out (-3: the red fragments of
+    octree_dda.frag.glsl:837-840) and a drawLevel above the leaves (hits at
+    coarser levels): colours still equal the oracle's."""
+    L, files = scene_files
+    buf = files[fmt]
+    p = W.trace_params(144, 96, seed=9, levels=L, shadows=True, light=(1.0, 0.6, 1.0), max_iters=24)
+    want, wprim = O.render_frame(buf, fmt, p, S.ViewParams.defaults(render_mode=3, norm_samples=2))
+    assert (wprim["t"] == -3).sum() > 50
+    got, gprim = gpu_render(ctx, fmt, buf, p, S.ViewParams.defaults(render_mode=3, norm_samples=2), 0.0)
+    np.testing.assert_array_equal(gprim, wprim)
+    np.testing.assert_array_equal(got, want)
+    red = (wprim["t"] == -3).reshape(p.height, p.width)
+    np.testing.assert_array_equal(got[red], np.tile(np.float32([1, 0, 0]), (red.sum(), 1)))
+    p = W.trace_params(144, 96, seed=9, levels=L, shadows=True, light=(1.0, 0.6, 1.0))
+    p.draw_level = L - 3
+    for v in (S.ViewParams.defaults(render_mode=2), S.ViewParams.defaults(render_mode=0, random_colors=0)):
+        want, wprim = O.render_frame(buf, fmt, p, v)
+        hit = wprim["t"] >= 0
+        assert hit.any() and wprim["level"][hit].max() <= L - 4
+        got, gprim = gpu_render(ctx, fmt, buf, p, v, 0.0)
+        np.testing.assert_array_equal(gprim, wprim)
+        np.testing.assert_array_equal(got, want)
+
+
 @pytest.mark.parametrize("fmt", [S.FMT_SVDAG, S.FMT_SSVDAG])
 @pytest.mark.parametrize("size", [(160, 112), (203, 121)])
 def test_render_beam_matches_oracle(ctx, scene_files, fmt, size):
