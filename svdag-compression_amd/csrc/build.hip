@@ -6,6 +6,7 @@
 // Node records are 9 u32 words: w0 = mask | mX<<8 | mY<<16 | mZ<<24,
 // w1..w8 = children[0..7]; the finest level stores only the mask word.
 #include "svd_internal.hpp"
+#include "block_scan.hpp"
 #include <algorithm>
 
 namespace svd {
@@ -174,6 +175,89 @@ __device__ __forceinline__ uint32_t lookup(const uint32_t* __restrict__ rec, con
   }
 }
 
+// A small inner level of toDAG (n <= kSmallLevel) in one workgroup: the
+// children remap, the first-occurrence insert (table in LDS), the
+// representative flags, their scan and the compaction of k_remap_children,
+// k_insert, k_rep_flags, scan_exclusive and k_compact, with workgroup
+// barriers instead of launches.  The table size does not change the result:
+// each key's slot ends with its smallest index whatever the probe sequence.
+constexpr uint32_t kSmallLevel = 4096;
+constexpr uint32_t kSmallCap = 8192;        // power of two >= 2 * kSmallLevel
+constexpr int kSmallThreads = 1024;
+__global__ void __launch_bounds__(kSmallThreads) k_dedup_small(const uint32_t* __restrict__ src, uint32_t n,
+                                                               const uint32_t* __restrict__ remapIn,
+                                                               uint32_t* __restrict__ keyed, uint32_t* __restrict__ out,
+                                                               uint32_t* __restrict__ remapOut,
+                                                               uint32_t* __restrict__ total) {
+  __shared__ TEntry table[kSmallCap];
+  __shared__ uint16_t slotOf[kSmallLevel];
+  __shared__ uint32_t posOf[kSmallLevel];
+  __shared__ uint32_t lds[kSmallThreads / 64 + 1];
+  const uint32_t tid = threadIdx.x;
+  for (uint32_t j = tid; j < kSmallCap; j += kSmallThreads) table[j] = kEmpty;
+  for (uint32_t i = tid; i < n; i += kSmallThreads) {
+    uint32_t w[kRecWords];
+    load_rec(src, i, w);
+    const uint32_t m = w[0] & 0xFFu;
+#pragma unroll
+    for (int c = 0; c < 8; ++c)
+      if ((m >> c) & 1u) w[1 + c] = remapIn[w[1 + c]];
+    uint32_t* o = keyed + (uint64_t)i * kRecWords;
+#pragma unroll
+    for (int k = 0; k < kRecWords; ++k) o[k] = w[k];
+  }
+  __syncthreads();
+  for (uint32_t i = tid; i < n; i += kSmallThreads) {
+    uint32_t w[kRecWords];
+    load_rec(keyed, i, w);
+    if ((w[0] & 0xFFu) == 0) { slotOf[i] = 0xFFFFu; continue; }
+    const uint64_t hv = hash_rec(w);
+    const uint32_t fp = (uint32_t)(hv >> 32);
+    const TEntry mine = ((TEntry)fp << 32) | (TEntry)i;
+    uint32_t h = (uint32_t)hv & (kSmallCap - 1);
+    while (true) {
+      TEntry e = table[h];
+      if (e == kEmpty) {
+        const TEntry old = atomicCAS(&table[h], kEmpty, mine);
+        if (old == kEmpty) break;
+        e = old;
+      }
+      if ((uint32_t)(e >> 32) == fp && rec_eq(keyed, (uint32_t)e, w)) {
+        if (mine < e) atomicMin(&table[h], mine);
+        break;
+      }
+      h = (h + 1) & (kSmallCap - 1);
+    }
+    slotOf[i] = (uint16_t)h;
+  }
+  __syncthreads();
+  uint32_t carry = 0;
+  for (uint32_t base = 0; base < n; base += kSmallThreads) {
+    const uint32_t i = base + tid;
+    uint32_t f = 0;
+    if (i < n && slotOf[i] != 0xFFFFu) f = entry_index(table[slotOf[i]]) == i ? 1u : 0u;
+    uint32_t agg;
+    const uint32_t ex = block_exclusive_sum(f, lds, agg) + carry;
+    if (i < n) posOf[i] = ex;
+    carry += agg;
+  }
+  __syncthreads();
+  for (uint32_t i = tid; i < n; i += kSmallThreads) {
+    const uint32_t sl = slotOf[i];
+    // an empty (skipped) node keeps correspondence 0 (geom_octree.cpp:489-495)
+    if (sl == 0xFFFFu) { remapOut[i] = 0; continue; }
+    const uint32_t r = entry_index(table[sl]);
+    remapOut[i] = posOf[r];
+    if (r == i) {
+      const uint32_t* a = keyed + (uint64_t)i * kRecWords;
+      uint32_t* b = out + (uint64_t)posOf[i] * kRecWords;
+#pragma unroll
+      for (int k = 0; k < kRecWords; ++k) b[k] = a[k];
+    }
+  }
+  if (tid == 0) *total = carry;
+}
+
 __global__ void k_rep_flags(const uint32_t* __restrict__ slot, const TEntry* __restrict__ table, uint64_t n,
                             uint32_t* __restrict__ flag) {
   uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -336,10 +420,9 @@ __device__ __forceinline__ bool same_rec(const uint32_t a[kRecWords], const uint
 // Identity key K0 of each DAG node of a level: children moved to the SDAG
 // ids of the level below, mirror bits set from their correspondence
 // (geom_octree.cpp:666-679); plus raw invariance bits (:603-606).
-__global__ void k_sdag_keys(const uint32_t* __restrict__ dag, uint64_t n, const uint32_t* __restrict__ childMap,
-                            uint32_t* __restrict__ key, uint8_t* __restrict__ inv) {
-  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+__device__ __forceinline__ void sdag_key(const uint32_t* __restrict__ dag, uint64_t i,
+                                         const uint32_t* __restrict__ childMap, uint32_t* __restrict__ key,
+                                         uint8_t* __restrict__ inv) {
   uint32_t w[kRecWords];
   load_rec(dag, i, w);
   uint32_t mx = (w[0] >> 8) & 0xFFu, my = (w[0] >> 16) & 0xFFu, mz = (w[0] >> 24) & 0xFFu;
@@ -364,6 +447,12 @@ __global__ void k_sdag_keys(const uint32_t* __restrict__ dag, uint64_t n, const 
     make_variant(w, 3, none, v); b |= same_rec(v, w) ? 4u : 0u;
     inv[i] = (uint8_t)b;
   }
+}
+
+__global__ void k_sdag_keys(const uint32_t* __restrict__ dag, uint64_t n, const uint32_t* __restrict__ childMap,
+                            uint32_t* __restrict__ key, uint8_t* __restrict__ inv) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) sdag_key(dag, i, childMap, key, inv);
 }
 
 enum : uint32_t { kUndecided = 0, kRep = 1, kNonRep = 2 };
@@ -480,6 +569,171 @@ __global__ void k_sdag_outcome(const uint32_t* __restrict__ key, uint64_t n, con
   }
   map[i] = kNone;
   atomicAdd(err, 1u);
+}
+
+// A small level of toSDAG (n <= kSmallLevel) in one workgroup: keys,
+// first-occurrence insert (table and node states in LDS), candidates, the
+// resolution rounds until every node is decided, representative flags and
+// their scan, and the outcome, as k_sdag_keys ... k_sdag_outcome do over
+// several launches.  Every thread reads the round's undecided count after
+// a barrier, so all leave the loop together; each round decides at least
+// the earliest undecided node, so it ends within n rounds.
+__global__ void __launch_bounds__(kSmallThreads) k_sdag_small(const uint32_t* __restrict__ dag, uint32_t n,
+                                                              const uint32_t* __restrict__ childMap,
+                                                              const uint8_t* __restrict__ childInv,
+                                                              uint32_t* __restrict__ key, uint8_t* __restrict__ inv,
+                                                              uint32_t* __restrict__ cand, uint32_t* __restrict__ map,
+                                                              uint32_t* __restrict__ out, uint8_t* __restrict__ invOut,
+                                                              uint32_t* __restrict__ total, uint32_t* __restrict__ err) {
+  __shared__ TEntry table[kSmallCap];
+  __shared__ uint16_t slotOf[kSmallLevel];
+  __shared__ uint32_t state[kSmallLevel];
+  __shared__ uint32_t posOf[kSmallLevel];
+  __shared__ uint32_t lds[kSmallThreads / 64 + 1];
+  __shared__ uint32_t remaining;
+  const uint32_t tid = threadIdx.x;
+  for (uint32_t j = tid; j < kSmallCap; j += kSmallThreads) table[j] = kEmpty;
+  for (uint32_t i = tid; i < n; i += kSmallThreads) sdag_key(dag, i, childMap, key, inv);
+  __syncthreads();
+  // insert (as k_insert)
+  for (uint32_t i = tid; i < n; i += kSmallThreads) {
+    uint32_t w[kRecWords];
+    load_rec(key, i, w);
+    if ((w[0] & 0xFFu) == 0) { slotOf[i] = 0xFFFFu; continue; }
+    const uint64_t hv = hash_rec(w);
+    const uint32_t fp = (uint32_t)(hv >> 32);
+    const TEntry mine = ((TEntry)fp << 32) | (TEntry)i;
+    uint32_t h = (uint32_t)hv & (kSmallCap - 1);
+    while (true) {
+      TEntry e = table[h];
+      if (e == kEmpty) {
+        const TEntry old = atomicCAS(&table[h], kEmpty, mine);
+        if (old == kEmpty) break;
+        e = old;
+      }
+      if ((uint32_t)(e >> 32) == fp && rec_eq(key, (uint32_t)e, w)) {
+        if (mine < e) atomicMin(&table[h], mine);
+        break;
+      }
+      h = (h + 1) & (kSmallCap - 1);
+    }
+    slotOf[i] = (uint16_t)h;
+  }
+  __syncthreads();
+  auto find = [&](const uint32_t w[kRecWords]) -> uint32_t {
+    const uint64_t hv = hash_rec(w);
+    const uint32_t fp = (uint32_t)(hv >> 32);
+    uint32_t h = (uint32_t)hv & (kSmallCap - 1);
+    while (true) {
+      const TEntry e = table[h];
+      if (e == kEmpty) return kNone;
+      if ((uint32_t)(e >> 32) == fp && rec_eq(key, (uint32_t)e, w)) return (uint32_t)e;
+      h = (h + 1) & (kSmallCap - 1);
+    }
+  };
+  // candidates (as k_sdag_cands)
+  for (uint32_t i = tid; i < n; i += kSmallThreads) {
+    uint32_t* cd = cand + (uint64_t)i * 7;
+    const bool first = slotOf[i] != 0xFFFFu && entry_index(table[slotOf[i]]) == i;
+    if (!first) {
+#pragma unroll
+      for (int t = 0; t < 7; ++t) cd[t] = kNone;
+      state[i] = kNonRep;
+      continue;
+    }
+    uint32_t k0[kRecWords], v[kRecWords];
+    load_rec(key, i, k0);
+    const ChildInv ci = child_inv(k0, childInv);
+    bool any = false;
+    for (int t = 1; t < 8; ++t) {
+      make_variant(k0, t, ci, v);
+      const uint32_t r = same_rec(v, k0) ? i : find(v);
+      if (r != kNone && r < i) { cd[t - 1] = r; any = true; }
+      else cd[t - 1] = kNone;
+    }
+    state[i] = any ? kUndecided : kRep;
+  }
+  __syncthreads();
+  // resolution rounds (as k_sdag_resolve), to convergence
+  for (uint32_t round = 0; round <= n; ++round) {
+    if (tid == 0) remaining = 0;
+    __syncthreads();
+    for (uint32_t i = tid; i < n; i += kSmallThreads) {
+      if (__hip_atomic_load(&state[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) != kUndecided) continue;
+      const uint32_t* cd = cand + (uint64_t)i * 7;
+      bool allNon = true, anyRep = false;
+      for (int t = 0; t < 7; ++t) {
+        const uint32_t cc = cd[t];
+        if (cc == kNone) continue;
+        const uint32_t st = __hip_atomic_load(&state[cc], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        if (st == kRep) { anyRep = true; break; }
+        if (st != kNonRep) allNon = false;
+      }
+      if (anyRep) __hip_atomic_store(&state[i], kNonRep, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      else if (allNon) __hip_atomic_store(&state[i], kRep, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      else atomicAdd(&remaining, 1u);
+    }
+    __syncthreads();
+    const uint32_t left = remaining;
+    __syncthreads();
+    if (left == 0) break;
+  }
+  // flags + scan (as k_state_flags + scan_exclusive)
+  uint32_t carry = 0;
+  for (uint32_t base = 0; base < n; base += kSmallThreads) {
+    const uint32_t i = base + tid;
+    const uint32_t f = (i < n && state[i] == kRep) ? 1u : 0u;
+    uint32_t agg;
+    const uint32_t ex = block_exclusive_sum(f, lds, agg) + carry;
+    if (i < n) posOf[i] = ex;
+    carry += agg;
+  }
+  __syncthreads();
+  // outcome (as k_sdag_outcome)
+  for (uint32_t i = tid; i < n; i += kSmallThreads) {
+    const uint32_t st = state[i];
+    const bool first = slotOf[i] != 0xFFFFu && entry_index(table[slotOf[i]]) == i;
+    if (st != kRep && first) {
+      const uint32_t* cd = cand + (uint64_t)i * 7;
+      bool found = false;
+      for (int t = 1; t < 8 && !found; ++t) {
+        const uint32_t r = cd[t - 1];
+        if (r != kNone && state[r] == kRep) {
+          const uint32_t f = kFlip[t];
+          map[i] = posOf[r] | (((f >> 2) & 1u) << 29) | (((f >> 1) & 1u) << 30) | ((f & 1u) << 31);
+          found = true;
+        }
+      }
+      if (!found) { map[i] = kNone; atomicAdd(err, 1u); }
+      continue;
+    }
+    uint32_t k0[kRecWords];
+    load_rec(key, i, k0);
+    if (st == kRep) {
+      const uint32_t p = posOf[i];
+      map[i] = p;
+      uint32_t* o = out + (uint64_t)p * kRecWords;
+#pragma unroll
+      for (int k = 0; k < kRecWords; ++k) o[k] = k0[k];
+      invOut[p] = inv[i];
+      continue;
+    }
+    uint32_t v[kRecWords];
+    const ChildInv ci = child_inv(k0, childInv);
+    bool found = false;
+    for (int t = 0; t < 8 && !found; ++t) {
+      uint32_t r;
+      if (t == 0) r = slotOf[i] != 0xFFFFu ? entry_index(table[slotOf[i]]) : kNone;
+      else { make_variant(k0, t, ci, v); r = find(v); }
+      if (r != kNone && r < i && state[r] == kRep) {
+        const uint32_t f = kFlip[t];
+        map[i] = posOf[r] | (((f >> 2) & 1u) << 29) | (((f >> 1) & 1u) << 30) | ((f & 1u) << 31);
+        found = true;
+      }
+    }
+    if (!found) { map[i] = kNone; atomicAdd(err, 1u); }
+  }
+  if (tid == 0) *total = carry;
 }
 
 // Finest level.  The reference loop walks the nodes in order and makes a node
@@ -703,13 +957,23 @@ uint64_t dedup_all(svd_ctx* c, uint32_t L, const std::vector<Level>& src, std::v
   for (int lev = (int)L - 2; lev >= 0; --lev) {
     const Level& s = src[lev];
     uint32_t* keyed = dalloc<uint32_t>(c, (s.n ? s.n : 1) * kRecWords);
-    if (s.n) k_remap_children<<<div_up(s.n, kB), kB, 0, c->stream>>>(s.rec, s.n, remap, keyed);
+    const bool small = lev != 0 && s.n <= kSmallLevel;   // k_dedup_small remaps the children itself
+    if (s.n && !small) k_remap_children<<<div_up(s.n, kB), kB, 0, c->stream>>>(s.rec, s.n, remap, keyed);
     check_launch("k_remap_children");
     Level& d = dst[lev];
     if (lev == 0) {
       // the root level is never reduced (geom_octree.cpp:483)
       d.rec = keyed;
       d.n = s.n;
+    } else if (small) {
+      uint32_t* nremap = dalloc<uint32_t>(c, s.n + 1);
+      d.rec = dalloc<uint32_t>(c, (s.n ? s.n : 1) * kRecWords);
+      k_dedup_small<<<1, kSmallThreads, 0, c->stream>>>(s.rec, (uint32_t)s.n, remap, keyed, d.rec, nremap,
+                                                        counts + lev);
+      check_launch("k_dedup_small");
+      dfree(c, keyed);
+      dfree(c, remap);
+      remap = nremap;
     } else {
       uint32_t* nremap = dalloc<uint32_t>(c, s.n + 1);
       dedup_level_dev(c, keyed, s.n, nremap, &d.rec, counts + lev);
@@ -780,6 +1044,18 @@ bool reduce_sym_pass(svd_octree* o, bool speculative) {
     const uint8_t* childInv = (lev < (int)L - 1) ? o->sdag[lev + 1].inv : nullptr;
     uint32_t* key = dalloc<uint32_t>(c, (n ? n : 1) * kRecWords);
     uint8_t* inv = dalloc<uint8_t>(c, n + 1);
+    if (lev > 0 && n <= kSmallLevel) {
+      // the whole level in one workgroup (it computes the keys too)
+      uint32_t* cand = dalloc<uint32_t>(c, (n ? n : 1) * 7);
+      s.rec = dalloc<uint32_t>(c, (n ? n : 1) * kRecWords);
+      s.inv = dalloc<uint8_t>(c, n + 1);
+      o->sdagMap[lev] = dalloc<uint32_t>(c, n + 1);
+      k_sdag_small<<<1, kSmallThreads, 0, c->stream>>>(d.rec, (uint32_t)n, o->sdagMap[lev + 1], childInv, key, inv,
+                                                       cand, o->sdagMap[lev], s.rec, s.inv, counts + lev, err);
+      check_launch("k_sdag_small");
+      dfree(c, key); dfree(c, inv); dfree(c, cand);
+      continue;
+    }
     if (n) k_sdag_keys<<<div_up(n, kB), kB, 0, c->stream>>>(d.rec, n, o->sdagMap[lev + 1], key, inv);
     check_launch("k_sdag_keys");
     if (lev == 0) {
