@@ -147,6 +147,12 @@ __global__ void __launch_bounds__(256) k_knn_refine(const uint32_t* __restrict__
   for (int t = 0; t < KM; ++t) { bd[t] = 0xFFFFFFFFu; bi[t] = 0xFFFFFFFFu; }
   auto consider = [&](uint32_t z) {
     if (z == 0xFFFFFFFFu || (uint64_t)z == x || (uint64_t)z >= n) return;
+    // a candidate already kept would be rejected below whatever its
+    // distance: test membership before reading its signature
+    bool kept = false;
+#pragma unroll
+    for (int t = 0; t < KM; ++t) kept |= (t < (int)k) && bi[t] == z;
+    if (kept) return;
     const uint4* pz = reinterpret_cast<const uint4*>(sig + (uint64_t)z * 16);
     uint32_t h = 0;
 #pragma unroll
@@ -161,10 +167,6 @@ __global__ void __launch_bounds__(256) k_knn_refine(const uint32_t* __restrict__
         if (t == (int)k - 1) { wd = bd[t]; wi = bi[t]; }
     }
     if (!(h < wd || (h == wd && z < wi))) return;
-    bool dup = false;
-#pragma unroll
-    for (int t = 0; t < KM; ++t) dup |= (t < (int)k) && bi[t] == z;
-    if (dup) return;
     uint32_t cd = h, ci = z;
 #pragma unroll
     for (int t = 0; t < KM; ++t) {
