@@ -133,59 +133,73 @@ __global__ void __launch_bounds__(kTile) k_window_topk(const float* __restrict__
 // One neighbour-of-neighbour round (NN-descent) on the signature Hamming
 // metric: node x keeps the k smallest distinct (distance, index) among its
 // own list and the lists of its neighbours.  Reads `in`, writes `out`.
+// One wavefront per node: the k + k*k candidate ids are deduplicated through
+// a hash set in LDS, each distinct candidate's signature is read once (by the
+// lane whose insert won), and k wave-wide minimum reductions over
+// (distance << 32 | index) take the k smallest.
 template <int KM>
-__global__ void __launch_bounds__(256) k_knn_refine(const uint32_t* __restrict__ sig, uint64_t n, uint32_t k,
-                                                    const uint32_t* __restrict__ in, uint32_t* __restrict__ out) {
-  const uint64_t x = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (x >= n) return;
-  uint32_t sx[16];
-  const uint4* px = reinterpret_cast<const uint4*>(sig + x * 16);
+__global__ void __launch_bounds__(256) k_knn_refine_wave(const uint32_t* __restrict__ sig, uint64_t n, uint32_t k,
+                                                         const uint32_t* __restrict__ in, uint32_t* __restrict__ out) {
+  constexpr int kCand = KM * (KM + 1);
+  constexpr int kPer = (kCand + 63) / 64;                 // candidates per lane
+  constexpr int kSlots = KM <= 16 ? 1024 : 4096;          // >= 2 * kCand, power of two
+  constexpr int kShift = KM <= 16 ? 22 : 20;              // 32 - log2(kSlots): the product's high bits
+  __shared__ uint32_t set[4][kSlots];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const uint64_t x = (uint64_t)blockIdx.x * 4 + wave;
+  const bool live = x < n;
+  uint32_t* S = set[wave];
+  for (int j = lane; j < kSlots; j += 64) S[j] = 0xFFFFFFFFu;
+  __syncthreads();
+  const uint32_t nc = k + k * k;
+  uint64_t key[kPer];
 #pragma unroll
-  for (int q = 0; q < 4; ++q) { const uint4 t = px[q]; sx[4 * q] = t.x; sx[4 * q + 1] = t.y; sx[4 * q + 2] = t.z; sx[4 * q + 3] = t.w; }
-  uint32_t bd[KM], bi[KM];
-#pragma unroll
-  for (int t = 0; t < KM; ++t) { bd[t] = 0xFFFFFFFFu; bi[t] = 0xFFFFFFFFu; }
-  auto consider = [&](uint32_t z) {
-    if (z == 0xFFFFFFFFu || (uint64_t)z == x || (uint64_t)z >= n) return;
-    // a candidate already kept would be rejected below whatever its
-    // distance: test membership before reading its signature
-    bool kept = false;
-#pragma unroll
-    for (int t = 0; t < KM; ++t) kept |= (t < (int)k) && bi[t] == z;
-    if (kept) return;
+  for (int p = 0; p < kPer; ++p) {
+    key[p] = ~0ull;
+    const uint32_t c = (uint32_t)(lane + 64 * p);
+    if (!live || c >= nc) continue;
+    uint32_t z;
+    if (c < k) {
+      z = in[x * k + c];
+    } else {
+      const uint32_t y = in[x * k + (c - k) / k];
+      z = (y == 0xFFFFFFFFu || (uint64_t)y >= n) ? 0xFFFFFFFFu : in[(uint64_t)y * k + (c - k) % k];
+    }
+    if (z == 0xFFFFFFFFu || (uint64_t)z == x || (uint64_t)z >= n) continue;
+    uint32_t h = (z * 2654435761u) >> kShift;
+    bool mine = false;
+    while (true) {
+      const uint32_t old = atomicCAS(&S[h], 0xFFFFFFFFu, z);
+      if (old == 0xFFFFFFFFu) { mine = true; break; }
+      if (old == z) break;
+      h = (h + 1) & (kSlots - 1);
+    }
+    if (!mine) continue;
+    const uint4* px = reinterpret_cast<const uint4*>(sig + x * 16);
     const uint4* pz = reinterpret_cast<const uint4*>(sig + (uint64_t)z * 16);
-    uint32_t h = 0;
+    uint32_t d = 0;
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-      const uint4 t = pz[q];
-      h += __popc(t.x ^ sx[4 * q]) + __popc(t.y ^ sx[4 * q + 1]) + __popc(t.z ^ sx[4 * q + 2]) + __popc(t.w ^ sx[4 * q + 3]);
+      const uint4 a = px[q], b = pz[q];
+      d += __popc(a.x ^ b.x) + __popc(a.y ^ b.y) + __popc(a.z ^ b.z) + __popc(a.w ^ b.w);
     }
-    uint32_t wd = bd[KM - 1], wi = bi[KM - 1];
-    if (KM != (int)k) {
-#pragma unroll
-      for (int t = 0; t < KM; ++t)
-        if (t == (int)k - 1) { wd = bd[t]; wi = bi[t]; }
-    }
-    if (!(h < wd || (h == wd && z < wi))) return;
-    uint32_t cd = h, ci = z;
-#pragma unroll
-    for (int t = 0; t < KM; ++t) {
-      if (t < (int)k && (cd < bd[t] || (cd == bd[t] && ci < bi[t]))) {
-        const uint32_t td = bd[t], ti = bi[t];
-        bd[t] = cd; bi[t] = ci;
-        cd = td; ci = ti;
-      }
-    }
-  };
-  for (uint32_t t = 0; t < k; ++t) {
-    const uint32_t y = in[x * k + t];
-    consider(y);
-    if (y == 0xFFFFFFFFu || (uint64_t)y >= n) continue;
-    for (uint32_t u = 0; u < k; ++u) consider(in[(uint64_t)y * k + u]);
+    key[p] = ((uint64_t)d << 32) | z;
   }
+  for (uint32_t t = 0; t < k; ++t) {
+    uint64_t m = key[0];
 #pragma unroll
-  for (int t = 0; t < KM; ++t)
-    if (t < (int)k) out[x * k + t] = bi[t];
+    for (int p = 1; p < kPer; ++p) m = key[p] < m ? key[p] : m;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      const uint32_t lo = __shfl_xor((uint32_t)m, off, 64), hi = __shfl_xor((uint32_t)(m >> 32), off, 64);
+      const uint64_t o = ((uint64_t)hi << 32) | lo;
+      m = o < m ? o : m;
+    }
+    if (live && lane == 0) out[x * k + t] = m == ~0ull ? 0xFFFFFFFFu : (uint32_t)m;
+#pragma unroll
+    for (int p = 0; p < kPer; ++p)
+      if (key[p] == m) key[p] = ~0ull;   // candidates are distinct: one lane holds m
+  }
 }
 
 }  // namespace
@@ -195,8 +209,8 @@ void knn_refine(svd_ctx* c, const uint32_t* sig, uint64_t n, uint32_t k, uint32_
   if (k < 1 || k > (uint32_t)kMaxK) throw Error{SVD_ERR_ARG, "knn: k must be 1..32"};
   uint32_t* tmp = dalloc<uint32_t>(c, n * k);
   for (uint32_t r = 0; r < rounds; ++r) {
-    if (k == 16) k_knn_refine<16><<<div_up(n, 256), 256, 0, c->stream>>>(sig, n, k, knn, tmp);
-    else k_knn_refine<kMaxK><<<div_up(n, 256), 256, 0, c->stream>>>(sig, n, k, knn, tmp);
+    if (k <= 16) k_knn_refine_wave<16><<<div_up(n, 4), 256, 0, c->stream>>>(sig, n, k, knn, tmp);
+    else k_knn_refine_wave<kMaxK><<<div_up(n, 4), 256, 0, c->stream>>>(sig, n, k, knn, tmp);
     SVD_HIP(hipMemcpyAsync(knn, tmp, n * k * 4, hipMemcpyDeviceToDevice, c->stream));
   }
   check_launch("k_knn_refine");
