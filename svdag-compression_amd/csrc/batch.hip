@@ -32,12 +32,20 @@ __device__ __forceinline__ uint32_t scene_of(const uint64_t* __restrict__ off, u
   return lo;
 }
 
+// One binary search per workgroup (its first code); a thread then walks the
+// few scene boundaries up to its own code: the last s with offsets[s] <= i,
+// as scene_of gives.
 __global__ void k_tag_codes(const uint64_t* __restrict__ code, uint64_t n, const uint64_t* __restrict__ off,
                             uint32_t S, const uint32_t* __restrict__ levels, uint64_t* __restrict__ tagged,
                             uint32_t* __restrict__ bad) {
-  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  __shared__ uint32_t s0;
+  const uint64_t i0 = (uint64_t)blockIdx.x * blockDim.x;
+  if (threadIdx.x == 0) s0 = scene_of(off, S, i0);
+  __syncthreads();
+  const uint64_t i = i0 + threadIdx.x;
   if (i >= n) return;
-  const uint32_t s = scene_of(off, S, i);
+  uint32_t s = s0;
+  while (s + 1 < S && off[s + 1] <= i) ++s;
   const uint64_t c = code[i];
   if (c >= (1ull << (3 * levels[s])) || (i > off[s] && c <= code[i - 1])) atomicAdd(bad, 1u);
   tagged[i] = ((uint64_t)s << kTagShift) | c;

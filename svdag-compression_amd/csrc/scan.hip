@@ -1,4 +1,4 @@
-// Device-wide exclusive prefix sums (reduce / scan-partials / down-sweep),
+// Device-wide exclusive prefix sums (tile reduce / scan of the tile sums / down-sweep),
 // wave64 shuffles + LDS, 2048 items per 256-thread workgroup.
 #include "svd_internal.hpp"
 #include "block_scan.hpp"
@@ -23,22 +23,6 @@ __global__ void __launch_bounds__(kThreads) k_scan_reduce(const uint32_t* __rest
   }
   s = block_reduce_sum(s, lds);
   if (threadIdx.x == 0) partial[blockIdx.x] = s;
-}
-
-__global__ void __launch_bounds__(kThreads) k_scan_partials(uint32_t* __restrict__ partial, uint32_t np,
-                                                            uint32_t* __restrict__ total) {
-  __shared__ uint32_t lds[kThreads / 64 + 1];
-  uint32_t carry = 0;
-  for (uint32_t base = 0; base < np; base += kThreads) {
-    uint32_t i = base + threadIdx.x;
-    uint32_t v = i < np ? partial[i] : 0u;
-    uint32_t agg;
-    uint32_t ex = block_exclusive_sum(v, lds, agg);
-    if (i < np) partial[i] = carry + ex;
-    carry += agg;
-    __syncthreads();
-  }
-  if (threadIdx.x == 0 && total) *total = carry;
 }
 
 __global__ void __launch_bounds__(kThreads) k_scan_down(const uint32_t* __restrict__ in, uint32_t* __restrict__ out,
@@ -170,7 +154,9 @@ void scan_exclusive(svd_ctx* c, const uint32_t* in, uint32_t* out, uint64_t n, u
   if (nb <= kFuseMax) {
     k_scan_down_fused<<<nb, kThreads, 0, c->stream>>>(in, out, n, partial, total);
   } else {
-    k_scan_partials<<<1, kThreads, 0, c->stream>>>(partial, nb, total);
+    // the tile sums are scanned by the same multi-workgroup scan (at most
+    // three levels deep for 2^32 elements)
+    scan_exclusive(c, partial, partial, nb, total);
     k_scan_down<<<nb, kThreads, 0, c->stream>>>(in, out, n, partial);
   }
   check_launch("scan_exclusive");
