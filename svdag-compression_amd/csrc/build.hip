@@ -105,6 +105,42 @@ __global__ void k_inner_level(const uint64_t* __restrict__ ccode, uint64_t n, co
   if (pcode) pcode[pidx[i]] = p;
 }
 
+// An inner SVO level from at most kSvoSmall child codes in one workgroup:
+// k_heads, the scan of the heads and k_inner_level with workgroup barriers.
+constexpr uint32_t kSvoSmall = 8192;
+__global__ void __launch_bounds__(1024) k_inner_level_small(const uint64_t* __restrict__ ccode, uint32_t n,
+                                                            uint32_t* __restrict__ rec, uint64_t* __restrict__ pcode) {
+  __shared__ uint32_t pidx[kSvoSmall];
+  __shared__ uint32_t lds[1024 / 64 + 1];
+  uint32_t carry = 0;
+  for (uint32_t base = 0; base < n; base += 1024) {
+    const uint32_t i = base + threadIdx.x;
+    const uint32_t h = (i < n && head64(ccode, i)) ? 1u : 0u;
+    uint32_t agg;
+    const uint32_t ex = block_exclusive_sum(h, lds, agg) + carry;
+    if (i < n) pidx[i] = ex;
+    carry += agg;
+  }
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < n; i += 1024) {
+    if (!head64(ccode, i)) continue;
+    const uint64_t p = ccode[i] >> 3;
+    uint32_t w[kRecWords];
+    w[0] = 0;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) w[1 + c] = kNull;
+    for (uint32_t j = i; j < n && j < i + 8 && (ccode[j] >> 3) == p; ++j) {
+      const uint32_t sl = (uint32_t)(ccode[j] & 7u);
+      w[0] |= 1u << sl;
+      w[1 + sl] = j;
+    }
+    uint32_t* o = rec + (uint64_t)pidx[i] * kRecWords;
+#pragma unroll
+    for (int k = 0; k < kRecWords; ++k) o[k] = w[k];
+    if (pcode) pcode[pidx[i]] = p;
+  }
+}
+
 // ---------------------------------------------------------------- hash dedup
 __device__ __forceinline__ void load_rec(const uint32_t* __restrict__ rec, uint64_t i, uint32_t w[kRecWords]) {
   const uint32_t* r = rec + i * kRecWords;
@@ -871,10 +907,22 @@ uint64_t svo_levels_from_codes(svd_ctx* c, const uint64_t* dcodes, uint64_t n, u
   uint64_t nNodes = (L >= 2) ? nl : 0;
   uint64_t nc = nl;
   for (int lev = (int)L - 2; lev >= 0; --lev) {
+    Level& lv = svo[lev];
+    if (sizes && nc <= kSvoSmall) {
+      const uint64_t np = sizes[lev];
+      lv.n = np;
+      lv.rec = dalloc<uint32_t>(c, np * kRecWords);
+      k_inner_level_small<<<1, 1024, 0, c->stream>>>(codeA, (uint32_t)nc, lv.rec, codeB);
+      check_launch("k_inner_level_small");
+      keep((uint32_t)lev, codeB, np);
+      std::swap(codeA, codeB);
+      nc = np;
+      if (lev > 0) nNodes += np;
+      continue;
+    }
     k_heads<<<div_up(nc, kB), kB, 0, c->stream>>>(codeA, nc, scanbuf);
     scan_exclusive(c, scanbuf, scanbuf, nc, total);
     uint64_t np = sizes ? sizes[lev] : read_u32(c, total);
-    Level& lv = svo[lev];
     lv.n = np;
     lv.rec = dalloc<uint32_t>(c, np * kRecWords);
     k_inner_level<<<div_up(nc, kB), kB, 0, c->stream>>>(codeA, nc, scanbuf, lv.rec, codeB);

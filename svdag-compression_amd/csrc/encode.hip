@@ -4,6 +4,7 @@
 //   EncodedSSVDAG  ::encode+save (encoded_ssvdag.cpp:84-466), on the SDAG
 //                  (.ssvdag) or on the DAG (.esvdag, svbuilder/main.cpp:243-244).
 #include "svd_internal.hpp"
+#include "block_scan.hpp"
 #include <algorithm>
 
 namespace svd {
@@ -162,6 +163,48 @@ __global__ void k_inner_sizes(const uint32_t* __restrict__ rec, const uint32_t* 
     else atomicAdd(err, 1u);
   }
   sz[i] = s;
+}
+
+// The layout of a small level (n <= kLayoutSmall) in one workgroup:
+// k_inner_sizes, the exclusive scan of the sizes and k_offsets_to_ind with
+// workgroup barriers instead of launches.
+constexpr uint32_t kLayoutSmall = 8192;
+constexpr int kLayoutThreads = 1024;
+__global__ void __launch_bounds__(kLayoutThreads) k_inner_layout_small(const uint32_t* __restrict__ rec,
+                                                                       const uint32_t* __restrict__ perm, uint32_t n,
+                                                                       const uint32_t* __restrict__ indChild,
+                                                                       uint32_t* __restrict__ offs,
+                                                                       uint32_t* __restrict__ ind,
+                                                                       uint32_t* __restrict__ total,
+                                                                       uint32_t* __restrict__ err) {
+  __shared__ uint32_t lds[kLayoutThreads / 64 + 1];
+  uint32_t carry = 0;
+  for (uint32_t base = 0; base < n; base += kLayoutThreads) {
+    const uint32_t i = base + threadIdx.x;
+    uint32_t sz = 0, p = 0;
+    if (i < n) {
+      p = perm[i];
+      const uint32_t* r = rec + (uint64_t)p * kRecWords;
+      const uint32_t m = r[0] & 0xFFu;
+      sz = 1;
+#pragma unroll
+      for (int c = 0; c < 8; ++c) {
+        if (!((m >> c) & 1u)) continue;
+        const uint32_t a = indChild[r[1 + c]];
+        if (a < (1u << 13)) sz += 1;
+        else if (a < (1u << 30)) sz += 2;
+        else atomicAdd(err, 1u);
+      }
+    }
+    uint32_t agg;
+    const uint32_t ex = block_exclusive_sum(sz, lds, agg) + carry;
+    if (i < n) {
+      offs[i] = ex;
+      ind[p] = ex;
+    }
+    carry += agg;
+  }
+  if (threadIdx.x == 0) *total = carry;
 }
 
 __global__ void k_offsets_to_ind(const uint32_t* __restrict__ perm, const uint32_t* __restrict__ offs, uint64_t n,
@@ -390,6 +433,12 @@ void encode_ssvdag(svd_octree* o, int fmt) {
       check_launch("k_scatter_rank");
     } else {
       offs[lev] = dalloc<uint32_t>(c, n + 1);
+      if (n && n <= kLayoutSmall) {
+        k_inner_layout_small<<<1, kLayoutThreads, 0, c->stream>>>(lv[lev].rec, perm[lev], (uint32_t)n, ind[lev + 1],
+                                                                  offs[lev], ind[lev], dev + lev, err);
+        check_launch("k_inner_layout_small");
+        continue;
+      }
       if (n) {
         k_inner_sizes<<<div_up(n, kB), kB, 0, c->stream>>>(lv[lev].rec, perm[lev], n, ind[lev + 1], offs[lev], err);
         check_launch("k_inner_sizes");
