@@ -260,6 +260,18 @@ def attr_bench(S, W, ctx, levels, log):
     log(f"[bench] attr scene {1 << levels}^3: {u.n} voxels ({time.perf_counter() - t0:.1f} s)")
     out = {"workload": f"fBm terrain (seed 1) + caves (seed 3) {1 << levels}^3 ({levels} levels, {u.n} voxels), "
                        f"u8 materials (value noise seed 4, 16 levels)", "voxels": int(u.n)}
+    # the full lossless build (the headline step's build region) at this size
+    tb = []
+    for _ in range(4):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        g = build_step(S, ctx, u.ptr, u.n, levels)
+        ctx.sync()
+        tb.append(time.perf_counter() - t)
+        g.free()
+    bt = float(np.median(tb[1:]))
+    out["lossless_build_ms"] = round(bt * 1e3, 2)
+    out["lossless_build_gvox_s"] = round(u.n / bt / 1e9, 3)
     g = S.Octree.from_morton(ctx, (u.ptr, u.n), levels)
     g.to_dag()
     out["nodes_dag"] = int(g.counts()[0])
@@ -500,6 +512,10 @@ def main():
     if not args.no_cpu_baseline:
         log("[bench] cpu baseline (oracle) ...")
         out["cpu_baseline"] = cpu_baseline(W, L, args.cpu_sample_levels)
+        if "cross_level_attr" in out:
+            # the CPU rate is measured on the 2048^3 terrain sample
+            a = out["cross_level_attr"]
+            a["lossless_build_vs_cpu_rate"] = round(a["lossless_build_gvox_s"] / out["cpu_baseline"]["value"], 1)
     print(json.dumps(out), flush=True)
 
 
