@@ -219,6 +219,9 @@ __device__ __forceinline__ uint32_t lookup(const uint32_t* __restrict__ rec, con
 // each key's slot ends with its smallest index whatever the probe sequence.
 constexpr uint32_t kSmallLevel = 4096;
 constexpr uint32_t kSmallCap = 8192;        // power of two >= 2 * kSmallLevel
+// toSDAG's one-workgroup path pays a long serial chain per node it handles:
+// measured on the 2048^3 terrain it wins up to one node per thread only
+constexpr uint32_t kSdagSmall = 1024;
 constexpr int kSmallThreads = 1024;
 __global__ void __launch_bounds__(kSmallThreads) k_dedup_small(const uint32_t* __restrict__ src, uint32_t n,
                                                                const uint32_t* __restrict__ remapIn,
@@ -1092,7 +1095,7 @@ bool reduce_sym_pass(svd_octree* o, bool speculative) {
     const uint8_t* childInv = (lev < (int)L - 1) ? o->sdag[lev + 1].inv : nullptr;
     uint32_t* key = dalloc<uint32_t>(c, (n ? n : 1) * kRecWords);
     uint8_t* inv = dalloc<uint8_t>(c, n + 1);
-    if (lev > 0 && n <= kSmallLevel) {
+    if (lev > 0 && n <= kSdagSmall) {
       // the whole level in one workgroup (it computes the keys too)
       uint32_t* cand = dalloc<uint32_t>(c, (n ? n : 1) * 7);
       s.rec = dalloc<uint32_t>(c, (n ? n : 1) * kRecWords);
