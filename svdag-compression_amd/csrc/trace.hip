@@ -144,9 +144,13 @@ struct Tracer {
       }
       return;
     }
-    int off = 1 + (int)s.levelOffsets[ts.level] + ts.node_index;
-    // childIndir table (octree_dda_renderer.cpp:533-561), computed in registers
-    for (int i = 7; i > (int)ts.cli; --i) { const int m = (int)((ts.hdr >> (2 * i)) & 3u); off += m < 2 ? m : 2; }
+    // childIndir table (octree_dda_renderer.cpp:533-561), computed in registers:
+    // children i > cli take min(code_i, 2) words, i.e. one per non-empty code
+    // plus one more per code with its high bit set
+    const uint32_t lo = ts.hdr & 0x5555u, hi = (ts.hdr >> 1) & 0x5555u;
+    const uint32_t above = 0xFFFFu & ~((2u << (2 * ts.cli + 1)) - 1u);   // fields cli+1 .. 7
+    int off = 1 + (int)s.levelOffsets[ts.level] + ts.node_index +
+              __popc((lo | hi) & above) + __popc(hi & above);
     const int cm = (int)((ts.hdr >> (ts.cli << 1)) & 3u);
     int cx = (int)half((uint32_t)off);
     bytes += 2;
