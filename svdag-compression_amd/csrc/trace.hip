@@ -106,11 +106,15 @@ struct Tracer {
     ts.mm0 = (int)(a & 1u); ts.mm1 = (int)((a >> 1) & 1u); ts.mm2 = (int)((a >> 2) & 1u);
     ts.level = (a >> 3) & 255u;
   }
+  // voxel_to_linear_idx: (1 - 2m) li + m (sz - 1) per axis, then i2 + sz (i1 + sz i0).
+  // sz is 2 or 4 and li is always in [0, sz) here, so the mirrored index is
+  // li ^ (sz - 1) and the products are shifts.
   __device__ __forceinline__ static uint32_t lin_idx(const TS& ts, int sz) {
-    const int i0 = (1 - 2 * ts.mm0) * ts.li[0] + ts.mm0 * (sz - 1);
-    const int i1 = (1 - 2 * ts.mm1) * ts.li[1] + ts.mm1 * (sz - 1);
-    const int i2 = (1 - 2 * ts.mm2) * ts.li[2] + ts.mm2 * (sz - 1);
-    return (uint32_t)(i2 + sz * (i1 + sz * i0));
+    const int hm = sz - 1, sh = sz == 4 ? 2 : 1;
+    const int i0 = ts.li[0] ^ (ts.mm0 ? hm : 0);
+    const int i1 = ts.li[1] ^ (ts.mm1 ? hm : 0);
+    const int i2 = ts.li[2] ^ (ts.mm2 ? hm : 0);
+    return (uint32_t)((((i0 << sh) | i1) << sh) | i2);
   }
   __device__ __forceinline__ bool fetch_voxel_bit(const TS& ts) const {
     if (FMT != SVD_FMT_SSVDAG) return (ts.hdr & (1u << ts.cli)) != 0;
@@ -182,7 +186,7 @@ struct Tracer {
     int m[3];
     next_mask(ts, m);
 #pragma unroll
-    for (int a = 0; a < 3; ++a) { delta[a] = m[a] * ts.di[a]; ts.idx[a] += delta[a]; ts.li[a] += delta[a]; }
+    for (int a = 0; a < 3; ++a) { delta[a] = m[a] ? ts.di[a] : 0; ts.idx[a] += delta[a]; ts.li[a] += delta[a]; }
     float tc = (float)m[0] * ts.tn[0];
     tc = tc + (float)m[1] * ts.tn[1];
     tc = tc + (float)m[2] * ts.tn[2];
@@ -191,7 +195,7 @@ struct Tracer {
     for (int a = 0; a < 3; ++a) ts.tn[a] = ts.tn[a] + ((float)m[a] * ts.cell) * fabsf(ts.inv[a]);
   }
   __device__ __forceinline__ static bool in_bounds(const int li[3], int sz) {
-    return li[0] < sz && li[1] < sz && li[2] < sz && 0 <= li[0] && 0 <= li[1] && 0 <= li[2];
+    return (uint32_t)li[0] < (uint32_t)sz && (uint32_t)li[1] < (uint32_t)sz && (uint32_t)li[2] < (uint32_t)sz;
   }
   __device__ __forceinline__ void up_in(const float o[3], TS& ts) {
     uint32_t dl = ts.level;
@@ -223,7 +227,7 @@ struct Tracer {
     next_mask(ts, m);
     int nl[3];
 #pragma unroll
-    for (int a = 0; a < 3; ++a) nl[a] = ts.li[a] + m[a] * ts.di[a];
+    for (int a = 0; a < 3; ++a) nl[a] = ts.li[a] + (m[a] ? ts.di[a] : 0);
     if (in_bounds(nl, 2)) push(ts.node_index, ts.hdr, ts.mm0, ts.mm1, ts.mm2, ts.level);
     fetch_child_index_in(ts);
     go_down_one_level(o, d, ts);
