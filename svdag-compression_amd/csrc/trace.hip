@@ -182,11 +182,12 @@ struct Tracer {
     m[1] = (ts.tn[1] < ts.tn[2]) & (ts.tn[1] <= ts.tn[0]);
     m[2] = (ts.tn[2] < ts.tn[0]) & (ts.tn[2] <= ts.tn[1]);
   }
-  __device__ __forceinline__ void dda_next(TS& ts, int delta[3]) {
-    int m[3];
-    next_mask(ts, m);
+  // The DDA step of the current cell: m, the axis whose plane is crossed
+  // first, and delta = m * di.  An empty cell takes the step (dda_next); a
+  // full one only tests whether li + delta stays in its node (down_in).
+  __device__ __forceinline__ void dda_next(TS& ts, const int m[3], const int delta[3]) {
 #pragma unroll
-    for (int a = 0; a < 3; ++a) { delta[a] = m[a] ? ts.di[a] : 0; ts.idx[a] += delta[a]; ts.li[a] += delta[a]; }
+    for (int a = 0; a < 3; ++a) { ts.idx[a] += delta[a]; ts.li[a] += delta[a]; }
     float tc = (float)m[0] * ts.tn[0];
     tc = tc + (float)m[1] * ts.tn[1];
     tc = tc + (float)m[2] * ts.tn[2];
@@ -222,13 +223,10 @@ struct Tracer {
 #pragma unroll
     for (int a = 0; a < 3; ++a) ts.li[a] = ts.idx[a] & (ts.cns - 1);
   }
-  __device__ __forceinline__ void down_in(const float o[3], const float d[3], TS& ts) {
-    int m[3];
-    next_mask(ts, m);
-    int nl[3];
-#pragma unroll
-    for (int a = 0; a < 3; ++a) nl[a] = ts.li[a] + (m[a] ? ts.di[a] : 0);
-    if (in_bounds(nl, 2)) push(ts.node_index, ts.hdr, ts.mm0, ts.mm1, ts.mm2, ts.level);
+  // next_in_node: li + delta is inside the current node (a full cell below the
+  // leaf level is in a node of side 2).
+  __device__ __forceinline__ void down_in(const float o[3], const float d[3], TS& ts, bool next_in_node) {
+    if (next_in_node) push(ts.node_index, ts.hdr, ts.mm0, ts.mm1, ts.mm2, ts.level);
     fetch_child_index_in(ts);
     go_down_one_level(o, d, ts);
     if ((int)ts.level == s.innerLevels) {
@@ -319,9 +317,16 @@ struct Tracer {
   __device__ __forceinline__ bool step(Ray& r, float out[4]) {
     TS& ts = r.ts;
     const bool full = fetch_voxel_bit(ts);
+    int m[3], delta[3], nl[3];
+    next_mask(ts, m);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) { delta[a] = m[a] ? ts.di[a] : 0; nl[a] = ts.li[a] + delta[a]; }
+    const bool next_in_node = in_bounds(nl, ts.cns);
     if (!full) {
-      dda_next(ts, r.step);
-      if (!in_bounds(ts.li, ts.cns)) {
+#pragma unroll
+      for (int a = 0; a < 3; ++a) r.step[a] = delta[a];
+      dda_next(ts, m, delta);
+      if (!next_in_node) {
         if (st.size == 0) { out[0] = -1.f; out[1] = 0.f; out[2] = (float)r.it; out[3] = -1.f; return true; }
         up_in(r.o, ts);
       }
@@ -332,7 +337,7 @@ struct Tracer {
         out[3] = (float)ts.node_index;
         return true;
       }
-      down_in(r.o, r.d, ts);
+      down_in(r.o, r.d, ts, next_in_node);
       fetch_data(ts);
     }
     ts.cli = lin_idx(ts, ts.cns);
