@@ -32,9 +32,9 @@ __device__ __forceinline__ int isign(float x) { return x > 0.f ? 1 : (x < 0.f ? 
 struct TS {
   float t_current;
   int node_index;
-  uint32_t hdr;
-  int mm0, mm1, mm2;
-  uint32_t leaf_lo, leaf_hi;
+  uint32_t hdr;       // node header; an SSVDAG leaf's low mask word
+  uint32_t mm;        // mirror flags, bit a for axis a
+  uint32_t leaf_hi;   // an SSVDAG leaf's high mask word
   int idx[3];
   int li[3];
   uint32_t cli;
@@ -89,8 +89,8 @@ struct Tracer {
 
   // SSVDAG inner headers are 16 bits; of an SVDAG header word only the mask
   // (bits 0-7) is ever read, so both fit beside the mirror bits and level.
-  __device__ __forceinline__ void push(int node, uint32_t hdr, int m0, int m1, int m2, uint32_t level) {
-    const uint32_t mask = (uint32_t)(m0 | (m1 << 1) | (m2 << 2)) | (level << 3);
+  __device__ __forceinline__ void push(int node, uint32_t hdr, uint32_t mm, uint32_t level) {
+    const uint32_t mask = mm | (level << 3);
     const uint32_t o = st.size * kBT + st.lane;
     st.node[o] = (uint32_t)node;
     if (kStackWords<FMT> == 3) { st.aux[o] = mask; st.hdr[o] = hdr; }
@@ -103,7 +103,7 @@ struct Tracer {
     ts.node_index = (int)st.node[o];
     const uint32_t a = st.aux[o];
     ts.hdr = kStackWords<FMT> == 3 ? st.hdr[o] : (a >> 16);
-    ts.mm0 = (int)(a & 1u); ts.mm1 = (int)((a >> 1) & 1u); ts.mm2 = (int)((a >> 2) & 1u);
+    ts.mm = a & 7u;
     ts.level = (a >> 3) & 255u;
   }
   // voxel_to_linear_idx: (1 - 2m) li + m (sz - 1) per axis, then i2 + sz (i1 + sz i0).
@@ -111,9 +111,9 @@ struct Tracer {
   // li ^ (sz - 1) and the products are shifts.
   __device__ __forceinline__ static uint32_t lin_idx(const TS& ts, int sz) {
     const int hm = sz - 1, sh = sz == 4 ? 2 : 1;
-    const int i0 = ts.li[0] ^ (ts.mm0 ? hm : 0);
-    const int i1 = ts.li[1] ^ (ts.mm1 ? hm : 0);
-    const int i2 = ts.li[2] ^ (ts.mm2 ? hm : 0);
+    const int i0 = ts.li[0] ^ ((ts.mm & 1u) ? hm : 0);
+    const int i1 = ts.li[1] ^ ((ts.mm & 2u) ? hm : 0);
+    const int i2 = ts.li[2] ^ ((ts.mm & 4u) ? hm : 0);
     return (uint32_t)((((i0 << sh) | i1) << sh) | i2);
   }
   __device__ __forceinline__ bool fetch_voxel_bit(const TS& ts) const {
@@ -121,7 +121,7 @@ struct Tracer {
     if ((int)ts.level < s.innerLevels) return ((ts.hdr >> (ts.cli << 1)) & 3u) != 0;
     const uint32_t word = ts.cli & 32u;
     const uint32_t bm = 1u << (ts.cli & 31u);
-    return ((word == 0 ? ts.leaf_lo : ts.leaf_hi) & bm) != 0;
+    return ((word == 0 ? ts.hdr : ts.leaf_hi) & bm) != 0;
   }
   __device__ __forceinline__ void fetch_data(TS& ts) {
     if (FMT != SVD_FMT_SSVDAG) { ts.hdr = word((uint32_t)ts.node_index); bytes += 4; return; }
@@ -133,7 +133,7 @@ struct Tracer {
       if ((uint32_t)ts.node_index < s.nLeaves) v = s.leaves[(uint32_t)ts.node_index];
       else bad = true;
       bytes += 8;
-      ts.leaf_lo = v.x; ts.leaf_hi = v.y;
+      ts.hdr = v.x; ts.leaf_hi = v.y;
     }
   }
   __device__ __forceinline__ void fetch_child_index_in(TS& ts) {
@@ -142,9 +142,8 @@ struct Tracer {
       ts.node_index = (int)word((uint32_t)(ts.node_index + pos));
       bytes += 4;
       if (FMT == SVD_FMT_USSVDAG) {
-        ts.mm0 ^= (ts.hdr & (1u << (ts.cli + 8))) != 0;
-        ts.mm1 ^= (ts.hdr & (1u << (ts.cli + 16))) != 0;
-        ts.mm2 ^= (ts.hdr & (1u << (ts.cli + 24))) != 0;
+        const uint32_t b = ts.hdr >> ts.cli;
+        ts.mm ^= ((b >> 8) & 1u) | ((b >> 15) & 2u) | ((b >> 22) & 4u);
       }
       return;
     }
@@ -158,7 +157,7 @@ struct Tracer {
     const int cm = (int)((ts.hdr >> (ts.cli << 1)) & 3u);
     int cx = (int)half((uint32_t)off);
     bytes += 2;
-    const int m0 = (cx >> 13) & 1, m1 = (cx >> 14) & 1, m2 = (cx >> 15) & 1;
+    const uint32_t mf = ((uint32_t)cx >> 13) & 7u;
     cx &= -57345;
     if (cm > 1) {
       const int cy = (int)half((uint32_t)off + 1);
@@ -166,7 +165,7 @@ struct Tracer {
       cx = (cx << 16) | cy;
       cx |= (cm & 1) << 29;
     }
-    ts.mm0 ^= m0; ts.mm1 ^= m1; ts.mm2 ^= m2;
+    ts.mm ^= mf;
     ts.node_index = cx;
   }
   __device__ __forceinline__ static void next_crossing(TS& ts, const float o[3]) {
@@ -226,7 +225,7 @@ struct Tracer {
   // next_in_node: li + delta is inside the current node (a full cell below the
   // leaf level is in a node of side 2).
   __device__ __forceinline__ void down_in(const float o[3], const float d[3], TS& ts, bool next_in_node) {
-    if (next_in_node) push(ts.node_index, ts.hdr, ts.mm0, ts.mm1, ts.mm2, ts.level);
+    if (next_in_node) push(ts.node_index, ts.hdr, ts.mm, ts.level);
     fetch_child_index_in(ts);
     go_down_one_level(o, d, ts);
     if ((int)ts.level == s.innerLevels) {
@@ -245,14 +244,18 @@ struct Tracer {
     float tmax, projf;
     int it;
     uint32_t maxLevel, maxIters;
-    int step[3];
+    uint32_t stepm;   // axis mask of the last DDA step; the step is mask * di
+    __device__ __forceinline__ void steps(int st[3]) const {
+#pragma unroll
+      for (int a = 0; a < 3; ++a) st[a] = (stepm >> a) & 1u ? ts.di[a] : 0;
+    }
   };
 
   // Setup of traceRay; returns true when the ray is already finished (out set).
   __device__ __forceinline__ bool begin(Ray& r, const float oin[3], const float din[3], float tmin, float tmax,
                                         float projf, uint32_t maxIters, uint32_t drawLevel, float out[4]) {
     st.size = 0;
-    r.step[0] = r.step[1] = r.step[2] = 0;
+    r.stepm = 0;
     const float eps = 1e-4f;
     const float scale = 1.0f / (2.0f * s.rootHalfSide);
     float omin[3];
@@ -286,10 +289,10 @@ struct Tracer {
     ts.t_current = tmin;
 #pragma unroll
     for (int a = 0; a < 3; ++a) { ts.inv[a] = 1.0f / r.d[a]; ts.di[a] = isign(r.d[a]); }
-    ts.mm0 = ts.mm1 = ts.mm2 = 0;
+    ts.mm = 0;
     ts.level = 0;
     ts.cell = 0.5f;
-    ts.leaf_lo = ts.leaf_hi = 0;
+    ts.leaf_hi = 0;
     {
       const float voxel_eps = 1.0f / (256.f * 1024.f);
       const float tt = ts.t_current + voxel_eps;
@@ -323,8 +326,7 @@ struct Tracer {
     for (int a = 0; a < 3; ++a) { delta[a] = m[a] ? ts.di[a] : 0; nl[a] = ts.li[a] + delta[a]; }
     const bool next_in_node = in_bounds(nl, ts.cns);
     if (!full) {
-#pragma unroll
-      for (int a = 0; a < 3; ++a) r.step[a] = delta[a];
+      r.stepm = (uint32_t)(m[0] | (m[1] << 1) | (m[2] << 2));
       dda_next(ts, m, delta);
       if (!next_in_node) {
         if (st.size == 0) { out[0] = -1.f; out[1] = 0.f; out[2] = (float)r.it; out[3] = -1.f; return true; }
@@ -354,7 +356,7 @@ struct Tracer {
     if (!begin(r, oin, din, tmin, tmax, projf, maxIters, drawLevel, out)) {
       while (!step(r, out)) {}
     }
-    step_out[0] = r.step[0]; step_out[1] = r.step[1]; step_out[2] = r.step[2];
+    r.steps(step_out);
   }
 };
 
@@ -808,7 +810,9 @@ k_trace_persist(Dev s, svd_trace_params p, svd_ray_result* __restrict__ prim, sv
       camera_ray(p, px, py, (float)p.width, (float)p.height, o, d);
       float tmin = 0.0f, tmax = 1e30f, projf = p.projection_factor;
       if (state == kWantShadow) {   // from the primary's result and steps
-        const ShadowRay sh = shadow_ray(s, p, o, d, out, ray.step);
+        int st[3];
+        ray.steps(st);
+        const ShadowRay sh = shadow_ray(s, p, o, d, out, st);
 #pragma unroll
         for (int a = 0; a < 3; ++a) { o[a] = sh.o[a]; d[a] = sh.d[a]; }
         tmin = sh.tmin;
