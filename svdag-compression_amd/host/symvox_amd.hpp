@@ -64,8 +64,9 @@ class Scene {
   // Scene::loadObj (scene.cpp:51-268): same line parser and quirks (face
   // tokens split on ' ', the last token loses its final character, a face
   // is split into two triangles only when it has more than 4 tokens, the
-  // box merges all vertices).  The binary cache of the reference is not
-  // written.
+  // box merges all vertices).  The reference's `.bincache` file
+  // (scene.cpp:55-71) is neither read nor written: tryLoadBinCache is
+  // accepted and ignored, so every call parses the OBJ text.
   void loadObj(std::string fileName, bool tryLoadBinCache = true, bool loadMaterials = false,
                bool loadNormals = false, bool recomputeNormals = false, bool doBuildTriVector = true);
   // ASCII PLY with "vertex" (x y z floats) and "face" (vertex_indices lists).
