@@ -148,12 +148,16 @@ __device__ __forceinline__ void load_rec(const uint32_t* __restrict__ rec, uint6
   for (int k = 0; k < kRecWords; ++k) w[k] = r[k];
 }
 
+// Record j is loaded whole before it is compared: a fingerprint match is
+// almost always an equal key, and nine loads issued together cost one memory
+// round trip where an early exit per word would chain nine.
 __device__ __forceinline__ bool rec_eq(const uint32_t* __restrict__ rec, uint32_t j, const uint32_t w[kRecWords]) {
-  const uint32_t* r = rec + (uint64_t)j * kRecWords;
+  uint32_t r[kRecWords];
+  load_rec(rec, j, r);
+  uint32_t diff = 0;
 #pragma unroll
-  for (int k = 0; k < kRecWords; ++k)
-    if (r[k] != w[k]) return false;
-  return true;
+  for (int k = 0; k < kRecWords; ++k) diff |= r[k] ^ w[k];
+  return diff == 0;
 }
 
 // Open-addressing table of node indices.  An entry is (fingerprint << 32 |
