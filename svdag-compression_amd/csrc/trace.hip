@@ -209,7 +209,11 @@ struct Tracer {
     for (int a = 0; a < 3; ++a) ts.li[a] = ts.idx[a] & 1;
     next_crossing(ts, o);
   }
-  __device__ __forceinline__ void go_down_one_level(const float o[3], const float d[3], TS& ts) {
+  // The position step of go_down_one_level (octree_dda.frag.glsl:434-451).
+  // Its next-crossing and local-index updates read nothing this step reads,
+  // so a descent that steps several levels (into a leaf brick,
+  // octree_dda.frag.glsl:466-476) computes them once, at its end.
+  __device__ __forceinline__ void step_down(const float o[3], const float d[3], TS& ts) {
     ++ts.level;
     ts.cell *= 0.5f;
 #pragma unroll
@@ -218,21 +222,21 @@ struct Tracer {
       const float pc = (float)(ts.idx[a] * 2 + 1) * ts.cell;
       ts.idx[a] = ts.idx[a] * 2 + (pc < pa ? 1 : 0);
     }
-    next_crossing(ts, o);
-#pragma unroll
-    for (int a = 0; a < 3; ++a) ts.li[a] = ts.idx[a] & (ts.cns - 1);
   }
   // next_in_node: li + delta is inside the current node (a full cell below the
   // leaf level is in a node of side 2).
   __device__ __forceinline__ void down_in(const float o[3], const float d[3], TS& ts, bool next_in_node) {
     if (next_in_node) push(ts.node_index, ts.hdr, ts.mm, ts.level);
     fetch_child_index_in(ts);
-    go_down_one_level(o, d, ts);
+    step_down(o, d, ts);
     if ((int)ts.level == s.innerLevels) {
       ts.cns = s.leafSize;
       int vc = s.leafSize / 2;
-      while (vc > 1) { go_down_one_level(o, d, ts); vc >>= 1; }
+      while (vc > 1) { step_down(o, d, ts); vc >>= 1; }
     }
+    next_crossing(ts, o);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) ts.li[a] = ts.idx[a] & (ts.cns - 1);
     path = (path ^ (uint32_t)ts.node_index) * 16777619u;
     path = (path ^ ts.level) * 16777619u;
   }
