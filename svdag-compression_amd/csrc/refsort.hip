@@ -477,6 +477,24 @@ __global__ void __launch_bounds__(kBT) k_block_sort(const Seg* __restrict__ segs
   for (uint32_t k = tid; k < n; k += kBT) { val[g.first + k] = sv[k]; idx[g.first + k] = si[k]; }
 }
 
+// The initial segment lists and their counters, written from the kernel
+// arguments in one launch instead of one copy per list: roots of the big,
+// small and leaf lists in that order, n = the four list sizes (heap empty).
+constexpr int kSeedMax = 48;
+struct Seeds {
+  Seg seg[kSeedMax];
+  uint32_t n[4];
+};
+__global__ void k_seed_lists(Seeds sd, Seg* __restrict__ big, Seg* __restrict__ small, Seg* __restrict__ leaf,
+                             uint32_t* __restrict__ counters) {
+  const uint32_t t = threadIdx.x;
+  const uint32_t n0 = sd.n[0], n1 = sd.n[1], n3 = sd.n[3];
+  if (t < n0) big[t] = sd.seg[t];
+  else if (t < n0 + n1) small[t - n0] = sd.seg[t];
+  else if (t < n0 + n1 + n3) leaf[t - n0 - n1] = sd.seg[t];
+  if (t < 8) counters[t] = t < 4 ? sd.n[t] : 0u;
+}
+
 }  // namespace
 
 void refsort_ranges(svd_ctx* c, uint32_t* idx, uint32_t* val, uint64_t n, const uint64_t* rfirst,
@@ -509,12 +527,22 @@ void refsort_ranges(svd_ctx* c, uint32_t* idx, uint32_t* val, uint64_t n, const 
   uint32_t* counters = dalloc<uint32_t>(c, 8);
   L.nbig = counters + 0; L.nsmall = counters + 1; L.nheap = counters + 2; L.nleaf = counters + 3;
   uint32_t hcnt[8] = {(uint32_t)roots[0].size(), (uint32_t)roots[1].size(), 0, (uint32_t)roots[3].size(), 0, 0, 0, 0};
-  Seg* dstOf[4] = {bigA, L.small, L.heap, L.leaf};
-  for (int k = 0; k < 4; ++k)
-    if (!roots[k].empty())
-      SVD_HIP(hipMemcpyAsync(dstOf[k], roots[k].data(), sizeof(Seg) * roots[k].size(), hipMemcpyHostToDevice,
-                             c->stream));
-  SVD_HIP(hipMemcpyAsync(counters, hcnt, sizeof(hcnt), hipMemcpyHostToDevice, c->stream));
+  if (roots[0].size() + roots[1].size() + roots[3].size() <= (size_t)kSeedMax) {
+    Seeds sd{};
+    size_t k = 0;
+    for (int l : {0, 1, 3})
+      for (const Seg& g : roots[l]) sd.seg[k++] = g;
+    for (int l = 0; l < 4; ++l) sd.n[l] = hcnt[l];
+    k_seed_lists<<<1, 64, 0, c->stream>>>(sd, bigA, L.small, L.leaf, counters);
+    check_launch("k_seed_lists");
+  } else {
+    Seg* dstOf[4] = {bigA, L.small, L.heap, L.leaf};
+    for (int k = 0; k < 4; ++k)
+      if (!roots[k].empty())
+        SVD_HIP(hipMemcpyAsync(dstOf[k], roots[k].data(), sizeof(Seg) * roots[k].size(), hipMemcpyHostToDevice,
+                               c->stream));
+    SVD_HIP(hipMemcpyAsync(counters, hcnt, sizeof(hcnt), hipMemcpyHostToDevice, c->stream));
+  }
   if (hcnt[0]) {
     uint32_t* Lpos = dalloc<uint32_t>(c, n);
     uint32_t* Rpos = dalloc<uint32_t>(c, n);

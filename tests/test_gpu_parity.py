@@ -210,6 +210,25 @@ def test_refcount_order_ranges_equal_separate_sorts(ctx):
         np.testing.assert_array_equal(gi[a:b], wi)
 
 
+def test_refcount_order_ranges_many(ctx):
+    """More ranges than the initial lists can take from kernel arguments (48):
+    the lists are then uploaded by copies.  Each range still equals its own
+    std::sort."""
+    rng = np.random.default_rng(13)
+    sizes = list(rng.integers(0, 3000, 66)) + [20000, 1, 17, 2048]
+    parts = [np.minimum(rng.geometric(0.4, int(s)), 30).astype(np.uint32) for s in sizes]
+    vals = np.concatenate(parts)
+    idx = rng.permutation(vals.size).astype(np.uint32)
+    bounds = np.cumsum([0] + [p.size for p in parts])
+    ranges = list(zip(bounds[:-1], bounds[1:]))
+    assert len(ranges) > 48
+    gi, gv = ctx.refcount_order_ranges(idx, vals, ranges)
+    for a, b in ranges:
+        wi, wv = O.std_sort_refcount(idx[a:b], vals[a:b])
+        np.testing.assert_array_equal(gv[a:b], wv)
+        np.testing.assert_array_equal(gi[a:b], wi)
+
+
 def test_refcount_order_ranges_bad_range(ctx):
     idx = np.arange(10, dtype=np.uint32)
     with pytest.raises(S.SvdError):
