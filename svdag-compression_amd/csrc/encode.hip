@@ -134,10 +134,17 @@ __global__ void __launch_bounds__(256) k_refcount(LevelSet a, uint32_t* __restri
     if (key[t] != kNone && cnt[t]) atomicAdd(&hist[key[t]], cnt[t]);
 }
 
-// perm[i] = position of element i within its level (the identity order of every level)
-__global__ void k_level_iota(LevelSet a, uint64_t n, uint32_t* __restrict__ perm) {
+// perm[i] = position of element i within its level (the identity order of
+// every level).  The same launch clears the refcount histogram (n words) and
+// the nz counters in zero, which would otherwise take two fill launches.
+__global__ void k_level_iota(LevelSet a, uint64_t n, uint32_t* __restrict__ perm, uint32_t* __restrict__ hist,
+                             uint32_t* __restrict__ zero, uint32_t nz) {
   uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) perm[i] = (uint32_t)(i - a.start[level_of(a, i)]);
+  if (i < n) {
+    perm[i] = (uint32_t)(i - a.start[level_of(a, i)]);
+    hist[i] = 0u;
+  }
+  if (i < nz) zero[i] = 0u;
 }
 
 __global__ void k_scatter_rank(const uint32_t* __restrict__ perm, uint64_t n, uint32_t* __restrict__ ind) {
@@ -393,16 +400,14 @@ void encode_ssvdag(svd_octree* o, int fmt) {
   std::vector<uint32_t> innerSize(L, 0);
   // dev[0..L-1]: inner-size totals per level, dev[L]: pointer-size error flag
   uint32_t* dev = dalloc<uint32_t>(c, L + 1);
-  uint32_t* err = dev + L;
-  fill_u32(c, dev, 0u, L + 1);
+  uint32_t* err = dev + L;   // dev is cleared by k_level_iota below
   // Phase 1a: the reference counts of every level land in one array, and one
   // segmented sort orders all levels (each level is its own std::sort call).
   std::vector<uint64_t> off(L, 0), rfirst, rlast;
   uint64_t tot = 0;
   for (int lev = 0; lev <= L - 2; ++lev) { off[lev] = tot; tot += lv[lev].n; }
   uint32_t* permAll = dalloc<uint32_t>(c, tot + 1);
-  uint32_t* hist = dalloc<uint32_t>(c, tot + 1);
-  fill_u32(c, hist, 0u, tot);
+  uint32_t* hist = dalloc<uint32_t>(c, tot + 1);   // cleared by k_level_iota
   // levels 0..L-2 as one concatenation (start = off)
   std::vector<Level> upper(lv.begin(), lv.begin() + (L - 1));
   const LevelSet ua = level_set(upper, (uint32_t)(L - 1), false);
@@ -415,7 +420,8 @@ void encode_ssvdag(svd_octree* o, int fmt) {
     }
     if (lev <= L - 3) maxParent = std::max<uint64_t>(maxParent, lv[lev].n);
   }
-  if (tot) k_level_iota<<<div_up(tot, kB), kB, 0, c->stream>>>(ua, tot, permAll);
+  k_level_iota<<<div_up(std::max<uint64_t>(tot, (uint64_t)L + 1), kB), kB, 0, c->stream>>>(ua, tot, permAll, hist,
+                                                                                          dev, (uint32_t)(L + 1));
   check_launch("k_level_iota");
   if (maxParent) {
     const dim3 rg((uint32_t)std::min<uint64_t>(div_up(maxParent, kB), kRcBlocks), (uint32_t)(L - 2));
