@@ -492,9 +492,14 @@ __device__ __forceinline__ void sdag_key(const uint32_t* __restrict__ dag, uint6
   }
 }
 
+// Also clears the level's hash table (nt entries), which would otherwise take
+// a fill launch of its own.
 __global__ void k_sdag_keys(const uint32_t* __restrict__ dag, uint64_t n, const uint32_t* __restrict__ childMap,
-                            uint32_t* __restrict__ key, uint8_t* __restrict__ inv) {
-  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+                            uint32_t* __restrict__ key, uint8_t* __restrict__ inv, TEntry* __restrict__ table,
+                            uint32_t nt) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t j = i; j < nt; j += stride) table[j] = kEmpty;
   if (i < n) sdag_key(dag, i, childMap, key, inv);
 }
 
@@ -1111,7 +1116,12 @@ bool reduce_sym_pass(svd_octree* o, bool speculative) {
       dfree(c, key); dfree(c, inv); dfree(c, cand);
       continue;
     }
-    if (n) k_sdag_keys<<<div_up(n, kB), kB, 0, c->stream>>>(d.rec, n, o->sdagMap[lev + 1], key, inv);
+    // the root level is never reduced and needs no table
+    const uint32_t cap = lev == 0 ? 0u : table_cap(n);
+    TEntry* table = cap ? dalloc<TEntry>(c, cap) : nullptr;
+    if (n || cap)
+      k_sdag_keys<<<std::max<uint64_t>(div_up(n, kB), 1), kB, 0, c->stream>>>(d.rec, n, o->sdagMap[lev + 1], key, inv,
+                                                                            table, cap);
     check_launch("k_sdag_keys");
     if (lev == 0) {
       // root: children remapped, never reduced, invariant bits stay clear
@@ -1122,13 +1132,10 @@ bool reduce_sym_pass(svd_octree* o, bool speculative) {
       o->sdagMap[0] = nullptr;
       break;
     }
-    const uint32_t cap = table_cap(n);
-    TEntry* table = dalloc<TEntry>(c, cap);
     uint32_t* slot = dalloc<uint32_t>(c, n);
     uint32_t* cand = dalloc<uint32_t>(c, n * 7);
     uint32_t* state = dalloc<uint32_t>(c, n);
     uint32_t* pos = dalloc<uint32_t>(c, n + 1);
-    fill_u32(c, (uint32_t*)table, kNone, 2ull * cap);
     k_insert<<<div_up(n, kB), kB, 0, c->stream>>>(key, n, table, cap - 1, slot);
     k_sdag_cands<<<div_up(n, kB), kB, 0, c->stream>>>(key, n, table, cap - 1, slot, childInv, cand, state);
     check_launch("k_sdag_cands");
