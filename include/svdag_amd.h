@@ -219,6 +219,20 @@ typedef struct svd_view_params {
  *    randomColors.                                                          */
 int svd_render(svd_scene* sc, const svd_trace_params* p, const svd_view_params* v, float beam_projection_factor,
                float* rgb, svd_ray_result* primary);
+/* svd_render_noalloc: svd_render for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  No counterpart in the reference.  Same arguments and
+ * results as svd_render, bit for bit; the minimum-depth pre-pass texture
+ * lives in a static device array of 131072 floats (512 KiB) that every GPU
+ * the library runs on holds until the process ends, so the call makes no
+ * hipMalloc / hipFree and no allocating library call.  With
+ * beam_projection_factor > 0, (width / 8) * (height / 8) must be at most
+ * 131072 (3840 x 2160 fits), else SVD_ERR_ARG.  Work is enqueued on the
+ * context stream and the call returns without synchronising (and without
+ * recording the "trace" kernel time).  Every call on one GPU shares the
+ * array: calls that overlap on one GPU (contexts on different streams) race. */
+int svd_render_noalloc(svd_scene* sc, const svd_trace_params* p, const svd_view_params* v,
+                       float beam_projection_factor, float* rgb, svd_ray_result* primary);
 /* svd_trace_ao: AO_MODE main() (octree_dda.frag.glsl:884-919) with num_rays
  * (<= 256; the renderer's default is 16) hemisphere rays of length 0.5 per
  * pixel, over the 256 cosine-weighted Halton samples of

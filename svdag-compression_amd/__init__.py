@@ -149,6 +149,7 @@ def lib():
         "svd_trace_beam": (i32, [vp, P(TraceParams), C.c_float, vp, vp, vp]),
         "svd_render": (i32, [vp, P(TraceParams), P(ViewParams), C.c_float, vp, vp]),
         "svd_trace_ao": (i32, [vp, P(TraceParams), C.c_uint32, vp, vp]),
+        "svd_render_noalloc": (i32, [vp, P(TraceParams), P(ViewParams), C.c_float, vp, vp]),
         "svd_last_kernel_ms": (i32, [vp, C.c_char_p, P(C.c_double)]),
         "svd_refcount_order": (i32, [vp, vp, vp, u64]),
         "svd_refcount_order_ranges": (i32, [vp, vp, vp, u64, vp, vp, u64]),
@@ -199,7 +200,7 @@ EXPORTED = ["svd_ctx_create", "svd_ctx_destroy", "svd_last_error", "svd_ctx_sync
             "svd_reduce_sym", "svd_cross_level_merge", "svd_octree_state", "svd_octree_levels",
             "svd_octree_counts", "svd_level_size", "svd_get_level", "svd_encode", "svd_encoded_copy",
             "svd_encoded_device", "svd_scene_create", "svd_scene_from_octree", "svd_scene_free", "svd_trace", "svd_trace_beam",
-            "svd_render", "svd_trace_ao",
+            "svd_render", "svd_trace_ao", "svd_render_noalloc",
             "svd_last_kernel_ms", "svd_refcount_order", "svd_refcount_order_ranges", "svd_attr_bittree_encode", "svd_lossy_merge",
             "svd_lossy_signatures", "svd_mlp_create", "svd_mlp_free", "svd_mlp_get_params", "svd_mlp_set_params",
             "svd_mlp_embed", "svd_mlp_loss_grad", "svd_mlp_train", "svd_knn", "svd_workload_caves",

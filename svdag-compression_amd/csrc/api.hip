@@ -136,6 +136,8 @@ int svd_ctx_create(int device, svd_ctx** out) {
   }
   c->arena = new Arena(device);
   if (hipDeviceGetAttribute(&c->numCU, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) c->numCU = 256;
+  // svd_render_noalloc's static texture: its device address on this device
+  render_scratch_init(c);
   {
     const char* e = std::getenv("SVD_TOSDAG_CHECKED");
     c->sdagChecked = e && e[0] && !(e[0] == '0' && e[1] == 0);
@@ -581,6 +583,12 @@ int svd_render(svd_scene* s, const svd_trace_params* p, const svd_view_params* v
     Timer t(s->ctx, "trace");
     render_frame(s, p, v, beam_projection_factor, rgb, primary);
   });
+}
+
+int svd_render_noalloc(svd_scene* s, const svd_trace_params* p, const svd_view_params* v,
+                       float beam_projection_factor, float* rgb, svd_ray_result* primary) {
+  if (!s || !p || !v || !rgb) return SVD_ERR_ARG;
+  return guarded(s->ctx, [&] { render_frame_noalloc(s, p, v, beam_projection_factor, rgb, primary); });
 }
 
 int svd_trace_ao(svd_scene* s, const svd_trace_params* p, uint32_t num_rays, float* visibility,

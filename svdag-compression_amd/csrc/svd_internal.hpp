@@ -15,6 +15,9 @@ constexpr uint32_t kNull = 0xFFFFFFFEu;    // octree.cpp:22 null child pointer
 constexpr uint32_t kNone = 0xFFFFFFFFu;    // "no entry" marker in device tables
 constexpr int kRecWords = 9;               // node record: mask|mX<<8|mY<<16|mZ<<24, children[8]
 constexpr uint32_t kMaxLevels = 21;        // 3*21 = 63 Morton bits
+// svd_render_noalloc: texels of its static depth pre-pass texture (512 KiB),
+// enough for (width / 8) * (height / 8) of frames up to 4096 x 2048 or 3840 x 2160
+constexpr uint64_t kRenderScratchTexels = 131072;
 
 struct Error {
   int code;
@@ -50,6 +53,8 @@ struct svd_ctx {
   int numCU = 0;                             // compute units of the device
   bool sdagChecked = false;                  // toSDAG: check convergence after every round (SVD_TOSDAG_CHECKED=1)
   std::map<uint64_t, int> occupancy;         // (kernel id, LDS bytes) -> resident workgroups per CU
+  float* renderScratch = nullptr;            // device address of svd_render_noalloc's static texture
+  hipError_t renderScratchErr = hipSuccess;  // why it is null, if it is
 };
 
 namespace svd {
@@ -206,8 +211,14 @@ void trace_frame(svd_scene* s, const svd_trace_params* p, svd_ray_result* prim, 
 void trace_frame_beam(svd_scene* s, const svd_trace_params* p, float depthProjf, svd_ray_result* prim,
                       svd_ray_result* shad, float* minDepthOut);
 // viewer colour per pixel (beamProjf > 0: after the depth pre-pass) and the AO pass
+// depthBuf: the pre-pass texture ((width/8)*(height/8) floats), or null to take it from the arena
 void render_frame(svd_scene* s, const svd_trace_params* p, const svd_view_params* v, float beamProjf, float* rgb,
-                  svd_ray_result* prim);
+                  svd_ray_result* prim, float* depthBuf = nullptr);
+// the same frame with the pre-pass texture in a static device array (no allocation)
+void render_frame_noalloc(svd_scene* s, const svd_trace_params* p, const svd_view_params* v, float beamProjf,
+                          float* rgb, svd_ray_result* prim);
+// once per device, with it current: resolve the static texture's device address into the context
+void render_scratch_init(svd_ctx* c);
 void ao_frame(svd_scene* s, const svd_trace_params* p, uint32_t nrays, float* vis, svd_ray_result* prim);
 
 // ---- lossy DAG (lossy.hip), learned encoder (mlp.hip), kNN (knn.hip) ----------

@@ -5,6 +5,7 @@
 // product / sum rounds exactly like the shader's individual GLSL operations.
 // min/max follow the GLSL definitions: min(x,y) = y<x ? y : x.
 #include <algorithm>
+#include <mutex>
 #include "svd_internal.hpp"
 
 namespace svd {
@@ -1112,10 +1113,49 @@ void trace_frame_impl(svd_scene* sc, const svd_trace_params* p, svd_ray_result* 
   if (depth && depth != minDepthOut) dfree(c, depth);
 }
 
+namespace {
+
+// Scratch of svd_render_noalloc: the minimum-depth pre-pass texture,
+// (width / 8) x (height / 8) floats, in a static device array so that the
+// entry point allocates nothing.  Every GPU the library runs on holds its own
+// copy until the process ends.  k_trace_depth writes every texel of the
+// current frame's texture before k_render reads it, so nothing needs clearing;
+// calls that overlap on one GPU (two streams) share the array and race.
+static __device__ __align__(256) float scratch[kRenderScratchTexels];
+
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratchMu;
+float* g_scratchPtr[kMaxScratchDevices] = {};
+hipError_t g_scratchErr[kMaxScratchDevices] = {};
+bool g_scratchDone[kMaxScratchDevices] = {};
+
+}  // namespace
+
+// Resolve the device address of `scratch` on the current device (once per
+// device; later contexts of the same device reuse it).  The array's name is a
+// host-side stand-in: host code only ever passes the resolved pointer.
+void render_scratch_init(svd_ctx* c) {
+  c->renderScratch = nullptr;
+  c->renderScratchErr = hipErrorInvalidDevice;
+  if (c->device < 0 || c->device >= kMaxScratchDevices) return;
+  std::lock_guard<std::mutex> lock(g_scratchMu);
+  if (!g_scratchDone[c->device]) {
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) (void)hipGetLastError();
+    g_scratchPtr[c->device] = e == hipSuccess ? (float*)ptr : nullptr;
+    g_scratchErr[c->device] = e;
+    g_scratchDone[c->device] = true;
+  }
+  c->renderScratch = g_scratchPtr[c->device];
+  c->renderScratchErr = g_scratchErr[c->device];
+}
+
 // The viewer's frame (VIEWER_MODE, octree_dda.frag.glsl:671-858), optionally
 // after the minimum-depth pre-pass (beamProjf > 0).  rgb: 3 floats per pixel.
+// depthBuf: the pre-pass texture, or null to take it from the context arena.
 void render_frame(svd_scene* sc, const svd_trace_params* p, const svd_view_params* v, float beamProjf, float* rgb,
-                  svd_ray_result* prim) {
+                  svd_ray_result* prim, float* depthBuf) {
   svd_ctx* c = sc->ctx;
   const Dev d = scene_dev(sc);
   if (v->norm_samples < 0 || v->norm_samples > 64) throw Error{SVD_ERR_ARG, "norm_samples must be in [0, 64]"};
@@ -1138,7 +1178,7 @@ void render_frame(svd_scene* sc, const svd_trace_params* p, const svd_view_param
   MinDepth md{nullptr, 0, 0};
   float* depth = nullptr;
   if (beamProjf > 0.f) {
-    depth = dalloc<float>(c, (uint64_t)(p->width / 8) * (p->height / 8) + 1);
+    depth = depthBuf ? depthBuf : dalloc<float>(c, (uint64_t)(p->width / 8) * (p->height / 8) + 1);
     md = depth_prepass(sc, d, p, beamProjf, depth);
   }
   switch (sc->fmt) {
@@ -1149,7 +1189,27 @@ void render_frame(svd_scene* sc, const svd_trace_params* p, const svd_view_param
     default: k_render<SVD_FMT_SSVDAG><<<grid, block, shmem, c->stream>>>(d, *p, va, rgb, prim, md); break;
   }
   check_launch("k_render");
-  dfree(c, depth);
+  if (depth != depthBuf) dfree(c, depth);
+}
+
+// svd_render with the pre-pass texture in the static scratch array: no device
+// allocation, no synchronisation.
+void render_frame_noalloc(svd_scene* sc, const svd_trace_params* p, const svd_view_params* v, float beamProjf,
+                          float* rgb, svd_ray_result* prim) {
+  svd_ctx* c = sc->ctx;
+  float* depth = nullptr;
+  if (beamProjf > 0.f) {
+    const uint64_t texels = (uint64_t)(p->width / 8) * (p->height / 8);
+    if (texels > kRenderScratchTexels)
+      throw Error{SVD_ERR_ARG, "frame too large for the no-alloc render: the depth pre-pass needs " +
+                                   std::to_string(texels) + " texels, the scratch holds " +
+                                   std::to_string(kRenderScratchTexels)};
+    if (!c->renderScratch)
+      throw Error{SVD_ERR_DEVICE, std::string("render scratch unavailable: hipGetSymbolAddress: ") +
+                                      hipGetErrorString(c->renderScratchErr)};
+    depth = c->renderScratch;
+  }
+  render_frame(sc, p, v, beamProjf, rgb, prim, depth);
 }
 
 namespace {
