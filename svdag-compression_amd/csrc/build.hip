@@ -513,7 +513,10 @@ __global__ void k_sdag_cands(const uint32_t* __restrict__ key, uint64_t n, const
                              uint32_t* __restrict__ cand, uint32_t* __restrict__ state) {
   uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const bool first = entry_index(table[slot[i]]) == (uint32_t)i;
+  // a node without children was never inserted (slot kNone): it is no
+  // representative and k_sdag_outcome reports it, as k_sdag_small does
+  const uint32_t si = slot[i];
+  const bool first = si != kNone && entry_index(table[si]) == (uint32_t)i;
   uint32_t* cd = cand + i * 7;
   if (!first) {
 #pragma unroll
@@ -576,7 +579,8 @@ __global__ void k_sdag_outcome(const uint32_t* __restrict__ key, uint64_t n, con
   uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const uint32_t st = state[i];
-  if (st != kRep && entry_index(table[slot[i]]) == (uint32_t)i) {
+  const uint32_t si = slot[i];
+  if (st != kRep && si != kNone && entry_index(table[si]) == (uint32_t)i) {
     // the first node of its key: k_sdag_cands already looked up transforms
     // 1..7 and kept the earlier nodes found; the identity finds itself
     const uint32_t* cd = cand + i * 7;
@@ -607,7 +611,7 @@ __global__ void k_sdag_outcome(const uint32_t* __restrict__ key, uint64_t n, con
   const ChildInv ci = child_inv(k0, childInv);
   for (int t = 0; t < 8; ++t) {
     uint32_t r;
-    if (t == 0) r = entry_index(table[slot[i]]);
+    if (t == 0) r = si == kNone ? kNone : entry_index(table[si]);
     else { make_variant(k0, t, ci, v); r = lookup(key, table, cmask, v); }
     if (r != kNone && r < (uint32_t)i && state[r] == kRep) {
       const uint32_t f = kFlip[t];

@@ -27,8 +27,10 @@ constexpr int kB = 256;
 
 // Node i of an inner level at word starts[i]: the 9-word record with raw
 // (absolute word) child pointers; marks the targets; folds min/max pointer.
+// Below the root a node without children never comes out of the encoder
+// (buildSVO's cleanEmptyNodes and toDAG drop them), so it flags the file.
 __global__ void k_dec_inner(const uint32_t* __restrict__ data, uint32_t count, const uint32_t* __restrict__ starts,
-                            uint32_t n, uint32_t* __restrict__ rec, uint32_t* __restrict__ mark,
+                            uint32_t n, uint32_t belowRoot, uint32_t* __restrict__ rec, uint32_t* __restrict__ mark,
                             uint32_t* __restrict__ range, uint32_t* __restrict__ err) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -37,6 +39,7 @@ __global__ void k_dec_inner(const uint32_t* __restrict__ data, uint32_t count, c
   const uint32_t nc = (uint32_t)__popc(m);
   uint32_t* r = rec + (uint64_t)i * kRecWords;
   r[0] = m;
+  if (belowRoot && m == 0) atomicOr(err, 4u);
   // tiling: this node ends where the next one starts
   const uint32_t end = w + 1 + nc;
   if (end > count || (i + 1 < n && starts[i + 1] != end)) { atomicOr(err, 1u); return; }
@@ -65,6 +68,7 @@ __global__ void k_dec_leaf(const uint32_t* __restrict__ data, const uint32_t* __
   if (i >= n) return;
   const uint32_t w = starts[i];
   rec[i] = data[w] & 0xFFu;
+  if ((data[w] & 0xFFu) == 0) atomicOr(err, 4u);
   const uint32_t next = i + 1 < n ? starts[i + 1] : count;
   if (next != w + 1) atomicOr(err, 1u);
 }
@@ -140,13 +144,16 @@ void octree_from_svdag(svd_octree* o, const uint8_t* file, uint64_t bytes, bool 
       k_dec_leaf<<<div_up(n, kB), kB, 0, c->stream>>>(data, starts, (uint32_t)n, count, lv[l].rec, dev + 3);
       check_launch("k_dec_leaf");
       read_u32s(c, dev, h, 4);
+      if (h[3] & 4u) fail("empty leaf node in level " + std::to_string(l));
       if (h[3]) fail("leaf words do not end the file");
       break;
     }
     lv[l].rec = dalloc<uint32_t>(c, n * kRecWords + 1);
-    k_dec_inner<<<div_up(n, kB), kB, 0, c->stream>>>(data, count, starts, (uint32_t)n, lv[l].rec, mark, dev, dev + 3);
+    k_dec_inner<<<div_up(n, kB), kB, 0, c->stream>>>(data, count, starts, (uint32_t)n, l > 0 ? 1u : 0u, lv[l].rec,
+                                                     mark, dev, dev + 3);
     check_launch("k_dec_inner");
     read_u32s(c, dev, h, 4);
+    if (h[3] & 4u) fail("node without children below the root in level " + std::to_string(l));
     if (h[3] & 2u) fail("child pointer outside the word array");
     if (h[3]) fail("nodes do not tile level " + std::to_string(l));
     if (h[0] == 0xFFFFFFFFu) fail("level " + std::to_string(l) + " has no children");

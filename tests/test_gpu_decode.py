@@ -78,3 +78,28 @@ def test_svdag_decode_rejects_bad_files(ctx):
     extra += (1).to_bytes(4, "little")
     with pytest.raises(S.SvdError):
         S.Octree.from_svdag(ctx, bytes(extra))
+
+
+def _svdag_file(levels, words):
+    import struct
+    head = struct.pack("<6ffIII", 0, 0, 0, 1, 1, 1, 1.0, levels, 4, len(words))
+    return head + struct.pack("<I", len(words)) + struct.pack(f"<{len(words)}I", *words)
+
+
+def test_svdag_decode_rejects_empty_nodes_below_root(ctx):
+    """A node without children below the root never comes out of the encoder
+    (cleanEmptyNodes and toDAG drop them); a file that holds one is rejected at
+    decode instead of reaching toSDAG.  The hand-made file tiles its levels
+    exactly, so the empty node is the only fault."""
+    # root (slots 1, 0) -> level 1: X at word 3, Y at word 5, both -> the leaf 0xFF at word 7
+    good = _svdag_file(3, [0x03, 5, 3, 0x01, 7, 0x01, 7, 0xFF])
+    g = S.Octree.from_svdag(ctx, good)
+    assert g.level_size(S.VIEW_DAG, 1) == 2
+    g.free()
+    # X without children at word 3, Y at word 4 -> the leaf at word 6
+    bad = _svdag_file(3, [0x03, 4, 3, 0x00, 0x01, 6, 0xFF])
+    with pytest.raises(S.SvdError, match="without children"):
+        S.Octree.from_svdag(ctx, bad)
+    leaf0 = _svdag_file(3, [0x01, 2, 0x01, 4, 0x00])
+    with pytest.raises(S.SvdError, match="empty leaf"):
+        S.Octree.from_svdag(ctx, leaf0)
