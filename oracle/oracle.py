@@ -63,6 +63,8 @@ def lib():
             "orc_build_mesh": (i32, [vp, vp, u64, u32, vp]),
             "orc_build_dag_mesh": (i32, [vp, vp, u64, u32, u32, vp]),
             "orc_to_dag": (i32, [vp]), "orc_to_sdag": (i32, [vp]),
+            "orc_tri_box": (i32, [vp, C.c_double, vp]),
+            "orc_sdag_map": (C.c_int64, [vp, u32, vp, u64]),
             "orc_init_child_levels": (None, [vp]), "orc_merge_cross_levels": (i32, [vp]),
             "orc_levels": (u32, [vp]), "orc_state": (i32, [vp]), "orc_level_size": (u64, [vp, u32]),
             "orc_nnodes": (u64, [vp]), "orc_nvoxels": (u64, [vp]), "orc_nnodes_svo": (u64, [vp]),
@@ -210,6 +212,15 @@ class OracleOctree:
         assert rc == 0, rc
         return st[: lev_hi - lev_lo + 1]
 
+    def sdag_map(self, lev):
+        """toSDAG's correspondence of every DAG node of level lev: (sdag id, mirror xyz bits)."""
+        n = self.L.orc_sdag_map(self.h, lev, None, 0)
+        assert n >= 0, n
+        m = np.zeros(n, np.uint32)
+        if n:
+            self.L.orc_sdag_map(self.h, lev, _p(m), n)
+        return m & np.uint32(0x1FFFFFFF), (m >> np.uint32(29)).astype(np.uint8)
+
     def lossy_signatures(self, lev):
         sig = np.zeros((self.level_size(lev), 16), np.uint32)
         assert self.L.orc_lossy_signatures(self.h, lev, _p(sig)) == 0
@@ -233,6 +244,13 @@ class OracleOctree:
         n2 = self.L.orc_encode(self.h, fmt, _p(buf), n)
         assert n2 == n
         return buf.tobytes()
+
+
+def tri_box(center, half: float, tri) -> bool:
+    """testTriBox (test_triangle_box.cpp:105-184): tri = 3 x 3 float32, cube centre (double) and half side."""
+    c = np.ascontiguousarray(center, dtype=np.float64)
+    t = np.ascontiguousarray(tri, dtype=np.float32).reshape(9)
+    return bool(lib().orc_tri_box(_p(c), float(half), _p(t)))
 
 
 def std_sort_refcount(idx: np.ndarray, val: np.ndarray):

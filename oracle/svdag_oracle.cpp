@@ -123,6 +123,9 @@ struct Octree {
   size_t nNodesSVO = 0, nNodesDAG = 0, nNodesSDAG = 0;
   size_t nCrossLevelMerged = 0;
   std::vector<size_t> dagCounts;   // "Reduced level l from N to U" (geom_octree.cpp:508-509)
+  // toSDAG's correspondence of every DAG node, per level (the MirroredNode
+  // list of geom_octree.cpp:566-572): sdag id | mX << 29 | mY << 30 | mZ << 31
+  std::vector<std::vector<uint32_t>> sdagMap;
   int lastError = 0;
 };
 
@@ -443,6 +446,7 @@ static int toSDAG(Octree& o) {
   std::map<Node, id_t> checker;
   std::vector<Node> uniq;
   o.nNodes = 0;
+  o.sdagMap.assign(o.levels, {});
   for (unsigned lev = o.levels - 1; lev > 0; --lev) {
     uniq.clear(); checker.clear();
     corr.resize(o.data[lev].size());   // not cleared, like the reference
@@ -483,6 +487,10 @@ static int toSDAG(Octree& o) {
         uniq.push_back(n);
       }
     }
+    auto& sm = o.sdagMap[lev];
+    sm.resize(o.data[lev].size());
+    for (size_t i = 0; i < sm.size(); ++i)
+      sm[i] = corr[i].id | ((uint32_t)corr[i].x << 29) | ((uint32_t)corr[i].y << 30) | ((uint32_t)corr[i].z << 31);
     o.data[lev] = uniq;
     o.nNodes += o.data[lev].size();
     for (Node& n : o.data[lev - 1])
@@ -1253,6 +1261,20 @@ int orc_build_dag_mesh(void* h, const float* tris, uint64_t ntris, uint32_t leve
   if (levels < 2 || levels > 21 || stepLevel == 0 || stepLevel + 1 >= levels) return -2;
   buildDAGMesh(*(Octree*)h, tris, (size_t)ntris, levels, stepLevel, bbox, bbox + 3);
   return 0;
+}
+
+// testTriBox (test_triangle_box.cpp:105-184) on one triangle (9 floats) and
+// one cube (centre, half side), for the exact-arithmetic check.
+int orc_tri_box(const double c[3], double h, const float* tri) { return testTriBox(c, h, tri) ? 1 : 0; }
+
+// toSDAG's correspondence of the DAG nodes of level lev (before the
+// reduction): sdag id | mX << 29 | mY << 30 | mZ << 31.  Returns the count.
+int64_t orc_sdag_map(void* h, uint32_t lev, uint32_t* out, uint64_t cap) {
+  Octree& o = *(Octree*)h;
+  if (lev >= o.sdagMap.size()) return -2;
+  const auto& m = o.sdagMap[lev];
+  if (out && cap >= m.size()) memcpy(out, m.data(), 4 * m.size());
+  return (int64_t)m.size();
 }
 
 int orc_to_dag(void* h) { return toDAG(*(Octree*)h); }
