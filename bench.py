@@ -11,6 +11,11 @@ value = build Gvox/s (voxels / build time); traversal Mrays/s is reported
 beside it.  The roofline object is for the traversal kernel (the dominant
 kernel): achieved = algorithmic bytes fetched by the shader's fetch functions
 (counted per ray by the kernel itself) / its event-timed duration.
+build_roofline: the build's algorithmic bytes (voxel list, level records and
+remaps of the DAG and symmetry passes, leaf masks, the four files) / build time.
+cpu_baseline: the reference's OpenMP CPU builder (buildDAG over subtrees in
+parallel, then the serial toDAG / toSDAG / encoders) restated in the oracle,
+on this host's cores, over the same workload and region.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
 """
@@ -52,27 +57,71 @@ def build_step(S, ctx, dcodes_ptr, n, L):
     return g
 
 
-def cpu_baseline(W, levels_full, sample_levels):
-    """The oracle (CPU restatement of the reference's single-thread svbuilder
-    reduction + encoders) timed on a bounded sample of the same terrain."""
+CPU_STEP_LEVEL = 3
+
+
+def host_cores():
+    """Logical CPUs of this host and physical cores (lscpu), for the baseline's record."""
+    import subprocess
+    phys = None
+    try:
+        out = subprocess.run(["lscpu", "-p=CORE,SOCKET"], capture_output=True, text=True, timeout=10).stdout
+        phys = len({ln for ln in out.splitlines() if ln and not ln.startswith("#")})
+    except (OSError, subprocess.SubprocessError):
+        pass
+    return os.cpu_count(), phys
+
+
+def cpu_build(codes, L, step_level=CPU_STEP_LEVEL):
+    """The reference's OpenMP CPU builder on the oracle (svbuilder's levelStep
+    path): buildDAG over subtrees in parallel, the final toDAG, then the serial
+    .svdag / .esvdag encoders, toSDAG and the .ussvdag / .ssvdag encoders.
+    Same region as the GPU step: voxel list -> four encoded images.  Returns
+    the wall time in seconds."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
-    # sample: the terrain generator at a smaller grid (same seed / octaves)
-    codes = W.terrain_codes(sample_levels, seed=1)
     t0 = time.perf_counter()
-    o = O.OracleOctree.from_morton(codes, sample_levels)
-    o.to_dag()
+    o = O.OracleOctree.from_morton_subtrees(codes, L, step_level)
     o.init_child_levels()
     o.encode(0)
     o.encode(2)
     o.to_sdag()
     o.encode(1)
     o.encode(2)
-    t2 = time.perf_counter()
-    return dict(value=codes.size / (t2 - t0) / 1e9, unit="Gvox/s", cores=1, kind="port",
-                sample=f"terrain {1 << sample_levels}^3 ({sample_levels} levels, {codes.size} voxels, seed 1): "
-                       f"SVO from the Morton list + toDAG + toSDAG + 4 encoders (the GPU step's build region), "
-                       f"std::map / std::sort, single thread, {t2 - t0:.2f} s")
+    t = time.perf_counter() - t0
+    del o
+    return t
+
+
+def cpu_baseline(codes, L):
+    """The OpenMP CPU builder timed on this host's cores over the full
+    headline workload (the same region and voxel list as the GPU step)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    t = cpu_build(codes, L)
+    nproc, phys = host_cores()
+    return dict(value=codes.size / t / 1e9, unit="Gvox/s", cores=O.omp_threads(), kind="port",
+                sample=f"the whole workload: terrain {1 << L}^3 ({L} levels, {codes.size} voxels, seed 1), "
+                       f"voxel list -> buildDAG (levelStep {CPU_STEP_LEVEL}: subtrees SVO + toDAG in an OpenMP "
+                       f"parallel for, schedule(dynamic,2), {O.omp_threads()} threads) -> toDAG -> .svdag + "
+                       f".esvdag -> toSDAG -> .ussvdag + .ssvdag (std::map / std::sort, the reference's "
+                       f"algorithms), {t:.2f} s; host: {nproc} logical CPUs, {phys} physical cores")
+
+
+def build_bytes(S, g, L, V):
+    """Algorithmic HBM bytes of one lossless build: the 8-byte voxel list; per
+    inner level a 36-byte SVO record read, a 4-byte remap and a 36-byte DAG
+    record written, then a 36-byte DAG record read, a 4-byte remap and a
+    36-byte SDAG record written; 4 + 1 bytes per SVO leaf mask and 4 + 1 per
+    DAG leaf mask; plus the four encoded files written."""
+    N = [g.level_size(S.VIEW_SVO, l) for l in range(L)]
+    D = [g.level_size(S.VIEW_DAG, l) for l in range(L)]
+    Q = [g.level_size(S.VIEW_SDAG, l) for l in range(L)]
+    F = sum(int(g.encoded_device(f)[1]) for f in (S.FMT_SVDAG, S.FMT_ESVDAG, S.FMT_USSVDAG, S.FMT_SSVDAG))
+    B = 8 * V
+    B += sum(40 * N[l] + 36 * D[l] for l in range(L - 1)) + 5 * N[L - 1] + D[L - 1]
+    B += sum(40 * D[l] + 36 * Q[l] for l in range(L - 1)) + 5 * D[L - 1] + Q[L - 1]
+    return int(B + F), int(F)
 
 
 LOSSY_BYTES_PER_NODE = 1284   # 64 B occupancy + 64 B embedding write + 64 B read + 16 x (64 + 4) B verify + 4 B remap
@@ -246,7 +295,7 @@ def mesh_bench(S, W, ctx, levels, log, cpu=True):
     return out
 
 
-def attr_bench(S, W, ctx, levels, log):
+def attr_bench(S, W, ctx, levels, log, cpu=True):
     """Config-5 scene: fBm terrain (seed 1) union caves (seed 3) at 2^levels,
     per-voxel u8 materials (value noise seed 4, 16 levels).  Cross-level merge
     of the DAG, Gray vs binary attribute bit-trees, and a 3840x2160 frame
@@ -272,6 +321,18 @@ def attr_bench(S, W, ctx, levels, log):
     bt = float(np.median(tb[1:]))
     out["lossless_build_ms"] = round(bt * 1e3, 2)
     out["lossless_build_gvox_s"] = round(u.n / bt / 1e9, 3)
+    if cpu:
+        # the OpenMP CPU builder on this scene, same region (voxel list -> four images)
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+        host = u.numpy()
+        t_cpu = cpu_build(host, levels)
+        del host
+        out["cpu_build_s"] = round(t_cpu, 2)
+        out["cpu_build_gvox_s"] = round(u.n / t_cpu / 1e9, 5)
+        out["cpu_build_threads"] = O.omp_threads()
+        out["lossless_build_vs_cpu_same_scene"] = round(t_cpu / bt, 1)
+        log(f"[bench] attr: OpenMP CPU builder {t_cpu:.2f} s on {O.omp_threads()} threads")
     g = S.Octree.from_morton(ctx, (u.ptr, u.n), levels)
     g.to_dag()
     out["nodes_dag"] = int(g.counts()[0])
@@ -348,7 +409,6 @@ def main():
     ap.add_argument("--width", type=int, default=1920)
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-sample-levels", type=int, default=11)
     ap.add_argument("--mesh-levels", type=int, default=8,
                     help="mesh (svbuilder input) section: 64 icospheres at 2^L; 0 = skip")
     ap.add_argument("--attr-levels", type=int, default=12,
@@ -437,6 +497,8 @@ def main():
 
     # the viewer's default frame: minimum-depth pre-pass at 1/8 resolution, then the same rays
     g = build_step(S, ctx, dcodes.data_ptr(), n, L)
+    b_build, f_bytes = build_bytes(S, g, L, n)
+    build_achieved = b_build / build_s / 1e9
     sc = S.Scene.from_octree(g, S.FMT_SSVDAG)
     dpf = W.projection_factor(args.height, screen_divisor=8.0)
     tbm = []
@@ -500,22 +562,22 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": measured_traffic(L, args),
                      "kernel": "k_trace<SSVDAG>", "algorithmic_bytes": bytes_fetched},
+        "build_roofline": {"bound": "hbm", "achieved": round(build_achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                           "frac": round(build_achieved / HBM_PEAK_GBS, 5), "algorithmic_bytes": b_build,
+                           "file_bytes": f_bytes, "ms": round(build_s * 1e3, 3)},
     }
     if args.lossy_levels:
         out["lossy"] = lossy_bench(S, ctx, args.lossy_levels, log)
     if args.batch_scenes:
         out["batch"] = batch_bench(S, W, ctx, args.batch_scenes, log)
     if args.attr_levels:
-        out["cross_level_attr"] = attr_bench(S, W, ctx, args.attr_levels, log)
+        out["cross_level_attr"] = attr_bench(S, W, ctx, args.attr_levels, log, cpu=not args.no_cpu_baseline)
     if args.mesh_levels:
         out["mesh"] = mesh_bench(S, W, ctx, args.mesh_levels, log, cpu=not args.no_cpu_baseline)
     if not args.no_cpu_baseline:
-        log("[bench] cpu baseline (oracle) ...")
-        out["cpu_baseline"] = cpu_baseline(W, L, args.cpu_sample_levels)
-        if "cross_level_attr" in out:
-            # the CPU rate is measured on the 2048^3 terrain sample
-            a = out["cross_level_attr"]
-            a["lossless_build_vs_cpu_rate"] = round(a["lossless_build_gvox_s"] / out["cpu_baseline"]["value"], 1)
+        log("[bench] cpu baseline (OpenMP builder on the oracle) ...")
+        out["cpu_baseline"] = cpu_baseline(codes, L)
+        log(f"[bench] cpu baseline: {out['cpu_baseline']}")
     print(json.dumps(out), flush=True)
 
 

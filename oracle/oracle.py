@@ -64,6 +64,8 @@ def lib():
             "orc_build_dag_mesh": (i32, [vp, vp, u64, u32, u32, vp]),
             "orc_to_dag": (i32, [vp]), "orc_to_sdag": (i32, [vp]),
             "orc_tri_box": (i32, [vp, C.c_double, vp]),
+            "orc_build_dag_morton": (i32, [vp, vp, u64, u32, u32]),
+            "orc_omp_threads": (i32, []),
             "orc_sdag_map": (C.c_int64, [vp, u32, vp, u64]),
             "orc_init_child_levels": (None, [vp]), "orc_merge_cross_levels": (i32, [vp]),
             "orc_levels": (u32, [vp]), "orc_state": (i32, [vp]), "orc_level_size": (u64, [vp, u32]),
@@ -142,6 +144,16 @@ class OracleOctree:
         t = np.ascontiguousarray(tris, dtype=np.float32)
         bb = np.array(list(bbox_min) + list(bbox_max), dtype=np.float64)
         rc = o.L.orc_build_dag_mesh(o.h, _p(t), t.shape[0], levels, step_level, _p(bb))
+        assert rc == 0, rc
+        return o
+
+    @classmethod
+    def from_morton_subtrees(cls, codes: np.ndarray, levels: int, step_level: int):
+        """GeomOctree::buildDAG's OpenMP subtree builder on a voxel list
+        (geom_octree.cpp:289-435): ends in the DAG state."""
+        o = cls()
+        codes = np.ascontiguousarray(codes, dtype=np.uint64)
+        rc = o.L.orc_build_dag_morton(o.h, _p(codes), codes.size, levels, step_level)
         assert rc == 0, rc
         return o
 
@@ -244,6 +256,11 @@ class OracleOctree:
         n2 = self.L.orc_encode(self.h, fmt, _p(buf), n)
         assert n2 == n
         return buf.tobytes()
+
+
+def omp_threads() -> int:
+    """Threads of the oracle's OpenMP loops (OMP_NUM_THREADS or all cores)."""
+    return int(lib().orc_omp_threads())
 
 
 def tri_box(center, half: float, tri) -> bool:

@@ -27,6 +27,9 @@
 #include <algorithm>
 #include <array>
 #include <utility>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 namespace orc {
 
@@ -392,6 +395,74 @@ static void buildSVOMorton(Octree& o, const uint64_t* codes, size_t n, unsigned 
     }
   }
   finishSVO(o);
+}
+
+// GeomOctree::buildDAG (geom_octree.cpp:289-435) for a voxel list, the
+// reference's OpenMP CPU builder: the top SVO of stepLevel + 1 levels (its
+// finest nodes in first-arrival order = ascending Morton prefix); every set
+// child j of a finest node i is a subtree of levels - stepLevel - 1 levels
+// whose voxels are the codes below that cell, built and reduced with toDAG by
+// an OpenMP thread (parallel for schedule(dynamic,2) over i, :331-378); the
+// subtrees are joined in (i ascending, j = 7..0) order with pointer offsets
+// (:397-417) and the whole octree is reduced once more (:425).
+static void buildDAGMorton(Octree& o, const uint64_t* codes, size_t n, unsigned levels, unsigned stepLevel) {
+  const unsigned stepLevels = stepLevel + 1;
+  const unsigned subLevels = levels - stepLevels;
+  const unsigned cellShift = 3 * subLevels;                  // code bits below a subtree root cell
+  std::vector<uint64_t> top;
+  top.reserve(n ? 1 : 0);
+  for (size_t v = 0; v < n; ++v) {
+    const uint64_t c = codes[v] >> cellShift;
+    if (top.empty() || top.back() != c) top.push_back(c);
+  }
+  buildSVOMorton(o, top.data(), top.size(), stepLevels);
+  const auto& fin = o.data[stepLevels - 1];
+  std::vector<uint64_t> finPrefix;                           // prefix of finest node i
+  for (uint64_t c : top) {
+    const uint64_t p = c >> 3;
+    if (finPrefix.empty() || finPrefix.back() != p) finPrefix.push_back(p);
+  }
+  const int nTop = (int)fin.size();
+  std::vector<Octree*> subs((size_t)nTop * 8, nullptr);
+  const uint64_t lowMask = cellShift >= 64 ? ~0ull : ((1ull << cellShift) - 1);
+#pragma omp parallel for schedule(dynamic, 2)
+  for (int i = 0; i < nTop; ++i) {
+    for (int j = 7; j >= 0; --j) {
+      if (!fin[i].existsChild(j)) continue;
+      const uint64_t cell = (finPrefix[i] << 3) | (uint64_t)j;
+      const uint64_t* lo = std::lower_bound(codes, codes + n, cell << cellShift);
+      const uint64_t* hi = std::lower_bound(lo, codes + n, (cell + 1) << cellShift);
+      std::vector<uint64_t> local((size_t)(hi - lo));
+      for (size_t k = 0; k < local.size(); ++k) local[k] = lo[k] & lowMask;
+      Octree* s = new Octree();
+      buildSVOMorton(*s, local.data(), local.size(), subLevels);
+      toDAG(*s);
+      subs[(size_t)i * 8 + j] = s;
+    }
+  }
+  size_t nVox = 0;
+  o.data.resize(levels);
+  o.levels = levels;
+  for (int i = 0; i < nTop; ++i)
+    for (int j = 7; j >= 0; --j) {
+      Octree* s = subs[(size_t)i * 8 + j];
+      if (!s) continue;
+      nVox += s->nVoxels;
+      o.data[stepLevels - 1][i].children[j] = (id_t)o.data[stepLevels].size();
+      for (unsigned k = stepLevels; k < levels; ++k) {
+        if (k < levels - 1) {
+          const id_t off = (id_t)o.data[k + 1].size();
+          for (Node& m : s->data[k - stepLevels])
+            for (int c = 0; c < 8; ++c)
+              if (m.existsChildPointer(c)) m.children[c] += off;
+        }
+        o.data[k].insert(o.data[k].end(), s->data[k - stepLevels].begin(), s->data[k - stepLevels].end());
+      }
+      delete s;
+    }
+  o.nVoxels = nVox;
+  o.state = S_SVO;
+  toDAG(o);
 }
 
 // toDAG (geom_octree.cpp:462-548): per level L-1..1, dedup by exact key in a
@@ -1275,6 +1346,22 @@ int64_t orc_sdag_map(void* h, uint32_t lev, uint32_t* out, uint64_t cap) {
   const auto& m = o.sdagMap[lev];
   if (out && cap >= m.size()) memcpy(out, m.data(), 4 * m.size());
   return (int64_t)m.size();
+}
+
+// The OpenMP subtree builder on a sorted voxel list; ends in the DAG state.
+int orc_build_dag_morton(void* h, const uint64_t* codes, uint64_t n, uint32_t levels, uint32_t stepLevel) {
+  if (levels < 2 || levels > 21 || stepLevel == 0 || stepLevel + 1 >= levels) return -2;
+  buildDAGMorton(*(Octree*)h, codes, (size_t)n, levels, stepLevel);
+  return 0;
+}
+
+// Threads the OpenMP loops use (omp_get_max_threads), 1 without OpenMP.
+int orc_omp_threads() {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
 }
 
 int orc_to_dag(void* h) { return toDAG(*(Octree*)h); }

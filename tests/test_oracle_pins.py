@@ -35,7 +35,7 @@ import pytest
 
 import oracle as O
 import svdag_amd.workloads as W
-from decode import decode_words
+from decode import decode_ssvdag, decode_words
 from test_oracle import clustered_codes, random_codes, subtree_keys
 
 
@@ -582,3 +582,31 @@ def test_trace_lod_is_a_prefix_of_the_full_traversal():
         code = int(W.morton_encode(c[:1], c[1:2], c[2:3])[0])
         occupied += code in set((codes >> np.uint64(3 * (L - int(lev[r]) - 1))).tolist())
     assert occupied >= 0.99 * ok and ok > 0.9 * stop.sum()
+
+
+# ============================================================== OpenMP subtree builder (the CPU baseline)
+@pytest.mark.parametrize("name,L,step", [("terrain", 7, 1), ("terrain", 7, 3), ("clustered", 6, 2),
+                                         ("random", 6, 4)])
+def test_builddag_morton_subtrees_equal_plain_build(name, L, step):
+    """buildDAG's OpenMP subtree builder on a voxel list joins subtrees in
+    (i ascending, j = 7..0) order, so its node order differs, but it reduces
+    the same voxel set: per-level DAG counts and the decoded voxels equal the
+    single-pass build's.  toSDAG then depends on that order where invertInvs
+    lets two representatives share a reflection orbit, so its counts may
+    differ by a few nodes; its files still decode to the same voxels."""
+    codes = {"terrain": lambda: W.terrain_codes(L, seed=1), "clustered": lambda: clustered_codes(L, 3),
+             "random": lambda: random_codes(L, 5000, 4)}[name]()
+    a = O.OracleOctree.from_morton(codes, L)
+    a.to_dag()
+    b = O.OracleOctree.from_morton_subtrees(codes, L, step)
+    assert b.n_voxels == a.n_voxels == codes.size
+    assert [b.level_size(l) for l in range(L)] == [a.level_size(l) for l in range(L)]
+    assert b.n_nodes == a.n_nodes
+    b.init_child_levels()
+    np.testing.assert_array_equal(decode_words(b.encode(0), False), codes)
+    a.to_sdag()
+    b.to_sdag()
+    sa, sb = [a.level_size(l) for l in range(L)], [b.level_size(l) for l in range(L)]
+    assert all(abs(x - y) <= max(2, x // 100) for x, y in zip(sa, sb)), (sa, sb)
+    np.testing.assert_array_equal(decode_ssvdag(b.encode(2)), codes)
+    np.testing.assert_array_equal(decode_words(b.encode(1), True), codes)
