@@ -111,6 +111,19 @@ int svd_octree_from_mesh_subtrees(svd_ctx* ctx, const float* tris, uint64_t ntri
 int svd_octree_from_svo(svd_ctx* ctx, uint32_t levels, const uint64_t* level_sizes,
                         const uint32_t* const* words, const float bbox[6], double root_side,
                         svd_octree** out);
+/* svd_octree_from_levels: GeomOctree(NodeData, bbox, rootSide, levels, Stats)
+ * (geom_octree.cpp:49-60): host level arrays in the layout of
+ * svd_octree_from_svo become a DAG-state octree (no reduction is run; the
+ * levels are taken as they are).  Like the reference constructor, the octree
+ * reports the node and voxel counts it is given (stats.nNodesDAG,
+ * stats.nTotalVoxels).                                                      */
+int svd_octree_from_levels(svd_ctx* ctx, uint32_t levels, const uint64_t* level_sizes,
+                           const uint32_t* const* words, const float bbox[6], double root_side, uint64_t n_nodes,
+                           uint64_t n_voxels, svd_octree** out);
+/* svd_octree_copy: the GeomOctree copy constructor (geom_octree.cpp:41-46):
+ * a deep copy on the same context of every level view, the toSDAG maps and
+ * the encoded images.                                                        */
+int svd_octree_copy(const svd_octree* src, svd_octree** out);
 int svd_octree_free(svd_octree* oct);
 
 /* ---- reductions -------------------------------------------------------- */
@@ -118,6 +131,12 @@ int svd_octree_free(svd_octree* oct);
 int svd_reduce_dag(svd_octree* oct);
 /* GeomOctree::toSDAG(false,false) (geom_octree.cpp:551-697).  Needs S_DAG. */
 int svd_reduce_sym(svd_octree* oct);
+/* GeomOctree::toSDAG(internalCall, skipSymmetry = true) (geom_octree.cpp:
+ * 591-596): pointer-only compression.  Every node with children becomes its
+ * own representative with no mirror bits and no invariance bits, nodes
+ * without children are dropped (parents remapped), the root is kept.  Needs
+ * S_SVO or S_DAG; ends in S_SDAG with nNodes = sum of levels 1..L-1.       */
+int svd_reduce_sym_pointers(svd_octree* oct);
 /* GeomOctree::mergeAcrossAllLevels (geom_octree_extension.cpp:1192-1544).
  * Needs S_DAG; writes the cross-level view used by SVD_FMT_SVDAG.          */
 int svd_cross_level_merge(svd_octree* oct, uint64_t* nodes_eliminated);
@@ -322,6 +341,7 @@ typedef struct svd_lossy_level_stats {
   uint64_t error_bound;        /* sum over merges of voxel difference x effective refs */
   double ms_embed, ms_knn, ms_lists, ms_verify, ms_merge;
   double ms_total;             /* device time of the level, signatures to remap */
+  uint64_t clusters;           /* representatives that took in at least one merged node */
 } svd_lossy_level_stats;
 
 /* stats: HOST array of stats_cap entries (levels hi..lo); may be NULL. */

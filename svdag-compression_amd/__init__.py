@@ -74,7 +74,7 @@ class LossyParams(C.Structure):
 LOSSY_STATS_DTYPE = np.dtype([(f, "<u8") for f in ("level", "bound", "nodes_before", "nodes_after", "candidates",
                                                    "listed", "verified", "merged", "error_bound")]
                              + [(f, "<f8") for f in ("ms_embed", "ms_knn", "ms_lists", "ms_verify", "ms_merge",
-                                                     "ms_total")])
+                                                     "ms_total")] + [("clusters", "<u8")])
 
 
 class TrainParams(C.Structure):
@@ -131,6 +131,9 @@ def lib():
         "svd_octree_from_mesh": (i32, [vp, vp, u64, u32, P(C.c_double), i32, P(vp)]),
         "svd_octree_from_mesh_subtrees": (i32, [vp, vp, u64, u32, u32, P(C.c_double), i32, P(vp)]),
         "svd_octree_free": (i32, [vp]),
+        "svd_octree_from_levels": (i32, [vp, u32, P(u64), P(vp), P(C.c_float), C.c_double, u64, u64, P(vp)]),
+        "svd_octree_copy": (i32, [vp, P(vp)]),
+        "svd_reduce_sym_pointers": (i32, [vp]),
         "svd_reduce_dag": (i32, [vp]),
         "svd_reduce_sym": (i32, [vp]),
         "svd_cross_level_merge": (i32, [vp, P(u64)]),
@@ -196,7 +199,8 @@ def lib():
 
 EXPORTED = ["svd_ctx_create", "svd_ctx_destroy", "svd_last_error", "svd_ctx_synchronize", "svd_ctx_stream",
             "svd_octree_from_morton", "svd_octree_from_svo", "svd_octree_from_svdag", "svd_octree_from_mesh", "svd_octree_from_mesh_subtrees",
-            "svd_octree_free", "svd_reduce_dag",
+            "svd_octree_free", "svd_octree_from_levels", "svd_octree_copy", "svd_reduce_sym_pointers",
+            "svd_reduce_dag",
             "svd_reduce_sym", "svd_cross_level_merge", "svd_octree_state", "svd_octree_levels",
             "svd_octree_counts", "svd_level_size", "svd_get_level", "svd_encode", "svd_encoded_copy",
             "svd_encoded_device", "svd_scene_create", "svd_scene_from_octree", "svd_scene_free", "svd_trace", "svd_trace_beam",
@@ -560,6 +564,26 @@ class Octree:
         ctx.check(ctx.L.svd_octree_from_svo(ctx.h, L, sizes, ptrs, bb, float(root_side), C.byref(h)))
         return cls(ctx, h)
 
+    @classmethod
+    def from_levels(cls, ctx: Context, level_words, bbox, root_side: float, n_nodes: int, n_voxels: int):
+        """GeomOctree(NodeData, bbox, rootSide, levels, Stats): the given DAG
+        levels as a DAG-state octree reporting the given counts."""
+        L = len(level_words)
+        arrs = [np.ascontiguousarray(w, dtype=np.uint32).reshape(-1, 9) for w in level_words]
+        sizes = (C.c_uint64 * L)(*[a.shape[0] for a in arrs])
+        ptrs = (C.c_void_p * L)(*[a.ctypes.data_as(C.c_void_p).value for a in arrs])
+        bb = (C.c_float * 6)(*[float(x) for x in bbox])
+        h = C.c_void_p()
+        ctx.check(ctx.L.svd_octree_from_levels(ctx.h, L, sizes, ptrs, bb, float(root_side), int(n_nodes),
+                                               int(n_voxels), C.byref(h)))
+        return cls(ctx, h)
+
+    def copy(self) -> "Octree":
+        """A deep copy (the GeomOctree copy constructor)."""
+        h = C.c_void_p()
+        self.ctx.check(self.ctx.L.svd_octree_copy(self.h, C.byref(h)))
+        return Octree(self.ctx, h)
+
     def free(self):
         if self.h and self.ctx.h:
             self.ctx.L.svd_octree_free(self.h)
@@ -574,8 +598,12 @@ class Octree:
     def to_dag(self):
         self.ctx.check(self.ctx.L.svd_reduce_dag(self.h))
 
-    def to_sdag(self):
-        self.ctx.check(self.ctx.L.svd_reduce_sym(self.h))
+    def to_sdag(self, skip_symmetry: bool = False):
+        """toSDAG; skip_symmetry = the reference's pointer-only variant."""
+        if skip_symmetry:
+            self.ctx.check(self.ctx.L.svd_reduce_sym_pointers(self.h))
+        else:
+            self.ctx.check(self.ctx.L.svd_reduce_sym(self.h))
 
     def merge_cross_levels(self) -> int:
         n = C.c_uint64()

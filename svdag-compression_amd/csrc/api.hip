@@ -251,6 +251,98 @@ int mesh_octree(svd_ctx* c, const float* tris, uint64_t ntris, uint32_t levels, 
 
 extern "C" {
 
+namespace {
+
+// Host level arrays (9 words per node; the finest level's mask in word 0) into
+// device levels; child pointers are checked against their masks first.
+void upload_levels(svd_ctx* c, uint32_t levels, const uint64_t* sizes, const uint32_t* const* words,
+                   std::vector<Level>& dst, uint64_t* nNodes, uint64_t* nVox) {
+  dst.assign(levels, Level());
+  *nNodes = 0;
+  *nVox = 0;
+  if (sizes[0] != 1) throw Error{SVD_ERR_ARG, "level 0 must hold exactly the root"};
+  // validate child pointers on the host: every set mask bit of an inner node
+  // points into the next level, cleared bits hold the null pointer
+  for (uint32_t l = 0; l + 1 < levels; ++l) {
+    if (!sizes[l]) continue;
+    if (!words[l]) throw Error{SVD_ERR_ARG, "missing level words"};
+    for (uint64_t i = 0; i < sizes[l]; ++i) {
+      const uint32_t* w = words[l] + 9 * i;
+      for (int c = 0; c < 8; ++c) {
+        const bool bit = (w[0] >> c) & 1u;
+        if (bit ? (w[1 + c] >= sizes[l + 1]) : (w[1 + c] != kNull))
+          throw Error{SVD_ERR_POINTER, "level " + std::to_string(l) + " node " + std::to_string(i) +
+                                           ": child pointer inconsistent with its mask"};
+      }
+    }
+  }
+  for (uint32_t l = 0; l < levels; ++l) {
+    Level& lv = dst[l];
+    lv.n = sizes[l];
+    lv.leaf = (l == levels - 1);
+    if (l > 0) *nNodes += lv.n;
+    if (!lv.n) continue;
+    if (!words[l]) throw Error{SVD_ERR_ARG, "missing level words"};
+    if (lv.leaf) {
+      std::vector<uint32_t> m(lv.n);
+      for (uint64_t i = 0; i < lv.n; ++i) { m[i] = words[l][9 * i] & 0xFFu; *nVox += __builtin_popcount(m[i]); }
+      lv.rec = dalloc<uint32_t>(c, lv.n);
+      SVD_HIP(hipMemcpyAsync(lv.rec, m.data(), 4 * lv.n, hipMemcpyHostToDevice, c->stream));
+      SVD_HIP(hipStreamSynchronize(c->stream));
+    } else {
+      std::vector<uint32_t> w(words[l], words[l] + kRecWords * lv.n);
+      for (uint64_t i = 0; i < lv.n; ++i) w[kRecWords * i] &= 0xFFu;   // masks only: no mirror bits
+      lv.rec = dalloc<uint32_t>(c, lv.n * kRecWords);
+      SVD_HIP(hipMemcpyAsync(lv.rec, w.data(), 4 * kRecWords * lv.n, hipMemcpyHostToDevice, c->stream));
+      SVD_HIP(hipStreamSynchronize(c->stream));
+    }
+  }
+}
+
+// Deep copy of a level view: records (9 words, 1 per leaf), invariance bytes
+// and child levels where present.
+std::vector<Level> copy_levels(svd_ctx* c, const std::vector<Level>& src) {
+  std::vector<Level> dst(src.size());
+  try {
+    for (size_t l = 0; l < src.size(); ++l) {
+      const Level& a = src[l];
+      Level& b = dst[l];
+      b.n = a.n;
+      b.leaf = a.leaf;
+      const uint64_t words = a.n * (a.leaf ? 1 : kRecWords);
+      if (a.rec) {
+        b.rec = dalloc<uint32_t>(c, words ? words : 1);
+        if (words) SVD_HIP(hipMemcpyAsync(b.rec, a.rec, 4 * words, hipMemcpyDeviceToDevice, c->stream));
+      }
+      if (a.inv) {
+        b.inv = dalloc<uint8_t>(c, a.n + 1);
+        if (a.n) SVD_HIP(hipMemcpyAsync(b.inv, a.inv, a.n, hipMemcpyDeviceToDevice, c->stream));
+      }
+      if (a.childLevels) {
+        b.childLevels = dalloc<uint32_t>(c, 8 * a.n + 1);
+        if (a.n) SVD_HIP(hipMemcpyAsync(b.childLevels, a.childLevels, 32 * a.n, hipMemcpyDeviceToDevice, c->stream));
+      }
+    }
+  } catch (...) {
+    free_levels(c, dst);
+    throw;
+  }
+  return dst;
+}
+
+void destroy_octree(svd_octree* o) {
+  svd_ctx* c = o->ctx;
+  free_levels(c, o->svo);
+  free_levels(c, o->dag);
+  free_levels(c, o->sdag);
+  free_levels(c, o->xdag);
+  for (auto* m : o->sdagMap) dfree(c, m);
+  for (auto& e : o->enc) dfree(c, e.d);
+  delete o;
+}
+
+}  // namespace
+
 int svd_octree_from_svo(svd_ctx* c, uint32_t levels, const uint64_t* sizes, const uint32_t* const* words,
                         const float bbox[6], double rootSide, svd_octree** out) {
   if (!out || !sizes || !words || !bbox) return SVD_ERR_ARG;
@@ -262,48 +354,80 @@ int svd_octree_from_svo(svd_ctx* c, uint32_t levels, const uint64_t* sizes, cons
   for (int i = 0; i < 6; ++i) o->bbox[i] = bbox[i];
   o->rootSide = rootSide;
   int rc = guarded(c, [&] {
-    o->svo.assign(levels, Level());
     uint64_t nNodes = 0, nVox = 0;
-    if (sizes[0] != 1) throw Error{SVD_ERR_ARG, "level 0 must hold exactly the root"};
-    // validate child pointers on the host: every set mask bit of an inner node
-    // points into the next level, cleared bits hold the null pointer
-    for (uint32_t l = 0; l + 1 < levels; ++l) {
-      if (!sizes[l]) continue;
-      if (!words[l]) throw Error{SVD_ERR_ARG, "missing level words"};
-      for (uint64_t i = 0; i < sizes[l]; ++i) {
-        const uint32_t* w = words[l] + 9 * i;
-        for (int c = 0; c < 8; ++c) {
-          const bool bit = (w[0] >> c) & 1u;
-          if (bit ? (w[1 + c] >= sizes[l + 1]) : (w[1 + c] != kNull))
-            throw Error{SVD_ERR_POINTER, "level " + std::to_string(l) + " node " + std::to_string(i) +
-                                             ": child pointer inconsistent with its mask"};
-        }
-      }
-    }
-    for (uint32_t l = 0; l < levels; ++l) {
-      Level& lv = o->svo[l];
-      lv.n = sizes[l];
-      lv.leaf = (l == levels - 1);
-      if (l > 0) nNodes += lv.n;
-      if (!lv.n) continue;
-      if (!words[l]) throw Error{SVD_ERR_ARG, "missing level words"};
-      if (lv.leaf) {
-        std::vector<uint32_t> m(lv.n);
-        for (uint64_t i = 0; i < lv.n; ++i) { m[i] = words[l][9 * i] & 0xFFu; nVox += __builtin_popcount(m[i]); }
-        lv.rec = dalloc<uint32_t>(c, lv.n);
-        SVD_HIP(hipMemcpyAsync(lv.rec, m.data(), 4 * lv.n, hipMemcpyHostToDevice, c->stream));
-        SVD_HIP(hipStreamSynchronize(c->stream));
-      } else {
-        lv.rec = dalloc<uint32_t>(c, lv.n * kRecWords);
-        SVD_HIP(hipMemcpyAsync(lv.rec, words[l], 4 * kRecWords * lv.n, hipMemcpyHostToDevice, c->stream));
-      }
-    }
-    SVD_HIP(hipStreamSynchronize(c->stream));
+    upload_levels(c, levels, sizes, words, o->svo, &nNodes, &nVox);
     o->nVoxels = nVox;
     o->nNodesSVO = nNodes;
     o->state = SVD_S_SVO;
   });
   if (rc) { free_levels(c, o->svo); delete o; return rc; }
+  *out = o;
+  return SVD_OK;
+}
+
+int svd_octree_from_levels(svd_ctx* c, uint32_t levels, const uint64_t* sizes, const uint32_t* const* words,
+                           const float bbox[6], double rootSide, uint64_t n_nodes, uint64_t n_voxels,
+                           svd_octree** out) {
+  if (!out || !sizes || !words || !bbox) return SVD_ERR_ARG;
+  if (levels < 2 || levels > kMaxLevels) return SVD_ERR_LEVELS;
+  *out = nullptr;
+  svd_octree* o = new svd_octree();
+  o->ctx = c;
+  o->levels = levels;
+  for (int i = 0; i < 6; ++i) o->bbox[i] = bbox[i];
+  o->rootSide = rootSide;
+  int rc = guarded(c, [&] {
+    uint64_t nNodes = 0, nVox = 0;
+    upload_levels(c, levels, sizes, words, o->dag, &nNodes, &nVox);
+    o->nVoxels = n_voxels;     // GeomOctree(NodeData, ...) takes the counts from the Stats it is given
+    o->nNodesDAG = n_nodes;
+    o->state = SVD_S_DAG;
+  });
+  if (rc) { free_levels(c, o->dag); delete o; return rc; }
+  *out = o;
+  return SVD_OK;
+}
+
+int svd_octree_copy(const svd_octree* src, svd_octree** out) {
+  if (!src || !out) return SVD_ERR_ARG;
+  *out = nullptr;
+  svd_ctx* c = src->ctx;
+  svd_octree* o = new svd_octree();
+  o->ctx = c;
+  o->levels = src->levels;
+  o->state = src->state;
+  for (int i = 0; i < 6; ++i) o->bbox[i] = src->bbox[i];
+  o->rootSide = src->rootSide;
+  o->nVoxels = src->nVoxels;
+  o->nNodesSVO = src->nNodesSVO;
+  o->nNodesDAG = src->nNodesDAG;
+  o->nNodesSDAG = src->nNodesSDAG;
+  o->nNodesX = src->nNodesX;
+  o->crossMerged = src->crossMerged;
+  o->hasX = src->hasX;
+  int rc = guarded(c, [&] {
+    o->svo = copy_levels(c, src->svo);
+    o->dag = copy_levels(c, src->dag);
+    o->sdag = copy_levels(c, src->sdag);
+    o->xdag = copy_levels(c, src->xdag);
+    o->sdagMap.assign(src->sdagMap.size(), nullptr);
+    for (size_t l = 0; l < src->sdagMap.size(); ++l) {
+      if (!src->sdagMap[l]) continue;
+      const uint64_t n = l < src->dag.size() ? src->dag[l].n : 0;
+      o->sdagMap[l] = dalloc<uint32_t>(c, n + 1);
+      if (n) SVD_HIP(hipMemcpyAsync(o->sdagMap[l], src->sdagMap[l], 4 * n, hipMemcpyDeviceToDevice, c->stream));
+    }
+    for (int f = 0; f < 4; ++f) {
+      const Encoded& e = src->enc[f];
+      if (!e.d) continue;
+      o->enc[f].d = dalloc<uint8_t>(c, e.bytes + 1);
+      if (e.bytes) SVD_HIP(hipMemcpyAsync(o->enc[f].d, e.d, e.bytes, hipMemcpyDeviceToDevice, c->stream));
+      o->enc[f].bytes = e.bytes;
+      o->enc[f].valid = e.valid;
+    }
+    SVD_HIP(hipStreamSynchronize(c->stream));
+  });
+  if (rc) { destroy_octree(o); return rc; }
   *out = o;
   return SVD_OK;
 }
@@ -327,13 +451,7 @@ int svd_octree_free(svd_octree* o) {
   svd_ctx* c = o->ctx;
   (void)hipSetDevice(c->device);
   (void)hipStreamSynchronize(c->stream);
-  free_levels(c, o->svo);
-  free_levels(c, o->dag);
-  free_levels(c, o->sdag);
-  free_levels(c, o->xdag);
-  for (auto* m : o->sdagMap) dfree(c, m);
-  for (auto& e : o->enc) dfree(c, e.d);
-  delete o;
+  destroy_octree(o);
   return SVD_OK;
 }
 
@@ -353,6 +471,14 @@ int svd_reduce_sym(svd_octree* o) {
   return guarded(o->ctx, [&] {
     Timer t(o->ctx, "to_sdag");
     reduce_sym(o);
+  });
+}
+
+int svd_reduce_sym_pointers(svd_octree* o) {
+  if (!o) return SVD_ERR_ARG;
+  return guarded(o->ctx, [&] {
+    Timer t(o->ctx, "to_sdag");
+    reduce_sym_pointers(o);
   });
 }
 

@@ -1185,6 +1185,101 @@ bool reduce_sym_pass(svd_octree* o, bool speculative) {
 
 }  // namespace
 
+namespace {
+
+// toSDAG with skipSymmetry (geom_octree.cpp:591-596): a node with children
+// becomes its own representative with no mirror bits; a node without children
+// is dropped.  flag[i] = 1 for the nodes kept.
+__global__ void k_nonempty(const uint32_t* __restrict__ rec, uint64_t n, uint32_t leaf, uint32_t* __restrict__ flag) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t w0 = leaf ? rec[i] : rec[i * kRecWords];
+  flag[i] = (w0 & 0xFFu) ? 1u : 0u;
+}
+
+// Node i of the level moves to pos[i] (kept nodes only; the root always
+// stays), its children to childMap[child]; map[i] is its new index or kNone.
+__global__ void k_pointer_level(const uint32_t* __restrict__ rec, uint64_t n, uint32_t leaf, uint32_t keepAll,
+                                const uint32_t* __restrict__ pos, const uint32_t* __restrict__ childMap,
+                                uint32_t* __restrict__ out, uint32_t* __restrict__ map) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t w0 = leaf ? rec[i] : rec[i * kRecWords];
+  const uint32_t m = w0 & 0xFFu;
+  if (!m && !keepAll) {
+    if (map) map[i] = kNone;
+    return;
+  }
+  const uint32_t p = keepAll ? (uint32_t)i : pos[i];
+  if (map) map[i] = p;
+  if (leaf) {
+    out[p] = m;
+    return;
+  }
+  uint32_t w[kRecWords];
+  load_rec(rec, i, w);
+  w[0] = m;
+#pragma unroll
+  for (int c = 0; c < 8; ++c) w[1 + c] = ((m >> c) & 1u) ? childMap[w[1 + c]] : kNull;
+  uint32_t* o = out + (uint64_t)p * kRecWords;
+#pragma unroll
+  for (int k = 0; k < kRecWords; ++k) o[k] = w[k];
+}
+
+}  // namespace
+
+void reduce_sym_pointers(svd_octree* o) {
+  if (o->state != SVD_S_SVO && o->state != SVD_S_DAG)
+    throw Error{SVD_ERR_STATE, "toSDAG: octree is not an SVO or a DAG"};
+  svd_ctx* c = o->ctx;
+  const uint32_t L = o->levels;
+  const std::vector<Level>& src = o->state == SVD_S_DAG ? o->dag : o->svo;
+  for (auto& lv : o->sdag) { dfree(c, lv.rec); dfree(c, lv.inv); }
+  for (auto* m : o->sdagMap) dfree(c, m);
+  o->sdag.assign(L, Level());
+  o->sdagMap.assign(L, nullptr);
+  uint32_t* counts = dalloc<uint32_t>(c, L);
+  fill_u32(c, counts, 0u, L);
+  for (int lev = (int)L - 1; lev >= 0; --lev) {
+    const Level& d = src[lev];
+    Level& s = o->sdag[lev];
+    const uint64_t n = d.n;
+    s.leaf = d.leaf;
+    s.rec = dalloc<uint32_t>(c, (n ? n : 1) * (d.leaf ? 1 : kRecWords));
+    s.inv = dalloc<uint8_t>(c, n + 1);
+    fill_u8(c, s.inv, 0, n + 1);
+    const uint32_t* childMap = lev + 1 < (int)L ? o->sdagMap[lev + 1] : nullptr;
+    if (lev == 0) {
+      // the root is never reduced (the reference loops lev > 0)
+      if (n) k_pointer_level<<<div_up(n, kB), kB, 0, c->stream>>>(d.rec, n, d.leaf ? 1u : 0u, 1u, nullptr,
+                                                                    childMap, s.rec, nullptr);
+      check_launch("k_pointer_level");
+      s.n = n;
+      break;
+    }
+    uint32_t* pos = dalloc<uint32_t>(c, n + 1);
+    o->sdagMap[lev] = dalloc<uint32_t>(c, n + 1);
+    if (n) {
+      k_nonempty<<<div_up(n, kB), kB, 0, c->stream>>>(d.rec, n, d.leaf ? 1u : 0u, pos);
+      scan_exclusive(c, pos, pos, n, counts + lev);
+      k_pointer_level<<<div_up(n, kB), kB, 0, c->stream>>>(d.rec, n, d.leaf ? 1u : 0u, 0u, pos, childMap, s.rec,
+                                                            o->sdagMap[lev]);
+      check_launch("k_pointer_level");
+    }
+    dfree(c, pos);
+  }
+  std::vector<uint32_t> h(L);
+  read_u32s(c, counts, h.data(), L);
+  dfree(c, counts);
+  uint64_t nNodes = 0;
+  for (uint32_t lev = 1; lev < L; ++lev) {
+    o->sdag[lev].n = h[lev];
+    nNodes += h[lev];
+  }
+  o->nNodesSDAG = nNodes;
+  o->state = SVD_S_SDAG;
+}
+
 void reduce_sym(svd_octree* o) {
   if (o->state != SVD_S_DAG && o->state != SVD_S_SDAG) throw Error{SVD_ERR_STATE, "toSDAG: octree is not a DAG"};
   if (o->levels == 1) throw Error{SVD_ERR_LEVELS, "toSDAG needs at least 2 levels"};

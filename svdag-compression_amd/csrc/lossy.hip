@@ -307,16 +307,22 @@ __global__ void k_resolve(LevelView v, uint32_t lev, const uint32_t* __restrict_
   __atomic_store_n(&st[j], (unsigned long long)(kStRep | j), __ATOMIC_RELAXED);
 }
 
-__global__ void k_rep_flag(const unsigned long long* __restrict__ st, uint64_t n, uint32_t* __restrict__ flag) {
+// Also clears hit[], which k_lossy_compact sets on every representative that
+// takes in a merged node.
+__global__ void k_rep_flag(const unsigned long long* __restrict__ st, uint64_t n, uint32_t* __restrict__ flag,
+                           uint32_t* __restrict__ hit) {
   uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (j < n) flag[j] = (st[j] >> 32) == kRep ? 1u : 0u;
+  if (j >= n) return;
+  flag[j] = (st[j] >> 32) == kRep ? 1u : 0u;
+  hit[j] = 0u;
 }
 
 __global__ void k_lossy_compact(const uint32_t* __restrict__ rec, const uint64_t* __restrict__ vox,
                                 const unsigned long long* __restrict__ st, const uint32_t* __restrict__ tdist,
                                 const uint32_t* __restrict__ pos, const uint64_t* __restrict__ refs, uint64_t n,
                                 uint32_t* __restrict__ out, uint64_t* __restrict__ outVox,
-                                uint32_t* __restrict__ corr, unsigned long long* __restrict__ stats) {
+                                uint32_t* __restrict__ corr, uint32_t* __restrict__ hit,
+                                unsigned long long* __restrict__ stats) {
   uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= n) return;
   const unsigned long long w = st[j];
@@ -332,6 +338,8 @@ __global__ void k_lossy_compact(const uint32_t* __restrict__ rec, const uint64_t
     corr[j] = pos[(uint32_t)w];
     atomicAdd(&stats[3], 1ull);
     atomicAdd(&stats[4], (unsigned long long)tdist[j] * refs[j]);
+    // clusters: representatives with at least one member
+    if (atomicExch(&hit[(uint32_t)w], 1u) == 0u) atomicAdd(&stats[8], 1ull);
   }
 }
 
@@ -430,7 +438,7 @@ void lossy_merge(svd_octree* o, const svd_lossy_params* p, std::vector<svd_lossy
                                                           lev + 1 < (int)L ? vox[lev + 1] : nullptr, vox[lev]);
   }
   check_launch("voxel counts");
-  unsigned long long* dstats = dalloc<unsigned long long>(c, 8);
+  unsigned long long* dstats = dalloc<unsigned long long>(c, 9);
   uint32_t* unresolved = dalloc<uint32_t>(c, 1);
   PhaseClock clk(c, 7);
 
@@ -448,7 +456,7 @@ void lossy_merge(svd_octree* o, const svd_lossy_params* p, std::vector<svd_lossy
     st.nodes_before = n;
     vox[lev] = dalloc<uint64_t>(c, n + 1);
     if (n) k_vox_level<<<div_up(n, kB), kB, 0, c->stream>>>(lv.rec, n, 0, vox[lev + 1], vox[lev]);
-    k_fill_u64<<<1, 8, 0, c->stream>>>((uint64_t*)dstats, 0ull, 8);
+    k_fill_u64<<<1, 64, 0, c->stream>>>((uint64_t*)dstats, 0ull, 9);
     uint32_t* sig = dalloc<uint32_t>(c, (n + 1) * 16);
     uint32_t* lists = dalloc<uint32_t>(c, (n + 1) * cap);
     uint32_t* counts = dalloc<uint32_t>(c, n + 1);
@@ -458,6 +466,7 @@ void lossy_merge(svd_octree* o, const svd_lossy_params* p, std::vector<svd_lossy
     uint32_t* pos = dalloc<uint32_t>(c, n + 1);
     uint32_t* total = dalloc<uint32_t>(c, 1);
     uint32_t* corr = dalloc<uint32_t>(c, n + 1);
+    uint32_t* hit = dalloc<uint32_t>(c, n + 1);
     LevelView v = make_view(o, vox);
     clk.mark(0);
     if (n) k_signatures<<<div_up(n, kB), kB, 0, c->stream>>>(v, (uint32_t)lev, n, sig);
@@ -509,19 +518,19 @@ void lossy_merge(svd_octree* o, const svd_lossy_params* p, std::vector<svd_lossy
       if (read_u32(c, unresolved) == 0) break;
       if (round > n) throw Error{SVD_ERR_DEVICE, "lossy resolution did not converge"};
     }
-    if (n) k_rep_flag<<<div_up(n, kB), kB, 0, c->stream>>>(state, n, pos);
+    if (n) k_rep_flag<<<div_up(n, kB), kB, 0, c->stream>>>(state, n, pos, hit);
     scan_exclusive(c, pos, pos, n, total);
     const uint64_t kept = n ? read_u32(c, total) : 0;
     uint32_t* newRec = dalloc<uint32_t>(c, (kept + 1) * kRecWords);
     uint64_t* newVox = dalloc<uint64_t>(c, kept + 1);
     if (n) k_lossy_compact<<<div_up(n, kB), kB, 0, c->stream>>>(lv.rec, vox[lev], state, tdist, pos, refs[lev], n,
-                                                              newRec, newVox, corr, dstats);
+                                                              newRec, newVox, corr, hit, dstats);
     check_launch("k_lossy_compact");
     const uint64_t np = o->dag[lev - 1].n;
     if (np) k_remap_inplace<<<div_up(np, kB), kB, 0, c->stream>>>(o->dag[lev - 1].rec, np, corr);
     check_launch("k_remap_inplace");
     clk.mark(6);
-    uint64_t hs[8];
+    uint64_t hs[9];
     SVD_HIP(hipMemcpyAsync(hs, dstats, sizeof(hs), hipMemcpyDeviceToHost, c->stream));
     SVD_HIP(hipStreamSynchronize(c->stream));
     st.merged = hs[3];
@@ -529,6 +538,7 @@ void lossy_merge(svd_octree* o, const svd_lossy_params* p, std::vector<svd_lossy
     st.listed = hs[5];
     st.verified = hs[6];
     st.candidates = hs[7];
+    st.clusters = hs[8];
     st.nodes_after = kept;
     st.ms_embed = learned ? clk.ms(1, 2) : 0.0;
     st.ms_knn = learned ? clk.ms(2, 3) : 0.0;
@@ -543,7 +553,7 @@ void lossy_merge(svd_octree* o, const svd_lossy_params* p, std::vector<svd_lossy
     dfree(c, vox[lev]);
     vox[lev] = newVox;
     dfree(c, sig); dfree(c, lists); dfree(c, counts); dfree(c, dv); dfree(c, state); dfree(c, tdist);
-    dfree(c, pos); dfree(c, total); dfree(c, corr);
+    dfree(c, pos); dfree(c, total); dfree(c, corr); dfree(c, hit);
   }
   for (auto* r : refs) dfree(c, r);
   for (auto* x : vox) dfree(c, x);

@@ -185,6 +185,8 @@ void remap_children(svd_ctx* c, const uint32_t* in, uint64_t n, const uint32_t* 
 void reduce_dag(svd_octree* o);
 uint64_t dedup_all(svd_ctx* c, uint32_t L, const std::vector<Level>& src, std::vector<Level>& dst);
 void reduce_sym(svd_octree* o);
+// toSDAG(..., skipSymmetry = true): every non-empty node its own representative, no mirror bits
+void reduce_sym_pointers(svd_octree* o);
 void cross_level_merge(svd_octree* o);
 
 // ---- refcount order: exact emulation of libstdc++ std::sort (refsort.hip) --

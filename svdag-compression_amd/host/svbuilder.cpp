@@ -51,6 +51,16 @@ static std::string hq(size_t n) {
   else snprintf(b, sizeof b, "%zu", n);
   return b;
 }
+// human-readable duration of a time in seconds
+static std::string hd(double sec) {
+  char b[64];
+  if (sec >= 3600.0) snprintf(b, sizeof b, "%.2f h", sec / 3600.0);
+  else if (sec >= 60.0) snprintf(b, sizeof b, "%.2f min", sec / 60.0);
+  else if (sec >= 1.0) snprintf(b, sizeof b, "%.2f s", sec);
+  else if (sec >= 1e-3) snprintf(b, sizeof b, "%.2f ms", sec * 1e3);
+  else snprintf(b, sizeof b, "%.2f us", sec * 1e6);
+  return b;
+}
 static std::string hs(size_t n) {
   char b[64];
   if (n >= (1ull << 30)) snprintf(b, sizeof b, "%.2f GB", n / double(1ull << 30));
@@ -86,7 +96,9 @@ int main(int argc, char** argv) {
     if (!scene.loadPly(inputFile)) exit(1);
   } else if (strstr(inputFile.c_str(), ".svdag")) {   // main.cpp:97-101
     EncodedSVDAG inputSVDAG;
-    if (!inputSVDAG.load(inputFile) || !inputSVDAG.decode(octree)) exit(1);
+    if (!inputSVDAG.load(inputFile)) exit(1);
+    octree = inputSVDAG.decode(scene);
+    if (octree.getState() != GeomOctree::S_DAG) exit(1);
     isInputDAG = true;
   } else {
     printf("Can't read input file '%s'. Only supported ASCII Obj (or PLY) files.\n", inputFile.c_str());
@@ -203,34 +215,46 @@ int main(int argc, char** argv) {
   printf("Encoded SSVDAG     : %s\t(%.3f bits/vox)\n", hs(ssvdag.getDataSize()).c_str(), (8 * ssvdag.getDataSize()) / float(nv));
   printf("SSVDAG / DAG       : %.1f %%\n", 100.f * ssvdag.getDataSize() / (float)svdag.getDataSize());
   printf("SSVDAG / SVO       : %.1f %%\n", 100.f * ssvdag.getDataSize() / (float)stats.nNodesSVO);
-  printf("SVO->SVDAG time    : %.3f ms\n", stats.toDAGTime * 1e3);
-  printf("SVDAG->LSVDAG time : %.3f ms\n", stats.toLSVDAGTime * 1e3);
-  printf("SVDAG->SSVDAG time : %.3f ms\n", stats.toSDAGTime * 1e3);
-  printf("Total time         : %.3f s\n", totalTime);
+  printf("SVO->SVDAG time    : %s\n", hd(stats.toDAGTime).c_str());
+  printf("SVDAG->LSVDAG time : %s\n", hd(stats.toLSVDAGTime).c_str());
+  printf("SVDAG->SSVDAG time : %s\n", hd(stats.toSDAGTime).c_str());
+  printf("Total time         : %s\n", hd(totalTime).c_str());
   printf("===================================================================\n\n");
 
-  // main.cpp:320-366: append the result block to ./stats.txt
-  FILE* st = fopen("stats.txt", "a");
-  if (st) {
-    const std::string baseName = base_without_ext(inputFile);
+  // main.cpp:320-366: the same block to the console, then appended to ./stats.txt
+  const std::string baseName = base_without_ext(inputFile);
+  auto ms = [](double sec) { return (size_t)(sec * 1e3); };
+  for (int pass = 0; pass < 2; ++pass) {
+    FILE* st = pass == 0 ? stdout : fopen("stats.txt", "a");
+    if (!st) break;
     fprintf(st, "%s, %d, lossy: %d, cross-level: %d\n", baseName.c_str(), nLevels, lossy, multiLevel);
     fprintf(st, "#Voxels, %zu\n", stats.nTotalVoxels);
     fprintf(st, ", SVDAG, ESVDAG, SSVDAG, SVO\n");
     fprintf(st, "#nodes, %zu, '', %zu, %zu\n", stats.nNodesDAG, stats.nNodesSDAG, stats.nNodesSVO);
-    fprintf(st, "memory (bytes), %zu, %zu, %zu, %zu\n", svdag.getDataSize(), esvdag.getDataSize(), ssvdag.getDataSize(),
-            stats.nNodesSVO);
-    if (multiLevel) {
-      fprintf(st, "Construction times:\n,SVDAG, Total, Cross-level\n");
-      fprintf(st, "time (ms), %zu, %zu, %zu\n", (size_t)(stats.toDAGTime * 1e3), (size_t)(totalTime * 1e3),
-              (size_t)(stats.crossMergeTime * 1e3));
+    fprintf(st, "memory (bytes), %zu, %zu, %zu, %zu\n", svdag.getDataSize(), esvdag.getDataSize(),
+            ssvdag.getDataSize(), stats.nNodesSVO);
+    if (lossy) {
+      fprintf(st, "Construction times:\n");
+      fprintf(st, ",SVDAG, LSVDAG, Total, Hashing, SimNodes, Clustering\n");
+      fprintf(st, "time (ms), %zu, %zu, %zu, %zu, %zu, %zu\n", ms(stats.toDAGTime), ms(stats.toLSVDAGTime),
+              ms(totalTime), ms(stats.lHashing), ms(stats.lSimNodes), ms(stats.lClustering));
+      fprintf(st, "Lossy stats, %.2f, %.2f, %i\n", lossyInflation, allowedLossyDiffFactor, includedNodeRefCount);
+      fprintf(st, "TotalVoxDifference, #NodesIn, #ClustersOut, #edges, svdag mem, lsvdag mem\n");
+      fprintf(st, "%zu, %zu, %zu, %zu, %zu, %zu\n", stats.totalLossyVoxelDifference, stats.nClusteredNodes,
+              stats.nClusters, stats.nEdges, svdag2.getDataSize(), svdag.getDataSize());
+    } else if (multiLevel) {
+      fprintf(st, "Construction times:\n");
+      fprintf(st, ",SVDAG, Total, Cross-level\n");
+      fprintf(st, "time (ms), %zu, %zu, %zu\n", ms(stats.toDAGTime), ms(totalTime), ms(stats.crossMergeTime));
       fprintf(st, "Cross-level, nodes eliminated, svdag mem, csvdag mem\n");
       fprintf(st, ", %zu, %zu, %zu\n", stats.nCrossLevelMerged, svdag2.getDataSize(), svdag.getDataSize());
     } else {
-      fprintf(st, "Construction times:\n,SVDAG, Total\n");
-      fprintf(st, "time (ms), %zu, %zu\n", (size_t)(stats.toDAGTime * 1e3), (size_t)(totalTime * 1e3));
+      fprintf(st, "Construction times:\n");
+      fprintf(st, ",SVDAG, Total\n");
+      fprintf(st, "time (ms), %zu, %zu\n", ms(stats.toDAGTime), ms(totalTime));
     }
     fprintf(st, "\n\n");
-    fclose(st);
+    if (pass == 1) fclose(st);
   }
   return 0;
 }

@@ -68,8 +68,24 @@ using namespace svd;
 // ============================================================================ Scene
 Scene::Scene() { _bbox.to_empty(); }
 
-void Scene::loadObj(std::string fileName, bool, bool, bool, bool, bool doBuildTriVector) {
+void Scene::loadObj(std::string fileName, bool tryLoadBinCache, bool, bool loadNormals, bool recomputeNormals,
+                    bool doBuildTriVector) {
   printf("* Loading '%s'...\n", fileName.c_str());
+  const std::string cache = fileName + ".bincache";
+  if (tryLoadBinCache) {
+    std::ifstream probe(cache);
+    if (probe.good()) {
+      probe.close();
+      printf("\t- Binary cache found! Loading '%s'... ", cache.c_str());
+      loadBinObj(cache);
+      printf("OK!\n");
+      if (doBuildTriVector) buildTriVector(true);
+      printf("\t- Loaded %zu triangles\n", getNRawTriangles());
+      printf("\t- Bbox: [%.3f %.3f %.3f]  [%.3f %.3f %.3f]\n", _bbox[0][0], _bbox[0][1], _bbox[0][2], _bbox[1][0],
+             _bbox[1][1], _bbox[1][2]);
+      return;
+    }
+  }
   FILE* file = fopen(fileName.c_str(), "r");
   if (!file) {
     printf("ERROR: Scene:loadOBJ: Can't open file %s\n", fileName.c_str());
@@ -77,17 +93,31 @@ void Scene::loadObj(std::string fileName, bool, bool, bool, bool, bool doBuildTr
   }
   _bbox.to_empty();
   _vertices.clear();
-  _indexed.clear();
+  _normals.clear();
+  _indexedTris.clear();
   const int len = 10000;
   std::vector<char> buf(len);
   char* str = buf.data();
-  size_t nTris = 0;
-  std::vector<int64_t> tri(3);
+  size_t nLine = 0;
+  // one face's indices; a token that does not parse leaves the previous value
+  TIndexedTri itri{};
   while (fgets(str, len, file) != NULL) {
+    nLine++;
     if (str[0] == 'v') {
-      if (str[1] == 'n' || str[1] == 't') continue;
+      if (str[1] == 'n') {
+        if (loadNormals) {
+          point3f nv;
+          sscanf(str + 2, "%f %f %f", &nv[0], &nv[1], &nv[2]);
+          _normals.push_back(nv);
+        }
+        continue;
+      }
+      if (str[1] == 't') continue;
       point3f p;
-      if (sscanf(str + 1, "%f %f %f", &p[0], &p[1], &p[2]) != 3) continue;
+      if (sscanf(str + 1, "%f %f %f", &p[0], &p[1], &p[2]) != 3) {
+        printf("Read in line %zu a bad vertex format\n", nLine);
+        continue;
+      }
       _bbox.merge(p);
       _vertices.push_back(p);
     } else if (str[0] == 'f') {
@@ -97,27 +127,98 @@ void Scene::loadObj(std::string fileName, bool, bool, bool, bool, bool doBuildTr
       for (char* pch = strtok_r(str + 1, " ", &save); pch; pch = strtok_r(nullptr, " ", &save)) triples.push_back(pch);
       if (triples.empty()) continue;
       if (!triples.back().empty()) triples.back().resize(triples.back().size() - 1);
-      auto parse = [&](const std::string& s, int64_t& out) {
+      bool noNormal = false;
+      auto parse = [&](const std::string& t, int i) {
         int idxV = 0, idxN = 0;
-        int n = sscanf(s.c_str(), "%d//%d", &idxV, &idxN);
+        const int n = sscanf(t.c_str(), "%d//%d", &idxV, &idxN);
         if (idxV < 0) idxV = int(_vertices.size()) + idxV + 1;
-        if (n >= 1) out = (int64_t)idxV - 1;
+        if (idxN < 0) idxN = int(_normals.size()) + idxN + 1;
+        if (n == 0) printf("ERROR:Scene:loadObj: problem parsing face instruction\n");
+        else if (n == 1) { itri.verticesIdx[i] = (std::size_t)(int64_t)(idxV - 1); noNormal = true; }
+        else if (n == 2) {
+          itri.verticesIdx[i] = (std::size_t)(int64_t)(idxV - 1);
+          if (!recomputeNormals) itri.normalsIdx[i] = (std::size_t)(int64_t)(idxN - 1);
+        }
       };
-      for (int i = 0; i < 3 && i < (int)triples.size(); ++i) parse(triples[i], tri[i]);
-      for (int i = 0; i < 3; ++i) _indexed.push_back((uint32_t)tri[i]);
-      nTris++;
-      if (triples.size() > 4) {
-        for (int i = 0; i < 3; ++i) parse(triples[(i + 2) % 4], tri[i]);
-        for (int i = 0; i < 3; ++i) _indexed.push_back((uint32_t)tri[i]);
-        nTris++;
+      auto faceNormal = [&]() {
+        if (!(loadNormals && (noNormal || recomputeNormals))) return;
+        const size_t nv = _vertices.size();
+        point3f nrm;
+        if (itri.verticesIdx[0] < nv && itri.verticesIdx[1] < nv && itri.verticesIdx[2] < nv) {
+          const point3f& a = _vertices[itri.verticesIdx[0]];
+          const point3f& b = _vertices[itri.verticesIdx[1]];
+          const point3f& c = _vertices[itri.verticesIdx[2]];
+          const float e1[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
+          const float e2[3] = {c[0] - a[0], c[1] - a[1], c[2] - a[2]};
+          nrm = point3f(e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2], e1[0] * e2[1] - e1[1] * e2[0]);
+        }
+        itri.normalsIdx[0] = itri.normalsIdx[1] = itri.normalsIdx[2] = _normals.size();
+        _normals.push_back(nrm);
+      };
+      itri.material = 0;
+      for (int i = 0; i < 3 && i < (int)triples.size(); ++i) parse(triples[i], i);
+      faceNormal();
+      _indexedTris.push_back(itri);
+      if (triples.size() > 4) {          // a quad, when a trailing token follows its 4 indices
+        for (int i = 0; i < 3; ++i) parse(triples[(i + 2) % 4], i);
+        faceNormal();
+        _indexedTris.push_back(itri);
       }
     }
   }
   fclose(file);
-  printf("\t- Loaded %zu triangles\n", nTris);
+  printf("\t- Loaded %zu triangles in %.2fM obj lines\n", _indexedTris.size(), nLine / 1e6);
   printf("\t- Bbox: [%.3f %.3f %.3f]  [%.3f %.3f %.3f]\n", _bbox[0][0], _bbox[0][1], _bbox[0][2], _bbox[1][0],
          _bbox[1][1], _bbox[1][2]);
+  printf("\t- Saving '%s' binary cache... ", cache.c_str());
+  saveBinObj(cache);
+  printf("OK!\n");
   if (doBuildTriVector) buildTriVector(true);
+}
+
+namespace {
+// scene.hpp:109-119 and the 300-byte TMaterial record (char name[256], three
+// colours, specular coefficient, transparency)
+struct BinObjHeader {
+  uint64_t nVertices, nNormals, nMaterials, nIndexedTris;
+  float bboxMin[3], bboxMax[3];
+};
+static_assert(sizeof(BinObjHeader) == 56, "bincache header layout");
+static_assert(sizeof(Scene::TIndexedTri) == 56, "bincache triangle layout");
+constexpr size_t kMaterialBytes = 300;
+}  // namespace
+
+void Scene::saveBinObj(std::string fileName) const {
+  BinObjHeader h{};
+  h.nVertices = _vertices.size();
+  h.nNormals = _normals.size();
+  h.nMaterials = 0;
+  h.nIndexedTris = _indexedTris.size();
+  for (int i = 0; i < 3; ++i) { h.bboxMin[i] = _bbox[0][i]; h.bboxMax[i] = _bbox[1][i]; }
+  std::ofstream f(fileName, std::ios::out | std::ios::binary);
+  if (!f.is_open()) return;
+  f.write((const char*)&h, sizeof(h));
+  if (!_vertices.empty()) f.write((const char*)_vertices.data(), _vertices.size() * sizeof(point3f));
+  if (!_normals.empty()) f.write((const char*)_normals.data(), _normals.size() * sizeof(point3f));
+  if (!_indexedTris.empty()) f.write((const char*)_indexedTris.data(), _indexedTris.size() * sizeof(TIndexedTri));
+}
+
+void Scene::loadBinObj(std::string fileName) {
+  std::ifstream f(fileName, std::ios::in | std::ios::binary);
+  BinObjHeader h{};
+  f.read((char*)&h, sizeof(h));
+  for (int i = 0; i < 3; ++i) { _bbox[0][i] = h.bboxMin[i]; _bbox[1][i] = h.bboxMax[i]; }
+  _vertices.resize(h.nVertices);
+  if (h.nVertices) f.read((char*)_vertices.data(), h.nVertices * sizeof(point3f));
+  _normals.resize(h.nNormals);
+  if (h.nNormals) f.read((char*)_normals.data(), h.nNormals * sizeof(point3f));
+  f.seekg((std::streamoff)(h.nMaterials * kMaterialBytes), std::ios::cur);   // materials are not kept
+  _indexedTris.resize(h.nIndexedTris);
+  if (h.nIndexedTris) f.read((char*)_indexedTris.data(), h.nIndexedTris * sizeof(TIndexedTri));
+  if (!f) {
+    printf("ERROR: Scene:loadBinObj: '%s' is truncated\n", fileName.c_str());
+    exit(1);
+  }
 }
 
 bool Scene::loadPly(std::string fileName) {
@@ -137,13 +238,19 @@ bool Scene::loadPly(std::string fileName) {
     in >> _vertices[i][0] >> _vertices[i][1] >> _vertices[i][2];
     _bbox.merge(_vertices[i]);
   }
-  _indexed.clear();
+  _indexedTris.clear();
   for (size_t f = 0; f < nf; ++f) {
     size_t k;
     in >> k;
-    std::vector<uint32_t> idx(k);
+    std::vector<size_t> idx(k);
     for (size_t j = 0; j < k; ++j) in >> idx[j];
-    for (size_t j = 1; j + 1 < k; ++j) { _indexed.push_back(idx[0]); _indexed.push_back(idx[j]); _indexed.push_back(idx[j + 1]); }
+    for (size_t j = 1; j + 1 < k; ++j) {
+      TIndexedTri t{};
+      t.verticesIdx[0] = idx[0];
+      t.verticesIdx[1] = idx[j];
+      t.verticesIdx[2] = idx[j + 1];
+      _indexedTris.push_back(t);
+    }
   }
   buildTriVector(true);
   printf("\t- Loaded %zu triangles\n", getNRawTriangles());
@@ -153,19 +260,84 @@ bool Scene::loadPly(std::string fileName) {
 // scene.cpp:394-416: only triangles whose three indices are valid
 void Scene::buildTriVector(bool clearOtherData) {
   _triangles.clear();
-  for (size_t t = 0; t + 2 < _indexed.size(); t += 3) {
-    const uint32_t a = _indexed[t], b = _indexed[t + 1], c = _indexed[t + 2];
+  for (const TIndexedTri& t : _indexedTris) {
+    const size_t a = t.verticesIdx[0], b = t.verticesIdx[1], c = t.verticesIdx[2];
     if (a < _vertices.size() && b < _vertices.size() && c < _vertices.size()) {
       _triangles.push_back(_vertices[a]);
       _triangles.push_back(_vertices[b]);
       _triangles.push_back(_vertices[c]);
     }
   }
-  if (clearOtherData) { _vertices.clear(); _vertices.shrink_to_fit(); _indexed.clear(); _indexed.shrink_to_fit(); }
+  if (clearOtherData) {
+    _vertices.clear(); _vertices.shrink_to_fit();
+    _normals.clear(); _normals.shrink_to_fit();
+    _indexedTris.clear(); _indexedTris.shrink_to_fit();
+  }
 }
 
 // ============================================================================ GeomOctree
 GeomOctree::GeomOctree(Scene* scene) : _scene(scene) { _bbox.to_empty(); }
+
+GeomOctree::GeomOctree(const GeomOctree& other)
+    : _scene(other._scene), _bbox(other._bbox), _rootSide(other._rootSide), _levels(other._levels),
+      _state(other._state), _stats(other._stats) {
+  if (other._oct) check(svd_octree_copy(other._oct, &_oct), "octree copy");
+}
+
+GeomOctree& GeomOctree::operator=(const GeomOctree& other) {
+  if (this == &other) return *this;
+  GeomOctree tmp(other);
+  *this = std::move(tmp);
+  return *this;
+}
+
+GeomOctree::GeomOctree(GeomOctree&& other) noexcept
+    : _scene(other._scene), _oct(other._oct), _bbox(other._bbox), _rootSide(other._rootSide),
+      _levels(other._levels), _state(other._state), _stats(other._stats), _cache(std::move(other._cache)),
+      _cacheState(other._cacheState) {
+  other._oct = nullptr;
+  other._state = S_EMPTY;
+  other._cacheState = -1;
+}
+
+GeomOctree& GeomOctree::operator=(GeomOctree&& other) noexcept {
+  if (this == &other) return *this;
+  if (_oct) svd_octree_free(_oct);
+  _scene = other._scene;
+  _oct = other._oct;
+  _bbox = other._bbox;
+  _rootSide = other._rootSide;
+  _levels = other._levels;
+  _state = other._state;
+  _stats = other._stats;
+  _cache = std::move(other._cache);
+  _cacheState = other._cacheState;
+  other._oct = nullptr;
+  other._state = S_EMPTY;
+  other._cacheState = -1;
+  return *this;
+}
+
+GeomOctree::GeomOctree(const NodeData& data, aabox3f bbox, double rootSide, unsigned int levels, Stats stats)
+    : _scene(nullptr), _bbox(bbox), _rootSide(rootSide), _levels(levels), _state(S_DAG), _stats(stats) {
+  std::vector<std::vector<uint32_t>> words(levels);
+  std::vector<uint64_t> sizes(levels, 0);
+  std::vector<const uint32_t*> ptrs(levels, nullptr);
+  for (unsigned l = 0; l < levels && l < data.size(); ++l) {
+    sizes[l] = data[l].size();
+    words[l].resize(9 * data[l].size() + 9);
+    for (size_t i = 0; i < data[l].size(); ++i) {
+      const Node& n = data[l][i];
+      words[l][9 * i] = n.childrenBitmask;
+      for (int c = 0; c < 8; ++c) words[l][9 * i + 1 + c] = n.children[c];
+    }
+    ptrs[l] = words[l].data();
+  }
+  const float bb[6] = {bbox[0][0], bbox[0][1], bbox[0][2], bbox[1][0], bbox[1][1], bbox[1][2]};
+  check(svd_octree_from_levels(default_context(), levels, sizes.data(), ptrs.data(), bb, rootSide, stats.nNodesDAG,
+                               stats.nTotalVoxels, &_oct),
+        "GeomOctree(NodeData)");
+}
 
 GeomOctree::~GeomOctree() {
   if (_oct) svd_octree_free(_oct);
@@ -287,6 +459,12 @@ bool EncodedSVDAG::decode(GeomOctree& octree) const {
   return octree.fromSVDAG(_file);
 }
 
+GeomOctree EncodedSVDAG::decode(Scene& scene) const {
+  GeomOctree octree(&scene);
+  decode(octree);
+  return octree;
+}
+
 void GeomOctree::toDAG(bool internalCall) {
   if (_state == S_SVO || _state == S_DAG) {
     if (!internalCall) printf("* Transforming SVO -> DAG ... ");
@@ -325,10 +503,11 @@ void GeomOctree::toSDAG(bool internalCall, bool skipSymmetry) {
     else { printf("ERROR! This is not a DAG or SDAG!\n"); return; }
     fflush(stdout);
   }
-  if (skipSymmetry) { printf("ERROR! pointer-only SDAG (skipSymmetry) is not supported\n"); return; }
-  if (_state == S_SVO) toDAG(true);
+  if (_state != S_SVO && _state != S_DAG) return;
+  if (_state == S_SVO && !skipSymmetry) toDAG(true);
   const double t0 = now_s();
-  check(svd_reduce_sym(_oct), "toSDAG");
+  // skipSymmetry: pointer-only compression, every node its own representative (geom_octree.cpp:591-596)
+  check(skipSymmetry ? svd_reduce_sym_pointers(_oct) : svd_reduce_sym(_oct), "toSDAG");
   _stats.toSDAGTime = now_s() - t0;
   uint64_t nn = 0;
   svd_octree_counts(_oct, &nn, nullptr);
@@ -372,19 +551,34 @@ void GeomOctree::toLossyDag(float lossyInflation, float allowedLossyDiffFactor, 
   const int rc = svd_lossy_merge(_oct, &p, st.data(), (uint32_t)st.size());
   _stats.toLSVDAGTime = now_s() - t0;
   if (rc != SVD_OK) { printf("\t\t- %s\n", svd_last_error(default_context())); return; }
-  size_t err = 0, merged = 0;
+  // the reference's statistics (geom_octree_extension.cpp:930, 1050, 1061, 1073-1089) in this rule's terms:
+  // hashing = candidate lists (and the learned proposer's embedding and kNN), similar nodes = exact
+  // verification, clustering = the representative resolution and merge; an edge is a verified pair, a
+  // cluster a representative with its merged members
+  size_t err = 0, merged = 0, clusters = 0, edges = 0;
+  double msHash = 0, msSim = 0, msClust = 0;
   for (unsigned i = 0; i + 2 < _levels; ++i) {
     printf("Level %u: max diff %u, %zu -> %zu nodes (%zu merged)\n", (unsigned)st[i].level, (unsigned)st[i].bound,
            (size_t)st[i].nodes_before, (size_t)st[i].nodes_after, (size_t)st[i].merged);
     err += st[i].error_bound;
     merged += st[i].merged;
+    clusters += st[i].clusters;
+    edges += st[i].verified;
+    msHash += st[i].ms_embed + st[i].ms_knn + st[i].ms_lists;
+    msSim += st[i].ms_verify;
+    msClust += st[i].ms_merge;
   }
   const size_t before = _stats.nNodesDAG;
   uint64_t nn = 0;
   svd_octree_counts(_oct, &nn, nullptr);
   _stats.nNodesDAG = nn;
   _stats.totalLossyVoxelDifference = err;
-  _stats.nClusteredNodes = merged;
+  _stats.nClusters = clusters;
+  _stats.nClusteredNodes = merged + clusters;
+  _stats.nEdges = edges;
+  _stats.lHashing = msHash * 1e-3;
+  _stats.lSimNodes = msSim * 1e-3;
+  _stats.lClustering = msClust * 1e-3;
   _cacheState = -1;
   printf("Lossy compressed from %zu to %zu nodes.\n", before, (size_t)nn);
 }
@@ -430,7 +624,104 @@ bool EncodedOctree::encodeWith(const GeomOctree& octree, int fmt, int needState,
   _file.resize(bytes);
   check(svd_encoded_copy(octree.handle(), fmt, _file.data(), bytes), "encoded copy");
   parse();
+  _nVoxels = octree.getNVoxels();       // encoded_svdag.cpp:118, encoded_ssvdag.cpp:226
   return true;
+}
+
+EncodedOctree::TravNode EncodedOctree::getRootTravNode() const { return TravNode(); }
+
+// ---- TravNode API over the host file image (encoded_svdag.cpp:318-354,
+// encoded_ussvdag.cpp:172-210, encoded_ssvdag.cpp:468-524).  Indices outside
+// the arrays (a malformed image) read as absent children instead of memory.
+bool EncodedSVDAG::hasChild(const TravNode& node, const int c) const {
+  const uint32_t* d = getDataPtr();
+  if (!d || node.idx >= _nWords) return false;
+  return (d[node.idx] & (1u << c)) != 0;
+}
+
+EncodedOctree::TravNode EncodedSVDAG::getChild(const TravNode& node, const int c, bool& mX, bool& mY,
+                                               bool& mZ) const {
+  TravNode tn;
+  if (node.level >= _levels - 1) {
+    printf("EncodedSVDAG::getChild: WARNING! Asking for a node child, but node is in its max level\n");
+    return tn;
+  }
+  const uint32_t* d = getDataPtr();
+  if (!d || node.idx >= _nWords) return tn;
+  const uint8_t mask = (uint8_t)(d[node.idx] & 0xFFu);
+  const size_t at = node.idx + (size_t)__builtin_popcount((unsigned)(mask >> c));
+  tn.idx = at < _nWords ? d[at] : 0;
+  tn.level = node.level + 1;
+  mX = mY = mZ = false;
+  return tn;
+}
+
+bool EncodedSVDAG::hasVoxel(const TravNode& leaf, const int i, const int j, const int k) const {
+  return hasChild(leaf, 4 * i + 2 * j + k);
+}
+
+EncodedOctree::TravNode EncodedUSSVDAG::getChild(const TravNode& node, const int c, bool& mX, bool& mY,
+                                                 bool& mZ) const {
+  TravNode tn;
+  if (node.level >= _levels - 1) {
+    printf("EncodedUSSVDAG::getChild: WARNING! Asking for a node child, but node is in its max level\n");
+    return tn;
+  }
+  const uint32_t* d = getDataPtr();
+  if (!d || node.idx >= _nWords) return tn;
+  const uint32_t h = d[node.idx];
+  const size_t at = node.idx + (size_t)__builtin_popcount((unsigned)((h & 0xFFu) >> c));
+  tn.idx = at < _nWords ? d[at] : 0;
+  tn.level = node.level + 1;
+  mX = (h & (1u << (c + 8))) != 0;
+  mY = (h & (1u << (c + 16))) != 0;
+  mZ = (h & (1u << (c + 24))) != 0;
+  return tn;
+}
+
+bool EncodedSSVDAG::hasChild(const TravNode& node, const int c) const {
+  const uint16_t* in = innerPtr();
+  if (!in || node.idx >= _nInner) return false;
+  return ((in[node.idx] >> (2u * c)) & 3u) != 0;
+}
+
+EncodedOctree::TravNode EncodedSSVDAG::getChild(const TravNode& node, const int c, bool& mX, bool& mY,
+                                                bool& mZ) const {
+  TravNode tn;
+  if (node.level >= _levels - 2) {
+    printf("EncodedSSVDAG::getChild: WARNING! Asking for a node child, but node is in its max level\n");
+    return tn;
+  }
+  const uint16_t* in = innerPtr();
+  if (!in || node.idx >= _nInner) return tn;
+  const uint32_t hdr = in[node.idx];
+  // 16-bit pointers take one u16, 32-bit ones two (size codes 1 / 2, 3)
+  size_t off = 1;
+  for (int i = 7; i > c; --i) off += std::min<uint32_t>((hdr >> (2u * i)) & 3u, 2u);
+  if (node.idx + off >= _nInner) return tn;
+  uint32_t idx = in[node.idx + off];
+  const uint32_t code = (hdr >> (2u * c)) & 3u;
+  mX = ((idx >> 13u) & 1u) == 1;
+  mY = ((idx >> 14u) & 1u) == 1;
+  mZ = ((idx >> 15u) & 1u) == 1;
+  idx &= ~(7u << 13u);
+  if (code > 1) {
+    if (node.idx + off + 1 >= _nInner) return tn;
+    idx = (idx << 16) | in[node.idx + off + 1];
+    idx |= (code & 1u) << 29u;
+  }
+  tn.level = node.level + 1;
+  if (tn.level < _levels - 2) tn.idx = idx + (tn.level < _levelOffsets.size() ? _levelOffsets[tn.level] : 0u);
+  else tn.idx = idx * 8;
+  return tn;
+}
+
+bool EncodedSSVDAG::hasVoxel(const TravNode& leaf, const int i, const int j, const int k) const {
+  const uint8_t* lv = leafPtr();
+  const int c = i + 4 * j + 16 * k;
+  const size_t byte = (size_t)leaf.idx + (size_t)(c / 8);
+  if (!lv || byte >= _nLeafBytes) return false;
+  return (lv[byte] & (1u << (c % 8))) != 0;
 }
 
 bool EncodedOctree::save(const std::string filename) const {
@@ -450,6 +741,7 @@ bool EncodedOctree::load(const std::string filename) {
   if (!ifs.is_open()) { printf("FAILED!!!\n"); return false; }
   _file.assign(std::istreambuf_iterator<char>(ifs), std::istreambuf_iterator<char>());
   parse();
+  _nVoxels = 0;                         // the files hold no voxel count
   printf("OK!\n");
   return true;
 }

@@ -7,9 +7,9 @@
 // error behaviour (printf + return false / silent return on a wrong state).
 // The only source-level difference: the SpaceLand geometry types of the
 // reference signatures are replaced by the small svd::vec3 / svd::aabox types
-// below.  All reductions and encoders run on the GPU through the C ABI
-// (include/svdag_amd.h); the mesh voxelizer runs on the host, in FP64, in the
-// reference's triangle-major order.
+// below.  The mesh voxelizer (FP64 triangle/box tests in the reference's
+// triangle-major order), the reductions and the encoders all run on the GPU
+// through the C ABI (include/svdag_amd.h); the host parses the input files.
 #pragma once
 
 #include <cstdint>
@@ -60,15 +60,30 @@ svd_ctx* default_context();
 // ----------------------------------------------------------------------------
 class Scene {
  public:
+  // scene.hpp:40-44: vertex and normal indices and the material of one triangle
+  struct TIndexedTri {
+    std::size_t verticesIdx[3];
+    std::size_t normalsIdx[3];
+    std::size_t material;
+  };
   Scene();
   // Scene::loadObj (scene.cpp:51-268): same line parser and quirks (face
   // tokens split on ' ', the last token loses its final character, a face
-  // is split into two triangles only when it has more than 4 tokens, the
-  // box merges all vertices).  The reference's `.bincache` file
-  // (scene.cpp:55-71) is neither read nor written: tryLoadBinCache is
-  // accepted and ignored, so every call parses the OBJ text.
+  // is split into two triangles only when it has more than 4 tokens,
+  // negative indices count from the end, "v//n" gives a normal index, the
+  // box merges all vertices).  With tryLoadBinCache, `<fileName>.bincache`
+  // is loaded instead of the text when it exists; after parsing the text the
+  // cache is always written (scene.cpp:55-71, 257-258, 324-374), in the
+  // reference's layout: a header of four size_t counts and the box, then the
+  // vertices, normals, materials (300-byte records) and indexed triangles.
+  // loadNormals keeps "vn" normals (recomputeNormals or a face without them
+  // adds the face's cross product, :166-174); materials (mtllib / usemtl) are
+  // not parsed, so every triangle has material 0 and the cache holds none.
   void loadObj(std::string fileName, bool tryLoadBinCache = true, bool loadMaterials = false,
                bool loadNormals = false, bool recomputeNormals = false, bool doBuildTriVector = true);
+  // scene.cpp:324-374
+  void saveBinObj(std::string fileName) const;
+  void loadBinObj(std::string fileName);
   // ASCII PLY with "vertex" (x y z floats) and "face" (vertex_indices lists).
   bool loadPly(std::string fileName);
   void buildTriVector(bool clearOtherData = false);
@@ -78,12 +93,16 @@ class Scene {
   }
   inline svd::aabox3f getAABB() const { return _bbox; }
   inline void setAABB(svd::aabox3f bbox) { _bbox = bbox; }
-  inline std::size_t getNTriangles() const { return _indexed.size() > 0 ? _indexed.size() / 3 : _triangles.size() / 3; }
+  inline std::size_t getNTriangles() const { return _indexedTris.size() > 0 ? _indexedTris.size() : _triangles.size() / 3; }
+  inline std::size_t getNVertices() const { return _vertices.size(); }
+  inline std::vector<TIndexedTri>* getIndexedGeom() { return &_indexedTris; }
+  inline const std::vector<svd::point3f>& getNormals() const { return _normals; }
 
  private:
   svd::aabox3f _bbox;
   std::vector<svd::point3f> _vertices;
-  std::vector<uint32_t> _indexed;       // 3 vertex indices per triangle
+  std::vector<svd::point3f> _normals;
+  std::vector<TIndexedTri> _indexedTris;
   std::vector<svd::point3f> _triangles; // 3 points per triangle
 };
 
@@ -123,8 +142,14 @@ class GeomOctree {
   typedef std::vector<std::vector<Node>> NodeData;
 
   explicit GeomOctree(Scene* scene);
-  GeomOctree(const GeomOctree&) = delete;
-  GeomOctree& operator=(const GeomOctree&) = delete;
+  // geom_octree.cpp:41-46: a deep copy (its device levels and encoded images too)
+  GeomOctree(const GeomOctree& other);
+  GeomOctree& operator=(const GeomOctree& other);
+  GeomOctree(GeomOctree&& other) noexcept;
+  GeomOctree& operator=(GeomOctree&& other) noexcept;
+  // geom_octree.cpp:49-60: a DAG-state octree from host levels; it reports
+  // stats.nNodesDAG nodes and stats.nTotalVoxels voxels
+  GeomOctree(const NodeData& data, svd::aabox3f bbox, double rootSide, unsigned int levels, Stats stats);
   ~GeomOctree();
 
   // geom_octree.cpp:171-280: voxelization on the device in the reference's node order
@@ -137,6 +162,8 @@ class GeomOctree {
   // An .svdag file image decoded on the device (svd_octree_from_svdag); used by EncodedSVDAG::decode.
   bool fromSVDAG(const std::vector<uint8_t>& file);
   void toDAG(bool internalCall = false);
+  // geom_octree.cpp:551-697; skipSymmetry: every node its own representative
+  // (pointer-only compression, :591-596)
   void toSDAG(bool internalCall = false, bool skipSymmetry = false);
   unsigned int mergeAcrossAllLevels();
   // geom_octree_extension.cpp:797-1179 on the GPU: levels L-2..1, bound
@@ -184,10 +211,35 @@ class EncodedOctree {
   virtual bool save(const std::string filename) const;
   virtual std::string getDescription() const = 0;
   virtual size_t getDataSize() const = 0;
+
+  // Traversal interface (encoded_octree.hpp:60-76), read from the host file
+  // image.  A TravNode's idx is the node's position in the format's array:
+  // a word index (SVDAG, USSVDAG), a u16 index of an inner node or the byte
+  // offset of a 4^3 leaf brick (SSVDAG).  Like the reference, getChild
+  // leaves the child's center at (0,0,0) and prints a warning when asked for
+  // a child of a node at the maximum level.
+  struct TravNode {
+    TravNode() : idx(0), level(0), center(0.f, 0.f, 0.f), mX(false), mY(false), mZ(false) {}
+    uint32_t idx;
+    unsigned int level;
+    svd::point3f center;
+    bool mX, mY, mZ;
+  };
+  virtual TravNode getRootTravNode() const;
+  virtual bool hasChild(const TravNode& node, const int c) const = 0;
+  virtual TravNode getChild(const TravNode& node, const int c, bool& mX, bool& mY, bool& mZ) const = 0;
+  virtual bool isLeaf(const TravNode& node) const = 0;
+  virtual int getLeafSize() const = 0;
+  virtual bool hasVoxel(const TravNode& leaf, const int i, const int j, const int k) const = 0;
+
   inline svd::aabox3f getSceneBBox() const { return _sceneBBox; }
   inline float getRootSide() const { return _rootSide; }
+  inline svd::point3f getCenter() const { return _sceneBBox.center(); }
   inline unsigned int getNLevels() const { return _levels; }
   inline size_t getNNodes() const { return _nNodes; }
+  // encode() takes the octree's voxel count; a loaded file has none (0)
+  inline size_t getNVoxels() const { return _nVoxels; }
+  inline size_t getNLeaves() const { return _nLeaves; }
   inline float getHalfSide(const unsigned int level = 0) const { return _rootSide / float(1u << (level + 1)); }
   // the complete file image (what save() writes)
   const std::vector<uint8_t>& fileImage() const { return _file; }
@@ -199,7 +251,7 @@ class EncodedOctree {
   svd::aabox3f _sceneBBox;
   float _rootSide = 0;
   unsigned int _levels = 0;
-  size_t _nNodes = 0;
+  size_t _nNodes = 0, _nVoxels = 0, _nLeaves = 0;
   std::vector<uint8_t> _file;
 };
 
@@ -211,10 +263,16 @@ class EncodedSVDAG : public EncodedOctree {
   uint32_t getFirstLeafPtr() const { return _firstLeafPtr; }
   const uint32_t* getDataPtr() const { return _file.empty() ? nullptr : (const uint32_t*)(_file.data() + 44); }
   int format() const override { return SVD_FMT_SVDAG; }
-  // encoded_svdag.cpp:216-281.  The reference returns a GeomOctree by value;
-  // here the decoded DAG goes into an existing octree (state S_DAG), because
-  // the octree owns a device handle and is not copyable.
+  // encoded_svdag.cpp:216-281: the DAG-state octree of this file, by value
+  // (on a malformed file: printed, and an empty octree comes back)
+  GeomOctree decode(Scene& scene) const;
+  // the same into an existing octree; false on a malformed file
   bool decode(GeomOctree& octree) const;
+  bool hasChild(const TravNode& node, const int c) const override;
+  TravNode getChild(const TravNode& node, const int c, bool& mX, bool& mY, bool& mZ) const override;
+  bool isLeaf(const TravNode& node) const override { return node.level == (_levels - 1); }
+  int getLeafSize() const override { return 2; }
+  bool hasVoxel(const TravNode& leaf, const int i, const int j, const int k) const override;
 
  protected:
   void parse() override;
@@ -227,6 +285,8 @@ class EncodedUSSVDAG : public EncodedSVDAG {
   void encode(const GeomOctree& octree) override;
   std::string getDescription() const override { return "USSVDAG. Symmetry-aware DAG, uncompressed encoding."; }
   int format() const override { return SVD_FMT_USSVDAG; }
+  // encoded_ussvdag.cpp:185-199: the child's mirror bits come from the header word
+  TravNode getChild(const TravNode& node, const int c, bool& mX, bool& mY, bool& mZ) const override;
 };
 
 class EncodedSSVDAG : public EncodedOctree {
@@ -241,10 +301,22 @@ class EncodedSSVDAG : public EncodedOctree {
   unsigned int getLeafNodesSize() const { return (unsigned)_nLeafBytes; }
   unsigned int getNLevelOffsets() const { return (unsigned)_levelOffsets.size(); }
   const uint32_t* getLevelOffsetsPtr() const { return _levelOffsets.data(); }
+  // encoded_ssvdag.hpp:58-61: the u16 inner-node array and the leaf bytes
+  void* getInnerNodesDataPtr() const { return (void*)innerPtr(); }
+  void* getLeafNodesDataPtr() const { return (void*)leafPtr(); }
   int format() const override { return _esvdag ? SVD_FMT_ESVDAG : SVD_FMT_SSVDAG; }
+  bool hasChild(const TravNode& node, const int c) const override;
+  TravNode getChild(const TravNode& node, const int c, bool& mX, bool& mY, bool& mZ) const override;
+  bool isLeaf(const TravNode& node) const override { return node.level == (_levels - 2); }
+  int getLeafSize() const override { return 4; }
+  bool hasVoxel(const TravNode& leaf, const int i, const int j, const int k) const override;
 
  protected:
   void parse() override;
+  const uint16_t* innerPtr() const { return _file.size() >= 40 ? (const uint16_t*)(_file.data() + 40) : nullptr; }
+  const uint8_t* leafPtr() const {
+    return _file.size() >= 44 + 2 * _nInner ? _file.data() + 44 + 2 * _nInner : nullptr;
+  }
   size_t _nInner = 0, _nLeafBytes = 0;
   std::vector<uint32_t> _levelOffsets;
   bool _esvdag = false;
