@@ -47,13 +47,13 @@ def log(msg):
 
 
 def build_step(S, ctx, dcodes_ptr, n, L):
+    """The lossless build: voxel list -> SVO -> toDAG -> toSDAG -> the four
+    images (.svdag, .esvdag of the DAG; .ussvdag, .ssvdag of the SDAG), the
+    images in one call whose passes the formats share."""
     g = S.Octree.from_morton(ctx, (dcodes_ptr, n), L)
     g.to_dag()
-    g.encode(S.FMT_SVDAG)
-    g.encode(S.FMT_ESVDAG)
     g.to_sdag()
-    g.encode(S.FMT_USSVDAG)
-    g.encode(S.FMT_SSVDAG)
+    g.encode_many([S.FMT_SVDAG, S.FMT_ESVDAG, S.FMT_USSVDAG, S.FMT_SSVDAG])
     return g
 
 

@@ -160,6 +160,13 @@ int svd_get_level(svd_octree* oct, int view, uint32_t lev, uint32_t* words, uint
  * bytes (exactly what save() writes), svd_encoded_copy copies it to the host
  * and svd_encoded_device exposes the device buffer.                          */
 int svd_encode(svd_octree* oct, int fmt, uint64_t* file_bytes);
+/* svd_encode_formats: svd_encode of n distinct formats in one call (svbuilder
+ * encodes all four, main.cpp:220-271), with the same images.  .ssvdag and
+ * .esvdag share every pass: one reference-count launch, one segmented
+ * refcount order over the levels of both (each level still its own std::sort
+ * call), one layout read; .svdag and .ussvdag share one size read.
+ * file_bytes (may be NULL) receives n sizes in the order of fmts.          */
+int svd_encode_formats(svd_octree* oct, const int* fmts, uint32_t n, uint64_t* file_bytes);
 int svd_encoded_copy(svd_octree* oct, int fmt, void* dst, uint64_t cap);
 int svd_encoded_device(svd_octree* oct, int fmt, const void** dptr, uint64_t* file_bytes);
 

@@ -144,6 +144,7 @@ def lib():
         "svd_get_level": (i32, [vp, i32, u32, vp, vp, vp]),
         "svd_encode": (i32, [vp, i32, P(u64)]),
         "svd_encoded_copy": (i32, [vp, i32, vp, u64]),
+        "svd_encode_formats": (i32, [vp, vp, u32, vp]),
         "svd_encoded_device": (i32, [vp, i32, P(vp), P(u64)]),
         "svd_scene_create": (i32, [vp, i32, vp, u64, i32, P(vp)]),
         "svd_scene_from_octree": (i32, [vp, i32, P(vp)]),
@@ -202,7 +203,7 @@ EXPORTED = ["svd_ctx_create", "svd_ctx_destroy", "svd_last_error", "svd_ctx_sync
             "svd_octree_free", "svd_octree_from_levels", "svd_octree_copy", "svd_reduce_sym_pointers",
             "svd_reduce_dag",
             "svd_reduce_sym", "svd_cross_level_merge", "svd_octree_state", "svd_octree_levels",
-            "svd_octree_counts", "svd_level_size", "svd_get_level", "svd_encode", "svd_encoded_copy",
+            "svd_octree_counts", "svd_level_size", "svd_get_level", "svd_encode", "svd_encode_formats", "svd_encoded_copy",
             "svd_encoded_device", "svd_scene_create", "svd_scene_from_octree", "svd_scene_free", "svd_trace", "svd_trace_beam",
             "svd_render", "svd_trace_ao", "svd_render_noalloc",
             "svd_last_kernel_ms", "svd_refcount_order", "svd_refcount_order_ranges", "svd_attr_bittree_encode", "svd_lossy_merge",
@@ -652,6 +653,14 @@ class Octree:
         n = C.c_uint64()
         self.ctx.check(self.ctx.L.svd_encode(self.h, fmt, C.byref(n)))
         return n.value
+
+    def encode_many(self, fmts) -> list:
+        """Several formats in one call (passes shared); returns their sizes."""
+        f = np.ascontiguousarray(list(fmts), dtype=np.int32)
+        out = np.zeros(f.size, np.uint64)
+        self.ctx.check(self.ctx.L.svd_encode_formats(self.h, f.ctypes.data_as(C.c_void_p), f.size,
+                                                     out.ctypes.data_as(C.c_void_p)))
+        return [int(x) for x in out]
 
     def encoded_bytes(self, fmt: int) -> bytes:
         ptr, n = C.c_void_p(), C.c_uint64()

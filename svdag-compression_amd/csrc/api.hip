@@ -565,6 +565,18 @@ int svd_encode(svd_octree* o, int fmt, uint64_t* bytes) {
   });
 }
 
+int svd_encode_formats(svd_octree* o, const int* fmts, uint32_t n, uint64_t* bytes) {
+  if (!o || (n && !fmts) || n > 4) return SVD_ERR_ARG;
+  for (uint32_t k = 0; k < n; ++k)
+    if (fmts[k] < 0 || fmts[k] > 3) return SVD_ERR_ARG;
+  return guarded(o->ctx, [&] {
+    Timer t(o->ctx, "encode");
+    encode_formats(o, fmts, (int)n);
+    if (bytes)
+      for (uint32_t k = 0; k < n; ++k) bytes[k] = o->enc[fmts[k]].bytes;
+  });
+}
+
 int svd_encoded_copy(svd_octree* o, int fmt, void* dst, uint64_t cap) {
   if (!o || fmt < 0 || fmt > 3 || !dst) return SVD_ERR_ARG;
   if (!o->enc[fmt].valid) return SVD_ERR_STATE;

@@ -88,17 +88,33 @@ __global__ void k_put_count(const uint32_t* __restrict__ total, uint32_t* __rest
 }
 
 // ---------------------------------------------------------------- SSVDAG
+// The levels 0..L-2 of one or two octree views (.ssvdag of the SDAG, .esvdag
+// of the DAG) concatenated: set s level l is span level s * P + l, P = L - 1.
+struct Span {
+  const uint32_t* rec[2 * kMaxLevels];
+  uint32_t start[2 * kMaxLevels + 1];
+  uint32_t L;     // span levels (K * P)
+  uint32_t P;     // levels per set
+};
+
+__device__ __forceinline__ uint32_t span_level(const Span& a, uint64_t i) {
+  uint32_t l = 0;
+  while (l + 1 < a.L && i >= a.start[l + 1]) ++l;
+  return l;
+}
+
 // Reference counts of the children of a level.  A few children (common
 // bricks) take most references, so each block first aggregates in an LDS
 // hash table (1024 slots, at most 8 probes; a child that finds no slot goes
 // straight to the global histogram) and flushes one add per distinct child.
 constexpr int kRcSlots = 1024;
 constexpr int kRcBlocks = 2048;
-// Levels 1..L-2 in one launch: blockIdx.y = parent level l, whose children
-// are level l+1 (element start[l+1] + child of the concatenated histogram).
-__global__ void __launch_bounds__(256) k_refcount(LevelSet a, uint32_t* __restrict__ histAll,
+// Levels 1..L-2 of every set in one launch: blockIdx.y = parent level
+// (P - 1 of them per set), whose children are the next span level (element
+// start[l+1] + child of the concatenated histogram).
+__global__ void __launch_bounds__(256) k_refcount(Span a, uint32_t* __restrict__ histAll,
                                                   uint32_t* __restrict__ err) {
-  const uint32_t l = blockIdx.y;
+  const uint32_t l = (blockIdx.y / (a.P - 1)) * a.P + blockIdx.y % (a.P - 1);
   const uint32_t* __restrict__ parent = a.rec[l];
   const uint64_t np = a.start[l + 1] - a.start[l];
   uint32_t* __restrict__ hist = histAll + a.start[l + 1];
@@ -137,11 +153,11 @@ __global__ void __launch_bounds__(256) k_refcount(LevelSet a, uint32_t* __restri
 // perm[i] = position of element i within its level (the identity order of
 // every level).  The same launch clears the refcount histogram (n words) and
 // the nz counters in zero, which would otherwise take two fill launches.
-__global__ void k_level_iota(LevelSet a, uint64_t n, uint32_t* __restrict__ perm, uint32_t* __restrict__ hist,
+__global__ void k_level_iota(Span a, uint64_t n, uint32_t* __restrict__ perm, uint32_t* __restrict__ hist,
                              uint32_t* __restrict__ zero, uint32_t nz) {
   uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) {
-    perm[i] = (uint32_t)(i - a.start[level_of(a, i)]);
+    perm[i] = (uint32_t)(i - a.start[span_level(a, i)]);
     hist[i] = 0u;
   }
   if (i < nz) zero[i] = 0u;
@@ -350,81 +366,114 @@ void put_bytes(svd_ctx* c, uint8_t* dst, const std::vector<uint8_t>& src) {
   check_launch("k_put_bytes");
 }
 
-void encode_words(svd_octree* o, int fmt) {
+// .svdag (of the DAG, or its cross-level view) and .ussvdag (of the SDAG):
+// each format's passes are enqueued, and the word counts of all are read back
+// once at the end.
+void encode_words_sets(svd_octree* o, const int* fmts, int K) {
   svd_ctx* c = o->ctx;
-  const bool ussv = fmt == SVD_FMT_USSVDAG;
-  const std::vector<Level>& lv = ussv ? o->sdag : (o->hasX ? o->xdag : o->dag);
   const uint32_t L = o->levels;
-  const LevelSet a = level_set(lv, L, !ussv);
-  const uint64_t N = a.start[L];
-  uint32_t* ptrs = dalloc<uint32_t>(c, N + 1);
-  uint32_t* dev = dalloc<uint32_t>(c, 2);   // word count, error flags
-  fill_u32(c, dev, 0u, 2);
-  if (N) k_word_counts<<<div_up(N, kB), kB, 0, c->stream>>>(a, N, ptrs);
-  check_launch("k_word_counts");
-  scan_exclusive(c, ptrs, ptrs, N, dev);
-  // The reference assigns _firstLeafPtr after the loop over ALL levels
-  // (encoded_svdag.cpp:129-137), so the stored value is the total word count.
-  // Every node takes at most 9 words: the image is allocated for that bound,
-  // the header's two count words are written on the device and the count is
-  // read back once, with the error flags, at the end.
-  uint64_t maxWords = 0;
-  for (uint32_t l = 0; l < L; ++l) maxWords += lv[l].leaf ? lv[l].n : lv[l].n * kRecWords;
-  const uint64_t nNodes = ussv ? o->nNodesSDAG : (o->hasX ? o->nNodesX : o->nNodesDAG);
-  std::vector<uint8_t> hdr = base_header(o, nNodes);
-  const uint64_t hdrBytes = hdr.size() + 8;   // + firstLeafPtr + count
-  Encoded& e = o->enc[fmt];
-  dfree(c, e.d);
-  e.d = dalloc<uint8_t>(c, hdrBytes + 4 * maxWords);
-  put_bytes(c, e.d, hdr);
-  k_put_count<<<1, 64, 0, c->stream>>>(dev, (uint32_t*)(e.d + hdr.size()));
-  uint32_t* words = (uint32_t*)(e.d + hdrBytes);
-  if (N) k_write_words<<<div_up(N, kB), kB, 0, c->stream>>>(a, ussv ? 1 : 0, ptrs, N, words, dev + 1);
-  check_launch("k_write_words");
-  uint32_t h[2];
-  read_u32s(c, dev, h, 2);
-  dfree(c, ptrs); dfree(c, dev);
-  if (h[1]) throw Error{SVD_ERR_POINTER, "encode: child pointer without a target node"};
-  e.bytes = hdrBytes + 4ull * h[0];
-  e.valid = true;
+  uint32_t* dev = dalloc<uint32_t>(c, 2 * K);   // per format: word count, error flags
+  fill_u32(c, dev, 0u, 2 * K);
+  std::vector<uint32_t*> ptrsOf(K, nullptr);
+  std::vector<uint64_t> hdrBytesOf(K, 0);
+  for (int s = 0; s < K; ++s) {
+    const int fmt = fmts[s];
+    const bool ussv = fmt == SVD_FMT_USSVDAG;
+    const std::vector<Level>& lv = ussv ? o->sdag : (o->hasX ? o->xdag : o->dag);
+    const LevelSet a = level_set(lv, L, !ussv);
+    const uint64_t N = a.start[L];
+    uint32_t* ptrs = dalloc<uint32_t>(c, N + 1);
+    ptrsOf[s] = ptrs;
+    uint32_t* d = dev + 2 * s;
+    if (N) k_word_counts<<<div_up(N, kB), kB, 0, c->stream>>>(a, N, ptrs);
+    check_launch("k_word_counts");
+    scan_exclusive(c, ptrs, ptrs, N, d);
+    // The reference assigns _firstLeafPtr after the loop over ALL levels
+    // (encoded_svdag.cpp:129-137), so the stored value is the total word
+    // count.  Every node takes at most 9 words: the image is allocated for
+    // that bound, the header's two count words are written on the device and
+    // the count is read back once, with the error flags, at the end.
+    uint64_t maxWords = 0;
+    for (uint32_t l = 0; l < L; ++l) maxWords += lv[l].leaf ? lv[l].n : lv[l].n * kRecWords;
+    const uint64_t nNodes = ussv ? o->nNodesSDAG : (o->hasX ? o->nNodesX : o->nNodesDAG);
+    std::vector<uint8_t> hdr = base_header(o, nNodes);
+    const uint64_t hdrBytes = hdr.size() + 8;   // + firstLeafPtr + count
+    hdrBytesOf[s] = hdrBytes;
+    Encoded& e = o->enc[fmt];
+    dfree(c, e.d);
+    e.d = dalloc<uint8_t>(c, hdrBytes + 4 * maxWords);
+    e.valid = false;
+    put_bytes(c, e.d, hdr);
+    k_put_count<<<1, 64, 0, c->stream>>>(d, (uint32_t*)(e.d + hdr.size()));
+    uint32_t* words = (uint32_t*)(e.d + hdrBytes);
+    if (N) k_write_words<<<div_up(N, kB), kB, 0, c->stream>>>(a, ussv ? 1 : 0, ptrs, N, words, d + 1);
+    check_launch("k_write_words");
+  }
+  std::vector<uint32_t> h(2 * K);
+  read_u32s(c, dev, h.data(), 2 * K);
+  for (int s = 0; s < K; ++s) dfree(c, ptrsOf[s]);
+  dfree(c, dev);
+  for (int s = 0; s < K; ++s)
+    if (h[2 * s + 1]) throw Error{SVD_ERR_POINTER, "encode: child pointer without a target node"};
+  for (int s = 0; s < K; ++s) {
+    Encoded& e = o->enc[fmts[s]];
+    e.bytes = hdrBytesOf[s] + 4ull * h[2 * s];
+    e.valid = true;
+  }
 }
 
-void encode_ssvdag(svd_octree* o, int fmt) {
+// One or two SSVDAG-layout images (.ssvdag of the SDAG, .esvdag of the DAG)
+// with every pass shared: one launch clears and numbers all their levels, one
+// counts the references of all, one segmented sort orders every level of
+// both (each level is its own std::sort call), one host read returns the
+// layout sizes, and the writes follow.
+void encode_ssvdag_sets(svd_octree* o, const int* fmts, int K) {
   svd_ctx* c = o->ctx;
-  const std::vector<Level>& lv = (fmt == SVD_FMT_SSVDAG) ? o->sdag : o->dag;
   const int L = (int)o->levels;
   if (L < 3) throw Error{SVD_ERR_LEVELS, "SSVDAG encoding needs at least 3 levels"};
-  for (int l = 0; l < L; ++l)
-    if (lv[l].n > (1ull << 30)) throw Error{SVD_ERR_CAPACITY, "level too big for encoding in 30 bits"};
-  std::vector<uint32_t*> perm(L, nullptr), ind(L, nullptr), offs(L, nullptr);
-  std::vector<uint32_t> innerSize(L, 0);
-  // dev[0..L-1]: inner-size totals per level, dev[L]: pointer-size error flag
-  uint32_t* dev = dalloc<uint32_t>(c, L + 1);
-  uint32_t* err = dev + L;   // dev is cleared by k_level_iota below
+  const int P = L - 1;                       // levels 0..L-2 of each set in the span
+  std::vector<const std::vector<Level>*> lvs(K);
+  for (int s = 0; s < K; ++s) {
+    lvs[s] = fmts[s] == SVD_FMT_SSVDAG ? &o->sdag : &o->dag;
+    for (int l = 0; l < L; ++l)
+      if ((*lvs[s])[l].n > (1ull << 30)) throw Error{SVD_ERR_CAPACITY, "level too big for encoding in 30 bits"};
+  }
+  // perm / ind / offs of set s, level lev at [s * L + lev]
+  std::vector<uint32_t*> perm(K * L, nullptr), ind(K * L, nullptr), offs(K * L, nullptr);
+  // dev[s * L + lev]: inner-size totals per level, dev[K * L]: pointer-size error flag
+  uint32_t* dev = dalloc<uint32_t>(c, K * L + 1);
+  uint32_t* err = dev + K * L;   // dev is cleared by k_level_iota below
   // Phase 1a: the reference counts of every level land in one array, and one
-  // segmented sort orders all levels (each level is its own std::sort call).
-  std::vector<uint64_t> off(L, 0), rfirst, rlast;
-  uint64_t tot = 0;
-  for (int lev = 0; lev <= L - 2; ++lev) { off[lev] = tot; tot += lv[lev].n; }
+  // segmented sort orders all levels of all sets.
+  Span ua{};
+  ua.L = (uint32_t)(K * P);
+  ua.P = (uint32_t)P;
+  std::vector<uint64_t> off(K * L, 0), rfirst, rlast;
+  uint64_t tot = 0, maxParent = 0;
+  for (int s = 0; s < K; ++s)
+    for (int lev = 0; lev <= L - 2; ++lev) {
+      const Level& v = (*lvs[s])[lev];
+      off[s * L + lev] = tot;
+      ua.rec[s * P + lev] = v.rec;
+      ua.start[s * P + lev] = (uint32_t)tot;
+      if (lev > 0 && v.n > 0) {
+        rfirst.push_back(tot);
+        rlast.push_back(tot + v.n);
+      }
+      if (lev <= L - 3) maxParent = std::max<uint64_t>(maxParent, v.n);
+      tot += v.n;
+    }
+  if (tot >= 0xFFFFFFFFull) throw Error{SVD_ERR_CAPACITY, "too many nodes for one refcount order"};
+  ua.start[K * P] = (uint32_t)tot;
   uint32_t* permAll = dalloc<uint32_t>(c, tot + 1);
   uint32_t* hist = dalloc<uint32_t>(c, tot + 1);   // cleared by k_level_iota
-  // levels 0..L-2 as one concatenation (start = off)
-  std::vector<Level> upper(lv.begin(), lv.begin() + (L - 1));
-  const LevelSet ua = level_set(upper, (uint32_t)(L - 1), false);
-  uint64_t maxParent = 0;
-  for (int lev = 0; lev <= L - 2; ++lev) {
-    perm[lev] = permAll + off[lev];
-    if (lev > 0 && lv[lev].n > 0) {
-      rfirst.push_back(off[lev]);
-      rlast.push_back(off[lev] + lv[lev].n);
-    }
-    if (lev <= L - 3) maxParent = std::max<uint64_t>(maxParent, lv[lev].n);
-  }
-  k_level_iota<<<div_up(std::max<uint64_t>(tot, (uint64_t)L + 1), kB), kB, 0, c->stream>>>(ua, tot, permAll, hist,
-                                                                                          dev, (uint32_t)(L + 1));
+  for (int s = 0; s < K; ++s)
+    for (int lev = 0; lev <= L - 2; ++lev) perm[s * L + lev] = permAll + off[s * L + lev];
+  k_level_iota<<<div_up(std::max<uint64_t>(tot, (uint64_t)(K * L + 1)), kB), kB, 0, c->stream>>>(
+      ua, tot, permAll, hist, dev, (uint32_t)(K * L + 1));
   check_launch("k_level_iota");
   if (maxParent) {
-    const dim3 rg((uint32_t)std::min<uint64_t>(div_up(maxParent, kB), kRcBlocks), (uint32_t)(L - 2));
+    const dim3 rg((uint32_t)std::min<uint64_t>(div_up(maxParent, kB), kRcBlocks), (uint32_t)(K * (L - 2)));
     k_refcount<<<rg, kB, 0, c->stream>>>(ua, hist, err);
   }
   check_launch("k_refcount");
@@ -432,108 +481,130 @@ void encode_ssvdag(svd_octree* o, int fmt) {
   dfree(c, hist);
   // Phase 1b, bottom-up: children addresses of each level in its order.
   for (int lev = L - 2; lev >= 0; --lev) {
-    const uint64_t n = lv[lev].n;
-    ind[lev] = dalloc<uint32_t>(c, n + 1);
-    if (lev == L - 2) {
-      if (n) k_scatter_rank<<<div_up(n, kB), kB, 0, c->stream>>>(perm[lev], n, ind[lev]);
-      check_launch("k_scatter_rank");
-    } else {
-      offs[lev] = dalloc<uint32_t>(c, n + 1);
+    for (int s = 0; s < K; ++s) {
+      const Level& v = (*lvs[s])[lev];
+      const uint64_t n = v.n;
+      const int q = s * L + lev;
+      ind[q] = dalloc<uint32_t>(c, n + 1);
+      if (lev == L - 2) {
+        if (n) k_scatter_rank<<<div_up(n, kB), kB, 0, c->stream>>>(perm[q], n, ind[q]);
+        check_launch("k_scatter_rank");
+        continue;
+      }
+      offs[q] = dalloc<uint32_t>(c, n + 1);
       if (n && n <= kLayoutSmall) {
-        k_inner_layout_small<<<1, kLayoutThreads, 0, c->stream>>>(lv[lev].rec, perm[lev], (uint32_t)n, ind[lev + 1],
-                                                                  offs[lev], ind[lev], dev + lev, err);
+        k_inner_layout_small<<<1, kLayoutThreads, 0, c->stream>>>(v.rec, perm[q], (uint32_t)n, ind[q + 1], offs[q],
+                                                                  ind[q], dev + q, err);
         check_launch("k_inner_layout_small");
         continue;
       }
       if (n) {
-        k_inner_sizes<<<div_up(n, kB), kB, 0, c->stream>>>(lv[lev].rec, perm[lev], n, ind[lev + 1], offs[lev], err);
+        k_inner_sizes<<<div_up(n, kB), kB, 0, c->stream>>>(v.rec, perm[q], n, ind[q + 1], offs[q], err);
         check_launch("k_inner_sizes");
       }
-      scan_exclusive(c, offs[lev], offs[lev], n, dev + lev);
-      if (n) k_offsets_to_ind<<<div_up(n, kB), kB, 0, c->stream>>>(perm[lev], offs[lev], n, ind[lev]);
+      scan_exclusive(c, offs[q], offs[q], n, dev + q);
+      if (n) k_offsets_to_ind<<<div_up(n, kB), kB, 0, c->stream>>>(perm[q], offs[q], n, ind[q]);
       check_launch("k_offsets_to_ind");
     }
   }
-  {
-    std::vector<uint32_t> h(L + 1);
-    read_u32s(c, dev, h.data(), L + 1);
-    if (h[L]) throw Error{SVD_ERR_CAPACITY, "unsupported pointer size (>= 2^30)"};
-    for (int lev = 0; lev < L - 2; ++lev) innerSize[lev] = h[lev];
-  }
-  // Layout of the file image.
-  std::vector<uint32_t> levelOffsets(L - 2, 0);
-  for (int i = 1; i < L - 2; ++i) levelOffsets[i] = innerSize[i - 1] + levelOffsets[i - 1];
-  uint64_t nInner = 0;
-  for (int i = 0; i < L - 2; ++i) nInner += innerSize[i];
-  const uint64_t nBricks = lv[L - 2].n;
-  const uint64_t nNodes = (fmt == SVD_FMT_SSVDAG) ? o->nNodesSDAG : o->nNodesDAG;
-  std::vector<uint8_t> head = base_header(o, nNodes);
-  const uint32_t nInner32 = (uint32_t)nInner;
-  put(head, &nInner32, 4);
-  const uint64_t innerAt = head.size();
-  const uint64_t leafCntAt = innerAt + 2 * nInner;
-  const uint64_t leafAt = leafCntAt + 4;
-  const uint64_t tailAt = leafAt + 8 * nBricks;
-  std::vector<uint8_t> mid, tail;
-  const uint32_t nLeafBytes = (uint32_t)(8 * nBricks);
-  put(mid, &nLeafBytes, 4);
-  const uint32_t nOff = (uint32_t)levelOffsets.size();
-  put(tail, &nOff, 4);
-  put(tail, levelOffsets.data(), 4 * levelOffsets.size());
-  Encoded& e = o->enc[fmt];
-  dfree(c, e.d);
-  e.bytes = tailAt + tail.size();
-  e.d = dalloc<uint8_t>(c, e.bytes);
-  put_bytes(c, e.d, head);
-  put_bytes(c, e.d + leafCntAt, mid);
-  put_bytes(c, e.d + tailAt, tail);
-  // Phase 2: write every level into the image (inner levels in one launch).
-  uint16_t* inner = (uint16_t*)(e.d + innerAt);
-  if (off[L - 2]) {
-    InnerLevels ia{};
-    ia.L = (uint32_t)(L - 2);
-    for (int lev = 0; lev <= L - 3; ++lev) {
-      ia.rec[lev] = lv[lev].rec;
-      ia.indChild[lev] = ind[lev + 1];
-      ia.offs[lev] = offs[lev];
-      ia.outOff[lev] = levelOffsets[lev];
-      ia.start[lev] = (uint32_t)off[lev];
+  std::vector<uint32_t> h(K * L + 1);
+  read_u32s(c, dev, h.data(), K * L + 1);
+  if (h[K * L]) throw Error{SVD_ERR_CAPACITY, "unsupported pointer size (>= 2^30)"};
+  for (int s = 0; s < K; ++s) {
+    const std::vector<Level>& lv = *lvs[s];
+    const int fmt = fmts[s];
+    // Layout of the file image.
+    std::vector<uint32_t> levelOffsets(L - 2, 0);
+    for (int i = 1; i < L - 2; ++i) levelOffsets[i] = h[s * L + i - 1] + levelOffsets[i - 1];
+    uint64_t nInner = 0;
+    for (int i = 0; i < L - 2; ++i) nInner += h[s * L + i];
+    const uint64_t nBricks = lv[L - 2].n;
+    const uint64_t nNodes = (fmt == SVD_FMT_SSVDAG) ? o->nNodesSDAG : o->nNodesDAG;
+    std::vector<uint8_t> head = base_header(o, nNodes);
+    const uint32_t nInner32 = (uint32_t)nInner;
+    put(head, &nInner32, 4);
+    const uint64_t innerAt = head.size();
+    const uint64_t leafCntAt = innerAt + 2 * nInner;
+    const uint64_t leafAt = leafCntAt + 4;
+    const uint64_t tailAt = leafAt + 8 * nBricks;
+    std::vector<uint8_t> mid, tail;
+    const uint32_t nLeafBytes = (uint32_t)(8 * nBricks);
+    put(mid, &nLeafBytes, 4);
+    const uint32_t nOff = (uint32_t)levelOffsets.size();
+    put(tail, &nOff, 4);
+    put(tail, levelOffsets.data(), 4 * levelOffsets.size());
+    Encoded& e = o->enc[fmt];
+    dfree(c, e.d);
+    e.bytes = tailAt + tail.size();
+    e.d = dalloc<uint8_t>(c, e.bytes);
+    put_bytes(c, e.d, head);
+    put_bytes(c, e.d + leafCntAt, mid);
+    put_bytes(c, e.d + tailAt, tail);
+    // Phase 2: write every level into the image (inner levels in one launch).
+    uint16_t* inner = (uint16_t*)(e.d + innerAt);
+    const uint64_t base = off[s * L];
+    const uint64_t nInnerNodes = off[s * L + L - 2] - base;
+    if (nInnerNodes) {
+      InnerLevels ia{};
+      ia.L = (uint32_t)(L - 2);
+      for (int lev = 0; lev <= L - 3; ++lev) {
+        ia.rec[lev] = lv[lev].rec;
+        ia.indChild[lev] = ind[s * L + lev + 1];
+        ia.offs[lev] = offs[s * L + lev];
+        ia.outOff[lev] = levelOffsets[lev];
+        ia.start[lev] = (uint32_t)(off[s * L + lev] - base);
+      }
+      ia.start[L - 2] = (uint32_t)nInnerNodes;
+      k_inner_write<<<div_up(nInnerNodes, kB), kB, 0, c->stream>>>(ia, permAll + base, nInnerNodes, inner);
     }
-    ia.start[L - 2] = (uint32_t)off[L - 2];
-    k_inner_write<<<div_up(off[L - 2], kB), kB, 0, c->stream>>>(ia, permAll, off[L - 2], inner);
+    if (nBricks)
+      k_brick_write<<<div_up(nBricks, kB), kB, 0, c->stream>>>(lv[L - 2].rec, perm[s * L + L - 2], nBricks,
+                                                                lv[L - 1].rec, (uint16_t*)(e.d + leafAt));
+    check_launch("ssvdag write");
+    e.valid = true;
   }
-  if (nBricks)
-    k_brick_write<<<div_up(nBricks, kB), kB, 0, c->stream>>>(lv[L - 2].rec, perm[L - 2], nBricks, lv[L - 1].rec,
-                                                              (uint16_t*)(e.d + leafAt));
-  check_launch("ssvdag write");
-  for (int l = 0; l < L; ++l) { dfree(c, ind[l]); dfree(c, offs[l]); }
+  for (auto* p : ind) dfree(c, p);
+  for (auto* p : offs) dfree(c, p);
   dfree(c, permAll); dfree(c, dev);
-  e.valid = true;
 }
 
 }  // namespace
 
-void encode(svd_octree* o, int fmt) {
+namespace {
+void check_state(const svd_octree* o, int fmt) {
   switch (fmt) {
     case SVD_FMT_SVDAG:
       if (o->state < SVD_S_DAG) throw Error{SVD_ERR_STATE, "SVDAG encoding needs a DAG"};
-      encode_words(o, fmt);
       break;
     case SVD_FMT_USSVDAG:
       if (o->state != SVD_S_SDAG) throw Error{SVD_ERR_STATE, "USSVDAG encoding needs an SDAG"};
-      encode_words(o, fmt);
       break;
     case SVD_FMT_SSVDAG:
       if (o->state != SVD_S_SDAG) throw Error{SVD_ERR_STATE, "SSVDAG encoding needs an SDAG"};
-      encode_ssvdag(o, fmt);
       break;
     case SVD_FMT_ESVDAG:
       if (o->state < SVD_S_DAG) throw Error{SVD_ERR_STATE, "ESVDAG encoding needs a DAG"};
-      encode_ssvdag(o, fmt);
       break;
     default:
       throw Error{SVD_ERR_ARG, "unknown format"};
   }
+}
+}  // namespace
+
+void encode(svd_octree* o, int fmt) { encode_formats(o, &fmt, 1); }
+
+void encode_formats(svd_octree* o, const int* fmts, int n) {
+  int words[2], ss[2], nw = 0, nss = 0;
+  bool seen[4] = {false, false, false, false};
+  for (int k = 0; k < n; ++k) {
+    check_state(o, fmts[k]);
+    if (seen[fmts[k]]) throw Error{SVD_ERR_ARG, "a format is listed twice"};
+    seen[fmts[k]] = true;
+    if (fmts[k] == SVD_FMT_SVDAG || fmts[k] == SVD_FMT_USSVDAG) words[nw++] = fmts[k];
+    else ss[nss++] = fmts[k];
+  }
+  if (nw) encode_words_sets(o, words, nw);
+  if (nss) encode_ssvdag_sets(o, ss, nss);
 }
 
 }  // namespace svd

@@ -204,6 +204,9 @@ uint64_t attr_bittree(svd_ctx* c, const uint8_t* dattrs, uint64_t V, int gray, u
 
 // ---- encoders (encode.hip) -----------------------------------------------------
 void encode(svd_octree* o, int fmt);
+// several formats (each at most once); .svdag/.ussvdag share one size read,
+// .ssvdag/.esvdag share every pass (one segmented refcount order for both)
+void encode_formats(svd_octree* o, const int* fmts, int n);
 // .svdag file image -> DAG-state octree (decode.hip)
 void octree_from_svdag(svd_octree* o, const uint8_t* file, uint64_t bytes, bool onDevice);
 

@@ -74,6 +74,12 @@ def lossless_parity(ctx, dev, L, name):
         files[fmt] = g.encoded_bytes(fmt)
         assert files[fmt] == o.encode(ofmt), f"{name}: {S.FMT_EXT[fmt]} differs"
         say(f"  {S.FMT_EXT[fmt]} identical")
+    # the four images again in one call with shared passes (the bench's build step)
+    want = {f: g.encoded_bytes(f) for f in (S.FMT_SVDAG, S.FMT_ESVDAG, S.FMT_USSVDAG, S.FMT_SSVDAG)}
+    g.encode_many(list(want))
+    for f, data in want.items():
+        assert g.encoded_bytes(f) == data, f"{name}: {S.FMT_EXT[f]} differs when encoded with the others"
+    say("  shared-pass encode of the four images identical")
     return g, files
 
 

@@ -426,3 +426,31 @@ def test_attr_bittree_full_size(ctx):
     for gray in (True, False):
         blob = ctx.attr_bittree(a, gray)
         np.testing.assert_array_equal(O.attr_bittree_decode(blob, a.size), a)
+
+
+@pytest.mark.parametrize("name", list(SCENES))
+def test_encode_formats_shared_passes_equal_oracle(ctx, name):
+    """svd_encode_formats: the four images in one call (the .ssvdag / .esvdag
+    pair in shared passes with one segmented refcount order, the .svdag /
+    .ussvdag pair with one size read) equal the oracle's; subsets too."""
+    codes, L = SCENES[name]()
+    if L < 3:
+        pytest.skip("SSVDAG layouts need 3 levels")
+    print(f"[case] encode_formats {name} L={L}", flush=True)
+    _, _, _, files_o = oracle_pipeline(codes, L)
+    g = S.Octree.from_morton(ctx, codes, L)
+    g.to_dag()
+    sizes = g.encode_many([S.FMT_ESVDAG, S.FMT_SVDAG])            # DAG state: the DAG's two images
+    assert sizes == [len(files_o[S.FMT_ESVDAG]), len(files_o[S.FMT_SVDAG])]
+    assert g.encoded_bytes(S.FMT_ESVDAG) == files_o[S.FMT_ESVDAG]
+    assert g.encoded_bytes(S.FMT_SVDAG) == files_o[S.FMT_SVDAG]
+    with pytest.raises(S.SvdError):
+        g.encode_many([S.FMT_SSVDAG])                               # needs the SDAG
+    g.to_sdag()
+    allf = [S.FMT_SSVDAG, S.FMT_USSVDAG, S.FMT_ESVDAG, S.FMT_SVDAG]
+    sizes = g.encode_many(allf)
+    for f, n in zip(allf, sizes):
+        assert n == len(files_o[f]) and g.encoded_bytes(f) == files_o[f], f
+    with pytest.raises(S.SvdError):
+        g.encode_many([S.FMT_SVDAG, S.FMT_SVDAG])                   # a format twice
+    g.free()
