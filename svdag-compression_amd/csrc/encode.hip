@@ -109,12 +109,21 @@ __device__ __forceinline__ uint32_t span_level(const Span& a, uint64_t i) {
 // straight to the global histogram) and flushes one add per distinct child.
 constexpr int kRcSlots = 1024;
 constexpr int kRcBlocks = 2048;
-// Levels 1..L-2 of every set in one launch: blockIdx.y = parent level
-// (P - 1 of them per set), whose children are the next span level (element
-// start[l+1] + child of the concatenated histogram).
-__global__ void __launch_bounds__(256) k_refcount(Span a, uint32_t* __restrict__ histAll,
+// Levels 1..L-2 of every set in one launch.  Parent level job j is span
+// level lev[j], whose children are the next span level (element start[l+1] +
+// child of the concatenated histogram); it owns workgroups blk0[j] ..
+// blk0[j+1]-1, one per 256 parents up to kRcBlocks.
+struct RcJobs {
+  uint32_t lev[2 * kMaxLevels];
+  uint32_t blk0[2 * kMaxLevels + 1];
+  uint32_t n;
+};
+__global__ void __launch_bounds__(256) k_refcount(Span a, RcJobs jb, uint32_t* __restrict__ histAll,
                                                   uint32_t* __restrict__ err) {
-  const uint32_t l = (blockIdx.y / (a.P - 1)) * a.P + blockIdx.y % (a.P - 1);
+  uint32_t j = 0;
+  while (j + 1 < jb.n && blockIdx.x >= jb.blk0[j + 1]) ++j;
+  const uint32_t l = jb.lev[j];
+  const uint32_t nb = jb.blk0[j + 1] - jb.blk0[j], b = blockIdx.x - jb.blk0[j];
   const uint32_t* __restrict__ parent = a.rec[l];
   const uint64_t np = a.start[l + 1] - a.start[l];
   uint32_t* __restrict__ hist = histAll + a.start[l + 1];
@@ -122,8 +131,8 @@ __global__ void __launch_bounds__(256) k_refcount(Span a, uint32_t* __restrict__
   __shared__ uint32_t key[kRcSlots], cnt[kRcSlots];
   for (int t = threadIdx.x; t < kRcSlots; t += blockDim.x) { key[t] = kNone; cnt[t] = 0; }
   __syncthreads();
-  const uint64_t chunk = (np + gridDim.x - 1) / gridDim.x;
-  const uint64_t lo = (uint64_t)blockIdx.x * chunk, hi = lo + chunk < np ? lo + chunk : np;
+  const uint64_t chunk = (np + nb - 1) / nb;
+  const uint64_t lo = (uint64_t)b * chunk, hi = lo + chunk < np ? lo + chunk : np;
   for (uint64_t i = lo + threadIdx.x; i < hi; i += blockDim.x) {
     const uint32_t* r = parent + i * kRecWords;
 #pragma unroll
@@ -449,7 +458,7 @@ void encode_ssvdag_sets(svd_octree* o, const int* fmts, int K) {
   ua.L = (uint32_t)(K * P);
   ua.P = (uint32_t)P;
   std::vector<uint64_t> off(K * L, 0), rfirst, rlast;
-  uint64_t tot = 0, maxParent = 0;
+  uint64_t tot = 0;
   for (int s = 0; s < K; ++s)
     for (int lev = 0; lev <= L - 2; ++lev) {
       const Level& v = (*lvs[s])[lev];
@@ -460,7 +469,6 @@ void encode_ssvdag_sets(svd_octree* o, const int* fmts, int K) {
         rfirst.push_back(tot);
         rlast.push_back(tot + v.n);
       }
-      if (lev <= L - 3) maxParent = std::max<uint64_t>(maxParent, v.n);
       tot += v.n;
     }
   if (tot >= 0xFFFFFFFFull) throw Error{SVD_ERR_CAPACITY, "too many nodes for one refcount order"};
@@ -472,10 +480,16 @@ void encode_ssvdag_sets(svd_octree* o, const int* fmts, int K) {
   k_level_iota<<<div_up(std::max<uint64_t>(tot, (uint64_t)(K * L + 1)), kB), kB, 0, c->stream>>>(
       ua, tot, permAll, hist, dev, (uint32_t)(K * L + 1));
   check_launch("k_level_iota");
-  if (maxParent) {
-    const dim3 rg((uint32_t)std::min<uint64_t>(div_up(maxParent, kB), kRcBlocks), (uint32_t)(K * (L - 2)));
-    k_refcount<<<rg, kB, 0, c->stream>>>(ua, hist, err);
-  }
+  RcJobs jb{};
+  for (int s = 0; s < K; ++s)
+    for (int lev = 0; lev <= L - 3; ++lev) {
+      const uint64_t np = (*lvs[s])[lev].n;
+      if (!np) continue;
+      jb.lev[jb.n] = (uint32_t)(s * P + lev);
+      jb.blk0[jb.n + 1] = jb.blk0[jb.n] + (uint32_t)std::min<uint64_t>(div_up(np, kB), kRcBlocks);
+      ++jb.n;
+    }
+  if (jb.n) k_refcount<<<jb.blk0[jb.n], kB, 0, c->stream>>>(ua, jb, hist, err);
   check_launch("k_refcount");
   refsort_ranges(c, permAll, hist, tot, rfirst.data(), rlast.data(), rfirst.size());
   dfree(c, hist);

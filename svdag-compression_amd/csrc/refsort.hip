@@ -293,8 +293,19 @@ __global__ void k_heap_sort(const Seg* __restrict__ segs, const uint32_t* __rest
 // ---------------------------------------------------------------- workgroup phase
 // One workgroup runs the complete introsort recursion of one segment of
 // 17..2048 elements in LDS, all subsegments of one depth together.
+// Element k of thread (wave w, lane j) is w * 64 * kPer + 64 * k + j: every
+// LDS access of a per-element loop hits 64 consecutive words, and a wave's
+// stopper prefix comes from ballots instead of a workgroup scan.
+__device__ __forceinline__ uint32_t blk_elem(int tid, int k) {
+  return (uint32_t)((tid >> 6) * (64 * kPer) + 64 * k + (tid & 63));
+}
+__device__ __forceinline__ uint32_t lanes_below(uint64_t m) {
+  return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
 __global__ void __launch_bounds__(kBT) k_block_sort(const Seg* __restrict__ segs, const uint32_t* __restrict__ nsp,
                                                     uint32_t* val, uint32_t* idx) {
+  constexpr int kWaves = kBT / 64;
   __shared__ uint32_t sv[kLdsN], si[kLdsN];
   __shared__ uint16_t segOf[kLdsN];
   __shared__ uint16_t exL[kLdsN + 1], exR[kLdsN + 1];
@@ -304,7 +315,7 @@ __global__ void __launch_bounds__(kBT) k_block_sort(const Seg* __restrict__ segs
   __shared__ uint32_t aPivot[128];
   __shared__ uint16_t aK[128], aCut[128], aLeft[128], aRight[128];
   __shared__ uint16_t lfFirst[kLdsN / 2], lfLast[kLdsN / 2];   // leaves of 2..16 elements
-  __shared__ uint32_t lds[kBT / 64 + 1];
+  __shared__ uint32_t wTot[2][kWaves];
   __shared__ uint32_t nActive[2];
   __shared__ uint32_t nLeaf;
   // segOf: < kLeafTag active segment slot, kLeafTag | q leaf q, kDone finished
@@ -312,7 +323,7 @@ __global__ void __launch_bounds__(kBT) k_block_sort(const Seg* __restrict__ segs
   if (blockIdx.x >= *nsp) return;
   const Seg g = segs[blockIdx.x];
   const uint32_t n = g.last - g.first;
-  const int tid = threadIdx.x;
+  const int tid = threadIdx.x, wv = tid >> 6;
   for (uint32_t k = tid; k < n; k += kBT) { sv[k] = val[g.first + k]; si[k] = idx[g.first + k]; segOf[k] = 0; }
   if (tid == 0) {
     nActive[0] = 0;
@@ -342,40 +353,50 @@ __global__ void __launch_bounds__(kBT) k_block_sort(const Seg* __restrict__ segs
     }
     if (tid == 0) nActive[nxt] = 0;
     __syncthreads();
-    // 2. stopper flags + workgroup prefix (exclusive) over all n positions
-    uint32_t fl[kPer], fr[kPer], sl = 0, sr = 0;
+    // 2. stopper flags and their exclusive prefix over all kLdsN positions
+    bool fl[kPer], fr[kPer];
+    uint32_t pl[kPer], pr[kPer], cl = 0, cr = 0;
 #pragma unroll
     for (int k = 0; k < kPer; ++k) {
-      const uint32_t e = tid * kPer + k;
-      fl[k] = fr[k] = 0;
+      const uint32_t e = blk_elem(tid, k);
+      fl[k] = fr[k] = false;
       if (e < n) {
         const uint32_t s = segOf[e];
         if (s < kLeafTag) {
           const uint32_t p = aPivot[s];
-          fl[k] = (e > aFirst[cur][s] && sv[e] <= p) ? 1u : 0u;
-          fr[k] = (sv[e] >= p) ? 1u : 0u;
+          fl[k] = e > aFirst[cur][s] && sv[e] <= p;
+          fr[k] = sv[e] >= p;
         }
       }
-      sl += fl[k]; sr += fr[k];
+      const uint64_t bl = __ballot(fl[k]), br = __ballot(fr[k]);
+      pl[k] = cl + lanes_below(bl);
+      pr[k] = cr + lanes_below(br);
+      cl += (uint32_t)__popcll(bl);
+      cr += (uint32_t)__popcll(br);
     }
-    uint32_t totL, totR;
-    uint32_t pl = block_exclusive_sum(sl, lds, totL);
-    uint32_t pr = block_exclusive_sum(sr, lds, totR);
+    if ((tid & 63) == 0) { wTot[0][wv] = cl; wTot[1][wv] = cr; }
+    __syncthreads();
+    uint32_t offL = 0, offR = 0, totL = 0, totR = 0;
+#pragma unroll
+    for (int v = 0; v < kWaves; ++v) {
+      const uint32_t a = wTot[0][v], b = wTot[1][v];
+      if (v < wv) { offL += a; offR += b; }
+      totL += a; totR += b;
+    }
 #pragma unroll
     for (int k = 0; k < kPer; ++k) {
-      const uint32_t e = tid * kPer + k;
-      if (e < kLdsN) { exL[e] = (uint16_t)pl; exR[e] = (uint16_t)pr; }
-      pl += fl[k]; pr += fr[k];
+      const uint32_t e = blk_elem(tid, k);
+      exL[e] = (uint16_t)(offL + pl[k]);
+      exR[e] = (uint16_t)(offR + pr[k]);
     }
     if (tid == 0) { exL[kLdsN] = (uint16_t)totL; exR[kLdsN] = (uint16_t)totR; }
     __syncthreads();
     // 3. ranks -> Lp / Rp
 #pragma unroll
     for (int k = 0; k < kPer; ++k) {
-      const uint32_t e = tid * kPer + k;
-      if (e >= n) continue;
+      if (!fl[k] && !fr[k]) continue;
+      const uint32_t e = blk_elem(tid, k);
       const uint32_t s = segOf[e];
-      if (s >= kLeafTag) continue;
       const uint32_t f = aFirst[cur][s], l = aLast[cur][s];
       if (fl[k]) Lp[f + (exL[e] - exL[f])] = (uint16_t)e;
       if (fr[k]) Rp[f + (exR[l] - 1 - exR[e])] = (uint16_t)e;
@@ -384,7 +405,7 @@ __global__ void __launch_bounds__(kBT) k_block_sort(const Seg* __restrict__ segs
     // 4. paired swaps, K
 #pragma unroll
     for (int k = 0; k < kPer; ++k) {
-      const uint32_t e = tid * kPer + k;
+      const uint32_t e = blk_elem(tid, k);
       if (e >= n) continue;
       const uint32_t s = segOf[e];
       if (s >= kLeafTag) continue;
@@ -393,11 +414,11 @@ __global__ void __launch_bounds__(kBT) k_block_sort(const Seg* __restrict__ segs
       const uint32_t m = min(nL, nR);
       const uint32_t kk = e - f;
       if (kk >= m) continue;
-      const uint32_t a = Lp[f + kk], b = Rp[f + kk];
+      const uint32_t a = Lp[e], b = Rp[e];
       if (a < b) {
         const uint32_t tv = sv[a], ti = si[a];
         sv[a] = sv[b]; si[a] = si[b]; sv[b] = tv; si[b] = ti;
-        const bool nextOk = (kk + 1 < m) && (Lp[f + kk + 1] < Rp[f + kk + 1]);
+        const bool nextOk = (kk + 1 < m) && (Lp[e + 1] < Rp[e + 1]);
         if (!nextOk) aK[s] = (uint16_t)(kk + 1);
       }
     }
@@ -412,22 +433,22 @@ __global__ void __launch_bounds__(kBT) k_block_sort(const Seg* __restrict__ segs
       const int d = aDepth[cur][tid] - 1;
       aCut[tid] = (uint16_t)cut;
       uint16_t ids[2];
-      const uint32_t cf[2] = {f, cut}, cl[2] = {cut, l};
+      const uint32_t cf[2] = {f, cut}, cl2[2] = {cut, l};
       for (int h = 0; h < 2; ++h) {
-        const uint32_t sz = cl[h] - cf[h];
+        const uint32_t sz = cl2[h] - cf[h];
         ids[h] = kDone;
         LArr a{sv + cf[h], si + cf[h]};
         if (sz <= (uint32_t)kThreshold) {
           if (sz > 1) {   // sorted after the rounds, all leaves at once
             const uint32_t q = atomicAdd(&nLeaf, 1u);
             lfFirst[q] = (uint16_t)cf[h];
-            lfLast[q] = (uint16_t)cl[h];
+            lfLast[q] = (uint16_t)cl2[h];
             ids[h] = (uint16_t)(kLeafTag | q);
           }
         } else if (d == 0) heap_sort(a, sz);
         else {
           const uint32_t slotn = atomicAdd(&nActive[nxt], 1u);
-          aFirst[nxt][slotn] = (uint16_t)cf[h]; aLast[nxt][slotn] = (uint16_t)cl[h]; aDepth[nxt][slotn] = (int8_t)d;
+          aFirst[nxt][slotn] = (uint16_t)cf[h]; aLast[nxt][slotn] = (uint16_t)cl2[h]; aDepth[nxt][slotn] = (int8_t)d;
           ids[h] = (uint16_t)slotn;
         }
       }
@@ -437,7 +458,7 @@ __global__ void __launch_bounds__(kBT) k_block_sort(const Seg* __restrict__ segs
     // 6. new segment ids
 #pragma unroll
     for (int k = 0; k < kPer; ++k) {
-      const uint32_t e = tid * kPer + k;
+      const uint32_t e = blk_elem(tid, k);
       if (e >= n) continue;
       const uint32_t s = segOf[e];
       if (s >= kLeafTag) continue;
@@ -453,7 +474,7 @@ __global__ void __launch_bounds__(kBT) k_block_sort(const Seg* __restrict__ segs
   uint16_t rp[kPer];
 #pragma unroll
   for (int k = 0; k < kPer; ++k) {
-    const uint32_t e = tid * kPer + k;
+    const uint32_t e = blk_elem(tid, k);
     rp[k] = kDone;
     if (e >= n) continue;
     const uint32_t s = segOf[e];
