@@ -28,6 +28,8 @@ constexpr int kBT = 256;              // threads per workgroup
 constexpr int kPer = kLdsN / kBT;     // elements per thread in the workgroup phase
 constexpr int kTile = 2048;           // elements per workgroup in the device-wide rounds
 constexpr uint32_t kLeafTag = 0x8000u; // workgroup phase: segOf tags
+constexpr uint32_t kEqTag = 0xC000u;
+constexpr uint32_t kTagMask = 0xC000u;
 constexpr uint32_t kDone = 0xFFFFu;
 
 struct Seg { uint32_t first, last; int32_t depth; uint32_t pad; };
@@ -128,37 +130,82 @@ struct Lists {
   Seg* leaf; uint32_t* nleaf;
 };
 
-__device__ __forceinline__ void classify(const Lists& L, uint32_t first, uint32_t last, int depth) {
-  const uint32_t size = last - first;
-  Seg s{first, last, depth, 0};
-  if (size <= (uint32_t)kThreshold) { if (size > 1) L.leaf[atomicAdd(L.nleaf, 1u)] = s; }
-  else if (size <= (uint32_t)kLdsN) L.small[atomicAdd(L.nsmall, 1u)] = s;
-  else if (depth == 0) L.heap[atomicAdd(L.nheap, 1u)] = s;
-  else L.big[atomicAdd(L.nbig, 1u)] = s;
-}
-
 // ---------------------------------------------------------------- device-wide rounds
 struct SegState { uint32_t pivot, K, nL, nR, blk0, nblk, cut, pad; };
 
-// nblk has cap+1 entries; every entry at or past the live count is zeroed so
-// that a scan over all of them ends in the total tile count.
-__global__ void k_g_median(const Seg* __restrict__ segs, const uint32_t* __restrict__ nsp, uint32_t cap,
-                           uint32_t* val, uint32_t* idx, SegState* __restrict__ st, uint32_t* __restrict__ nblk,
-                           uint32_t* __restrict__ nnext) {
-  uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s == 0) *nnext = 0;   // the list this round's splits append to
-  if (s > cap) return;
-  if (s >= *nsp) { nblk[s] = 0; return; }
-  const Seg g = segs[s];
-  auto vf = [&](int64_t k) { return val[k]; };
-  const int64_t m = median_pos(vf, g.first, g.last);
-  const uint32_t tv = val[g.first], ti = idx[g.first];
-  val[g.first] = val[m]; idx[g.first] = idx[m];
-  val[m] = tv; idx[m] = ti;
-  SegState z{};
-  z.pivot = val[g.first];
-  st[s] = z;
-  nblk[s] = (g.last - g.first + kTile - 1) / kTile;
+__device__ __forceinline__ uint32_t lanes_below(uint64_t m) {
+  return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+__device__ __forceinline__ uint32_t atomic_read(const uint32_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// The cut of a partitioned segment: L_K if L_K < R_{K-1}, else R_{K-1}
+// (R_{-1} = last).
+__device__ __forceinline__ uint32_t seg_cut(const Seg& g, const SegState& z, const uint32_t* __restrict__ Lpos,
+                                            const uint32_t* __restrict__ Rpos) {
+  const uint32_t rprev = z.K ? Rpos[g.first + z.K - 1] : g.last;
+  return (z.K < z.nL && Lpos[g.first + z.K] < rprev) ? Lpos[g.first + z.K] : rprev;
+}
+
+// One workgroup between two device-wide rounds.  Phase A splits the segments
+// of the round just finished (prev, when there is one) into the four lists;
+// the big children form the new list cur.  Phase B moves the median of three
+// of every segment of cur to its front (its pivot) and lays the segments'
+// tiles out one after another: blk0[s] = first tile of segment s, blk0[ns] =
+// the tile count.
+constexpr int kPlanT = 1024;
+__global__ void __launch_bounds__(kPlanT) k_g_plan(const Seg* __restrict__ prev, const uint32_t* __restrict__ nprev,
+                                                   const uint32_t* __restrict__ Lpos, const uint32_t* __restrict__ Rpos,
+                                                   Lists L, uint32_t* val, uint32_t* idx, SegState* __restrict__ st,
+                                                   uint32_t* __restrict__ blk0) {
+  __shared__ uint32_t lds[kPlanT / 64 + 1];
+  __shared__ uint32_t nBig;
+  const uint32_t tid = threadIdx.x;
+  if (prev) {
+    if (tid == 0) nBig = 0;
+    __syncthreads();
+    const uint32_t np = *nprev;
+    for (uint32_t s = tid; s < np; s += kPlanT) {
+      const Seg g = prev[s];
+      const uint32_t cut = seg_cut(g, st[s], Lpos, Rpos);
+      const uint32_t ff[2] = {cut, g.first}, ll[2] = {g.last, cut};
+      for (int h = 0; h < 2; ++h) {
+        const uint32_t size = ll[h] - ff[h];
+        const Seg c{ff[h], ll[h], g.depth - 1, 0};
+        if (size <= (uint32_t)kThreshold) { if (size > 1) L.leaf[atomicAdd(L.nleaf, 1u)] = c; }
+        else if (size <= (uint32_t)kLdsN) L.small[atomicAdd(L.nsmall, 1u)] = c;
+        else if (c.depth == 0) L.heap[atomicAdd(L.nheap, 1u)] = c;
+        else L.big[atomicAdd(&nBig, 1u)] = c;
+      }
+    }
+    __syncthreads();
+    if (tid == 0) *L.nbig = nBig;
+  }
+  const uint32_t ns = prev ? nBig : atomic_read(L.nbig);
+  uint32_t carry = 0;
+  for (uint32_t s0 = 0; s0 < ns; s0 += kPlanT) {
+    const uint32_t s = s0 + tid;
+    uint32_t nb = 0;
+    if (s < ns) {
+      const Seg g = L.big[s];
+      auto vf = [&](int64_t k) { return val[k]; };
+      const int64_t m = median_pos(vf, g.first, g.last);
+      const uint32_t tv = val[g.first], ti = idx[g.first];
+      val[g.first] = val[m]; idx[g.first] = idx[m];
+      val[m] = tv; idx[m] = ti;
+      SegState z{};
+      z.pivot = val[g.first];
+      st[s] = z;
+      nb = (g.last - g.first + kTile - 1) / kTile;
+    }
+    uint32_t agg;
+    const uint32_t ex = block_exclusive_sum(nb, lds, agg);
+    if (s < ns) blk0[s] = carry + ex;
+    carry += agg;
+  }
+  if (tid == 0) blk0[ns] = carry;
 }
 
 __device__ __forceinline__ int find_seg(const uint32_t* __restrict__ blk0, uint32_t ns, uint32_t b) {
@@ -172,66 +219,113 @@ __device__ __forceinline__ int find_seg(const uint32_t* __restrict__ blk0, uint3
   return (int)lo;
 }
 
-// Per-tile counts of left/right stoppers.
+// Position k of a tile for thread t: wave w covers 64 * (kTile / kBT)
+// consecutive elements, 64 at a time, so ballots give in-order ranks.
+__device__ __forceinline__ uint32_t tile_pos(int t, int k) {
+  return (uint32_t)((t >> 6) * (64 * (kTile / kBT)) + 64 * k + (t & 63));
+}
+
+// Per-tile counts of left stoppers (value <= pivot, past the front) and
+// right stoppers (value >= pivot).
 __global__ void __launch_bounds__(kBT) k_g_count(const Seg* __restrict__ segs, const uint32_t* __restrict__ nsp,
                                                  const uint32_t* __restrict__ blk0, const SegState* __restrict__ st,
                                                  const uint32_t* __restrict__ val, uint32_t* __restrict__ cntL,
                                                  uint32_t* __restrict__ cntR) {
-  __shared__ uint32_t lds[kBT / 64 + 1];
+  __shared__ uint32_t wl[kBT / 64], wr[kBT / 64];
   const int s = find_seg(blk0, *nsp, blockIdx.x);
-  if (s < 0) {   // tiles past the live ones (and the trailing entry) count zero
-    if (threadIdx.x == 0) { cntL[blockIdx.x] = 0; cntR[blockIdx.x] = 0; }
-    return;
-  }
+  if (s < 0) return;
   const Seg g = segs[s];
   const uint32_t p = st[s].pivot;
   const uint64_t base = (uint64_t)g.first + (uint64_t)(blockIdx.x - blk0[s]) * kTile;
   uint32_t l = 0, r = 0;
+#pragma unroll
   for (int k = 0; k < kTile / kBT; ++k) {
-    const uint64_t pos = base + (uint64_t)k * kBT + threadIdx.x;
+    const uint64_t pos = base + tile_pos(threadIdx.x, k);
+    bool fl = false, fr = false;
     if (pos < g.last) {
       const uint32_t v = val[pos];
-      l += (pos > g.first && v <= p) ? 1u : 0u;
-      r += (v >= p) ? 1u : 0u;
+      fl = pos > g.first && v <= p;
+      fr = v >= p;
     }
+    l += (uint32_t)__popcll(__ballot(fl));
+    r += (uint32_t)__popcll(__ballot(fr));
   }
-  l = block_reduce_sum(l, lds);
-  r = block_reduce_sum(r, lds);
-  if (threadIdx.x == 0) { cntL[blockIdx.x] = l; cntR[blockIdx.x] = r; }
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) { wl[w] = l; wr[w] = r; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t tl = 0, tr = 0;
+    for (int v = 0; v < kBT / 64; ++v) { tl += wl[v]; tr += wr[v]; }
+    cntL[blockIdx.x] = tl;
+    cntR[blockIdx.x] = tr;
+  }
 }
 
-// Rank every stopper inside its segment; Lpos[first+k] / Rpos[first+k] = k-th stop.
+// Rank every stopper inside its segment; Lpos[first+k] / Rpos[first+k] = k-th
+// stop.  A tile's offset is the sum of the counts of the tiles before it in
+// its segment, and the segment's totals the sum of all of them (a segment has
+// at most n / kTile tiles).
 __global__ void __launch_bounds__(kBT) k_g_scatter(const Seg* __restrict__ segs, const uint32_t* __restrict__ nsp,
                                                    const uint32_t* __restrict__ blk0, SegState* __restrict__ st,
-                                                   const uint32_t* __restrict__ val, const uint32_t* __restrict__ exL,
-                                                   const uint32_t* __restrict__ exR, uint32_t* __restrict__ Lpos,
+                                                   const uint32_t* __restrict__ val, const uint32_t* __restrict__ cntL,
+                                                   const uint32_t* __restrict__ cntR, uint32_t* __restrict__ Lpos,
                                                    uint32_t* __restrict__ Rpos) {
-  __shared__ uint32_t lds[kBT / 64 + 1];
+  __shared__ uint32_t red[4][kBT / 64];
+  __shared__ uint32_t wl[kBT / 64], wr[kBT / 64];
   const int s = find_seg(blk0, *nsp, blockIdx.x);
   if (s < 0) return;
   const Seg g = segs[s];
   const uint32_t p = st[s].pivot;
   const uint32_t b0 = blk0[s], b1 = blk0[s + 1];
-  const uint32_t segL0 = exL[b0], segR0 = exR[b0];
-  const uint32_t nR = exR[b1] - segR0;
-  if (blockIdx.x == b0 && threadIdx.x == 0) { st[s].nL = exL[b1] - segL0; st[s].nR = nR; }
+  const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+  uint32_t q[4] = {0, 0, 0, 0};   // left before, left total, right before, right total
+  for (uint32_t j = b0 + tid; j < b1; j += kBT) {
+    const uint32_t cl = cntL[j], cr = cntR[j];
+    q[1] += cl; q[3] += cr;
+    if (j < blockIdx.x) { q[0] += cl; q[2] += cr; }
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    q[i] = wave_sum(q[i]);
+    if (lane == 0) red[i][w] = q[i];
+  }
   const uint64_t base = (uint64_t)g.first + (uint64_t)(blockIdx.x - b0) * kTile;
-  uint32_t offL = exL[blockIdx.x] - segL0, offR = exR[blockIdx.x] - segR0;
+  uint64_t bl[kTile / kBT], br[kTile / kBT];
+  uint32_t cl = 0, cr = 0;
+#pragma unroll
   for (int k = 0; k < kTile / kBT; ++k) {
-    const uint64_t pos = base + (uint64_t)k * kBT + threadIdx.x;
-    uint32_t fl = 0, fr = 0;
+    const uint64_t pos = base + tile_pos(tid, k);
+    bool fl = false, fr = false;
     if (pos < g.last) {
       const uint32_t v = val[pos];
-      fl = (pos > g.first && v <= p) ? 1u : 0u;
-      fr = (v >= p) ? 1u : 0u;
+      fl = pos > g.first && v <= p;
+      fr = v >= p;
     }
-    uint32_t aggL, aggR;
-    const uint32_t rl = block_exclusive_sum(fl, lds, aggL);
-    const uint32_t rr = block_exclusive_sum(fr, lds, aggR);
-    if (fl) Lpos[g.first + offL + rl] = (uint32_t)pos;
-    if (fr) Rpos[g.first + (nR - 1 - (offR + rr))] = (uint32_t)pos;
-    offL += aggL;
-    offR += aggR;
+    bl[k] = __ballot(fl);
+    br[k] = __ballot(fr);
+  }
+#pragma unroll
+  for (int k = 0; k < kTile / kBT; ++k) { cl += (uint32_t)__popcll(bl[k]); cr += (uint32_t)__popcll(br[k]); }
+  if (lane == 0) { wl[w] = cl; wr[w] = cr; }
+  __syncthreads();
+  uint32_t t[4] = {0, 0, 0, 0}, offL = 0, offR = 0;
+  for (int v = 0; v < kBT / 64; ++v) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) t[i] += red[i][v];
+    if (v < w) { offL += wl[v]; offR += wr[v]; }
+  }
+  const uint32_t nR = t[3];
+  if (blockIdx.x == b0 && tid == 0) { st[s].nL = t[1]; st[s].nR = nR; }
+  offL += t[0];
+  offR += t[2];
+#pragma unroll
+  for (int k = 0; k < kTile / kBT; ++k) {
+    const uint64_t pos = base + tile_pos(tid, k);
+    const uint64_t below = (1ull << lane) - 1ull;
+    if ((bl[k] >> lane) & 1ull) Lpos[g.first + offL + (uint32_t)__popcll(bl[k] & below)] = (uint32_t)pos;
+    if ((br[k] >> lane) & 1ull) Rpos[g.first + (nR - 1 - (offR + (uint32_t)__popcll(br[k] & below)))] = (uint32_t)pos;
+    offL += (uint32_t)__popcll(bl[k]);
+    offR += (uint32_t)__popcll(br[k]);
   }
 }
 
@@ -259,19 +353,6 @@ __global__ void __launch_bounds__(kBT) k_g_swap(const Seg* __restrict__ segs, co
   }
 }
 
-__global__ void k_g_split(const Seg* __restrict__ segs, const uint32_t* __restrict__ nsp,
-                          const SegState* __restrict__ st, const uint32_t* __restrict__ Lpos,
-                          const uint32_t* __restrict__ Rpos, Lists L) {
-  uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= *nsp) return;
-  const Seg g = segs[s];
-  const SegState z = st[s];
-  const uint32_t rprev = z.K ? Rpos[g.first + z.K - 1] : g.last;
-  const uint32_t cut = (z.K < z.nL && Lpos[g.first + z.K] < rprev) ? Lpos[g.first + z.K] : rprev;
-  classify(L, cut, g.last, g.depth - 1);
-  classify(L, g.first, cut, g.depth - 1);
-}
-
 // ---------------------------------------------------------------- per-thread finishers
 __global__ void k_leaf_sort(const Seg* __restrict__ segs, const uint32_t* __restrict__ ns, uint32_t* val, uint32_t* idx) {
   uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
@@ -293,14 +374,43 @@ __global__ void k_heap_sort(const Seg* __restrict__ segs, const uint32_t* __rest
 // ---------------------------------------------------------------- workgroup phase
 // One workgroup runs the complete introsort recursion of one segment of
 // 17..2048 elements in LDS, all subsegments of one depth together.
+// Introsort of a segment of s equal values with depth budget d.  The median
+// of three is the middle element B = f + s/2 (no comparison holds), so the
+// front and B swap; every element stops both scanners, so the partition
+// swaps f+1+k with l-1-k for k < K = floor((s-1)/2) and cuts at f+1+K; the
+// halves recurse, and the final insertion sort moves no equal key.
+// eq_depth_ok: the recursion reaches no heap sort (a segment of more than
+// kThreshold elements at depth 0); the larger half has ceil(s/2) elements.
+__device__ __forceinline__ bool eq_depth_ok(uint32_t s, int d) {
+  while (s > (uint32_t)kThreshold) {
+    if (d <= 0) return false;
+    s = (s + 1) / 2;
+    --d;
+  }
+  return true;
+}
+// Final place (0..s-1) of the element at place x of such a segment.
+__device__ __forceinline__ uint32_t eq_dest(uint32_t x, uint32_t s) {
+  uint32_t f = 0, l = s;
+  while (l - f > (uint32_t)kThreshold) {
+    const uint32_t sz = l - f, mid = f + sz / 2, K = (sz - 1) / 2;
+    if (x == f) x = mid;
+    else if (x == mid) x = f;
+    if (x > f) {
+      if (x - (f + 1) < K) x = l - 1 - (x - (f + 1));
+      else if (l - 1 - x < K) x = f + 1 + (l - 1 - x);
+    }
+    const uint32_t cut = f + 1 + K;
+    if (x < cut) l = cut; else f = cut;
+  }
+  return x;
+}
+
 // Element k of thread (wave w, lane j) is w * 64 * kPer + 64 * k + j: every
 // LDS access of a per-element loop hits 64 consecutive words, and a wave's
 // stopper prefix comes from ballots instead of a workgroup scan.
 __device__ __forceinline__ uint32_t blk_elem(int tid, int k) {
   return (uint32_t)((tid >> 6) * (64 * kPer) + 64 * k + (tid & 63));
-}
-__device__ __forceinline__ uint32_t lanes_below(uint64_t m) {
-  return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
 }
 
 __global__ void __launch_bounds__(kBT) k_block_sort(const Seg* __restrict__ segs, const uint32_t* __restrict__ nsp,
@@ -315,19 +425,34 @@ __global__ void __launch_bounds__(kBT) k_block_sort(const Seg* __restrict__ segs
   __shared__ uint32_t aPivot[128];
   __shared__ uint16_t aK[128], aCut[128], aLeft[128], aRight[128];
   __shared__ uint16_t lfFirst[kLdsN / 2], lfLast[kLdsN / 2];   // leaves of 2..16 elements
+  __shared__ uint16_t eqFirst[128], eqLast[128];               // equal-value segments
   __shared__ uint32_t wTot[2][kWaves];
   __shared__ uint32_t nActive[2];
-  __shared__ uint32_t nLeaf;
-  // segOf: < kLeafTag active segment slot, kLeafTag | q leaf q, kDone finished
+  __shared__ uint32_t nLeaf, nEq;
+  // segOf: < kLeafTag active segment slot, kLeafTag | q leaf q, kEqTag | q
+  // equal-value segment q, kDone finished
 
   if (blockIdx.x >= *nsp) return;
   const Seg g = segs[blockIdx.x];
   const uint32_t n = g.last - g.first;
   const int tid = threadIdx.x, wv = tid >> 6;
-  for (uint32_t k = tid; k < n; k += kBT) { sv[k] = val[g.first + k]; si[k] = idx[g.first + k]; segOf[k] = 0; }
+  const uint32_t v0 = val[g.first];
+  bool differs = false;
+  for (uint32_t k = tid; k < n; k += kBT) {
+    sv[k] = val[g.first + k];
+    si[k] = idx[g.first + k];
+    segOf[k] = 0;
+    differs |= sv[k] != v0;
+  }
+  if (!__syncthreads_or(differs) && eq_depth_ok(n, g.depth)) {
+    // all values equal: only the indices move, each straight to its place
+    for (uint32_t k = tid; k < n; k += kBT) idx[g.first + eq_dest(k, n)] = si[k];
+    return;
+  }
   if (tid == 0) {
     nActive[0] = 0;
     nLeaf = 0;
+    nEq = 0;
     if (g.depth == 0) {
       LArr a{sv, si};
       heap_sort(a, n);
@@ -426,8 +551,11 @@ __global__ void __launch_bounds__(kBT) k_block_sort(const Seg* __restrict__ segs
     // 5. cut, children: <=16 -> insertion sort now, depth 0 -> heap sort now, else next round
     if (tid < (int)na) {
       const uint32_t f = aFirst[cur][tid], l = aLast[cur][tid];
-      const uint32_t nL = exL[l] - exL[f];
+      const uint32_t nL = exL[l] - exL[f], nR = exR[l] - exR[f];
       const uint32_t K = aK[tid];
+      // no value above the pivot: the left half is all pivots; none below:
+      // the right half is
+      const bool eqc[2] = {nL == l - f - 1, nR == l - f};
       const uint32_t rprev = K ? Rp[f + K - 1] : l;
       const uint32_t cut = (K < nL && Lp[f + K] < rprev) ? Lp[f + K] : rprev;
       const int d = aDepth[cur][tid] - 1;
@@ -445,6 +573,11 @@ __global__ void __launch_bounds__(kBT) k_block_sort(const Seg* __restrict__ segs
             lfLast[q] = (uint16_t)cl2[h];
             ids[h] = (uint16_t)(kLeafTag | q);
           }
+        } else if (eqc[h] && eq_depth_ok(sz, d)) {   // placed after the rounds
+          const uint32_t q = atomicAdd(&nEq, 1u);
+          eqFirst[q] = (uint16_t)cf[h];
+          eqLast[q] = (uint16_t)cl2[h];
+          ids[h] = (uint16_t)(kEqTag | q);
         } else if (d == 0) heap_sort(a, sz);
         else {
           const uint32_t slotn = atomicAdd(&nActive[nxt], 1u);
@@ -479,16 +612,22 @@ __global__ void __launch_bounds__(kBT) k_block_sort(const Seg* __restrict__ segs
     if (e >= n) continue;
     const uint32_t s = segOf[e];
     if (s < kLeafTag || s == kDone) continue;
-    const uint32_t f = lfFirst[s & ~kLeafTag], l = lfLast[s & ~kLeafTag];
     const uint32_t v = sv[e];
-    uint32_t r = 0;
-    for (uint32_t j = f; j < l; ++j) {
-      const uint32_t u = sv[j];
-      r += (comp(u, v) || (u == v && j < e)) ? 1u : 0u;
+    uint32_t r;
+    if ((s & kTagMask) == kEqTag) {
+      const uint32_t f = eqFirst[s & ~kTagMask], l = eqLast[s & ~kTagMask];
+      r = f + eq_dest(e - f, l - f);
+    } else {
+      const uint32_t f = lfFirst[s & ~kTagMask], l = lfLast[s & ~kTagMask];
+      r = f;
+      for (uint32_t j = f; j < l; ++j) {
+        const uint32_t u = sv[j];
+        r += (comp(u, v) || (u == v && j < e)) ? 1u : 0u;
+      }
     }
     rv[k] = v;
     ri[k] = si[e];
-    rp[k] = (uint16_t)(f + r);
+    rp[k] = (uint16_t)r;
   }
   __syncthreads();
 #pragma unroll
@@ -571,32 +710,33 @@ void refsort_ranges(svd_ctx* c, uint32_t* idx, uint32_t* val, uint64_t n, const 
     uint32_t* blk0 = dalloc<uint32_t>(c, capBig + 1);
     // tiles of one round: sum of ceil(size / kTile) <= n / kTile + live segments
     const uint32_t G = (uint32_t)(n / kTile + capBig + 1);
-    uint32_t* cntL = dalloc<uint32_t>(c, G + 1);
-    uint32_t* cntR = dalloc<uint32_t>(c, G + 1);
-    const uint32_t capB = (uint32_t)capBig;
+    uint32_t* cntL = dalloc<uint32_t>(c, G);
+    uint32_t* cntR = dalloc<uint32_t>(c, G);
     Seg* cur = bigA;
     Seg* nxt = bigB;
     uint32_t* ncur = counters + 0;
     uint32_t* nnxt = counters + 4;
-    // Every round leaves a piece of at least half the largest segment, so the
-    // first ceil(log2(max / kLdsN)) rounds are all needed: run them without
-    // looking at the list size, then check it every second round.  A round
-    // over an empty list changes nothing.
+    // A round is plan (split the last round's segments, pick the pivots of
+    // the new ones), count, scatter and swap.  Every round leaves a piece of
+    // at least half the largest segment, so the first ceil(log2(max /
+    // kLdsN)) rounds are all needed: run them without looking at the list
+    // size, then check it every second round.  A round over an empty list
+    // changes nothing.
     uint64_t maxRoot = 0;
     for (const Seg& r : roots[0]) maxRoot = std::max<uint64_t>(maxRoot, r.last - r.first);
     int batch = 0;
     while (((uint64_t)kLdsN << batch) < maxRoot) ++batch;
+    L.big = cur;
+    L.nbig = ncur;
+    k_g_plan<<<1, kPlanT, 0, c->stream>>>(nullptr, nullptr, Lpos, Rpos, L, val, idx, st, blk0);
     while (true) {
       for (int r = 0; r < batch; ++r) {
-        L.big = nxt;
-        L.nbig = nnxt;
-        k_g_median<<<div_up(capB + 1, 256), 256, 0, c->stream>>>(cur, ncur, capB, val, idx, st, blk0, nnxt);
-        scan_exclusive(c, blk0, blk0, capB + 1, nullptr);
         k_g_count<<<G, kBT, 0, c->stream>>>(cur, ncur, blk0, st, val, cntL, cntR);
-        scan_exclusive2(c, cntL, cntR, G);
         k_g_scatter<<<G, kBT, 0, c->stream>>>(cur, ncur, blk0, st, val, cntL, cntR, Lpos, Rpos);
         k_g_swap<<<G, kBT, 0, c->stream>>>(cur, ncur, blk0, st, val, idx, Lpos, Rpos);
-        k_g_split<<<div_up(capB, 256), 256, 0, c->stream>>>(cur, ncur, st, Lpos, Rpos, L);
+        L.big = nxt;
+        L.nbig = nnxt;
+        k_g_plan<<<1, kPlanT, 0, c->stream>>>(cur, ncur, Lpos, Rpos, L, val, idx, st, blk0);
         check_launch("refsort round");
         std::swap(cur, nxt);
         std::swap(ncur, nnxt);

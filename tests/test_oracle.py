@@ -303,8 +303,46 @@ def test_trace_formats_agree_without_lod():
 
 
 # ---------------------------------------------------------------- refcount order
-def introsort_model(vals):
-    """The segment-parallel restatement run by refsort.hip, in plain Python."""
+def eq_depth_ok(s, d):
+    """refsort.hip: the introsort of s equal values at depth budget d reaches
+    no heap sort (the larger half of a segment has ceil(s/2) elements)."""
+    while s > 16:
+        if d <= 0:
+            return False
+        s, d = (s + 1) // 2, d - 1
+    return True
+
+
+def eq_dest(x, s):
+    """refsort.hip: final place of the element at place x among s equal
+    values: the median of three is the middle, every element stops both
+    scanners (pairs f+1+k <-> l-1-k swap for k < (s-1)//2), the cut is
+    f+1+(s-1)//2."""
+    f, l = 0, s
+    while l - f > 16:
+        sz = l - f
+        mid, K = f + sz // 2, (sz - 1) // 2
+        if x == f:
+            x = mid
+        elif x == mid:
+            x = f
+        if x > f:
+            if x - (f + 1) < K:
+                x = l - 1 - (x - (f + 1))
+            elif l - 1 - x < K:
+                x = f + 1 + (l - 1 - x)
+        cut = f + 1 + K
+        if x < cut:
+            l = cut
+        else:
+            f = cut
+    return x
+
+
+def introsort_model(vals, eq_shortcut=False):
+    """The segment-parallel restatement run by refsort.hip, in plain Python
+    (eq_shortcut: segments of equal values placed in closed form, as its
+    workgroup phase does)."""
     a = [(i, v) for i, v in enumerate(vals)]
     n = len(a)
     if n == 0:
@@ -344,6 +382,11 @@ def introsort_model(vals):
         for first, last, depth in active:
             if last - first <= 16:
                 leaves.append((first, last)); continue
+            if eq_shortcut and len({x[1] for x in a[first:last]}) == 1 and eq_depth_ok(last - first, depth):
+                seg = a[first:last]                  # the workgroup phase's closed form
+                for x in range(last - first):
+                    a[first + eq_dest(x, last - first)] = seg[x]
+                continue
             if depth == 0:
                 heapsort(first, last - first); continue
             mid = first + (last - first) // 2
@@ -382,6 +425,19 @@ def test_introsort_model_equals_std_sort(seed):
         vals = np.sort(vals)
     idx, _ = O.std_sort_refcount(np.arange(n, dtype=np.uint32), vals)
     assert introsort_model(vals.tolist()) == idx.tolist()
+    assert introsort_model(vals.tolist(), eq_shortcut=True) == idx.tolist()
+
+
+def test_equal_values_closed_form_equals_std_sort():
+    """The closed-form place of every element among s equal values is where
+    libstdc++ std::sort puts it, for every s up to 3000; and the shortcut
+    never applies where the recursion would reach a heap sort."""
+    for s in range(1, 3001):
+        idx, _ = O.std_sort_refcount(np.arange(s, dtype=np.uint32), np.full(s, 7, np.uint32))
+        want = np.arange(s) if s <= 16 else np.array([eq_dest(int(i), s) for i in idx])
+        np.testing.assert_array_equal(want, np.arange(s), err_msg=f"s={s}")
+        assert eq_depth_ok(s, 2 * (s.bit_length() - 1))
+    assert not eq_depth_ok(17, 0) and eq_depth_ok(16, 0) and not eq_depth_ok(33, 1)
 
 
 # ---------------------------------------------------------------- C ABI surface
