@@ -7,6 +7,7 @@
 // w1..w8 = children[0..7]; the finest level stores only the mask word.
 #include "svd_internal.hpp"
 #include "block_scan.hpp"
+#include "lookback.hpp"
 #include <algorithm>
 
 namespace svd {
@@ -20,49 +21,231 @@ __device__ __forceinline__ bool head64(const uint64_t* code, uint64_t i) {
   return i == 0 || (code[i] >> 3) != (code[i - 1] >> 3);
 }
 
+__device__ __forceinline__ uint32_t lanes_below(uint64_t m) {
+  return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+// A tile of sorted codes for the one-pass level builds: thread (wave w, lane
+// j) holds codes w * 64 * PER + 64 * k + j (k < PER) of the tile, so every
+// load is coalesced; a run of codes sharing code >> 3 is one parent node.
+// After run_tile: head bit k = code k starts a run, rank[k] = the run's
+// number within the tile counted from the tile's first head (-1: the code
+// continues the previous tile's last run), agg = the tile's runs.
+constexpr int kRunT = 256;
+template <int PER>
+struct RunTile {
+  uint64_t v[PER];
+  int32_t rank[PER];
+  uint32_t heads = 0;
+  uint32_t agg = 0;
+};
+template <int PER>
+__device__ __forceinline__ void run_tile(const uint64_t* __restrict__ code, uint64_t n, uint64_t tileBase,
+                                         RunTile<PER>& t, uint32_t* wtot) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint64_t wbase = tileBase + (uint64_t)wave * (64 * PER) + lane;
+#pragma unroll
+  for (int k = 0; k < PER; ++k) {
+    const uint64_t i = wbase + 64 * k;
+    t.v[k] = i < n ? code[i] : 0ull;
+  }
+  const uint64_t first = tileBase + (uint64_t)wave * (64 * PER);
+  const uint64_t before = (lane == 0 && first > 0 && first - 1 < n) ? code[first - 1] : 0ull;
+  uint32_t c = 0;
+#pragma unroll
+  for (int k = 0; k < PER; ++k) {
+    const uint64_t i = wbase + 64 * k;
+    const uint64_t up = __shfl_up(t.v[k], 1, 64);
+    const uint64_t wrap = k ? __shfl(t.v[k - 1], 63, 64) : before;
+    const uint64_t u = lane ? up : wrap;
+    const bool h = i < n && (i == 0 || (t.v[k] >> 3) != (u >> 3));
+    const uint64_t b = __ballot(h);
+    t.heads |= (h ? 1u : 0u) << k;
+    // inclusive head count up to this code, minus one
+    t.rank[k] = (int32_t)(c + lanes_below(b) + (h ? 1u : 0u)) - 1;
+    c += (uint32_t)__popcll(b);
+  }
+  if (lane == 0) wtot[wave] = c;
+  __syncthreads();
+  uint32_t off = 0, agg = 0;
+#pragma unroll
+  for (int w = 0; w < kRunT / 64; ++w) {
+    if (w < wave) off += wtot[w];
+    agg += wtot[w];
+  }
+#pragma unroll
+  for (int k = 0; k < PER; ++k) t.rank[k] += (int32_t)off;
+  t.agg = agg;
+}
+
+// Mask bits of the codes from position `end` on that continue the run of
+// parent code pc (at most 7), loaded together.
+__device__ __forceinline__ uint32_t run_tail_bits(const uint64_t* __restrict__ code, uint64_t n, uint64_t end,
+                                                  uint64_t pc) {
+  uint64_t y[7];
+#pragma unroll
+  for (int j = 0; j < 7; ++j) y[j] = end + j < n ? code[end + j] : ~0ull;
+  uint32_t m = 0;
+  bool same = true;
+#pragma unroll
+  for (int j = 0; j < 7; ++j) {
+    same = same && (y[j] >> 3) == pc;
+    if (same) m |= 1u << (uint32_t)(y[j] & 7u);
+  }
+  return m;
+}
+
 // Input contract: strictly ascending codes below 2^(3*levels).
-// The input check and every level's node count in one pass.  Code i starts
-// a node of the level k steps above the voxels (prefix code >> 3k) exactly
-// when k <= K_i = floor(msb(code[i] ^ code[i-1]) / 3) (K_0 = L), so the
-// level k steps up has #{i : K_i >= k} nodes.  hist[0] = bad codes,
-// hist[k] = #{i : K_i >= k} (1 <= k <= L).  Counts stay in registers over a
-// grid-stride loop and are reduced per workgroup: one global atomic per
-// workgroup and counter.
-constexpr int kCountBlocks = 1024;
-__global__ void __launch_bounds__(256) k_check_and_count(const uint64_t* __restrict__ code, uint64_t n, uint32_t L,
-                                                         uint32_t* __restrict__ hist) {
+// One pass over the codes for the finest level: the input check, every
+// level's node count, the run heads, their numbering by a single-pass scan,
+// and the leaf masks and parent codes.  Code i starts a node of the level k
+// steps above the voxels (prefix code >> 3k) exactly when k <= K_i =
+// floor(msb(code[i] ^ code[i-1]) / 3) (K_0 = L), so the level k steps up has
+// #{i : K_i >= k} nodes; part[tile * (L + 1) + k] = that count over the tile
+// (k >= 1), part[tile * (L + 1)] = the tile's bad codes.  Counts go per tile
+// rather than to device atomics, whose same-address updates from thousands
+// of workgroups would queue.  Each code ORs its bit into its run's LDS mask;
+// a run that crosses the tile's end reads the codes past it.  With bad input
+// the writes stay below n and the caller throws on the bad count.
+constexpr int kLeafPer = 32;
+constexpr uint32_t kLeafTile = kRunT * kLeafPer;
+__global__ void __launch_bounds__(kRunT) k_leaf_build(const uint64_t* __restrict__ code, uint64_t n, uint32_t L,
+                                                      LbPass p, uint32_t* __restrict__ part,
+                                                      uint32_t* __restrict__ mask, uint64_t* __restrict__ pcode) {
+  __shared__ uint32_t lds[kRunT / 64 + 2];
+  __shared__ uint32_t word;
+  __shared__ uint32_t cpart[kMaxLevels + 1][kRunT / 64];
+  __shared__ uint32_t lmask[kLeafTile];
+  const uint32_t tile = lb_take_tile(p, &word);
+  const uint64_t tileBase = (uint64_t)tile * kLeafTile;
+  for (uint32_t j = threadIdx.x; j < kLeafTile; j += kRunT) lmask[j] = 0;
+  RunTile<kLeafPer> t;
+  run_tile(code, n, tileBase, t, lds);   // its barrier also orders the clearing
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint64_t wbase = tileBase + (uint64_t)wave * (64 * kLeafPer) + lane;
   uint32_t cnt[kMaxLevels + 1];
 #pragma unroll
-  for (int k = 0; k <= (int)kMaxLevels; ++k) cnt[k] = 0;
-  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-    const uint64_t v = code[i];
+  for (int q = 0; q <= (int)kMaxLevels; ++q) cnt[q] = 0;
+#pragma unroll
+  for (int k = 0; k < kLeafPer; ++k) {
+    const uint64_t i = wbase + 64 * k;
+    const uint64_t x = t.v[k];
+    const uint64_t up = __shfl_up(x, 1, 64);
+    const uint64_t wrap = k ? __shfl(t.v[k - 1], 63, 64) : (lane == 0 && i > 0 && i - 1 < n ? code[i - 1] : 0ull);
+    const uint64_t u = lane ? up : wrap;
+    if (i >= n) continue;
     uint32_t K = L;
-    bool bad = (v >> (3 * L)) != 0;   // L <= 21: the shift is at most 63
+    bool bad = (x >> (3 * L)) != 0;   // L <= 21: the shift is at most 63
     if (i > 0) {
-      const uint64_t u = code[i - 1];
-      if (v <= u) bad = true;
-      else K = min(L, (uint32_t)(63 - __clzll((long long)(v ^ u))) / 3u);
+      if (x <= u) bad = true;
+      else K = min(L, (uint32_t)(63 - __clzll((long long)(x ^ u))) / 3u);
     }
     cnt[0] += bad ? 1u : 0u;
     if (!bad) {
 #pragma unroll
-      for (int k = 1; k <= (int)kMaxLevels; ++k) cnt[k] += K >= (uint32_t)k ? 1u : 0u;
+      for (int q = 1; q <= (int)kMaxLevels; ++q) cnt[q] += K >= (uint32_t)q ? 1u : 0u;
     }
+    if (t.rank[k] >= 0) atomicOr(&lmask[t.rank[k]], 1u << (uint32_t)(x & 7u));
   }
-  __shared__ uint32_t part[kMaxLevels + 1][4];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
-  for (int k = 0; k <= (int)kMaxLevels; ++k) {
-    uint32_t x = cnt[k];
+  for (int q = 0; q <= (int)kMaxLevels; ++q) {
+    uint32_t x = cnt[q];
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off, 64);
-    if (lane == 0) part[k][wave] = x;
+    if (lane == 0) cpart[q][wave] = x;
   }
   __syncthreads();
-  if (threadIdx.x <= kMaxLevels) {
-    const uint32_t x = part[threadIdx.x][0] + part[threadIdx.x][1] + part[threadIdx.x][2] + part[threadIdx.x][3];
-    if (x) atomicAdd(&hist[threadIdx.x], x);
+  if (threadIdx.x <= L)
+    part[(uint64_t)tile * (L + 1) + threadIdx.x] =
+        cpart[threadIdx.x][0] + cpart[threadIdx.x][1] + cpart[threadIdx.x][2] + cpart[threadIdx.x][3];
+  const uint32_t prefix = lb_exclusive(p, tile, t.agg, lds);
+  const uint64_t tileEnd = tileBase + kLeafTile;
+#pragma unroll
+  for (int k = 0; k < kLeafPer; ++k) {
+    if (!((t.heads >> k) & 1u)) continue;
+    const uint32_t r = (uint32_t)t.rank[k];
+    uint32_t m = lmask[r];
+    const uint64_t pc = t.v[k] >> 3;
+    if (r + 1 == t.agg && tileEnd < n) m |= run_tail_bits(code, n, tileEnd, pc);
+    mask[prefix + r] = m;
+    pcode[prefix + r] = pc;
+  }
+}
+
+// An inner level in one pass: run heads of the child codes, their numbering
+// by a single-pass scan, the records (child pointer = the child's position
+// in its level) and the parent codes.  Every code stores its own child
+// pointer into its parent's record and ORs its slot into the run's LDS mask;
+// the head writes the mask and kNull into the other slots.
+constexpr int kInnerPer = 16;
+constexpr uint32_t kInnerTile = kRunT * kInnerPer;
+__global__ void __launch_bounds__(kRunT) k_inner_build(const uint64_t* __restrict__ ccode, uint64_t n, LbPass p,
+                                                       uint32_t* __restrict__ rec, uint64_t* __restrict__ pcode) {
+  __shared__ uint32_t lds[kRunT / 64 + 2];
+  __shared__ uint32_t word;
+  __shared__ uint32_t lmask[kInnerTile];
+  const uint32_t tile = lb_take_tile(p, &word);
+  const uint64_t tileBase = (uint64_t)tile * kInnerTile;
+  for (uint32_t j = threadIdx.x; j < kInnerTile; j += kRunT) lmask[j] = 0;
+  RunTile<kInnerPer> t;
+  run_tile(ccode, n, tileBase, t, lds);
+  const uint32_t prefix = lb_exclusive(p, tile, t.agg, lds);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint64_t wbase = tileBase + (uint64_t)wave * (64 * kInnerPer) + lane;
+#pragma unroll
+  for (int k = 0; k < kInnerPer; ++k) {
+    const uint64_t i = wbase + 64 * k;
+    if (i >= n) continue;
+    const uint32_t sl = (uint32_t)(t.v[k] & 7u);
+    // rank -1: the run began in the previous tile, whose last node is prefix - 1
+    rec[(uint64_t)(prefix + t.rank[k]) * kRecWords + 1 + sl] = (uint32_t)i;
+    if (t.rank[k] >= 0) atomicOr(&lmask[t.rank[k]], 1u << sl);
+  }
+  __syncthreads();
+  const uint64_t tileEnd = tileBase + kInnerTile;
+#pragma unroll
+  for (int k = 0; k < kInnerPer; ++k) {
+    if (!((t.heads >> k) & 1u)) continue;
+    const uint32_t r = (uint32_t)t.rank[k];
+    uint32_t m = lmask[r];
+    const uint64_t pc = t.v[k] >> 3;
+    if (r + 1 == t.agg && tileEnd < n) m |= run_tail_bits(ccode, n, tileEnd, pc);
+    uint32_t* o = rec + (uint64_t)(prefix + r) * kRecWords;
+    o[0] = m;
+#pragma unroll
+    for (int c = 0; c < 8; ++c)
+      if (!((m >> c) & 1u)) o[1 + c] = kNull;
+    if (pcode) pcode[prefix + r] = pc;
+  }
+}
+
+// Level counts of k_leaf_build: hist[k] = sum over tiles of part[t][k], all
+// counters in one pass over the tiles.
+__global__ void __launch_bounds__(1024) k_level_counts(const uint32_t* __restrict__ part, uint32_t tiles, uint32_t L,
+                                                       uint32_t* __restrict__ hist) {
+  __shared__ uint32_t red[kMaxLevels + 1][1024 / 64];
+  uint32_t s[kMaxLevels + 1];
+#pragma unroll
+  for (int q = 0; q <= (int)kMaxLevels; ++q) s[q] = 0;
+  for (uint32_t t = threadIdx.x; t < tiles; t += 1024) {
+    const uint32_t* pt = part + (uint64_t)t * (L + 1);
+#pragma unroll
+    for (int q = 0; q <= (int)kMaxLevels; ++q)
+      if ((uint32_t)q <= L) s[q] += pt[q];
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int q = 0; q <= (int)kMaxLevels; ++q) {
+    uint32_t x = s[q];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off, 64);
+    if (lane == 0) red[q][wave] = x;
+  }
+  __syncthreads();
+  if (threadIdx.x <= L) {
+    uint32_t x = 0;
+    for (int v = 0; v < 1024 / 64; ++v) x += red[threadIdx.x][v];
+    hist[threadIdx.x] = x;
   }
 }
 
@@ -167,39 +350,123 @@ __device__ __forceinline__ bool rec_eq(const uint32_t* __restrict__ rec, uint32_
 using TEntry = unsigned long long;
 constexpr TEntry kEmpty = ~0ull;
 
-// Insert node i; the table slot ends holding the smallest index of all nodes
-// with an equal key (atomicMin: equal keys share the fingerprint).  Empty
-// nodes (mask 0) are skipped, like the reference's `if (!n.hasChildren()) continue;`.
-__global__ void k_insert(const uint32_t* __restrict__ rec, uint64_t n, TEntry* __restrict__ table, uint32_t cmask,
-                         uint32_t* __restrict__ slot) {
-  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  uint32_t w[kRecWords];
-  load_rec(rec, i, w);
-  if ((w[0] & 0xFFu) == 0) { slot[i] = kNone; return; }
-  const uint64_t hv = hash_rec(w);
+// Probe the table for node i's key (record w, hash hv) and insert it; the
+// slot ends holding the smallest index of all nodes with an equal key (equal
+// keys share the fingerprint).  eq(j) compares record j with w.  Returns the
+// slot.
+template <typename Eq>
+__device__ __forceinline__ uint32_t insert_key(TEntry* table, uint32_t cmask, uint64_t hv, uint32_t i, Eq eq) {
   const uint32_t fp = (uint32_t)(hv >> 32);
-  const TEntry mine = ((TEntry)fp << 32) | (TEntry)(uint32_t)i;
+  const TEntry mine = ((TEntry)fp << 32) | (TEntry)i;
   uint32_t h = (uint32_t)hv & cmask;
   while (true) {
     TEntry e = __atomic_load_n(&table[h], __ATOMIC_RELAXED);
     if (e == kEmpty) {
       const TEntry old = atomicCAS(&table[h], kEmpty, mine);
-      if (old == kEmpty) { slot[i] = h; return; }
+      if (old == kEmpty) return h;
       e = old;
     }
-    if ((uint32_t)(e >> 32) == fp && rec_eq(rec, (uint32_t)e, w)) {
+    if ((uint32_t)(e >> 32) == fp && eq((uint32_t)e)) {
       // the slot only decreases, so a reader above the value it saw has
-      // nothing to add: hot keys cost one atomic per decrease, not per node
+      // nothing to add: a key costs one atomic per decrease, not per node
       if (mine < e) atomicMin(&table[h], mine);
-      slot[i] = h;
-      return;
+      return h;
     }
     h = (h + 1) & cmask;
   }
 }
 
+// Insert the nodes of a level; slot[i] = the table slot of node i's key, whose
+// entry ends holding the smallest index of all nodes with that key.  Empty
+// nodes (mask 0) are skipped, like the reference's `if (!n.hasChildren())
+// continue;`.  A common key (a full brick, say) occurs in most workgroups many
+// times, and atomics on one address serialize, so each workgroup first
+// dedups its nodes in an LDS table (keys compared with the records staged in
+// LDS) and only the smallest index of each of its keys goes to the device
+// table; the others take its slot.  PER nodes per thread: 4 on the big levels
+// (fewer workgroups per common key), 1 on the others, where the chain of
+// memory round trips per node sets the time.
+constexpr int kInsT = 256;
+constexpr uint64_t kInsWide = 1u << 19;   // levels from this size use 4 nodes per thread
+template <int PER>
+__global__ void __launch_bounds__(kInsT) k_insert(const uint32_t* __restrict__ rec, uint64_t n,
+                                                  TEntry* __restrict__ table, uint32_t cmask,
+                                                  uint32_t* __restrict__ slot) {
+  constexpr uint32_t kNodes = kInsT * PER, kCap = 2 * kNodes;   // LDS table load <= 1/2
+  __shared__ TEntry lt[kCap];
+  __shared__ uint32_t gs[kCap];
+  __shared__ uint16_t ls[kNodes];
+  __shared__ uint32_t lrec[kNodes * kRecWords];
+  const uint32_t tid = threadIdx.x;
+  const uint64_t base = (uint64_t)blockIdx.x * kNodes;
+  for (uint32_t j = tid; j < kCap; j += kInsT) lt[j] = kEmpty;
+  uint32_t w[PER][kRecWords];
+  bool live[PER];
+#pragma unroll
+  for (int k = 0; k < PER; ++k) {
+    const uint32_t loc = k * kInsT + tid;
+    const uint64_t i = base + loc;
+    live[k] = false;
+    ls[loc] = 0xFFFFu;
+    if (i >= n) continue;
+    load_rec(rec, i, w[k]);
+#pragma unroll
+    for (int q = 0; q < kRecWords; ++q) lrec[loc * kRecWords + q] = w[k][q];
+    if ((w[k][0] & 0xFFu) == 0) slot[i] = kNone;
+    else live[k] = true;
+  }
+  __syncthreads();
+  uint64_t hv[PER];
+#pragma unroll
+  for (int k = 0; k < PER; ++k) {
+    if (!live[k]) continue;
+    const uint32_t loc = k * kInsT + tid;
+    hv[k] = hash_rec(w[k]);
+    const uint32_t* wk = w[k];
+    auto eqLds = [&](uint32_t j) {
+      const uint32_t* r = lrec + (j - (uint32_t)base) * kRecWords;
+      uint32_t diff = 0;
+#pragma unroll
+      for (int q = 0; q < kRecWords; ++q) diff |= r[q] ^ wk[q];
+      return diff == 0;
+    };
+    ls[loc] = (uint16_t)insert_key(lt, kCap - 1, hv[k], (uint32_t)(base + loc), eqLds);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < PER; ++k) {
+    const uint32_t loc = k * kInsT + tid;
+    const uint32_t l = ls[loc];
+    if (!live[k] || (uint32_t)lt[l] != (uint32_t)(base + loc)) continue;
+    const uint32_t* wk = w[k];
+    auto eqDev = [&](uint32_t j) { return rec_eq(rec, j, wk); };
+    gs[l] = insert_key(table, cmask, hv[k], (uint32_t)(base + loc), eqDev);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < PER; ++k) {
+    const uint32_t loc = k * kInsT + tid;
+    if (live[k]) slot[base + loc] = gs[ls[loc]];
+  }
+}
+
+void launch_insert(svd_ctx* c, const uint32_t* rec, uint64_t n, TEntry* table, uint32_t cmask, uint32_t* slot) {
+  if (n >= kInsWide) k_insert<4><<<div_up(n, 4 * kInsT), kInsT, 0, c->stream>>>(rec, n, table, cmask, slot);
+  else k_insert<1><<<div_up(n, kInsT), kInsT, 0, c->stream>>>(rec, n, table, cmask, slot);
+}
+
 __device__ __forceinline__ uint32_t entry_index(TEntry e) { return (uint32_t)e; }
+
+// lookup from slot h on (fp = the key hash's upper half).
+__device__ __forceinline__ uint32_t lookup_from(const uint32_t* __restrict__ rec, const TEntry* __restrict__ table,
+                                                uint32_t cmask, const uint32_t w[kRecWords], uint32_t fp, uint32_t h) {
+  while (true) {
+    const TEntry e = table[h];
+    if (e == kEmpty) return kNone;
+    if ((uint32_t)(e >> 32) == fp && rec_eq(rec, (uint32_t)e, w)) return (uint32_t)e;
+    h = (h + 1) & cmask;
+  }
+}
 
 // Probe for a key without inserting; returns the stored (minimum) index.
 __device__ __forceinline__ uint32_t lookup(const uint32_t* __restrict__ rec, const TEntry* __restrict__ table,
@@ -527,14 +794,37 @@ __global__ void k_sdag_cands(const uint32_t* __restrict__ key, uint64_t n, const
   uint32_t k0[kRecWords], v[kRecWords];
   load_rec(key, i, k0);
   const ChildInv ci = child_inv(k0, childInv);
-  bool any = false;
+  // The 7 lookups run side by side: all first probes are loaded together,
+  // then the records of the fingerprint matches; only a probe that meets
+  // another key walks on, one slot at a time.
+  uint32_t fp[7], hs[7];
+  bool self[7];
+#pragma unroll
   for (int t = 1; t < 8; ++t) {
     make_variant(k0, t, ci, v);
     // a variant equal to the node's own key finds the node itself
-    bool self = true;
+    bool same = true;
 #pragma unroll
-    for (int k = 0; k < kRecWords; ++k) self = self && v[k] == k0[k];
-    const uint32_t r = self ? (uint32_t)i : lookup(key, table, cmask, v);
+    for (int k = 0; k < kRecWords; ++k) same = same && v[k] == k0[k];
+    self[t - 1] = same;
+    const uint64_t hv = hash_rec(v);
+    fp[t - 1] = (uint32_t)(hv >> 32);
+    hs[t - 1] = (uint32_t)hv & cmask;
+  }
+  TEntry e[7];
+#pragma unroll
+  for (int t = 0; t < 7; ++t) e[t] = self[t] ? kEmpty : table[hs[t]];
+  bool any = false;
+#pragma unroll
+  for (int t = 1; t < 8; ++t) {
+    const TEntry et = e[t - 1];
+    uint32_t r = kNone;
+    if (self[t - 1]) r = (uint32_t)i;
+    else if (et != kEmpty) {
+      make_variant(k0, t, ci, v);
+      if ((uint32_t)(et >> 32) == fp[t - 1] && rec_eq(key, (uint32_t)et, v)) r = (uint32_t)et;
+      else r = lookup_from(key, table, cmask, v, fp[t - 1], (hs[t - 1] + 1) & cmask);
+    }
     if (r != kNone && r < (uint32_t)i) { cd[t - 1] = r; any = true; }
     else cd[t - 1] = kNone;
   }
@@ -875,21 +1165,81 @@ void build_svo_from_morton(svd_octree* o, const uint64_t* dcodes, uint64_t n) {
     SVD_HIP(hipStreamSynchronize(c->stream));
     return;
   }
+  // finest level, input check and level sizes in one pass over the codes
+  const uint64_t tiles = div_up(n, kLeafTile);
+  uint32_t* part = dalloc<uint32_t>(c, tiles * (L + 1));
   uint32_t* hist = dalloc<uint32_t>(c, kMaxLevels + 1);
-  fill_u32(c, hist, 0u, kMaxLevels + 1);
-  const uint32_t nb = std::min<uint64_t>(div_up(n, kB), (uint64_t)kCountBlocks);
-  k_check_and_count<<<nb, kB, 0, c->stream>>>(dcodes, n, L, hist);
-  check_launch("k_check_and_count");
+  uint32_t* leafMask = dalloc<uint32_t>(c, n);
+  uint64_t* codeA = dalloc<uint64_t>(c, n);
+  k_leaf_build<<<(uint32_t)tiles, kRunT, 0, c->stream>>>(dcodes, n, L, lb_pass(c, tiles), part, leafMask, codeA);
+  k_level_counts<<<1, 1024, 0, c->stream>>>(part, (uint32_t)tiles, L, hist);
+  check_launch("k_leaf_build");
   uint32_t h[kMaxLevels + 1];
   read_u32s(c, hist, h, L + 1);
   dfree(c, hist);
-  if (h[0]) throw Error{SVD_ERR_ARG, "Morton codes must be strictly ascending and below 2^(3*levels)"};
+  dfree(c, part);
+  if (h[0]) {
+    dfree(c, leafMask);
+    dfree(c, codeA);
+    throw Error{SVD_ERR_ARG, "Morton codes must be strictly ascending and below 2^(3*levels)"};
+  }
   // level lev is L - lev steps above the voxels
   std::vector<uint64_t> sizes(L);
   for (uint32_t lev = 0; lev < L; ++lev) sizes[lev] = h[L - lev];
-  const uint64_t nNodes = svo_levels_from_codes(c, dcodes, n, L, o->svo, nullptr, sizes.data());
+  Level& leaf = o->svo[L - 1];
+  leaf.leaf = true;
+  leaf.n = sizes[L - 1];
+  leaf.rec = leafMask;
+  uint64_t nNodes = (L >= 2) ? leaf.n : 0;
+  nNodes += svo_inner_levels(c, codeA, leaf.n, L, o->svo, nullptr, sizes.data());
   o->nNodesSVO = nNodes;   // non-root nodes created (GeomOctree::_nNodes after buildSVO)
   o->state = SVD_S_SVO;
+}
+
+// Inner levels L-2 .. 0 from the finest level's parent codes (codeA, nc of
+// them, taken over and freed).  Returns the non-root nodes created.
+uint64_t svo_inner_levels(svd_ctx* c, uint64_t* codeA, uint64_t nc, uint32_t L, std::vector<Level>& svo,
+                          std::vector<uint64_t*>* levelCodes, const uint64_t* sizes) {
+  uint32_t* scanbuf = dalloc<uint32_t>(c, nc + 1);
+  uint32_t* total = dalloc<uint32_t>(c, 1);
+  uint64_t* codeB = dalloc<uint64_t>(c, nc ? nc : 1);
+  auto keep = [&](uint32_t lev, const uint64_t* codes, uint64_t m) {
+    if (!levelCodes) return;
+    (*levelCodes)[lev] = dalloc<uint64_t>(c, m + 1);
+    if (m) SVD_HIP(hipMemcpyAsync((*levelCodes)[lev], codes, m * 8, hipMemcpyDeviceToDevice, c->stream));
+  };
+  uint64_t nNodes = 0;
+  for (int lev = (int)L - 2; lev >= 0; --lev) {
+    Level& lv = svo[lev];
+    if (sizes && nc <= kSvoSmall) {
+      const uint64_t np = sizes[lev];
+      lv.n = np;
+      lv.rec = dalloc<uint32_t>(c, np * kRecWords);
+      k_inner_level_small<<<1, 1024, 0, c->stream>>>(codeA, (uint32_t)nc, lv.rec, codeB);
+      check_launch("k_inner_level_small");
+    } else if (sizes) {
+      const uint64_t np = sizes[lev];
+      lv.n = np;
+      lv.rec = dalloc<uint32_t>(c, np * kRecWords);
+      const uint64_t tiles = div_up(nc, kInnerTile);
+      k_inner_build<<<(uint32_t)tiles, kRunT, 0, c->stream>>>(codeA, nc, lb_pass(c, tiles), lv.rec, codeB);
+      check_launch("k_inner_build");
+    } else {
+      k_heads<<<div_up(nc, kB), kB, 0, c->stream>>>(codeA, nc, scanbuf);
+      scan_exclusive(c, scanbuf, scanbuf, nc, total);
+      const uint64_t np = read_u32(c, total);
+      lv.n = np;
+      lv.rec = dalloc<uint32_t>(c, np * kRecWords);
+      k_inner_level<<<div_up(nc, kB), kB, 0, c->stream>>>(codeA, nc, scanbuf, lv.rec, codeB);
+      check_launch("k_inner_level");
+    }
+    keep((uint32_t)lev, codeB, lv.n);
+    std::swap(codeA, codeB);
+    nc = lv.n;
+    if (lev > 0) nNodes += lv.n;
+  }
+  dfree(c, scanbuf); dfree(c, total); dfree(c, codeA); dfree(c, codeB);
+  return nNodes;
 }
 
 // SVO levels from strictly ascending codes (the caller checked them).  Node
@@ -903,12 +1253,6 @@ uint64_t svo_levels_from_codes(svd_ctx* c, const uint64_t* dcodes, uint64_t n, u
   uint32_t* scanbuf = dalloc<uint32_t>(c, n + 1);
   uint32_t* total = dalloc<uint32_t>(c, 1);
   uint64_t* codeA = dalloc<uint64_t>(c, n);
-  uint64_t* codeB = dalloc<uint64_t>(c, n);
-  auto keep = [&](uint32_t lev, const uint64_t* codes, uint64_t m) {
-    if (!levelCodes) return;
-    (*levelCodes)[lev] = dalloc<uint64_t>(c, m + 1);
-    if (m) SVD_HIP(hipMemcpyAsync((*levelCodes)[lev], codes, m * 8, hipMemcpyDeviceToDevice, c->stream));
-  };
   // finest level
   k_heads<<<div_up(n, kB), kB, 0, c->stream>>>(dcodes, n, scanbuf);
   scan_exclusive(c, scanbuf, scanbuf, n, total);
@@ -919,37 +1263,14 @@ uint64_t svo_levels_from_codes(svd_ctx* c, const uint64_t* dcodes, uint64_t n, u
   leaf.rec = dalloc<uint32_t>(c, nl);
   k_leaf_level<<<div_up(n, kB), kB, 0, c->stream>>>(dcodes, n, scanbuf, leaf.rec, codeA);
   check_launch("k_leaf_level");
-  keep(L - 1, codeA, nl);
-  uint64_t nNodes = (L >= 2) ? nl : 0;
-  uint64_t nc = nl;
-  for (int lev = (int)L - 2; lev >= 0; --lev) {
-    Level& lv = svo[lev];
-    if (sizes && nc <= kSvoSmall) {
-      const uint64_t np = sizes[lev];
-      lv.n = np;
-      lv.rec = dalloc<uint32_t>(c, np * kRecWords);
-      k_inner_level_small<<<1, 1024, 0, c->stream>>>(codeA, (uint32_t)nc, lv.rec, codeB);
-      check_launch("k_inner_level_small");
-      keep((uint32_t)lev, codeB, np);
-      std::swap(codeA, codeB);
-      nc = np;
-      if (lev > 0) nNodes += np;
-      continue;
-    }
-    k_heads<<<div_up(nc, kB), kB, 0, c->stream>>>(codeA, nc, scanbuf);
-    scan_exclusive(c, scanbuf, scanbuf, nc, total);
-    uint64_t np = sizes ? sizes[lev] : read_u32(c, total);
-    lv.n = np;
-    lv.rec = dalloc<uint32_t>(c, np * kRecWords);
-    k_inner_level<<<div_up(nc, kB), kB, 0, c->stream>>>(codeA, nc, scanbuf, lv.rec, codeB);
-    check_launch("k_inner_level");
-    keep((uint32_t)lev, codeB, np);
-    std::swap(codeA, codeB);
-    nc = np;
-    if (lev > 0) nNodes += np;
+  dfree(c, scanbuf);
+  dfree(c, total);
+  if (levelCodes) {
+    (*levelCodes)[L - 1] = dalloc<uint64_t>(c, nl + 1);
+    if (nl) SVD_HIP(hipMemcpyAsync((*levelCodes)[L - 1], codeA, nl * 8, hipMemcpyDeviceToDevice, c->stream));
   }
-  dfree(c, scanbuf); dfree(c, total); dfree(c, codeA); dfree(c, codeB);
-  return nNodes;
+  const uint64_t nNodes = (L >= 2) ? nl : 0;
+  return nNodes + svo_inner_levels(c, codeA, nl, L, svo, levelCodes, sizes);
 }
 
 void remap_children(svd_ctx* c, const uint32_t* in, uint64_t n, const uint32_t* remap, uint32_t* out) {
@@ -967,7 +1288,7 @@ void dedup_level_dev(svd_ctx* c, const uint32_t* rec, uint64_t n, uint32_t* rema
   uint32_t* slot = dalloc<uint32_t>(c, n);
   uint32_t* pos = dalloc<uint32_t>(c, n + 1);
   fill_u32(c, (uint32_t*)table, kNone, 2ull * cap);
-  k_insert<<<div_up(n, kB), kB, 0, c->stream>>>(rec, n, table, cap - 1, slot);
+  launch_insert(c, rec, n, table, cap - 1, slot);
   k_rep_flags<<<div_up(n, kB), kB, 0, c->stream>>>(slot, table, n, pos);
   check_launch("k_insert");
   scan_exclusive(c, pos, pos, n, total);
@@ -1140,7 +1461,7 @@ bool reduce_sym_pass(svd_octree* o, bool speculative) {
     uint32_t* cand = dalloc<uint32_t>(c, n * 7);
     uint32_t* state = dalloc<uint32_t>(c, n);
     uint32_t* pos = dalloc<uint32_t>(c, n + 1);
-    k_insert<<<div_up(n, kB), kB, 0, c->stream>>>(key, n, table, cap - 1, slot);
+    launch_insert(c, key, n, table, cap - 1, slot);
     k_sdag_cands<<<div_up(n, kB), kB, 0, c->stream>>>(key, n, table, cap - 1, slot, childInv, cand, state);
     check_launch("k_sdag_cands");
     if (speculative) {

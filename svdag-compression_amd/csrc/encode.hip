@@ -103,60 +103,61 @@ __device__ __forceinline__ uint32_t span_level(const Span& a, uint64_t i) {
   return l;
 }
 
-// Reference counts of the children of a level.  A few children (common
-// bricks) take most references, so each block first aggregates in an LDS
-// hash table (1024 slots, at most 8 probes; a child that finds no slot goes
-// straight to the global histogram) and flushes one add per distinct child.
-constexpr int kRcSlots = 1024;
-constexpr int kRcBlocks = 2048;
+// Reference counts of the children of a level.  Most references go to a few
+// children (common bricks), whose counters share cache lines, and device
+// atomics on one line serialize; so each workgroup counts one range of
+// kRcRange children over one chunk of parents in LDS and adds each nonzero
+// counter to the histogram once: a child gets at most one device atomic per
+// parent chunk, and those of a workgroup are coalesced.
+constexpr uint32_t kRcRange = 32768;
+constexpr uint32_t kRcParents = 4096;   // parents per workgroup, about
+constexpr int kRcT = 1024;
 // Levels 1..L-2 of every set in one launch.  Parent level job j is span
 // level lev[j], whose children are the next span level (element start[l+1] +
 // child of the concatenated histogram); it owns workgroups blk0[j] ..
-// blk0[j+1]-1, one per 256 parents up to kRcBlocks.
+// blk0[j+1]-1: nr[j] child ranges times (blk0[j+1]-blk0[j]) / nr[j] parent
+// chunks.
 struct RcJobs {
   uint32_t lev[2 * kMaxLevels];
+  uint32_t nr[2 * kMaxLevels];
   uint32_t blk0[2 * kMaxLevels + 1];
   uint32_t n;
 };
-__global__ void __launch_bounds__(256) k_refcount(Span a, RcJobs jb, uint32_t* __restrict__ histAll,
-                                                  uint32_t* __restrict__ err) {
+__global__ void __launch_bounds__(kRcT) k_refcount(Span a, RcJobs jb, uint32_t* __restrict__ histAll,
+                                                   uint32_t* __restrict__ err) {
+  __shared__ uint32_t cnt[kRcRange];
   uint32_t j = 0;
   while (j + 1 < jb.n && blockIdx.x >= jb.blk0[j + 1]) ++j;
-  const uint32_t l = jb.lev[j];
-  const uint32_t nb = jb.blk0[j + 1] - jb.blk0[j], b = blockIdx.x - jb.blk0[j];
+  const uint32_t l = jb.lev[j], nr = jb.nr[j];
+  const uint32_t b = blockIdx.x - jb.blk0[j], nch = (jb.blk0[j + 1] - jb.blk0[j]) / nr;
+  const uint32_t range = b % nr, chunk = b / nr;
   const uint32_t* __restrict__ parent = a.rec[l];
   const uint64_t np = a.start[l + 1] - a.start[l];
   uint32_t* __restrict__ hist = histAll + a.start[l + 1];
-  const uint64_t nChild = a.start[l + 2] - a.start[l + 1];
-  __shared__ uint32_t key[kRcSlots], cnt[kRcSlots];
-  for (int t = threadIdx.x; t < kRcSlots; t += blockDim.x) { key[t] = kNone; cnt[t] = 0; }
+  const uint32_t nChild = a.start[l + 2] - a.start[l + 1];
+  const uint32_t r0 = range * kRcRange, rn = min(kRcRange, nChild - r0);
+  for (uint32_t t = threadIdx.x; t < rn; t += kRcT) cnt[t] = 0;
   __syncthreads();
-  const uint64_t chunk = (np + nb - 1) / nb;
-  const uint64_t lo = (uint64_t)b * chunk, hi = lo + chunk < np ? lo + chunk : np;
-  for (uint64_t i = lo + threadIdx.x; i < hi; i += blockDim.x) {
+  const uint64_t lo = np * chunk / nch, hi = np * (chunk + 1) / nch;
+  for (uint64_t i = lo + threadIdx.x; i < hi; i += kRcT) {
     const uint32_t* r = parent + i * kRecWords;
 #pragma unroll
     for (int c = 0; c < 8; ++c) {
       const uint32_t ch = r[1 + c];
       if (ch == kNull) continue;
-      if (ch >= nChild) { atomicAdd(err, 1u); continue; }
-      uint32_t h = (ch * 2654435761u) >> 22;                 // 10-bit slot
-      bool done = false;
-#pragma unroll 1
-      for (int p = 0; p < 8 && !done; ++p, h = (h + 1) & (kRcSlots - 1)) {
-        uint32_t k = key[h];
-        if (k == kNone) {
-          const uint32_t old = atomicCAS(&key[h], kNone, ch);
-          k = old == kNone ? ch : old;
-        }
-        if (k == ch) { atomicAdd(&cnt[h], 1u); done = true; }
+      if (ch >= nChild) {
+        if (range == 0) atomicAdd(err, 1u);
+        continue;
       }
-      if (!done) atomicAdd(&hist[ch], 1u);
+      const uint32_t o = ch - r0;
+      if (o < rn) atomicAdd(&cnt[o], 1u);
     }
   }
   __syncthreads();
-  for (int t = threadIdx.x; t < kRcSlots; t += blockDim.x)
-    if (key[t] != kNone && cnt[t]) atomicAdd(&hist[key[t]], cnt[t]);
+  for (uint32_t t = threadIdx.x; t < rn; t += kRcT) {
+    const uint32_t v = cnt[t];
+    if (v) atomicAdd(&hist[r0 + t], v);
+  }
 }
 
 // perm[i] = position of element i within its level (the identity order of
@@ -483,13 +484,16 @@ void encode_ssvdag_sets(svd_octree* o, const int* fmts, int K) {
   RcJobs jb{};
   for (int s = 0; s < K; ++s)
     for (int lev = 0; lev <= L - 3; ++lev) {
-      const uint64_t np = (*lvs[s])[lev].n;
+      const uint64_t np = (*lvs[s])[lev].n, nc = (*lvs[s])[lev + 1].n;
       if (!np) continue;
+      const uint32_t nr = (uint32_t)std::max<uint64_t>(1, div_up(nc, kRcRange));
+      const uint32_t nch = (uint32_t)std::max<uint64_t>(1, div_up(np, kRcParents));
       jb.lev[jb.n] = (uint32_t)(s * P + lev);
-      jb.blk0[jb.n + 1] = jb.blk0[jb.n] + (uint32_t)std::min<uint64_t>(div_up(np, kB), kRcBlocks);
+      jb.nr[jb.n] = nr;
+      jb.blk0[jb.n + 1] = jb.blk0[jb.n] + nr * nch;
       ++jb.n;
     }
-  if (jb.n) k_refcount<<<jb.blk0[jb.n], kB, 0, c->stream>>>(ua, jb, hist, err);
+  if (jb.n) k_refcount<<<jb.blk0[jb.n], kRcT, 0, c->stream>>>(ua, jb, hist, err);
   check_launch("k_refcount");
   refsort_ranges(c, permAll, hist, tot, rfirst.data(), rlast.data(), rfirst.size());
   dfree(c, hist);

@@ -1,7 +1,9 @@
-// Device-wide exclusive prefix sums (tile reduce / scan of the tile sums / down-sweep),
-// wave64 shuffles + LDS, 2048 items per 256-thread workgroup.
+// Device-wide exclusive prefix sums: one 1024-thread workgroup up to 8192
+// items, else one single-pass launch with decoupled look-back (4096 items per
+// 256-thread workgroup).
 #include "svd_internal.hpp"
 #include "block_scan.hpp"
+#include "lookback.hpp"
 
 namespace svd {
 
@@ -9,42 +11,35 @@ namespace {
 
 constexpr int kThreads = 256;
 constexpr int kItems = 8;
-constexpr int kTile = kThreads * kItems;
 
-__global__ void __launch_bounds__(kThreads) k_scan_reduce(const uint32_t* __restrict__ in, uint64_t n,
-                                                          uint32_t* __restrict__ partial) {
-  __shared__ uint32_t lds[kThreads / 64];
-  const uint64_t base = (uint64_t)blockIdx.x * kTile;
+// One tile of kLbItems items per thread; out may alias in (every item is
+// read before any is written).  The last tile writes the total.
+constexpr int kLbItems = 16;
+constexpr uint32_t kLbTile = kThreads * kLbItems;
+__global__ void __launch_bounds__(kThreads) k_scan_lb(const uint32_t* in, uint32_t* out, uint64_t n, LbPass p,
+                                                      uint32_t* __restrict__ total) {
+  __shared__ uint32_t lds[kThreads / 64 + 2];
+  __shared__ uint32_t word;
+  const uint32_t tile = lb_take_tile(p, &word);
+  const uint64_t base = (uint64_t)tile * kLbTile + (uint64_t)threadIdx.x * kLbItems;
+  uint32_t v[kLbItems];
   uint32_t s = 0;
 #pragma unroll
-  for (int k = 0; k < kItems; ++k) {
-    uint64_t i = base + (uint64_t)k * kThreads + threadIdx.x;
-    if (i < n) s += in[i];
-  }
-  s = block_reduce_sum(s, lds);
-  if (threadIdx.x == 0) partial[blockIdx.x] = s;
-}
-
-__global__ void __launch_bounds__(kThreads) k_scan_down(const uint32_t* __restrict__ in, uint32_t* __restrict__ out,
-                                                        uint64_t n, const uint32_t* __restrict__ partial) {
-  __shared__ uint32_t lds[kThreads / 64 + 1];
-  const uint64_t base = (uint64_t)blockIdx.x * kTile + (uint64_t)threadIdx.x * kItems;
-  uint32_t v[kItems];
-  uint32_t s = 0;
-#pragma unroll
-  for (int k = 0; k < kItems; ++k) {
-    uint64_t i = base + k;
+  for (int k = 0; k < kLbItems; ++k) {
+    const uint64_t i = base + k;
     v[k] = i < n ? in[i] : 0u;
     s += v[k];
   }
   uint32_t agg;
-  uint32_t ex = block_exclusive_sum(s, lds, agg) + partial[blockIdx.x];
+  uint32_t ex = block_exclusive_sum(s, lds, agg);
+  ex += lb_exclusive(p, tile, agg, lds);
 #pragma unroll
-  for (int k = 0; k < kItems; ++k) {
-    uint64_t i = base + k;
+  for (int k = 0; k < kLbItems; ++k) {
+    const uint64_t i = base + k;
     if (i < n) out[i] = ex;
     ex += v[k];
   }
+  if (total && tile + 1 == p.tiles && threadIdx.x == kThreads - 1) *total = ex;
 }
 
 // Small inputs: one 1024-thread workgroup walks the array in chunks of 8192
@@ -89,38 +84,6 @@ __global__ void __launch_bounds__(kSmallThreads) k_scan_small(ScanJobs jobs, uin
   if (threadIdx.x == 0 && total) *total = carry;
 }
 
-// Down-sweep for up to kFuseMax tiles that also replaces the partials pass:
-// each workgroup sums the partial sums of the tiles before it (they sit in
-// L2), and the last one writes the total.
-constexpr uint32_t kFuseMax = 1024;
-__global__ void __launch_bounds__(kThreads) k_scan_down_fused(const uint32_t* __restrict__ in, uint32_t* __restrict__ out,
-                                                              uint64_t n, const uint32_t* __restrict__ partial,
-                                                              uint32_t* __restrict__ total) {
-  __shared__ uint32_t red[kThreads / 64];
-  __shared__ uint32_t lds[kThreads / 64 + 1];
-  uint32_t p = 0;
-  for (uint32_t k = threadIdx.x; k < blockIdx.x; k += kThreads) p += partial[k];
-  const uint32_t prefix = block_reduce_sum(p, red);
-  const uint64_t base = (uint64_t)blockIdx.x * kTile + (uint64_t)threadIdx.x * kItems;
-  uint32_t v[kItems];
-  uint32_t s = 0;
-#pragma unroll
-  for (int k = 0; k < kItems; ++k) {
-    uint64_t i = base + k;
-    v[k] = i < n ? in[i] : 0u;
-    s += v[k];
-  }
-  uint32_t agg;
-  uint32_t ex = block_exclusive_sum(s, lds, agg) + prefix;
-#pragma unroll
-  for (int k = 0; k < kItems; ++k) {
-    uint64_t i = base + k;
-    if (i < n) out[i] = ex;
-    ex += v[k];
-  }
-  if (total && blockIdx.x + 1 == gridDim.x && threadIdx.x == 0) *total = prefix + agg;
-}
-
 __global__ void k_fill(uint32_t* d, uint32_t v, uint64_t n) {
   uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) d[i] = v;
@@ -148,19 +111,10 @@ void scan_exclusive(svd_ctx* c, const uint32_t* in, uint32_t* out, uint64_t n, u
     check_launch("scan_exclusive");
     return;
   }
-  const uint32_t nb = div_up(n, kTile);
-  uint32_t* partial = dalloc<uint32_t>(c, nb);
-  k_scan_reduce<<<nb, kThreads, 0, c->stream>>>(in, n, partial);
-  if (nb <= kFuseMax) {
-    k_scan_down_fused<<<nb, kThreads, 0, c->stream>>>(in, out, n, partial, total);
-  } else {
-    // the tile sums are scanned by the same multi-workgroup scan (at most
-    // three levels deep for 2^32 elements)
-    scan_exclusive(c, partial, partial, nb, total);
-    k_scan_down<<<nb, kThreads, 0, c->stream>>>(in, out, n, partial);
-  }
+  const uint64_t tiles = div_up(n, kLbTile);
+  const LbPass p = lb_pass(c, tiles);
+  k_scan_lb<<<(uint32_t)tiles, kThreads, 0, c->stream>>>(in, out, n, p, total);
   check_launch("scan_exclusive");
-  dfree(c, partial);
 }
 
 void scan_exclusive2(svd_ctx* c, uint32_t* a, uint32_t* b, uint64_t n) {
@@ -175,6 +129,27 @@ void scan_exclusive2(svd_ctx* c, uint32_t* a, uint32_t* b, uint64_t n) {
   jobs.j[1] = ScanJob{b, b, nullptr};
   k_scan_small<<<2, kSmallThreads, 0, c->stream>>>(jobs, n);
   check_launch("scan_exclusive2");
+}
+
+LbPass lb_pass(svd_ctx* c, uint64_t tiles) {
+  if (tiles >= (1ull << 31)) throw Error{SVD_ERR_CAPACITY, "single-pass scan: too many tiles"};
+  bool clear = false;
+  if (tiles > c->lbCap) {
+    dfree(c, c->lbStatus);
+    c->lbCap = std::max<uint64_t>(tiles, 4096);
+    c->lbStatus = dalloc<unsigned long long>(c, c->lbCap);
+    clear = true;
+  }
+  if (!c->lbCounter) {
+    c->lbCounter = dalloc<uint32_t>(c, 1);
+    fill_u32(c, c->lbCounter, 0u, 1);
+  }
+  if (++c->lbEpoch >= (1u << 30)) {
+    c->lbEpoch = 1;
+    clear = true;
+  }
+  if (clear) fill_u32(c, (uint32_t*)c->lbStatus, 0u, 2 * c->lbCap);
+  return LbPass{c->lbStatus, c->lbCounter, c->lbEpoch, (uint32_t)tiles};
 }
 
 void fill_u32(svd_ctx* c, uint32_t* d, uint32_t v, uint64_t n) {

@@ -55,6 +55,10 @@ struct svd_ctx {
   std::map<uint64_t, int> occupancy;         // (kernel id, LDS bytes) -> resident workgroups per CU
   float* renderScratch = nullptr;            // device address of svd_render_noalloc's static texture
   hipError_t renderScratchErr = hipSuccess;  // why it is null, if it is
+  unsigned long long* lbStatus = nullptr;    // single-pass scans: one published word per tile
+  uint64_t lbCap = 0;                        // tiles lbStatus holds
+  uint32_t* lbCounter = nullptr;             // tile counter, zero between passes
+  uint32_t lbEpoch = 0;                      // epoch of the last pass (words of older passes are stale)
 };
 
 namespace svd {
@@ -160,6 +164,12 @@ uint64_t read_u32(svd_ctx* c, const uint32_t* d);
 void read_u32s(svd_ctx* c, const uint32_t* d, uint32_t* h, size_t n);
 
 // ---- scan (scan.hip) ---------------------------------------------------------
+// The look-back state of one single-pass scan over `tiles` tiles on the
+// context's stream (lookback.hpp): a fresh epoch, the status array grown to
+// fit.  Passes are ordered by the stream; a pass must not be replayed from a
+// captured graph (its epoch would repeat).
+struct LbPass;
+LbPass lb_pass(svd_ctx* c, uint64_t tiles);
 // Exclusive prefix sum of n u32 values; out may alias in.  *total (device) gets
 // the sum (may be null).  Scratch comes from the context arena.
 void scan_exclusive(svd_ctx* c, const uint32_t* in, uint32_t* out, uint64_t n, uint32_t* total);
@@ -179,6 +189,9 @@ void build_svo_from_mesh_subtrees(svd_octree* o, const float* dtris, uint64_t nt
 // sizes (optional, host): the node count of every level, known in advance
 uint64_t svo_levels_from_codes(svd_ctx* c, const uint64_t* dcodes, uint64_t n, uint32_t L, std::vector<Level>& svo,
                                std::vector<uint64_t*>* levelCodes, const uint64_t* sizes = nullptr);
+// inner levels L-2 .. 0 from the finest level's parent codes (taken over and freed)
+uint64_t svo_inner_levels(svd_ctx* c, uint64_t* codeA, uint64_t nc, uint32_t L, std::vector<Level>& svo,
+                          std::vector<uint64_t*>* levelCodes, const uint64_t* sizes);
 // one inner level: exact-key dedup, first occurrence wins -> remap + compacted records
 uint64_t dedup_level(svd_ctx* c, const uint32_t* rec, uint64_t n, uint32_t* remap, uint32_t** outRec);
 void remap_children(svd_ctx* c, const uint32_t* in, uint64_t n, const uint32_t* remap, uint32_t* out);
