@@ -8,6 +8,7 @@
 #include "svd_internal.hpp"
 #include "block_scan.hpp"
 #include "lookback.hpp"
+#include "scan_op.hpp"
 #include <algorithm>
 
 namespace svd {
@@ -324,6 +325,56 @@ __global__ void __launch_bounds__(1024) k_inner_level_small(const uint64_t* __re
   }
 }
 
+// Every inner SVO level from the first with at most kSvoSmall child codes up
+// to the root in one workgroup: per level the heads scan and the records as
+// k_inner_level_small, the parent codes of one level the child codes of the
+// next (ping-pong between ca and cb in global memory, ordered by barriers).
+struct SvoTop {
+  uint32_t* rec[kMaxLevels];
+  int32_t top;                 // first level handled (levels top .. 0)
+};
+__global__ void __launch_bounds__(1024) k_inner_levels_small(uint64_t* ca, uint64_t* cb, uint32_t n, SvoTop lv) {
+  __shared__ uint32_t pidx[kSvoSmall];
+  __shared__ uint32_t lds[1024 / 64 + 1];
+  for (int lev = lv.top; lev >= 0; --lev) {
+    const uint64_t* ccode = ca;
+    uint32_t carry = 0;
+    for (uint32_t base = 0; base < n; base += 1024) {
+      const uint32_t i = base + threadIdx.x;
+      const uint32_t h = (i < n && head64(ccode, i)) ? 1u : 0u;
+      uint32_t agg;
+      const uint32_t ex = block_exclusive_sum(h, lds, agg) + carry;
+      if (i < n) pidx[i] = ex;
+      carry += agg;
+    }
+    __syncthreads();
+    uint32_t* rec = lv.rec[lev];
+    for (uint32_t i = threadIdx.x; i < n; i += 1024) {
+      if (!head64(ccode, i)) continue;
+      const uint64_t p = ccode[i] >> 3;
+      uint32_t w[kRecWords];
+      w[0] = 0;
+#pragma unroll
+      for (int c = 0; c < 8; ++c) w[1 + c] = kNull;
+      for (uint32_t j = i; j < n && j < i + 8 && (ccode[j] >> 3) == p; ++j) {
+        const uint32_t sl = (uint32_t)(ccode[j] & 7u);
+        w[0] |= 1u << sl;
+        w[1 + sl] = j;
+      }
+      uint32_t* o = rec + (uint64_t)pidx[i] * kRecWords;
+#pragma unroll
+      for (int k = 0; k < kRecWords; ++k) o[k] = w[k];
+      cb[pidx[i]] = p;
+    }
+    __threadfence_block();
+    __syncthreads();
+    n = carry;
+    uint64_t* t = ca;
+    ca = cb;
+    cb = t;
+  }
+}
+
 // ---------------------------------------------------------------- hash dedup
 __device__ __forceinline__ void load_rec(const uint32_t* __restrict__ rec, uint64_t i, uint32_t w[kRecWords]) {
   const uint32_t* r = rec + i * kRecWords;
@@ -457,17 +508,6 @@ void launch_insert(svd_ctx* c, const uint32_t* rec, uint64_t n, TEntry* table, u
 
 __device__ __forceinline__ uint32_t entry_index(TEntry e) { return (uint32_t)e; }
 
-// lookup from slot h on (fp = the key hash's upper half).
-__device__ __forceinline__ uint32_t lookup_from(const uint32_t* __restrict__ rec, const TEntry* __restrict__ table,
-                                                uint32_t cmask, const uint32_t w[kRecWords], uint32_t fp, uint32_t h) {
-  while (true) {
-    const TEntry e = table[h];
-    if (e == kEmpty) return kNone;
-    if ((uint32_t)(e >> 32) == fp && rec_eq(rec, (uint32_t)e, w)) return (uint32_t)e;
-    h = (h + 1) & cmask;
-  }
-}
-
 // Probe for a key without inserting; returns the stored (minimum) index.
 __device__ __forceinline__ uint32_t lookup(const uint32_t* __restrict__ rec, const TEntry* __restrict__ table,
                                            uint32_t cmask, const uint32_t w[kRecWords]) {
@@ -485,7 +525,7 @@ __device__ __forceinline__ uint32_t lookup(const uint32_t* __restrict__ rec, con
 // A small inner level of toDAG (n <= kSmallLevel) in one workgroup: the
 // children remap, the first-occurrence insert (table in LDS), the
 // representative flags, their scan and the compaction of k_remap_children,
-// k_insert, k_rep_flags, scan_exclusive and k_compact, with workgroup
+// k_insert, the representative flags and their scan, and k_compact, with workgroup
 // barriers instead of launches.  The table size does not change the result:
 // each key's slot ends with its smallest index whatever the probe sequence.
 constexpr uint32_t kSmallLevel = 4096;
@@ -494,15 +534,19 @@ constexpr uint32_t kSmallCap = 8192;        // power of two >= 2 * kSmallLevel
 // measured on the 2048^3 terrain it wins up to one node per thread only
 constexpr uint32_t kSdagSmall = 1024;
 constexpr int kSmallThreads = 1024;
-__global__ void __launch_bounds__(kSmallThreads) k_dedup_small(const uint32_t* __restrict__ src, uint32_t n,
-                                                               const uint32_t* __restrict__ remapIn,
-                                                               uint32_t* __restrict__ keyed, uint32_t* __restrict__ out,
-                                                               uint32_t* __restrict__ remapOut,
-                                                               uint32_t* __restrict__ total) {
-  __shared__ TEntry table[kSmallCap];
-  __shared__ uint16_t slotOf[kSmallLevel];
-  __shared__ uint32_t posOf[kSmallLevel];
-  __shared__ uint32_t lds[kSmallThreads / 64 + 1];
+struct SmallLds {
+  TEntry table[kSmallCap];
+  uint16_t slotOf[kSmallLevel];
+  uint32_t posOf[kSmallLevel];
+  uint32_t lds[kSmallThreads / 64 + 1];
+};
+__device__ void dedup_small_level(const uint32_t* __restrict__ src, uint32_t n, const uint32_t* __restrict__ remapIn,
+                                  uint32_t* __restrict__ keyed, uint32_t* __restrict__ out,
+                                  uint32_t* __restrict__ remapOut, uint32_t* __restrict__ total, SmallLds& sh) {
+  TEntry* table = sh.table;
+  uint16_t* slotOf = sh.slotOf;
+  uint32_t* posOf = sh.posOf;
+  uint32_t* lds = sh.lds;
   const uint32_t tid = threadIdx.x;
   for (uint32_t j = tid; j < kSmallCap; j += kSmallThreads) table[j] = kEmpty;
   for (uint32_t i = tid; i < n; i += kSmallThreads) {
@@ -568,13 +612,59 @@ __global__ void __launch_bounds__(kSmallThreads) k_dedup_small(const uint32_t* _
   if (tid == 0) *total = carry;
 }
 
-__global__ void k_rep_flags(const uint32_t* __restrict__ slot, const TEntry* __restrict__ table, uint64_t n,
-                            uint32_t* __restrict__ flag) {
-  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const uint32_t s = slot[i];
-  flag[i] = (s != kNone && entry_index(table[s]) == (uint32_t)i) ? 1u : 0u;
+
+// toDAG of every level from `top` up to the root in one workgroup, each of
+// at most kSmallLevel nodes: per level dedup_small_level (children remapped
+// by the level below's correspondence, then the first-occurrence dedup); the
+// root only has its children remapped (it is never reduced,
+// geom_octree.cpp:483).  A level's correspondence goes to global memory and
+// is read by the next level after a barrier.
+struct DagTop {
+  const uint32_t* src[kMaxLevels];
+  uint32_t* out[kMaxLevels];
+  uint32_t n[kMaxLevels];
+  const uint32_t* remapIn;   // correspondence of level top + 1
+  uint32_t* remap[2];        // ping-pong correspondences of the levels handled
+  uint32_t* keyed;           // scratch: n[top] records
+  uint32_t* counts;          // counts[lev] = unique nodes of level lev
+  int32_t top;
+};
+__global__ void __launch_bounds__(kSmallThreads) k_dedup_levels_small(DagTop a) {
+  __shared__ SmallLds sh;
+  const uint32_t* remapIn = a.remapIn;
+  for (int lev = a.top; lev >= 1; --lev) {
+    uint32_t* remapOut = a.remap[(a.top - lev) & 1];
+    dedup_small_level(a.src[lev], a.n[lev], remapIn, a.keyed, a.out[lev], remapOut, a.counts + lev, sh);
+    __threadfence_block();
+    __syncthreads();
+    remapIn = remapOut;
+  }
+  for (uint32_t i = threadIdx.x; i < a.n[0]; i += kSmallThreads) {
+    uint32_t w[kRecWords];
+    load_rec(a.src[0], i, w);
+    const uint32_t m = w[0] & 0xFFu;
+#pragma unroll
+    for (int c = 0; c < 8; ++c)
+      if ((m >> c) & 1u) w[1 + c] = remapIn[w[1 + c]];
+    uint32_t* o = a.out[0] + (uint64_t)i * kRecWords;
+#pragma unroll
+    for (int k = 0; k < kRecWords; ++k) o[k] = w[k];
+  }
 }
+
+// pos[i] = number of representatives before node i (scan_op): a node is its
+// key's representative when the table slot holds its own index.
+struct RepFlagOp {
+  const uint32_t* __restrict__ slot;
+  const TEntry* __restrict__ table;
+  uint32_t* __restrict__ pos;
+  __device__ uint32_t value(uint64_t i, uint32_t&) const {
+    const uint32_t s = slot[i];
+    return (s != kNone && entry_index(table[s]) == (uint32_t)i) ? 1u : 0u;
+  }
+  __device__ void apply(uint64_t i, uint32_t ex) const { pos[i] = ex; }
+  __device__ void report(uint32_t) const {}
+};
 
 __global__ void k_compact(const uint32_t* __restrict__ rec, const uint32_t* __restrict__ slot,
                           const TEntry* __restrict__ table, const uint32_t* __restrict__ pos, uint64_t n,
@@ -774,61 +864,42 @@ enum : uint32_t { kUndecided = 0, kRep = 1, kNonRep = 2 };
 
 // First-occurrence nodes look up their 7 mirrored variants; an earlier node
 // carrying that key is a candidate that, if it is a representative, takes
-// this node.  Later duplicates of a key are never representatives.
+// this node.  Later duplicates of a key are never representatives.  Eight
+// lanes per node: lane t of the group looks up variant t, lane 0 sets the
+// state from the group's ballot, so a node's chain is one lookup long.
 __global__ void k_sdag_cands(const uint32_t* __restrict__ key, uint64_t n, const TEntry* __restrict__ table,
                              uint32_t cmask, const uint32_t* __restrict__ slot, const uint8_t* __restrict__ childInv,
                              uint32_t* __restrict__ cand, uint32_t* __restrict__ state) {
-  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+  const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t i = g >> 3;
+  const int t = (int)(g & 7u);
+  const bool live = i < n;
   // a node without children was never inserted (slot kNone): it is no
   // representative and k_sdag_outcome reports it, as k_sdag_small does
-  const uint32_t si = slot[i];
-  const bool first = si != kNone && entry_index(table[si]) == (uint32_t)i;
-  uint32_t* cd = cand + i * 7;
-  if (!first) {
-#pragma unroll
-    for (int t = 0; t < 7; ++t) cd[t] = kNone;
-    state[i] = kNonRep;
-    return;
+  bool first = false;
+  if (live) {
+    const uint32_t si = slot[i];
+    first = si != kNone && entry_index(table[si]) == (uint32_t)i;
   }
-  uint32_t k0[kRecWords], v[kRecWords];
-  load_rec(key, i, k0);
-  const ChildInv ci = child_inv(k0, childInv);
-  // The 7 lookups run side by side: all first probes are loaded together,
-  // then the records of the fingerprint matches; only a probe that meets
-  // another key walks on, one slot at a time.
-  uint32_t fp[7], hs[7];
-  bool self[7];
-#pragma unroll
-  for (int t = 1; t < 8; ++t) {
+  bool found = false;
+  uint32_t r = kNone;
+  if (first && t > 0) {
+    uint32_t k0[kRecWords], v[kRecWords];
+    load_rec(key, i, k0);
+    const ChildInv ci = child_inv(k0, childInv);
     make_variant(k0, t, ci, v);
     // a variant equal to the node's own key finds the node itself
-    bool same = true;
+    bool self = true;
 #pragma unroll
-    for (int k = 0; k < kRecWords; ++k) same = same && v[k] == k0[k];
-    self[t - 1] = same;
-    const uint64_t hv = hash_rec(v);
-    fp[t - 1] = (uint32_t)(hv >> 32);
-    hs[t - 1] = (uint32_t)hv & cmask;
+    for (int k = 0; k < kRecWords; ++k) self = self && v[k] == k0[k];
+    r = self ? (uint32_t)i : lookup(key, table, cmask, v);
+    found = r != kNone && r < (uint32_t)i;
   }
-  TEntry e[7];
-#pragma unroll
-  for (int t = 0; t < 7; ++t) e[t] = self[t] ? kEmpty : table[hs[t]];
-  bool any = false;
-#pragma unroll
-  for (int t = 1; t < 8; ++t) {
-    const TEntry et = e[t - 1];
-    uint32_t r = kNone;
-    if (self[t - 1]) r = (uint32_t)i;
-    else if (et != kEmpty) {
-      make_variant(k0, t, ci, v);
-      if ((uint32_t)(et >> 32) == fp[t - 1] && rec_eq(key, (uint32_t)et, v)) r = (uint32_t)et;
-      else r = lookup_from(key, table, cmask, v, fp[t - 1], (hs[t - 1] + 1) & cmask);
-    }
-    if (r != kNone && r < (uint32_t)i) { cd[t - 1] = r; any = true; }
-    else cd[t - 1] = kNone;
-  }
-  state[i] = any ? kUndecided : kRep;
+  const uint64_t b = __ballot(found);
+  if (!live) return;
+  uint32_t* cd = cand + i * 7;
+  if (t > 0) cd[t - 1] = found ? r : kNone;
+  else state[i] = !first ? kNonRep : (((b >> (threadIdx.x & 63)) & 0xFEull) ? kUndecided : kRep);
 }
 
 // One resolution round.  state only moves Undecided -> final, so a stale
@@ -852,10 +923,14 @@ __global__ void k_sdag_resolve(const uint32_t* __restrict__ cand, uint64_t n, ui
   else atomicAdd(remaining, 1u);
 }
 
-__global__ void k_state_flags(const uint32_t* __restrict__ state, uint64_t n, uint32_t* __restrict__ flag) {
-  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) flag[i] = state[i] == kRep ? 1u : 0u;
-}
+// pos[i] = number of representatives before node i (scan_op).
+struct StateFlagOp {
+  const uint32_t* __restrict__ state;
+  uint32_t* __restrict__ pos;
+  __device__ uint32_t value(uint64_t i, uint32_t&) const { return state[i] == kRep ? 1u : 0u; }
+  __device__ void apply(uint64_t i, uint32_t ex) const { pos[i] = ex; }
+  __device__ void report(uint32_t) const {}
+};
 
 // Correspondence of every DAG node: itself if a representative, else the
 // first transform (in reference order) whose key is an earlier
@@ -913,26 +988,32 @@ __global__ void k_sdag_outcome(const uint32_t* __restrict__ key, uint64_t n, con
   atomicAdd(err, 1u);
 }
 
-// A small level of toSDAG (n <= kSmallLevel) in one workgroup: keys,
+// A small level of toSDAG (n <= kSdagSmall) in one workgroup: keys,
 // first-occurrence insert (table and node states in LDS), candidates, the
 // resolution rounds until every node is decided, representative flags and
 // their scan, and the outcome, as k_sdag_keys ... k_sdag_outcome do over
 // several launches.  Every thread reads the round's undecided count after
 // a barrier, so all leave the loop together; each round decides at least
 // the earliest undecided node, so it ends within n rounds.
-__global__ void __launch_bounds__(kSmallThreads) k_sdag_small(const uint32_t* __restrict__ dag, uint32_t n,
-                                                              const uint32_t* __restrict__ childMap,
-                                                              const uint8_t* __restrict__ childInv,
-                                                              uint32_t* __restrict__ key, uint8_t* __restrict__ inv,
-                                                              uint32_t* __restrict__ cand, uint32_t* __restrict__ map,
-                                                              uint32_t* __restrict__ out, uint8_t* __restrict__ invOut,
-                                                              uint32_t* __restrict__ total, uint32_t* __restrict__ err) {
-  __shared__ TEntry table[kSmallCap];
-  __shared__ uint16_t slotOf[kSmallLevel];
-  __shared__ uint32_t state[kSmallLevel];
-  __shared__ uint32_t posOf[kSmallLevel];
-  __shared__ uint32_t lds[kSmallThreads / 64 + 1];
-  __shared__ uint32_t remaining;
+struct SdagLds {
+  TEntry table[kSmallCap];
+  uint16_t slotOf[kSmallLevel];
+  uint32_t state[kSmallLevel];
+  uint32_t posOf[kSmallLevel];
+  uint32_t lds[kSmallThreads / 64 + 1];
+  uint32_t remaining;
+};
+__device__ void sdag_small_level(const uint32_t* __restrict__ dag, uint32_t n, const uint32_t* __restrict__ childMap,
+                                 const uint8_t* __restrict__ childInv, uint32_t* __restrict__ key,
+                                 uint8_t* __restrict__ inv, uint32_t* __restrict__ cand, uint32_t* __restrict__ map,
+                                 uint32_t* __restrict__ out, uint8_t* __restrict__ invOut,
+                                 uint32_t* __restrict__ total, uint32_t* __restrict__ err, SdagLds& sh) {
+  TEntry* table = sh.table;
+  uint16_t* slotOf = sh.slotOf;
+  uint32_t* state = sh.state;
+  uint32_t* posOf = sh.posOf;
+  uint32_t* lds = sh.lds;
+  uint32_t& remaining = sh.remaining;
   const uint32_t tid = threadIdx.x;
   for (uint32_t j = tid; j < kSmallCap; j += kSmallThreads) table[j] = kEmpty;
   for (uint32_t i = tid; i < n; i += kSmallThreads) sdag_key(dag, i, childMap, key, inv);
@@ -1020,7 +1101,7 @@ __global__ void __launch_bounds__(kSmallThreads) k_sdag_small(const uint32_t* __
     __syncthreads();
     if (left == 0) break;
   }
-  // flags + scan (as k_state_flags + scan_exclusive)
+  // flags + scan (as StateFlagOp)
   uint32_t carry = 0;
   for (uint32_t base = 0; base < n; base += kSmallThreads) {
     const uint32_t i = base + tid;
@@ -1076,6 +1157,39 @@ __global__ void __launch_bounds__(kSmallThreads) k_sdag_small(const uint32_t* __
     if (!found) { map[i] = kNone; atomicAdd(err, 1u); }
   }
   if (tid == 0) *total = carry;
+}
+
+// toSDAG of every level from `top` up to the root in one workgroup, each of
+// at most kSdagSmall nodes: per level sdag_small_level, whose
+// correspondence and invariance bits the next level reads after a barrier;
+// the root only gets its identity key (children moved to the SDAG ids below,
+// mirror bits from their correspondence) and clear invariance bits: it is
+// never reduced.
+struct SdagTop {
+  const uint32_t* dag[kMaxLevels];
+  uint32_t n[kMaxLevels];
+  uint32_t* map[kMaxLevels + 1];      // map[lev + 1]: correspondence of the level below lev
+  uint8_t* invOut[kMaxLevels + 1];    // SDAG invariance bits per level (inv of level lev + 1 = childInv)
+  uint32_t* out[kMaxLevels];
+  uint32_t* key;                      // scratch, max n records
+  uint8_t* inv;                       // scratch, max n
+  uint32_t* cand;                     // scratch, max n * 7
+  uint32_t* counts;
+  uint32_t* err;
+  int32_t top;
+};
+__global__ void __launch_bounds__(kSmallThreads) k_sdag_levels_small(SdagTop a) {
+  __shared__ SdagLds sh;
+  for (int lev = a.top; lev >= 1; --lev) {
+    sdag_small_level(a.dag[lev], a.n[lev], a.map[lev + 1], a.invOut[lev + 1], a.key, a.inv, a.cand, a.map[lev],
+                     a.out[lev], a.invOut[lev], a.counts + lev, a.err, sh);
+    __threadfence_block();
+    __syncthreads();
+  }
+  for (uint32_t i = threadIdx.x; i < a.n[0]; i += kSmallThreads) {
+    sdag_key(a.dag[0], i, a.map[1], a.out[0], nullptr);
+    a.invOut[0][i] = 0;
+  }
 }
 
 // Finest level.  The reference loop walks the nodes in order and makes a node
@@ -1211,6 +1325,20 @@ uint64_t svo_inner_levels(svd_ctx* c, uint64_t* codeA, uint64_t nc, uint32_t L, 
   uint64_t nNodes = 0;
   for (int lev = (int)L - 2; lev >= 0; --lev) {
     Level& lv = svo[lev];
+    if (sizes && nc <= kSvoSmall && !levelCodes) {
+      // this level and all above it in one launch
+      SvoTop top{};
+      top.top = lev;
+      for (int l = lev; l >= 0; --l) {
+        svo[l].n = sizes[l];
+        svo[l].rec = dalloc<uint32_t>(c, (sizes[l] ? sizes[l] : 1) * kRecWords);
+        top.rec[l] = svo[l].rec;
+        if (l > 0) nNodes += sizes[l];
+      }
+      k_inner_levels_small<<<1, 1024, 0, c->stream>>>(codeA, codeB, (uint32_t)nc, top);
+      check_launch("k_inner_levels_small");
+      break;
+    }
     if (sizes && nc <= kSvoSmall) {
       const uint64_t np = sizes[lev];
       lv.n = np;
@@ -1289,9 +1417,8 @@ void dedup_level_dev(svd_ctx* c, const uint32_t* rec, uint64_t n, uint32_t* rema
   uint32_t* pos = dalloc<uint32_t>(c, n + 1);
   fill_u32(c, (uint32_t*)table, kNone, 2ull * cap);
   launch_insert(c, rec, n, table, cap - 1, slot);
-  k_rep_flags<<<div_up(n, kB), kB, 0, c->stream>>>(slot, table, n, pos);
   check_launch("k_insert");
-  scan_exclusive(c, pos, pos, n, total);
+  scan_op<4>(c, n, RepFlagOp{slot, table, pos}, total);
   uint32_t* out = dalloc<uint32_t>(c, (n ? n : 1) * kRecWords);
   k_compact<<<div_up(n, kB), kB, 0, c->stream>>>(rec, slot, table, pos, n, out, remap);
   check_launch("k_compact");
@@ -1341,31 +1468,38 @@ uint64_t dedup_all(svd_ctx* c, uint32_t L, const std::vector<Level>& src, std::v
   }
   for (int lev = (int)L - 2; lev >= 0; --lev) {
     const Level& s = src[lev];
+    // this level and all above it at most kSmallLevel nodes: one launch
+    uint64_t maxTop = 0;
+    for (int l = lev; l >= 1; --l) maxTop = std::max<uint64_t>(maxTop, src[l].n);
+    if (maxTop <= kSmallLevel) {
+      DagTop top{};
+      top.top = lev;
+      top.remapIn = remap;
+      for (int l = lev; l >= 0; --l) {
+        top.src[l] = src[l].rec;
+        top.n[l] = (uint32_t)src[l].n;
+        dst[l].rec = dalloc<uint32_t>(c, (src[l].n ? src[l].n : 1) * kRecWords);
+        top.out[l] = dst[l].rec;
+      }
+      dst[0].n = src[0].n;
+      top.remap[0] = dalloc<uint32_t>(c, maxTop + 1);
+      top.remap[1] = dalloc<uint32_t>(c, maxTop + 1);
+      top.keyed = dalloc<uint32_t>(c, (maxTop ? maxTop : 1) * kRecWords);
+      top.counts = counts;
+      k_dedup_levels_small<<<1, kSmallThreads, 0, c->stream>>>(top);
+      check_launch("k_dedup_levels_small");
+      dfree(c, top.remap[0]); dfree(c, top.remap[1]); dfree(c, top.keyed);
+      break;
+    }
     uint32_t* keyed = dalloc<uint32_t>(c, (s.n ? s.n : 1) * kRecWords);
-    const bool small = lev != 0 && s.n <= kSmallLevel;   // k_dedup_small remaps the children itself
-    if (s.n && !small) k_remap_children<<<div_up(s.n, kB), kB, 0, c->stream>>>(s.rec, s.n, remap, keyed);
+    if (s.n) k_remap_children<<<div_up(s.n, kB), kB, 0, c->stream>>>(s.rec, s.n, remap, keyed);
     check_launch("k_remap_children");
     Level& d = dst[lev];
-    if (lev == 0) {
-      // the root level is never reduced (geom_octree.cpp:483)
-      d.rec = keyed;
-      d.n = s.n;
-    } else if (small) {
-      uint32_t* nremap = dalloc<uint32_t>(c, s.n + 1);
-      d.rec = dalloc<uint32_t>(c, (s.n ? s.n : 1) * kRecWords);
-      k_dedup_small<<<1, kSmallThreads, 0, c->stream>>>(s.rec, (uint32_t)s.n, remap, keyed, d.rec, nremap,
-                                                        counts + lev);
-      check_launch("k_dedup_small");
-      dfree(c, keyed);
-      dfree(c, remap);
-      remap = nremap;
-    } else {
-      uint32_t* nremap = dalloc<uint32_t>(c, s.n + 1);
-      dedup_level_dev(c, keyed, s.n, nremap, &d.rec, counts + lev);
-      dfree(c, keyed);
-      dfree(c, remap);
-      remap = nremap;
-    }
+    uint32_t* nremap = dalloc<uint32_t>(c, s.n + 1);
+    dedup_level_dev(c, keyed, s.n, nremap, &d.rec, counts + lev);
+    dfree(c, keyed);
+    dfree(c, remap);
+    remap = nremap;
   }
   dfree(c, remap);
   std::vector<uint32_t> h(L);
@@ -1429,17 +1563,39 @@ bool reduce_sym_pass(svd_octree* o, bool speculative) {
     const uint8_t* childInv = (lev < (int)L - 1) ? o->sdag[lev + 1].inv : nullptr;
     uint32_t* key = dalloc<uint32_t>(c, (n ? n : 1) * kRecWords);
     uint8_t* inv = dalloc<uint8_t>(c, n + 1);
-    if (lev > 0 && n <= kSdagSmall) {
-      // the whole level in one workgroup (it computes the keys too)
-      uint32_t* cand = dalloc<uint32_t>(c, (n ? n : 1) * 7);
-      s.rec = dalloc<uint32_t>(c, (n ? n : 1) * kRecWords);
-      s.inv = dalloc<uint8_t>(c, n + 1);
-      o->sdagMap[lev] = dalloc<uint32_t>(c, n + 1);
-      k_sdag_small<<<1, kSmallThreads, 0, c->stream>>>(d.rec, (uint32_t)n, o->sdagMap[lev + 1], childInv, key, inv,
-                                                       cand, o->sdagMap[lev], s.rec, s.inv, counts + lev, err);
-      check_launch("k_sdag_small");
-      dfree(c, key); dfree(c, inv); dfree(c, cand);
-      continue;
+    // this level and all above it at most kSdagSmall nodes: one launch
+    uint64_t maxTop = 0;
+    for (int l = lev; l >= 1; --l) maxTop = std::max<uint64_t>(maxTop, o->dag[l].n);
+    if (maxTop <= kSdagSmall) {
+      dfree(c, key);
+      dfree(c, inv);
+      SdagTop top{};
+      top.top = lev;
+      for (int l = lev; l >= 0; --l) {
+        const uint64_t m = o->dag[l].n;
+        top.dag[l] = o->dag[l].rec;
+        top.n[l] = (uint32_t)m;
+        Level& sl = o->sdag[l];
+        sl.rec = dalloc<uint32_t>(c, (m ? m : 1) * kRecWords);
+        sl.inv = dalloc<uint8_t>(c, m + 1);
+        top.out[l] = sl.rec;
+        top.invOut[l] = sl.inv;
+        if (l > 0) o->sdagMap[l] = dalloc<uint32_t>(c, m + 1);
+        top.map[l] = o->sdagMap[l];
+      }
+      top.map[lev + 1] = o->sdagMap[lev + 1];
+      top.invOut[lev + 1] = o->sdag[lev + 1].inv;
+      o->sdag[0].n = o->dag[0].n;
+      o->sdagMap[0] = nullptr;
+      top.key = dalloc<uint32_t>(c, (maxTop ? maxTop : 1) * kRecWords);
+      top.inv = dalloc<uint8_t>(c, maxTop + 1);
+      top.cand = dalloc<uint32_t>(c, (maxTop ? maxTop : 1) * 7);
+      top.counts = counts;
+      top.err = err;
+      k_sdag_levels_small<<<1, kSmallThreads, 0, c->stream>>>(top);
+      check_launch("k_sdag_levels_small");
+      dfree(c, top.key); dfree(c, top.inv); dfree(c, top.cand);
+      break;
     }
     // the root level is never reduced and needs no table
     const uint32_t cap = lev == 0 ? 0u : table_cap(n);
@@ -1462,7 +1618,7 @@ bool reduce_sym_pass(svd_octree* o, bool speculative) {
     uint32_t* state = dalloc<uint32_t>(c, n);
     uint32_t* pos = dalloc<uint32_t>(c, n + 1);
     launch_insert(c, key, n, table, cap - 1, slot);
-    k_sdag_cands<<<div_up(n, kB), kB, 0, c->stream>>>(key, n, table, cap - 1, slot, childInv, cand, state);
+    k_sdag_cands<<<div_up(8 * n, kB), kB, 0, c->stream>>>(key, n, table, cap - 1, slot, childInv, cand, state);
     check_launch("k_sdag_cands");
     if (speculative) {
       k_sdag_resolve<<<div_up(n, kB), kB, 0, c->stream>>>(cand, n, state, count);
@@ -1477,8 +1633,7 @@ bool reduce_sym_pass(svd_octree* o, bool speculative) {
         if (round > (int)n + 2) throw Error{SVD_ERR_DEVICE, "toSDAG: resolution did not converge"};
       }
     }
-    k_state_flags<<<div_up(n, kB), kB, 0, c->stream>>>(state, n, pos);
-    scan_exclusive(c, pos, pos, n, counts + lev);
+    scan_op<4>(c, n, StateFlagOp{state, pos}, counts + lev);
     // n records: the upper bound of the representative count, known on the host
     s.rec = dalloc<uint32_t>(c, (n ? n : 1) * kRecWords);
     s.inv = dalloc<uint8_t>(c, n + 1);

@@ -4,6 +4,7 @@
 //   EncodedSSVDAG  ::encode+save (encoded_ssvdag.cpp:84-466), on the SDAG
 //                  (.ssvdag) or on the DAG (.esvdag, svbuilder/main.cpp:243-244).
 #include "svd_internal.hpp"
+#include "scan_op.hpp"
 #include "block_scan.hpp"
 #include <algorithm>
 
@@ -179,38 +180,47 @@ __global__ void k_scatter_rank(const uint32_t* __restrict__ perm, uint64_t n, ui
 }
 
 // Encoded size (u16 words) of the node at sorted position i.
-__global__ void k_inner_sizes(const uint32_t* __restrict__ rec, const uint32_t* __restrict__ perm, uint64_t n,
-                              const uint32_t* __restrict__ indChild, uint32_t* __restrict__ sz,
-                              uint32_t* __restrict__ err) {
-  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const uint32_t* r = rec + (uint64_t)perm[i] * kRecWords;
-  const uint32_t m = r[0] & 0xFFu;
-  uint32_t s = 1;
+// The layout of an inner level in its refcount order (scan_op): the encoded
+// size in u16 words of the node at sorted position i (one word of mask plus
+// one or two per child by the child's address), its offset = the sizes
+// before it, and ind[node] = that offset.
+struct LayoutOp {
+  const uint32_t* __restrict__ rec;
+  const uint32_t* __restrict__ perm;
+  const uint32_t* __restrict__ indChild;
+  uint32_t* __restrict__ offs;
+  uint32_t* __restrict__ ind;
+  uint32_t* __restrict__ err;
+  __device__ uint32_t value(uint64_t i, uint32_t& flags) const {
+    const uint32_t* r = rec + (uint64_t)perm[i] * kRecWords;
+    const uint32_t m = r[0] & 0xFFu;
+    uint32_t s = 1;
 #pragma unroll
-  for (int c = 0; c < 8; ++c) {
-    if (!((m >> c) & 1u)) continue;
-    const uint32_t a = indChild[r[1 + c]];
-    if (a < (1u << 13)) s += 1;
-    else if (a < (1u << 30)) s += 2;
-    else atomicAdd(err, 1u);
+    for (int c = 0; c < 8; ++c) {
+      if (!((m >> c) & 1u)) continue;
+      const uint32_t a = indChild[r[1 + c]];
+      if (a < (1u << 13)) s += 1;
+      else if (a < (1u << 30)) s += 2;
+      else flags = 1;
+    }
+    return s;
   }
-  sz[i] = s;
-}
+  __device__ void report(uint32_t) const { atomicAdd(err, 1u); }
+  __device__ void apply(uint64_t i, uint32_t ex) const {
+    offs[i] = ex;
+    ind[perm[i]] = ex;
+  }
+};
 
-// The layout of a small level (n <= kLayoutSmall) in one workgroup:
-// k_inner_sizes, the exclusive scan of the sizes and k_offsets_to_ind with
-// workgroup barriers instead of launches.
+// The layout of a small level (n <= kLayoutSmall) in one workgroup: the
+// sizes, their exclusive scan and the node addresses of LayoutOp with
+// workgroup barriers.
 constexpr uint32_t kLayoutSmall = 4096;   // above this the three launches are faster (measured)
 constexpr int kLayoutThreads = 1024;
-__global__ void __launch_bounds__(kLayoutThreads) k_inner_layout_small(const uint32_t* __restrict__ rec,
-                                                                       const uint32_t* __restrict__ perm, uint32_t n,
-                                                                       const uint32_t* __restrict__ indChild,
-                                                                       uint32_t* __restrict__ offs,
-                                                                       uint32_t* __restrict__ ind,
-                                                                       uint32_t* __restrict__ total,
-                                                                       uint32_t* __restrict__ err) {
-  __shared__ uint32_t lds[kLayoutThreads / 64 + 1];
+__device__ void inner_layout_small(const uint32_t* __restrict__ rec, const uint32_t* __restrict__ perm, uint32_t n,
+                                   const uint32_t* __restrict__ indChild, uint32_t* __restrict__ offs,
+                                   uint32_t* __restrict__ ind, uint32_t* __restrict__ total,
+                                   uint32_t* __restrict__ err, uint32_t* lds) {
   uint32_t carry = 0;
   for (uint32_t base = 0; base < n; base += kLayoutThreads) {
     const uint32_t i = base + threadIdx.x;
@@ -240,10 +250,28 @@ __global__ void __launch_bounds__(kLayoutThreads) k_inner_layout_small(const uin
   if (threadIdx.x == 0) *total = carry;
 }
 
-__global__ void k_offsets_to_ind(const uint32_t* __restrict__ perm, const uint32_t* __restrict__ offs, uint64_t n,
-                                 uint32_t* __restrict__ ind) {
-  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) ind[perm[i]] = offs[i];
+// The layouts of every level from top[s] up to the root of set s in
+// workgroup s, each level of at most kLayoutSmall nodes: inner_layout_small
+// per level, a level's child addresses read after a barrier.
+struct LayoutTop {
+  const uint32_t* rec[2][kMaxLevels];
+  const uint32_t* perm[2][kMaxLevels];
+  uint32_t* offs[2][kMaxLevels];
+  uint32_t* ind[2][kMaxLevels + 1];
+  uint32_t* total[2][kMaxLevels];
+  uint32_t n[2][kMaxLevels];
+  int32_t top[2];
+  uint32_t* err;
+};
+__global__ void __launch_bounds__(kLayoutThreads) k_inner_layouts_small(LayoutTop a) {
+  __shared__ uint32_t lds[kLayoutThreads / 64 + 1];
+  const int s = blockIdx.x;
+  for (int lev = a.top[s]; lev >= 0; --lev) {
+    inner_layout_small(a.rec[s][lev], a.perm[s][lev], a.n[s][lev], a.ind[s][lev + 1], a.offs[s][lev],
+                       a.ind[s][lev], a.total[s][lev], a.err, lds);
+    __threadfence_block();
+    __syncthreads();
+  }
 }
 
 // Inner levels 0..L-3 of the SSVDAG image in one launch.
@@ -438,6 +466,7 @@ void encode_words_sets(svd_octree* o, const int* fmts, int K) {
 // both (each level is its own std::sort call), one host read returns the
 // layout sizes, and the writes follow.
 void encode_ssvdag_sets(svd_octree* o, const int* fmts, int K) {
+  if (K < 1 || K > 2) throw Error{SVD_ERR_ARG, "SSVDAG-layout encodings: one or two sets"};
   svd_ctx* c = o->ctx;
   const int L = (int)o->levels;
   if (L < 3) throw Error{SVD_ERR_LEVELS, "SSVDAG encoding needs at least 3 levels"};
@@ -497,9 +526,15 @@ void encode_ssvdag_sets(svd_octree* o, const int* fmts, int K) {
   check_launch("k_refcount");
   refsort_ranges(c, permAll, hist, tot, rfirst.data(), rlast.data(), rfirst.size());
   dfree(c, hist);
-  // Phase 1b, bottom-up: children addresses of each level in its order.
+  // Phase 1b, bottom-up: children addresses of each level in its order.  A
+  // set's levels from the first whose level and all above are small go to
+  // one workgroup of one launch at the end.
+  LayoutTop lt{};
+  lt.err = err;
+  lt.top[0] = lt.top[1] = -1;
   for (int lev = L - 2; lev >= 0; --lev) {
     for (int s = 0; s < K; ++s) {
+      if (lt.top[s] >= 0) continue;   // handled by the launch at the end
       const Level& v = (*lvs[s])[lev];
       const uint64_t n = v.n;
       const int q = s * L + lev;
@@ -510,19 +545,47 @@ void encode_ssvdag_sets(svd_octree* o, const int* fmts, int K) {
         continue;
       }
       offs[q] = dalloc<uint32_t>(c, n + 1);
-      if (n && n <= kLayoutSmall) {
-        k_inner_layout_small<<<1, kLayoutThreads, 0, c->stream>>>(v.rec, perm[q], (uint32_t)n, ind[q + 1], offs[q],
-                                                                  ind[q], dev + q, err);
-        check_launch("k_inner_layout_small");
+      uint64_t maxTop = 0;
+      for (int l = lev; l >= 0; --l) maxTop = std::max<uint64_t>(maxTop, (*lvs[s])[l].n);
+      if (maxTop <= kLayoutSmall) {
+        lt.top[s] = lev;
+        for (int l = lev; l >= 0; --l) {
+          const int ql = s * L + l;
+          if (l < lev) {
+            ind[ql] = dalloc<uint32_t>(c, (*lvs[s])[l].n + 1);
+            offs[ql] = dalloc<uint32_t>(c, (*lvs[s])[l].n + 1);
+          }
+          lt.rec[s][l] = (*lvs[s])[l].rec;
+          lt.perm[s][l] = perm[ql];
+          lt.offs[s][l] = offs[ql];
+          lt.ind[s][l] = ind[ql];
+          lt.total[s][l] = dev + ql;
+          lt.n[s][l] = (uint32_t)(*lvs[s])[l].n;
+        }
+        lt.ind[s][lev + 1] = ind[q + 1];
         continue;
       }
-      if (n) {
-        k_inner_sizes<<<div_up(n, kB), kB, 0, c->stream>>>(v.rec, perm[q], n, ind[q + 1], offs[q], err);
-        check_launch("k_inner_sizes");
+      scan_op<1>(c, n, LayoutOp{v.rec, perm[q], ind[q + 1], offs[q], ind[q], err}, dev + q);
+    }
+  }
+  {
+    // sets whose every level was big (never here for a DAG: the root level has one node)
+    int order[2], nb = 0;
+    for (int s = 0; s < K; ++s)
+      if (lt.top[s] >= 0) order[nb++] = s;
+    if (nb) {
+      LayoutTop ltc = lt;
+      for (int k = 0; k < nb; ++k) {
+        const int s = order[k];
+        for (int l = 0; l < kMaxLevels; ++l) {
+          ltc.rec[k][l] = lt.rec[s][l]; ltc.perm[k][l] = lt.perm[s][l]; ltc.offs[k][l] = lt.offs[s][l];
+          ltc.total[k][l] = lt.total[s][l]; ltc.n[k][l] = lt.n[s][l];
+        }
+        for (int l = 0; l <= kMaxLevels; ++l) ltc.ind[k][l] = lt.ind[s][l];
+        ltc.top[k] = lt.top[s];
       }
-      scan_exclusive(c, offs[q], offs[q], n, dev + q);
-      if (n) k_offsets_to_ind<<<div_up(n, kB), kB, 0, c->stream>>>(perm[q], offs[q], n, ind[q]);
-      check_launch("k_offsets_to_ind");
+      k_inner_layouts_small<<<nb, kLayoutThreads, 0, c->stream>>>(ltc);
+      check_launch("k_inner_layouts_small");
     }
   }
   std::vector<uint32_t> h(K * L + 1);

@@ -354,21 +354,25 @@ __global__ void __launch_bounds__(kBT) k_g_swap(const Seg* __restrict__ segs, co
 }
 
 // ---------------------------------------------------------------- per-thread finishers
+// The finishers take their list length from the device (grid-stride), so
+// the host launches them without reading it.
 __global__ void k_leaf_sort(const Seg* __restrict__ segs, const uint32_t* __restrict__ ns, uint32_t* val, uint32_t* idx) {
-  uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= *ns) return;
-  const Seg g = segs[s];
-  GArr a{val + g.first, idx + g.first};
-  insertion_sort(a, (int64_t)(g.last - g.first));
+  const uint32_t m = *ns;
+  for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < m; s += gridDim.x * blockDim.x) {
+    const Seg g = segs[s];
+    GArr a{val + g.first, idx + g.first};
+    insertion_sort(a, (int64_t)(g.last - g.first));
+  }
 }
 
 __global__ void k_heap_sort(const Seg* __restrict__ segs, const uint32_t* __restrict__ ns, uint32_t* val, uint32_t* idx) {
-  uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= *ns) return;
-  const Seg g = segs[s];
-  GArr a{val + g.first, idx + g.first};
-  heap_sort(a, (int64_t)(g.last - g.first));
-  insertion_sort(a, (int64_t)(g.last - g.first));   // no-op on sorted input (kept for exactness)
+  const uint32_t m = *ns;
+  for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < m; s += gridDim.x * blockDim.x) {
+    const Seg g = segs[s];
+    GArr a{val + g.first, idx + g.first};
+    heap_sort(a, (int64_t)(g.last - g.first));
+    insertion_sort(a, (int64_t)(g.last - g.first));   // no-op on sorted input (kept for exactness)
+  }
 }
 
 // ---------------------------------------------------------------- workgroup phase
@@ -432,8 +436,9 @@ __global__ void __launch_bounds__(kBT) k_block_sort(const Seg* __restrict__ segs
   // segOf: < kLeafTag active segment slot, kLeafTag | q leaf q, kEqTag | q
   // equal-value segment q, kDone finished
 
-  if (blockIdx.x >= *nsp) return;
-  const Seg g = segs[blockIdx.x];
+  const uint32_t nSeg = *nsp;
+  for (uint32_t sIdx = blockIdx.x; sIdx < nSeg; sIdx += gridDim.x) {
+  const Seg g = segs[sIdx];
   const uint32_t n = g.last - g.first;
   const int tid = threadIdx.x, wv = tid >> 6;
   const uint32_t v0 = val[g.first];
@@ -447,7 +452,8 @@ __global__ void __launch_bounds__(kBT) k_block_sort(const Seg* __restrict__ segs
   if (!__syncthreads_or(differs) && eq_depth_ok(n, g.depth)) {
     // all values equal: only the indices move, each straight to its place
     for (uint32_t k = tid; k < n; k += kBT) idx[g.first + eq_dest(k, n)] = si[k];
-    return;
+    __syncthreads();   // the next segment reuses the LDS arrays
+    continue;
   }
   if (tid == 0) {
     nActive[0] = 0;
@@ -635,6 +641,8 @@ __global__ void __launch_bounds__(kBT) k_block_sort(const Seg* __restrict__ segs
     if (rp[k] != kDone) { sv[rp[k]] = rv[k]; si[rp[k]] = ri[k]; }
   __syncthreads();
   for (uint32_t k = tid; k < n; k += kBT) { val[g.first + k] = sv[k]; idx[g.first + k] = si[k]; }
+  __syncthreads();   // the next segment reuses the LDS arrays
+  }
 }
 
 // The initial segment lists and their counters, written from the kernel
@@ -746,12 +754,10 @@ void refsort_ranges(svd_ctx* c, uint32_t* idx, uint32_t* val, uint64_t n, const 
     }
     dfree(c, Lpos); dfree(c, Rpos); dfree(c, st); dfree(c, blk0); dfree(c, cntL); dfree(c, cntR);
   }
-  // finishers: the lists only grow now; read their sizes once
-  uint32_t h[4];
-  read_u32s(c, counters, h, 4);
-  if (h[1]) k_block_sort<<<h[1], kBT, 0, c->stream>>>(L.small, L.nsmall, val, idx);
-  if (h[2]) k_heap_sort<<<div_up(h[2], 64), 64, 0, c->stream>>>(L.heap, L.nheap, val, idx);
-  if (h[3]) k_leaf_sort<<<div_up(h[3], 64), 64, 0, c->stream>>>(L.leaf, L.nleaf, val, idx);
+  // finishers, sized without reading the list lengths (grid-stride)
+  k_block_sort<<<(uint32_t)std::max(c->numCU, 1) * 3, kBT, 0, c->stream>>>(L.small, L.nsmall, val, idx);
+  k_heap_sort<<<64, 64, 0, c->stream>>>(L.heap, L.nheap, val, idx);
+  k_leaf_sort<<<512, 64, 0, c->stream>>>(L.leaf, L.nleaf, val, idx);
   check_launch("refsort finish");
   dfree(c, bigA); dfree(c, bigB); dfree(c, L.small); dfree(c, L.heap); dfree(c, L.leaf); dfree(c, counters);
 }
