@@ -405,13 +405,15 @@ constexpr TEntry kEmpty = ~0ull;
 // slot ends holding the smallest index of all nodes with an equal key (equal
 // keys share the fingerprint).  eq(j) compares record j with w.  Returns the
 // slot.
-template <typename Eq>
+// The device table goes straight to the CAS (it returns the entry it meets,
+// one round trip); the LDS table first loads.
+template <bool CasFirst, typename Eq>
 __device__ __forceinline__ uint32_t insert_key(TEntry* table, uint32_t cmask, uint64_t hv, uint32_t i, Eq eq) {
   const uint32_t fp = (uint32_t)(hv >> 32);
   const TEntry mine = ((TEntry)fp << 32) | (TEntry)i;
   uint32_t h = (uint32_t)hv & cmask;
   while (true) {
-    TEntry e = __atomic_load_n(&table[h], __ATOMIC_RELAXED);
+    TEntry e = CasFirst ? kEmpty : __atomic_load_n(&table[h], __ATOMIC_RELAXED);
     if (e == kEmpty) {
       const TEntry old = atomicCAS(&table[h], kEmpty, mine);
       if (old == kEmpty) return h;
@@ -481,7 +483,7 @@ __global__ void __launch_bounds__(kInsT) k_insert(const uint32_t* __restrict__ r
       for (int q = 0; q < kRecWords; ++q) diff |= r[q] ^ wk[q];
       return diff == 0;
     };
-    ls[loc] = (uint16_t)insert_key(lt, kCap - 1, hv[k], (uint32_t)(base + loc), eqLds);
+    ls[loc] = (uint16_t)insert_key<false>(lt, kCap - 1, hv[k], (uint32_t)(base + loc), eqLds);
   }
   __syncthreads();
 #pragma unroll
@@ -491,7 +493,7 @@ __global__ void __launch_bounds__(kInsT) k_insert(const uint32_t* __restrict__ r
     if (!live[k] || (uint32_t)lt[l] != (uint32_t)(base + loc)) continue;
     const uint32_t* wk = w[k];
     auto eqDev = [&](uint32_t j) { return rec_eq(rec, j, wk); };
-    gs[l] = insert_key(table, cmask, hv[k], (uint32_t)(base + loc), eqDev);
+    gs[l] = insert_key<true>(table, cmask, hv[k], (uint32_t)(base + loc), eqDev);
   }
   __syncthreads();
 #pragma unroll

@@ -330,26 +330,26 @@ __global__ void __launch_bounds__(kBT) k_g_scatter(const Seg* __restrict__ segs,
 }
 
 // Swap the k-th left stop with the k-th right stop while L_k < R_k; record K.
+// kTile / kBT workgroups per tile, one pair per thread.
 __global__ void __launch_bounds__(kBT) k_g_swap(const Seg* __restrict__ segs, const uint32_t* __restrict__ nsp,
                                                 const uint32_t* __restrict__ blk0, SegState* __restrict__ st,
                                                 uint32_t* val, uint32_t* idx, const uint32_t* __restrict__ Lpos,
                                                 const uint32_t* __restrict__ Rpos) {
-  const int s = find_seg(blk0, *nsp, blockIdx.x);
+  constexpr uint32_t kSub = kTile / kBT;
+  const uint32_t tile = blockIdx.x / kSub;
+  const int s = find_seg(blk0, *nsp, tile);
   if (s < 0) return;
   const Seg g = segs[s];
   const uint32_t m = min(st[s].nL, st[s].nR);
-  const uint64_t kb = (uint64_t)(blockIdx.x - blk0[s]) * kTile;
-  for (int j = 0; j < kTile / kBT; ++j) {
-    const uint64_t k = kb + (uint64_t)j * kBT + threadIdx.x;
-    if (k >= m) break;
-    const uint32_t l = Lpos[g.first + k], r = Rpos[g.first + k];
-    if (l < r) {
-      const uint32_t tv = val[l], ti = idx[l];
-      val[l] = val[r]; idx[l] = idx[r];
-      val[r] = tv; idx[r] = ti;
-      const bool nextOk = (k + 1 < m) && (Lpos[g.first + k + 1] < Rpos[g.first + k + 1]);
-      if (!nextOk) st[s].K = (uint32_t)(k + 1);
-    }
+  const uint64_t k = (uint64_t)(tile - blk0[s]) * kTile + (uint64_t)(blockIdx.x % kSub) * kBT + threadIdx.x;
+  if (k >= m) return;
+  const uint32_t l = Lpos[g.first + k], r = Rpos[g.first + k];
+  if (l < r) {
+    const uint32_t tv = val[l], ti = idx[l];
+    val[l] = val[r]; idx[l] = idx[r];
+    val[r] = tv; idx[r] = ti;
+    const bool nextOk = (k + 1 < m) && (Lpos[g.first + k + 1] < Rpos[g.first + k + 1]);
+    if (!nextOk) st[s].K = (uint32_t)(k + 1);
   }
 }
 
@@ -741,7 +741,7 @@ void refsort_ranges(svd_ctx* c, uint32_t* idx, uint32_t* val, uint64_t n, const 
       for (int r = 0; r < batch; ++r) {
         k_g_count<<<G, kBT, 0, c->stream>>>(cur, ncur, blk0, st, val, cntL, cntR);
         k_g_scatter<<<G, kBT, 0, c->stream>>>(cur, ncur, blk0, st, val, cntL, cntR, Lpos, Rpos);
-        k_g_swap<<<G, kBT, 0, c->stream>>>(cur, ncur, blk0, st, val, idx, Lpos, Rpos);
+        k_g_swap<<<G * (kTile / kBT), kBT, 0, c->stream>>>(cur, ncur, blk0, st, val, idx, Lpos, Rpos);
         L.big = nxt;
         L.nbig = nnxt;
         k_g_plan<<<1, kPlanT, 0, c->stream>>>(cur, ncur, Lpos, Rpos, L, val, idx, st, blk0);
