@@ -91,6 +91,18 @@ int svd_octree_from_svdag(svd_ctx* ctx, const void* file, uint64_t bytes, int fi
  * on the device.                                                             */
 int svd_octree_from_mesh(svd_ctx* ctx, const float* tris, uint64_t ntris, uint32_t levels, const double bbox[6],
                          int on_device, svd_octree** out);
+/* svd_octree_from_points: GeomOctree::buildSVOFromPoints
+ * (geom_octree.cpp:62-169) on the device for npts points (x y z doubles each,
+ * in the order the reference would read them from its LAS file) inside
+ * bbox = min xyz, max xyz (double).  Every point walks the reference's DFS:
+ * child centres are the parent's plus or minus getHalfSideD in FP64 and a
+ * point is in a child box when min <= p < max on all three axes, so points on
+ * rounding-overlapped boundaries reach every box that holds them, as there.
+ * Node order per level is the order of creation; cleanEmptyNodes follows.
+ * The reference reads the points with libLAS; the point array replaces the
+ * file.  xyz on the host (on_device = 0) or on the device.                   */
+int svd_octree_from_points(svd_ctx* ctx, const double* xyz, uint64_t npts, uint32_t levels, const double bbox[6],
+                           int on_device, svd_octree** out);
 /* svd_octree_from_mesh_subtrees: the SVO that GeomOctree::buildDAG
  * (geom_octree.cpp:289-435) reduces: a top SVO of step_level + 1 levels, then
  * one subtree per set voxel of its finest level, each voxelized from all

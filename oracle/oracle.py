@@ -61,6 +61,7 @@ def lib():
             "orc_new": (vp, []), "orc_free": (None, [vp]),
             "orc_build_morton": (i32, [vp, vp, u64, u32]),
             "orc_build_mesh": (i32, [vp, vp, u64, u32, vp]),
+            "orc_build_points": (i32, [vp, vp, u64, u32, vp]),
             "orc_build_dag_mesh": (i32, [vp, vp, u64, u32, u32, vp]),
             "orc_to_dag": (i32, [vp]), "orc_to_sdag": (i32, [vp]),
             "orc_tri_box": (i32, [vp, C.c_double, vp]),
@@ -134,6 +135,16 @@ class OracleOctree:
         t = np.ascontiguousarray(tris, dtype=np.float32)
         bb = np.array(list(bbox_min) + list(bbox_max), dtype=np.float64)
         rc = o.L.orc_build_mesh(o.h, _p(t), t.shape[0], levels, _p(bb))
+        assert rc == 0, rc
+        return o
+
+    @classmethod
+    def from_points(cls, xyz: np.ndarray, levels: int, bbox_min, bbox_max):
+        """buildSVOFromPoints (geom_octree.cpp:62-169): points float64 [N, 3] in file order."""
+        o = cls()
+        pt = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+        bb = np.array(list(bbox_min) + list(bbox_max), dtype=np.float64)
+        rc = o.L.orc_build_points(o.h, _p(pt), pt.shape[0], levels, _p(bb))
         assert rc == 0, rc
         return o
 

@@ -266,6 +266,59 @@ static void buildSVOMesh(Octree& o, const float* tris, size_t ntris, unsigned le
   finishSVO(o);
 }
 
+// buildSVOFromPoints (geom_octree.cpp:62-169) with the points given in
+// memory (the reference reads them with libLAS): point-major DFS with an
+// explicit stack, the child centres of buildSVO, a point in child i when
+// c - k <= p < c + k on every axis (:131-133), children created in the
+// 7..0 loop, then cleanEmptyNodes (:155).
+static void buildSVOPoints(Octree& o, const double* xyz, size_t npts, unsigned levels, const double bmin[3],
+                           const double bmax[3]) {
+  for (int i = 0; i < 3; ++i) { o.bboxMin[i] = (float)bmin[i]; o.bboxMax[i] = (float)bmax[i]; }
+  o.levels = levels;
+  float sides[3];
+  for (int i = 0; i < 3; ++i) sides[i] = ((o.bboxMax[i] - o.bboxMin[i]) * 0.5f) * 2.0f;   // half_side_lengths()*2
+  o.rootSide = std::max(std::max(sides[0], sides[1]), sides[2]);
+  o.data.assign(levels, {});
+  o.data[0].push_back(Node());
+  o.nNodes = 0;
+  double center[3];
+  for (int i = 0; i < 3; ++i) center[i] = (bmin[i] + bmax[i]) * 0.5;
+  struct QI { id_t id; uint8_t level; double c[3]; };
+  std::stack<QI> st;
+  for (size_t t = 0; t < npts; ++t) {
+    const double* p = xyz + 3 * t;
+    st.push(QI{0, 0, {center[0], center[1], center[2]}});
+    while (!st.empty()) {
+      QI qi = st.top(); st.pop();
+      const double k = o.rootSide / double(1u << (qi.level + 2));   // getHalfSideD(level+1)
+      double cc[8][3];
+      for (int c = 0; c < 8; ++c) {
+        cc[c][0] = qi.c[0] + ((c & 4) ? +k : -k);
+        cc[c][1] = qi.c[1] + ((c & 2) ? +k : -k);
+        cc[c][2] = qi.c[2] + ((c & 1) ? +k : -k);
+      }
+      for (int i = 7; i >= 0; --i) {
+        if ((p[0] >= cc[i][0] - k && p[0] < cc[i][0] + k) && (p[1] >= cc[i][1] - k && p[1] < cc[i][1] + k) &&
+            (p[2] >= cc[i][2] - k && p[2] < cc[i][2] + k)) {
+          Node& node = o.data[qi.level][qi.id];
+          node.mask |= (uint8_t)(1u << i);
+          if (!node.existsChildPointer(i) && (qi.level < (levels - 1))) {
+            node.children[i] = (id_t)o.data[qi.level + 1].size();
+            o.data[qi.level + 1].emplace_back();
+            o.nNodes++;
+          }
+          if ((qi.level + 1u) < levels) {
+            const Node& nn = o.data[qi.level][qi.id];
+            st.push(QI{nn.children[i], (uint8_t)(qi.level + 1), {cc[i][0], cc[i][1], cc[i][2]}});
+          }
+        }
+      }
+    }
+  }
+  cleanEmptyNodes(o);
+  finishSVO(o);
+}
+
 static int toDAG(Octree& o);
 
 // GeomOctree::buildDAG (geom_octree.cpp:289-435): a top SVO of stepLevel+1
@@ -1322,6 +1375,12 @@ int orc_build_morton(void* h, const uint64_t* codes, uint64_t n, uint32_t levels
 int orc_build_mesh(void* h, const float* tris, uint64_t ntris, uint32_t levels, const double bbox[6]) {
   if (levels < 1 || levels > 21) return -2;
   buildSVOMesh(*(Octree*)h, tris, (size_t)ntris, levels, bbox, bbox + 3);
+  return 0;
+}
+
+int orc_build_points(void* h, const double* xyz, uint64_t npts, uint32_t levels, const double bbox[6]) {
+  if (levels < 1 || levels > 21) return -2;
+  buildSVOPoints(*(Octree*)h, xyz, (size_t)npts, levels, bbox, bbox + 3);
   return 0;
 }
 

@@ -130,6 +130,7 @@ def lib():
         "svd_octree_from_svdag": (i32, [vp, vp, u64, i32, P(vp)]),
         "svd_octree_from_mesh": (i32, [vp, vp, u64, u32, P(C.c_double), i32, P(vp)]),
         "svd_octree_from_mesh_subtrees": (i32, [vp, vp, u64, u32, u32, P(C.c_double), i32, P(vp)]),
+        "svd_octree_from_points": (i32, [vp, vp, u64, u32, P(C.c_double), i32, P(vp)]),
         "svd_octree_free": (i32, [vp]),
         "svd_octree_from_levels": (i32, [vp, u32, P(u64), P(vp), P(C.c_float), C.c_double, u64, u64, P(vp)]),
         "svd_octree_copy": (i32, [vp, P(vp)]),
@@ -200,6 +201,7 @@ def lib():
 
 EXPORTED = ["svd_ctx_create", "svd_ctx_destroy", "svd_last_error", "svd_ctx_synchronize", "svd_ctx_stream",
             "svd_octree_from_morton", "svd_octree_from_svo", "svd_octree_from_svdag", "svd_octree_from_mesh", "svd_octree_from_mesh_subtrees",
+            "svd_octree_from_points",
             "svd_octree_free", "svd_octree_from_levels", "svd_octree_copy", "svd_reduce_sym_pointers",
             "svd_reduce_dag",
             "svd_reduce_sym", "svd_cross_level_merge", "svd_octree_state", "svd_octree_levels",
@@ -529,6 +531,21 @@ class Octree:
             t = np.ascontiguousarray(tris, dtype=np.float32).reshape(-1, 9)
             ctx.check(ctx.L.svd_octree_from_mesh(ctx.h, t.ctypes.data_as(C.c_void_p), t.shape[0], levels, bb, 0,
                                                  C.byref(h)))
+        return cls(ctx, h)
+
+    @classmethod
+    def from_points(cls, ctx: Context, points, levels: int, bbox_min, bbox_max):
+        """buildSVOFromPoints on the device (geom_octree.cpp:62-169): points float64 [N, 3]
+        (host array, the file's order) or (device_ptr, N)."""
+        h = C.c_void_p()
+        bb = (C.c_double * 6)(*[float(x) for x in list(bbox_min) + list(bbox_max)])
+        if isinstance(points, tuple):
+            ptr, n = points
+            ctx.check(ctx.L.svd_octree_from_points(ctx.h, C.c_void_p(ptr), n, levels, bb, 1, C.byref(h)))
+        else:
+            pt = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, 3)
+            ctx.check(ctx.L.svd_octree_from_points(ctx.h, pt.ctypes.data_as(C.c_void_p), pt.shape[0], levels, bb, 0,
+                                                   C.byref(h)))
         return cls(ctx, h)
 
     @classmethod

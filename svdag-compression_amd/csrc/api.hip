@@ -194,7 +194,7 @@ int svd_octree_from_morton(svd_ctx* c, const uint64_t* codes, uint64_t n, uint32
 }
 
 namespace {
-int mesh_octree(svd_ctx* c, const float* tris, uint64_t ntris, uint32_t levels, int stepLevel, const double bbox[6],
+int mesh_octree(svd_ctx* c, const void* prims, uint64_t ntris, uint32_t levels, int stepLevel, const double bbox[6],
                 int on_device, svd_octree** out);
 }
 
@@ -209,12 +209,19 @@ int svd_octree_from_mesh_subtrees(svd_ctx* c, const float* tris, uint64_t ntris,
   return mesh_octree(c, tris, ntris, levels, (int)step_level, bbox, on_device, out);
 }
 
+int svd_octree_from_points(svd_ctx* c, const double* xyz, uint64_t npts, uint32_t levels, const double bbox[6],
+                           int on_device, svd_octree** out) {
+  return mesh_octree(c, xyz, npts, levels, -2, bbox, on_device, out);
+}
+
 }  // extern "C"
 
 namespace {
-int mesh_octree(svd_ctx* c, const float* tris, uint64_t ntris, uint32_t levels, int stepLevel, const double bbox[6],
+// stepLevel: -1 buildSVO, -2 buildSVOFromPoints (prims = points, 3 doubles
+// each), else buildDAG's joined SVO with that step level.
+int mesh_octree(svd_ctx* c, const void* prims, uint64_t ntris, uint32_t levels, int stepLevel, const double bbox[6],
                 int on_device, svd_octree** out) {
-  if (!c || !out || !bbox || (ntris && !tris)) return SVD_ERR_ARG;
+  if (!c || !out || !bbox || (ntris && !prims)) return SVD_ERR_ARG;
   if (levels < 2 || levels > kMaxLevels) return SVD_ERR_LEVELS;
   *out = nullptr;
   svd_octree* o = new svd_octree();
@@ -222,15 +229,17 @@ int mesh_octree(svd_ctx* c, const float* tris, uint64_t ntris, uint32_t levels, 
   o->levels = levels;
   int rc = guarded(c, [&] {
     Timer t(c, "build_svo");
-    const float* d = tris;
-    float* tmp = nullptr;
+    const uint64_t primBytes = stepLevel == -2 ? 24 : 36;
+    const void* d = prims;
+    uint8_t* tmp = nullptr;
     if (!on_device && ntris) {
-      tmp = dalloc<float>(c, 9 * ntris);
-      SVD_HIP(hipMemcpyAsync(tmp, tris, 36 * ntris, hipMemcpyHostToDevice, c->stream));
+      tmp = dalloc<uint8_t>(c, primBytes * ntris);
+      SVD_HIP(hipMemcpyAsync(tmp, prims, primBytes * ntris, hipMemcpyHostToDevice, c->stream));
       d = tmp;
     }
-    if (stepLevel < 0) build_svo_from_mesh(o, d, ntris, bbox, bbox + 3);
-    else build_svo_from_mesh_subtrees(o, d, ntris, bbox, bbox + 3, (uint32_t)stepLevel);
+    if (stepLevel == -2) build_svo_from_points(o, (const double*)d, ntris, bbox, bbox + 3);
+    else if (stepLevel < 0) build_svo_from_mesh(o, (const float*)d, ntris, bbox, bbox + 3);
+    else build_svo_from_mesh_subtrees(o, (const float*)d, ntris, bbox, bbox + 3, (uint32_t)stepLevel);
     dfree(c, tmp);
     // nVoxels: set bits of the leaf level
     const Level& leaf = o->svo[levels - 1];

@@ -183,6 +183,8 @@ void fill_u8(svd_ctx* c, uint8_t* d, uint8_t v, uint64_t n);
 void build_svo_from_morton(svd_octree* o, const uint64_t* dcodes, uint64_t n);
 // buildSVO of a triangle soup (voxelize.hip); dtris = 9 floats per triangle on the device
 void build_svo_from_mesh(svd_octree* o, const float* dtris, uint64_t ntris, const double bmin[3], const double bmax[3]);
+// buildSVOFromPoints of an in-memory point array (voxelize.hip); dxyz = 3 doubles per point on the device
+void build_svo_from_points(svd_octree* o, const double* dxyz, uint64_t npts, const double bmin[3], const double bmax[3]);
 // buildDAG's joined SVO: top of stepLevel+1 levels + one voxelized subtree per set voxel of its finest level
 void build_svo_from_mesh_subtrees(svd_octree* o, const float* dtris, uint64_t ntris, const double bmin[3],
                                   const double bmax[3], uint32_t stepLevel);

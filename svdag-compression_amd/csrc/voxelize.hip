@@ -1,6 +1,8 @@
 // Mesh voxelization into the reference's SVO: GeomOctree::buildSVO
 // (geom_octree.cpp:171-280) + cleanEmptyNodes (:437-456), with the FP64
-// triangle/box overlap test of test_triangle_box.cpp:105-184.
+// triangle/box overlap test of test_triangle_box.cpp:105-184; and the point
+// form, buildSVOFromPoints (:62-169), whose DFS is the same with a point
+// inside a half-open child box in place of the overlap test.
 //
 // The reference walks the triangles in order; for each it runs a DFS from the
 // root with an explicit stack.  Popping node p tests p's 8 child boxes, slot 7
@@ -102,20 +104,36 @@ struct NodeGeo { double* cx; double* cy; double* cz; uint64_t* path; uint32_t* r
 // 1. every (triangle, node) pair tests the node's 8 child boxes; hit bits go
 //    to the node's mask; the number of hits is the pair's visit count
 // k = getHalfSideD(level + 1) of the node's root: rootSide / 2^(level + 2)
-__global__ void k_vox_test(const float* __restrict__ tris, const uint32_t* __restrict__ pt,
-                           const uint32_t* __restrict__ pn, uint64_t np, NodeGeo g, const double* __restrict__ side,
-                           double scale, uint32_t* __restrict__ mask, uint32_t* __restrict__ cnt,
-                           uint8_t* __restrict__ hits) {
+// The primitives: a triangle overlaps the child box (test_triangle_box.cpp,
+// buildSVO), or a point lies in it, half-open on every axis
+// (childrenCenters[i] -/+ k, buildSVOFromPoints, geom_octree.cpp:131-133).
+struct TriPrim {
+  const float* __restrict__ tris;
+  __device__ bool hit(uint32_t t, const double cc[3], double k) const { return tri_box(cc, k, tris + 9ull * t); }
+};
+struct PointPrim {
+  const double* __restrict__ xyz;
+  __device__ bool hit(uint32_t t, const double cc[3], double k) const {
+    const double* q = xyz + 3ull * t;
+    return (q[0] >= cc[0] - k && q[0] < cc[0] + k) && (q[1] >= cc[1] - k && q[1] < cc[1] + k) &&
+           (q[2] >= cc[2] - k && q[2] < cc[2] + k);
+  }
+};
+
+template <typename Prim>
+__global__ void k_vox_test(Prim prim, const uint32_t* __restrict__ pt, const uint32_t* __restrict__ pn, uint64_t np,
+                           NodeGeo g, const double* __restrict__ side, double scale, uint32_t* __restrict__ mask,
+                           uint32_t* __restrict__ cnt, uint8_t* __restrict__ hits) {
   const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= np) return;
-  const float* tri = tris + 9ull * pt[j];
+  const uint32_t t = pt[j];
   const uint32_t p = pn[j];
   const double k = side[g.root[p]] / scale;
   const double c0 = g.cx[p], c1 = g.cy[p], c2 = g.cz[p];
   uint32_t m = 0;
   for (int i = 7; i >= 0; --i) {
     const double cc[3] = {c0 + ((i & 4) ? +k : -k), c1 + ((i & 2) ? +k : -k), c2 + ((i & 1) ? +k : -k)};
-    if (tri_box(cc, k, tri)) m |= 1u << i;
+    if (prim.hit(t, cc, k)) m |= 1u << i;
   }
   if (m) atomicOr(&mask[p], m);
   if (cnt) { cnt[j] = (uint32_t)__popc(m); hits[j] = (uint8_t)m; }
@@ -309,7 +327,8 @@ VoxRoot root_of_box(const double bmin[3], const double bmax[3]) {
 // order, child pointers into the joined next level), cleanEmptyNodes applied
 // inside every SVO.  Returns the nodes created below the roots; lastCenters
 // (optional) receives the centres of the finest level's nodes.
-uint64_t voxelize_roots(svd_ctx* c, const float* dtris, uint64_t ntris, uint32_t L, const std::vector<VoxRoot>& roots,
+template <typename Prim>
+uint64_t voxelize_roots(svd_ctx* c, Prim prim, uint64_t ntris, uint32_t L, const std::vector<VoxRoot>& roots,
                         std::vector<Level>& out, std::vector<double>* lastCenters) {
   const uint64_t R = roots.size();
   if (ntris >= 0xFFFFFFFFull || R == 0 || ntris * R >= 0xFFFFFFF0ull)
@@ -347,7 +366,7 @@ uint64_t voxelize_roots(svd_ctx* c, const float* dtris, uint64_t ntris, uint32_t
     lv.leaf = (l == L - 1);
     if (l > 0) nNodes += n;
     if (lv.leaf) {   // voxel bits of the leaf-level nodes
-      if (np) k_vox_test<<<div_up(np, kB), kB, 0, c->stream>>>(dtris, pt, pn, np, geo, side, scale, mask, nullptr,
+      if (np) k_vox_test<<<div_up(np, kB), kB, 0, c->stream>>>(prim, pt, pn, np, geo, side, scale, mask, nullptr,
                                                                  nullptr);
       check_launch("k_vox_test");
       lv.rec = mask;
@@ -355,7 +374,7 @@ uint64_t voxelize_roots(svd_ctx* c, const float* dtris, uint64_t ntris, uint32_t
     }
     uint32_t* cnt = dalloc<uint32_t>(c, np + 1);
     uint8_t* hits = dalloc<uint8_t>(c, np + 1);
-    if (np) k_vox_test<<<div_up(np, kB), kB, 0, c->stream>>>(dtris, pt, pn, np, geo, side, scale, mask, cnt, hits);
+    if (np) k_vox_test<<<div_up(np, kB), kB, 0, c->stream>>>(prim, pt, pn, np, geo, side, scale, mask, cnt, hits);
     check_launch("k_vox_test");
     scan_exclusive(c, cnt, cnt, np, total);
     const uint64_t nv = read_u32(c, total);
@@ -449,7 +468,19 @@ void build_svo_from_mesh(svd_octree* o, const float* dtris, uint64_t ntris, cons
   const VoxRoot root = root_of_box(bmin, bmax);
   for (int i = 0; i < 6; ++i) o->bbox[i] = root.bb[i];
   o->rootSide = root.side;
-  o->nNodesSVO = voxelize_roots(o->ctx, dtris, ntris, o->levels, {root}, o->svo, nullptr);
+  o->nNodesSVO = voxelize_roots(o->ctx, TriPrim{dtris}, ntris, o->levels, {root}, o->svo, nullptr);
+}
+
+// GeomOctree::buildSVOFromPoints (geom_octree.cpp:62-169) for points given in
+// memory (x y z doubles, the file's order) instead of read from a LAS file:
+// the same DFS per point, child boxes half-open, the root box and the child
+// centres as buildSVO's, creation order and cleanEmptyNodes as there.
+void build_svo_from_points(svd_octree* o, const double* dxyz, uint64_t npts, const double bmin[3],
+                           const double bmax[3]) {
+  const VoxRoot root = root_of_box(bmin, bmax);
+  for (int i = 0; i < 6; ++i) o->bbox[i] = root.bb[i];
+  o->rootSide = root.side;
+  o->nNodesSVO = voxelize_roots(o->ctx, PointPrim{dxyz}, npts, o->levels, {root}, o->svo, nullptr);
 }
 
 // GeomOctree::buildDAG's SVO (geom_octree.cpp:289-435): a top SVO of
@@ -469,7 +500,7 @@ void build_svo_from_mesh_subtrees(svd_octree* o, const float* dtris, uint64_t nt
   o->rootSide = top.side;
   std::vector<Level> T;
   std::vector<double> centers;
-  const uint64_t topNodes = voxelize_roots(c, dtris, ntris, stepLevels, {top}, T, &centers);
+  const uint64_t topNodes = voxelize_roots(c, TriPrim{dtris}, ntris, stepLevels, {top}, T, &centers);
   Level& fin = T[stepLevels - 1];
   std::vector<uint32_t> masks(fin.n);
   if (fin.n) SVD_HIP(hipMemcpyAsync(masks.data(), fin.rec, 4 * fin.n, hipMemcpyDeviceToHost, c->stream));
@@ -490,7 +521,7 @@ void build_svo_from_mesh_subtrees(svd_octree* o, const float* dtris, uint64_t nt
     }
   std::vector<Level> sub;
   uint64_t subNodes = 0;
-  if (!roots.empty()) subNodes = voxelize_roots(c, dtris, ntris, L - stepLevels, roots, sub, nullptr);
+  if (!roots.empty()) subNodes = voxelize_roots(c, TriPrim{dtris}, ntris, L - stepLevels, roots, sub, nullptr);
   // join: the top's finest level points at the subtree roots
   uint32_t* joined = dalloc<uint32_t>(c, fin.n * kRecWords + 1);
   uint32_t* base = dalloc<uint32_t>(c, fin.n + 1);

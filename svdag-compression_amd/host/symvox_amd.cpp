@@ -379,6 +379,71 @@ void GeomOctree::buildSVO(unsigned int levels, aabox3d bbox, bool internalCall) 
   if (!internalCall) printf("OK! [%.3f s]\n", _stats.buildSVOTime);
 }
 
+void GeomOctree::buildSVOFromPoints(const double* xyz, size_t n, unsigned int levels, aabox3d bbox,
+                                    bool internalCall) {
+  if (!internalCall) printf("* Building SVO... ");
+  fflush(stdout);
+  const double t0 = now_s();
+  if (_oct) { svd_octree_free(_oct); _oct = nullptr; }
+  const double bb[6] = {bbox[0][0], bbox[0][1], bbox[0][2], bbox[1][0], bbox[1][1], bbox[1][2]};
+  check(svd_octree_from_points(default_context(), xyz, n, levels, bb, 0, &_oct), "buildSVOFromPoints");
+  _bbox = aabox3f(point3f((float)bb[0], (float)bb[1], (float)bb[2]), point3f((float)bb[3], (float)bb[4], (float)bb[5]));
+  const point3f sides = _bbox.half_side_lengths();
+  _rootSide = std::max(std::max(sides[0] * 2.0f, sides[1] * 2.0f), sides[2] * 2.0f);
+  _levels = levels;
+  _stats.buildSVOTime = now_s() - t0;
+  uint64_t nn = 0, nv = 0, nl = 0;
+  svd_octree_counts(_oct, &nn, &nv);
+  svd_level_size(_oct, SVD_VIEW_SVO, levels - 1, &nl);
+  _stats.nNodesSVO = nn;
+  _stats.nNodesLastLevSVO = nl;
+  _stats.simulatedEncodedSVOSize = (_stats.nNodesSVO - _stats.nNodesLastLevSVO) * 4;
+  _stats.nTotalVoxels = nv;
+  _state = S_SVO;
+  _cacheState = -1;
+  if (!internalCall) printf("OK! [%.3f s]\n", _stats.buildSVOTime);
+}
+
+void GeomOctree::buildSVOFromPoints(std::string fileName, unsigned int levels, aabox3d bbox, bool internalCall) {
+  std::vector<double> xyz;
+  if (!readLasPoints(fileName, xyz)) throw std::runtime_error("buildSVOFromPoints: cannot read LAS file " + fileName);
+  buildSVOFromPoints(xyz.data(), xyz.size() / 3, levels, bbox, internalCall);
+}
+
+bool readLasPoints(const std::string& fileName, std::vector<double>& xyz) {
+  std::ifstream f(fileName, std::ios::binary);
+  if (!f) return false;
+  uint8_t h[375] = {0};
+  f.read((char*)h, sizeof(h));
+  const std::streamsize got = f.gcount();
+  if (got < 227 || memcmp(h, "LASF", 4) != 0) return false;
+  auto u16 = [&](int o) { uint16_t v; memcpy(&v, h + o, 2); return v; };
+  auto u32 = [&](int o) { uint32_t v; memcpy(&v, h + o, 4); return v; };
+  auto u64 = [&](int o) { uint64_t v; memcpy(&v, h + o, 8); return v; };
+  auto f64 = [&](int o) { double v; memcpy(&v, h + o, 8); return v; };
+  const uint8_t verMinor = h[25];
+  const uint32_t dataOffset = u32(96);
+  const uint16_t recLen = u16(105);
+  uint64_t count = u32(107);
+  if (count == 0 && verMinor >= 4 && got >= 255) count = u64(247);   // LAS 1.4: 64-bit count
+  const double scale[3] = {f64(131), f64(139), f64(147)}, offset[3] = {f64(155), f64(163), f64(171)};
+  if (recLen < 12) return false;
+  f.clear();
+  f.seekg(dataOffset, std::ios::beg);
+  xyz.assign(3 * count, 0.0);
+  std::vector<uint8_t> rec(recLen);
+  for (uint64_t i = 0; i < count; ++i) {
+    f.read((char*)rec.data(), recLen);
+    if (f.gcount() != recLen) return false;
+    for (int a = 0; a < 3; ++a) {
+      int32_t raw;
+      memcpy(&raw, rec.data() + 4 * a, 4);
+      xyz[3 * i + a] = (raw * scale[a]) + offset[a];
+    }
+  }
+  return true;
+}
+
 // geom_octree.cpp:289-435 on the device (svd_octree_from_mesh_subtrees): the
 // top SVO, one subtree per set voxel of its finest level voxelized in its own
 // box, joined in the reference's order; one GPU reduction of the joined

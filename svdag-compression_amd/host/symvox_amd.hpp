@@ -154,6 +154,13 @@ class GeomOctree {
 
   // geom_octree.cpp:171-280: voxelization on the device in the reference's node order
   void buildSVO(unsigned int levels, svd::aabox3d bbox, bool internalCall = false);
+  // geom_octree.cpp:62-169 on the device (svd_octree_from_points): the points
+  // of a LAS file, read by readLasPoints in place of libLAS.  leavesCenters
+  // (the reference's buildDAG hook, unused by its point path) is not offered.
+  void buildSVOFromPoints(std::string fileName, unsigned int levels, svd::aabox3d bbox, bool internalCall = false);
+  // the same from points in memory: xyz = n points, x y z doubles each
+  void buildSVOFromPoints(const double* xyz, size_t n, unsigned int levels, svd::aabox3d bbox,
+                          bool internalCall = false);
   // geom_octree.cpp:289-435: the top SVO and every subtree (each in its own
   // box) voxelized on the device, joined in the reference's order and reduced.
   void buildDAG(unsigned int levels, unsigned int stepLevel, svd::aabox3d bbox, bool verbose = false);
@@ -203,6 +210,12 @@ class GeomOctree {
 // ----------------------------------------------------------------------------
 // Encoded octrees (src/symvox/encoded_octree.hpp:28-98 and the three encoders)
 // ----------------------------------------------------------------------------
+// The point records of a LAS file (public header block of LAS 1.0-1.4, any
+// point data format; the first 12 bytes of a record are the raw X Y Z int32s)
+// as libLAS's Point::GetX/GetY/GetZ return them: raw * scale + offset, in
+// file order.  false when the file is not a readable LAS file.
+bool readLasPoints(const std::string& fileName, std::vector<double>& xyz);
+
 class EncodedOctree {
  public:
   virtual ~EncodedOctree() {}

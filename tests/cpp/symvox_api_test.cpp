@@ -73,7 +73,31 @@ void traverse(const EncodedOctree& e, const std::string& path, const char* name)
 
 }  // namespace
 
+// symvox_api_test --las <file.las> <levels> <outdir> x0 y0 z0 x1 y1 z1:
+// buildSVOFromPoints from the LAS file in the box, toDAG, outdir/points.svdag.
+int las_mode(int argc, char** argv) {
+  if (argc < 11) {
+    fprintf(stderr, "usage: symvox_api_test --las file.las levels outdir x0 y0 z0 x1 y1 z1\n");
+    return 2;
+  }
+  const unsigned L = (unsigned)atoi(argv[3]);
+  const std::string out = argv[4];
+  double b[6];
+  for (int i = 0; i < 6; ++i) b[i] = atof(argv[5 + i]);
+  const svd::aabox3d box(svd::point3d(b[0], b[1], b[2]), svd::point3d(b[3], b[4], b[5]));
+  Scene scene;
+  GeomOctree g(&scene);
+  g.buildSVOFromPoints(std::string(argv[2]), L, box);
+  printf("points_voxels %zu\n", (size_t)g.getStats().nTotalVoxels);
+  g.toDAG(true);
+  EncodedSVDAG e;
+  e.encode(g);
+  e.save(out + "/points.svdag");
+  return 0;
+}
+
 int main(int argc, char** argv) {
+  if (argc >= 2 && std::string(argv[1]) == "--las") return las_mode(argc, argv);
   if (argc < 4) {
     fprintf(stderr, "usage: symvox_api_test codes.bin levels outdir\n");
     return 2;

@@ -350,3 +350,47 @@ def test_cpp_api_travnode_copy_nodedata_pointer_sdag(tmp_path):
     assert (tmp_path / "sym.ssvdag").read_bytes() == o.encode(2)
     assert kv["inner_ptr_ok"] == "1" and kv["leaf_ptr_ok"] == "1" and kv["leaf_child_default"] == "1"
     assert "WARNING! Asking for a node child" in out
+
+
+def write_las(path, raw, scale, offset):
+    """A LAS 1.2 file, point data format 0 (20-byte records), raw int32 X Y Z."""
+    import struct
+    n = raw.shape[0]
+    xyz = raw * np.asarray(scale) + np.asarray(offset)
+    hdr = bytearray(227)
+    hdr[0:4] = b"LASF"
+    hdr[24], hdr[25] = 1, 2
+    struct.pack_into("<HIIBHI", hdr, 94, 227, 227, 0, 0, 20, n)
+    struct.pack_into("<3d3d", hdr, 131, *scale, *offset)
+    struct.pack_into("<6d", hdr, 179, xyz[:, 0].max(), xyz[:, 0].min(), xyz[:, 1].max(), xyz[:, 1].min(),
+                     xyz[:, 2].max(), xyz[:, 2].min())
+    rec = np.zeros(n, dtype=[("x", "<i4"), ("y", "<i4"), ("z", "<i4"), ("rest", "V8")])
+    rec["x"], rec["y"], rec["z"] = raw[:, 0], raw[:, 1], raw[:, 2]
+    with open(path, "wb") as f:
+        f.write(bytes(hdr))
+        f.write(rec.tobytes())
+    return xyz
+
+
+def test_cpp_api_build_svo_from_las_points(tmp_path):
+    """GeomOctree::buildSVOFromPoints(fileName, ...) of the drop-in
+    (geom_octree.cpp:62-169) on a LAS file: its points are libLAS's raw *
+    scale + offset, and the DAG's .svdag equals the oracle's from the same
+    points in file order."""
+    rng = np.random.default_rng(21)
+    L = 8
+    raw = np.concatenate([rng.integers(-40000, 40000, size=(50000, 3)),
+                          rng.integers(-5, 5, size=(5000, 3)) * 8000]).astype(np.int64)
+    raw = raw[rng.permutation(len(raw))]
+    scale, offset = (0.001, 0.0025, 0.0005), (100.5, -20.25, 3.0)
+    xyz = write_las(tmp_path / "cloud.las", raw, scale, offset)
+    bmin = xyz.min(axis=0) - 0.01
+    bmax = xyz.max(axis=0) - 1.0          # some points fall outside the box
+    args = [os.path.join(BIN, "symvox_api_test"), "--las", "cloud.las", str(L), str(tmp_path)]
+    args += [repr(float(v)) for v in list(bmin) + list(bmax)]
+    out = run(args, tmp_path)
+    o = O.OracleOctree.from_points(xyz, L, bmin, bmax)
+    assert f"points_voxels {o.n_voxels}" in out
+    o.to_dag()
+    o.init_child_levels()
+    assert (tmp_path / "points.svdag").read_bytes() == o.encode(0)

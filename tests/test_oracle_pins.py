@@ -610,3 +610,77 @@ def test_builddag_morton_subtrees_equal_plain_build(name, L, step):
     assert all(abs(x - y) <= max(2, x // 100) for x, y in zip(sa, sb)), (sa, sb)
     np.testing.assert_array_equal(decode_ssvdag(b.encode(2)), codes)
     np.testing.assert_array_equal(decode_words(b.encode(1), True), codes)
+
+
+# ---------------------------------------------------------------- buildSVOFromPoints
+def _morton_of(ix, iy, iz, L):
+    """Morton codes of integer voxel coordinates (x most significant within
+    each level's triple, so code & 7 = child slot 4x + 2y + z)."""
+    code = np.zeros(ix.shape, np.uint64)
+    for b in range(L - 1, -1, -1):
+        code = (code << np.uint64(3)) | (((ix >> b) & 1) << 2).astype(np.uint64) \
+            | (((iy >> b) & 1) << 1).astype(np.uint64) | ((iz >> b) & 1).astype(np.uint64)
+    return code
+
+
+def _levels_equal(a, b, L, what):
+    for lev in range(L):
+        np.testing.assert_array_equal(a.level(lev), b.level(lev), err_msg=f"{what}: level {lev}")
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_buildsvo_points_dyadic_box_is_quantization_in_point_order(seed):
+    """buildSVOFromPoints (geom_octree.cpp:62-169) in a dyadic box, where
+    every double operation of its DFS is exact: a point inside [min, max)
+    lands in the voxel floor((p - min) / voxel), points outside or on the max
+    faces add nothing, and every level's node order is the order of first
+    arrival in the point sequence, i.e. the Morton builder (pinned by brute
+    force) fed the points' codes in point order."""
+    rng = np.random.default_rng(seed)
+    L, lo, hi = 6, -4.0, 4.0
+    vox = (hi - lo) / 2 ** L
+    pts = rng.uniform(lo - 0.5, hi + 0.5, size=(3000, 3))
+    on_grid = lo + vox * rng.integers(0, 2 ** L + 1, size=(400, 3))     # exact voxel faces, max face included
+    pts = np.concatenate([pts, on_grid])
+    pts = pts[rng.permutation(len(pts))]
+    inside = np.all((pts >= lo) & (pts < hi), axis=1)
+    assert inside.sum() > 2000 and (~inside).sum() > 100
+    q = np.floor((pts[inside] - lo) / vox).astype(np.int64)
+    codes = _morton_of(q[:, 0], q[:, 1], q[:, 2], L)
+    o = O.OracleOctree.from_points(pts, L, (lo, lo, lo), (hi, hi, hi))
+    ref = O.OracleOctree.from_morton(codes, L)
+    _levels_equal(o, ref, L, "points vs Morton in point order")
+    assert o.n_voxels == np.unique(codes).size
+
+
+@pytest.mark.parametrize("seed", [3, 4])
+def test_buildsvo_points_general_box_away_from_faces(seed):
+    """A box whose bounds are not dyadic (float32 root side, double centre,
+    geom_octree.cpp:66-73): every point farther than 1e-7 voxel from every
+    voxel face in exact rational arithmetic lands in the voxel the exact
+    quantization gives, in point order as above."""
+    from fractions import Fraction as F
+    rng = np.random.default_rng(seed)
+    L = 5
+    bmin = np.array([0.1, -3.7, 2.2])
+    bmax = np.array([5.3, 1.1, 9.9])
+    side = max(float(np.float32((np.float32(b) - np.float32(a)) * np.float32(0.5)) * np.float32(2.0))
+               for a, b in zip(bmin, bmax))
+    centre = (bmin + bmax) * 0.5
+    vox = F(side) / 2 ** L
+    root_lo = [F(float(c)) - F(side) / 2 for c in centre]
+    pts = rng.uniform(bmin - 0.2, bmax + 0.2, size=(1500, 3))
+    keep, cells = [], []
+    for p in pts:
+        t = [(F(float(p[a])) - root_lo[a]) / vox for a in range(3)]
+        fl = [x.numerator // x.denominator for x in t]
+        frac = [x - f for x, f in zip(t, fl)]
+        if all(0 <= f < 2 ** L for f in fl) and all(F(1, 10 ** 7) < r < 1 - F(1, 10 ** 7) for r in frac):
+            keep.append(p)
+            cells.append(fl)
+    keep, cells = np.array(keep), np.array(cells, np.int64)
+    assert len(keep) > 500
+    codes = _morton_of(cells[:, 0], cells[:, 1], cells[:, 2], L)
+    o = O.OracleOctree.from_points(keep, L, bmin, bmax)
+    ref = O.OracleOctree.from_morton(codes, L)
+    _levels_equal(o, ref, L, "points vs exact quantization")
