@@ -129,8 +129,10 @@ LOSSY_BYTES_PER_NODE = 1284   # 64 B occupancy + 64 B embedding write + 64 B rea
 
 def lossy_bench(S, ctx, levels, log):
     """Lossy DAG on the cave volume: learned proposer (MLP embeddings + kNN,
-    every candidate verified exactly) against the exhaustive proposer, same
-    voxel bound, levels L-6..L-3, nodes with at most one effective reference."""
+    every candidate verified exactly) against the exhaustive proposers (first
+    fit, and ranked: the nearest earlier nodes by Hamming distance), same
+    voxel bound, levels L-6..L-3, nodes with at most one effective reference;
+    and the learned kNN's recall@16 on the first merged level."""
     lo, hi, bound = levels - 6, levels - 3, 10
     t0 = time.perf_counter()
     codes = ctx.caves(levels, seed=3)
@@ -156,25 +158,93 @@ def lossy_bench(S, ctx, levels, log):
            "nodes_dag": int(nodes_dag),
            "training": {"pairs": int(tr.pairs), "ms": round(tr.ms_train, 1), "first_loss": round(tr.first_loss, 3),
                         "last_loss": round(tr.last_loss, 3)}}
-    for tag, model in (("learned", m), ("exhaustive", None)):
+    g = fresh()
+    sig_hi = level_signatures(S, g, hi)
+    g.free()
+    # exhaustive = the first list_cap earlier nodes within the bound;
+    # ranked_exhaustive = the list_cap nearest of all of them (nearest first)
+    for tag, model, ranked in (("learned", m, False), ("exhaustive", None, False), ("ranked_exhaustive", None, True)):
         g = fresh()
-        st = g.lossy_merge(lo, hi, max_diff=bound, model=model)
-        cand = int(st["candidates"].sum())
-        ms = float(st["ms_total"].sum())
-        out[tag] = {"nodes_after": int(g.counts()[0]), "merged": int(st["merged"].sum()), "candidates": cand,
-                    "ms": round(ms, 3), "nodes_s": round(cand / (ms * 1e-3), 1),
-                    "ms_phases": {k: round(float(st[k].sum()), 3) for k in
-                                  ("ms_embed", "ms_knn", "ms_lists", "ms_verify", "ms_merge")}}
+        r = lossy_run(g, lo, hi, bound, model, ranked=ranked, record=model is not None)
+        if model is not None:
+            out[tag], knn = r
+            out["recall_at_16"] = knn_recall(sig_hi, knn[: sig_hi.shape[0] * 16].reshape(-1, 16), 16, 4096, seed=9)
+            log(f"[bench] learned recall@16 on level {hi}: {out['recall_at_16']}")
+        else:
+            out[tag] = r
         log(f"[bench] lossy {tag}: {out[tag]}")
         g.free()
     codes.free()
-    le, ex = out["learned"], out["exhaustive"]
+    le, ex, rk = out["learned"], out["exhaustive"], out["ranked_exhaustive"]
     out["compression_gap"] = round((le["nodes_after"] - ex["nodes_after"]) / ex["nodes_after"], 5)
     out["speedup_vs_exhaustive"] = round(le["nodes_s"] / ex["nodes_s"], 2)
+    out["compression_gap_vs_ranked"] = round((le["nodes_after"] - rk["nodes_after"]) / rk["nodes_after"], 5)
+    out["speedup_vs_ranked"] = round(le["nodes_s"] / rk["nodes_s"], 2)
     achieved = le["candidates"] * LOSSY_BYTES_PER_NODE / (le["ms"] * 1e-3) / 1e9
     out["roofline"] = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                        "frac": round(achieved / HBM_PEAK_GBS, 5), "bytes_per_node": LOSSY_BYTES_PER_NODE}
     return out
+
+
+def lossy_run(g, lo, hi, bound, model, ranked, record):
+    """One lossy merge; its counts, device time, throughput and phases."""
+    r = g.lossy_merge(lo, hi, max_diff=bound, model=model, ranked=ranked, record_knn=record)
+    st, knn = r if record else (r, None)
+    cand = int(st["candidates"].sum())
+    ms = float(st["ms_total"].sum())
+    res = {"nodes_after": int(g.counts()[0]), "merged": int(st["merged"].sum()), "candidates": cand,
+           "ms": round(ms, 3), "nodes_s": round(cand / (ms * 1e-3), 1),
+           "ms_phases": {k: round(float(st[k].sum()), 3) for k in
+                         ("ms_embed", "ms_knn", "ms_lists", "ms_verify", "ms_merge")}}
+    return (res, knn) if record else res
+
+
+def level_signatures(S, g, lev):
+    """The 512-bit occupancies of a DAG level, as a device tensor (n x 8 int64)."""
+    import torch
+    n = g.level_size(S.VIEW_DAG, lev)
+    sig = torch.zeros(max(n, 1) * 16, dtype=torch.int32, device="cuda:0")
+    g.lossy_signatures(lev, sig.data_ptr())
+    torch.cuda.synchronize()
+    return sig.view(torch.int64).view(-1, 8)[:n]
+
+
+def _popcount64(x):
+    # SWAR popcount on int64 (arithmetic shifts are masked off)
+    x = x - ((x >> 1) & 0x5555555555555555)
+    x = (x & 0x3333333333333333) + ((x >> 2) & 0x3333333333333333)
+    x = (x + (x >> 4)) & 0x0F0F0F0F0F0F0F0F
+    return (x * 0x0101010101010101) >> 56
+
+
+def knn_recall(sig, knn, k, sample, seed):
+    """Recall@k of proposed neighbour lists against the true k nearest nodes by
+    Hamming distance of the signatures (exact, on the device), over a sample
+    of nodes: the share of a node's k true-nearest distance classes that its
+    proposals reach (a proposal counts when its distance is at most the k-th
+    smallest true distance, so ties are not penalised)."""
+    import torch
+    n = sig.shape[0]
+    rng = np.random.default_rng(seed)
+    q = np.sort(rng.choice(n, size=min(sample, n), replace=False))
+    prop = torch.from_numpy(knn[q].astype(np.int64)).to(sig.device)
+    hits, tot = 0, 0
+    for a in range(0, q.size, 16):
+        qa = torch.from_numpy(q[a:a + 16]).to(sig.device)
+        d = _popcount64(sig[qa][:, None, :] ^ sig[None, :, :]).sum(dim=2)      # [c, n]
+        d[torch.arange(qa.numel(), device=sig.device), qa] = 1 << 20            # not itself
+        kth = torch.kthvalue(d, k, dim=1).values                                # k-th smallest true distance
+        p = prop[a:a + 16]
+        valid = (p >= 0) & (p < n) & (p != qa[:, None])
+        pd = torch.where(valid, d.gather(1, p.clamp(0, n - 1)), torch.full_like(p, 1 << 20))
+        # distinct proposals only
+        srt, _ = torch.sort(torch.where(valid, p, torch.full_like(p, -1)), dim=1)
+        dup = torch.zeros_like(valid)
+        dup[:, 1:] = (srt[:, 1:] == srt[:, :-1]) & (srt[:, 1:] >= 0)
+        good = (pd <= kth[:, None]).sum(dim=1) - dup.sum(dim=1)
+        hits += int(torch.clamp(good, max=k).sum())
+        tot += k * qa.numel()
+    return round(hits / tot, 4)
 
 
 def batch_bench(S, W, ctx, n_scenes, log):

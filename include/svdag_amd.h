@@ -338,6 +338,9 @@ typedef struct svd_mlp svd_mlp;
 
 #define SVD_PROPOSER_EXHAUSTIVE 0
 #define SVD_PROPOSER_LEARNED    1
+/* every earlier node within the bound, ranked: the list_cap nearest by
+ * (occupancy Hamming distance, index), list_cap <= 64, walked nearest first */
+#define SVD_PROPOSER_RANKED     2
 
 typedef struct svd_lossy_params {
   uint32_t lev_lo, lev_hi;     /* levels merged, processed lev_hi .. lev_lo (1 <= lo <= hi <= L-2) */

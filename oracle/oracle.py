@@ -34,7 +34,8 @@ class TraceParams(C.Structure):
 class LossyParams(C.Structure):
     """Leading fields of svd_lossy_params (include/svdag_amd.h)."""
     _fields_ = [("lev_lo", C.c_uint32), ("lev_hi", C.c_uint32), ("max_diff", C.c_uint32),
-                ("max_refs", C.c_uint32), ("list_cap", C.c_uint32), ("diff_factor", C.c_float)]
+                ("max_refs", C.c_uint32), ("list_cap", C.c_uint32), ("diff_factor", C.c_float),
+                ("proposer", C.c_uint32)]
 
 
 LOSSY_STATS_DTYPE = np.dtype([(f, "<u8") for f in ("level", "bound", "nodes_before", "nodes_after", "candidates",
@@ -88,6 +89,7 @@ def lib():
             "orc_attr_bittree_decode": (i32, [vp, u64, vp, u64]),
             "orc_lossy_merge": (i32, [vp, C.POINTER(LossyParams), vp, u32, vp, u32]),
             "orc_lossy_signatures": (i32, [vp, u32, vp]),
+            "orc_ranked_lists": (i32, [vp, u64, u32, u32, vp, vp]),
             "orc_effective_refcounts": (i32, [vp, u32, vp]),
             "orc_subtree_diff": (u64, [vp, u32, u32, u32, u64]),
             "orc_knn_lsh": (None, [vp, u64, u32, u32, u32, u32, u32, u64, vp, vp]),
@@ -225,10 +227,11 @@ class OracleOctree:
         return out[0] if len(out) == 1 else tuple(out)
 
     def lossy_merge(self, lev_lo, lev_hi, max_diff=0, diff_factor=1.0, max_refs=1, list_cap=32,
-                    proposals=None, k=16):
-        """Lossy DAG with the exhaustive proposer, or with given per-level kNN
-        lists (concatenated n_l x k arrays, levels hi..lo).  Returns per-level stats."""
-        p = LossyParams(lev_lo, lev_hi, max_diff, max_refs, list_cap, diff_factor)
+                    proposals=None, k=16, ranked=False):
+        """Lossy DAG with the exhaustive proposer (ranked: the list_cap nearest
+        earlier nodes within the bound), or with given per-level kNN lists
+        (concatenated n_l x k arrays, levels hi..lo).  Returns per-level stats."""
+        p = LossyParams(lev_lo, lev_hi, max_diff, max_refs, list_cap, diff_factor, 2 if ranked else 0)
         st = np.zeros(64, LOSSY_STATS_DTYPE)
         prop = None if proposals is None else np.ascontiguousarray(proposals, dtype=np.uint32)
         rc = self.L.orc_lossy_merge(self.h, C.byref(p), _p(prop) if prop is not None else None, k, _p(st), 64)
@@ -413,3 +416,14 @@ def knn_refine(sig: np.ndarray, lists: np.ndarray, rounds: int = 1) -> np.ndarra
         lib().orc_knn_refine(_p(s), n, k, _p(cur), _p(out))
         cur = out
     return cur
+
+
+def ranked_lists(sig: np.ndarray, bound: int, cap: int):
+    """The ranked proposer's lists of a signature array (n x 16 uint32): per
+    node its cap nearest earlier nodes within the bound by (Hamming, index)."""
+    sig = np.ascontiguousarray(sig, dtype=np.uint32)
+    n = sig.shape[0]
+    lists = np.full((max(n, 1), cap), 0xFFFFFFFF, np.uint32)
+    counts = np.zeros(max(n, 1), np.uint32)
+    assert lib().orc_ranked_lists(_p(sig), n, bound, cap, _p(lists), _p(counts)) == 0
+    return lists[:n], counts[:n]

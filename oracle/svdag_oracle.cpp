@@ -1968,16 +1968,18 @@ int orc_attr_bittree_decode(const uint8_t* f, uint64_t len, uint8_t* a, uint64_t
 //   * signature = 512-bit occupancy of the node's cells three levels down
 //     (voxels for level L-3); Hamming(sig) <= d, so it is an exact pre-filter;
 //   * node j (effective refcount <= max_refs) lists the first list_cap
-//     earlier nodes i < j that pass the pre-filter (exhaustive), or the
-//     proposed ones (learned), keeps those with d <= bound, and walks them in
-//     order: a representative becomes its target; an entry that merged
-//     earlier offers its own target, taken when within the bound of j.
+//     earlier nodes i < j that pass the pre-filter (exhaustive), the
+//     list_cap nearest of them by (Hamming, i) (ranked), or the proposed
+//     ones (learned), keeps those with d <= bound, and walks them in order:
+//     a representative becomes its target; an entry that merged earlier
+//     offers its own target, taken when within the bound of j.
 // ============================================================================
 namespace orc {
 
 struct LossyParams {             // identical layout to svd_lossy_params' leading fields
   uint32_t lev_lo, lev_hi, max_diff, max_refs, list_cap;
   float diff_factor;
+  uint32_t proposer;             // 2: ranked (used when no proposals are given)
 };
 
 struct LossyLevelStats {         // per processed level, hi..lo
@@ -2031,6 +2033,19 @@ static unsigned hamming512(const uint32_t* a, const uint32_t* b) {
   unsigned h = 0;
   for (int w = 0; w < 16; ++w) h += (unsigned)__builtin_popcount(a[w] ^ b[w]);
   return h;
+}
+
+// The ranked proposer's list of node j: every earlier node within the bound,
+// the cap nearest by (Hamming distance, index).
+static void rankedList(const uint32_t* sig, size_t j, uint64_t bound, uint32_t cap, std::vector<id_t>& pre) {
+  std::vector<std::pair<unsigned, id_t>> all;
+  for (size_t i = 0; i < j; ++i) {
+    const unsigned h = hamming512(&sig[16 * i], &sig[16 * j]);
+    if (h <= bound) all.push_back({h, (id_t)i});
+  }
+  std::sort(all.begin(), all.end());
+  pre.clear();
+  for (size_t q = 0; q < all.size() && q < cap; ++q) pre.push_back(all[q].second);
 }
 
 // acc + d(a, b), returning as soon as the running total exceeds cap.
@@ -2103,6 +2118,8 @@ static int lossyMerge(Octree& o, const LossyParams& p, const uint32_t* proposals
           if (hamming512(&sig[16 * i], &sig[16 * j]) <= bound) pre.push_back(i);
           if (pre.size() == p.list_cap) break;
         }
+      } else if (p.proposer == 2) {
+        rankedList(sig.data(), j, bound, p.list_cap, pre);
       } else {
         for (size_t i = 0; i < j && pre.size() < p.list_cap; ++i)
           if (hamming512(&sig[16 * i], &sig[16 * j]) <= bound) pre.push_back((id_t)i);
@@ -2255,6 +2272,18 @@ int orc_lossy_merge(void* h, const orc::LossyParams* p, const uint32_t* proposal
   const int rc = orc::lossyMerge(*(orc::Octree*)h, *p, proposals, k, st);
   for (size_t i = 0; i < st.size() && i < stats_cap; ++i) stats[i] = st[i];
   return rc;
+}
+
+// rankedList of every node j < n of a signature array: lists[j * cap + q], counts[j]
+int orc_ranked_lists(const uint32_t* sig, uint64_t n, uint32_t bound, uint32_t cap, uint32_t* lists,
+                     uint32_t* counts) {
+  std::vector<orc::id_t> pre;
+  for (uint64_t j = 0; j < n; ++j) {
+    orc::rankedList(sig, (size_t)j, bound, cap, pre);
+    counts[j] = (uint32_t)pre.size();
+    for (size_t q = 0; q < pre.size(); ++q) lists[j * cap + q] = pre[q];
+  }
+  return 0;
 }
 
 int orc_lossy_signatures(void* h, uint32_t lev, uint32_t* sig) {

@@ -59,7 +59,7 @@ class ViewParams(C.Structure):
         return v
 
 
-PROPOSER_EXHAUSTIVE, PROPOSER_LEARNED = 0, 1
+PROPOSER_EXHAUSTIVE, PROPOSER_LEARNED, PROPOSER_RANKED = 0, 1, 2
 MLP_PARAMS = 168352
 
 
@@ -693,12 +693,16 @@ class Octree:
 
     def lossy_merge(self, lev_lo: int, lev_hi: int, max_diff: int = 0, diff_factor: float = 1.0, max_refs: int = 1,
                     list_cap: int = 32, model: "MLP | None" = None, k: int = 16, tables: int = 8, bits: int = 16,
-                    window: int = 64, refine: int = 2, knn_seed: int = 7, record_knn: bool = False):
+                    window: int = 64, refine: int = 2, knn_seed: int = 7, record_knn: bool = False,
+                    ranked: bool = False):
         """GeomOctree::toLossyDag on the device.  model=None: exhaustive
-        proposer; else the learned one.  Returns per-level stats (and the kNN
-        lists used, concatenated level by level, when record_knn)."""
+        proposer (the first list_cap earlier nodes within the bound, or the
+        list_cap nearest of them when ranked); else the learned one.  Returns
+        per-level stats (and the kNN lists used, concatenated level by level,
+        when record_knn)."""
+        prop = PROPOSER_LEARNED if model is not None else (PROPOSER_RANKED if ranked else PROPOSER_EXHAUSTIVE)
         p = LossyParams(lev_lo, lev_hi, max_diff, max_refs, list_cap, diff_factor,
-                        PROPOSER_LEARNED if model is not None else PROPOSER_EXHAUSTIVE, k, tables, bits, window, refine,
+                        prop, k, tables, bits, window, refine,
                         knn_seed, model.h if model is not None else None, None, 0)
         rec = None
         if record_knn and model is not None:

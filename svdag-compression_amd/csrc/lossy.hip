@@ -8,8 +8,9 @@
 //   2. lists: for each candidate j (effective reference count <= max_refs,
 //      :945) the first list_cap earlier nodes i < j with
 //      popcount(sig_i ^ sig_j) <= bound -- exhaustively (an LDS-tiled
-//      all-pairs XOR-popcount scan) or among the learned proposals (kNN of the
-//      MLP embeddings, sorted and de-duplicated);
+//      all-pairs XOR-popcount scan), or the list_cap nearest of them by
+//      (popcount, i) (ranked, the same scan to the end), or among the learned
+//      proposals (kNN of the MLP embeddings, sorted and de-duplicated);
 //   3. verify: exact voxel difference of the two subtrees by a joint descent
 //      with early abort (the Hamming distance above is a lower bound of it);
 //   4. resolve: j walks its verified entries in order; a representative
@@ -148,6 +149,64 @@ __global__ void __launch_bounds__(256) k_exhaustive_lists(const uint32_t* __rest
     __syncthreads();
   }
   if (j < n) counts[j] = active ? cnt : 0u;
+}
+
+// Ranked pre-filter lists: every earlier node i < j within the bound, the K
+// nearest kept by (hamming << 32 | i) in a sorted register list (an insertion
+// bubbles the new key in and drops the largest); the same LDS tiles as
+// k_exhaustive_lists, scanned to the end (no early stop).
+template <int K>
+__global__ void __launch_bounds__(256) k_ranked_lists(const uint32_t* __restrict__ sig, uint64_t n,
+                                                      const uint64_t* __restrict__ refs, uint64_t maxRefs,
+                                                      uint32_t bound, uint32_t cap, uint32_t* __restrict__ lists,
+                                                      uint32_t* __restrict__ counts) {
+  __shared__ uint4 tile[256 * 4];
+  const uint64_t j0 = (uint64_t)blockIdx.x * 256;
+  const uint64_t j = j0 + threadIdx.x;
+  const bool active = j < n && refs[j] <= maxRefs;
+  uint32_t s[16];
+  if (j < n) {
+    const uint4* p = reinterpret_cast<const uint4*>(sig + j * 16);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) { const uint4 t = p[q]; s[4 * q] = t.x; s[4 * q + 1] = t.y; s[4 * q + 2] = t.z; s[4 * q + 3] = t.w; }
+  }
+  unsigned long long top[K];
+#pragma unroll
+  for (int q = 0; q < K; ++q) top[q] = ~0ull;
+  uint32_t cnt = 0;
+  const uint64_t jend = j0 + 256 < n ? j0 + 256 : n;
+  for (uint64_t i0 = 0; i0 < jend; i0 += 256) {
+    if (!__syncthreads_or(active)) break;
+    for (int q = threadIdx.x; q < 256 * 4; q += 256) {
+      const uint64_t i = i0 + (q >> 2);
+      tile[q] = i < n ? reinterpret_cast<const uint4*>(sig + i * 16)[q & 3] : make_uint4(0, 0, 0, 0);
+    }
+    __syncthreads();
+    if (active) {
+      const uint64_t iend = i0 + 256 < j ? i0 + 256 : j;
+      for (uint64_t i = i0; i < iend; ++i) {
+        const uint32_t h = hamming512(&tile[(i - i0) * 4], s);
+        if (h > bound) continue;
+        unsigned long long x = ((unsigned long long)h << 32) | (unsigned long long)i;
+        if (x >= top[K - 1]) continue;
+#pragma unroll
+        for (int q = 0; q < K; ++q) {
+          const unsigned long long t = top[q];
+          const bool lt = x < t;
+          top[q] = lt ? x : t;
+          x = lt ? t : x;
+        }
+        ++cnt;
+      }
+    }
+    __syncthreads();
+  }
+  if (j >= n) return;
+  const uint32_t m = active ? min(cnt, cap) : 0u;
+#pragma unroll
+  for (int q = 0; q < K; ++q)
+    if ((uint32_t)q < m) lists[j * cap + q] = (uint32_t)top[q];
+  counts[j] = m;
 }
 
 // Learned proposals -> (hi << 32 | lo) keys; invalid entries get ~0.
@@ -408,7 +467,8 @@ void lossy_merge(svd_octree* o, const svd_lossy_params* p, std::vector<svd_lossy
     throw Error{SVD_ERR_ARG, "lossy levels must satisfy 1 <= lev_lo <= lev_hi <= L-2"};
   if (p->list_cap < 1 || p->list_cap > 64) throw Error{SVD_ERR_ARG, "list_cap must be 1..64"};
   const bool learned = p->proposer == SVD_PROPOSER_LEARNED;
-  if (p->proposer > SVD_PROPOSER_LEARNED) throw Error{SVD_ERR_ARG, "unknown proposer"};
+  const bool ranked = p->proposer == SVD_PROPOSER_RANKED;
+  if (p->proposer > SVD_PROPOSER_RANKED) throw Error{SVD_ERR_ARG, "unknown proposer"};
   if (learned && (!p->model || p->knn_k < 1 || p->knn_k > 32 || p->knn_tables < 1 || p->knn_bits < 1 || p->knn_refine > 16 ||
                   p->knn_bits > 24 || p->knn_window < 1 || p->knn_window > 96))
     throw Error{SVD_ERR_ARG, "learned proposer needs a model, 1<=k<=32, tables>=1, 1<=bits<=24, 1<=window<=96"};
@@ -474,9 +534,19 @@ void lossy_merge(svd_octree* o, const svd_lossy_params* p, std::vector<svd_lossy
     clk.mark(1);
     if (!learned) {
       clk.mark(2);
-      if (n) k_exhaustive_lists<<<div_up(n, 256), 256, 0, c->stream>>>(sig, n, refs[lev], maxRefs, bound, cap, lists,
-                                                                    counts);
-      check_launch("k_exhaustive_lists");
+      if (n && !ranked)
+        k_exhaustive_lists<<<div_up(n, 256), 256, 0, c->stream>>>(sig, n, refs[lev], maxRefs, bound, cap, lists,
+                                                                  counts);
+      else if (n && cap <= 16)
+        k_ranked_lists<16><<<div_up(n, 256), 256, 0, c->stream>>>(sig, n, refs[lev], maxRefs, bound, cap, lists,
+                                                                   counts);
+      else if (n && cap <= 32)
+        k_ranked_lists<32><<<div_up(n, 256), 256, 0, c->stream>>>(sig, n, refs[lev], maxRefs, bound, cap, lists,
+                                                                   counts);
+      else if (n)
+        k_ranked_lists<64><<<div_up(n, 256), 256, 0, c->stream>>>(sig, n, refs[lev], maxRefs, bound, cap, lists,
+                                                                   counts);
+      check_launch("pre-filter lists");
     } else {
       const uint32_t k = p->knn_k;
       float* emb = dalloc<float>(c, (n + 32) * 32);

@@ -83,6 +83,27 @@ def test_lossy_exhaustive_bit_exact(ctx, name, mode):
     assert g.encoded_bytes(S.FMT_SSVDAG) == o.encode(2)
 
 
+@pytest.mark.parametrize("name", list(SCENES))
+@pytest.mark.parametrize("mode", ["max10cap8", "factor1cap32", "refs3cap40"])
+def test_lossy_ranked_bit_exact(ctx, name, mode):
+    """The ranked exhaustive proposer (the list_cap nearest earlier nodes by
+    Hamming distance, nearest first) equals the oracle's, level by level and
+    statistic by statistic; the three list capacities take the 16-, 32- and
+    64-entry register lists."""
+    codes, L = SCENES[name]()
+    kw = {"max10cap8": dict(max_diff=10, list_cap=8), "factor1cap32": dict(max_diff=0, diff_factor=1.0, list_cap=32),
+          "refs3cap40": dict(max_diff=6, max_refs=3, list_cap=40)}[mode]
+    print(f"[case] ranked {name} {mode} L={L} voxels={codes.size}", flush=True)
+    g, o = gpu_dag(ctx, codes, L), oracle_dag(codes, L)
+    st_g = g.lossy_merge(1, L - 2, ranked=True, **kw)
+    st_o = o.lossy_merge(1, L - 2, ranked=True, **kw)
+    assert_same_stats(st_g, st_o)
+    assert_same_dag(g, o, L)
+    g.encode(S.FMT_SVDAG)
+    o.init_child_levels()
+    assert g.encoded_bytes(S.FMT_SVDAG) == o.encode(0)
+
+
 def test_lossy_error_within_accounted_bound(ctx):
     codes, L = W.cave_codes(7, seed=3), 7
     g = gpu_dag(ctx, codes, L)

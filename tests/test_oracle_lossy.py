@@ -292,3 +292,53 @@ def test_mlp_adam_step_closed_form():
     M.adam(p, g, m, v, 1, 0.01)
     want = -0.01 * g.astype(np.float64) / (np.abs(g) + 1e-8)
     np.testing.assert_allclose(p, want, rtol=1e-4, atol=1e-7)
+
+
+# ---------------------------------------------------------------- ranked proposer
+def _hamming_all(sig):
+    """Pairwise Hamming distances of 512-bit signatures (n x 16 uint32)."""
+    bits = np.unpackbits(np.ascontiguousarray(sig).view(np.uint8), axis=1).astype(np.int32)
+    return (bits[:, None, :] != bits[None, :, :]).sum(axis=2)
+
+
+@pytest.mark.parametrize("name", list(SCENES))
+@pytest.mark.parametrize("bound,cap", [(0, 4), (6, 3), (20, 8), (40, 16)])
+def test_ranked_lists_brute_force(name, bound, cap):
+    """The ranked proposer's list of node j is its cap nearest earlier nodes
+    within the bound by (Hamming distance, index): a brute force over all
+    pairs from numpy's bit unpacking, on real signatures (ties included)."""
+    codes, L = SCENES[name]()
+    o = dag_of(codes, L)
+    lev = L - 3
+    sig = o.lossy_signatures(lev)
+    D = _hamming_all(sig)
+    lists, counts = O.ranked_lists(sig, bound, cap)
+    for j in range(sig.shape[0]):
+        cand = sorted((int(D[i, j]), i) for i in range(j) if D[i, j] <= bound)[:cap]
+        want = [i for _, i in cand]
+        assert counts[j] == len(want), j
+        assert list(lists[j, :counts[j]]) == want, j
+
+
+@pytest.mark.parametrize("name", list(SCENES))
+def test_ranked_zero_bound_is_exhaustive(name):
+    """Within a zero bound all listed nodes are at distance 0, so the nearest
+    first order is the index order: the same merge as the exhaustive one."""
+    codes, L = SCENES[name]()
+    a, b = dag_of(codes, L), dag_of(codes, L)
+    sa = a.lossy_merge(1, L - 2, max_diff=0, diff_factor=0.0, list_cap=8)
+    sb = b.lossy_merge(1, L - 2, max_diff=0, diff_factor=0.0, list_cap=8, ranked=True)
+    for f in ("nodes_after", "listed", "verified", "merged"):
+        np.testing.assert_array_equal(sa[f], sb[f], err_msg=f)
+    for x, y in zip(a.levels_words(), b.levels_words()):
+        np.testing.assert_array_equal(x, y)
+
+
+@pytest.mark.parametrize("name", list(SCENES))
+def test_ranked_merge_error_within_bound(name):
+    codes, L = SCENES[name]()
+    o = dag_of(codes, L)
+    st = o.lossy_merge(1, L - 2, max_diff=10, list_cap=8, ranked=True)
+    vox = np.array(expander(o.levels_words(), L)(0, 0), np.uint64)
+    diff = np.setxor1d(vox, codes).size
+    assert st["merged"].sum() > 0 and 0 < diff <= int(st["error_bound"].sum())
