@@ -877,7 +877,7 @@ __global__ void k_sdag_cands(const uint32_t* __restrict__ key, uint64_t n, const
   const int t = (int)(g & 7u);
   const bool live = i < n;
   // a node without children was never inserted (slot kNone): it is no
-  // representative and k_sdag_outcome reports it, as k_sdag_small does
+  // representative and k_sdag_outcome reports it, as sdag_small_level does
   bool first = false;
   if (live) {
     const uint32_t si = slot[i];
