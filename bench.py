@@ -125,6 +125,16 @@ def build_bytes(S, g, L, V):
 
 
 LOSSY_BYTES_PER_NODE = 1284   # 64 B occupancy + 64 B embedding write + 64 B read + 16 x (64 + 4) B verify + 4 B remap
+MFMA_BF16_PEAK_TFLOPS = 2500.0
+MLP_FWD_FLOP_PER_ROW = 2 * (512 * 256 + 256 * 128 + 128 * 32)
+# forward + activation gradients of layers 3 and 2 + the three weight gradients
+MLP_TRAIN_FLOP_PER_ROW = 2 * MLP_FWD_FLOP_PER_ROW + 2 * (128 * 32 + 256 * 128)
+
+
+def mfma_rate(flop, ms):
+    t = flop / (ms * 1e-3) / 1e12
+    return {"tflops": round(t, 2), "peak": MFMA_BF16_PEAK_TFLOPS, "frac": round(t / MFMA_BF16_PEAK_TFLOPS, 5),
+            "flop": int(flop)}
 
 
 def lossy_bench(S, ctx, levels, log):
@@ -157,7 +167,9 @@ def lossy_bench(S, ctx, levels, log):
                        f"voxel difference)",
            "nodes_dag": int(nodes_dag),
            "training": {"pairs": int(tr.pairs), "ms": round(tr.ms_train, 1), "first_loss": round(tr.first_loss, 3),
-                        "last_loss": round(tr.last_loss, 3)}}
+                        "last_loss": round(tr.last_loss, 3),
+                        # whole training steps (sampling, loss, Adam included) over the GEMM work
+                        "mfma": mfma_rate(2 * tr.pairs * MLP_TRAIN_FLOP_PER_ROW, tr.ms_train)}}
     g = fresh()
     sig_hi = level_signatures(S, g, hi)
     g.free()
@@ -168,6 +180,8 @@ def lossy_bench(S, ctx, levels, log):
         r = lossy_run(g, lo, hi, bound, model, ranked=ranked, record=model is not None)
         if model is not None:
             out[tag], knn = r
+            out["embedding_mfma"] = mfma_rate(out[tag]["nodes_embedded"] * MLP_FWD_FLOP_PER_ROW,
+                                              out[tag]["ms_phases"]["ms_embed"])
             out["recall_at_16"] = knn_recall(sig_hi, knn[: sig_hi.shape[0] * 16].reshape(-1, 16), 16, 4096, seed=9)
             log(f"[bench] learned recall@16 on level {hi}: {out['recall_at_16']}")
         else:
@@ -196,6 +210,8 @@ def lossy_run(g, lo, hi, bound, model, ranked, record):
            "ms": round(ms, 3), "nodes_s": round(cand / (ms * 1e-3), 1),
            "ms_phases": {k: round(float(st[k].sum()), 3) for k in
                          ("ms_embed", "ms_knn", "ms_lists", "ms_verify", "ms_merge")}}
+    if model is not None:
+        res["nodes_embedded"] = int(st["nodes_before"].sum())
     return (res, knn) if record else res
 
 

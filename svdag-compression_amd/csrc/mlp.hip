@@ -1,6 +1,8 @@
 // Learned candidate encoder for the lossy DAG: MLP 512 -> 256 -> 128 -> 32.
 // bf16 weights and activations, fp32 accumulation on the CDNA matrix cores
-// (v_mfma_f32_32x32x8_bf16), hand-written forward and backward passes, Adam.
+// (v_mfma_f32_32x32x16_bf16 on LDS-staged tiles; the weight gradients split
+// over the batch and summed in a fixed order by the optimizer), hand-written
+// forward and backward passes, Adam.
 // The reference has no learned part (its lossy path clusters an exhaustive
 // similarity graph with MCL, geom_octree_extension.cpp:797-1179).
 //
@@ -20,12 +22,11 @@ namespace svd {
 
 namespace {
 
-typedef short bf16x4_t __attribute__((ext_vector_type(4)));
-typedef float f32x16_t __attribute__((ext_vector_type(16)));
 
 constexpr int D0 = 512, D1 = 256, D2 = 128, D3 = 32;
 constexpr uint64_t OFF_W1 = 0, OFF_B1 = OFF_W1 + D0 * D1, OFF_W2 = OFF_B1 + D1, OFF_B2 = OFF_W2 + D1 * D2,
                    OFF_W3 = OFF_B2 + D2, OFF_B3 = OFF_W3 + D2 * D3, N_PARAMS = OFF_B3 + D3;
+// bf16 weights: W1 | W2 | W3 row-major [in][out], then the same transposed [out][in]
 constexpr uint64_t WB_W1 = 0, WB_W2 = WB_W1 + D0 * D1, WB_W3 = WB_W2 + D1 * D2, N_WB = WB_W3 + D2 * D3;
 constexpr float kHCap = 32.f;
 static_assert(N_PARAMS == SVD_MLP_PARAMS, "parameter layout");
@@ -37,182 +38,350 @@ __device__ __forceinline__ uint16_t f2bf(float f) {   // round to nearest even (
 }
 __device__ __forceinline__ float bf2f(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
 
-enum AMode { A_BITS = 0, A_BITS_T = 1, A_BF16 = 2 };
-enum Epi { E_BIAS_RELU_BF16 = 0, E_BIAS_F32 = 1, E_MASK_BF16 = 2, E_F32 = 3 };
+// C = A (M x K) B (K x N) with bf16 operands and fp32 accumulation.
+// Operand layouts:
+//   AL_BITS   A[m][k] = bit k of signature m (16 words per signature), as 0/1
+//   AL_BITS_T A[m][k] = bit m of signature k
+//   AL_MK     A[m][k] = a[m lda + k]        AL_KM  A[m][k] = a[k lda + m]
+//   BL_NK     B[k][n] = b[n ldb + k]        BL_KN  B[k][n] = b[k ldb + n]
+// Epilogues: bf16(relu(C + bias)); C + bias (fp32); bf16(C) where mask > 0
+// (else 0); E_PART: the split's partial C (fp32) and, with partBias, the
+// split's column sums of B (a bias gradient when B is a layer's delta);
+// E_PAIR_LOSS: e = C + bias is not stored, rows (2p, 2p+1) are pair p and the
+// pair loss's gradient w.r.t. e goes out as bf16, its loss into *loss.
+enum ALay { AL_BITS = 0, AL_BITS_T = 1, AL_MK = 2, AL_KM = 3 };
+enum BLay { BL_NK = 0, BL_KN = 1 };
+enum Epi { E_BIAS_RELU_BF16 = 0, E_BIAS_F32 = 1, E_MASK_BF16 = 2, E_PART = 3, E_PAIR_LOSS = 4 };
 
-struct GemmArgs {
-  uint64_t mValid;                 // rows of C that exist (grid may pad to 32)
+struct MmArgs {
+  uint64_t M;
   int N, K;
-  const uint32_t* abits;           // A_BITS: A[m][k] = bit k of signature m; A_BITS_T: bit m of signature k
-  const uint16_t* a; int64_t a_sm, a_sk;   // A_BF16: A[m][k] = a[m*a_sm + k*a_sk]
-  const uint16_t* b; int64_t b_sk, b_sn;   // B[k][n] = b[k*b_sk + n*b_sn]
+  int kPer;                        // E_PART: split z = blockIdx.z covers k in [z kPer, min(K, (z+1) kPer))
+  const uint32_t* abits;
+  const uint16_t* a; int64_t lda;
+  const uint16_t* b; int64_t ldb;
   const float* bias;
-  const uint16_t* mask; int64_t mask_sm;   // E_MASK_BF16: keep C[m][n] where mask[m*mask_sm + n] > 0
-  uint16_t* outb; float* outf; int64_t o_sm;
-  int kPer; int64_t o_sz;          // split-K: blockIdx.z covers k in [z*kPer, (z+1)*kPer), writes outf + z*o_sz
+  const uint16_t* mask; int64_t ldmask;
+  uint16_t* outb; float* outf; int64_t ldo;
+  float* part; float* partBias;    // E_PART: part + z M N (row-major, ld N), partBias + z N
+  const uint32_t* labels;          // E_PAIR_LOSS: Hamming distance of pair p
+  float* loss; const uint32_t* lossStep;   // loss[*lossStep] (loss[0] without lossStep) += the batch loss
+  uint32_t* bump;                  // if set, workgroup 0 adds 1 to *bump
 };
 
-// One wavefront computes one 32x32 tile of C = A(MxK) B(KxN) with
-// v_mfma_f32_32x32x8_bf16: lane l holds A[m0 + l%32][k0 + 4(l/32) + i] and
-// B[k0 + 4(l/32) + i][n0 + l%32], i = 0..3; accumulator element v of lane l is
-// C[m0 + 8(v/4) + 4(l/32) + v%4][n0 + l%32].
-template <int AM, int EP>
-__global__ void __launch_bounds__(64) k_gemm(GemmArgs g) {
-  const int lane = threadIdx.x;
-  const uint64_t m0 = (uint64_t)blockIdx.x * 32;
-  const int n0 = blockIdx.y * 32;
-  const int r = lane & 31, kq = (lane >> 5) * 4;
-  const uint64_t am = m0 + (uint64_t)r;
-  const int bn = n0 + r;
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef float f32x16_t __attribute__((ext_vector_type(16)));
+
+// bf16 0/1 pair from the two low bits of w (element 0 in the low half)
+__device__ __forceinline__ uint32_t bit_pair(uint32_t w) {
+  return ((w & 1u) ? 0x3F80u : 0u) | ((w & 2u) ? 0x3F800000u : 0u);
+}
+
+// Four wavefronts per workgroup, each a 32x32 sub-tile of a TM x TN tile
+// (64x64 for WN = 2, 128x32 for WN = 1).  K runs in stages of KB: a stage is
+// loaded from global memory into registers while the previous one feeds the
+// matrix cores from LDS (two LDS buffers, one barrier per stage).  In LDS, A
+// is kept [m][k] and B [n][k], rows padded by 16 bytes; bits are expanded to
+// bf16 and the m- or n-contiguous layouts transposed as they are stored.  Per
+// 16 of K, lane l of a wavefront reads A[row l%32][k + 8(l/32) + i] and
+// B[k + 8(l/32) + i][col l%32], i = 0..7 (16 bytes each), for one
+// v_mfma_f32_32x32x16_bf16; accumulator element v of lane l is
+// C[8(v/4) + 4(l/32) + v%4][l%32] of the wavefront's sub-tile.
+template <int AL, int BL, int EP, int WN, int KB>
+__global__ void __launch_bounds__(256) k_mm(MmArgs g) {
+  constexpr int WM = 4 / WN, TM = 32 * WM, TN = 32 * WN, ROW = KB + 8;
+  constexpr bool kBits = AL == AL_BITS || AL == AL_BITS_T;
+  constexpr int A_CH = kBits ? TM * KB / 32 : TM * KB / 8;   // words or 16-byte chunks per stage
+  constexpr int B_CH = TN * KB / 8;
+  constexpr int NA = (A_CH + 255) / 256, NB = (B_CH + 255) / 256;
+  constexpr int QN = 256 / TN, KQ = KB / QN;                   // column-sum split of a stage
+  static_assert(EP != E_PART || KQ % 8 == 0, "column sums read 16-byte chunks");
+  static_assert(EP != E_PAIR_LOSS || WN == 1, "the pair loss needs all 32 columns of e in one tile");
+  __shared__ __attribute__((aligned(16))) uint16_t As[2][TM * ROW];
+  __shared__ __attribute__((aligned(16))) uint16_t Bs[2][TN * ROW];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave % WM, wn = wave / WM;
+  const uint64_t m0 = (uint64_t)blockIdx.x * TM;
+  const int n0 = blockIdx.y * TN;
+  const int z = EP == E_PART ? (int)blockIdx.z : 0;
+  const int kBeg = EP == E_PART ? z * g.kPer : 0;
+  const int kEnd = EP == E_PART ? min(g.K, kBeg + g.kPer) : g.K;
+  const bool colSums = EP == E_PART && g.partBias && blockIdx.x == 0;
+  if (g.bump && tid == 0 && blockIdx.x == 0 && blockIdx.y == 0 && blockIdx.z == 0) *g.bump += 1u;
+  uint4 ra[kBits ? 1 : NA];
+  uint32_t rw[kBits ? NA : 1];
+  uint4 rb[NB];
+
+  auto load = [&](int k0) {
+#pragma unroll
+    for (int j = 0; j < NA; ++j) {
+      const int c = tid + 256 * j;
+      if (c >= A_CH) break;
+      if (AL == AL_BITS) {
+        const int r = c / (KB / 32), k = k0 + 32 * (c % (KB / 32));
+        const uint64_t m = m0 + (uint64_t)r;
+        rw[j] = (m < g.M && k < kEnd) ? g.abits[m * 16 + (k >> 5)] : 0u;
+      } else if (AL == AL_BITS_T) {
+        const int k = k0 + c / (TM / 32);
+        const uint64_t m = m0 + 32 * (uint64_t)(c % (TM / 32));
+        rw[j] = (m < g.M && k < kEnd) ? g.abits[(uint64_t)k * 16 + (m >> 5)] : 0u;
+      } else if (AL == AL_MK) {
+        const int r = c / (KB / 8), k = k0 + 8 * (c % (KB / 8));
+        const uint64_t m = m0 + (uint64_t)r;
+        ra[j] = (m < g.M && k < kEnd) ? *reinterpret_cast<const uint4*>(g.a + (int64_t)m * g.lda + k)
+                                      : make_uint4(0u, 0u, 0u, 0u);
+      } else {
+        const int k = k0 + c / (TM / 8);
+        const uint64_t m = m0 + 8 * (uint64_t)(c % (TM / 8));
+        ra[j] = (m < g.M && k < kEnd) ? *reinterpret_cast<const uint4*>(g.a + (int64_t)k * g.lda + (int64_t)m)
+                                      : make_uint4(0u, 0u, 0u, 0u);
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+      const int c = tid + 256 * j;
+      if (c >= B_CH) break;
+      if (BL == BL_NK) {
+        const int nn = c / (KB / 8), k = k0 + 8 * (c % (KB / 8));
+        rb[j] = k < kEnd ? *reinterpret_cast<const uint4*>(g.b + (int64_t)(n0 + nn) * g.ldb + k)
+                         : make_uint4(0u, 0u, 0u, 0u);
+      } else {
+        const int k = k0 + c / (TN / 8), nc = 8 * (c % (TN / 8));
+        rb[j] = k < kEnd ? *reinterpret_cast<const uint4*>(g.b + (int64_t)k * g.ldb + (n0 + nc))
+                         : make_uint4(0u, 0u, 0u, 0u);
+      }
+    }
+  };
+  auto store = [&](int buf) {
+    uint16_t* as = As[buf];
+    uint16_t* bs = Bs[buf];
+#pragma unroll
+    for (int j = 0; j < NA; ++j) {
+      const int c = tid + 256 * j;
+      if (c >= A_CH) break;
+      if (AL == AL_BITS) {
+        const int r = c / (KB / 32), kc = 32 * (c % (KB / 32));
+        const uint32_t w = rw[j];
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+          *reinterpret_cast<uint4*>(as + r * ROW + kc + 8 * q) =
+              make_uint4(bit_pair(w >> (8 * q)), bit_pair(w >> (8 * q + 2)), bit_pair(w >> (8 * q + 4)),
+                         bit_pair(w >> (8 * q + 6)));
+      } else if (AL == AL_BITS_T) {
+        const int kr = c / (TM / 32), mc = 32 * (c % (TM / 32));
+        const uint32_t w = rw[j];
+#pragma unroll
+        for (int i = 0; i < 32; ++i) as[(mc + i) * ROW + kr] = ((w >> i) & 1u) ? (uint16_t)0x3F80 : (uint16_t)0;
+      } else if (AL == AL_MK) {
+        const int r = c / (KB / 8), kc = 8 * (c % (KB / 8));
+        *reinterpret_cast<uint4*>(as + r * ROW + kc) = ra[j];
+      } else {
+        const int kr = c / (TM / 8), mc = 8 * (c % (TM / 8));
+        const uint32_t e[4] = {ra[j].x, ra[j].y, ra[j].z, ra[j].w};
+#pragma unroll
+        for (int i = 0; i < 8; ++i) as[(mc + i) * ROW + kr] = (uint16_t)(e[i >> 1] >> (16 * (i & 1)));
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+      const int c = tid + 256 * j;
+      if (c >= B_CH) break;
+      if (BL == BL_NK) {
+        const int nn = c / (KB / 8), kc = 8 * (c % (KB / 8));
+        *reinterpret_cast<uint4*>(bs + nn * ROW + kc) = rb[j];
+      } else {
+        const int kr = c / (TN / 8), nc = 8 * (c % (TN / 8));
+        const uint32_t e[4] = {rb[j].x, rb[j].y, rb[j].z, rb[j].w};
+#pragma unroll
+        for (int i = 0; i < 8; ++i) bs[(nc + i) * ROW + kr] = (uint16_t)(e[i >> 1] >> (16 * (i & 1)));
+      }
+    }
+  };
+
+  // the epilogue's own operands are loaded up front, beside the first stage
+  const int nOut = n0 + wn * 32 + (lane & 31);
+  const float bn = (EP == E_BIAS_RELU_BF16 || EP == E_BIAS_F32 || EP == E_PAIR_LOSS) ? g.bias[nOut] : 0.f;
+  uint32_t keep = 0;   // E_MASK_BF16: bit v = mask of accumulator element v is > 0
+  uint32_t lab[8];     // E_PAIR_LOSS: labels of the lane's pairs
+#pragma unroll
+  for (int v = 0; v < 16; ++v) {
+    const uint64_t m = m0 + (uint64_t)(wm * 32 + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3));
+    if (EP == E_MASK_BF16 && m < g.M) keep |= (bf2f(g.mask[(int64_t)m * g.ldmask + nOut]) > 0.f ? 1u : 0u) << v;
+    if (EP == E_PAIR_LOSS && (v & 1) == 0) lab[v >> 1] = m < g.M ? g.labels[m >> 1] : 0u;
+  }
   f32x16_t acc;
 #pragma unroll
   for (int v = 0; v < 16; ++v) acc[v] = 0.f;
-  const int kBeg = (int)blockIdx.z * g.kPer;
-  const int kEnd = min(g.K, kBeg + g.kPer);
-  for (int k0 = kBeg; k0 < kEnd; k0 += 8) {
-    bf16x4_t a, b;
-    const int kb = k0 + kq;
-    if (AM == A_BITS) {
-      const uint32_t w = am < g.mValid ? g.abits[am * 16 + (kb >> 5)] >> (kb & 31) : 0u;
+  float colAcc = 0.f;
+  const int nk = kEnd > kBeg ? (kEnd - kBeg + KB - 1) / KB : 0;
+  if (nk > 0) {
+    load(kBeg);
+    store(0);
+  }
+  __syncthreads();
+  for (int t = 0; t < nk; ++t) {
+    const int buf = t & 1;
+    if (t + 1 < nk) load(kBeg + (t + 1) * KB);
+    const uint16_t* ar = &As[buf][(wm * 32 + (lane & 31)) * ROW];
+    const uint16_t* br = &Bs[buf][(wn * 32 + (lane & 31)) * ROW];
 #pragma unroll
-      for (int i = 0; i < 4; ++i) a[i] = ((w >> i) & 1u) ? (short)0x3F80 : (short)0;
-    } else if (AM == A_BITS_T) {
-#pragma unroll
-      for (int i = 0; i < 4; ++i)
-        a[i] = ((g.abits[(uint64_t)(kb + i) * 16 + (am >> 5)] >> (am & 31)) & 1u) ? (short)0x3F80 : (short)0;
-    } else {
-#pragma unroll
-      for (int i = 0; i < 4; ++i)
-        a[i] = am < g.mValid ? (short)g.a[(int64_t)am * g.a_sm + (int64_t)(kb + i) * g.a_sk] : (short)0;
+    for (int ks = 0; ks < KB; ks += 16) {
+      const int kk = ks + 8 * (lane >> 5);
+      const bf16x8_t a = *reinterpret_cast<const bf16x8_t*>(ar + kk);
+      const bf16x8_t b = *reinterpret_cast<const bf16x8_t*>(br + kk);
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
     }
+    if (EP == E_PART && colSums) {
+      const uint16_t* cr = &Bs[buf][(tid % TN) * ROW + (tid / TN) * KQ];
 #pragma unroll
-    for (int i = 0; i < 4; ++i) b[i] = (short)g.b[(int64_t)(kb + i) * g.b_sk + (int64_t)bn * g.b_sn];
-    acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc, 0, 0, 0);
+      for (int q = 0; q < KQ; q += 8) {
+        const uint4 v = *reinterpret_cast<const uint4*>(cr + q);
+        const uint32_t e[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+        for (int i = 0; i < 4; ++i) colAcc += __uint_as_float(e[i] << 16) + __uint_as_float(e[i] & 0xFFFF0000u);
+      }
+    }
+    if (t + 1 < nk) store(buf ^ 1);
+    __syncthreads();
+  }
+  if (EP == E_PAIR_LOSS) {
+    // rows 2p and 2p+1 are accumulator elements v and v+1 (v even) of a lane;
+    // d = |e_a - e_b|^2 is summed over the 32 lanes of a half-wavefront
+    const int n = nOut;
+    const float invP = 1.0f / (float)(g.M / 2);
+    float lsum = 0.f;
+#pragma unroll
+    for (int v = 0; v < 16; v += 2) {
+      const uint64_t m = m0 + (uint64_t)(wm * 32 + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3));
+      const float diff = (acc[v] + bn) - (acc[v + 1] + bn);
+      float d = diff * diff;
+#pragma unroll
+      for (int o = 16; o >= 1; o >>= 1) d += __shfl_xor(d, o, 64);
+      if (m < g.M) {
+        const float y = fminf((float)lab[v >> 1], kHCap);
+        const float gr = 2.f * (d - y) * invP;
+        const uint16_t dv = f2bf(2.f * gr * diff);
+        g.outb[(int64_t)m * g.ldo + n] = dv;
+        g.outb[(int64_t)(m + 1) * g.ldo + n] = dv ^ 0x8000u;     // bf16(-x) = -bf16(x)
+        lsum += (d - y) * (d - y) * invP;
+      }
+    }
+    if (n == 0) atomicAdd(&g.loss[g.lossStep ? *g.lossStep : 0u], lsum);
   }
 #pragma unroll
   for (int v = 0; v < 16; ++v) {
-    const uint64_t m = m0 + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
-    const int n = n0 + (lane & 31);
-    if (m >= g.mValid) continue;
+    if (EP == E_PAIR_LOSS) break;
+    const uint64_t m = m0 + (uint64_t)(wm * 32 + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3));
+    const int n = nOut;
+    if (m >= g.M) continue;
     const float x = acc[v];
-    const int64_t o = (int64_t)m * g.o_sm + n;
     if (EP == E_BIAS_RELU_BF16) {
-      const float z = x + g.bias[n];
-      g.outb[o] = f2bf(z > 0.f ? z : 0.f);
+      const float zz = x + bn;
+      g.outb[(int64_t)m * g.ldo + n] = f2bf(zz > 0.f ? zz : 0.f);
     } else if (EP == E_BIAS_F32) {
-      g.outf[o] = x + g.bias[n];
+      g.outf[(int64_t)m * g.ldo + n] = x + bn;
     } else if (EP == E_MASK_BF16) {
-      const uint16_t mk = g.mask[(int64_t)m * g.mask_sm + n];
-      g.outb[o] = f2bf(bf2f(mk) > 0.f ? x : 0.f);
+      g.outb[(int64_t)m * g.ldo + n] = f2bf(((keep >> v) & 1u) ? x : 0.f);
     } else {
-      g.outf[(int64_t)blockIdx.z * g.o_sz + o] = x;
+      g.part[((uint64_t)z * g.M + m) * (uint64_t)g.N + n] = x;
+    }
+  }
+  if (EP == E_PART && colSums) {
+    float* red = reinterpret_cast<float*>(As[0]);   // the stages are done
+    red[tid] = colAcc;
+    __syncthreads();
+    if (tid < TN) {
+      float s = 0.f;
+      for (int q = 0; q < QN; ++q) s += red[q * TN + tid];
+      g.partBias[(uint64_t)z * g.N + n0 + tid] = s;
     }
   }
 }
 
-template <int AM, int EP>
-void gemm(svd_ctx* c, uint64_t M, int N, int K, const GemmArgs& g0) {
-  GemmArgs g = g0;
-  g.N = N; g.K = K;
-  g.kPer = K;
-  g.o_sz = 0;
-  dim3 grid(div_up(M, 32), N / 32);
-  k_gemm<AM, EP><<<grid, 64, 0, c->stream>>>(g);
+template <int AL, int BL, int EP, int WN, int KB>
+void mm(svd_ctx* c, const MmArgs& g, int splits = 1) {
+  constexpr int TM = 128 / WN, TN = 32 * WN;
+  if (g.N % TN || (EP != E_PART && g.K % KB) || (EP == E_PART && g.kPer % KB) || splits < 1)
+    throw Error{SVD_ERR_ARG, "mlp gemm: unsupported shape"};
+  if (g.M == 0) return;
+  const dim3 grid((uint32_t)div_up(g.M, (uint64_t)TM), (uint32_t)(g.N / TN), (uint32_t)splits);
+  k_mm<AL, BL, EP, WN, KB><<<grid, 256, 0, c->stream>>>(g);
 }
 
-// out[m][n] = sum over splits s = 0..S-1 (in this order) of part[s][m][n]
-__global__ void k_split_sum(const float* __restrict__ part, int S, uint64_t MN, int N, float* __restrict__ out,
-                            int64_t o_sm) {
-  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= MN) return;
-  float t = 0.f;
-  for (int s = 0; s < S; ++s) t += part[(uint64_t)s * MN + i];
-  out[(int64_t)(i / N) * o_sm + (int64_t)(i % N)] = t;
-}
-
-// Weight gradients: M x N is small and K (the rows of the batch) is long, so
-// the K range is split over enough workgroups to fill the device; the partial
-// tiles are summed in a fixed order (deterministic, no atomics).
-constexpr int kSplitTiles = 2048;     // workgroups to aim for
-constexpr int kMinKPer = 128;         // at least 16 MFMA steps per workgroup
-uint64_t split_part_floats(uint64_t M, int N, int K) {
-  const uint64_t tiles = div_up(M, 32) * (uint64_t)(N / 32);
-  uint64_t S = std::max<uint64_t>(1, kSplitTiles / tiles);
-  S = std::min<uint64_t>(S, std::max(1, K / kMinKPer));
-  return S * M * N;
-}
-
-template <int AM>
-void gemm_wgrad(svd_ctx* c, uint64_t M, int N, int K, const GemmArgs& g0, float* part) {
-  GemmArgs g = g0;
-  const uint64_t tiles = div_up(M, 32) * (uint64_t)(N / 32);
-  int S = (int)std::max<uint64_t>(1, kSplitTiles / tiles);
-  S = std::min(S, std::max(1, K / kMinKPer));
+// Weight gradients: M x N is small and K (the rows of the batch) long, so K
+// is split over about kWgradBlocks workgroups; each split writes its partial
+// tile and the optimizer (or the gradient read-out) sums the splits in order
+// (deterministic, no atomics).
+constexpr int kWgradKB = 128;
+constexpr uint64_t kWgradBlocks = 512;
+constexpr int kMaxSplits = 32;
+struct Split { int S, kPer; };
+Split wgrad_split(uint64_t M, int N, int K, int TM, int TN) {
+  const uint64_t tiles = div_up(M, (uint64_t)TM) * (uint64_t)(N / TN);
+  const int maxS = M * (uint64_t)N <= 16384 ? 2 * kMaxSplits : kMaxSplits;   // small partials: split further
+  int S = (int)std::min<uint64_t>(maxS, std::max<uint64_t>(1, kWgradBlocks / tiles));
+  S = std::min(S, std::max(1, (K + kWgradKB - 1) / kWgradKB));
   int kPer = (K + S - 1) / S;
-  kPer = (kPer + 7) / 8 * 8;
-  S = (K + kPer - 1) / kPer;
-  float* out = g.outf;
-  const int64_t osm = g.o_sm;
-  g.N = N; g.K = K;
-  g.kPer = kPer;
-  g.outf = part;
-  g.o_sm = N;
-  g.o_sz = (int64_t)M * N;
-  dim3 grid(div_up(M, 32), N / 32, S);
-  k_gemm<AM, E_F32><<<grid, 64, 0, c->stream>>>(g);
-  k_split_sum<<<div_up(M * N, 256), 256, 0, c->stream>>>(part, S, M * N, N, out, osm);
+  kPer = (kPer + kWgradKB - 1) / kWgradKB * kWgradKB;
+  return Split{(K + kPer - 1) / kPer, kPer};
 }
 
-// Column sums of a bf16 [R x N] matrix (bias gradients), one block per
-// column, fixed reduction tree.
-__global__ void __launch_bounds__(256) k_colsum_bf16(const uint16_t* __restrict__ x, uint64_t R, int N,
-                                                     float* __restrict__ out) {
-  __shared__ float red[256];
-  const int n = blockIdx.x;
+// The split partial gradients of the three layers.
+struct GradParts {
+  const float* w[3];
+  const float* b[3];
+  uint32_t S[3];
+};
+
+__device__ __forceinline__ float grad_sum(const GradParts& gp, uint64_t i) {
+  constexpr uint64_t offW[3] = {OFF_W1, OFF_W2, OFF_W3}, offB[3] = {OFF_B1, OFF_B2, OFF_B3};
+  constexpr uint64_t dout[3] = {D1, D2, D3}, din[3] = {D0, D1, D2};
+  const int l = i < OFF_W2 ? 0 : (i < OFF_W3 ? 1 : 2);
+  const bool isB = i >= offB[l];
+  const float* p = isB ? gp.b[l] + (i - offB[l]) : gp.w[l] + (i - offW[l]);
+  const uint64_t stride = isB ? dout[l] : din[l] * dout[l];
+  // splits in order; eight loads in flight at a time
   float s = 0.f;
-  for (uint64_t r = threadIdx.x; r < R; r += 256) s += bf2f(x[r * N + n]);
-  red[threadIdx.x] = s;
-  __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
-    __syncthreads();
+  uint32_t z = 0;
+  for (; z + 8 <= gp.S[l]; z += 8) {
+    float x[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) x[q] = p[(z + q) * stride];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) s += x[q];
   }
-  if (threadIdx.x == 0) out[n] = red[0];
+  for (; z < gp.S[l]; ++z) s += p[z * stride];
+  return s;
 }
 
-// Pair loss and its gradient w.r.t. the embeddings (rounded to bf16).
-__global__ void k_pair_loss(const float* __restrict__ e, const uint32_t* __restrict__ labels, uint64_t P,
-                            uint16_t* __restrict__ dE, float* __restrict__ lossOut, const uint32_t* __restrict__ step,
-                            float* __restrict__ lossHist) {
-  uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= P) return;
-  const float* ea = e + (2 * p) * D3;
-  const float* eb = e + (2 * p + 1) * D3;
-  float diff[D3];
-  float d = 0.f;
-#pragma unroll
-  for (int k = 0; k < D3; ++k) { diff[k] = ea[k] - eb[k]; d += diff[k] * diff[k]; }
-  const float y = fminf((float)labels[p], kHCap);
-  const float invP = 1.0f / (float)P;
-  const float g = 2.f * (d - y) * invP;
-#pragma unroll
-  for (int k = 0; k < D3; ++k) {
-    const uint16_t v = f2bf(2.f * g * diff[k]);
-    dE[(2 * p) * D3 + k] = v;
-    dE[(2 * p + 1) * D3 + k] = v ^ 0x8000u;     // bf16(-x) = -bf16(x)
-  }
-  const float l = (d - y) * (d - y) * invP;
-  if (lossOut) atomicAdd(lossOut, l);
-  if (lossHist) atomicAdd(&lossHist[*step], l);
+__global__ void k_grad_sum(GradParts gp, float* __restrict__ grads) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < N_PARAMS) grads[i] = grad_sum(gp, i);
 }
 
-__global__ void k_adam(float* __restrict__ w, const float* __restrict__ g, float* __restrict__ m,
-                       float* __restrict__ v, uint16_t* __restrict__ wb, const uint32_t* __restrict__ step, float lr) {
+// bf16 copies of weight parameter i (a no-op for the biases, which stay fp32):
+// row-major at wb, transposed at wb + N_WB.
+__device__ __forceinline__ void store_wb(uint16_t* __restrict__ wb, uint64_t i, float w) {
+  constexpr uint64_t offW[3] = {OFF_W1, OFF_W2, OFF_W3}, offB[3] = {OFF_B1, OFF_B2, OFF_B3};
+  constexpr uint64_t offWb[3] = {WB_W1, WB_W2, WB_W3};
+  constexpr uint64_t dout[3] = {D1, D2, D3}, din[3] = {D0, D1, D2};
+  const int l = i < OFF_W2 ? 0 : (i < OFF_W3 ? 1 : 2);
+  if (i >= offB[l]) return;
+  const uint64_t j = i - offW[l];
+  const uint16_t v = f2bf(w);
+  wb[offWb[l] + j] = v;
+  wb[N_WB + offWb[l] + (j % dout[l]) * din[l] + j / dout[l]] = v;
+}
+
+// Adam on the gradients summed from the split partials, at step
+// step0 + *step + 1 (bias correction).
+__global__ void k_adam(float* __restrict__ w, GradParts gp, float* __restrict__ m, float* __restrict__ v,
+                       uint16_t* __restrict__ wb, const uint32_t* __restrict__ step, uint32_t step0, float lr) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= N_PARAMS) return;
   const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
-  const float t = (float)(*step + 1u);
-  const float gi = g[i];
+  const float t = (float)(step0 + *step + 1u);
+  const float gi = grad_sum(gp, i);
   const float mi = b1 * m[i] + (1.f - b1) * gi;
   const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
   m[i] = mi; v[i] = vi;
@@ -220,21 +389,15 @@ __global__ void k_adam(float* __restrict__ w, const float* __restrict__ g, float
   const float vh = vi / (1.f - powf(b2, t));
   const float wn = w[i] - lr * mh / (sqrtf(vh) + eps);
   w[i] = wn;
-  // bf16 copies of the weight matrices (the biases stay fp32)
-  if (i < OFF_B1) wb[WB_W1 + i] = f2bf(wn);
-  else if (i >= OFF_W2 && i < OFF_B2) wb[WB_W2 + (i - OFF_W2)] = f2bf(wn);
-  else if (i >= OFF_W3 && i < OFF_B3) wb[WB_W3 + (i - OFF_W3)] = f2bf(wn);
+  store_wb(wb, i, wn);
 }
 
 __global__ void k_refresh_wb(const float* __restrict__ w, uint16_t* __restrict__ wb) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < OFF_B1) wb[WB_W1 + i] = f2bf(w[i]);
-  else if (i >= OFF_W2 && i < OFF_B2) wb[WB_W2 + (i - OFF_W2)] = f2bf(w[i]);
-  else if (i >= OFF_W3 && i < OFF_B3) wb[WB_W3 + (i - OFF_W3)] = f2bf(w[i]);
+  if (i < N_PARAMS) store_wb(wb, i, w[i]);
 }
 
 __global__ void k_set_u32(uint32_t* d, uint32_t v) { *d = v; }
-__global__ void k_inc_u32(uint32_t* d) { *d += 1u; }
 __global__ void k_zero_f32(float* d, uint64_t n) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) d[i] = 0.f;
@@ -267,11 +430,14 @@ __global__ void k_inverse_perm(const uint32_t* __restrict__ order, uint64_t n, u
 
 // Batch of P pairs: near_pct percent are neighbours (+-1..8) in the near-pair
 // ordering, the rest random.  Rows 2p, 2p+1 get the two signatures; label =
-// Hamming distance.
+// Hamming distance.  The batch number *step is copied to *cur for the later
+// kernels of the step (the first layer's GEMM advances *step).
 __global__ void k_sample_batch(const uint32_t* __restrict__ sig, uint64_t n, const uint32_t* __restrict__ order,
                                const uint32_t* __restrict__ pos, uint64_t seed, const uint32_t* __restrict__ step,
-                               uint64_t P, uint32_t nearPct, uint32_t* __restrict__ X, uint32_t* __restrict__ labels) {
+                               uint64_t P, uint32_t nearPct, uint32_t* __restrict__ X, uint32_t* __restrict__ labels,
+                               uint32_t* __restrict__ cur) {
   const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p == 0) *cur = *step;
   if (p >= P) return;
   const uint64_t gidx = (uint64_t)(*step) * P + p;
   const uint64_t z1 = mix64(seed ^ ((gidx + 1) * 0x9E3779B97F4A7C15ull));
@@ -305,71 +471,97 @@ __global__ void k_sample_batch(const uint32_t* __restrict__ sig, uint64_t n, con
 struct Work {
   uint64_t R = 0;
   uint16_t *h1 = nullptr, *h2 = nullptr, *dE = nullptr, *dz2 = nullptr, *dz1 = nullptr;
-  float* e = nullptr;
-  float* part = nullptr;   // split-K partial weight gradients
+  Split sp[3] = {};            // weight-gradient splits of layers 1..3
+  float* partW[3] = {};
+  float* partB[3] = {};
   void alloc(svd_ctx* c, uint64_t rows) {
     R = rows;
     h1 = dalloc<uint16_t>(c, R * D1); h2 = dalloc<uint16_t>(c, R * D2);
     dE = dalloc<uint16_t>(c, R * D3); dz2 = dalloc<uint16_t>(c, R * D2); dz1 = dalloc<uint16_t>(c, R * D1);
-    e = dalloc<float>(c, R * D3);
-    const uint64_t np = std::max(split_part_floats(D0, D1, (int)R),
-                                 std::max(split_part_floats(D1, D2, (int)R), split_part_floats(D2, D3, (int)R)));
-    part = dalloc<float>(c, np);
+    const int din[3] = {D0, D1, D2}, dout[3] = {D1, D2, D3};
+    for (int l = 0; l < 3; ++l) {
+      const int wn = dout[l] % 64 ? 1 : 2;
+      sp[l] = wgrad_split(din[l], dout[l], (int)R, 128 / wn, 32 * wn);
+      partW[l] = dalloc<float>(c, (uint64_t)sp[l].S * din[l] * dout[l]);
+      partB[l] = dalloc<float>(c, (uint64_t)sp[l].S * dout[l]);
+    }
+  }
+  GradParts parts() const {
+    GradParts gp;
+    for (int l = 0; l < 3; ++l) { gp.w[l] = partW[l]; gp.b[l] = partB[l]; gp.S[l] = (uint32_t)sp[l].S; }
+    return gp;
   }
   void release(svd_ctx* c) {
-    dfree(c, h1); dfree(c, h2); dfree(c, dE); dfree(c, dz2); dfree(c, dz1); dfree(c, e); dfree(c, part);
+    dfree(c, h1); dfree(c, h2); dfree(c, dE); dfree(c, dz2); dfree(c, dz1);
+    for (int l = 0; l < 3; ++l) { dfree(c, partW[l]); dfree(c, partB[l]); }
   }
 };
 
-void forward(svd_mlp* m, const uint32_t* X, uint64_t rows, uint16_t* h1, uint16_t* h2, float* e) {
+// Layers 1 and 2 over `rows` signatures: h1, h2 (bf16).  The weights are
+// read transposed ([out][in], k-contiguous).
+void forward12(svd_mlp* m, const uint32_t* X, uint64_t rows, uint16_t* h1, uint16_t* h2, uint32_t* bump = nullptr) {
   svd_ctx* c = m->ctx;
-  GemmArgs g = {};
-  g.mValid = rows;
-  g.abits = X; g.b = m->wb + WB_W1; g.b_sk = D1; g.b_sn = 1; g.bias = m->params + OFF_B1; g.outb = h1; g.o_sm = D1;
-  gemm<A_BITS, E_BIAS_RELU_BF16>(c, rows, D1, D0, g);
+  const uint16_t* wt = m->wb + N_WB;
+  MmArgs g = {};
+  g.M = rows; g.N = D1; g.K = D0; g.bump = bump;
+  g.abits = X; g.b = wt + WB_W1; g.ldb = D0; g.bias = m->params + OFF_B1; g.outb = h1; g.ldo = D1;
+  mm<AL_BITS, BL_NK, E_BIAS_RELU_BF16, 2, 128>(c, g);
   g = {};
-  g.mValid = rows;
-  g.a = h1; g.a_sm = D1; g.a_sk = 1; g.b = m->wb + WB_W2; g.b_sk = D2; g.b_sn = 1; g.bias = m->params + OFF_B2;
-  g.outb = h2; g.o_sm = D2;
-  gemm<A_BF16, E_BIAS_RELU_BF16>(c, rows, D2, D1, g);
-  g = {};
-  g.mValid = rows;
-  g.a = h2; g.a_sm = D2; g.a_sk = 1; g.b = m->wb + WB_W3; g.b_sk = D3; g.b_sn = 1; g.bias = m->params + OFF_B3;
-  g.outf = e; g.o_sm = D3;
-  gemm<A_BF16, E_BIAS_F32>(c, rows, D3, D2, g);
+  g.M = rows; g.N = D2; g.K = D1;
+  g.a = h1; g.lda = D1; g.b = wt + WB_W2; g.ldb = D1; g.bias = m->params + OFF_B2; g.outb = h2; g.ldo = D2;
+  mm<AL_MK, BL_NK, E_BIAS_RELU_BF16, 2, 128>(c, g);
 }
 
-// Backward from dE (bf16, R x 32) into m->grads.
+// Forward: the embeddings e (fp32) of `rows` signatures.
+void forward(svd_mlp* m, const uint32_t* X, uint64_t rows, uint16_t* h1, uint16_t* h2, float* e) {
+  forward12(m, X, rows, h1, h2);
+  MmArgs g = {};
+  g.M = rows; g.N = D3; g.K = D2;
+  g.a = h2; g.lda = D2; g.b = m->wb + N_WB + WB_W3; g.ldb = D2; g.bias = m->params + OFF_B3; g.outf = e; g.ldo = D3;
+  mm<AL_MK, BL_NK, E_BIAS_F32, 1, 128>(m->ctx, g);
+}
+
+// Forward over the R = 2P rows of a pair batch, ending in the pair loss: dE
+// (bf16) into w, the batch loss added to loss[*lossStep] (loss[0] if null).
+void forward_loss(svd_mlp* m, const uint32_t* X, const uint32_t* labels, const Work& w, float* loss,
+                  const uint32_t* lossStep, uint32_t* bump = nullptr) {
+  forward12(m, X, w.R, w.h1, w.h2, bump);
+  MmArgs g = {};
+  g.M = w.R; g.N = D3; g.K = D2;
+  g.a = w.h2; g.lda = D2; g.b = m->wb + N_WB + WB_W3; g.ldb = D2; g.bias = m->params + OFF_B3;
+  g.outb = w.dE; g.ldo = D3; g.labels = labels; g.loss = loss; g.lossStep = lossStep;
+  mm<AL_MK, BL_NK, E_PAIR_LOSS, 1, 128>(m->ctx, g);
+}
+
+// Backward from dE (bf16, R x 32) into the split partial gradients of w.
 void backward(svd_mlp* m, const uint32_t* X, const Work& w) {
   svd_ctx* c = m->ctx;
   const uint64_t R = w.R;
-  GemmArgs g = {};
-  // dW3 = h2^T dE  (128 x 32), K = R
-  g.mValid = D2; g.a = w.h2; g.a_sm = 1; g.a_sk = D2; g.b = w.dE; g.b_sk = D3; g.b_sn = 1;
-  g.outf = m->grads + OFF_W3; g.o_sm = D3;
-  gemm_wgrad<A_BF16>(c, D2, D3, (int)R, g, w.part);
-  k_colsum_bf16<<<D3, 256, 0, c->stream>>>(w.dE, R, D3, m->grads + OFF_B3);
-  // dz2 = (dE W3^T) * (h2 > 0)  (R x 128), K = 32
+  MmArgs g = {};
+  // dW3 = h2^T dE (128 x 32), db3 = column sums of dE; K = R
+  g.M = D2; g.N = D3; g.K = (int)R; g.kPer = w.sp[2].kPer;
+  g.a = w.h2; g.lda = D2; g.b = w.dE; g.ldb = D3; g.part = w.partW[2]; g.partBias = w.partB[2];
+  mm<AL_KM, BL_KN, E_PART, 1, kWgradKB>(c, g, w.sp[2].S);
+  // dz2 = (dE W3^T) * (h2 > 0) (R x 128), K = 32
   g = {};
-  g.mValid = R; g.a = w.dE; g.a_sm = D3; g.a_sk = 1; g.b = m->wb + WB_W3; g.b_sk = 1; g.b_sn = D3;
-  g.mask = w.h2; g.mask_sm = D2; g.outb = w.dz2; g.o_sm = D2;
-  gemm<A_BF16, E_MASK_BF16>(c, R, D2, D3, g);
-  // dW2 = h1^T dz2  (256 x 128), K = R
+  g.M = R; g.N = D2; g.K = D3;
+  g.a = w.dE; g.lda = D3; g.b = m->wb + WB_W3; g.ldb = D3; g.mask = w.h2; g.ldmask = D2; g.outb = w.dz2; g.ldo = D2;
+  mm<AL_MK, BL_NK, E_MASK_BF16, 2, 32>(c, g);
+  // dW2 = h1^T dz2 (256 x 128), db2; K = R
   g = {};
-  g.mValid = D1; g.a = w.h1; g.a_sm = 1; g.a_sk = D1; g.b = w.dz2; g.b_sk = D2; g.b_sn = 1;
-  g.outf = m->grads + OFF_W2; g.o_sm = D2;
-  gemm_wgrad<A_BF16>(c, D1, D2, (int)R, g, w.part);
-  k_colsum_bf16<<<D2, 256, 0, c->stream>>>(w.dz2, R, D2, m->grads + OFF_B2);
-  // dz1 = (dz2 W2^T) * (h1 > 0)  (R x 256), K = 128
+  g.M = D1; g.N = D2; g.K = (int)R; g.kPer = w.sp[1].kPer;
+  g.a = w.h1; g.lda = D1; g.b = w.dz2; g.ldb = D2; g.part = w.partW[1]; g.partBias = w.partB[1];
+  mm<AL_KM, BL_KN, E_PART, 2, kWgradKB>(c, g, w.sp[1].S);
+  // dz1 = (dz2 W2^T) * (h1 > 0) (R x 256), K = 128
   g = {};
-  g.mValid = R; g.a = w.dz2; g.a_sm = D2; g.a_sk = 1; g.b = m->wb + WB_W2; g.b_sk = 1; g.b_sn = D2;
-  g.mask = w.h1; g.mask_sm = D1; g.outb = w.dz1; g.o_sm = D1;
-  gemm<A_BF16, E_MASK_BF16>(c, R, D1, D2, g);
-  // dW1 = x^T dz1  (512 x 256), K = R
+  g.M = R; g.N = D1; g.K = D2;
+  g.a = w.dz2; g.lda = D2; g.b = m->wb + WB_W2; g.ldb = D2; g.mask = w.h1; g.ldmask = D1; g.outb = w.dz1; g.ldo = D1;
+  mm<AL_MK, BL_NK, E_MASK_BF16, 2, 128>(c, g);
+  // dW1 = x^T dz1 (512 x 256), db1; K = R
   g = {};
-  g.mValid = D0; g.abits = X; g.b = w.dz1; g.b_sk = D1; g.b_sn = 1; g.outf = m->grads + OFF_W1; g.o_sm = D1;
-  gemm_wgrad<A_BITS_T>(c, D0, D1, (int)R, g, w.part);
-  k_colsum_bf16<<<D1, 256, 0, c->stream>>>(w.dz1, R, D1, m->grads + OFF_B1);
+  g.M = D0; g.N = D1; g.K = (int)R; g.kPer = w.sp[0].kPer;
+  g.abits = X; g.b = w.dz1; g.ldb = D1; g.part = w.partW[0]; g.partBias = w.partB[0];
+  mm<AL_BITS_T, BL_KN, E_PART, 2, kWgradKB>(c, g, w.sp[0].S);
 }
 
 }  // namespace
@@ -411,7 +603,7 @@ void mlp_alloc(svd_mlp* m) {
   m->grads = dalloc<float>(c, N_PARAMS);
   m->m = dalloc<float>(c, N_PARAMS);
   m->v = dalloc<float>(c, N_PARAMS);
-  m->wb = dalloc<uint16_t>(c, N_WB);
+  m->wb = dalloc<uint16_t>(c, 2 * N_WB);
 }
 
 void mlp_release(svd_mlp* m) {
@@ -454,9 +646,9 @@ void mlp_loss_grad(svd_mlp* m, const uint32_t* X, const uint32_t* labels, uint64
   w.alloc(c, 2 * P);
   float* loss = dalloc<float>(c, 1);
   k_zero_f32<<<1, 32, 0, c->stream>>>(loss, 1);
-  forward(m, X, 2 * P, w.h1, w.h2, w.e);
-  k_pair_loss<<<div_up(P, 256), 256, 0, c->stream>>>(w.e, labels, P, w.dE, loss, nullptr, nullptr);
+  forward_loss(m, X, labels, w, loss, nullptr);
   backward(m, X, w);
+  k_grad_sum<<<div_up(N_PARAMS, 256), 256, 0, c->stream>>>(w.parts(), m->grads);
   check_launch("mlp loss/grad");
   SVD_HIP(hipMemcpyAsync(lossHost, loss, 4, hipMemcpyDeviceToHost, c->stream));
   SVD_HIP(hipMemcpyAsync(gradsHost, m->grads, N_PARAMS * 4, hipMemcpyDeviceToHost, c->stream));
@@ -519,26 +711,23 @@ void mlp_train(svd_mlp* m, svd_octree* o, const svd_mlp_train_params* p, svd_mlp
   w.alloc(c, R);
   uint32_t* X = dalloc<uint32_t>(c, R * 16);
   uint32_t* labels = dalloc<uint32_t>(c, P);
-  uint32_t* dstep = dalloc<uint32_t>(c, 1);
+  uint32_t* dstep = dalloc<uint32_t>(c, 2);   // batch counter, the current step's number
   float* lossHist = dalloc<float>(c, p->steps);
   k_zero_f32<<<div_up(p->steps, 256), 256, 0, c->stream>>>(lossHist, p->steps);
   k_set_u32<<<1, 1, 0, c->stream>>>(dstep, 0u);
   check_launch("train setup");
+  // Adam's step index continues across calls (step0 + the batch counter),
+  // the loss history starts at 0
   const uint32_t step0 = (uint32_t)m->steps;
-  // Adam's step index continues across calls: the counter starts at step0
-  // for the bias correction, the loss history at 0.
-  uint32_t* dadam = dalloc<uint32_t>(c, 1);
-  k_set_u32<<<1, 1, 0, c->stream>>>(dadam, step0);
   SVD_HIP(hipStreamSynchronize(c->stream));
 
   auto enqueue_step = [&]() {
-    k_sample_batch<<<div_up(P, 256), 256, 0, c->stream>>>(sig, n, order, pos, p->seed, dstep, P, nearPct, X, labels);
-    forward(m, X, R, w.h1, w.h2, w.e);
-    k_pair_loss<<<div_up(P, 256), 256, 0, c->stream>>>(w.e, labels, P, w.dE, nullptr, dstep, lossHist);
+    k_sample_batch<<<div_up(P, 256), 256, 0, c->stream>>>(sig, n, order, pos, p->seed, dstep, P, nearPct, X, labels,
+                                                          dstep + 1);
+    forward_loss(m, X, labels, w, lossHist, dstep + 1, dstep);
     backward(m, X, w);
-    k_adam<<<div_up(N_PARAMS, 256), 256, 0, c->stream>>>(m->params, m->grads, m->m, m->v, m->wb, dadam, p->lr);
-    k_inc_u32<<<1, 1, 0, c->stream>>>(dstep);
-    k_inc_u32<<<1, 1, 0, c->stream>>>(dadam);
+    k_adam<<<div_up(N_PARAMS, 256), 256, 0, c->stream>>>(m->params, w.parts(), m->m, m->v, m->wb, dstep + 1, step0,
+                                                         p->lr);
   };
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -568,7 +757,7 @@ void mlp_train(svd_mlp* m, svd_octree* o, const svd_mlp_train_params* p, svd_mlp
     st->pairs = P * p->steps;
   }
   w.release(c);
-  dfree(c, X); dfree(c, labels); dfree(c, dstep); dfree(c, dadam); dfree(c, lossHist);
+  dfree(c, X); dfree(c, labels); dfree(c, dstep); dfree(c, lossHist);
   dfree(c, sig); dfree(c, keys); dfree(c, vals); dfree(c, keys2); dfree(c, order); dfree(c, pos); dfree(c, bitpos);
 }
 
