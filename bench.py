@@ -536,6 +536,8 @@ def main():
     nray = args.width * args.height
     prim = torch.zeros(nray * 6, dtype=torch.int32, device="cuda:0")
     shad = torch.zeros(nray * 6, dtype=torch.int32, device="cuda:0")
+    hprim = torch.zeros(nray * 4, dtype=torch.float32, device="cuda:0")
+    hshad = torch.zeros(nray * 4, dtype=torch.float32, device="cuda:0")
     torch.cuda.synchronize()
 
     def step():
@@ -546,7 +548,7 @@ def main():
         sc = S.Scene.from_octree(g, S.FMT_SSVDAG)
         ctx.sync()
         t2 = time.perf_counter()
-        sc.trace_device(params, prim.data_ptr(), shad.data_ptr())
+        sc.trace_hits_device(params, hprim.data_ptr(), hshad.data_ptr())
         ctx.sync()
         t3 = time.perf_counter()
         trace_ms = ctx.kernel_ms("trace")
@@ -569,9 +571,21 @@ def main():
     torch.cuda.synchronize()
     t_total = time.perf_counter() - t_start
 
-    # per-ray stats of the last frame
+    # per-ray stats of the same frame from one untimed run of the instrumented
+    # kernel (descent-path hash and fetched bytes per ray); its records equal
+    # the timed kernel's
+    g = build_step(S, ctx, dcodes.data_ptr(), n, L)
+    b_build, f_bytes = build_bytes(S, g, L, n)
+    sc = S.Scene.from_octree(g, S.FMT_SSVDAG)
+    sc.trace_device(params, prim.data_ptr(), shad.data_ptr())
+    ctx.sync()
+    instr_ms = ctx.kernel_ms("trace")
     pr = prim.cpu().numpy().view(S.RAY_DTYPE)
     sh = shad.cpu().numpy().view(S.RAY_DTYPE)
+    hp = hprim.cpu().numpy().reshape(-1, 4)
+    same = all(np.array_equal(hp[:, i], pr[k]) for i, k in enumerate(("t", "level", "iters", "node")))
+    if not same:
+        raise SystemExit("bench: the uninstrumented frame differs from the instrumented one")
     n_shadow = int((pr["t"] >= 0).sum())
     rays = nray + n_shadow
     bytes_fetched = int(pr["bytes"].astype(np.int64).sum() + sh["bytes"].astype(np.int64).sum())
@@ -580,12 +594,9 @@ def main():
     gvox = n / build_s / 1e9
     mrays = rays / (trace_kernel_ms * 1e-3) / 1e6
     achieved = bytes_fetched / (trace_kernel_ms * 1e-3) / 1e9
+    build_achieved = b_build / build_s / 1e9
 
     # the viewer's default frame: minimum-depth pre-pass at 1/8 resolution, then the same rays
-    g = build_step(S, ctx, dcodes.data_ptr(), n, L)
-    b_build, f_bytes = build_bytes(S, g, L, n)
-    build_achieved = b_build / build_s / 1e9
-    sc = S.Scene.from_octree(g, S.FMT_SSVDAG)
     dpf = W.projection_factor(args.height, screen_divisor=8.0)
     tbm = []
     for _ in range(1 + max(args.steps, 3)):
@@ -643,11 +654,12 @@ def main():
         "build_ms": round(build_s * 1e3, 3),
         "traversal_mrays_s": round(mrays, 2),
         "trace_kernel_ms": round(trace_kernel_ms, 4),
+        "trace_kernel_ms_instrumented": round(instr_ms, 4),
         "traversal_beam": beam,
         "viewer": viewer,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": measured_traffic(L, args),
-                     "kernel": "k_trace<SSVDAG>", "algorithmic_bytes": bytes_fetched},
+                     "kernel": "k_trace_hits<SSVDAG>", "algorithmic_bytes": bytes_fetched},
         "build_roofline": {"bound": "hbm", "achieved": round(build_achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                            "frac": round(build_achieved / HBM_PEAK_GBS, 5), "algorithmic_bytes": b_build,
                            "file_bytes": f_bytes, "ms": round(build_s * 1e3, 3)},

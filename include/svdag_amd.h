@@ -215,6 +215,16 @@ int svd_scene_free(svd_scene* sc);
 /* One frame: width*height primary rays (+ shadow rays).  primary / shadow are
  * DEVICE arrays of width*height svd_ray_result; shadow may be NULL.          */
 int svd_trace(svd_scene* sc, const svd_trace_params* p, svd_ray_result* primary, svd_ray_result* shadow);
+/* svd_trace_hits: the same frame without the path/bytes instrumentation:
+ * per ray trace_ray's vec4 alone (octree_dda.frag.glsl:528-537), 4 floats
+ * (t, level, iters, node), the values svd_trace gives.  primary: DEVICE array
+ * of width*height vec4.  shadow (DEVICE, optional, width*height vec4) is
+ * written only for the pixels that cast a shadow ray (p->shadows set and the
+ * primary t >= 0); its other entries are left as they are.  Always uses the
+ * tiled launch.  No reference counterpart as an entry point: it is the
+ * traversal of octree_dda.frag.glsl as the viewer runs it, whose results the
+ * reference keeps on the GPU.                                                */
+int svd_trace_hits(svd_scene* sc, const svd_trace_params* p, float* primary, float* shadow);
 /* svd_trace_beam: the same frame with the viewer's minimum-depth optimisation
  * (OctreeDDARenderer::draw with _useMinDepthOptimization, its default,
  * octree_dda_renderer.cpp:389-408): a DEPTH_MODE pre-pass over a

@@ -151,6 +151,7 @@ def lib():
         "svd_scene_from_octree": (i32, [vp, i32, P(vp)]),
         "svd_scene_free": (i32, [vp]),
         "svd_trace": (i32, [vp, P(TraceParams), vp, vp]),
+        "svd_trace_hits": (i32, [vp, P(TraceParams), vp, vp]),
         "svd_trace_beam": (i32, [vp, P(TraceParams), C.c_float, vp, vp, vp]),
         "svd_render": (i32, [vp, P(TraceParams), P(ViewParams), C.c_float, vp, vp]),
         "svd_trace_ao": (i32, [vp, P(TraceParams), C.c_uint32, vp, vp]),
@@ -206,7 +207,7 @@ EXPORTED = ["svd_ctx_create", "svd_ctx_destroy", "svd_last_error", "svd_ctx_sync
             "svd_reduce_dag",
             "svd_reduce_sym", "svd_cross_level_merge", "svd_octree_state", "svd_octree_levels",
             "svd_octree_counts", "svd_level_size", "svd_get_level", "svd_encode", "svd_encode_formats", "svd_encoded_copy",
-            "svd_encoded_device", "svd_scene_create", "svd_scene_from_octree", "svd_scene_free", "svd_trace", "svd_trace_beam",
+            "svd_encoded_device", "svd_scene_create", "svd_scene_from_octree", "svd_scene_free", "svd_trace", "svd_trace_hits", "svd_trace_beam",
             "svd_render", "svd_trace_ao", "svd_render_noalloc",
             "svd_last_kernel_ms", "svd_refcount_order", "svd_refcount_order_ranges", "svd_attr_bittree_encode", "svd_lossy_merge",
             "svd_lossy_signatures", "svd_mlp_create", "svd_mlp_free", "svd_mlp_get_params", "svd_mlp_set_params",
@@ -813,6 +814,12 @@ class Scene:
         """AO visibility per pixel (float, -1 without a primary hit) into a device buffer."""
         self.ctx.check(self.ctx.L.svd_trace_ao(self.h, C.byref(params), int(num_rays), C.c_void_p(vis_ptr),
                                                C.c_void_p(prim_ptr) if prim_ptr else None))
+
+    def trace_hits_device(self, params: TraceParams, prim_ptr: int, shad_ptr: int | None):
+        """Trace one frame without instrumentation: 4 floats (t, level, iters, node) per
+        ray; shadow entries are written only where a shadow ray is cast."""
+        self.ctx.check(self.ctx.L.svd_trace_hits(self.h, C.byref(params), C.c_void_p(prim_ptr),
+                                                 C.c_void_p(shad_ptr) if shad_ptr else None))
 
     def trace_device(self, params: TraceParams, prim_ptr: int, shad_ptr: int | None):
         """Trace one frame into device buffers (e.g. torch tensors' data_ptr())."""

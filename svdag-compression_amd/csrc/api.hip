@@ -714,6 +714,14 @@ int svd_trace(svd_scene* s, const svd_trace_params* p, svd_ray_result* prim, svd
   });
 }
 
+int svd_trace_hits(svd_scene* s, const svd_trace_params* p, float* prim, float* shad) {
+  if (!s || !p || !prim) return SVD_ERR_ARG;
+  return guarded(s->ctx, [&] {
+    Timer t(s->ctx, "trace");
+    trace_frame_hits(s, p, prim, shad);
+  });
+}
+
 int svd_trace_beam(svd_scene* s, const svd_trace_params* p, float depth_projection_factor, svd_ray_result* prim,
                    svd_ray_result* shad, float* min_depth) {
   if (!s || !p || !prim) return SVD_ERR_ARG;

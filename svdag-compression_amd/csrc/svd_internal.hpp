@@ -227,6 +227,7 @@ void octree_from_svdag(svd_octree* o, const uint8_t* file, uint64_t bytes, bool 
 
 // ---- traversal (trace.hip) -----------------------------------------------------
 void trace_frame(svd_scene* s, const svd_trace_params* p, svd_ray_result* prim, svd_ray_result* shad);
+void trace_frame_hits(svd_scene* s, const svd_trace_params* p, float* prim, float* shad);
 // beam optimisation: depth pre-pass at 1/8 resolution, then the frame from getMinT(8)
 void trace_frame_beam(svd_scene* s, const svd_trace_params* p, float depthProjf, svd_ray_result* prim,
                       svd_ray_result* shad, float* minDepthOut);

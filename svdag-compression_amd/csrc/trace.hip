@@ -67,7 +67,10 @@ __device__ __forceinline__ Stack make_stack(uint32_t* mem, int depth, int lane) 
   return Stack{mem, mem + depth * kBT, kStackWords<FMT> == 3 ? mem + 2 * depth * kBT : nullptr, lane, 0};
 }
 
-template <int FMT>
+// INSTR: keep the per-ray descent-path hash and the fetched-byte count (the
+// test and byte-count variant); without it the two stay at their initial
+// values and cost no instructions.
+template <int FMT, bool INSTR = true>
 struct Tracer {
   const Dev& s;
   Stack st;
@@ -120,28 +123,35 @@ struct Tracer {
   __device__ __forceinline__ bool fetch_voxel_bit(const TS& ts) const {
     if (FMT != SVD_FMT_SSVDAG) return (ts.hdr & (1u << ts.cli)) != 0;
     if ((int)ts.level < s.innerLevels) return ((ts.hdr >> (ts.cli << 1)) & 3u) != 0;
-    const uint32_t word = ts.cli & 32u;
-    const uint32_t bm = 1u << (ts.cli & 31u);
-    return ((word == 0 ? ts.hdr : ts.leaf_hi) & bm) != 0;
+    // the leaf's 64 bits as one value: a select between the two words can
+    // become a select between their addresses, which keeps them in memory
+    const uint64_t bits = ((uint64_t)ts.leaf_hi << 32) | ts.hdr;
+    return ((bits >> ts.cli) & 1u) != 0;
   }
   __device__ __forceinline__ void fetch_data(TS& ts) {
-    if (FMT != SVD_FMT_SSVDAG) { ts.hdr = word((uint32_t)ts.node_index); bytes += 4; return; }
+    if (FMT != SVD_FMT_SSVDAG) { ts.hdr = word((uint32_t)ts.node_index); if (INSTR) bytes += 4; return; }
+    // both words assigned on both paths: stores of one field or the other
+    // could be merged into one store through a selected address, which keeps
+    // the two in scratch memory
+    uint32_t lo, hi = ts.leaf_hi;
     if ((int)ts.level < s.innerLevels) {
-      ts.hdr = half((uint32_t)(ts.node_index + (int)s.levelOffsets[ts.level]));
-      bytes += 2;
+      lo = half((uint32_t)(ts.node_index + (int)s.levelOffsets[ts.level]));
+      if (INSTR) bytes += 2;
     } else {
       uint2 v = make_uint2(0u, 0u);
       if ((uint32_t)ts.node_index < s.nLeaves) v = s.leaves[(uint32_t)ts.node_index];
       else bad = true;
-      bytes += 8;
-      ts.hdr = v.x; ts.leaf_hi = v.y;
+      if (INSTR) bytes += 8;
+      lo = v.x; hi = v.y;
     }
+    ts.hdr = lo;
+    ts.leaf_hi = hi;
   }
   __device__ __forceinline__ void fetch_child_index_in(TS& ts) {
     if (FMT != SVD_FMT_SSVDAG) {
       const int pos = __popc((ts.hdr & 0xFFu) >> ts.cli);
       ts.node_index = (int)word((uint32_t)(ts.node_index + pos));
-      bytes += 4;
+      if (INSTR) bytes += 4;
       if (FMT == SVD_FMT_USSVDAG) {
         const uint32_t b = ts.hdr >> ts.cli;
         ts.mm ^= ((b >> 8) & 1u) | ((b >> 15) & 2u) | ((b >> 22) & 4u);
@@ -157,12 +167,12 @@ struct Tracer {
               __popc((lo | hi) & above) + __popc(hi & above);
     const int cm = (int)((ts.hdr >> (ts.cli << 1)) & 3u);
     int cx = (int)half((uint32_t)off);
-    bytes += 2;
+    if (INSTR) bytes += 2;
     const uint32_t mf = ((uint32_t)cx >> 13) & 7u;
     cx &= -57345;
     if (cm > 1) {
       const int cy = (int)half((uint32_t)off + 1);
-      bytes += 2;
+      if (INSTR) bytes += 2;
       cx = (cx << 16) | cy;
       cx |= (cm & 1) << 29;
     }
@@ -238,8 +248,10 @@ struct Tracer {
     next_crossing(ts, o);
 #pragma unroll
     for (int a = 0; a < 3; ++a) ts.li[a] = ts.idx[a] & (ts.cns - 1);
-    path = (path ^ (uint32_t)ts.node_index) * 16777619u;
-    path = (path ^ ts.level) * 16777619u;
+    if (INSTR) {
+      path = (path ^ (uint32_t)ts.node_index) * 16777619u;
+      path = (path ^ ts.level) * 16777619u;
+    }
   }
 
   // Per-ray state between iterations of the traversal loop.
@@ -465,15 +477,20 @@ __device__ __forceinline__ float min_t(const MinDepth& md, uint32_t px, uint32_t
 
 // One pixel of one camera: primary ray, then the shadow ray of a hit.  With a
 // minimum-depth texture the primary ray starts at getMinT(8) - 1e-3
-// (octree_dda.frag.glsl:681).
-template <int FMT>
+// (octree_dda.frag.glsl:681).  INSTR writes svd_ray_result records (prim,
+// shad; a pixel without a shadow ray gets the -5 record); otherwise traceRay's
+// vec4 (t, level, iters, node) goes to primHit, and to shadHit only where a
+// shadow ray is cast.
+template <int FMT, bool INSTR = true>
 __device__ __forceinline__ void trace_pixel(const Dev& s, const svd_trace_params& p, uint32_t px, uint32_t py,
                                             uint32_t* stackMem, svd_ray_result* __restrict__ prim,
-                                            svd_ray_result* __restrict__ shad, MinDepth md = MinDepth{nullptr, 0, 0}) {
+                                            svd_ray_result* __restrict__ shad, MinDepth md = MinDepth{nullptr, 0, 0},
+                                            float4* __restrict__ primHit = nullptr,
+                                            float4* __restrict__ shadHit = nullptr) {
   const int depth = s.innerLevels + 1;
   const int lane = threadIdx.y * blockDim.x + threadIdx.x;
   Stack stk = make_stack<FMT>(stackMem, depth, lane);
-  Tracer<FMT> tr(s, stk);
+  Tracer<FMT, INSTR> tr(s, stk);
   float o[3], d[3];
   camera_ray(p, px, py, (float)p.width, (float)p.height, o, d);
   float res[4];
@@ -482,8 +499,9 @@ __device__ __forceinline__ void trace_pixel(const Dev& s, const svd_trace_params
   tr.trace(o, d, tmin, 1e30f, p.projection_factor, p.max_iters, p.draw_level, res, step);
   if (tr.bad) { res[0] = -6.f; res[1] = 0.f; res[3] = -1.f; }
   const size_t k = (size_t)py * p.width + px;
-  prim[k] = svd_ray_result{res[0], res[1], res[2], res[3], tr.path, tr.bytes};
-  if (!shad) return;
+  if (INSTR) prim[k] = svd_ray_result{res[0], res[1], res[2], res[3], tr.path, tr.bytes};
+  else primHit[k] = make_float4(res[0], res[1], res[2], res[3]);
+  if (INSTR ? !shad : !shadHit) return;
   svd_ray_result sr{-5.f, 0.f, 0.f, -1.f, 0u, 0u};
   if (p.shadows && res[0] >= 0.f) {
     const ShadowRay sh = shadow_ray(s, p, o, d, res, step);
@@ -494,9 +512,13 @@ __device__ __forceinline__ void trace_pixel(const Dev& s, const svd_trace_params
     tr.bad = false;
     tr.trace(sh.o, sh.d, sh.tmin, sh.tmax, sh.projf, p.max_iters, p.draw_level, sres, sstep);
     if (tr.bad) { sres[0] = -6.f; sres[1] = 0.f; sres[3] = -1.f; }
+    if (!INSTR) {
+      shadHit[k] = make_float4(sres[0], sres[1], sres[2], sres[3]);
+      return;
+    }
     sr = svd_ray_result{sres[0], sres[1], sres[2], sres[3], tr.path, tr.bytes};
   }
-  shad[k] = sr;
+  if (INSTR) shad[k] = sr;
 }
 
 // ---- viewer shading ---------------------------------------------------------
@@ -879,6 +901,18 @@ __global__ void __launch_bounds__(kBT) SVD_TRACE_OCC k_trace(Dev s, svd_trace_pa
   trace_pixel<FMT>(s, p, px, py, stackMem, prim, shad, md);
 }
 
+// The same frame without instrumentation: traceRay's vec4 per ray (16 bytes),
+// shadow records only where a shadow ray is cast.
+template <int FMT>
+__global__ void __launch_bounds__(kBT) SVD_TRACE_OCC k_trace_hits(Dev s, svd_trace_params p, float4* __restrict__ prim,
+                                                    float4* __restrict__ shad, MinDepth md) {
+  extern __shared__ uint32_t stackMem[];
+  const uint32_t px = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t py = blockIdx.y * blockDim.y + threadIdx.y;
+  if (px >= p.width || py >= p.height) return;
+  trace_pixel<FMT, false>(s, p, px, py, stackMem, nullptr, nullptr, md, prim, shad);
+}
+
 // DEPTH_MODE pre-pass (octree_dda.frag.glsl:862-872) over the (w x h)
 // texture: screenRes = (sw, sh), its own projection factor; a texel keeps the
 // hit distance when it is > 0 and the cleared 0 otherwise.
@@ -891,7 +925,7 @@ __global__ void __launch_bounds__(kBT) SVD_TRACE_OCC k_trace_depth(Dev s, svd_tr
   const uint32_t py = blockIdx.y * blockDim.y + threadIdx.y;
   if (px >= w || py >= h) return;
   const int lane = threadIdx.y * blockDim.x + threadIdx.x;
-  Tracer<FMT> tr(s, make_stack<FMT>(stackMem, s.innerLevels + 1, lane));
+  Tracer<FMT, false> tr(s, make_stack<FMT>(stackMem, s.innerLevels + 1, lane));
   float o[3], d[3];
   camera_ray(p, px, py, sw, sh, o, d);
   float res[4];
@@ -1111,6 +1145,28 @@ void trace_frame_impl(svd_scene* sc, const svd_trace_params* p, svd_ray_result* 
   }
   check_launch("k_trace");
   if (depth && depth != minDepthOut) dfree(c, depth);
+}
+
+// The uninstrumented frame (k_trace_hits, always one workgroup per pixel
+// tile): prim gets width*height vec4, shad (optional) the vec4 of every cast
+// shadow ray.
+void trace_frame_hits(svd_scene* sc, const svd_trace_params* p, float* prim, float* shad) {
+  svd_ctx* c = sc->ctx;
+  const Dev d = scene_dev(sc);
+  if (p->width == 0 || p->height == 0) return;
+  if (p->draw_level == 0) throw Error{SVD_ERR_ARG, "draw_level must be >= 1"};
+  const uint32_t tw = c->traceTileW, th = kBT / tw;
+  const dim3 grid(div_up(p->width, tw), div_up(p->height, th)), block(tw, th);
+  const size_t shmem = stack_bytes(sc, d);
+  const MinDepth md{nullptr, 0, 0};
+  float4* ph = reinterpret_cast<float4*>(prim);
+  float4* sh = reinterpret_cast<float4*>(shad);
+  switch (sc->fmt) {
+    case SVD_FMT_SVDAG: k_trace_hits<SVD_FMT_SVDAG><<<grid, block, shmem, c->stream>>>(d, *p, ph, sh, md); break;
+    case SVD_FMT_USSVDAG: k_trace_hits<SVD_FMT_USSVDAG><<<grid, block, shmem, c->stream>>>(d, *p, ph, sh, md); break;
+    default: k_trace_hits<SVD_FMT_SSVDAG><<<grid, block, shmem, c->stream>>>(d, *p, ph, sh, md); break;
+  }
+  check_launch("k_trace_hits");
 }
 
 namespace {

@@ -101,12 +101,37 @@ def frame_parity(ctx, g, fmt, file_bytes, p, name, beam_factor=0.0):
     ctx.sync()
     got_p = prim.cpu().numpy().view(S.RAY_DTYPE)
     got_s = shad.cpu().numpy().view(S.RAY_DTYPE)
+    if beam_factor <= 0:
+        hits_parity(sc, p, want_p, want_s, name)
     sc.free()
     hits = int((want_p["t"] >= 0).sum())
     say(f"  {name}: {n} primary rays, {hits} hits / shadow rays")
     assert hits > 0.05 * n
     np.testing.assert_array_equal(got_p, want_p, err_msg=f"{name}: primary rays")
     np.testing.assert_array_equal(got_s, want_s, err_msg=f"{name}: shadow rays")
+
+
+SENTINEL = 0x7FC0DEAD   # a NaN pattern no traversal result takes
+
+
+def hits_parity(sc, p, want_p, want_s, name):
+    """The uninstrumented frame (svd_trace_hits): the same (t, level, iters,
+    node) per primary ray, the shadow vec4 exactly where a shadow ray was cast
+    and the caller's contents elsewhere."""
+    import torch
+    n = p.width * p.height
+    prim = torch.full((n * 4,), SENTINEL, dtype=torch.int32, device="cuda")
+    shad = torch.full((n * 4,), SENTINEL, dtype=torch.int32, device="cuda")
+    torch.cuda.synchronize()
+    sc.trace_hits_device(p, prim.data_ptr(), shad.data_ptr())
+    sc.ctx.sync()
+    gp = prim.cpu().numpy().reshape(n, 4)
+    gs = shad.cpu().numpy().reshape(n, 4)
+    f4 = lambda r: np.stack([r[k] for k in ("t", "level", "iters", "node")], axis=1).astype(np.float32).view(np.int32)
+    np.testing.assert_array_equal(gp, f4(want_p), err_msg=f"{name}: uninstrumented primary rays")
+    cast = want_p["t"] >= 0 if p.shadows else np.zeros(n, bool)
+    np.testing.assert_array_equal(gs[cast], f4(want_s)[cast], err_msg=f"{name}: uninstrumented shadow rays")
+    assert (gs[~cast] == SENTINEL).all(), f"{name}: shadow entries written without a shadow ray"
 
 
 @pytest.fixture(scope="module")

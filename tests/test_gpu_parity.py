@@ -258,6 +258,11 @@ def test_trace_matches_shader_port(ctx, fmt):
     for k in ("t", "level", "iters", "node", "path", "bytes"):
         np.testing.assert_array_equal(got_p[k], want_p[k], err_msg=f"primary {k}")
         np.testing.assert_array_equal(got_s[k], want_s[k], err_msg=f"shadow {k}")
+    from test_gpu_fullsize import hits_parity
+    hits_parity(sc, p, want_p, want_s, S.FMT_EXT[fmt])
+    p.shadows = 0
+    want_p0, _ = O.trace_frame(buf, fmt, p, shadows=False)
+    hits_parity(sc, p, want_p0, want_s, S.FMT_EXT[fmt] + " without shadows")
 
 
 @pytest.mark.parametrize("fmt", [S.FMT_SVDAG, S.FMT_USSVDAG, S.FMT_SSVDAG])
