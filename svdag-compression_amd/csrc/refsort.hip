@@ -457,6 +457,7 @@ __global__ void __launch_bounds__(kBT) k_block_sort(const Seg* __restrict__ segs
   }
   if (tid == 0) {
     nActive[0] = 0;
+    nActive[1] = 0;
     nLeaf = 0;
     nEq = 0;
     if (g.depth == 0) {
@@ -468,13 +469,10 @@ __global__ void __launch_bounds__(kBT) k_block_sort(const Seg* __restrict__ segs
     }
   }
   __syncthreads();
-  int cur = 0;
-  while (nActive[cur] > 0) {
-    const uint32_t na = nActive[cur];
-    const int nxt = cur ^ 1;
-    // 1. median-of-3 to the front, pivot
-    if (tid < (int)na) {
-      const int64_t f = aFirst[cur][tid], l = aLast[cur][tid];
+  // 1. median-of-3 to the front of every active segment, its pivot
+  auto pivots = [&](int c) {
+    if (tid < (int)nActive[c]) {
+      const int64_t f = aFirst[c][tid], l = aLast[c][tid];
       auto vf = [&](int64_t k) { return sv[k]; };
       const int64_t m = median_pos(vf, f, l);
       const uint32_t tv = sv[f], ti = si[f];
@@ -482,23 +480,39 @@ __global__ void __launch_bounds__(kBT) k_block_sort(const Seg* __restrict__ segs
       aPivot[tid] = sv[f];
       aK[tid] = 0;
     }
-    if (tid == 0) nActive[nxt] = 0;
-    __syncthreads();
+  };
+  pivots(0);
+  __syncthreads();
+  // Every phase below first issues all the LDS loads of its kPer elements and
+  // only then uses them: one LDS round trip per dependent step instead of one
+  // per element and step.
+  int cur = 0;
+  while (nActive[cur] > 0) {
+    const uint32_t na = nActive[cur];
+    const int nxt = cur ^ 1;
     // 2. stopper flags and their exclusive prefix over all kLdsN positions
+    uint32_t sOf[kPer], v[kPer], fst[kPer], lst[kPer], piv[kPer];
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+      const uint32_t e = blk_elem(tid, k);
+      sOf[k] = e < n ? (uint32_t)segOf[e] : (uint32_t)kDone;
+      v[k] = sv[e];
+    }
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+      const uint32_t q = sOf[k] < kLeafTag ? sOf[k] : 0u;
+      piv[k] = aPivot[q];
+      fst[k] = aFirst[cur][q];
+      lst[k] = aLast[cur][q];
+    }
     bool fl[kPer], fr[kPer];
     uint32_t pl[kPer], pr[kPer], cl = 0, cr = 0;
 #pragma unroll
     for (int k = 0; k < kPer; ++k) {
       const uint32_t e = blk_elem(tid, k);
-      fl[k] = fr[k] = false;
-      if (e < n) {
-        const uint32_t s = segOf[e];
-        if (s < kLeafTag) {
-          const uint32_t p = aPivot[s];
-          fl[k] = e > aFirst[cur][s] && sv[e] <= p;
-          fr[k] = sv[e] >= p;
-        }
-      }
+      const bool act = sOf[k] < kLeafTag;
+      fl[k] = act && e > fst[k] && v[k] <= piv[k];
+      fr[k] = act && v[k] >= piv[k];
       const uint64_t bl = __ballot(fl[k]), br = __ballot(fr[k]);
       pl[k] = cl + lanes_below(bl);
       pr[k] = cr + lanes_below(br);
@@ -509,9 +523,9 @@ __global__ void __launch_bounds__(kBT) k_block_sort(const Seg* __restrict__ segs
     __syncthreads();
     uint32_t offL = 0, offR = 0, totL = 0, totR = 0;
 #pragma unroll
-    for (int v = 0; v < kWaves; ++v) {
-      const uint32_t a = wTot[0][v], b = wTot[1][v];
-      if (v < wv) { offL += a; offR += b; }
+    for (int w = 0; w < kWaves; ++w) {
+      const uint32_t a = wTot[0][w], b = wTot[1][w];
+      if (w < wv) { offL += a; offR += b; }
       totL += a; totR += b;
     }
 #pragma unroll
@@ -522,117 +536,173 @@ __global__ void __launch_bounds__(kBT) k_block_sort(const Seg* __restrict__ segs
     }
     if (tid == 0) { exL[kLdsN] = (uint16_t)totL; exR[kLdsN] = (uint16_t)totR; }
     __syncthreads();
-    // 3. ranks -> Lp / Rp
+    // 3. ranks -> Lp / Rp; the segment's stopper counts stay in registers
+    uint32_t nLk[kPer], nRk[kPer];
+    {
+      uint32_t eLf[kPer], eLl[kPer], eRf[kPer], eRl[kPer];
 #pragma unroll
-    for (int k = 0; k < kPer; ++k) {
-      if (!fl[k] && !fr[k]) continue;
-      const uint32_t e = blk_elem(tid, k);
-      const uint32_t s = segOf[e];
-      const uint32_t f = aFirst[cur][s], l = aLast[cur][s];
-      if (fl[k]) Lp[f + (exL[e] - exL[f])] = (uint16_t)e;
-      if (fr[k]) Rp[f + (exR[l] - 1 - exR[e])] = (uint16_t)e;
+      for (int k = 0; k < kPer; ++k) {
+        eLf[k] = exL[fst[k]]; eLl[k] = exL[lst[k]];
+        eRf[k] = exR[fst[k]]; eRl[k] = exR[lst[k]];
+      }
+#pragma unroll
+      for (int k = 0; k < kPer; ++k) {
+        const uint32_t e = blk_elem(tid, k);
+        nLk[k] = eLl[k] - eLf[k];
+        nRk[k] = eRl[k] - eRf[k];
+        // exL[e] - exL[f] = offL + pl[k] - exL[f]; exR[l] - 1 - exR[e] likewise
+        if (fl[k]) Lp[fst[k] + (offL + pl[k] - eLf[k])] = (uint16_t)e;
+        if (fr[k]) Rp[fst[k] + (eRl[k] - 1 - (offR + pr[k]))] = (uint16_t)e;
+      }
     }
     __syncthreads();
     // 4. paired swaps, K
+    {
+      uint32_t la[kPer], rb[kPer], la1[kPer], rb1[kPer];
+      bool sw[kPer];
 #pragma unroll
-    for (int k = 0; k < kPer; ++k) {
-      const uint32_t e = blk_elem(tid, k);
-      if (e >= n) continue;
-      const uint32_t s = segOf[e];
-      if (s >= kLeafTag) continue;
-      const uint32_t f = aFirst[cur][s], l = aLast[cur][s];
-      const uint32_t nL = exL[l] - exL[f], nR = exR[l] - exR[f];
-      const uint32_t m = min(nL, nR);
-      const uint32_t kk = e - f;
-      if (kk >= m) continue;
-      const uint32_t a = Lp[e], b = Rp[e];
-      if (a < b) {
-        const uint32_t tv = sv[a], ti = si[a];
-        sv[a] = sv[b]; si[a] = si[b]; sv[b] = tv; si[b] = ti;
-        const bool nextOk = (kk + 1 < m) && (Lp[e + 1] < Rp[e + 1]);
-        if (!nextOk) aK[s] = (uint16_t)(kk + 1);
+      for (int k = 0; k < kPer; ++k) {
+        const uint32_t e = blk_elem(tid, k);
+        const uint32_t m = min(nLk[k], nRk[k]);
+        sw[k] = sOf[k] < kLeafTag && e - fst[k] < m;
+        const uint32_t q = sw[k] ? e : 0u;
+        const uint32_t q1 = sw[k] && e - fst[k] + 1 < m ? e + 1 : 0u;
+        la[k] = Lp[q]; rb[k] = Rp[q];
+        la1[k] = Lp[q1]; rb1[k] = Rp[q1];
+      }
+      uint32_t va[kPer], ia[kPer], vb[kPer], ib[kPer];
+#pragma unroll
+      for (int k = 0; k < kPer; ++k) {
+        const bool go = sw[k] && la[k] < rb[k];
+        const uint32_t a = go ? la[k] : 0u, b = go ? rb[k] : 0u;
+        va[k] = sv[a]; ia[k] = si[a]; vb[k] = sv[b]; ib[k] = si[b];
+      }
+#pragma unroll
+      for (int k = 0; k < kPer; ++k) {
+        if (!(sw[k] && la[k] < rb[k])) continue;
+        const uint32_t e = blk_elem(tid, k);
+        const uint32_t kk = e - fst[k], m = min(nLk[k], nRk[k]);
+        sv[la[k]] = vb[k]; si[la[k]] = ib[k]; sv[rb[k]] = va[k]; si[rb[k]] = ia[k];
+        const bool nextOk = (kk + 1 < m) && (la1[k] < rb1[k]);
+        if (!nextOk) aK[sOf[k]] = (uint16_t)(kk + 1);
       }
     }
     __syncthreads();
     // 5. cut, children: <=16 -> insertion sort now, depth 0 -> heap sort now, else next round
     if (tid < (int)na) {
-      const uint32_t f = aFirst[cur][tid], l = aLast[cur][tid];
-      const uint32_t nL = exL[l] - exL[f], nR = exR[l] - exR[f];
-      const uint32_t K = aK[tid];
+      const uint32_t f = aFirst[cur][tid], l = aLast[cur][tid], K = aK[tid];
+      const int d = aDepth[cur][tid] - 1;
+      const uint32_t eLf = exL[f], eLl = exL[l], eRf = exR[f], eRl = exR[l];
+      const uint32_t rK1 = Rp[K ? f + K - 1 : f], lK = Lp[f + K];
+      const uint32_t nL = eLl - eLf, nR = eRl - eRf;
       // no value above the pivot: the left half is all pivots; none below:
       // the right half is
       const bool eqc[2] = {nL == l - f - 1, nR == l - f};
-      const uint32_t rprev = K ? Rp[f + K - 1] : l;
-      const uint32_t cut = (K < nL && Lp[f + K] < rprev) ? Lp[f + K] : rprev;
-      const int d = aDepth[cur][tid] - 1;
+      const uint32_t rprev = K ? rK1 : l;
+      const uint32_t cut = (K < nL && lK < rprev) ? lK : rprev;
       aCut[tid] = (uint16_t)cut;
-      uint16_t ids[2];
       const uint32_t cf[2] = {f, cut}, cl2[2] = {cut, l};
+      // the halves' kinds first, then their list slots (independent atomics)
+      int kind[2];   // 0 nothing, 1 leaf, 2 equal values, 3 heap sort now, 4 next round
+#pragma unroll
       for (int h = 0; h < 2; ++h) {
         const uint32_t sz = cl2[h] - cf[h];
+        kind[h] = sz <= (uint32_t)kThreshold ? (sz > 1 ? 1 : 0)
+                  : (eqc[h] && eq_depth_ok(sz, d)) ? 2 : (d == 0 ? 3 : 4);
+      }
+      uint32_t slot[2];
+#pragma unroll
+      for (int h = 0; h < 2; ++h)
+        slot[h] = kind[h] == 1 ? atomicAdd(&nLeaf, 1u)
+                  : kind[h] == 2 ? atomicAdd(&nEq, 1u) : kind[h] == 4 ? atomicAdd(&nActive[nxt], 1u) : 0u;
+      uint16_t ids[2];
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
         ids[h] = kDone;
-        LArr a{sv + cf[h], si + cf[h]};
-        if (sz <= (uint32_t)kThreshold) {
-          if (sz > 1) {   // sorted after the rounds, all leaves at once
-            const uint32_t q = atomicAdd(&nLeaf, 1u);
-            lfFirst[q] = (uint16_t)cf[h];
-            lfLast[q] = (uint16_t)cl2[h];
-            ids[h] = (uint16_t)(kLeafTag | q);
-          }
-        } else if (eqc[h] && eq_depth_ok(sz, d)) {   // placed after the rounds
-          const uint32_t q = atomicAdd(&nEq, 1u);
-          eqFirst[q] = (uint16_t)cf[h];
-          eqLast[q] = (uint16_t)cl2[h];
-          ids[h] = (uint16_t)(kEqTag | q);
-        } else if (d == 0) heap_sort(a, sz);
-        else {
-          const uint32_t slotn = atomicAdd(&nActive[nxt], 1u);
-          aFirst[nxt][slotn] = (uint16_t)cf[h]; aLast[nxt][slotn] = (uint16_t)cl2[h]; aDepth[nxt][slotn] = (int8_t)d;
-          ids[h] = (uint16_t)slotn;
+        if (kind[h] == 1) {   // sorted after the rounds, all leaves at once
+          lfFirst[slot[h]] = (uint16_t)cf[h];
+          lfLast[slot[h]] = (uint16_t)cl2[h];
+          ids[h] = (uint16_t)(kLeafTag | slot[h]);
+        } else if (kind[h] == 2) {   // placed after the rounds
+          eqFirst[slot[h]] = (uint16_t)cf[h];
+          eqLast[slot[h]] = (uint16_t)cl2[h];
+          ids[h] = (uint16_t)(kEqTag | slot[h]);
+        } else if (kind[h] == 4) {
+          aFirst[nxt][slot[h]] = (uint16_t)cf[h]; aLast[nxt][slot[h]] = (uint16_t)cl2[h];
+          aDepth[nxt][slot[h]] = (int8_t)d;
+          ids[h] = (uint16_t)slot[h];
         }
       }
+#pragma unroll
+      for (int h = 0; h < 2; ++h)
+        if (kind[h] == 3) {
+          LArr a{sv + cf[h], si + cf[h]};
+          heap_sort(a, cl2[h] - cf[h]);
+        }
       aLeft[tid] = ids[0]; aRight[tid] = ids[1];
     }
     __syncthreads();
-    // 6. new segment ids
+    // 6. new segment ids, and step 1 of the next round (it moves values, not ids)
+    {
+      uint32_t ct[kPer], lf[kPer], rt[kPer];
 #pragma unroll
-    for (int k = 0; k < kPer; ++k) {
-      const uint32_t e = blk_elem(tid, k);
-      if (e >= n) continue;
-      const uint32_t s = segOf[e];
-      if (s >= kLeafTag) continue;
-      segOf[e] = e < aCut[s] ? aLeft[s] : aRight[s];
+      for (int k = 0; k < kPer; ++k) {
+        const uint32_t q = sOf[k] < kLeafTag ? sOf[k] : 0u;
+        ct[k] = aCut[q]; lf[k] = aLeft[q]; rt[k] = aRight[q];
+      }
+#pragma unroll
+      for (int k = 0; k < kPer; ++k) {
+        if (sOf[k] >= kLeafTag) continue;
+        const uint32_t e = blk_elem(tid, k);
+        segOf[e] = (uint16_t)(e < ct[k] ? lf[k] : rt[k]);
+      }
     }
+    pivots(nxt);
+    if (tid == 0) nActive[cur] = 0;   // the round after next appends here
     cur = nxt;
     __syncthreads();
   }
   // The final insertion sort only acts inside leaves and is stable: every
   // element's place is the number of larger elements of its leaf plus the
   // equal ones before it.
-  uint32_t rv[kPer], ri[kPer];
+  // (loads first, as in the rounds: a leaf has at most kThreshold elements,
+  // all of them read in one unrolled window)
+  uint32_t rv[kPer], ri[kPer], sO[kPer], fq[kPer], lq[kPer];
   uint16_t rp[kPer];
 #pragma unroll
   for (int k = 0; k < kPer; ++k) {
     const uint32_t e = blk_elem(tid, k);
+    sO[k] = e < n ? (uint32_t)segOf[e] : (uint32_t)kDone;
+    rv[k] = sv[e];
+    ri[k] = si[e];
+  }
+#pragma unroll
+  for (int k = 0; k < kPer; ++k) {
+    const uint32_t s = sO[k], q = s & ~kTagMask;
+    const bool tagged = s >= kLeafTag && s != kDone;
+    const bool eq = (s & kTagMask) == kEqTag;
+    fq[k] = tagged ? (eq ? eqFirst[q] : lfFirst[q]) : 0u;
+    lq[k] = tagged ? (eq ? eqLast[q] : lfLast[q]) : 0u;
+  }
+#pragma unroll
+  for (int k = 0; k < kPer; ++k) {
+    const uint32_t e = blk_elem(tid, k);
+    const uint32_t s = sO[k];
     rp[k] = kDone;
-    if (e >= n) continue;
-    const uint32_t s = segOf[e];
     if (s < kLeafTag || s == kDone) continue;
-    const uint32_t v = sv[e];
+    const uint32_t f = fq[k], l = lq[k], v = rv[k];
     uint32_t r;
     if ((s & kTagMask) == kEqTag) {
-      const uint32_t f = eqFirst[s & ~kTagMask], l = eqLast[s & ~kTagMask];
       r = f + eq_dest(e - f, l - f);
     } else {
-      const uint32_t f = lfFirst[s & ~kTagMask], l = lfLast[s & ~kTagMask];
       r = f;
-      for (uint32_t j = f; j < l; ++j) {
-        const uint32_t u = sv[j];
-        r += (comp(u, v) || (u == v && j < e)) ? 1u : 0u;
+#pragma unroll
+      for (uint32_t j = 0; j < (uint32_t)kThreshold; ++j) {
+        const uint32_t pos = f + j;
+        const uint32_t u = sv[pos < l ? pos : f];
+        r += (pos < l && (comp(u, v) || (u == v && pos < e))) ? 1u : 0u;
       }
     }
-    rv[k] = v;
-    ri[k] = si[e];
     rp[k] = (uint16_t)r;
   }
   __syncthreads();
