@@ -997,27 +997,36 @@ __global__ void k_sdag_outcome(const uint32_t* __restrict__ key, uint64_t n, con
 // several launches.  Every thread reads the round's undecided count after
 // a barrier, so all leave the loop together; each round decides at least
 // the earliest undecided node, so it ends within n rounds.
+// The level's keys, candidates and invariance bits live in LDS beside the
+// table (91 KB): every probe's record compare and every resolution step is
+// then an LDS access instead of a global memory round trip.
+constexpr uint32_t kSdagCap = 2 * kSdagSmall;   // table slots, power of two
 struct SdagLds {
-  TEntry table[kSmallCap];
-  uint16_t slotOf[kSmallLevel];
-  uint32_t state[kSmallLevel];
-  uint32_t posOf[kSmallLevel];
+  TEntry table[kSdagCap];
+  uint32_t key[kSdagSmall * kRecWords];
+  uint32_t cand[kSdagSmall * 7];
+  uint16_t slotOf[kSdagSmall];
+  uint32_t state[kSdagSmall];
+  uint32_t posOf[kSdagSmall];
+  uint8_t inv[kSdagSmall];
   uint32_t lds[kSmallThreads / 64 + 1];
   uint32_t remaining;
 };
 __device__ void sdag_small_level(const uint32_t* __restrict__ dag, uint32_t n, const uint32_t* __restrict__ childMap,
-                                 const uint8_t* __restrict__ childInv, uint32_t* __restrict__ key,
-                                 uint8_t* __restrict__ inv, uint32_t* __restrict__ cand, uint32_t* __restrict__ map,
+                                 const uint8_t* __restrict__ childInv, uint32_t* __restrict__ map,
                                  uint32_t* __restrict__ out, uint8_t* __restrict__ invOut,
                                  uint32_t* __restrict__ total, uint32_t* __restrict__ err, SdagLds& sh) {
   TEntry* table = sh.table;
+  uint32_t* key = sh.key;
+  uint32_t* cand = sh.cand;
+  uint8_t* inv = sh.inv;
   uint16_t* slotOf = sh.slotOf;
   uint32_t* state = sh.state;
   uint32_t* posOf = sh.posOf;
   uint32_t* lds = sh.lds;
   uint32_t& remaining = sh.remaining;
   const uint32_t tid = threadIdx.x;
-  for (uint32_t j = tid; j < kSmallCap; j += kSmallThreads) table[j] = kEmpty;
+  for (uint32_t j = tid; j < kSdagCap; j += kSmallThreads) table[j] = kEmpty;
   for (uint32_t i = tid; i < n; i += kSmallThreads) sdag_key(dag, i, childMap, key, inv);
   __syncthreads();
   // insert (as k_insert)
@@ -1028,7 +1037,7 @@ __device__ void sdag_small_level(const uint32_t* __restrict__ dag, uint32_t n, c
     const uint64_t hv = hash_rec(w);
     const uint32_t fp = (uint32_t)(hv >> 32);
     const TEntry mine = ((TEntry)fp << 32) | (TEntry)i;
-    uint32_t h = (uint32_t)hv & (kSmallCap - 1);
+    uint32_t h = (uint32_t)hv & (kSdagCap - 1);
     while (true) {
       TEntry e = table[h];
       if (e == kEmpty) {
@@ -1040,7 +1049,7 @@ __device__ void sdag_small_level(const uint32_t* __restrict__ dag, uint32_t n, c
         if (mine < e) atomicMin(&table[h], mine);
         break;
       }
-      h = (h + 1) & (kSmallCap - 1);
+      h = (h + 1) & (kSdagCap - 1);
     }
     slotOf[i] = (uint16_t)h;
   }
@@ -1048,35 +1057,44 @@ __device__ void sdag_small_level(const uint32_t* __restrict__ dag, uint32_t n, c
   auto find = [&](const uint32_t w[kRecWords]) -> uint32_t {
     const uint64_t hv = hash_rec(w);
     const uint32_t fp = (uint32_t)(hv >> 32);
-    uint32_t h = (uint32_t)hv & (kSmallCap - 1);
+    uint32_t h = (uint32_t)hv & (kSdagCap - 1);
     while (true) {
       const TEntry e = table[h];
       if (e == kEmpty) return kNone;
       if ((uint32_t)(e >> 32) == fp && rec_eq(key, (uint32_t)e, w)) return (uint32_t)e;
-      h = (h + 1) & (kSmallCap - 1);
+      h = (h + 1) & (kSdagCap - 1);
     }
   };
-  // candidates (as k_sdag_cands)
-  for (uint32_t i = tid; i < n; i += kSmallThreads) {
+  // candidates (as k_sdag_cands): P lanes per node (8 when the level is
+  // small enough), lane q trying the variants t = 1 + q, 1 + q + P, ...; a
+  // variant is a hash and a probe chain, so spreading them shortens the
+  // longest chain of a thread
+  const uint32_t P = n * 8 <= kSmallThreads ? 8u : n * 4 <= kSmallThreads ? 4u : n * 2 <= kSmallThreads ? 2u : 1u;
+  for (uint32_t x = tid; x < n * P; x += kSmallThreads) {
+    const uint32_t i = x / P, q = x % P;
     uint32_t* cd = cand + (uint64_t)i * 7;
     const bool first = slotOf[i] != 0xFFFFu && entry_index(table[slotOf[i]]) == i;
     if (!first) {
-#pragma unroll
-      for (int t = 0; t < 7; ++t) cd[t] = kNone;
-      state[i] = kNonRep;
+      for (uint32_t t = 1 + q; t < 8; t += P) cd[t - 1] = kNone;
       continue;
     }
     uint32_t k0[kRecWords], v[kRecWords];
     load_rec(key, i, k0);
     const ChildInv ci = child_inv(k0, childInv);
-    bool any = false;
-    for (int t = 1; t < 8; ++t) {
-      make_variant(k0, t, ci, v);
+    for (uint32_t t = 1 + q; t < 8; t += P) {
+      make_variant(k0, (int)t, ci, v);
       const uint32_t r = same_rec(v, k0) ? i : find(v);
-      if (r != kNone && r < i) { cd[t - 1] = r; any = true; }
-      else cd[t - 1] = kNone;
+      cd[t - 1] = (r != kNone && r < i) ? r : kNone;
     }
-    state[i] = any ? kUndecided : kRep;
+  }
+  __syncthreads();
+  for (uint32_t i = tid; i < n; i += kSmallThreads) {
+    const bool first = slotOf[i] != 0xFFFFu && entry_index(table[slotOf[i]]) == i;
+    const uint32_t* cd = cand + (uint64_t)i * 7;
+    bool any = false;
+#pragma unroll
+    for (int t = 0; t < 7; ++t) any |= cd[t] != kNone;
+    state[i] = !first ? kNonRep : (any ? kUndecided : kRep);
   }
   __syncthreads();
   // resolution rounds (as k_sdag_resolve), to convergence
@@ -1173,9 +1191,6 @@ struct SdagTop {
   uint32_t* map[kMaxLevels + 1];      // map[lev + 1]: correspondence of the level below lev
   uint8_t* invOut[kMaxLevels + 1];    // SDAG invariance bits per level (inv of level lev + 1 = childInv)
   uint32_t* out[kMaxLevels];
-  uint32_t* key;                      // scratch, max n records
-  uint8_t* inv;                       // scratch, max n
-  uint32_t* cand;                     // scratch, max n * 7
   uint32_t* counts;
   uint32_t* err;
   int32_t top;
@@ -1183,8 +1198,8 @@ struct SdagTop {
 __global__ void __launch_bounds__(kSmallThreads) k_sdag_levels_small(SdagTop a) {
   __shared__ SdagLds sh;
   for (int lev = a.top; lev >= 1; --lev) {
-    sdag_small_level(a.dag[lev], a.n[lev], a.map[lev + 1], a.invOut[lev + 1], a.key, a.inv, a.cand, a.map[lev],
-                     a.out[lev], a.invOut[lev], a.counts + lev, a.err, sh);
+    sdag_small_level(a.dag[lev], a.n[lev], a.map[lev + 1], a.invOut[lev + 1], a.map[lev], a.out[lev], a.invOut[lev],
+                     a.counts + lev, a.err, sh);
     __threadfence_block();
     __syncthreads();
   }
@@ -1589,14 +1604,10 @@ bool reduce_sym_pass(svd_octree* o, bool speculative) {
       top.invOut[lev + 1] = o->sdag[lev + 1].inv;
       o->sdag[0].n = o->dag[0].n;
       o->sdagMap[0] = nullptr;
-      top.key = dalloc<uint32_t>(c, (maxTop ? maxTop : 1) * kRecWords);
-      top.inv = dalloc<uint8_t>(c, maxTop + 1);
-      top.cand = dalloc<uint32_t>(c, (maxTop ? maxTop : 1) * 7);
       top.counts = counts;
       top.err = err;
       k_sdag_levels_small<<<1, kSmallThreads, 0, c->stream>>>(top);
       check_launch("k_sdag_levels_small");
-      dfree(c, top.key); dfree(c, top.inv); dfree(c, top.cand);
       break;
     }
     // the root level is never reduced and needs no table

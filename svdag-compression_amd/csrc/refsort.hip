@@ -797,13 +797,16 @@ void refsort_ranges(svd_ctx* c, uint32_t* idx, uint32_t* val, uint64_t n, const 
     // A round is plan (split the last round's segments, pick the pivots of
     // the new ones), count, scatter and swap.  Every round leaves a piece of
     // at least half the largest segment, so the first ceil(log2(max /
-    // kLdsN)) rounds are all needed: run them without looking at the list
-    // size, then check it every second round.  A round over an empty list
-    // changes nothing.
+    // kLdsN)) rounds are all needed.  Refcount arrays hold long runs of equal
+    // values, which split unevenly: the lossless builds measured need up to
+    // four rounds more, so the first batch has those too (a round over an
+    // empty list changes nothing and costs four short launches, a look at the
+    // list size a host round trip); after it the size is checked every
+    // second round.
     uint64_t maxRoot = 0;
     for (const Seg& r : roots[0]) maxRoot = std::max<uint64_t>(maxRoot, r.last - r.first);
-    int batch = 0;
-    while (((uint64_t)kLdsN << batch) < maxRoot) ++batch;
+    int batch = 4;
+    while (((uint64_t)kLdsN << (batch - 4)) < maxRoot) ++batch;
     L.big = cur;
     L.nbig = ncur;
     k_g_plan<<<1, kPlanT, 0, c->stream>>>(nullptr, nullptr, Lpos, Rpos, L, val, idx, st, blk0);

@@ -298,11 +298,23 @@ __device__ __forceinline__ uint64_t mix64(uint64_t z) {
   return z;
 }
 
+// Hash of a 9-word record for the device tables: two 32-bit lanes mixed with
+// 32-bit multiplies (murmur3-style word steps and finalisers; a 64-bit
+// multiply costs four 32-bit ones here), the low word indexing the table,
+// the high word the fingerprint.  Only equality of records decides what a
+// table returns, so the hash affects probe lengths, never results.
 __device__ __forceinline__ uint64_t hash_rec(const uint32_t w[kRecWords]) {
-  uint64_t h = 0x9e3779b97f4a7c15ull;
+  uint32_t a = 0x9E3779B9u, b = 0x7F4A7C15u;
 #pragma unroll
-  for (int i = 0; i < kRecWords; ++i) h = mix64(h ^ ((uint64_t)w[i] + ((uint64_t)i << 40)));
-  return h;
+  for (int i = 0; i < kRecWords; ++i) {
+    a = (a ^ w[i]) * 0xCC9E2D51u;
+    a = (a << 15) | (a >> 17);
+    b = (b + w[i]) * 0x1B873593u;
+    b ^= b >> 13;
+  }
+  a ^= a >> 16; a *= 0x85EBCA6Bu; a ^= a >> 13; a *= 0xC2B2AE35u; a ^= a >> 16;
+  b ^= b >> 16; b *= 0x7FEB352Du; b ^= b >> 15; b *= 0x846CA68Bu; b ^= b >> 16;
+  return ((uint64_t)b << 32) | a;
 }
 
 }  // namespace svd
