@@ -57,6 +57,22 @@ void Arena::trim() {
   free_.clear();
 }
 
+// Two small arrays (together at most 2048 words) in one host round trip.
+void read_u32s_2(svd_ctx* c, const uint32_t* d1, uint32_t* h1, size_t n1, const uint32_t* d2, uint32_t* h2,
+                 size_t n2) {
+  if (n1 + n2 > 2048) {
+    read_u32s(c, d1, h1, n1);
+    read_u32s(c, d2, h2, n2);
+    return;
+  }
+  uint32_t* stage = (uint32_t*)c->pinned;
+  if (n1) SVD_HIP(hipMemcpyAsync(stage, d1, 4 * n1, hipMemcpyDeviceToHost, c->stream));
+  if (n2) SVD_HIP(hipMemcpyAsync(stage + n1, d2, 4 * n2, hipMemcpyDeviceToHost, c->stream));
+  SVD_HIP(hipStreamSynchronize(c->stream));
+  memcpy(h1, stage, 4 * n1);
+  memcpy(h2, stage + n1, 4 * n2);
+}
+
 uint64_t read_u32(svd_ctx* c, const uint32_t* d) {
   uint32_t v = 0;
   read_u32s(c, d, &v, 1);

@@ -32,7 +32,9 @@ __device__ __forceinline__ uint32_t lanes_below(uint64_t m) {
 // After run_tile: head bit k = code k starts a run, rank[k] = the run's
 // number within the tile counted from the tile's first head (-1: the code
 // continues the previous tile's last run), agg = the tile's runs.
-constexpr int kRunT = 256;
+// 512 threads per run tile: tile numbers come from one device counter whose
+// same-address atomics serialise, so fewer, larger tiles start sooner.
+constexpr int kRunT = 512;
 template <int PER>
 struct RunTile {
   uint64_t v[PER];
@@ -156,9 +158,12 @@ __global__ void __launch_bounds__(kRunT) k_leaf_build(const uint64_t* __restrict
     if (lane == 0) cpart[q][wave] = x;
   }
   __syncthreads();
-  if (threadIdx.x <= L)
-    part[(uint64_t)tile * (L + 1) + threadIdx.x] =
-        cpart[threadIdx.x][0] + cpart[threadIdx.x][1] + cpart[threadIdx.x][2] + cpart[threadIdx.x][3];
+  if (threadIdx.x <= L) {
+    uint32_t x = 0;
+#pragma unroll
+    for (int w = 0; w < kRunT / 64; ++w) x += cpart[threadIdx.x][w];
+    part[(uint64_t)tile * (L + 1) + threadIdx.x] = x;
+  }
   const uint32_t prefix = lb_exclusive(p, tile, t.agg, lds);
   const uint64_t tileEnd = tileBase + kLeafTile;
 #pragma unroll

@@ -406,15 +406,27 @@ void put_bytes(svd_ctx* c, uint8_t* dst, const std::vector<uint8_t>& src) {
 
 // .svdag (of the DAG, or its cross-level view) and .ussvdag (of the SDAG):
 // each format's passes are enqueued, and the word counts of all are read back
-// once at the end.
-void encode_words_sets(svd_octree* o, const int* fmts, int K) {
+// once at the end (or, when SSVDAG-layout images follow, together with their
+// layout sizes).
+struct WordsPending {
+  int K = 0;
+  int fmts[2] = {0, 0};
+  uint32_t* dev = nullptr;        // per format: word count, error flags
+  uint32_t* ptrs[2] = {nullptr, nullptr};
+  uint64_t hdrBytes[2] = {0, 0};
+};
+
+void encode_words_enqueue(svd_octree* o, const int* fmts, int K, WordsPending& wp) {
   svd_ctx* c = o->ctx;
   const uint32_t L = o->levels;
-  uint32_t* dev = dalloc<uint32_t>(c, 2 * K);   // per format: word count, error flags
+  uint32_t* dev = dalloc<uint32_t>(c, 2 * K);
   fill_u32(c, dev, 0u, 2 * K);
-  std::vector<uint32_t*> ptrsOf(K, nullptr);
-  std::vector<uint64_t> hdrBytesOf(K, 0);
+  wp.K = K;
+  wp.dev = dev;
+  uint32_t** ptrsOf = wp.ptrs;
+  uint64_t* hdrBytesOf = wp.hdrBytes;
   for (int s = 0; s < K; ++s) {
+    wp.fmts[s] = fmts[s];
     const int fmt = fmts[s];
     const bool ussv = fmt == SVD_FMT_USSVDAG;
     const std::vector<Level>& lv = ussv ? o->sdag : (o->hasX ? o->xdag : o->dag);
@@ -447,17 +459,29 @@ void encode_words_sets(svd_octree* o, const int* fmts, int K) {
     if (N) k_write_words<<<div_up(N, kB), kB, 0, c->stream>>>(a, ussv ? 1 : 0, ptrs, N, words, d + 1);
     check_launch("k_write_words");
   }
-  std::vector<uint32_t> h(2 * K);
-  read_u32s(c, dev, h.data(), 2 * K);
-  for (int s = 0; s < K; ++s) dfree(c, ptrsOf[s]);
-  dfree(c, dev);
-  for (int s = 0; s < K; ++s)
+}
+
+// h: the 2K words of wp.dev, read back.
+void encode_words_finish(svd_octree* o, WordsPending& wp, const uint32_t* h) {
+  svd_ctx* c = o->ctx;
+  for (int s = 0; s < wp.K; ++s) dfree(c, wp.ptrs[s]);
+  dfree(c, wp.dev);
+  wp.dev = nullptr;
+  for (int s = 0; s < wp.K; ++s)
     if (h[2 * s + 1]) throw Error{SVD_ERR_POINTER, "encode: child pointer without a target node"};
-  for (int s = 0; s < K; ++s) {
-    Encoded& e = o->enc[fmts[s]];
-    e.bytes = hdrBytesOf[s] + 4ull * h[2 * s];
+  for (int s = 0; s < wp.K; ++s) {
+    Encoded& e = o->enc[wp.fmts[s]];
+    e.bytes = wp.hdrBytes[s] + 4ull * h[2 * s];
     e.valid = true;
   }
+}
+
+void encode_words_sets(svd_octree* o, const int* fmts, int K) {
+  WordsPending wp;
+  encode_words_enqueue(o, fmts, K, wp);
+  uint32_t h[4];
+  read_u32s(o->ctx, wp.dev, h, 2 * K);
+  encode_words_finish(o, wp, h);
 }
 
 // One or two SSVDAG-layout images (.ssvdag of the SDAG, .esvdag of the DAG)
@@ -465,7 +489,7 @@ void encode_words_sets(svd_octree* o, const int* fmts, int K) {
 // counts the references of all, one segmented sort orders every level of
 // both (each level is its own std::sort call), one host read returns the
 // layout sizes, and the writes follow.
-void encode_ssvdag_sets(svd_octree* o, const int* fmts, int K) {
+void encode_ssvdag_sets(svd_octree* o, const int* fmts, int K, WordsPending* wp = nullptr) {
   if (K < 1 || K > 2) throw Error{SVD_ERR_ARG, "SSVDAG-layout encodings: one or two sets"};
   svd_ctx* c = o->ctx;
   const int L = (int)o->levels;
@@ -589,7 +613,14 @@ void encode_ssvdag_sets(svd_octree* o, const int* fmts, int K) {
     }
   }
   std::vector<uint32_t> h(K * L + 1);
-  read_u32s(c, dev, h.data(), K * L + 1);
+  if (wp) {
+    // the pending word counts ride on the same host round trip
+    uint32_t hw[4];
+    read_u32s_2(c, dev, h.data(), K * L + 1, wp->dev, hw, 2 * wp->K);
+    encode_words_finish(o, *wp, hw);
+  } else {
+    read_u32s(c, dev, h.data(), K * L + 1);
+  }
   if (h[K * L]) throw Error{SVD_ERR_CAPACITY, "unsupported pointer size (>= 2^30)"};
   for (int s = 0; s < K; ++s) {
     const std::vector<Level>& lv = *lvs[s];
@@ -683,6 +714,12 @@ void encode_formats(svd_octree* o, const int* fmts, int n) {
     seen[fmts[k]] = true;
     if (fmts[k] == SVD_FMT_SVDAG || fmts[k] == SVD_FMT_USSVDAG) words[nw++] = fmts[k];
     else ss[nss++] = fmts[k];
+  }
+  if (nw && nss) {
+    WordsPending wp;
+    encode_words_enqueue(o, words, nw, wp);
+    encode_ssvdag_sets(o, ss, nss, &wp);
+    return;
   }
   if (nw) encode_words_sets(o, words, nw);
   if (nss) encode_ssvdag_sets(o, ss, nss);

@@ -14,7 +14,10 @@
 
 namespace svd {
 
-constexpr int kScanOpT = 256;
+// 1024 threads per tile: every tile takes its number from one device counter,
+// and atomics on one address serialise (about 11 ns each on MI355X), so
+// fewer, larger tiles start sooner.
+constexpr int kScanOpT = 1024;
 
 // Element k of thread (wave w, lane j): w * 64 * PER + 64 * k + j of the
 // tile, so value() and apply() see 64 consecutive elements per row.  The

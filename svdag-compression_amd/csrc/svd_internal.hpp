@@ -162,6 +162,8 @@ inline void dfree(svd_ctx* c, void* p) { if (p) c->arena->put(p); }
 // Read back a few u32/u64 device values into host memory (synchronises).
 uint64_t read_u32(svd_ctx* c, const uint32_t* d);
 void read_u32s(svd_ctx* c, const uint32_t* d, uint32_t* h, size_t n);
+void read_u32s_2(svd_ctx* c, const uint32_t* d1, uint32_t* h1, size_t n1, const uint32_t* d2, uint32_t* h2,
+                 size_t n2);
 
 // ---- scan (scan.hip) ---------------------------------------------------------
 // The look-back state of one single-pass scan over `tiles` tiles on the
