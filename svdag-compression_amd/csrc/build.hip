@@ -110,7 +110,7 @@ __device__ __forceinline__ uint32_t run_tail_bits(const uint64_t* __restrict__ c
 // of workgroups would queue.  Each code ORs its bit into its run's LDS mask;
 // a run that crosses the tile's end reads the codes past it.  With bad input
 // the writes stay below n and the caller throws on the bad count.
-constexpr int kLeafPer = 32;
+constexpr int kLeafPer = 16;
 constexpr uint32_t kLeafTile = kRunT * kLeafPer;
 __global__ void __launch_bounds__(kRunT) k_leaf_build(const uint64_t* __restrict__ code, uint64_t n, uint32_t L,
                                                       LbPass p, uint32_t* __restrict__ part,
@@ -126,9 +126,12 @@ __global__ void __launch_bounds__(kRunT) k_leaf_build(const uint64_t* __restrict
   run_tile(code, n, tileBase, t, lds);   // its barrier also orders the clearing
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const uint64_t wbase = tileBase + (uint64_t)wave * (64 * kLeafPer) + lane;
-  uint32_t cnt[kMaxLevels + 1];
-#pragma unroll
-  for (int q = 0; q <= (int)kMaxLevels; ++q) cnt[q] = 0;
+  // per-thread histogram of K over the good codes, 6-bit bins packed in three
+  // 64-bit words (bins 0-9, 10-19, 20-21; at most kLeafPer < 64 per bin):
+  // a few operations per code instead of one counter per level
+  static_assert(kLeafPer < 64, "6-bit histogram bins");
+  uint64_t hA = 0, hB = 0, hC = 0;
+  uint32_t nBad = 0;
 #pragma unroll
   for (int k = 0; k < kLeafPer; ++k) {
     const uint64_t i = wbase + 64 * k;
@@ -143,12 +146,26 @@ __global__ void __launch_bounds__(kRunT) k_leaf_build(const uint64_t* __restrict
       if (x <= u) bad = true;
       else K = min(L, (uint32_t)(63 - __clzll((long long)(x ^ u))) / 3u);
     }
-    cnt[0] += bad ? 1u : 0u;
-    if (!bad) {
-#pragma unroll
-      for (int q = 1; q <= (int)kMaxLevels; ++q) cnt[q] += K >= (uint32_t)q ? 1u : 0u;
-    }
+    nBad += bad ? 1u : 0u;
+    const uint32_t kb = K < 10 ? K : (K < 20 ? K - 10 : K - 20);
+    const uint64_t one = bad ? 0ull : 1ull << (6 * kb);
+    hA += K < 10 ? one : 0ull;
+    hB += (K >= 10 && K < 20) ? one : 0ull;
+    hC += K >= 20 ? one : 0ull;
     if (t.rank[k] >= 0) atomicOr(&lmask[t.rank[k]], 1u << (uint32_t)(x & 7u));
+  }
+  // level q (q >= 1) counts the good codes with K >= q: suffix sums of the bins
+  uint32_t cnt[kMaxLevels + 1];
+  cnt[0] = nBad;
+  {
+    uint32_t run = 0;
+#pragma unroll
+    for (int q = (int)kMaxLevels; q >= 1; --q) {
+      const uint64_t w = q < 10 ? hA : (q < 20 ? hB : hC);
+      const int sh = 6 * (q < 10 ? q : (q < 20 ? q - 10 : q - 20));
+      run += (uint32_t)(w >> sh) & 63u;
+      cnt[q] = run;
+    }
   }
 #pragma unroll
   for (int q = 0; q <= (int)kMaxLevels; ++q) {
