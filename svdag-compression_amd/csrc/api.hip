@@ -157,6 +157,10 @@ int svd_ctx_create(int device, svd_ctx** out) {
   {
     const char* e = std::getenv("SVD_TOSDAG_CHECKED");
     c->sdagChecked = e && e[0] && !(e[0] == '0' && e[1] == 0);
+    // SVD_REFSORT_EXTRA_ROUNDS: device-wide refcount-order rounds beyond the
+    // size bound before the list size is first looked at (default 4; 0..16)
+    const char* r = std::getenv("SVD_REFSORT_EXTRA_ROUNDS");
+    if (r && r[0]) c->refsortExtra = std::min(16, std::max(0, std::atoi(r)));
   }
   *out = c;
   return SVD_OK;

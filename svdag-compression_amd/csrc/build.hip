@@ -710,9 +710,13 @@ __global__ void k_compact(const uint32_t* __restrict__ rec, const uint32_t* __re
 }
 
 // children[c] <- remap[children[c]] where mask bit c is set (geom_octree.cpp:519-538)
+// clear (optional): a hash table of nClear entries set to empty on the way,
+// grid-stride, in place of a fill launch before the next level's insert.
 __global__ void k_remap_children(const uint32_t* __restrict__ in, uint64_t n, const uint32_t* __restrict__ remap,
-                                 uint32_t* __restrict__ out) {
+                                 uint32_t* __restrict__ out, unsigned long long* __restrict__ clear = nullptr,
+                                 uint64_t nClear = 0) {
   uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  for (uint64_t j = i; j < nClear; j += (uint64_t)gridDim.x * blockDim.x) clear[j] = ~0ull;
   if (i >= n) return;
   uint32_t w[kRecWords];
   load_rec(in, i, w);
@@ -1448,13 +1452,15 @@ void remap_children(svd_ctx* c, const uint32_t* in, uint64_t n, const uint32_t* 
 // Generic dedup of one inner level's records -> remap + compacted records.
 // The unique count goes to the device word *total; the output holds n
 // records (the upper bound), so no host read is needed.
+// cleared: a table of table_cap(n) entries already set to empty (null: one is
+// allocated and cleared here); it is freed here either way.
 void dedup_level_dev(svd_ctx* c, const uint32_t* rec, uint64_t n, uint32_t* remap, uint32_t** outRec,
-                     uint32_t* total) {
+                     uint32_t* total, TEntry* cleared = nullptr) {
   const uint32_t cap = table_cap(n);
-  TEntry* table = dalloc<TEntry>(c, cap);
+  TEntry* table = cleared ? cleared : dalloc<TEntry>(c, cap);
   uint32_t* slot = dalloc<uint32_t>(c, n);
   uint32_t* pos = dalloc<uint32_t>(c, n + 1);
-  fill_u32(c, (uint32_t*)table, kNone, 2ull * cap);
+  if (!cleared) fill_u32(c, (uint32_t*)table, kNone, 2ull * cap);
   launch_insert(c, rec, n, table, cap - 1, slot);
   check_launch("k_insert");
   scan_op<4>(c, n, RepFlagOp{slot, table, pos}, total);
@@ -1531,11 +1537,15 @@ uint64_t dedup_all(svd_ctx* c, uint32_t L, const std::vector<Level>& src, std::v
       break;
     }
     uint32_t* keyed = dalloc<uint32_t>(c, (s.n ? s.n : 1) * kRecWords);
-    if (s.n) k_remap_children<<<div_up(s.n, kB), kB, 0, c->stream>>>(s.rec, s.n, remap, keyed);
+    // the level's dedup table is cleared by the same launch
+    const uint32_t cap = table_cap(s.n);
+    TEntry* table = dalloc<TEntry>(c, cap);
+    if (s.n) k_remap_children<<<div_up(s.n, kB), kB, 0, c->stream>>>(s.rec, s.n, remap, keyed, table, cap);
+    else fill_u32(c, (uint32_t*)table, kNone, 2ull * cap);
     check_launch("k_remap_children");
     Level& d = dst[lev];
     uint32_t* nremap = dalloc<uint32_t>(c, s.n + 1);
-    dedup_level_dev(c, keyed, s.n, nremap, &d.rec, counts + lev);
+    dedup_level_dev(c, keyed, s.n, nremap, &d.rec, counts + lev, table);
     dfree(c, keyed);
     dfree(c, remap);
     remap = nremap;

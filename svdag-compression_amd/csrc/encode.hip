@@ -489,7 +489,10 @@ void encode_words_sets(svd_octree* o, const int* fmts, int K) {
 // counts the references of all, one segmented sort orders every level of
 // both (each level is its own std::sort call), one host read returns the
 // layout sizes, and the writes follow.
-void encode_ssvdag_sets(svd_octree* o, const int* fmts, int K, WordsPending* wp = nullptr) {
+// checked: the refcount order reads its list size back until it is done
+// (otherwise it reports an unfinished order with the layout sizes, and the
+// whole encoding runs again checked).
+void encode_ssvdag_sets(svd_octree* o, const int* fmts, int K, WordsPending* wp = nullptr, bool checked = false) {
   if (K < 1 || K > 2) throw Error{SVD_ERR_ARG, "SSVDAG-layout encodings: one or two sets"};
   svd_ctx* c = o->ctx;
   const int L = (int)o->levels;
@@ -503,8 +506,9 @@ void encode_ssvdag_sets(svd_octree* o, const int* fmts, int K, WordsPending* wp 
   }
   // perm / ind / offs of set s, level lev at [s * L + lev]
   std::vector<uint32_t*> perm(K * L, nullptr), ind(K * L, nullptr), offs(K * L, nullptr);
-  // dev[s * L + lev]: inner-size totals per level, dev[K * L]: pointer-size error flag
-  uint32_t* dev = dalloc<uint32_t>(c, K * L + 1);
+  // dev[s * L + lev]: inner-size totals per level, dev[K * L]: pointer-size
+  // error flag, dev[K * L + 1]: unfinished refcount order
+  uint32_t* dev = dalloc<uint32_t>(c, K * L + 2);
   uint32_t* err = dev + K * L;   // dev is cleared by k_level_iota below
   // Phase 1a: the reference counts of every level land in one array, and one
   // segmented sort orders all levels of all sets.
@@ -531,8 +535,8 @@ void encode_ssvdag_sets(svd_octree* o, const int* fmts, int K, WordsPending* wp 
   uint32_t* hist = dalloc<uint32_t>(c, tot + 1);   // cleared by k_level_iota
   for (int s = 0; s < K; ++s)
     for (int lev = 0; lev <= L - 2; ++lev) perm[s * L + lev] = permAll + off[s * L + lev];
-  k_level_iota<<<div_up(std::max<uint64_t>(tot, (uint64_t)(K * L + 1)), kB), kB, 0, c->stream>>>(
-      ua, tot, permAll, hist, dev, (uint32_t)(K * L + 1));
+  k_level_iota<<<div_up(std::max<uint64_t>(tot, (uint64_t)(K * L + 2)), kB), kB, 0, c->stream>>>(
+      ua, tot, permAll, hist, dev, (uint32_t)(K * L + 2));
   check_launch("k_level_iota");
   RcJobs jb{};
   for (int s = 0; s < K; ++s)
@@ -548,7 +552,8 @@ void encode_ssvdag_sets(svd_octree* o, const int* fmts, int K, WordsPending* wp 
     }
   if (jb.n) k_refcount<<<jb.blk0[jb.n], kRcT, 0, c->stream>>>(ua, jb, hist, err);
   check_launch("k_refcount");
-  refsort_ranges(c, permAll, hist, tot, rfirst.data(), rlast.data(), rfirst.size());
+  refsort_ranges(c, permAll, hist, tot, rfirst.data(), rlast.data(), rfirst.size(),
+                 checked ? nullptr : dev + K * L + 1);
   dfree(c, hist);
   // Phase 1b, bottom-up: children addresses of each level in its order.  A
   // set's levels from the first whose level and all above are small go to
@@ -612,14 +617,21 @@ void encode_ssvdag_sets(svd_octree* o, const int* fmts, int K, WordsPending* wp 
       check_launch("k_inner_layouts_small");
     }
   }
-  std::vector<uint32_t> h(K * L + 1);
+  std::vector<uint32_t> h(K * L + 2);
   if (wp) {
     // the pending word counts ride on the same host round trip
     uint32_t hw[4];
-    read_u32s_2(c, dev, h.data(), K * L + 1, wp->dev, hw, 2 * wp->K);
+    read_u32s_2(c, dev, h.data(), K * L + 2, wp->dev, hw, 2 * wp->K);
     encode_words_finish(o, *wp, hw);
   } else {
-    read_u32s(c, dev, h.data(), K * L + 1);
+    read_u32s(c, dev, h.data(), K * L + 2);
+  }
+  if (h[K * L + 1]) {   // the refcount order needed more rounds: again, checked
+    for (auto* p : ind) dfree(c, p);
+    for (auto* p : offs) dfree(c, p);
+    dfree(c, permAll); dfree(c, dev);
+    encode_ssvdag_sets(o, fmts, K, nullptr, true);
+    return;
   }
   if (h[K * L]) throw Error{SVD_ERR_CAPACITY, "unsupported pointer size (>= 2^30)"};
   for (int s = 0; s < K; ++s) {

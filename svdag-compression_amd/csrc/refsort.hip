@@ -735,8 +735,10 @@ __global__ void k_seed_lists(Seeds sd, Seg* __restrict__ big, Seg* __restrict__ 
 
 }  // namespace
 
+__global__ void k_copy_u32(const uint32_t* __restrict__ src, uint32_t* __restrict__ dst) { *dst = *src; }
+
 void refsort_ranges(svd_ctx* c, uint32_t* idx, uint32_t* val, uint64_t n, const uint64_t* rfirst,
-                    const uint64_t* rlast, size_t nr) {
+                    const uint64_t* rlast, size_t nr, uint32_t* unfinished) {
   if (n > 0xFFFFFFF0ull) throw Error{SVD_ERR_CAPACITY, "refsort: array too large"};
   // Each range is its own std::sort call: own depth budget 2*floor(log2 size).
   std::vector<Seg> roots[4];   // big, small, (heap), leaf
@@ -799,14 +801,14 @@ void refsort_ranges(svd_ctx* c, uint32_t* idx, uint32_t* val, uint64_t n, const 
     // at least half the largest segment, so the first ceil(log2(max /
     // kLdsN)) rounds are all needed.  Refcount arrays hold long runs of equal
     // values, which split unevenly: the lossless builds measured need up to
-    // four rounds more, so the first batch has those too (a round over an
-    // empty list changes nothing and costs four short launches, a look at the
-    // list size a host round trip); after it the size is checked every
-    // second round.
+    // four rounds more, so the first batch has those too (the context's
+    // refsortExtra; a round over an empty list changes nothing and costs four
+    // short launches, a look at the list size a host round trip); after it
+    // the size is checked every second round.
     uint64_t maxRoot = 0;
     for (const Seg& r : roots[0]) maxRoot = std::max<uint64_t>(maxRoot, r.last - r.first);
-    int batch = 4;
-    while (((uint64_t)kLdsN << (batch - 4)) < maxRoot) ++batch;
+    int batch = c->refsortExtra;
+    while (((uint64_t)kLdsN << (batch - c->refsortExtra)) < maxRoot) ++batch;
     L.big = cur;
     L.nbig = ncur;
     k_g_plan<<<1, kPlanT, 0, c->stream>>>(nullptr, nullptr, Lpos, Rpos, L, val, idx, st, blk0);
@@ -821,6 +823,11 @@ void refsort_ranges(svd_ctx* c, uint32_t* idx, uint32_t* val, uint64_t n, const 
         check_launch("refsort round");
         std::swap(cur, nxt);
         std::swap(ncur, nnxt);
+      }
+      if (unfinished) {   // the caller reads the list size later, with its own read-back
+        k_copy_u32<<<1, 1, 0, c->stream>>>(ncur, unfinished);
+        check_launch("k_copy_u32");
+        break;
       }
       if (!read_u32(c, ncur)) break;
       batch = 2;

@@ -52,6 +52,7 @@ struct svd_ctx {
   uint32_t traceMode = SVD_TRACE_TILED;      // SVD_TRACE_TILED or SVD_TRACE_PERSISTENT
   int numCU = 0;                             // compute units of the device
   bool sdagChecked = false;                  // toSDAG: check convergence after every round (SVD_TOSDAG_CHECKED=1)
+  int refsortExtra = 4;                      // refcount order: rounds past the size bound before the first look
   std::map<uint64_t, int> occupancy;         // (kernel id, LDS bytes) -> resident workgroups per CU
   float* renderScratch = nullptr;            // device address of svd_render_noalloc's static texture
   hipError_t renderScratchErr = hipSuccess;  // why it is null, if it is
@@ -213,8 +214,12 @@ void cross_level_merge(svd_octree* o);
 void refsort(svd_ctx* c, uint32_t* idx, uint32_t* val, uint64_t n);
 // Several independent std::sort calls at once: range r is [rfirst[r], rlast[r])
 // of the same arrays (n elements in all).  The ranges share every round.
+// unfinished (optional device word): run the device-wide rounds the sizes
+// call for without reading the list size back, and leave the count of
+// segments still too big for a workgroup there (nonzero: the order is not
+// finished and the caller must sort again without it).
 void refsort_ranges(svd_ctx* c, uint32_t* idx, uint32_t* val, uint64_t n, const uint64_t* rfirst,
-                    const uint64_t* rlast, size_t nr);
+                    const uint64_t* rlast, size_t nr, uint32_t* unfinished = nullptr);
 
 // ---- attribute bit-tree (attr.hip) ---------------------------------------------
 uint64_t attr_bittree(svd_ctx* c, const uint8_t* dattrs, uint64_t V, int gray, uint8_t* dst, uint64_t cap);

@@ -86,6 +86,36 @@ def test_build_levels_and_files_bit_exact(ctx, name):
         assert got == want, f"format {fmt} differs at byte {next(i for i in range(len(got)) if got[i] != want[i])}"
 
 
+def test_refcount_order_unfinished_batch_reruns_checked(ctx, monkeypatch):
+    """The SSVDAG-layout encoder runs the refcount order's device rounds
+    without reading the list size and finds out with its layout read whether
+    they were enough; if not it encodes again with the checked loop.  With no
+    rounds past the size bound (SVD_REFSORT_EXTRA_ROUNDS=0, read when a
+    context is created) the first pass is unfinished on these scenes, and the
+    files must still equal the oracle's; so must those of encode_many, which
+    reads the word encoders' sizes with the same read."""
+    codes, L = W.terrain_codes(9, seed=2), 9
+    _, _, _, files_o = oracle_pipeline(codes, L)
+    monkeypatch.setenv("SVD_REFSORT_EXTRA_ROUNDS", "0")
+    short = S.Context(0)
+    monkeypatch.delenv("SVD_REFSORT_EXTRA_ROUNDS")
+    try:
+        for c in (ctx, short):
+            _, _, _, _, files_g = gpu_pipeline(c, codes, L)
+            for f in (S.FMT_SVDAG, S.FMT_ESVDAG, S.FMT_USSVDAG, S.FMT_SSVDAG):
+                assert files_g[f] == files_o[f], S.FMT_EXT[f]
+            g = S.Octree.from_morton(c, codes, L)
+            g.to_dag()
+            g.to_sdag()
+            fm = [S.FMT_SVDAG, S.FMT_ESVDAG, S.FMT_USSVDAG, S.FMT_SSVDAG]
+            g.encode_many(fm)
+            for f in fm:
+                assert g.encoded_bytes(f) == files_o[f], S.FMT_EXT[f]
+            g.free()
+    finally:
+        short.close()
+
+
 def test_tosdag_checked_rounds_equal_speculative(ctx, monkeypatch):
     """toSDAG runs two resolution rounds per level without host reads and
     checks convergence once at the end; SVD_TOSDAG_CHECKED=1 (read when a
