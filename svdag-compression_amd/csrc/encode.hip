@@ -174,9 +174,13 @@ __global__ void k_level_iota(Span a, uint64_t n, uint32_t* __restrict__ perm, ui
   if (i < nz) zero[i] = 0u;
 }
 
-__global__ void k_scatter_rank(const uint32_t* __restrict__ perm, uint64_t n, uint32_t* __restrict__ ind) {
-  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) ind[perm[i]] = (uint32_t)i;
+// The rank of every brick in its set's order, ind[perm[i]] = i, for one or two
+// sets' brick levels in one launch (elements n0.. are the second set's).
+__global__ void k_scatter_rank2(const uint32_t* __restrict__ perm0, uint64_t n0, uint32_t* __restrict__ ind0,
+                                const uint32_t* __restrict__ perm1, uint64_t n1, uint32_t* __restrict__ ind1) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n0) ind0[perm0[i]] = (uint32_t)i;
+  else if (i < n0 + n1) ind1[perm1[i - n0]] = (uint32_t)(i - n0);
 }
 
 // Encoded size (u16 words) of the node at sorted position i.
@@ -562,6 +566,11 @@ void encode_ssvdag_sets(svd_octree* o, const int* fmts, int K, WordsPending* wp 
   lt.err = err;
   lt.top[0] = lt.top[1] = -1;
   for (int lev = L - 2; lev >= 0; --lev) {
+    // the level's passes of both sets go out together: one launch each
+    int nj = 0;
+    LayoutOp jop[2];
+    uint64_t jn[2] = {0, 0};
+    uint32_t* jtot[2] = {nullptr, nullptr};
     for (int s = 0; s < K; ++s) {
       if (lt.top[s] >= 0) continue;   // handled by the launch at the end
       const Level& v = (*lvs[s])[lev];
@@ -569,8 +578,10 @@ void encode_ssvdag_sets(svd_octree* o, const int* fmts, int K, WordsPending* wp 
       const int q = s * L + lev;
       ind[q] = dalloc<uint32_t>(c, n + 1);
       if (lev == L - 2) {
-        if (n) k_scatter_rank<<<div_up(n, kB), kB, 0, c->stream>>>(perm[q], n, ind[q]);
-        check_launch("k_scatter_rank");
+        jn[nj] = n;
+        jop[nj].perm = perm[q];
+        jop[nj].ind = ind[q];
+        ++nj;
         continue;
       }
       offs[q] = dalloc<uint32_t>(c, n + 1);
@@ -594,7 +605,22 @@ void encode_ssvdag_sets(svd_octree* o, const int* fmts, int K, WordsPending* wp 
         lt.ind[s][lev + 1] = ind[q + 1];
         continue;
       }
-      scan_op<1>(c, n, LayoutOp{v.rec, perm[q], ind[q + 1], offs[q], ind[q], err}, dev + q);
+      jop[nj] = LayoutOp{v.rec, perm[q], ind[q + 1], offs[q], ind[q], err};
+      jn[nj] = n;
+      jtot[nj] = dev + q;
+      ++nj;
+    }
+    if (lev == L - 2) {
+      const uint64_t nt = jn[0] + (nj > 1 ? jn[1] : 0);
+      if (nt) k_scatter_rank2<<<div_up(nt, kB), kB, 0, c->stream>>>(jop[0].perm, jn[0], jop[0].ind,
+                                                                      nj > 1 ? jop[1].perm : nullptr,
+                                                                      nj > 1 ? jn[1] : 0,
+                                                                      nj > 1 ? jop[1].ind : nullptr);
+      check_launch("k_scatter_rank2");
+    } else if (nj == 2) {
+      scan_op2<1>(c, jn[0], jop[0], jtot[0], jn[1], jop[1], jtot[1]);
+    } else if (nj == 1) {
+      scan_op<1>(c, jn[0], jop[0], jtot[0]);
     }
   }
   {

@@ -48,18 +48,22 @@ __device__ __forceinline__ uint32_t lb_take_tile(const LbPass& p, uint32_t* ldsW
 // every one gets the prefix.  Publishes the tile's inclusive prefix.  The
 // whole workgroup looks back, blockDim.x tiles per round trip: thread j reads
 // the word of tile look - j.  lds: blockDim.x / 64 + 2 words.
-__device__ __forceinline__ uint32_t lb_exclusive(const LbPass& p, uint32_t tile, uint32_t agg, uint32_t* lds) {
+// first: the first tile of the calling tile's scan (several independent scans
+// can share one pass, one after another; the tiles before `first` read as
+// an inclusive zero).
+__device__ __forceinline__ uint32_t lb_exclusive(const LbPass& p, uint32_t tile, uint32_t agg, uint32_t* lds,
+                                                 uint32_t first = 0) {
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nw = blockDim.x >> 6;
   uint32_t excl = 0;
-  if (tile == 0) {
-    if (tid == 0) lb_publish(p, 0, kLbInclusive, agg);
+  if (tile == first) {
+    if (tid == 0) lb_publish(p, tile, kLbInclusive, agg);
   } else {
     if (tid == 0) lb_publish(p, tile, kLbAggregate, agg);
     int64_t look = (int64_t)tile - 1;
     while (true) {
       const int64_t q = look - tid;
-      unsigned long long w = kLbInclusive;   // before tile 0: an inclusive zero
-      if (q >= 0) {
+      unsigned long long w = kLbInclusive;   // before the scan's first tile: an inclusive zero
+      if (q >= (int64_t)first) {
         while (true) {
           w = __hip_atomic_load(&p.status[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
           if ((uint32_t)(w >> 34) == p.epoch && (w & (3ull << 32)) != 0) break;

@@ -8,6 +8,7 @@
 // elements' loads would keep them from overlapping).  The value of element i
 // may not depend on another element's apply.
 #pragma once
+#include <algorithm>
 #include "svd_internal.hpp"
 #include "block_scan.hpp"
 #include "lookback.hpp"
@@ -67,6 +68,70 @@ __global__ void __launch_bounds__(kScanOpT) k_scan_op(uint64_t n, LbPass p, Op o
     if (i < n) op.apply(i, off + ex[k]);
   }
   if (total && tile + 1 == p.tiles && threadIdx.x == 0) *total = off + agg;   // wave 0's offset is the tile's prefix
+}
+
+// Two independent value-scan-apply passes (op0 over n0 elements, op1 over
+// n1) in one launch: the first T0 tiles are op0's, the rest op1's, each with
+// its own look-back chain in the shared pass.
+template <int PER, typename Op>
+__global__ void __launch_bounds__(kScanOpT) k_scan_op2(uint64_t n0, uint64_t n1, uint32_t T0, LbPass p, Op op0,
+                                                       Op op1, uint32_t* __restrict__ total0,
+                                                       uint32_t* __restrict__ total1) {
+  constexpr uint32_t kTile = kScanOpT * PER;
+  __shared__ uint32_t lds[kScanOpT / 64 + 2];
+  __shared__ uint32_t word;
+  const uint32_t tile = lb_take_tile(p, &word);
+  const bool second = tile >= T0;
+  const Op& op = second ? op1 : op0;
+  const uint64_t n = second ? n1 : n0;
+  const uint32_t first = second ? T0 : 0u;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint64_t base = (uint64_t)(tile - first) * kTile + (uint64_t)wave * (64 * PER) + lane;
+  uint32_t ex[PER];
+  uint32_t flags = 0;
+#pragma unroll
+  for (int k = 0; k < PER; ++k) {
+    const uint64_t i = base + 64 * k;
+    ex[k] = i < n ? op.value(i, flags) : 0u;
+  }
+  if (flags) op.report(flags);
+  uint32_t run = 0;
+#pragma unroll
+  for (int k = 0; k < PER; ++k) {
+    const uint32_t v = ex[k];
+    const uint32_t inc = wave_inclusive_sum(v);
+    ex[k] = run + inc - v;
+    run += (uint32_t)__builtin_amdgcn_readlane((int)inc, 63);
+  }
+  if (lane == 0) lds[wave] = run;
+  __syncthreads();
+  uint32_t off = 0, agg = 0;
+#pragma unroll
+  for (int w = 0; w < kScanOpT / 64; ++w) {
+    const uint32_t x = lds[w];
+    if (w < wave) off += x;
+    agg += x;
+  }
+  __syncthreads();
+  off += lb_exclusive(p, tile, agg, lds, first);
+#pragma unroll
+  for (int k = 0; k < PER; ++k) {
+    const uint64_t i = base + 64 * k;
+    if (i < n) op.apply(i, off + ex[k]);
+  }
+  uint32_t* total = second ? total1 : total0;
+  const uint32_t last = second ? p.tiles - 1 : T0 - 1;
+  if (total && tile == last && threadIdx.x == 0) *total = off + agg;
+}
+
+template <int PER, typename Op>
+void scan_op2(svd_ctx* c, uint64_t n0, const Op& op0, uint32_t* total0, uint64_t n1, const Op& op1,
+              uint32_t* total1) {
+  const uint64_t T0 = div_up(std::max<uint64_t>(n0, 1), (uint64_t)kScanOpT * PER);
+  const uint64_t T1 = div_up(std::max<uint64_t>(n1, 1), (uint64_t)kScanOpT * PER);
+  k_scan_op2<PER, Op><<<(uint32_t)(T0 + T1), kScanOpT, 0, c->stream>>>(n0, n1, (uint32_t)T0, lb_pass(c, T0 + T1),
+                                                                       op0, op1, total0, total1);
+  check_launch("scan_op2");
 }
 
 // total (device, may be null) gets the sum of all values.
