@@ -781,15 +781,20 @@ __global__ void k_leaf_remap(const uint32_t* __restrict__ mask, uint64_t n, cons
 }
 
 // ---------------------------------------------------------------- symmetry
-// Transform order of the reference lookup: I, X, Y, Z, XY, XZ, YZ, XYZ.
-__device__ __constant__ uint8_t kFlip[8] = {0, 4, 2, 1, 6, 5, 3, 7};
+// Transform order of the reference lookup: I, X, Y, Z, XY, XZ, YZ, XYZ, as
+// child-index xor masks {0, 4, 2, 1, 6, 5, 3, 7}, nibble t of one constant
+// (no table load in a lane's dependency chain).
+__device__ __forceinline__ uint32_t flip_of(int t) { return (0x73561240u >> (4 * t)) & 0xFu; }
 
-__device__ __forceinline__ uint32_t perm8(uint32_t m, uint32_t flip) {
-  uint32_t r = 0;
-#pragma unroll
-  for (int c = 0; c < 8; ++c) r |= ((m >> c) & 1u) << (c ^ flip);
-  return r;
+// Bit c of every byte of w moves to bit c ^ flip of that byte: one
+// conditional swap of bit groups per flip bit.
+__device__ __forceinline__ uint32_t perm_bytes(uint32_t w, uint32_t flip) {
+  if (flip & 1u) w = ((w & 0x55555555u) << 1) | ((w >> 1) & 0x55555555u);
+  if (flip & 2u) w = ((w & 0x33333333u) << 2) | ((w >> 2) & 0x33333333u);
+  if (flip & 4u) w = ((w & 0x0F0F0F0Fu) << 4) | ((w >> 4) & 0x0F0F0F0Fu);
+  return w;
 }
+__device__ __forceinline__ uint32_t perm8(uint32_t m, uint32_t flip) { return perm_bytes(m & 0xFFu, flip); }
 
 // Invariance bits of a node's children, one 8-bit mask per axis (bit c =
 // child c in the node's own order), loaded once per node; zero without a
@@ -815,26 +820,43 @@ __device__ __forceinline__ ChildInv child_inv(const uint32_t k0[kRecWords], cons
 // then GeomOctree::invertInvs (geom_octree.cpp:822-831): a mirrored child
 // that is invariant under the axis keeps its mirror bit clear.  Child c of
 // the variant is child c ^ flip of the node, so the invariance masks are
-// permuted like the children.
+// permuted like the children.  The children move by conditional swaps (one
+// per flip bit) rather than by an index computed at run time, and the four
+// mask bytes of word 0 are permuted together.
 __device__ __forceinline__ void make_variant(const uint32_t k0[kRecWords], int t, const ChildInv& inv,
                                              uint32_t v[kRecWords]) {
-  const uint32_t flip = kFlip[t];
+  const uint32_t flip = flip_of(t);
   const uint32_t tx = (flip >> 2) & 1u, ty = (flip >> 1) & 1u, tz = flip & 1u;
-  uint32_t mask = perm8(k0[0] & 0xFFu, flip);
-  uint32_t mx = perm8((k0[0] >> 8) & 0xFFu, flip);
-  uint32_t my = perm8((k0[0] >> 16) & 0xFFu, flip);
-  uint32_t mz = perm8((k0[0] >> 24) & 0xFFu, flip);
+  const uint32_t pw = perm_bytes(k0[0], flip);
+  const uint32_t mask = pw & 0xFFu;
+  uint32_t mx = (pw >> 8) & 0xFFu, my = (pw >> 16) & 0xFFu, mz = pw >> 24;
+  uint32_t ch[8];
 #pragma unroll
-  for (int c = 0; c < 8; ++c) v[1 + (c ^ flip)] = k0[1 + c];
+  for (int c = 0; c < 8; ++c) ch[c] = k0[1 + c];
+#pragma unroll
+  for (int b = 1; b <= 4; b <<= 1) {
+    const bool sw = (flip & (uint32_t)b) != 0;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      if (c & b) continue;
+      const uint32_t lo = ch[c], hi = ch[c | b];
+      ch[c] = sw ? hi : lo;
+      ch[c | b] = sw ? lo : hi;
+    }
+  }
   uint32_t ex = 0;
 #pragma unroll
-  for (int c = 0; c < 8; ++c) ex |= (v[1 + c] != kNull ? 1u : 0u) << c;
+  for (int c = 0; c < 8; ++c) {
+    v[1 + c] = ch[c];
+    ex |= (ch[c] != kNull ? 1u : 0u) << c;
+  }
   if (tx) mx ^= ex;
   if (ty) my ^= ex;
   if (tz) mz ^= ex;
-  if (tx) mx &= ~perm8(inv.x, flip);
-  if (ty) my &= ~perm8(inv.y, flip);
-  if (tz) mz &= ~perm8(inv.z, flip);
+  const uint32_t pinv = perm_bytes(inv.x | (inv.y << 8) | (inv.z << 16), flip);
+  if (tx) mx &= ~(pinv & 0xFFu);
+  if (ty) my &= ~((pinv >> 8) & 0xFFu);
+  if (tz) mz &= ~((pinv >> 16) & 0xFFu);
   v[0] = mask | (mx << 8) | (my << 16) | (mz << 24);
 }
 
@@ -980,7 +1002,7 @@ __global__ void k_sdag_outcome(const uint32_t* __restrict__ key, uint64_t n, con
     for (int t = 1; t < 8; ++t) {
       const uint32_t r = cd[t - 1];
       if (r != kNone && state[r] == kRep) {
-        const uint32_t f = kFlip[t];
+        const uint32_t f = flip_of(t);
         map[i] = pos[r] | (((f >> 2) & 1u) << 29) | (((f >> 1) & 1u) << 30) | ((f & 1u) << 31);
         return;
       }
@@ -1007,7 +1029,7 @@ __global__ void k_sdag_outcome(const uint32_t* __restrict__ key, uint64_t n, con
     if (t == 0) r = si == kNone ? kNone : entry_index(table[si]);
     else { make_variant(k0, t, ci, v); r = lookup(key, table, cmask, v); }
     if (r != kNone && r < (uint32_t)i && state[r] == kRep) {
-      const uint32_t f = kFlip[t];
+      const uint32_t f = flip_of(t);
       map[i] = pos[r] | (((f >> 2) & 1u) << 29) | (((f >> 1) & 1u) << 30) | ((f & 1u) << 31);
       return;
     }
@@ -1168,7 +1190,7 @@ __device__ void sdag_small_level(const uint32_t* __restrict__ dag, uint32_t n, c
       for (int t = 1; t < 8 && !found; ++t) {
         const uint32_t r = cd[t - 1];
         if (r != kNone && state[r] == kRep) {
-          const uint32_t f = kFlip[t];
+          const uint32_t f = flip_of(t);
           map[i] = posOf[r] | (((f >> 2) & 1u) << 29) | (((f >> 1) & 1u) << 30) | ((f & 1u) << 31);
           found = true;
         }
@@ -1195,7 +1217,7 @@ __device__ void sdag_small_level(const uint32_t* __restrict__ dag, uint32_t n, c
       if (t == 0) r = slotOf[i] != 0xFFFFu ? entry_index(table[slotOf[i]]) : kNone;
       else { make_variant(k0, t, ci, v); r = find(v); }
       if (r != kNone && r < i && state[r] == kRep) {
-        const uint32_t f = kFlip[t];
+        const uint32_t f = flip_of(t);
         map[i] = posOf[r] | (((f >> 2) & 1u) << 29) | (((f >> 1) & 1u) << 30) | ((f & 1u) << 31);
         found = true;
       }
@@ -1286,7 +1308,7 @@ __global__ void __launch_bounds__(256) k_sdag_leaves(const uint32_t* __restrict_
     }
     const uint32_t mr = dagMask[first] & 0xFFu;
     for (int k = 0; k < 8; ++k) {
-      const uint32_t f = kFlip[k];
+      const uint32_t f = flip_of(k);
       if (perm8(m, f) == mr) {
         map[i] = rank | (((f >> 2) & 1u) << 29) | (((f >> 1) & 1u) << 30) | ((f & 1u) << 31);
         break;
