@@ -57,7 +57,13 @@ int svd_ctx_create(int device, svd_ctx** out);
 int svd_ctx_destroy(svd_ctx* ctx);
 const char* svd_last_error(const svd_ctx* ctx);
 int svd_ctx_synchronize(svd_ctx* ctx);
-/* The HIP stream all work of this context is enqueued on (hipStream_t). */
+/* The HIP stream all work of this context is enqueued on (hipStream_t).
+ * svd_octree_from_morton with device codes, svd_reduce_dag, svd_reduce_sym
+ * and svd_encode_formats may return while their last kernels still run on
+ * it (they have read back every size they need; the codes have been read).
+ * Every call that returns data (counts, levels, sizes, copies, device
+ * pointers) is ordered after them; work of the caller's own on other streams
+ * is not: svd_ctx_synchronize first.                                        */
 void* svd_ctx_stream(svd_ctx* ctx);
 
 /* ---- construction --------------------------------------------------------
@@ -176,7 +182,8 @@ int svd_encode(svd_octree* oct, int fmt, uint64_t* file_bytes);
  * encodes all four, main.cpp:220-271), with the same images.  .ssvdag and
  * .esvdag share every pass: one reference-count launch, one segmented
  * refcount order over the levels of both (each level still its own std::sort
- * call), one layout read; .svdag and .ussvdag share one size read.
+ * call), one layout read; .svdag and .ussvdag share one size read, the same
+ * read when both kinds are asked for.
  * file_bytes (may be NULL) receives n sizes in the order of fmts.          */
 int svd_encode_formats(svd_octree* oct, const int* fmts, uint32_t n, uint64_t* file_bytes);
 int svd_encoded_copy(svd_octree* oct, int fmt, void* dst, uint64_t cap);

@@ -73,6 +73,15 @@ void read_u32s_2(svd_ctx* c, const uint32_t* d1, uint32_t* h1, size_t n1, const 
   memcpy(h2, stage + n1, 4 * n2);
 }
 
+double timer_ms(svd_ctx* c, const std::string& name) {
+  auto it = c->timers.find(name);
+  if (it == c->timers.end() || !it->second.second) return -1.0;
+  if (hipEventSynchronize(it->second.second) != hipSuccess) return -1.0;
+  float ms = 0.f;
+  if (hipEventElapsedTime(&ms, it->second.first, it->second.second) != hipSuccess) return -1.0;
+  return ms;
+}
+
 uint64_t read_u32(svd_ctx* c, const uint32_t* d) {
   uint32_t v = 0;
   read_u32s(c, d, &v, 1);
@@ -174,6 +183,10 @@ int svd_ctx_destroy(svd_ctx* c) {
   if (c->pinned) (void)hipHostFree(c->pinned);
   if (c->ev[0]) (void)hipEventDestroy(c->ev[0]);
   if (c->ev[1]) (void)hipEventDestroy(c->ev[1]);
+  for (auto& kv : c->timers) {
+    if (kv.second.first) (void)hipEventDestroy(kv.second.first);
+    if (kv.second.second) (void)hipEventDestroy(kv.second.second);
+  }
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return SVD_OK;
@@ -196,7 +209,9 @@ int svd_octree_from_morton(svd_ctx* c, const uint64_t* codes, uint64_t n, uint32
   o->ctx = c;
   o->levels = levels;
   int rc = guarded(c, [&] {
-    Timer t(c, "build_svo");
+    // device codes: return once the work is enqueued (host codes: their copy
+    // is waited for)
+    Timer t(c, "build_svo", !on_device);
     const uint64_t* d = codes;
     uint64_t* tmp = nullptr;
     if (!on_device && n) {
@@ -487,7 +502,7 @@ int svd_octree_free(svd_octree* o) {
 int svd_reduce_dag(svd_octree* o) {
   if (!o) return SVD_ERR_ARG;
   return guarded(o->ctx, [&] {
-    Timer t(o->ctx, "to_dag");
+    Timer t(o->ctx, "to_dag", false);
     reduce_dag(o);
     free_levels(o->ctx, o->xdag);
     o->hasX = false;
@@ -498,7 +513,7 @@ int svd_reduce_dag(svd_octree* o) {
 int svd_reduce_sym(svd_octree* o) {
   if (!o) return SVD_ERR_ARG;
   return guarded(o->ctx, [&] {
-    Timer t(o->ctx, "to_sdag");
+    Timer t(o->ctx, "to_sdag", false);
     reduce_sym(o);
   });
 }
@@ -599,7 +614,7 @@ int svd_encode_formats(svd_octree* o, const int* fmts, uint32_t n, uint64_t* byt
   for (uint32_t k = 0; k < n; ++k)
     if (fmts[k] < 0 || fmts[k] > 3) return SVD_ERR_ARG;
   return guarded(o->ctx, [&] {
-    Timer t(o->ctx, "encode");
+    Timer t(o->ctx, "encode", false);
     encode_formats(o, fmts, (int)n);
     if (bytes)
       for (uint32_t k = 0; k < n; ++k) bytes[k] = o->enc[fmts[k]].bytes;
@@ -620,6 +635,9 @@ int svd_encoded_copy(svd_octree* o, int fmt, void* dst, uint64_t cap) {
 int svd_encoded_device(svd_octree* o, int fmt, const void** dptr, uint64_t* bytes) {
   if (!o || fmt < 0 || fmt > 3 || !dptr) return SVD_ERR_ARG;
   if (!o->enc[fmt].valid) return SVD_ERR_STATE;
+  // the image may still be being written on the context stream: a pointer
+  // handed out is to finished bytes
+  if (hipStreamSynchronize(o->ctx->stream) != hipSuccess) return SVD_ERR_DEVICE;
   *dptr = o->enc[fmt].d;
   if (bytes) *bytes = o->enc[fmt].bytes;
   return SVD_OK;
@@ -833,9 +851,9 @@ int svd_attr_bittree_encode(svd_ctx* c, const uint8_t* attrs, uint64_t n, int gr
 
 int svd_last_kernel_ms(svd_ctx* c, const char* name, double* ms) {
   if (!c || !name || !ms) return SVD_ERR_ARG;
-  auto it = c->kernel_ms.find(name);
-  if (it == c->kernel_ms.end()) return SVD_ERR_ARG;
-  *ms = it->second;
+  const double v = timer_ms(c, name);
+  if (v < 0) return SVD_ERR_ARG;
+  *ms = v;
   return SVD_OK;
 }
 

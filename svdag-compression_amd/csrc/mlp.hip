@@ -752,7 +752,7 @@ void mlp_train(svd_mlp* m, svd_octree* o, const svd_mlp_train_params* p, svd_mlp
     for (uint32_t i = 0; i < q; ++i) { a += hist[i]; b += hist[p->steps - q + i]; }
     st->first_loss = (float)(a / q);
     st->last_loss = (float)(b / q);
-    st->ms_train = c->kernel_ms["mlp_train"];
+    st->ms_train = timer_ms(c, "mlp_train");
     st->nodes = n;
     st->pairs = P * p->steps;
   }

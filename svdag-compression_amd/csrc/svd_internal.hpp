@@ -47,7 +47,9 @@ struct svd_ctx {
   svd::Arena* arena = nullptr;
   uint64_t* pinned = nullptr;                // small pinned staging for size read-backs
   hipEvent_t ev[2] = {nullptr, nullptr};
-  std::map<std::string, double> kernel_ms;   // last measured per family
+  // per timed family: events around its last call (the elapsed time is read
+  // when asked for, so a call need not wait for its own kernels)
+  std::map<std::string, std::pair<hipEvent_t, hipEvent_t>> timers;
   uint32_t traceTileW = 16;                  // traversal workgroup tile: traceTileW x (256 / traceTileW) pixels
   uint32_t traceMode = SVD_TRACE_TILED;      // SVD_TRACE_TILED or SVD_TRACE_PERSISTENT
   int numCU = 0;                             // compute units of the device
@@ -284,18 +286,30 @@ uint64_t gen_terrain(svd_ctx* c, uint32_t levels, uint64_t seed, uint32_t octave
                      double baseCells, uint64_t** out);
 
 // ---- timing helpers ------------------------------------------------------------
+// Events around one API call on the context stream.  wait: return only once
+// the call's kernels are done (false for calls whose results stay in the
+// library until another call, itself on the stream, reads them).
 struct Timer {
   svd_ctx* c;
-  const char* name;
-  Timer(svd_ctx* ctx, const char* n) : c(ctx), name(n) { (void)hipEventRecord(c->ev[0], c->stream); }
-  ~Timer() {
-    (void)hipEventRecord(c->ev[1], c->stream);
-    if (hipEventSynchronize(c->ev[1]) == hipSuccess) {
-      float ms = 0.f;
-      if (hipEventElapsedTime(&ms, c->ev[0], c->ev[1]) == hipSuccess) c->kernel_ms[name] = ms;
+  hipEvent_t end = nullptr;
+  bool wait;
+  Timer(svd_ctx* ctx, const char* n, bool w = true) : c(ctx), wait(w) {
+    auto& ev = c->timers[n];
+    if (!ev.first && (hipEventCreate(&ev.first) != hipSuccess || hipEventCreate(&ev.second) != hipSuccess)) {
+      (void)hipGetLastError();
+      return;
     }
+    end = ev.second;
+    (void)hipEventRecord(ev.first, c->stream);
+  }
+  ~Timer() {
+    if (!end) return;
+    (void)hipEventRecord(end, c->stream);
+    if (wait) (void)hipEventSynchronize(end);
   }
 };
+// Elapsed ms of the family's last timed call (waits for it); < 0 if none.
+double timer_ms(svd_ctx* c, const std::string& name);
 
 // ---- device helpers --------------------------------------------------------------
 __device__ __forceinline__ uint64_t mix64(uint64_t z) {
