@@ -170,6 +170,13 @@ int svd_ctx_create(int device, svd_ctx** out) {
     // size bound before the list size is first looked at (default 4; 0..16)
     const char* r = std::getenv("SVD_REFSORT_EXTRA_ROUNDS");
     if (r && r[0]) c->refsortExtra = std::min(16, std::max(0, std::atoi(r)));
+    const char* sl = std::getenv("SVD_SMALL_LEVELS");
+    if (sl && sl[0]) {
+      unsigned v[4] = {0, 0, 0, 0};
+      if (std::sscanf(sl, "%u,%u,%u,%u", &v[0], &v[1], &v[2], &v[3]) == 4) {
+        c->smallSvo = v[0]; c->smallDag = v[1]; c->smallSdag = v[2]; c->smallLayout = v[3];
+      }
+    }
   }
   *out = c;
   return SVD_OK;

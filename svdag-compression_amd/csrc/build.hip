@@ -1390,7 +1390,7 @@ uint64_t svo_inner_levels(svd_ctx* c, uint64_t* codeA, uint64_t nc, uint32_t L, 
   uint64_t nNodes = 0;
   for (int lev = (int)L - 2; lev >= 0; --lev) {
     Level& lv = svo[lev];
-    if (sizes && nc <= kSvoSmall && !levelCodes) {
+    if (sizes && nc <= small_limit(c->smallSvo, kSvoSmall) && !levelCodes) {
       // this level and all above it in one launch
       SvoTop top{};
       top.top = lev;
@@ -1404,7 +1404,7 @@ uint64_t svo_inner_levels(svd_ctx* c, uint64_t* codeA, uint64_t nc, uint32_t L, 
       check_launch("k_inner_levels_small");
       break;
     }
-    if (sizes && nc <= kSvoSmall) {
+    if (sizes && nc <= small_limit(c->smallSvo, kSvoSmall)) {
       const uint64_t np = sizes[lev];
       lv.n = np;
       lv.rec = dalloc<uint32_t>(c, np * kRecWords);
@@ -1538,7 +1538,7 @@ uint64_t dedup_all(svd_ctx* c, uint32_t L, const std::vector<Level>& src, std::v
     // this level and all above it at most kSmallLevel nodes: one launch
     uint64_t maxTop = 0;
     for (int l = lev; l >= 1; --l) maxTop = std::max<uint64_t>(maxTop, src[l].n);
-    if (maxTop <= kSmallLevel) {
+    if (maxTop <= small_limit(c->smallDag, kSmallLevel)) {
       DagTop top{};
       top.top = lev;
       top.remapIn = remap;
@@ -1637,7 +1637,7 @@ bool reduce_sym_pass(svd_octree* o, bool speculative) {
     // this level and all above it at most kSdagSmall nodes: one launch
     uint64_t maxTop = 0;
     for (int l = lev; l >= 1; --l) maxTop = std::max<uint64_t>(maxTop, o->dag[l].n);
-    if (maxTop <= kSdagSmall) {
+    if (maxTop <= small_limit(c->smallSdag, kSdagSmall)) {
       dfree(c, key);
       dfree(c, inv);
       SdagTop top{};

@@ -587,7 +587,7 @@ void encode_ssvdag_sets(svd_octree* o, const int* fmts, int K, WordsPending* wp 
       offs[q] = dalloc<uint32_t>(c, n + 1);
       uint64_t maxTop = 0;
       for (int l = lev; l >= 0; --l) maxTop = std::max<uint64_t>(maxTop, (*lvs[s])[l].n);
-      if (maxTop <= kLayoutSmall) {
+      if (maxTop <= small_limit(c->smallLayout, kLayoutSmall)) {
         lt.top[s] = lev;
         for (int l = lev; l >= 0; --l) {
           const int ql = s * L + l;

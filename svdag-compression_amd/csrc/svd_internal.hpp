@@ -55,6 +55,12 @@ struct svd_ctx {
   int numCU = 0;                             // compute units of the device
   bool sdagChecked = false;                  // toSDAG: check convergence after every round (SVD_TOSDAG_CHECKED=1)
   int refsortExtra = 4;                      // refcount order: rounds past the size bound before the first look
+  // one-workgroup limits of the top levels (0: the compiled maximum; a limit
+  // above it is cut to it): SVO child codes, toDAG / toSDAG nodes, SSVDAG
+  // layout nodes (SVD_SMALL_LEVELS="svo,dag,sdag,layout").  The defaults were
+  // the fastest of a sweep on the 2048^3 terrain build (1.654 ms against 1.679
+  // ms for the compiled maxima 8192, 4096, 1024, 4096, interleaved runs).
+  uint32_t smallSvo = 2048, smallDag = 1024, smallSdag = 512, smallLayout = 2048;
   std::map<uint64_t, int> occupancy;         // (kernel id, LDS bytes) -> resident workgroups per CU
   float* renderScratch = nullptr;            // device address of svd_render_noalloc's static texture
   hipError_t renderScratchErr = hipSuccess;  // why it is null, if it is
@@ -161,6 +167,8 @@ inline T* dalloc(svd_ctx* c, uint64_t count) {
   return (T*)c->arena->get(count * sizeof(T) + 16);
 }
 inline void dfree(svd_ctx* c, void* p) { if (p) c->arena->put(p); }
+
+inline uint64_t small_limit(uint32_t want, uint32_t compiled) { return want && want < compiled ? want : compiled; }
 
 // Read back a few u32/u64 device values into host memory (synchronises).
 uint64_t read_u32(svd_ctx* c, const uint32_t* d);

@@ -116,6 +116,29 @@ def test_refcount_order_unfinished_batch_reruns_checked(ctx, monkeypatch):
         short.close()
 
 
+@pytest.mark.parametrize("limits", ["8192,4096,1024,4096", "1,1,1,1"])
+def test_one_workgroup_level_limits(ctx, monkeypatch, limits):
+    """The top levels of each pass go to one-workgroup kernels up to a size
+    limit per pass (SVD_SMALL_LEVELS, read when a context is created); at the
+    compiled maxima and with every such kernel off (limit 1) the levels and
+    files equal the oracle's."""
+    codes, L = W.terrain_codes(9, seed=2), 9
+    _, dag_o, sdag_o, files_o = oracle_pipeline(codes, L)
+    monkeypatch.setenv("SVD_SMALL_LEVELS", limits)
+    c = S.Context(0)
+    monkeypatch.delenv("SVD_SMALL_LEVELS")
+    try:
+        g, _, dag_g, sdag_g, files_g = gpu_pipeline(c, codes, L)
+        g.free()
+        for l in range(L):
+            np.testing.assert_array_equal(dag_g[l], dag_o[l], err_msg=f"DAG level {l}")
+            np.testing.assert_array_equal(sdag_g[l][0], sdag_o[l][0], err_msg=f"SDAG level {l}")
+        for f in (S.FMT_SVDAG, S.FMT_ESVDAG, S.FMT_USSVDAG, S.FMT_SSVDAG):
+            assert files_g[f] == files_o[f], S.FMT_EXT[f]
+    finally:
+        c.close()
+
+
 def test_tosdag_checked_rounds_equal_speculative(ctx, monkeypatch):
     """toSDAG runs two resolution rounds per level without host reads and
     checks convergence once at the end; SVD_TOSDAG_CHECKED=1 (read when a
