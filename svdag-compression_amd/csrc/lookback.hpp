@@ -51,10 +51,15 @@ __device__ __forceinline__ uint32_t lb_take_tile(const LbPass& p, uint32_t* ldsW
 // first: the first tile of the calling tile's scan (several independent scans
 // can share one pass, one after another; the tiles before `first` read as
 // an inclusive zero).
+// The caller may still be reading lds (a workgroup total it staged there):
+// the barrier on entry orders those reads before the look-back's writes, or
+// a wave that reached the look-back first would overwrite a word a slower
+// wave has not read yet.
 __device__ __forceinline__ uint32_t lb_exclusive(const LbPass& p, uint32_t tile, uint32_t agg, uint32_t* lds,
                                                  uint32_t first = 0) {
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nw = blockDim.x >> 6;
   uint32_t excl = 0;
+  __syncthreads();
   if (tile == first) {
     if (tid == 0) lb_publish(p, tile, kLbInclusive, agg);
   } else {

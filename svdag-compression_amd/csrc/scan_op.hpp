@@ -60,8 +60,7 @@ __global__ void __launch_bounds__(kScanOpT) k_scan_op(uint64_t n, LbPass p, Op o
     if (w < wave) off += x;
     agg += x;
   }
-  __syncthreads();
-  off += lb_exclusive(p, tile, agg, lds);
+  off += lb_exclusive(p, tile, agg, lds);   // its entry barrier orders the reads above
 #pragma unroll
   for (int k = 0; k < kScanOpPer; ++k) {
     const uint64_t i = base + 64 * k;
@@ -112,8 +111,7 @@ __global__ void __launch_bounds__(kScanOpT) k_scan_op2(uint64_t n0, uint64_t n1,
     if (w < wave) off += x;
     agg += x;
   }
-  __syncthreads();
-  off += lb_exclusive(p, tile, agg, lds, first);
+  off += lb_exclusive(p, tile, agg, lds, first);   // its entry barrier orders the reads above
 #pragma unroll
   for (int k = 0; k < PER; ++k) {
     const uint64_t i = base + 64 * k;
