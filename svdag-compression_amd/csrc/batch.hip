@@ -205,7 +205,7 @@ void batch_build(svd_batch* b, const uint64_t* dcodes, uint64_t n, const std::ve
   // ---- SVO of the whole batch
   std::vector<Level> svo;
   std::vector<uint64_t*> codes;
-  svo_levels_from_codes(c, tagged, n, Lm, svo, &codes);
+  svo_levels_from_codes(c, tagged, n, Lm, svo, &codes, 0);   // the tags sit above every scene's codes
   dfree(c, tagged);
   // scene of every SVO node, per level
   std::vector<uint32_t*> sid(Lm, nullptr);

@@ -98,7 +98,8 @@ __device__ __forceinline__ uint32_t run_tail_bits(const uint64_t* __restrict__ c
   return m;
 }
 
-// Input contract: strictly ascending codes below 2^(3*levels).
+// Input contract: strictly ascending codes with no bit of hiMask set (the
+// plain build: below 2^(3*levels); scene-tagged batch codes: any).
 // One pass over the codes for the finest level: the input check, every
 // level's node count, the run heads, their numbering by a single-pass scan,
 // and the leaf masks and parent codes.  Code i starts a node of the level k
@@ -113,7 +114,7 @@ __device__ __forceinline__ uint32_t run_tail_bits(const uint64_t* __restrict__ c
 constexpr int kLeafPer = 16;
 constexpr uint32_t kLeafTile = kRunT * kLeafPer;
 __global__ void __launch_bounds__(kRunT) k_leaf_build(const uint64_t* __restrict__ code, uint64_t n, uint32_t L,
-                                                      LbPass p, uint32_t* __restrict__ part,
+                                                      uint64_t hiMask, LbPass p, uint32_t* __restrict__ part,
                                                       uint32_t* __restrict__ mask, uint64_t* __restrict__ pcode) {
   __shared__ uint32_t lds[kRunT / 64 + 2];
   __shared__ uint32_t word;
@@ -141,7 +142,7 @@ __global__ void __launch_bounds__(kRunT) k_leaf_build(const uint64_t* __restrict
     const uint64_t u = lane ? up : wrap;
     if (i >= n) continue;
     uint32_t K = L;
-    bool bad = (x >> (3 * L)) != 0;   // L <= 21: the shift is at most 63
+    bool bad = (x & hiMask) != 0;
     if (i > 0) {
       if (x <= u) bad = true;
       else K = min(L, (uint32_t)(63 - __clzll((long long)(x ^ u))) / 3u);
@@ -272,47 +273,10 @@ __global__ void __launch_bounds__(1024) k_level_counts(const uint32_t* __restric
   }
 }
 
-__global__ void k_heads(const uint64_t* __restrict__ code, uint64_t n, uint32_t* __restrict__ head) {
-  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) head[i] = head64(code, i) ? 1u : 0u;
-}
-
-// Finest level: one node per run of voxels sharing code >> 3.
-__global__ void k_leaf_level(const uint64_t* __restrict__ code, uint64_t n, const uint32_t* __restrict__ pidx,
-                             uint32_t* __restrict__ mask, uint64_t* __restrict__ pcode) {
-  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n || !head64(code, i)) return;
-  const uint64_t p = code[i] >> 3;
-  uint32_t m = 0;
-  for (uint64_t j = i; j < n && j < i + 8 && (code[j] >> 3) == p; ++j) m |= 1u << (uint32_t)(code[j] & 7u);
-  mask[pidx[i]] = m;
-  pcode[pidx[i]] = p;
-}
-
-// Inner level: one node per run of child codes sharing code >> 3; the
-// child pointer is the child's position in its own level.
-__global__ void k_inner_level(const uint64_t* __restrict__ ccode, uint64_t n, const uint32_t* __restrict__ pidx,
-                              uint32_t* __restrict__ rec, uint64_t* __restrict__ pcode) {
-  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n || !head64(ccode, i)) return;
-  const uint64_t p = ccode[i] >> 3;
-  uint32_t w[kRecWords];
-  w[0] = 0;
-#pragma unroll
-  for (int c = 0; c < 8; ++c) w[1 + c] = kNull;
-  for (uint64_t j = i; j < n && j < i + 8 && (ccode[j] >> 3) == p; ++j) {
-    const uint32_t s = (uint32_t)(ccode[j] & 7u);
-    w[0] |= 1u << s;
-    w[1 + s] = (uint32_t)j;
-  }
-  uint32_t* o = rec + (uint64_t)pidx[i] * kRecWords;
-#pragma unroll
-  for (int k = 0; k < kRecWords; ++k) o[k] = w[k];
-  if (pcode) pcode[pidx[i]] = p;
-}
-
-// An inner SVO level from at most kSvoSmall child codes in one workgroup:
-// k_heads, the scan of the heads and k_inner_level with workgroup barriers.
+// An inner SVO level from at most kSvoSmall child codes in one workgroup: the
+// run heads (a code whose parent code differs from its predecessor's), their
+// scan, and one record per run (child pointer = the child's position in its
+// level), with workgroup barriers between the steps.
 constexpr uint32_t kSvoSmall = 8192;
 __global__ void __launch_bounds__(1024) k_inner_level_small(const uint64_t* __restrict__ ccode, uint32_t n,
                                                             uint32_t* __restrict__ rec, uint64_t* __restrict__ pcode) {
@@ -1344,13 +1308,75 @@ void build_svo_from_morton(svd_octree* o, const uint64_t* dcodes, uint64_t n) {
     SVD_HIP(hipStreamSynchronize(c->stream));
     return;
   }
-  // finest level, input check and level sizes in one pass over the codes
+  o->nNodesSVO = svo_levels_from_codes(c, dcodes, n, L, o->svo, nullptr, ~((1ull << (3 * L)) - 1));
+  o->state = SVD_S_SVO;
+}
+
+// Inner levels L-2 .. 0 from the finest level's parent codes (codeA, nc of
+// them, taken over and freed).  Returns the non-root nodes created.
+uint64_t svo_inner_levels(svd_ctx* c, uint64_t* codeA, uint64_t nc, uint32_t L, std::vector<Level>& svo,
+                          std::vector<uint64_t*>* levelCodes, const uint64_t* sizes) {
+  uint64_t* codeB = dalloc<uint64_t>(c, nc ? nc : 1);
+  auto keep = [&](uint32_t lev, const uint64_t* codes, uint64_t m) {
+    if (!levelCodes) return;
+    (*levelCodes)[lev] = dalloc<uint64_t>(c, m + 1);
+    if (m) SVD_HIP(hipMemcpyAsync((*levelCodes)[lev], codes, m * 8, hipMemcpyDeviceToDevice, c->stream));
+  };
+  uint64_t nNodes = 0;
+  for (int lev = (int)L - 2; lev >= 0; --lev) {
+    Level& lv = svo[lev];
+    if (nc <= small_limit(c->smallSvo, kSvoSmall) && !levelCodes) {
+      // this level and all above it in one launch
+      SvoTop top{};
+      top.top = lev;
+      for (int l = lev; l >= 0; --l) {
+        svo[l].n = sizes[l];
+        svo[l].rec = dalloc<uint32_t>(c, (sizes[l] ? sizes[l] : 1) * kRecWords);
+        top.rec[l] = svo[l].rec;
+        if (l > 0) nNodes += sizes[l];
+      }
+      k_inner_levels_small<<<1, 1024, 0, c->stream>>>(codeA, codeB, (uint32_t)nc, top);
+      check_launch("k_inner_levels_small");
+      break;
+    }
+    const uint64_t np = sizes[lev];
+    lv.n = np;
+    lv.rec = dalloc<uint32_t>(c, np * kRecWords);
+    if (nc <= small_limit(c->smallSvo, kSvoSmall)) {
+      k_inner_level_small<<<1, 1024, 0, c->stream>>>(codeA, (uint32_t)nc, lv.rec, codeB);
+      check_launch("k_inner_level_small");
+    } else {
+      const uint64_t tiles = div_up(nc, kInnerTile);
+      k_inner_build<<<(uint32_t)tiles, kRunT, 0, c->stream>>>(codeA, nc, lb_pass(c, tiles), lv.rec, codeB);
+      check_launch("k_inner_build");
+    }
+    keep((uint32_t)lev, codeB, lv.n);
+    std::swap(codeA, codeB);
+    nc = lv.n;
+    if (lev > 0) nNodes += lv.n;
+  }
+  dfree(c, codeA); dfree(c, codeB);
+  return nNodes;
+}
+
+// SVO levels from n > 0 strictly ascending codes with no bit of hiMask set,
+// in one pass over the codes for the finest level (k_leaf_build: the input
+// check, every level's node count, the leaf masks and parent codes) and one
+// per inner level.  Node order per level = order of first arrival.
+// levelCodes (optional) receives, per level, a device copy of every node's
+// code (code >> 3*(L-lev)).  Returns the number of non-root nodes; throws
+// SVD_ERR_ARG on bad input.
+uint64_t svo_levels_from_codes(svd_ctx* c, const uint64_t* dcodes, uint64_t n, uint32_t L, std::vector<Level>& svo,
+                               std::vector<uint64_t*>* levelCodes, uint64_t hiMask) {
+  svo.assign(L, Level());
+  if (levelCodes) levelCodes->assign(L, nullptr);
   const uint64_t tiles = div_up(n, kLeafTile);
   uint32_t* part = dalloc<uint32_t>(c, tiles * (L + 1));
   uint32_t* hist = dalloc<uint32_t>(c, kMaxLevels + 1);
   uint32_t* leafMask = dalloc<uint32_t>(c, n);
   uint64_t* codeA = dalloc<uint64_t>(c, n);
-  k_leaf_build<<<(uint32_t)tiles, kRunT, 0, c->stream>>>(dcodes, n, L, lb_pass(c, tiles), part, leafMask, codeA);
+  k_leaf_build<<<(uint32_t)tiles, kRunT, 0, c->stream>>>(dcodes, n, L, hiMask, lb_pass(c, tiles), part, leafMask,
+                                                         codeA);
   k_level_counts<<<1, 1024, 0, c->stream>>>(part, (uint32_t)tiles, L, hist);
   check_launch("k_leaf_build");
   uint32_t h[kMaxLevels + 1];
@@ -1365,105 +1391,16 @@ void build_svo_from_morton(svd_octree* o, const uint64_t* dcodes, uint64_t n) {
   // level lev is L - lev steps above the voxels
   std::vector<uint64_t> sizes(L);
   for (uint32_t lev = 0; lev < L; ++lev) sizes[lev] = h[L - lev];
-  Level& leaf = o->svo[L - 1];
+  Level& leaf = svo[L - 1];
   leaf.leaf = true;
   leaf.n = sizes[L - 1];
   leaf.rec = leafMask;
-  uint64_t nNodes = (L >= 2) ? leaf.n : 0;
-  nNodes += svo_inner_levels(c, codeA, leaf.n, L, o->svo, nullptr, sizes.data());
-  o->nNodesSVO = nNodes;   // non-root nodes created (GeomOctree::_nNodes after buildSVO)
-  o->state = SVD_S_SVO;
-}
-
-// Inner levels L-2 .. 0 from the finest level's parent codes (codeA, nc of
-// them, taken over and freed).  Returns the non-root nodes created.
-uint64_t svo_inner_levels(svd_ctx* c, uint64_t* codeA, uint64_t nc, uint32_t L, std::vector<Level>& svo,
-                          std::vector<uint64_t*>* levelCodes, const uint64_t* sizes) {
-  uint32_t* scanbuf = dalloc<uint32_t>(c, nc + 1);
-  uint32_t* total = dalloc<uint32_t>(c, 1);
-  uint64_t* codeB = dalloc<uint64_t>(c, nc ? nc : 1);
-  auto keep = [&](uint32_t lev, const uint64_t* codes, uint64_t m) {
-    if (!levelCodes) return;
-    (*levelCodes)[lev] = dalloc<uint64_t>(c, m + 1);
-    if (m) SVD_HIP(hipMemcpyAsync((*levelCodes)[lev], codes, m * 8, hipMemcpyDeviceToDevice, c->stream));
-  };
-  uint64_t nNodes = 0;
-  for (int lev = (int)L - 2; lev >= 0; --lev) {
-    Level& lv = svo[lev];
-    if (sizes && nc <= small_limit(c->smallSvo, kSvoSmall) && !levelCodes) {
-      // this level and all above it in one launch
-      SvoTop top{};
-      top.top = lev;
-      for (int l = lev; l >= 0; --l) {
-        svo[l].n = sizes[l];
-        svo[l].rec = dalloc<uint32_t>(c, (sizes[l] ? sizes[l] : 1) * kRecWords);
-        top.rec[l] = svo[l].rec;
-        if (l > 0) nNodes += sizes[l];
-      }
-      k_inner_levels_small<<<1, 1024, 0, c->stream>>>(codeA, codeB, (uint32_t)nc, top);
-      check_launch("k_inner_levels_small");
-      break;
-    }
-    if (sizes && nc <= small_limit(c->smallSvo, kSvoSmall)) {
-      const uint64_t np = sizes[lev];
-      lv.n = np;
-      lv.rec = dalloc<uint32_t>(c, np * kRecWords);
-      k_inner_level_small<<<1, 1024, 0, c->stream>>>(codeA, (uint32_t)nc, lv.rec, codeB);
-      check_launch("k_inner_level_small");
-    } else if (sizes) {
-      const uint64_t np = sizes[lev];
-      lv.n = np;
-      lv.rec = dalloc<uint32_t>(c, np * kRecWords);
-      const uint64_t tiles = div_up(nc, kInnerTile);
-      k_inner_build<<<(uint32_t)tiles, kRunT, 0, c->stream>>>(codeA, nc, lb_pass(c, tiles), lv.rec, codeB);
-      check_launch("k_inner_build");
-    } else {
-      k_heads<<<div_up(nc, kB), kB, 0, c->stream>>>(codeA, nc, scanbuf);
-      scan_exclusive(c, scanbuf, scanbuf, nc, total);
-      const uint64_t np = read_u32(c, total);
-      lv.n = np;
-      lv.rec = dalloc<uint32_t>(c, np * kRecWords);
-      k_inner_level<<<div_up(nc, kB), kB, 0, c->stream>>>(codeA, nc, scanbuf, lv.rec, codeB);
-      check_launch("k_inner_level");
-    }
-    keep((uint32_t)lev, codeB, lv.n);
-    std::swap(codeA, codeB);
-    nc = lv.n;
-    if (lev > 0) nNodes += lv.n;
-  }
-  dfree(c, scanbuf); dfree(c, total); dfree(c, codeA); dfree(c, codeB);
-  return nNodes;
-}
-
-// SVO levels from strictly ascending codes (the caller checked them).  Node
-// order per level = order of first arrival.  levelCodes (optional) receives,
-// per level, a device copy of every node's code (code >> 3*(L-lev)).
-// Returns the number of non-root nodes.
-uint64_t svo_levels_from_codes(svd_ctx* c, const uint64_t* dcodes, uint64_t n, uint32_t L, std::vector<Level>& svo,
-                               std::vector<uint64_t*>* levelCodes, const uint64_t* sizes) {
-  svo.assign(L, Level());
-  if (levelCodes) levelCodes->assign(L, nullptr);
-  uint32_t* scanbuf = dalloc<uint32_t>(c, n + 1);
-  uint32_t* total = dalloc<uint32_t>(c, 1);
-  uint64_t* codeA = dalloc<uint64_t>(c, n);
-  // finest level
-  k_heads<<<div_up(n, kB), kB, 0, c->stream>>>(dcodes, n, scanbuf);
-  scan_exclusive(c, scanbuf, scanbuf, n, total);
-  uint64_t nl = sizes ? sizes[L - 1] : read_u32(c, total);
-  Level& leaf = svo[L - 1];
-  leaf.leaf = true;
-  leaf.n = nl;
-  leaf.rec = dalloc<uint32_t>(c, nl);
-  k_leaf_level<<<div_up(n, kB), kB, 0, c->stream>>>(dcodes, n, scanbuf, leaf.rec, codeA);
-  check_launch("k_leaf_level");
-  dfree(c, scanbuf);
-  dfree(c, total);
   if (levelCodes) {
-    (*levelCodes)[L - 1] = dalloc<uint64_t>(c, nl + 1);
-    if (nl) SVD_HIP(hipMemcpyAsync((*levelCodes)[L - 1], codeA, nl * 8, hipMemcpyDeviceToDevice, c->stream));
+    (*levelCodes)[L - 1] = dalloc<uint64_t>(c, leaf.n + 1);
+    SVD_HIP(hipMemcpyAsync((*levelCodes)[L - 1], codeA, leaf.n * 8, hipMemcpyDeviceToDevice, c->stream));
   }
-  const uint64_t nNodes = (L >= 2) ? nl : 0;
-  return nNodes + svo_inner_levels(c, codeA, nl, L, svo, levelCodes, sizes);
+  const uint64_t nNodes = (L >= 2) ? leaf.n : 0;
+  return nNodes + svo_inner_levels(c, codeA, leaf.n, L, svo, levelCodes, sizes.data());
 }
 
 void remap_children(svd_ctx* c, const uint32_t* in, uint64_t n, const uint32_t* remap, uint32_t* out) {

@@ -201,10 +201,12 @@ void build_svo_from_points(svd_octree* o, const double* dxyz, uint64_t npts, con
 // buildDAG's joined SVO: top of stepLevel+1 levels + one voxelized subtree per set voxel of its finest level
 void build_svo_from_mesh_subtrees(svd_octree* o, const float* dtris, uint64_t ntris, const double bmin[3],
                                   const double bmax[3], uint32_t stepLevel);
-// sizes (optional, host): the node count of every level, known in advance
+// SVO levels of n > 0 strictly ascending codes with no bit of hiMask set; levelCodes
+// (optional) receives every level's node codes.  Returns the non-root nodes.
 uint64_t svo_levels_from_codes(svd_ctx* c, const uint64_t* dcodes, uint64_t n, uint32_t L, std::vector<Level>& svo,
-                               std::vector<uint64_t*>* levelCodes, const uint64_t* sizes = nullptr);
+                               std::vector<uint64_t*>* levelCodes, uint64_t hiMask);
 // inner levels L-2 .. 0 from the finest level's parent codes (taken over and freed)
+// and every level's node count (sizes, host)
 uint64_t svo_inner_levels(svd_ctx* c, uint64_t* codeA, uint64_t nc, uint32_t L, std::vector<Level>& svo,
                           std::vector<uint64_t*>* levelCodes, const uint64_t* sizes);
 // one inner level: exact-key dedup, first occurrence wins -> remap + compacted records
