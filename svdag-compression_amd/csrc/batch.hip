@@ -2,7 +2,7 @@
 // SVDAG images of all of them, and multi-camera traversal over the batch.
 //
 // Every scene's sorted Morton codes are tagged with the scene id above bit
-// 40 and the tagged list (ascending: scene-major) is built as ONE octree of
+// 40 (as the leaf pass reads them) and the tagged list (ascending: scene-major) is built as ONE octree of
 // depth Lmax = max depth: a node's code carries its scene id, so runs never
 // cross scenes and every level is scene-major with each scene's nodes in its
 // own first-arrival order.  A scene of depth L_s < Lmax occupies batched
@@ -20,36 +20,6 @@ namespace svd {
 namespace {
 
 constexpr int kB = 256;
-constexpr int kTagShift = 40;
-
-// Scene of code i: offsets[s] <= i < offsets[s+1] (binary search).
-__device__ __forceinline__ uint32_t scene_of(const uint64_t* __restrict__ off, uint32_t S, uint64_t i) {
-  uint32_t lo = 0, hi = S;
-  while (hi - lo > 1) {
-    const uint32_t mid = (lo + hi) >> 1;
-    if (off[mid] <= i) lo = mid; else hi = mid;
-  }
-  return lo;
-}
-
-// One binary search per workgroup (its first code); a thread then walks the
-// few scene boundaries up to its own code: the last s with offsets[s] <= i,
-// as scene_of gives.
-__global__ void k_tag_codes(const uint64_t* __restrict__ code, uint64_t n, const uint64_t* __restrict__ off,
-                            uint32_t S, const uint32_t* __restrict__ levels, uint64_t* __restrict__ tagged,
-                            uint32_t* __restrict__ bad) {
-  __shared__ uint32_t s0;
-  const uint64_t i0 = (uint64_t)blockIdx.x * blockDim.x;
-  if (threadIdx.x == 0) s0 = scene_of(off, S, i0);
-  __syncthreads();
-  const uint64_t i = i0 + threadIdx.x;
-  if (i >= n) return;
-  uint32_t s = s0;
-  while (s + 1 < S && off[s + 1] <= i) ++s;
-  const uint64_t c = code[i];
-  if (c >= (1ull << (3 * levels[s])) || (i > off[s] && c <= code[i - 1])) atomicAdd(bad, 1u);
-  tagged[i] = ((uint64_t)s << kTagShift) | c;
-}
 
 __global__ void k_node_scene(const uint64_t* __restrict__ codes, uint64_t n, uint32_t shift, uint32_t* __restrict__ sid) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -188,25 +158,21 @@ __global__ void k_scatter_headers(const uint32_t* __restrict__ hdr, uint32_t S, 
 void batch_build(svd_batch* b, const uint64_t* dcodes, uint64_t n, const std::vector<uint64_t>& off) {
   svd_ctx* c = b->ctx;
   const uint32_t S = b->S, Lm = b->Lmax;
-  // ---- tag + check
+  // ---- SVO of the whole batch: the codes are tagged with their scene id as
+  // the leaf pass reads them, and checked there
   uint64_t* dOff = dalloc<uint64_t>(c, S + 1);
   uint32_t* dLev = dalloc<uint32_t>(c, S);
-  uint32_t* bad = dalloc<uint32_t>(c, 1);
-  uint64_t* tagged = dalloc<uint64_t>(c, n + 1);
   SVD_HIP(hipMemcpyAsync(dOff, off.data(), 8 * (S + 1), hipMemcpyHostToDevice, c->stream));
   SVD_HIP(hipMemcpyAsync(dLev, b->L.data(), 4 * S, hipMemcpyHostToDevice, c->stream));
-  fill_u32(c, bad, 0u, 1);
-  k_tag_codes<<<div_up(n, kB), kB, 0, c->stream>>>(dcodes, n, dOff, S, dLev, tagged, bad);
-  check_launch("k_tag_codes");
-  if (read_u32(c, bad)) {
-    dfree(c, dOff); dfree(c, dLev); dfree(c, bad); dfree(c, tagged);
-    throw Error{SVD_ERR_ARG, "each scene's Morton codes must be strictly ascending and below 2^(3*levels)"};
-  }
-  // ---- SVO of the whole batch
   std::vector<Level> svo;
   std::vector<uint64_t*> codes;
-  svo_levels_from_codes(c, tagged, n, Lm, svo, &codes, 0);   // the tags sit above every scene's codes
-  dfree(c, tagged);
+  const SceneTags tags{dOff, dLev, nullptr, S};
+  try {
+    svo_levels_from_codes(c, dcodes, n, Lm, svo, &codes, 0, &tags);
+  } catch (...) {
+    dfree(c, dOff); dfree(c, dLev);
+    throw;
+  }
   // scene of every SVO node, per level
   std::vector<uint32_t*> sid(Lm, nullptr);
   uint32_t* startTab = dalloc<uint32_t>(c, (uint64_t)Lm * S);
@@ -214,7 +180,7 @@ void batch_build(svd_batch* b, const uint64_t* dcodes, uint64_t n, const std::ve
   for (uint32_t lev = 0; lev < Lm; ++lev) {
     const uint64_t m = svo[lev].n;
     sid[lev] = dalloc<uint32_t>(c, m + 1);
-    const uint32_t shift = kTagShift - 3 * (Lm - lev);
+    const uint32_t shift = kSceneTagShift - 3 * (Lm - lev);
     k_node_scene<<<div_up(m, kB), kB, 0, c->stream>>>(codes[lev], m, shift, sid[lev]);
     k_scene_starts<<<div_up(m, kB), kB, 0, c->stream>>>(sid[lev], m, startTab + (uint64_t)lev * S);
     dfree(c, codes[lev]);
@@ -354,7 +320,7 @@ void batch_build(svd_batch* b, const uint64_t* dcodes, uint64_t n, const std::ve
   for (auto* p : sidDag) dfree(c, p);
   dfree(c, totals); dfree(c, dStart); dfree(c, dFirst); dfree(c, dBase); dfree(c, dScanStart); dfree(c, dScanEnd);
   dfree(c, dFileWord);
-  dfree(c, dHdr); dfree(c, dOff); dfree(c, dLev); dfree(c, bad);
+  dfree(c, dHdr); dfree(c, dOff); dfree(c, dLev);
   // scenes for traversal
   b->scenes.assign(S, svd_scene());
   for (uint32_t s = 0; s < S; ++s) {

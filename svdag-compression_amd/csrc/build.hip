@@ -26,6 +26,61 @@ __device__ __forceinline__ uint32_t lanes_below(uint64_t m) {
   return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
 }
 
+// Sources of the codes the one-pass level builds read: plain codes, or a
+// batch's per-scene codes tagged on the fly with their scene id above bit
+// kSceneTagShift (scene-major, so the tagged list ascends when each scene's
+// does).  A code at or above 2^(3 * its scene's levels) comes back with bit
+// 63 set, which the leaf pass's input check catches.
+// at_tile(tileBase) lets a source cache what the tile's codes share (run_tile
+// calls it once the tile's code loads are issued); code i is tag(i, raw(i)),
+// split so that a tile's loads all issue before any of the (branching)
+// tagging.
+struct PlainCodes {
+  const uint64_t* __restrict__ code;
+  __device__ __forceinline__ void at_tile(uint64_t) {}
+  __device__ __forceinline__ uint64_t raw(uint64_t i) const { return code[i]; }
+  __device__ __forceinline__ uint64_t tag(uint64_t, uint64_t c) const { return c; }
+  __device__ __forceinline__ uint64_t operator()(uint64_t i) const { return code[i]; }
+};
+struct TaggedCodes {
+  const uint64_t* __restrict__ code;
+  SceneTags t;
+  uint32_t tileSize;
+  // the scene s0 of the tile's first code and the next one: their code
+  // ranges [b0, e0), [e0, e1) and level shifts, one 32-byte record per tile
+  // (k_tile_scenes).  A tile whose codes span more scenes, or a code before
+  // the tile, bisects: a slow tile holds up its successors' look-back.
+  uint32_t s0 = 0, sh0 = 0, sh1 = 0;
+  uint64_t b0 = 0, e0 = 0, e1 = 0;
+  __device__ __forceinline__ void at_tile(uint64_t tileBase) {
+    const uint32_t* r = t.tileScene + 8 * (tileBase / tileSize);
+    b0 = ((uint64_t)r[1] << 32) | r[0];
+    e0 = ((uint64_t)r[3] << 32) | r[2];
+    e1 = ((uint64_t)r[5] << 32) | r[4];
+    s0 = r[6];
+    sh0 = r[7] & 0xFFu;
+    sh1 = r[7] >> 8;
+  }
+  __device__ __forceinline__ uint64_t tagged(uint64_t c, uint32_t s, uint32_t sh) const {
+    const uint64_t out = (c >> sh) != 0 ? (1ull << 63) : 0ull;
+    return out | ((uint64_t)s << kSceneTagShift) | (c & ((1ull << kSceneTagShift) - 1));
+  }
+  __device__ __forceinline__ uint64_t raw(uint64_t i) const { return code[i]; }
+  __device__ __forceinline__ uint64_t operator()(uint64_t i) const { return tag(i, code[i]); }
+  __device__ __forceinline__ uint64_t tag(uint64_t i, uint64_t c) const {
+    if (i >= b0 && i < e0) return tagged(c, s0, sh0);
+    if (i >= e0 && i < e1) return tagged(c, s0 + 1, sh1);
+    // another scene: the last s with off[s] <= i, by bisection on the side
+    // of the tile's scenes that holds it
+    uint32_t lo = i < b0 ? 0u : s0 + 2, hi = i < b0 ? s0 : t.S;
+    while (hi - lo > 1) {
+      const uint32_t m = (lo + hi) >> 1;
+      if (t.off[m] <= i) lo = m; else hi = m;
+    }
+    return tagged(c, lo, 3 * t.levels[lo]);
+  }
+};
+
 // A tile of sorted codes for the one-pass level builds: thread (wave w, lane
 // j) holds codes w * 64 * PER + 64 * k + j (k < PER) of the tile, so every
 // load is coalesced; a run of codes sharing code >> 3 is one parent node.
@@ -42,18 +97,24 @@ struct RunTile {
   uint32_t heads = 0;
   uint32_t agg = 0;
 };
-template <int PER>
-__device__ __forceinline__ void run_tile(const uint64_t* __restrict__ code, uint64_t n, uint64_t tileBase,
-                                         RunTile<PER>& t, uint32_t* wtot) {
+template <int PER, typename Src>
+__device__ __forceinline__ void run_tile(Src& code, uint64_t n, uint64_t tileBase, RunTile<PER>& t,
+                                         uint32_t* wtot) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const uint64_t wbase = tileBase + (uint64_t)wave * (64 * PER) + lane;
 #pragma unroll
   for (int k = 0; k < PER; ++k) {
     const uint64_t i = wbase + 64 * k;
-    t.v[k] = i < n ? code[i] : 0ull;
+    t.v[k] = i < n ? code.raw(i) : 0ull;
+  }
+  code.at_tile(tileBase);   // its loads queue behind the codes'
+#pragma unroll
+  for (int k = 0; k < PER; ++k) {
+    const uint64_t i = wbase + 64 * k;
+    if (i < n) t.v[k] = code.tag(i, t.v[k]);
   }
   const uint64_t first = tileBase + (uint64_t)wave * (64 * PER);
-  const uint64_t before = (lane == 0 && first > 0 && first - 1 < n) ? code[first - 1] : 0ull;
+  const uint64_t before = (lane == 0 && first > 0 && first - 1 < n) ? code(first - 1) : 0ull;
   uint32_t c = 0;
 #pragma unroll
   for (int k = 0; k < PER; ++k) {
@@ -83,11 +144,11 @@ __device__ __forceinline__ void run_tile(const uint64_t* __restrict__ code, uint
 
 // Mask bits of the codes from position `end` on that continue the run of
 // parent code pc (at most 7), loaded together.
-__device__ __forceinline__ uint32_t run_tail_bits(const uint64_t* __restrict__ code, uint64_t n, uint64_t end,
-                                                  uint64_t pc) {
+template <typename Src>
+__device__ __forceinline__ uint32_t run_tail_bits(const Src& code, uint64_t n, uint64_t end, uint64_t pc) {
   uint64_t y[7];
 #pragma unroll
-  for (int j = 0; j < 7; ++j) y[j] = end + j < n ? code[end + j] : ~0ull;
+  for (int j = 0; j < 7; ++j) y[j] = end + j < n ? code(end + j) : ~0ull;
   uint32_t m = 0;
   bool same = true;
 #pragma unroll
@@ -98,8 +159,9 @@ __device__ __forceinline__ uint32_t run_tail_bits(const uint64_t* __restrict__ c
   return m;
 }
 
-// Input contract: strictly ascending codes with no bit of hiMask set (the
-// plain build: below 2^(3*levels); scene-tagged batch codes: any).
+// Input contract: strictly ascending codes with no bit of hiMask set (plain
+// codes: every bit from 3 * levels up; scene-tagged codes: bit 63, which
+// TaggedCodes sets on a code beyond its scene's depth).
 // One pass over the codes for the finest level: the input check, every
 // level's node count, the run heads, their numbering by a single-pass scan,
 // and the leaf masks and parent codes.  Code i starts a node of the level k
@@ -113,9 +175,10 @@ __device__ __forceinline__ uint32_t run_tail_bits(const uint64_t* __restrict__ c
 // the writes stay below n and the caller throws on the bad count.
 constexpr int kLeafPer = 16;
 constexpr uint32_t kLeafTile = kRunT * kLeafPer;
-__global__ void __launch_bounds__(kRunT) k_leaf_build(const uint64_t* __restrict__ code, uint64_t n, uint32_t L,
-                                                      uint64_t hiMask, LbPass p, uint32_t* __restrict__ part,
-                                                      uint32_t* __restrict__ mask, uint64_t* __restrict__ pcode) {
+template <typename Src>
+__global__ void __launch_bounds__(kRunT) k_leaf_build(Src src, uint64_t n, uint32_t L, uint64_t hiMask, LbPass p,
+                                                      uint32_t* __restrict__ part, uint32_t* __restrict__ mask,
+                                                      uint64_t* __restrict__ pcode) {
   __shared__ uint32_t lds[kRunT / 64 + 2];
   __shared__ uint32_t word;
   __shared__ uint32_t cpart[kMaxLevels + 1][kRunT / 64];
@@ -123,6 +186,7 @@ __global__ void __launch_bounds__(kRunT) k_leaf_build(const uint64_t* __restrict
   const uint32_t tile = lb_take_tile(p, &word);
   const uint64_t tileBase = (uint64_t)tile * kLeafTile;
   for (uint32_t j = threadIdx.x; j < kLeafTile; j += kRunT) lmask[j] = 0;
+  Src code = src;
   RunTile<kLeafPer> t;
   run_tile(code, n, tileBase, t, lds);   // its barrier also orders the clearing
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -138,7 +202,7 @@ __global__ void __launch_bounds__(kRunT) k_leaf_build(const uint64_t* __restrict
     const uint64_t i = wbase + 64 * k;
     const uint64_t x = t.v[k];
     const uint64_t up = __shfl_up(x, 1, 64);
-    const uint64_t wrap = k ? __shfl(t.v[k - 1], 63, 64) : (lane == 0 && i > 0 && i - 1 < n ? code[i - 1] : 0ull);
+    const uint64_t wrap = k ? __shfl(t.v[k - 1], 63, 64) : (lane == 0 && i > 0 && i - 1 < n ? code(i - 1) : 0ull);
     const uint64_t u = lane ? up : wrap;
     if (i >= n) continue;
     uint32_t K = L;
@@ -212,7 +276,8 @@ __global__ void __launch_bounds__(kRunT) k_inner_build(const uint64_t* __restric
   const uint64_t tileBase = (uint64_t)tile * kInnerTile;
   for (uint32_t j = threadIdx.x; j < kInnerTile; j += kRunT) lmask[j] = 0;
   RunTile<kInnerPer> t;
-  run_tile(ccode, n, tileBase, t, lds);
+  PlainCodes src{ccode};
+  run_tile(src, n, tileBase, t, lds);
   const uint32_t prefix = lb_exclusive(p, tile, t.agg, lds);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const uint64_t wbase = tileBase + (uint64_t)wave * (64 * kInnerPer) + lane;
@@ -233,7 +298,7 @@ __global__ void __launch_bounds__(kRunT) k_inner_build(const uint64_t* __restric
     const uint32_t r = (uint32_t)t.rank[k];
     uint32_t m = lmask[r];
     const uint64_t pc = t.v[k] >> 3;
-    if (r + 1 == t.agg && tileEnd < n) m |= run_tail_bits(ccode, n, tileEnd, pc);
+    if (r + 1 == t.agg && tileEnd < n) m |= run_tail_bits(src, n, tileEnd, pc);
     uint32_t* o = rec + (uint64_t)(prefix + r) * kRecWords;
     o[0] = m;
 #pragma unroll
@@ -1281,6 +1346,27 @@ __global__ void __launch_bounds__(256) k_sdag_leaves(const uint32_t* __restrict_
   }
 }
 
+// The scene s of code t * kLeafTile (the last s with off[s] <= it; every
+// scene holds at least one code, so off ascends strictly), as 8 words per
+// tile: off[s], off[s + 1], off[s + 2] (64-bit, low word first; off[S] past
+// the last scene), s, 3 * levels[s] | 3 * levels[s + 1] << 8.
+__global__ void k_tile_scenes(const uint64_t* __restrict__ off, const uint32_t* __restrict__ levels, uint32_t S,
+                              uint64_t tiles, uint32_t* __restrict__ tileScene) {
+  const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= tiles) return;
+  const uint64_t i = t * kLeafTile;
+  uint32_t lo = 0, hi = S;
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (off[mid] <= i) lo = mid; else hi = mid;
+  }
+  const uint64_t b = off[lo], e = off[lo + 1], e1 = lo + 1 < S ? off[lo + 2] : e;
+  const uint32_t sh1 = lo + 1 < S ? 3 * levels[lo + 1] : 0u;
+  uint4* r = (uint4*)(tileScene + 8 * t);
+  r[0] = make_uint4((uint32_t)b, (uint32_t)(b >> 32), (uint32_t)e, (uint32_t)(e >> 32));
+  r[1] = make_uint4((uint32_t)e1, (uint32_t)(e1 >> 32), lo, 3 * levels[lo] | sh1 << 8);
+}
+
 uint32_t table_cap(uint64_t n) {
   uint64_t c = 1024;
   while (c < 2 * n) c <<= 1;
@@ -1363,11 +1449,14 @@ uint64_t svo_inner_levels(svd_ctx* c, uint64_t* codeA, uint64_t nc, uint32_t L, 
 // in one pass over the codes for the finest level (k_leaf_build: the input
 // check, every level's node count, the leaf masks and parent codes) and one
 // per inner level.  Node order per level = order of first arrival.
+// tags (optional): the codes are a batch's scenes one after another, each
+// ascending below 2^(3 * its levels); they are tagged with their scene id as
+// they are read (hiMask is then ignored).
 // levelCodes (optional) receives, per level, a device copy of every node's
 // code (code >> 3*(L-lev)).  Returns the number of non-root nodes; throws
 // SVD_ERR_ARG on bad input.
 uint64_t svo_levels_from_codes(svd_ctx* c, const uint64_t* dcodes, uint64_t n, uint32_t L, std::vector<Level>& svo,
-                               std::vector<uint64_t*>* levelCodes, uint64_t hiMask) {
+                               std::vector<uint64_t*>* levelCodes, uint64_t hiMask, const SceneTags* tags) {
   svo.assign(L, Level());
   if (levelCodes) levelCodes->assign(L, nullptr);
   const uint64_t tiles = div_up(n, kLeafTile);
@@ -1375,18 +1464,30 @@ uint64_t svo_levels_from_codes(svd_ctx* c, const uint64_t* dcodes, uint64_t n, u
   uint32_t* hist = dalloc<uint32_t>(c, kMaxLevels + 1);
   uint32_t* leafMask = dalloc<uint32_t>(c, n);
   uint64_t* codeA = dalloc<uint64_t>(c, n);
-  k_leaf_build<<<(uint32_t)tiles, kRunT, 0, c->stream>>>(dcodes, n, L, hiMask, lb_pass(c, tiles), part, leafMask,
-                                                         codeA);
+  uint32_t* tileScene = nullptr;
+  if (tags) {
+    tileScene = dalloc<uint32_t>(c, 8 * tiles);
+    k_tile_scenes<<<div_up(tiles, kB), kB, 0, c->stream>>>(tags->off, tags->levels, tags->S, tiles, tileScene);
+    SceneTags t = *tags;
+    t.tileScene = tileScene;
+    k_leaf_build<<<(uint32_t)tiles, kRunT, 0, c->stream>>>(TaggedCodes{dcodes, t, kLeafTile}, n, L, 1ull << 63,
+                                                           lb_pass(c, tiles), part, leafMask, codeA);
+  } else {
+    k_leaf_build<<<(uint32_t)tiles, kRunT, 0, c->stream>>>(PlainCodes{dcodes}, n, L, hiMask, lb_pass(c, tiles), part,
+                                                           leafMask, codeA);
+  }
   k_level_counts<<<1, 1024, 0, c->stream>>>(part, (uint32_t)tiles, L, hist);
   check_launch("k_leaf_build");
   uint32_t h[kMaxLevels + 1];
   read_u32s(c, hist, h, L + 1);
   dfree(c, hist);
   dfree(c, part);
+  dfree(c, tileScene);
   if (h[0]) {
     dfree(c, leafMask);
     dfree(c, codeA);
-    throw Error{SVD_ERR_ARG, "Morton codes must be strictly ascending and below 2^(3*levels)"};
+    throw Error{SVD_ERR_ARG, tags ? "each scene's Morton codes must be strictly ascending and below 2^(3*levels)"
+                                  : "Morton codes must be strictly ascending and below 2^(3*levels)"};
   }
   // level lev is L - lev steps above the voxels
   std::vector<uint64_t> sizes(L);

@@ -201,10 +201,22 @@ void build_svo_from_points(svd_octree* o, const double* dxyz, uint64_t npts, con
 // buildDAG's joined SVO: top of stepLevel+1 levels + one voxelized subtree per set voxel of its finest level
 void build_svo_from_mesh_subtrees(svd_octree* o, const float* dtris, uint64_t ntris, const double bmin[3],
                                   const double bmax[3], uint32_t stepLevel);
-// SVO levels of n > 0 strictly ascending codes with no bit of hiMask set; levelCodes
-// (optional) receives every level's node codes.  Returns the non-root nodes.
+// A batch's scenes in one code list (device arrays): scene s holds codes
+// off[s] .. off[s+1]-1, each below 2^(3 * levels[s]); a build reads them
+// tagged with s above bit kSceneTagShift.  tileScene is the build's own table.
+constexpr int kSceneTagShift = 40;
+struct SceneTags {
+  const uint64_t* off;
+  const uint32_t* levels;
+  const uint32_t* tileScene;
+  uint32_t S;
+};
+// SVO levels of n > 0 strictly ascending codes with no bit of hiMask set (or a
+// batch's scenes, tagged as read, with tags); levelCodes (optional) receives
+// every level's node codes.  Returns the non-root nodes.
 uint64_t svo_levels_from_codes(svd_ctx* c, const uint64_t* dcodes, uint64_t n, uint32_t L, std::vector<Level>& svo,
-                               std::vector<uint64_t*>* levelCodes, uint64_t hiMask);
+                               std::vector<uint64_t*>* levelCodes, uint64_t hiMask,
+                               const SceneTags* tags = nullptr);
 // inner levels L-2 .. 0 from the finest level's parent codes (taken over and freed)
 // and every level's node count (sizes, host)
 uint64_t svo_inner_levels(svd_ctx* c, uint64_t* codeA, uint64_t nc, uint32_t L, std::vector<Level>& svo,
