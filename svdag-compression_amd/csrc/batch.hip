@@ -21,11 +21,6 @@ namespace {
 
 constexpr int kB = 256;
 
-__global__ void k_node_scene(const uint64_t* __restrict__ codes, uint64_t n, uint32_t shift, uint32_t* __restrict__ sid) {
-  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) sid[i] = (uint32_t)(codes[i] >> shift);
-}
-
 // start[s] = first index of scene s in a scene-major array (kNone if absent)
 __global__ void k_scene_starts(const uint32_t* __restrict__ sid, uint64_t n, uint32_t* __restrict__ start) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -165,25 +160,20 @@ void batch_build(svd_batch* b, const uint64_t* dcodes, uint64_t n, const std::ve
   SVD_HIP(hipMemcpyAsync(dOff, off.data(), 8 * (S + 1), hipMemcpyHostToDevice, c->stream));
   SVD_HIP(hipMemcpyAsync(dLev, b->L.data(), 4 * S, hipMemcpyHostToDevice, c->stream));
   std::vector<Level> svo;
-  std::vector<uint64_t*> codes;
+  std::vector<uint32_t*> sid;   // scene of every SVO node, per level
   const SceneTags tags{dOff, dLev, nullptr, S};
   try {
-    svo_levels_from_codes(c, dcodes, n, Lm, svo, &codes, 0, &tags);
+    svo_levels_from_codes(c, dcodes, n, Lm, svo, &sid, 0, &tags);
   } catch (...) {
     dfree(c, dOff); dfree(c, dLev);
     throw;
   }
-  // scene of every SVO node, per level
-  std::vector<uint32_t*> sid(Lm, nullptr);
+  // first node of every scene, per level
   uint32_t* startTab = dalloc<uint32_t>(c, (uint64_t)Lm * S);
   fill_u32(c, startTab, kNone, (uint64_t)Lm * S);
   for (uint32_t lev = 0; lev < Lm; ++lev) {
     const uint64_t m = svo[lev].n;
-    sid[lev] = dalloc<uint32_t>(c, m + 1);
-    const uint32_t shift = kSceneTagShift - 3 * (Lm - lev);
-    k_node_scene<<<div_up(m, kB), kB, 0, c->stream>>>(codes[lev], m, shift, sid[lev]);
     k_scene_starts<<<div_up(m, kB), kB, 0, c->stream>>>(sid[lev], m, startTab + (uint64_t)lev * S);
-    dfree(c, codes[lev]);
   }
   check_launch("scene tables");
   // ---- DAG: leaf masks keyed by (scene, mask), inner levels by the plain key

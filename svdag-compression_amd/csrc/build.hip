@@ -177,14 +177,15 @@ constexpr int kLeafPer = 16;
 constexpr uint32_t kLeafTile = kRunT * kLeafPer;
 template <typename Src>
 __global__ void __launch_bounds__(kRunT) k_leaf_build(Src src, uint64_t n, uint32_t L, uint64_t hiMask, LbPass p,
-                                                      uint32_t* __restrict__ part, uint32_t* __restrict__ mask,
-                                                      uint64_t* __restrict__ pcode) {
+                                                      uint32_t* __restrict__ part, uint32_t* __restrict__ hist,
+                                                      uint32_t* __restrict__ mask, uint64_t* __restrict__ pcode) {
   __shared__ uint32_t lds[kRunT / 64 + 2];
   __shared__ uint32_t word;
   __shared__ uint32_t cpart[kMaxLevels + 1][kRunT / 64];
   __shared__ uint32_t lmask[kLeafTile];
   const uint32_t tile = lb_take_tile(p, &word);
   const uint64_t tileBase = (uint64_t)tile * kLeafTile;
+  if (tile == 0 && threadIdx.x <= L) hist[threadIdx.x] = 0;   // k_level_counts adds to it
   for (uint32_t j = threadIdx.x; j < kLeafTile; j += kRunT) lmask[j] = 0;
   Src code = src;
   RunTile<kLeafPer> t;
@@ -309,14 +310,16 @@ __global__ void __launch_bounds__(kRunT) k_inner_build(const uint64_t* __restric
 }
 
 // Level counts of k_leaf_build: hist[k] = sum over tiles of part[t][k], all
-// counters in one pass over the tiles.
+// counters in one pass over the tiles; each workgroup adds its tiles' sums
+// to hist, which k_leaf_build's first tile cleared.
+constexpr uint32_t kCountTilesPerWg = 8192;
 __global__ void __launch_bounds__(1024) k_level_counts(const uint32_t* __restrict__ part, uint32_t tiles, uint32_t L,
                                                        uint32_t* __restrict__ hist) {
   __shared__ uint32_t red[kMaxLevels + 1][1024 / 64];
   uint32_t s[kMaxLevels + 1];
 #pragma unroll
   for (int q = 0; q <= (int)kMaxLevels; ++q) s[q] = 0;
-  for (uint32_t t = threadIdx.x; t < tiles; t += 1024) {
+  for (uint32_t t = blockIdx.x * 1024 + threadIdx.x; t < tiles; t += gridDim.x * 1024) {
     const uint32_t* pt = part + (uint64_t)t * (L + 1);
 #pragma unroll
     for (int q = 0; q <= (int)kMaxLevels; ++q)
@@ -334,7 +337,7 @@ __global__ void __launch_bounds__(1024) k_level_counts(const uint32_t* __restric
   if (threadIdx.x <= L) {
     uint32_t x = 0;
     for (int v = 0; v < 1024 / 64; ++v) x += red[threadIdx.x][v];
-    hist[threadIdx.x] = x;
+    atomicAdd(&hist[threadIdx.x], x);
   }
 }
 
@@ -1367,6 +1370,13 @@ __global__ void k_tile_scenes(const uint64_t* __restrict__ off, const uint32_t* 
   r[1] = make_uint4((uint32_t)e1, (uint32_t)(e1 >> 32), lo, 3 * levels[lo] | sh1 << 8);
 }
 
+// Scene id of every node of a level of scene-tagged codes (codes >> shift).
+__global__ void k_code_scenes(const uint64_t* __restrict__ codes, uint64_t n, uint32_t shift,
+                              uint32_t* __restrict__ sid) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) sid[i] = (uint32_t)(codes[i] >> shift);
+}
+
 uint32_t table_cap(uint64_t n) {
   uint64_t c = 1024;
   while (c < 2 * n) c <<= 1;
@@ -1401,17 +1411,19 @@ void build_svo_from_morton(svd_octree* o, const uint64_t* dcodes, uint64_t n) {
 // Inner levels L-2 .. 0 from the finest level's parent codes (codeA, nc of
 // them, taken over and freed).  Returns the non-root nodes created.
 uint64_t svo_inner_levels(svd_ctx* c, uint64_t* codeA, uint64_t nc, uint32_t L, std::vector<Level>& svo,
-                          std::vector<uint64_t*>* levelCodes, const uint64_t* sizes) {
+                          std::vector<uint32_t*>* levelScenes, const uint64_t* sizes) {
   uint64_t* codeB = dalloc<uint64_t>(c, nc ? nc : 1);
   auto keep = [&](uint32_t lev, const uint64_t* codes, uint64_t m) {
-    if (!levelCodes) return;
-    (*levelCodes)[lev] = dalloc<uint64_t>(c, m + 1);
-    if (m) SVD_HIP(hipMemcpyAsync((*levelCodes)[lev], codes, m * 8, hipMemcpyDeviceToDevice, c->stream));
+    if (!levelScenes) return;
+    (*levelScenes)[lev] = dalloc<uint32_t>(c, m + 1);
+    if (m)
+      k_code_scenes<<<div_up(m, kB), kB, 0, c->stream>>>(codes, m, kSceneTagShift - 3 * (L - lev),
+                                                         (*levelScenes)[lev]);
   };
   uint64_t nNodes = 0;
   for (int lev = (int)L - 2; lev >= 0; --lev) {
     Level& lv = svo[lev];
-    if (nc <= small_limit(c->smallSvo, kSvoSmall) && !levelCodes) {
+    if (nc <= small_limit(c->smallSvo, kSvoSmall) && !levelScenes) {
       // this level and all above it in one launch
       SvoTop top{};
       top.top = lev;
@@ -1452,13 +1464,13 @@ uint64_t svo_inner_levels(svd_ctx* c, uint64_t* codeA, uint64_t nc, uint32_t L, 
 // tags (optional): the codes are a batch's scenes one after another, each
 // ascending below 2^(3 * its levels); they are tagged with their scene id as
 // they are read (hiMask is then ignored).
-// levelCodes (optional) receives, per level, a device copy of every node's
+// levelScenes (optional, with tags) receives, per level, every node's scene
 // code (code >> 3*(L-lev)).  Returns the number of non-root nodes; throws
 // SVD_ERR_ARG on bad input.
 uint64_t svo_levels_from_codes(svd_ctx* c, const uint64_t* dcodes, uint64_t n, uint32_t L, std::vector<Level>& svo,
-                               std::vector<uint64_t*>* levelCodes, uint64_t hiMask, const SceneTags* tags) {
+                               std::vector<uint32_t*>* levelScenes, uint64_t hiMask, const SceneTags* tags) {
   svo.assign(L, Level());
-  if (levelCodes) levelCodes->assign(L, nullptr);
+  if (levelScenes) levelScenes->assign(L, nullptr);
   const uint64_t tiles = div_up(n, kLeafTile);
   uint32_t* part = dalloc<uint32_t>(c, tiles * (L + 1));
   uint32_t* hist = dalloc<uint32_t>(c, kMaxLevels + 1);
@@ -1471,12 +1483,12 @@ uint64_t svo_levels_from_codes(svd_ctx* c, const uint64_t* dcodes, uint64_t n, u
     SceneTags t = *tags;
     t.tileScene = tileScene;
     k_leaf_build<<<(uint32_t)tiles, kRunT, 0, c->stream>>>(TaggedCodes{dcodes, t, kLeafTile}, n, L, 1ull << 63,
-                                                           lb_pass(c, tiles), part, leafMask, codeA);
+                                                           lb_pass(c, tiles), part, hist, leafMask, codeA);
   } else {
     k_leaf_build<<<(uint32_t)tiles, kRunT, 0, c->stream>>>(PlainCodes{dcodes}, n, L, hiMask, lb_pass(c, tiles), part,
-                                                           leafMask, codeA);
+                                                           hist, leafMask, codeA);
   }
-  k_level_counts<<<1, 1024, 0, c->stream>>>(part, (uint32_t)tiles, L, hist);
+  k_level_counts<<<(uint32_t)div_up(tiles, kCountTilesPerWg), 1024, 0, c->stream>>>(part, (uint32_t)tiles, L, hist);
   check_launch("k_leaf_build");
   uint32_t h[kMaxLevels + 1];
   read_u32s(c, hist, h, L + 1);
@@ -1496,12 +1508,12 @@ uint64_t svo_levels_from_codes(svd_ctx* c, const uint64_t* dcodes, uint64_t n, u
   leaf.leaf = true;
   leaf.n = sizes[L - 1];
   leaf.rec = leafMask;
-  if (levelCodes) {
-    (*levelCodes)[L - 1] = dalloc<uint64_t>(c, leaf.n + 1);
-    SVD_HIP(hipMemcpyAsync((*levelCodes)[L - 1], codeA, leaf.n * 8, hipMemcpyDeviceToDevice, c->stream));
+  if (levelScenes) {
+    (*levelScenes)[L - 1] = dalloc<uint32_t>(c, leaf.n + 1);
+    k_code_scenes<<<div_up(leaf.n, kB), kB, 0, c->stream>>>(codeA, leaf.n, kSceneTagShift - 3, (*levelScenes)[L - 1]);
   }
   const uint64_t nNodes = (L >= 2) ? leaf.n : 0;
-  return nNodes + svo_inner_levels(c, codeA, leaf.n, L, svo, levelCodes, sizes.data());
+  return nNodes + svo_inner_levels(c, codeA, leaf.n, L, svo, levelScenes, sizes.data());
 }
 
 void remap_children(svd_ctx* c, const uint32_t* in, uint64_t n, const uint32_t* remap, uint32_t* out) {

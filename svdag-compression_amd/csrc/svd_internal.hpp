@@ -212,15 +212,15 @@ struct SceneTags {
   uint32_t S;
 };
 // SVO levels of n > 0 strictly ascending codes with no bit of hiMask set (or a
-// batch's scenes, tagged as read, with tags); levelCodes (optional) receives
-// every level's node codes.  Returns the non-root nodes.
+// batch's scenes, tagged as read, with tags); levelScenes (optional, with tags)
+// receives every level's node scene ids.  Returns the non-root nodes.
 uint64_t svo_levels_from_codes(svd_ctx* c, const uint64_t* dcodes, uint64_t n, uint32_t L, std::vector<Level>& svo,
-                               std::vector<uint64_t*>* levelCodes, uint64_t hiMask,
+                               std::vector<uint32_t*>* levelScenes, uint64_t hiMask,
                                const SceneTags* tags = nullptr);
 // inner levels L-2 .. 0 from the finest level's parent codes (taken over and freed)
 // and every level's node count (sizes, host)
 uint64_t svo_inner_levels(svd_ctx* c, uint64_t* codeA, uint64_t nc, uint32_t L, std::vector<Level>& svo,
-                          std::vector<uint64_t*>* levelCodes, const uint64_t* sizes);
+                          std::vector<uint32_t*>* levelScenes, const uint64_t* sizes);
 // one inner level: exact-key dedup, first occurrence wins -> remap + compacted records
 uint64_t dedup_level(svd_ctx* c, const uint32_t* rec, uint64_t n, uint32_t* remap, uint32_t** outRec);
 void remap_children(svd_ctx* c, const uint32_t* in, uint64_t n, const uint32_t* remap, uint32_t* out);
