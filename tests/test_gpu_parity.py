@@ -371,6 +371,36 @@ def test_trace_no_lod_all_formats(ctx):
         np.testing.assert_array_equal(shad.cpu().numpy().view(S.RAY_DTYPE), want_s)
 
 
+@pytest.mark.parametrize("fmt", [S.FMT_SVDAG, S.FMT_USSVDAG])
+def test_trace_flags_pointers_outside_the_scene(ctx, fmt):
+    """A malformed file whose root child pointers lie past the node array:
+    every fetch is bounds-checked (the word guard of the traversal), so a ray
+    that follows such a pointer ends with t = -6 and the others are unchanged."""
+    import torch
+    codes, L = W.terrain_codes(7, seed=1), 7
+    _, _, _, files = oracle_pipeline(codes, L)
+    p = W.trace_params(160, 120, seed=4, levels=L, shadows=True)
+    want_p, _ = O.trace_frame(files[fmt], fmt, p, shadows=True)
+    buf = bytearray(files[fmt])
+    words = np.frombuffer(buf, dtype=np.uint32, offset=44).copy()
+    k = bin(int(words[0]) & 0xFF).count("1")
+    assert k > 0
+    words[1:1 + k] = np.uint32(0xFFFFFFF0)
+    bad = bytes(buf[:44]) + words.tobytes()
+    sc = S.Scene.from_bytes(ctx, fmt, bad)
+    n = p.width * p.height
+    prim = torch.zeros(n * 6, dtype=torch.int32, device="cuda")
+    shad = torch.zeros(n * 6, dtype=torch.int32, device="cuda")
+    torch.cuda.synchronize()
+    sc.trace_device(p, prim.data_ptr(), shad.data_ptr())
+    ctx.sync()
+    got = prim.cpu().numpy().view(S.RAY_DTYPE)
+    flagged = got["t"] == -6.0
+    assert flagged.sum() >= (want_p["t"] >= 0).sum() > 0, "every ray that hit the scene must reach a bad pointer"
+    np.testing.assert_array_equal(got["t"][~flagged], want_p["t"][~flagged])
+    sc.free()
+
+
 def test_full_size_terrain_properties(ctx):
     """2048^3 fBm terrain (11 levels): size-independent properties.
     Every file decodes back to the input voxel set, and the per-level DAG node
