@@ -54,21 +54,44 @@ __global__ void __launch_bounds__(256) k_leaf_firstocc_scene(const uint32_t* __r
   if (lo < n && v != kNone) atomicMin(&firstocc[(uint64_t)s0 * 256 + threadIdx.x], v);
 }
 
-__global__ void k_leaf_repflag(const uint32_t* __restrict__ mask, const uint32_t* __restrict__ sid, uint64_t n,
-                               const uint32_t* __restrict__ firstocc, uint32_t* __restrict__ flag) {
-  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  flag[i] = firstocc[(uint64_t)sid[i] * 256 + (mask[i] & 0xFFu)] == (uint32_t)i ? 1u : 0u;
+// Per scene (one workgroup each): its distinct leaf masks ranked by first
+// occurrence, rank[s][m] (kNone for a mask the scene lacks), and their count.
+__global__ void __launch_bounds__(256) k_leaf_rank_scene(const uint32_t* __restrict__ firstocc,
+                                                         uint32_t* __restrict__ rank, uint32_t* __restrict__ cnt) {
+  __shared__ uint32_t fo[256];
+  __shared__ uint32_t total;
+  const uint32_t m = threadIdx.x;
+  const uint64_t row = (uint64_t)blockIdx.x * 256;
+  fo[m] = firstocc[row + m];
+  if (m == 0) total = 0;
+  __syncthreads();
+  uint32_t id = kNone;
+  if (fo[m] != kNone) {
+    uint32_t r = 0;
+    for (int k = 0; k < 256; ++k) r += fo[k] < fo[m] ? 1u : 0u;
+    id = r;
+    atomicAdd(&total, 1u);
+  }
+  rank[row + m] = id;
+  __syncthreads();
+  if (m == 0) cnt[blockIdx.x] = total;
 }
 
-__global__ void k_leaf_compact_scene(const uint32_t* __restrict__ mask, const uint32_t* __restrict__ sid, uint64_t n,
-                                     const uint32_t* __restrict__ firstocc, const uint32_t* __restrict__ pos,
-                                     uint32_t* __restrict__ out, uint32_t* __restrict__ remap) {
+// The batched leaf level's dedup: leaf i of scene s with mask m becomes DAG
+// leaf base[s] + rank[s][m] (scene-major, first occurrence within the scene:
+// the order of first occurrence over the whole level); the first occurrence
+// writes the mask.
+__global__ void k_leaf_remap_scene(const uint32_t* __restrict__ mask, const uint32_t* __restrict__ sid, uint64_t n,
+                                   const uint32_t* __restrict__ firstocc, const uint32_t* __restrict__ rank,
+                                   const uint32_t* __restrict__ base, uint32_t* __restrict__ out,
+                                   uint32_t* __restrict__ remap) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const uint32_t r = firstocc[(uint64_t)sid[i] * 256 + (mask[i] & 0xFFu)];
-  remap[i] = pos[r];
-  if (r == (uint32_t)i) out[pos[i]] = mask[i] & 0xFFu;
+  const uint32_t m = mask[i] & 0xFFu, s = sid[i];
+  const uint64_t e = (uint64_t)s * 256 + m;
+  const uint32_t id = base[s] + rank[e];
+  remap[i] = id;
+  if (firstocc[e] == (uint32_t)i) out[id] = m;
 }
 
 // Scene of each DAG node from the per-scene start table of its level.
@@ -183,25 +206,29 @@ void batch_build(svd_batch* b, const uint64_t* dcodes, uint64_t n, const std::ve
   {
     const Level& sl = svo[Lm - 1];
     const uint64_t m = sl.n;
+    // first occurrence of every (scene, mask), the masks of each scene ranked,
+    // the scenes' counts scanned; the level's output holds at most 255 masks
+    // per scene, so it is allocated before the count is read
     uint32_t* fo = dalloc<uint32_t>(c, (uint64_t)S * 256);
-    uint32_t* pos = dalloc<uint32_t>(c, m + 1);
+    uint32_t* rank = dalloc<uint32_t>(c, (uint64_t)S * 256);
+    uint32_t* base = dalloc<uint32_t>(c, S + 1);
     uint32_t* total = dalloc<uint32_t>(c, 1);
     fill_u32(c, fo, kNone, (uint64_t)S * 256);
     if (m) {
       const uint64_t nb = div_up(m, 256) < (uint64_t)kFoBlocks ? div_up(m, 256) : (uint64_t)kFoBlocks;
       k_leaf_firstocc_scene<<<nb, 256, 0, c->stream>>>(sl.rec, sid[Lm - 1], m, fo);
     }
-    k_leaf_repflag<<<div_up(m, kB), kB, 0, c->stream>>>(sl.rec, sid[Lm - 1], m, fo, pos);
-    scan_exclusive(c, pos, pos, m, total);
-    const uint64_t d = read_u32(c, total);
+    k_leaf_rank_scene<<<S, 256, 0, c->stream>>>(fo, rank, base);
+    scan_exclusive(c, base, base, S, total);
     Level& dl = dag[Lm - 1];
     dl.leaf = true;
-    dl.n = d;
-    dl.rec = dalloc<uint32_t>(c, d + 1);
+    dl.rec = dalloc<uint32_t>(c, std::min<uint64_t>(m, 255ull * S) + 1);
     remap = dalloc<uint32_t>(c, m + 1);
-    k_leaf_compact_scene<<<div_up(m, kB), kB, 0, c->stream>>>(sl.rec, sid[Lm - 1], m, fo, pos, dl.rec, remap);
+    if (m)
+      k_leaf_remap_scene<<<div_up(m, kB), kB, 0, c->stream>>>(sl.rec, sid[Lm - 1], m, fo, rank, base, dl.rec, remap);
     check_launch("batched leaf dedup");
-    dfree(c, fo); dfree(c, pos); dfree(c, total);
+    dl.n = read_u32(c, total);
+    dfree(c, fo); dfree(c, rank); dfree(c, base); dfree(c, total);
   }
   // per-scene first DAG node per level = remap of the scene's first SVO node
   // (every scene has nodes on every batched level: its chain above the root)
