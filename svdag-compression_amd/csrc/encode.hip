@@ -32,15 +32,18 @@ __device__ __forceinline__ uint32_t level_of(const LevelSet& a, uint64_t i) {
   return l;
 }
 
-// Up to 128 bytes written at dst + off[k] from the kernel arguments (file
-// headers): no host staging buffer has to outlive the launch.
-struct Bytes {
-  uint8_t b[128];
-  uint32_t n;
+// File header pieces (up to kPutMax of up to 128 bytes) written in one
+// launch, one workgroup per piece, from the kernel arguments: no host staging
+// buffer has to outlive the launch.
+constexpr uint32_t kPutMax = 8;
+struct PutBatch {
+  uint8_t* dst[kPutMax];
+  uint32_t n[kPutMax];
+  uint8_t b[kPutMax][128];
 };
-__global__ void k_put_bytes(Bytes src, uint8_t* __restrict__ dst) {
-  const uint32_t t = threadIdx.x;
-  if (t < src.n) dst[t] = src.b[t];
+__global__ void k_put_batch(PutBatch p) {
+  const uint32_t k = blockIdx.x, t = threadIdx.x;
+  if (t < p.n[k]) p.dst[k][t] = p.b[k][t];
 }
 
 // ---------------------------------------------------------------- SVDAG / USSVDAG
@@ -56,9 +59,13 @@ __global__ void k_word_counts(LevelSet a, uint64_t n, uint32_t* __restrict__ out
 //           (offset = first node of that level), c = 7..0.
 // ussv = 1: header word = mZ<<24 | mY<<16 | mX<<8 | mask; pointers into the
 //           next level (childLevels ignored, encoded_ussvdag.cpp:137).
+// The header's two count words (_firstLeafPtr and the array length, both =
+// the word count *total, see encode_words) go to cnt from thread 0.
 __global__ void k_write_words(LevelSet a, int ussv, const uint32_t* __restrict__ truePtrs, uint64_t totalNodes,
-                              uint32_t* __restrict__ words, uint32_t* __restrict__ err) {
+                              uint32_t* __restrict__ words, uint32_t* __restrict__ err,
+                              const uint32_t* __restrict__ total, uint32_t* __restrict__ cnt) {
   uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) cnt[0] = cnt[1] = *total;
   if (i >= totalNodes) return;
   const uint32_t lev = level_of(a, i);
   const uint64_t j = i - a.start[lev];
@@ -80,12 +87,6 @@ __global__ void k_write_words(LevelSet a, int ussv, const uint32_t* __restrict__
     if (g >= totalNodes) { atomicAdd(err, 1u); words[k++] = 0u; continue; }
     words[k++] = truePtrs[g];
   }
-}
-
-// The two header words that depend on the word count: _firstLeafPtr and the
-// array length (both = count, see encode_words).
-__global__ void k_put_count(const uint32_t* __restrict__ total, uint32_t* __restrict__ dst) {
-  if (threadIdx.x < 2) dst[threadIdx.x] = *total;
 }
 
 // ---------------------------------------------------------------- SSVDAG
@@ -288,11 +289,9 @@ struct InnerLevels {
   uint32_t L;
 };
 
-// Variable-length inner node (encoded_ssvdag.cpp:352-430).
-__global__ void k_inner_write(InnerLevels a, const uint32_t* __restrict__ permAll, uint64_t n,
-                              uint16_t* __restrict__ inner) {
-  uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= n) return;
+// Variable-length inner node (encoded_ssvdag.cpp:352-430): sorted position g.
+__device__ __forceinline__ void inner_write(const InnerLevels& a, const uint32_t* __restrict__ permAll, uint64_t g,
+                                            uint16_t* __restrict__ inner) {
   uint32_t l = 0;
   while (l + 1 < a.L && g >= a.start[l + 1]) ++l;
   const uint64_t i = g - a.start[l];
@@ -333,11 +332,10 @@ __device__ __forceinline__ uint32_t perm8(uint32_t m, uint32_t flip) {
 
 // 4^3 brick in matrix order (encoded_ssvdag.cpp:303-346): bit x+4y+16z of the
 // output <- byte (x>>1)+2(y>>1)+4(z>>1), bit (x&1)+2(y&1)+4(z&1) of the
-// children masks (each mirrored by the parent's mirror bits).
-__global__ void k_brick_write(const uint32_t* __restrict__ rec, const uint32_t* __restrict__ perm, uint64_t n,
-                              const uint32_t* __restrict__ leafMask, uint16_t* __restrict__ out) {
-  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+// children masks (each mirrored by the parent's mirror bits): brick i.
+__device__ __forceinline__ void brick_write(const uint32_t* __restrict__ rec, const uint32_t* __restrict__ perm,
+                                            uint64_t i, const uint32_t* __restrict__ leafMask,
+                                            uint16_t* __restrict__ out) {
   const uint32_t* r = rec + (uint64_t)perm[i] * kRecWords;
   const uint32_t w0 = r[0];
   uint32_t buf[8];
@@ -361,6 +359,39 @@ __global__ void k_brick_write(const uint32_t* __restrict__ rec, const uint32_t* 
       }
   uint16_t* o = out + 4 * i;
   o[0] = (uint16_t)bits; o[1] = (uint16_t)(bits >> 16); o[2] = (uint16_t)(bits >> 32); o[3] = (uint16_t)(bits >> 48);
+}
+
+// The write phase of one or two SSVDAG-layout images in one launch: the
+// blocks before end[0] write set 0's inner nodes, then set 0's bricks (to
+// end[1]), set 1's inner nodes (end[2]) and set 1's bricks (end[3]).
+struct SetWrite {
+  InnerLevels ia;
+  const uint32_t* permAll;        // the set's sorted inner positions
+  uint16_t* inner;
+  const uint32_t* brickRec;
+  const uint32_t* brickPerm;
+  const uint32_t* leafMask;
+  uint16_t* bricks;
+  uint64_t nInner, nBricks;
+};
+struct SetWrites {
+  SetWrite s[2];
+  uint32_t end[4];
+};
+__device__ __forceinline__ void set_write(const SetWrite& w, bool bricks, uint64_t g) {
+  if (bricks) {
+    if (g < w.nBricks) brick_write(w.brickRec, w.brickPerm, g, w.leafMask, w.bricks);
+  } else if (g < w.nInner) {
+    inner_write(w.ia, w.permAll, g, w.inner);
+  }
+}
+__global__ void k_ssvdag_write(SetWrites w) {
+  const uint32_t b = blockIdx.x;
+  const uint32_t part = b < w.end[0] ? 0u : b < w.end[1] ? 1u : b < w.end[2] ? 2u : 3u;
+  const uint32_t b0 = part ? w.end[part - 1] : 0u;
+  const uint64_t g = (uint64_t)(b - b0) * blockDim.x + threadIdx.x;
+  if (part < 2) set_write(w.s[0], part == 1, g);
+  else set_write(w.s[1], part == 3, g);
 }
 
 void put(std::vector<uint8_t>& b, const void* p, size_t n) {
@@ -398,15 +429,27 @@ LevelSet level_set(const std::vector<Level>& lv, uint32_t L, bool withChildLevel
   return a;
 }
 
-void put_bytes(svd_ctx* c, uint8_t* dst, const std::vector<uint8_t>& src) {
-  if (src.empty()) return;
-  if (src.size() > sizeof(Bytes::b)) throw Error{SVD_ERR_DEVICE, "header too long"};
-  Bytes b{};
-  memcpy(b.b, src.data(), src.size());
-  b.n = (uint32_t)src.size();
-  k_put_bytes<<<1, 128, 0, c->stream>>>(b, dst);
-  check_launch("k_put_bytes");
-}
+// Header pieces collected on the host and written by flush() in one launch
+// (or earlier, when the batch is full).
+struct Puts {
+  PutBatch b{};
+  uint32_t count = 0;
+  void add(svd_ctx* c, uint8_t* dst, const std::vector<uint8_t>& v) {
+    if (v.empty()) return;
+    if (v.size() > sizeof(b.b[0])) throw Error{SVD_ERR_DEVICE, "header too long"};
+    if (count == kPutMax) flush(c);
+    b.dst[count] = dst;
+    b.n[count] = (uint32_t)v.size();
+    memcpy(b.b[count], v.data(), v.size());
+    ++count;
+  }
+  void flush(svd_ctx* c) {
+    if (!count) return;
+    k_put_batch<<<count, 128, 0, c->stream>>>(b);
+    check_launch("k_put_batch");
+    count = 0;
+  }
+};
 
 // .svdag (of the DAG, or its cross-level view) and .ussvdag (of the SDAG):
 // each format's passes are enqueued, and the word counts of all are read back
@@ -418,6 +461,7 @@ struct WordsPending {
   uint32_t* dev = nullptr;        // per format: word count, error flags
   uint32_t* ptrs[2] = {nullptr, nullptr};
   uint64_t hdrBytes[2] = {0, 0};
+  Puts puts;                      // the images' header bytes, written with the next flush
 };
 
 void encode_words_enqueue(svd_octree* o, const int* fmts, int K, WordsPending& wp) {
@@ -457,10 +501,12 @@ void encode_words_enqueue(svd_octree* o, const int* fmts, int K, WordsPending& w
     dfree(c, e.d);
     e.d = dalloc<uint8_t>(c, hdrBytes + 4 * maxWords);
     e.valid = false;
-    put_bytes(c, e.d, hdr);
-    k_put_count<<<1, 64, 0, c->stream>>>(d, (uint32_t*)(e.d + hdr.size()));
+    if (!N) hdr.resize(hdr.size() + 8, 0);   // no words: both count words are 0
+    wp.puts.add(c, e.d, hdr);
     uint32_t* words = (uint32_t*)(e.d + hdrBytes);
-    if (N) k_write_words<<<div_up(N, kB), kB, 0, c->stream>>>(a, ussv ? 1 : 0, ptrs, N, words, d + 1);
+    if (N)
+      k_write_words<<<div_up(N, kB), kB, 0, c->stream>>>(a, ussv ? 1 : 0, ptrs, N, words, d + 1, d,
+                                                         (uint32_t*)(e.d + hdrBytes - 8));
     check_launch("k_write_words");
   }
 }
@@ -483,6 +529,7 @@ void encode_words_finish(svd_octree* o, WordsPending& wp, const uint32_t* h) {
 void encode_words_sets(svd_octree* o, const int* fmts, int K) {
   WordsPending wp;
   encode_words_enqueue(o, fmts, K, wp);
+  wp.puts.flush(o->ctx);
   uint32_t h[4];
   read_u32s(o->ctx, wp.dev, h, 2 * K);
   encode_words_finish(o, wp, h);
@@ -648,10 +695,12 @@ void encode_ssvdag_sets(svd_octree* o, const int* fmts, int K, WordsPending* wp 
     // the pending word counts ride on the same host round trip
     uint32_t hw[4];
     read_u32s_2(c, dev, h.data(), K * L + 2, wp->dev, hw, 2 * wp->K);
+    wp->puts.flush(c);
     encode_words_finish(o, *wp, hw);
   } else {
     read_u32s(c, dev, h.data(), K * L + 2);
   }
+  Puts puts;
   if (h[K * L + 1]) {   // the refcount order needed more rounds: again, checked
     for (auto* p : ind) dfree(c, p);
     for (auto* p : offs) dfree(c, p);
@@ -660,6 +709,8 @@ void encode_ssvdag_sets(svd_octree* o, const int* fmts, int K, WordsPending* wp 
     return;
   }
   if (h[K * L]) throw Error{SVD_ERR_CAPACITY, "unsupported pointer size (>= 2^30)"};
+  SetWrites sw{};
+  uint32_t blocks = 0;
   for (int s = 0; s < K; ++s) {
     const std::vector<Level>& lv = *lvs[s];
     const int fmt = fmts[s];
@@ -685,34 +736,44 @@ void encode_ssvdag_sets(svd_octree* o, const int* fmts, int K, WordsPending* wp 
     put(tail, levelOffsets.data(), 4 * levelOffsets.size());
     Encoded& e = o->enc[fmt];
     dfree(c, e.d);
+    e.d = nullptr;
+    e.valid = false;
     e.bytes = tailAt + tail.size();
     e.d = dalloc<uint8_t>(c, e.bytes);
-    put_bytes(c, e.d, head);
-    put_bytes(c, e.d + leafCntAt, mid);
-    put_bytes(c, e.d + tailAt, tail);
-    // Phase 2: write every level into the image (inner levels in one launch).
-    uint16_t* inner = (uint16_t*)(e.d + innerAt);
+    puts.add(c, e.d, head);
+    puts.add(c, e.d + leafCntAt, mid);
+    puts.add(c, e.d + tailAt, tail);
+    // Phase 2: every level of every set written in one launch.
+    SetWrite& w = sw.s[s];
     const uint64_t base = off[s * L];
     const uint64_t nInnerNodes = off[s * L + L - 2] - base;
-    if (nInnerNodes) {
-      InnerLevels ia{};
-      ia.L = (uint32_t)(L - 2);
-      for (int lev = 0; lev <= L - 3; ++lev) {
-        ia.rec[lev] = lv[lev].rec;
-        ia.indChild[lev] = ind[s * L + lev + 1];
-        ia.offs[lev] = offs[s * L + lev];
-        ia.outOff[lev] = levelOffsets[lev];
-        ia.start[lev] = (uint32_t)(off[s * L + lev] - base);
-      }
-      ia.start[L - 2] = (uint32_t)nInnerNodes;
-      k_inner_write<<<div_up(nInnerNodes, kB), kB, 0, c->stream>>>(ia, permAll + base, nInnerNodes, inner);
+    w.ia.L = (uint32_t)(L - 2);
+    for (int lev = 0; lev <= L - 3; ++lev) {
+      w.ia.rec[lev] = lv[lev].rec;
+      w.ia.indChild[lev] = ind[s * L + lev + 1];
+      w.ia.offs[lev] = offs[s * L + lev];
+      w.ia.outOff[lev] = levelOffsets[lev];
+      w.ia.start[lev] = (uint32_t)(off[s * L + lev] - base);
     }
-    if (nBricks)
-      k_brick_write<<<div_up(nBricks, kB), kB, 0, c->stream>>>(lv[L - 2].rec, perm[s * L + L - 2], nBricks,
-                                                                lv[L - 1].rec, (uint16_t*)(e.d + leafAt));
-    check_launch("ssvdag write");
-    e.valid = true;
+    w.ia.start[L - 2] = (uint32_t)nInnerNodes;
+    w.permAll = permAll + base;
+    w.inner = (uint16_t*)(e.d + innerAt);
+    w.nInner = nInnerNodes;
+    w.brickRec = lv[L - 2].rec;
+    w.brickPerm = perm[s * L + L - 2];
+    w.leafMask = lv[L - 1].rec;
+    w.bricks = (uint16_t*)(e.d + leafAt);
+    w.nBricks = nBricks;
+    blocks += (uint32_t)div_up(nInnerNodes, kB);
+    sw.end[2 * s] = blocks;
+    blocks += (uint32_t)div_up(nBricks, kB);
+    sw.end[2 * s + 1] = blocks;
   }
+  for (int p = 2 * K; p < 4; ++p) sw.end[p] = blocks;
+  if (blocks) k_ssvdag_write<<<blocks, kB, 0, c->stream>>>(sw);
+  check_launch("k_ssvdag_write");
+  puts.flush(c);
+  for (int s = 0; s < K; ++s) o->enc[fmts[s]].valid = true;
   for (auto* p : ind) dfree(c, p);
   for (auto* p : offs) dfree(c, p);
   dfree(c, permAll); dfree(c, dev);
