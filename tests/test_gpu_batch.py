@@ -58,6 +58,33 @@ def test_batch_build_equals_oracle(ctx):
         assert b.svdag_bytes(s) == o.encode(0), f"scene {s} ({L} levels)"
 
 
+def test_batch_many_tiny_scenes_equal_oracle(ctx):
+    """Hundreds of scenes of 1-40 voxels each: every 8192-code tile of the leaf
+    pass spans many scenes, so most codes find their scene by bisecting the
+    scene offsets (not from the tile's record of its first two scenes), and
+    scene boundaries fall inside and across tiles."""
+    rng = np.random.default_rng(11)
+    codes, levels = [], []
+    for s in range(1000):
+        L = int(rng.integers(2, 6))
+        k = int(rng.integers(1, 41))
+        side = 1 << L
+        xyz = rng.integers(0, side, size=(k, 3))
+        c = W.sorted_codes(xyz[:, 0], xyz[:, 1], xyz[:, 2])
+        codes.append(c)
+        levels.append(L)
+    allc, off = _pack(codes)
+    print(f"[case] batch of {len(codes)} tiny scenes, {allc.size} voxels", flush=True)
+    assert allc.size > 2 * 8192, allc.size
+    b = S.Batch(ctx, allc, off, levels)
+    for s, (c, L) in enumerate(zip(codes, levels)):
+        o = O.OracleOctree.from_morton(c, L)
+        o.to_dag()
+        o.init_child_levels()
+        assert b.info(s)["nodes"] == o.n_nodes, f"scene {s}"
+        assert b.svdag_bytes(s) == o.encode(0), f"scene {s} ({L} levels, {c.size} voxels)"
+
+
 def test_batch_device_scenes_equal_single_builds(ctx):
     specs = W.batch_specs(16, 2, (6, 8))
     dev = [ctx.spec_codes(sp) for sp in specs]
