@@ -52,7 +52,16 @@ typedef struct svd_octree svd_octree;
 typedef struct svd_scene svd_scene;
 
 /* Context: one device, one stream.  Replaces nothing in the reference (the
- * reference is single-threaded host code); it owns device memory pools.   */
+ * reference is single-threaded host code); it owns device memory pools.
+ * Environment read at creation (tuning only; results never change):
+ *   SVD_SMALL_LEVELS="svo,dag,sdag,layout"  the level sizes up to which the
+ *       top levels of a pass run in one workgroup (SVO child codes, toDAG and
+ *       toSDAG nodes, SSVDAG layout nodes; default 2048,1024,512,2048, each
+ *       capped at its compiled maximum 8192,4096,1024,4096; 0 = the maximum);
+ *   SVD_REFSORT_EXTRA_ROUNDS=k  device-wide refcount-order rounds run past the
+ *       size bound before the list size is first read back (default 4, 0..16);
+ *   SVD_TOSDAG_CHECKED=1  toSDAG reads its undecided count back after every
+ *       resolution round instead of running two rounds speculatively.        */
 int svd_ctx_create(int device, svd_ctx** out);
 int svd_ctx_destroy(svd_ctx* ctx);
 const char* svd_last_error(const svd_ctx* ctx);
