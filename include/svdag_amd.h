@@ -334,6 +334,25 @@ int svd_autotune_trace_modes(svd_scene* sc, const svd_trace_params* p, svd_ray_r
                              svd_ray_result* shadow, uint32_t reps, uint32_t* best_tile_w, uint32_t* best_mode,
                              double* ms);
 
+/* One-workgroup level limits of the lossless build (results never depend on
+ * them): limits[4] = SVO child codes, toDAG nodes, toSDAG nodes, SSVDAG layout
+ * nodes up to which the top levels of a pass run in one workgroup (0 = the
+ * compiled maximum 8192, 4096, 1024, 4096; larger values are cut to it).
+ * svd_autotune_build times the lossless build of `levels`-level device codes
+ * (svd_octree_from_morton, svd_reduce_dag, svd_reduce_sym, then the .svdag,
+ * .ussvdag, .esvdag and .ssvdag images; the first two for levels < 3) under
+ * SVD_BUILD_PRESETS presets of the limits (1 warm-up + `reps` builds each,
+ * the median host time to a drained stream), keeps the fastest in the context
+ * and reports it in best[4]; ms (may be NULL) receives the SVD_BUILD_PRESETS
+ * medians.  The presets: {2048,1024,512,2048} (the default),
+ * {8192,4096,1024,4096}, {4096,2048,1024,4096}, {1024,512,256,1024},
+ * {2048,1024,1024,2048}.                                                     */
+#define SVD_BUILD_PRESETS 5
+int svd_set_build_limits(svd_ctx* ctx, const uint32_t* limits);
+int svd_get_build_limits(svd_ctx* ctx, uint32_t* limits);
+int svd_autotune_build(svd_ctx* ctx, const uint64_t* device_codes, uint64_t n, uint32_t levels, uint32_t reps,
+                       uint32_t* best, double* ms);
+
 /* ---- Gray-code attribute bit-tree --------------------------------------------
  * The reference lists the feature (readme.md:8) but has no code for it; the
  * encoding is the one documented with this library: per-voxel u8 attributes

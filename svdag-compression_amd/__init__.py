@@ -191,6 +191,9 @@ def lib():
         "svd_set_trace_mode": (i32, [vp, u32]),
         "svd_get_trace_mode": (i32, [vp, P(u32)]),
         "svd_autotune_trace_modes": (i32, [vp, P(TraceParams), vp, vp, u32, P(u32), P(u32), P(C.c_double)]),
+        "svd_set_build_limits": (i32, [vp, P(u32)]),
+        "svd_get_build_limits": (i32, [vp, P(u32)]),
+        "svd_autotune_build": (i32, [vp, vp, u64, u32, u32, P(u32), P(C.c_double)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -216,7 +219,8 @@ EXPORTED = ["svd_ctx_create", "svd_ctx_destroy", "svd_last_error", "svd_ctx_sync
             "svd_workload_terrain", "svd_batch_build", "svd_batch_free", "svd_batch_scene_info", "svd_batch_svdag",
             "svd_batch_svdag_copy", "svd_batch_trace", "svd_device_copy",
             "svd_set_trace_tile", "svd_get_trace_tile", "svd_autotune_trace", "svd_set_trace_mode",
-            "svd_get_trace_mode", "svd_autotune_trace_modes", "svd_workload_materials",
+            "svd_get_trace_mode", "svd_autotune_trace_modes", "svd_set_build_limits", "svd_get_build_limits",
+            "svd_autotune_build", "svd_workload_materials",
             "svd_codes_union"]
 
 
@@ -356,6 +360,28 @@ class Context:
     @trace_mode.setter
     def trace_mode(self, mode: str):
         self.check(self.L.svd_set_trace_mode(self.h, TRACE_MODES.index(mode)))
+
+    @property
+    def build_limits(self) -> tuple:
+        """One-workgroup level limits of the build: (SVO codes, toDAG nodes,
+        toSDAG nodes, layout nodes)."""
+        v = (C.c_uint32 * 4)()
+        self.check(self.L.svd_get_build_limits(self.h, v))
+        return tuple(int(x) for x in v)
+
+    @build_limits.setter
+    def build_limits(self, limits):
+        v = (C.c_uint32 * 4)(*[int(x) for x in limits])
+        self.check(self.L.svd_set_build_limits(self.h, v))
+
+    def autotune_build(self, codes_ptr: int, n: int, levels: int, reps: int = 5) -> dict:
+        """Time the lossless build of these device codes under each preset of
+        the build limits, keep the fastest in the context; returns the chosen
+        limits and every preset's median time (ms)."""
+        best = (C.c_uint32 * 4)()
+        ms = (C.c_double * 5)()
+        self.check(self.L.svd_autotune_build(self.h, C.c_void_p(codes_ptr), n, levels, reps, best, ms))
+        return {"limits": tuple(int(x) for x in best), "ms": [round(x, 4) for x in ms]}
 
     def autotune_trace(self, scene: "Scene", params: TraceParams, prim_ptr: int, shad_ptr: int | None,
                        reps: int = 3, cache: str | None = None, key: str | None = None) -> int:

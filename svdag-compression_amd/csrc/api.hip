@@ -4,6 +4,7 @@
 #include <cstdlib>
 #include "svd_internal.hpp"
 #include <algorithm>
+#include <chrono>
 #include <new>
 
 namespace svd {
@@ -1148,6 +1149,72 @@ int svd_autotune_trace(svd_scene* s, const svd_trace_params* p, svd_ray_result* 
     if (best) *best = shapes[b];
     if (ms) for (int k = 0; k < 4; ++k) ms[k] = t[k];
   });
+}
+
+int svd_set_build_limits(svd_ctx* c, const uint32_t* limits) {
+  if (!c || !limits) return SVD_ERR_ARG;
+  c->smallSvo = limits[0];
+  c->smallDag = limits[1];
+  c->smallSdag = limits[2];
+  c->smallLayout = limits[3];
+  return SVD_OK;
+}
+
+int svd_get_build_limits(svd_ctx* c, uint32_t* limits) {
+  if (!c || !limits) return SVD_ERR_ARG;
+  limits[0] = c->smallSvo;
+  limits[1] = c->smallDag;
+  limits[2] = c->smallSdag;
+  limits[3] = c->smallLayout;
+  return SVD_OK;
+}
+
+// The lossless build of a device voxel list under each preset of the
+// one-workgroup level limits: 1 warm-up + reps builds each (host wall time
+// to a drained stream, median), the fastest preset kept.
+int svd_autotune_build(svd_ctx* c, const uint64_t* dcodes, uint64_t n, uint32_t levels, uint32_t reps,
+                       uint32_t* best, double* ms) {
+  if (!c || !dcodes || n == 0 || reps < 1 || levels < 2 || levels > kMaxLevels) return SVD_ERR_ARG;
+  static const uint32_t presets[SVD_BUILD_PRESETS][4] = {
+      {2048, 1024, 512, 2048}, {8192, 4096, 1024, 4096}, {4096, 2048, 1024, 4096},
+      {1024, 512, 256, 1024}, {2048, 1024, 1024, 2048}};
+  uint32_t keep[4];
+  svd_get_build_limits(c, keep);
+  auto build_once = [&]() -> int {
+    svd_octree* o = nullptr;
+    int rc = svd_octree_from_morton(c, dcodes, n, levels, 1, &o);
+    if (rc) return rc;
+    rc = svd_reduce_dag(o);
+    if (!rc) rc = svd_reduce_sym(o);
+    const int f[4] = {SVD_FMT_SVDAG, SVD_FMT_USSVDAG, SVD_FMT_ESVDAG, SVD_FMT_SSVDAG};
+    if (!rc) rc = svd_encode_formats(o, f, levels >= 3 ? 4u : 2u, nullptr);
+    if (!rc) rc = svd_ctx_synchronize(c);
+    svd_octree_free(o);
+    return rc;
+  };
+  double t[SVD_BUILD_PRESETS];
+  for (int k = 0; k < SVD_BUILD_PRESETS; ++k) {
+    svd_set_build_limits(c, presets[k]);
+    std::vector<double> v;
+    for (uint32_t r = 0; r <= reps; ++r) {
+      const auto t0 = std::chrono::steady_clock::now();
+      const int rc = build_once();
+      if (rc) {
+        svd_set_build_limits(c, keep);
+        return rc;
+      }
+      const double dt = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+      if (r) v.push_back(dt);
+    }
+    std::sort(v.begin(), v.end());
+    t[k] = v[v.size() / 2];
+  }
+  int b = 0;
+  for (int k = 1; k < SVD_BUILD_PRESETS; ++k) if (t[k] < t[b]) b = k;
+  svd_set_build_limits(c, presets[b]);
+  if (best) for (int i = 0; i < 4; ++i) best[i] = presets[b][i];
+  if (ms) for (int k = 0; k < SVD_BUILD_PRESETS; ++k) ms[k] = t[k];
+  return SVD_OK;
 }
 
 int svd_autotune_trace_modes(svd_scene* s, const svd_trace_params* p, svd_ray_result* prim, svd_ray_result* shad,

@@ -139,6 +139,30 @@ def test_one_workgroup_level_limits(ctx, monkeypatch, limits):
         c.close()
 
 
+def test_build_limits_and_autotune(ctx):
+    """svd_set/get_build_limits round-trip; svd_autotune_build keeps one of its
+    presets, and the build under the chosen limits gives the oracle's files."""
+    import torch
+    keep = ctx.build_limits
+    try:
+        ctx.build_limits = (1, 2, 3, 4)
+        assert ctx.build_limits == (1, 2, 3, 4)
+        codes, L = SCENES["terrain7"]()
+        d = torch.from_numpy(codes.view(np.int64)).to("cuda")
+        r = ctx.autotune_build(d.data_ptr(), codes.size, L, reps=2)
+        presets = [(2048, 1024, 512, 2048), (8192, 4096, 1024, 4096), (4096, 2048, 1024, 4096),
+                   (1024, 512, 256, 1024), (2048, 1024, 1024, 2048)]
+        assert r["limits"] in presets and ctx.build_limits == r["limits"], r
+        assert len(r["ms"]) == 5 and all(x > 0 for x in r["ms"]), r
+        _, _, _, files_o = oracle_pipeline(codes, L)
+        g, _, _, _, files_g = gpu_pipeline(ctx, codes, L)
+        for f, want in files_o.items():
+            assert files_g[f] == want, f
+        g.free()
+    finally:
+        ctx.build_limits = keep
+
+
 def test_tosdag_checked_rounds_equal_speculative(ctx, monkeypatch):
     """toSDAG runs two resolution rounds per level without host reads and
     checks convergence once at the end; SVD_TOSDAG_CHECKED=1 (read when a
